@@ -1,0 +1,204 @@
+"""Benchmark: timesteps/s of the 1M-node 3D RBF-FD lid-driven cavity.
+
+Workload (one GPU): 100^3 lattice on [0,1]^3, interior jitter +-0.3h (seeded),
+edges/corners removed -> 998,816 nodes; PHS r^3 + degree-3 polynomial (20
+monomials); stencils of 40 nodes, 60/55/50 on the face layer and the two
+layers under it; Re = 400 (rho = 1, mu = 0.0025), lid u = 1 on z = 1,
+dt = 5e-4, AB2; pressure-Poisson relative residual 1e-8.
+
+Two forms are timed over the same number of steps from the same warmed-up
+state: the plain FP64 path (FP64 BiCGSTAB, previous pressure as the guess) and
+the headline form (FP32 inner / FP64 outer BiCGSTAB, learned guess when
+trained).  `value` is the headline form's timesteps/s; the FP64 rate and the
+ratio are reported beside it, with the velocity difference between the two.
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "timesteps/s, 1M-node 3D RBF-FD NS: learned+mixed vs plain FP64, same residual"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def lid_bc(cloud):
+    nv = cloud.nv
+    u_dir = np.ones(nv, dtype=np.uint8)
+    p_dir = np.zeros(nv, dtype=np.uint8)
+    u0 = np.zeros(nv)
+    lid = (cloud.boundary == 1) & (np.abs(cloud.xyz[:, 2] - 1.0) < 1e-12)
+    u0[lid] = 1.0
+    return u_dir, p_dir, u0
+
+
+def run_form(M, ctx, cloud, prm, precision, steps, warmup, label):
+    u_dir, p_dir, u0 = lid_bc(cloud)
+    ctx.fs_create(prm["rho"], prm["mu"], prm["dt"], 2, prm["tol"], prm["n_iter"], u_dir, u_dir,
+                  u_dir, p_dir, precision=precision, inner_tol=prm["inner_tol"],
+                  check_every=prm["check_every"])
+    nv = cloud.nv
+    ctx.set_field(M.MP_U, u0)
+    ctx.set_field(M.MP_V, np.zeros(nv))
+    ctx.set_field(M.MP_W, np.zeros(nv))
+    ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
+    for it in range(warmup):
+        ctx.step(1, it)
+        st = ctx.stats()
+        log(f"[{label}] warmup step {it}: iters {st.iterations} rel_res {st.rel_res:.2e}")
+    ctx.kernel_timing(True)
+    it0 = warmup
+    iters = []
+    res = []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.step(1, it0 + k)
+        st = ctx.stats()
+        iters.append(st.iterations)
+        res.append(st.rel_res)
+    ctx.sync()
+    t1 = time.perf_counter()
+    spmv = ctx.kernel_timing_get(0)
+    pred = ctx.kernel_timing_get(1)
+    ctx.kernel_timing(False)
+    fields = [ctx.get_field(w) for w in (M.MP_U, M.MP_V, M.MP_W, M.MP_P)]
+    div = ctx.divergence()
+    log(f"[{label}] {steps} steps in {t1 - t0:.3f}s, iters/step {np.mean(iters):.1f}, "
+        f"max rel_res {max(res):.2e}, spmv {spmv}")
+    return dict(seconds=t1 - t0, iters=iters, rel_res=res, spmv=spmv, predictor=pred,
+                fields=fields, divergence=div.tolist())
+
+
+def cpu_baseline(cloud, w, prm, iters_per_step, max_iter_sample=10):
+    """Oracle FP64 step (1 thread) on the same 1M-node cloud: predictor/RHS and
+    corrector timed in full, BiCGSTAB (Jacobi) timed over a few iterations and
+    scaled to the iterations per step the FP64 GPU path needed."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    import oracle as O
+    u_dir, p_dir, u0 = lid_bc(cloud)
+    nv = cloud.nv
+    fs = O.FractionalStep(cloud, w, u_dir, u_dir, u_dir, p_dir, prm["rho"], prm["mu"], prm["dt"],
+                          2, solver_kind=0, precond=1, tol=1e-30, maxit=max_iter_sample)
+    u, v, ww = u0.copy(), np.zeros(nv), np.zeros(nv)
+    p = np.zeros(fs.n_sys)
+    t0 = time.perf_counter()
+    uh, b = fs.predict(0, u, v, ww, p)
+    t1 = time.perf_counter()
+    x, _, it, _ = fs.solve(b, p)
+    t2 = time.perf_counter()
+    fs.correct(u, v, ww, uh, x)
+    t3 = time.perf_counter()
+    t_fixed = (t1 - t0) + (t3 - t2)
+    t_iter = (t2 - t1) / max(it, 1)
+    per_step = t_fixed + iters_per_step * t_iter
+    return {"value": 1.0 / per_step, "unit": "timesteps/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle FP64 step on the same {nv}-node cloud, 1 thread: predictor+rhs "
+                       f"{t1 - t0:.2f}s and corrector {t3 - t2:.2f}s timed in full, Jacobi "
+                       f"BiCGSTAB {it} iterations ({t_iter * 1e3:.1f} ms/iter) scaled to "
+                       f"{iters_per_step:.0f} iterations/step (FP64 GPU path's count)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=100, help="lattice points per axis")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+    import memphys_amd as M
+
+    prm = dict(rho=1.0, mu=0.0025, dt=5e-4, tol=args.tol, n_iter=20000, inner_tol=1e-3,
+               check_every=8)
+    t = time.perf_counter()
+    cloud = M.box_cloud(dim=3, n=args.n, jitter=0.3, seed=7, k_interior=40,
+                        k_layers=(60, 55, 50))
+    log(f"cloud nv={cloud.nv} nnz={cloud.nnz} setup {time.perf_counter() - t:.2f}s")
+    ctx = M.Context(0)
+    ctx.upload(cloud)
+    ctx.kernel_timing(True)
+    t = time.perf_counter()
+    (gx, gy, gz, lap), _ = ctx.weights(3, 3, download=not args.no_cpu)
+    t_w = time.perf_counter() - t
+    wk = ctx.kernel_timing_get(4)
+    ctx.kernel_timing(False)
+    log(f"weights: {t_w:.3f}s wall incl. download, kernel {wk[1]:.2f} ms")
+
+    fp64 = run_form(M, ctx, cloud, prm, M.MP_PREC_FP64, args.steps, args.warmup, "fp64")
+    head = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps, args.warmup, "mixed")
+    vdiff = max(np.linalg.norm(a - b) / max(np.linalg.norm(a), 1e-300)
+                for a, b in zip(fp64["fields"][:3], head["fields"][:3]))
+    pdiff = np.linalg.norm(fp64["fields"][3] - head["fields"][3]) / np.linalg.norm(fp64["fields"][3])
+
+    sps_head = args.steps / head["seconds"]
+    sps_fp64 = args.steps / fp64["seconds"]
+    n_l, ms, bytes_l = head["spmv"]
+    achieved = bytes_l / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    n64, ms64, b64 = fp64["spmv"]
+    ach64 = b64 / (ms64 * 1e-3) / 1e9 if ms64 > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": sps_head,
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * head["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"3D lid-driven cavity Re=400, {cloud.nv} jittered nodes "
+                         f"({args.n}^3 lattice, +-0.3h), PHS r^3 + deg-3 poly, k=40/50/55/60, "
+                         f"dt=5e-4, AB2, PPE rel. residual {args.tol:g}"),
+            "nv": cloud.nv, "nnz": cloud.nnz, "form": "mixed FP32-inner/FP64-outer BiCGSTAB, "
+                                                      "previous-pressure guess",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_spmv_alpha/k_spmv_omega<float> (PPE SpMV)",
+            "bytes_per_launch": bytes_l, "mean_ms": ms, "launches": n_l,
+        },
+        "fp64_timesteps_per_s": sps_fp64,
+        "speedup_vs_fp64": sps_head / sps_fp64,
+        "ppe_iters_per_step": {"fp64": float(np.mean(fp64["iters"])),
+                               "headline": float(np.mean(head["iters"]))},
+        "max_ppe_rel_res": {"fp64": max(fp64["rel_res"]), "headline": max(head["rel_res"])},
+        "headline_vs_fp64_rel_diff": {"velocity": vdiff, "pressure": pdiff},
+        "fp64_spmv_roofline": {"achieved": ach64, "frac": ach64 / HBM_PEAK_GBS,
+                               "bytes_per_launch": b64, "mean_ms": ms64},
+        "weights_kernel_ms": wk[1],
+        "divergence_fp64": fp64["divergence"],
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(cloud, (gx, gy, gz, lap), prm,
+                                               float(np.mean(fp64["iters"])))
+        except Exception as e:  # the baseline is reported, never the product
+            out["cpu_baseline"] = {"error": str(e)}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,137 @@
+/* memphys_hip.h — C ABI of the MI355X (gfx950) meshless fractional-step path.
+ *
+ * Plain pointers and sizes only.  Host pointers are read or written before the
+ * call returns; all device state lives in the context.  Every entry point
+ * returns 0 on success and a non-zero mp_status otherwise; mp_last_error()
+ * then describes the failure.  One context per host thread; calls on one
+ * context are not re-entrant.  The C++ class layer (memphys_amd/host) turns a
+ * non-zero status into the reference's printf + throw bad_exception().
+ *
+ * Indices are 32-bit (the reference's CSR uses int, class.hpp:145-146).
+ * Node order at this boundary is the caller's (the RCM order CLOUD produces);
+ * the library keeps its own device order internally and translates.
+ */
+#ifndef MEMPHYS_HIP_H
+#define MEMPHYS_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mp_ctx mp_ctx;
+
+enum mp_status {
+    MP_OK = 0,
+    MP_ERR_ARG = 1,      /* invalid argument or size */
+    MP_ERR_HIP = 2,      /* HIP runtime error */
+    MP_ERR_STATE = 3,    /* call out of order (e.g. step before create) */
+    MP_ERR_NOCONV = 4    /* Krylov solve stopped above tolerance */
+};
+
+enum mp_field { MP_U = 0, MP_V = 1, MP_W = 2, MP_P = 3 };
+enum mp_op { MP_GX = 0, MP_GY = 1, MP_GZ = 2, MP_LAP = 3 };
+
+/* PPE arithmetic: plain FP64 BiCGSTAB, or FP32 inner BiCGSTAB inside FP64
+ * defect correction (outer residual b - A x in FP64). */
+enum mp_precision { MP_PREC_FP64 = 0, MP_PREC_MIXED = 1 };
+
+/* Initial guess for each pressure solve: the previous pressure (reference
+ * behaviour, solveWithGuess(rhs, p_old)) or the learned message-passing guess. */
+enum mp_guess { MP_GUESS_PREVIOUS = 0, MP_GUESS_LEARNED = 1 };
+
+typedef struct {
+    double rho, mu, dt;
+    int temporal_order;       /* 1 = Euler, 2 = AB2 after the first step */
+    double solver_tolerance;  /* relative residual |b - A p| / |b| */
+    int n_iter;               /* Krylov iteration cap per solve */
+    int precision;            /* enum mp_precision */
+    double inner_tolerance;   /* MIXED: inner FP32 relative reduction per cycle */
+    int check_every;          /* iterations launched between convergence polls (0 = default) */
+} mp_fs_params;
+
+typedef struct {
+    int iterations;       /* Krylov iterations of the last solve (all cycles) */
+    int outer_cycles;     /* FP64 defect-correction cycles (1 for FP64) */
+    int restarts;         /* BiCGSTAB restarts */
+    int converged;        /* 1 when rel_res <= tolerance */
+    double rel_res;       /* |A p - b| / |b| of the final pressure (fresh FP64 product) */
+    double abs_res;       /* |A p - b| */
+    double alpha;         /* regularisation unknown p[nv] (full-Neumann only, else 0) */
+    double guess_rel_res; /* |A p0 - b| / |b| of the initial guess */
+    long total_iterations;/* sum over all solves since context creation */
+    long total_solves;
+} mp_step_stats;
+
+/* Context ------------------------------------------------------------------ */
+int mp_ctx_create(int device, mp_ctx** out);
+int mp_ctx_destroy(mp_ctx* ctx);
+const char* mp_last_error(const mp_ctx* ctx);
+int mp_sync(mp_ctx* ctx);
+
+/* Point cloud and stencils (replaces the CLOUD data handed to the weight loop,
+ * cloud.cpp:207-279; POINTS xyz/normal/boundary_flag, class.hpp:99-124).
+ * xyz and normal: nv*dim AoS; boundary: nv bytes; CSR row_ptr[nv+1], col[...];
+ * entry 0 of every row must be the node itself. */
+int mp_cloud_upload(mp_ctx* ctx, int nv, int dim, const double* xyz, const uint8_t* boundary,
+                    const double* normal, const int* row_ptr, const int* col_idx);
+
+/* Batched PHS + polynomial stencil weights (replaces CLOUD::calc_grad_laplace_coeffs,
+ * cloud.cpp:207-279, and calc_PHS_RBF_grad_laplace_single_vert,
+ * coefficient_computations.cpp:294-328).  exps: m*dim monomial exponents.
+ * precision_mode: MP_PREC_FP64, or MP_PREC_MIXED (FP32 LU + FP64 refinement, FP64
+ * fallback per stencil).  Output arrays (CSR layout of the upload, may be NULL
+ * to keep the weights device-resident): gx, gy, gz (3D), lap; cond_out[nv]
+ * (may be NULL) receives a 1-norm condition estimate of each stencil matrix. */
+int mp_weights_assemble(mp_ctx* ctx, int phs_deg, int poly_deg, const int* exps,
+                        int precision_mode, double* gx, double* gy, double* gz, double* lap,
+                        double* cond_out);
+
+/* Upload weights computed elsewhere (CSR layout of the upload). */
+int mp_weights_set(mp_ctx* ctx, const double* gx, const double* gy, const double* gz,
+                   const double* lap);
+
+/* y = Op x for one stencil operator (replaces points.*_matrix_EIGEN * x). */
+int mp_operator_apply(mp_ctx* ctx, int op, const double* x, double* y);
+
+/* Implicit operator and Krylov solve (replaces SOLVER::init, solver.cpp:14-79 with
+ * calc_coeff_matrix 165-214, and SOLVER::general_solve 283-363).
+ * Rows: interior diff*L + conv*(Gx+Gy+Gz) + unsteady*I; boundary Dirichlet
+ * identity; boundary Neumann n.grad.  With no Dirichlet boundary row the system
+ * is bordered to nv+1 (ones column and row, zero corner). */
+int mp_solver_init(mp_ctx* ctx, const uint8_t* dirichlet, double unsteady, double conv,
+                   double diff, double tolerance, int n_iter, int precision);
+int mp_solver_system_size(mp_ctx* ctx, int* n);
+/* field_new = solve(rhs) from the guess field_old; field_old[nv] = field_new[nv]
+ * when bordered.  Arrays of system size. */
+int mp_solver_solve(mp_ctx* ctx, const double* rhs, double* field_new, double* field_old);
+
+/* Fractional step (replaces FRACTIONAL_STEP_1, fractional_step_1.cpp:22-222,
+ * generalised to 3D).  Builds the pressure operator as mp_solver_init with
+ * p_dir and (0, 0, 1).  w_dir is ignored in 2D. */
+int mp_fs_create(mp_ctx* ctx, const mp_fs_params* params, const uint8_t* u_dir,
+                 const uint8_t* v_dir, const uint8_t* w_dir, const uint8_t* p_dir);
+int mp_fields_set(mp_ctx* ctx, int which, const double* values);
+int mp_fields_get(mp_ctx* ctx, int which, double* values);
+/* n_steps of single_timestep followed by old <- new; it0 is the first step index
+ * (Euler at it == 0).  Fields stay on the device between calls. */
+int mp_fs_step(mp_ctx* ctx, int n_steps, int it0);
+int mp_stats_get(mp_ctx* ctx, mp_step_stats* stats);
+int mp_guess_mode(mp_ctx* ctx, int mode);
+
+/* Continuity residual Gx u + Gy v (+ Gz w) of the current fields
+ * (postprocessing_functions.cpp:192-195, 248-251):
+ * out = {max_int, mean_int, max_bnd, mean_bnd, max_all, mean_all}. */
+int mp_divergence(mp_ctx* ctx, double* out);
+
+/* Per-kernel timing with HIP events on the library's stream.
+ * which: 0 = PPE SpMV, 1 = predictor, 2 = pressure RHS, 3 = corrector,
+ *        4 = stencil weights.  Reports launches and mean milliseconds. */
+int mp_kernel_timing(mp_ctx* ctx, int enable);
+int mp_kernel_timing_get(mp_ctx* ctx, int which, long* launches, double* mean_ms,
+                         double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MEMPHYS_HIP_H */

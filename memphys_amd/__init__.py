@@ -1,0 +1,322 @@
+"""memphys_amd: MI355X (gfx950) meshless RBF-FD fractional-step path.
+
+Python access to the two native libraries built in-tree:
+
+* ``libmemphys_hip.so``  -- HIP kernels behind the C ABI of ``include/memphys_hip.h``;
+* ``libmemphys_host.so`` -- host set-up (jittered box clouds, kNN stencils, RCM).
+
+Nothing here computes on the CPU in place of the GPU: every numerical call
+goes to the HIP library, and loading fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB_PATH = os.path.join(_HERE, "libmemphys_hip.so")
+HOST_LIB_PATH = os.path.join(_HERE, "libmemphys_host.so")
+
+MP_U, MP_V, MP_W, MP_P = 0, 1, 2, 3
+MP_GX, MP_GY, MP_GZ, MP_LAP = 0, 1, 2, 3
+MP_PREC_FP64, MP_PREC_MIXED = 0, 1
+MP_GUESS_PREVIOUS, MP_GUESS_LEARNED = 0, 1
+
+# Every symbol include/memphys_hip.h declares.
+HIP_SYMBOLS = (
+    "mp_ctx_create", "mp_ctx_destroy", "mp_last_error", "mp_sync", "mp_cloud_upload",
+    "mp_weights_assemble", "mp_weights_set", "mp_operator_apply", "mp_solver_init",
+    "mp_solver_system_size", "mp_solver_solve", "mp_fs_create", "mp_fields_set",
+    "mp_fields_get", "mp_fs_step", "mp_stats_get", "mp_guess_mode", "mp_divergence",
+    "mp_kernel_timing", "mp_kernel_timing_get",
+)
+
+
+class MemphysError(RuntimeError):
+    pass
+
+
+class FsParams(C.Structure):
+    _fields_ = [
+        ("rho", C.c_double), ("mu", C.c_double), ("dt", C.c_double),
+        ("temporal_order", C.c_int), ("solver_tolerance", C.c_double),
+        ("n_iter", C.c_int), ("precision", C.c_int), ("inner_tolerance", C.c_double),
+        ("check_every", C.c_int),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int), ("outer_cycles", C.c_int), ("restarts", C.c_int),
+        ("converged", C.c_int), ("rel_res", C.c_double), ("abs_res", C.c_double),
+        ("alpha", C.c_double), ("guess_rel_res", C.c_double),
+        ("total_iterations", C.c_long), ("total_solves", C.c_long),
+    ]
+
+
+_hip = None
+_host = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def hip_lib() -> C.CDLL:
+    """Loads libmemphys_hip.so (raises if it has not been built)."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB_PATH):
+            raise MemphysError(f"{HIP_LIB_PATH} is missing: run __graft_entry__.build() first")
+        lib = C.CDLL(HIP_LIB_PATH)
+        vp = C.c_void_p
+        lib.mp_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+        lib.mp_ctx_destroy.argtypes = [vp]
+        lib.mp_last_error.argtypes = [vp]
+        lib.mp_last_error.restype = C.c_char_p
+        lib.mp_sync.argtypes = [vp]
+        lib.mp_cloud_upload.argtypes = [vp, C.c_int, C.c_int, _dp, _u8, _dp, _ip, _ip]
+        lib.mp_weights_assemble.argtypes = [vp, C.c_int, C.c_int, _ip, C.c_int, vp, vp, vp, vp, vp]
+        lib.mp_weights_set.argtypes = [vp, _dp, _dp, vp, _dp]
+        lib.mp_operator_apply.argtypes = [vp, C.c_int, _dp, _dp]
+        lib.mp_solver_init.argtypes = [vp, _u8, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_int, C.c_int]
+        lib.mp_solver_system_size.argtypes = [vp, C.POINTER(C.c_int)]
+        lib.mp_solver_solve.argtypes = [vp, _dp, _dp, _dp]
+        lib.mp_fs_create.argtypes = [vp, C.POINTER(FsParams), _u8, _u8, vp, _u8]
+        lib.mp_fields_set.argtypes = [vp, C.c_int, _dp]
+        lib.mp_fields_get.argtypes = [vp, C.c_int, _dp]
+        lib.mp_fs_step.argtypes = [vp, C.c_int, C.c_int]
+        lib.mp_stats_get.argtypes = [vp, C.POINTER(StepStats)]
+        lib.mp_guess_mode.argtypes = [vp, C.c_int]
+        lib.mp_divergence.argtypes = [vp, _dp]
+        lib.mp_kernel_timing.argtypes = [vp, C.c_int]
+        lib.mp_kernel_timing_get.argtypes = [vp, C.c_int, C.POINTER(C.c_long),
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        for name in HIP_SYMBOLS:
+            getattr(lib, name).restype = C.c_char_p if name == "mp_last_error" else C.c_int
+        _hip = lib
+    return _hip
+
+
+def host_lib() -> C.CDLL:
+    """Loads libmemphys_host.so (cloud generator, kNN, RCM)."""
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise MemphysError(f"{HOST_LIB_PATH} is missing: run __graft_entry__.build() first")
+        lib = C.CDLL(HOST_LIB_PATH)
+        lib.mph_box_cloud_count.argtypes = [C.c_int, C.c_int]
+        lib.mph_box_cloud_fill.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int, _ip,
+                                           _dp, _u8, _dp, _ip, _ip]
+        lib.mph_rng_uniform.argtypes = [C.c_uint64, C.c_int, C.c_void_p, C.c_void_p]
+        lib.mph_knn.argtypes = [C.c_int, C.c_int, _dp, _u8, _ip, _ip, _ip, C.c_int]
+        lib.mph_rcm.argtypes = [C.c_int, _ip, _ip, _ip]
+        lib.mph_apply_order.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp, _u8, C.c_void_p, _ip, _ip]
+        lib.mph_poly_exponents.argtypes = [C.c_int, C.c_int, _ip]
+        _host = lib
+    return _host
+
+
+# ----------------------------------------------------------------------------- host set-up
+@dataclass
+class Cloud:
+    """A point cloud with its stencils (CSR, entry 0 of each row is the node)."""
+    dim: int
+    xyz: np.ndarray       # (nv, dim)
+    boundary: np.ndarray  # (nv,) uint8
+    normal: np.ndarray    # (nv, dim)
+    row_ptr: np.ndarray   # (nv+1,) int32
+    col: np.ndarray       # (nnz,) int32
+    ksize: np.ndarray     # (nv,) int32
+    layer: np.ndarray     # (nv,) int32
+
+    @property
+    def nv(self) -> int:
+        return int(self.xyz.shape[0])
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_ptr[-1])
+
+
+def box_cloud(dim: int, n: int, jitter: float, seed: int, k_interior: int,
+              k_layers=(None, None, None), knn_threads: int = 0, rcm: bool = True) -> Cloud:
+    """Seeded jittered box cloud, kNN stencils and RCM order (CLOUD's set-up)."""
+    lib = host_lib()
+    nv = lib.mph_box_cloud_count(dim, n)
+    if nv <= 0:
+        raise MemphysError("box_cloud: bad dim/n")
+    kl = np.array([k if k is not None else k_interior for k in k_layers], dtype=np.int32)
+    xyz = np.zeros(nv * dim)
+    nrm = np.zeros(nv * dim)
+    bnd = np.zeros(nv, dtype=np.uint8)
+    ks = np.zeros(nv, dtype=np.int32)
+    layer = np.zeros(nv, dtype=np.int32)
+    got = lib.mph_box_cloud_fill(dim, n, float(jitter), int(seed), int(k_interior), kl, xyz, bnd,
+                                 nrm, ks, layer)
+    if got != nv:
+        raise MemphysError("box_cloud: generator failed")
+    rp = np.zeros(nv + 1, dtype=np.int32)
+    col = np.zeros(int(ks.sum()), dtype=np.int32)
+    if lib.mph_knn(nv, dim, xyz, bnd, ks, rp, col, int(knn_threads)) != 0:
+        raise MemphysError("box_cloud: kNN failed")
+    if rcm:
+        order = np.zeros(nv, dtype=np.int32)
+        if lib.mph_rcm(nv, rp, col, order) != 0:
+            raise MemphysError("box_cloud: RCM failed")
+        extra = layer.copy()
+        if lib.mph_apply_order(nv, dim, order, xyz, nrm, bnd, extra.ctypes.data_as(C.c_void_p),
+                               rp, col) != 0:
+            raise MemphysError("box_cloud: reorder failed")
+        layer = extra
+        ks = np.diff(rp).astype(np.int32)
+    return Cloud(dim, xyz.reshape(nv, dim), bnd, nrm.reshape(nv, dim), rp, col, ks, layer)
+
+
+def poly_exponents(dim: int, poly_deg: int) -> np.ndarray:
+    m = (poly_deg + 1) * (poly_deg + 2) // 2 if dim == 2 else \
+        (poly_deg + 1) * (poly_deg + 2) * (poly_deg + 3) // 6
+    out = np.zeros(m * dim, dtype=np.int32)
+    if host_lib().mph_poly_exponents(dim, poly_deg, out) != m:
+        raise MemphysError("poly_exponents failed")
+    return out.reshape(m, dim)
+
+
+# ----------------------------------------------------------------------------- GPU context
+class Context:
+    """One HIP context of the C ABI (one per host thread)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = hip_lib()
+        self.h = C.c_void_p()
+        rc = self.lib.mp_ctx_create(device, C.byref(self.h))
+        if rc != 0:
+            raise MemphysError(f"mp_ctx_create failed with status {rc}")
+        self.cloud = None
+        self.n_sys = None
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.mp_last_error(self.h)
+            raise MemphysError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self.lib.mp_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, cloud: Cloud):
+        self.cloud = cloud
+        self._check(self.lib.mp_cloud_upload(
+            self.h, cloud.nv, cloud.dim, np.ascontiguousarray(cloud.xyz.ravel()),
+            np.ascontiguousarray(cloud.boundary), np.ascontiguousarray(cloud.normal.ravel()),
+            np.ascontiguousarray(cloud.row_ptr), np.ascontiguousarray(cloud.col)),
+            "mp_cloud_upload")
+
+    def weights(self, phs_deg: int, poly_deg: int, precision: int = MP_PREC_FP64,
+                download: bool = True, cond: bool = False):
+        c = self.cloud
+        ex = np.ascontiguousarray(poly_exponents(c.dim, poly_deg).ravel())
+        outs = [np.zeros(c.nnz) if download else None for _ in range(4)]
+        if c.dim == 2 and download:
+            outs[2] = None
+        cnd = np.zeros(c.nv) if cond else None
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
+        self._check(self.lib.mp_weights_assemble(self.h, phs_deg, poly_deg, ex, precision,
+                                                 ptr(outs[0]), ptr(outs[1]), ptr(outs[2]),
+                                                 ptr(outs[3]), ptr(cnd)), "mp_weights_assemble")
+        return outs, cnd
+
+    def set_weights(self, gx, gy, gz, lap):
+        gzp = np.ascontiguousarray(gz).ctypes.data_as(C.c_void_p) if gz is not None else None
+        self._check(self.lib.mp_weights_set(self.h, np.ascontiguousarray(gx),
+                                            np.ascontiguousarray(gy), gzp,
+                                            np.ascontiguousarray(lap)), "mp_weights_set")
+
+    def apply(self, op: int, x: np.ndarray) -> np.ndarray:
+        y = np.zeros(self.cloud.nv)
+        self._check(self.lib.mp_operator_apply(self.h, op, np.ascontiguousarray(x, dtype=np.float64),
+                                               y), "mp_operator_apply")
+        return y
+
+    def solver_init(self, dirichlet, unsteady=0.0, conv=0.0, diff=1.0, tol=1e-10, n_iter=1000,
+                    precision=MP_PREC_FP64):
+        self._check(self.lib.mp_solver_init(self.h, np.ascontiguousarray(dirichlet, dtype=np.uint8),
+                                            unsteady, conv, diff, tol, n_iter, precision),
+                    "mp_solver_init")
+        n = C.c_int()
+        self._check(self.lib.mp_solver_system_size(self.h, C.byref(n)), "mp_solver_system_size")
+        self.n_sys = n.value
+        return n.value
+
+    def solve(self, rhs, guess):
+        xn = np.zeros(self.n_sys)
+        xo = np.ascontiguousarray(guess, dtype=np.float64).copy()
+        self._check(self.lib.mp_solver_solve(self.h, np.ascontiguousarray(rhs, dtype=np.float64),
+                                             xn, xo), "mp_solver_solve")
+        return xn, xo
+
+    def fs_create(self, rho, mu, dt, order, tol, n_iter, u_dir, v_dir, w_dir, p_dir,
+                  precision=MP_PREC_FP64, inner_tol=1e-3, check_every=8):
+        p = FsParams(rho, mu, dt, order, tol, n_iter, precision, inner_tol, check_every)
+        wd = np.ascontiguousarray(w_dir, dtype=np.uint8) if w_dir is not None else None
+        self._wd = wd
+        self._check(self.lib.mp_fs_create(
+            self.h, C.byref(p), np.ascontiguousarray(u_dir, dtype=np.uint8),
+            np.ascontiguousarray(v_dir, dtype=np.uint8),
+            wd.ctypes.data_as(C.c_void_p) if wd is not None else None,
+            np.ascontiguousarray(p_dir, dtype=np.uint8)), "mp_fs_create")
+        n = C.c_int()
+        self._check(self.lib.mp_solver_system_size(self.h, C.byref(n)), "mp_solver_system_size")
+        self.n_sys = n.value
+
+    def set_field(self, which: int, values):
+        self._check(self.lib.mp_fields_set(self.h, which, np.ascontiguousarray(values,
+                                                                               dtype=np.float64)),
+                    "mp_fields_set")
+
+    def get_field(self, which: int) -> np.ndarray:
+        n = self.n_sys if which == MP_P else self.cloud.nv
+        out = np.zeros(n)
+        self._check(self.lib.mp_fields_get(self.h, which, out), "mp_fields_get")
+        return out
+
+    def step(self, n_steps: int, it0: int):
+        self._check(self.lib.mp_fs_step(self.h, n_steps, it0), "mp_fs_step")
+
+    def stats(self) -> StepStats:
+        s = StepStats()
+        self._check(self.lib.mp_stats_get(self.h, C.byref(s)), "mp_stats_get")
+        return s
+
+    def divergence(self) -> np.ndarray:
+        out = np.zeros(6)
+        self._check(self.lib.mp_divergence(self.h, out), "mp_divergence")
+        return out
+
+    def guess_mode(self, mode: int):
+        self._check(self.lib.mp_guess_mode(self.h, mode), "mp_guess_mode")
+
+    def sync(self):
+        self._check(self.lib.mp_sync(self.h), "mp_sync")
+
+    def kernel_timing(self, enable: bool):
+        self._check(self.lib.mp_kernel_timing(self.h, 1 if enable else 0), "mp_kernel_timing")
+
+    def kernel_timing_get(self, which: int):
+        n = C.c_long()
+        ms = C.c_double()
+        b = C.c_double()
+        self._check(self.lib.mp_kernel_timing_get(self.h, which, C.byref(n), C.byref(ms),
+                                                  C.byref(b)), "mp_kernel_timing_get")
+        return n.value, ms.value, b.value

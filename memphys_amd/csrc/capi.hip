@@ -1,0 +1,705 @@
+// C ABI of the meshless fractional-step path (include/memphys_hip.h).
+//
+// Device order.  The caller's node order (RCM, as CLOUD produces) is
+// re-sorted by stencil length inside windows of SIGMA rows, then cut into
+// slices of 64 rows; each slice stores its neighbours column-major (SELL-64).
+// Every field, weight and operator lives on the device in that order; host
+// arrays crossing the ABI are permuted on the way in and out.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "krylov.hpp"
+#include "mp_internal.hpp"
+
+using namespace mp;
+
+struct mp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // cloud
+    bool has_cloud = false;
+    int nv = 0, dim = 0, nslices = 0, npad = 0, total_slots = 0, max_len = 0;
+    long nnz = 0;
+    std::vector<int> perm, iperm, csr_slot, h_rowptr;
+    std::vector<uint8_t> h_bnd;
+    DevBuf<int> sptr, col, len, brows;
+    int nb = 0;
+    DevBuf<double> xyz, normal;
+    DevBuf<uint8_t> bnd;
+    SellView sv{};
+
+    // stencil weights (SELL slots)
+    bool has_weights = false;
+    DevBuf<double> w[4];
+    DevBuf<double> vec_a, vec_b;
+
+    // standalone SOLVER
+    std::unique_ptr<PpeSolver> solver;
+    DevBuf<uint8_t> solver_dir;
+
+    // fractional step
+    bool has_fs = false;
+    mp_fs_params prm{};
+    DevBuf<uint8_t> udir[3], pdir;
+    std::unique_ptr<PpeSolver> ppe;
+    DevBuf<double> U[2][3];
+    int cur = 0;
+    DevBuf<double> P, b, uh[3], S[2];
+    int s_cur = 0;
+    mp_step_stats stats{};
+    int guess_mode = MP_GUESS_PREVIOUS;
+    DevBuf<double> div_partial;
+
+    KernelTimer timers[5];
+    double bytes[5] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(mp_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(mp_ctx* c, const char* fn, F&& body) {
+    if (!c) return MP_ERR_ARG;
+    try {
+        MP_HIP(hipSetDevice(c->device));
+        return body();
+    } catch (const HipError& e) {
+        return fail(c, MP_ERR_HIP,
+                    std::string(fn) + ": " + e.what + " -> " + hipGetErrorString(e.code));
+    } catch (const std::exception& e) {
+        return fail(c, MP_ERR_ARG, std::string(fn) + ": " + e.what());
+    }
+}
+
+void to_device_order(const mp_ctx* c, const double* src, std::vector<double>& dst, int n_sys) {
+    dst.assign((size_t)c->npad + 64, 0.0);
+    for (int d = 0; d < c->nv; ++d) dst[d] = src[c->perm[d]];
+    if (n_sys > c->nv) dst[c->nv] = src[c->nv];
+}
+
+void from_device_order(const mp_ctx* c, const std::vector<double>& src, double* dst, int n_sys) {
+    for (int d = 0; d < c->nv; ++d) dst[c->perm[d]] = src[d];
+    if (n_sys > c->nv) dst[c->nv] = src[c->nv];
+}
+
+void upload(const mp_ctx* c, double* dev, const std::vector<double>& h) {
+    MP_HIP(hipMemcpyAsync(dev, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    MP_HIP(hipStreamSynchronize(c->stream));
+}
+
+void download(const mp_ctx* c, const double* dev, std::vector<double>& h) {
+    h.resize((size_t)c->npad + 64);
+    MP_HIP(hipMemcpyAsync(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+    MP_HIP(hipStreamSynchronize(c->stream));
+}
+
+void alloc_vec(const mp_ctx* c, DevBuf<double>& b) {
+    b.alloc((size_t)c->npad + 64);
+    MP_HIP(hipMemsetAsync(b.p, 0, sizeof(double) * b.n, c->stream));
+}
+
+void slots_to_csr(const mp_ctx* c, const DevBuf<double>& dev, double* out) {
+    std::vector<double> h(c->total_slots);
+    MP_HIP(hipMemcpyAsync(h.data(), dev.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+    MP_HIP(hipStreamSynchronize(c->stream));
+    for (size_t t = 0; t < c->csr_slot.size(); ++t) out[t] = h[c->csr_slot[t]];
+}
+
+void csr_to_slots(const mp_ctx* c, const double* in, DevBuf<double>& dev) {
+    std::vector<double> h(c->total_slots, 0.0);
+    for (size_t t = 0; t < c->csr_slot.size(); ++t) h[c->csr_slot[t]] = in[t];
+    MP_HIP(hipMemcpyAsync(dev.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    MP_HIP(hipStreamSynchronize(c->stream));
+}
+
+void upload_flags(const mp_ctx* c, const uint8_t* src, DevBuf<uint8_t>& dev) {
+    std::vector<uint8_t> h((size_t)c->npad, 0);
+    if (src)
+        for (int d = 0; d < c->nv; ++d) h[d] = src[c->perm[d]] ? 1 : 0;
+    dev.alloc(h.size());
+    MP_HIP(hipMemcpy(dev.p, h.data(), h.size(), hipMemcpyHostToDevice));
+}
+
+}  // namespace
+
+extern "C" {
+
+int mp_ctx_create(int device, mp_ctx** out) {
+    if (!out) return MP_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MP_ERR_HIP;
+    if (device < 0 || device >= n) return MP_ERR_ARG;
+    mp_ctx* c = new mp_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return MP_ERR_HIP;
+    }
+    *out = c;
+    return MP_OK;
+}
+
+int mp_ctx_destroy(mp_ctx* c) {
+    if (!c) return MP_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& t : c->timers) t.release();
+    c->solver.reset();
+    c->ppe.reset();
+    hipStream_t s = c->stream;
+    delete c;
+    if (s) (void)hipStreamDestroy(s);
+    return MP_OK;
+}
+
+const char* mp_last_error(const mp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mp_sync(mp_ctx* c) {
+    return guarded(c, "mp_sync", [&]() -> int {
+        MP_HIP(hipStreamSynchronize(c->stream));
+        return MP_OK;
+    });
+}
+
+int mp_cloud_upload(mp_ctx* c, int nv, int dim, const double* xyz, const uint8_t* boundary,
+                    const double* normal, const int* row_ptr, const int* col_idx) {
+    return guarded(c, "mp_cloud_upload", [&]() -> int {
+        if (nv <= 0 || (dim != 2 && dim != 3) || !xyz || !boundary || !normal || !row_ptr ||
+            !col_idx)
+            return fail(c, MP_ERR_ARG, "mp_cloud_upload: bad arguments");
+        if (row_ptr[0] != 0) return fail(c, MP_ERR_ARG, "mp_cloud_upload: row_ptr[0] != 0");
+        int maxlen = 0;
+        for (int i = 0; i < nv; ++i) {
+            int l = row_ptr[i + 1] - row_ptr[i];
+            if (l < 1 || l > MAX_STENCIL)
+                return fail(c, MP_ERR_ARG,
+                            "mp_cloud_upload: row " + std::to_string(i) + " has " +
+                                std::to_string(l) + " entries (allowed 1.." +
+                                std::to_string(MAX_STENCIL) + ")");
+            if (col_idx[row_ptr[i]] != i)
+                return fail(c, MP_ERR_ARG, "mp_cloud_upload: entry 0 of row " +
+                                               std::to_string(i) + " is not the node itself");
+            for (int t = row_ptr[i]; t < row_ptr[i + 1]; ++t)
+                if (col_idx[t] < 0 || col_idx[t] >= nv)
+                    return fail(c, MP_ERR_ARG, "mp_cloud_upload: column index out of range");
+            maxlen = std::max(maxlen, l);
+        }
+        c->has_cloud = c->has_weights = c->has_fs = false;
+        c->solver.reset();
+        c->ppe.reset();
+        c->nv = nv;
+        c->dim = dim;
+        c->max_len = maxlen;
+        c->nslices = (nv + SLICE - 1) / SLICE;
+        c->npad = c->nslices * SLICE;
+        c->h_rowptr.assign(row_ptr, row_ptr + nv + 1);
+        c->nnz = row_ptr[nv];
+        // device order: stable sort by decreasing stencil length inside windows
+        c->perm.resize(nv);
+        std::iota(c->perm.begin(), c->perm.end(), 0);
+        for (int w0 = 0; w0 < nv; w0 += SIGMA) {
+            int w1 = std::min(nv, w0 + SIGMA);
+            std::stable_sort(c->perm.begin() + w0, c->perm.begin() + w1, [&](int a, int b) {
+                return (row_ptr[a + 1] - row_ptr[a]) > (row_ptr[b + 1] - row_ptr[b]);
+            });
+        }
+        c->iperm.assign(nv, 0);
+        for (int d = 0; d < nv; ++d) c->iperm[c->perm[d]] = d;
+        std::vector<int> hsptr(c->nslices + 1, 0), hlen(c->npad, 0);
+        for (int d = 0; d < nv; ++d) hlen[d] = row_ptr[c->perm[d] + 1] - row_ptr[c->perm[d]];
+        for (int s = 0; s < c->nslices; ++s) {
+            int wmax = 0;
+            for (int l = 0; l < SLICE; ++l) wmax = std::max(wmax, hlen[s * SLICE + l]);
+            hsptr[s + 1] = hsptr[s] + wmax * SLICE;
+        }
+        c->total_slots = hsptr[c->nslices];
+        std::vector<int> hcol(c->total_slots, 0);
+        c->csr_slot.assign(row_ptr[nv], 0);
+        for (int s = 0; s < c->nslices; ++s) {
+            int wmax = (hsptr[s + 1] - hsptr[s]) / SLICE;
+            for (int l = 0; l < SLICE; ++l) {
+                int d = s * SLICE + l;
+                for (int j = 0; j < wmax; ++j) {
+                    int slot = hsptr[s] + j * SLICE + l;
+                    if (d < nv && j < hlen[d]) {
+                        int t = row_ptr[c->perm[d]] + j;
+                        hcol[slot] = c->iperm[col_idx[t]];
+                        c->csr_slot[t] = slot;
+                    } else {
+                        hcol[slot] = (d < nv) ? d : 0;
+                    }
+                }
+            }
+        }
+        std::vector<double> hx((size_t)dim * c->npad, 0.0), hn((size_t)dim * c->npad, 0.0);
+        c->h_bnd.assign(c->npad, 0);
+        std::vector<int> hb;
+        for (int d = 0; d < nv; ++d) {
+            int o = c->perm[d];
+            for (int a = 0; a < dim; ++a) {
+                hx[(size_t)a * c->npad + d] = xyz[(size_t)o * dim + a];
+                hn[(size_t)a * c->npad + d] = normal[(size_t)o * dim + a];
+            }
+            c->h_bnd[d] = boundary[o] ? 1 : 0;
+            if (boundary[o]) hb.push_back(d);
+        }
+        for (int d : hb)
+            for (int t = row_ptr[c->perm[d]] + 1; t < row_ptr[c->perm[d] + 1]; ++t)
+                if (boundary[col_idx[t]])
+                    return fail(c, MP_ERR_ARG,
+                                "mp_cloud_upload: boundary node " + std::to_string(c->perm[d]) +
+                                    " couples to boundary node " + std::to_string(col_idx[t]));
+        c->nb = (int)hb.size();
+        c->sptr.alloc(hsptr.size());
+        c->col.alloc(hcol.size());
+        c->len.alloc(hlen.size());
+        c->brows.alloc(std::max<size_t>(1, hb.size()));
+        c->xyz.alloc(hx.size());
+        c->normal.alloc(hn.size());
+        c->bnd.alloc(c->h_bnd.size());
+        MP_HIP(hipMemcpy(c->sptr.p, hsptr.data(), sizeof(int) * hsptr.size(), hipMemcpyHostToDevice));
+        MP_HIP(hipMemcpy(c->col.p, hcol.data(), sizeof(int) * hcol.size(), hipMemcpyHostToDevice));
+        MP_HIP(hipMemcpy(c->len.p, hlen.data(), sizeof(int) * hlen.size(), hipMemcpyHostToDevice));
+        if (!hb.empty())
+            MP_HIP(hipMemcpy(c->brows.p, hb.data(), sizeof(int) * hb.size(), hipMemcpyHostToDevice));
+        MP_HIP(hipMemcpy(c->xyz.p, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice));
+        MP_HIP(hipMemcpy(c->normal.p, hn.data(), sizeof(double) * hn.size(), hipMemcpyHostToDevice));
+        MP_HIP(hipMemcpy(c->bnd.p, c->h_bnd.data(), c->h_bnd.size(), hipMemcpyHostToDevice));
+        c->sv.nv = nv;
+        c->sv.nslices = c->nslices;
+        c->sv.sptr = c->sptr.p;
+        c->sv.col = c->col.p;
+        c->sv.len = c->len.p;
+        for (auto& wb : c->w) {
+            wb.alloc(c->total_slots);
+            MP_HIP(hipMemsetAsync(wb.p, 0, sizeof(double) * wb.n, c->stream));
+        }
+        alloc_vec(c, c->vec_a);
+        alloc_vec(c, c->vec_b);
+        c->div_partial.alloc((size_t)((nv + BLOCK - 1) / BLOCK) * 4);
+        MP_HIP(hipStreamSynchronize(c->stream));
+        c->has_cloud = true;
+        return MP_OK;
+    });
+}
+
+int mp_weights_assemble(mp_ctx* c, int phs_deg, int poly_deg, const int* exps, int precision_mode,
+                        double* gx, double* gy, double* gz, double* lap, double* cond_out) {
+    return guarded(c, "mp_weights_assemble", [&]() -> int {
+        if (!c->has_cloud) return fail(c, MP_ERR_STATE, "mp_weights_assemble: no cloud uploaded");
+        if (phs_deg < 3 || phs_deg > 11 || (phs_deg % 2) == 0)
+            return fail(c, MP_ERR_ARG, "mp_weights_assemble: phs_deg must be 3, 5, 7, 9 or 11");
+        int m = (c->dim == 2) ? (poly_deg + 1) * (poly_deg + 2) / 2
+                              : (poly_deg + 1) * (poly_deg + 2) * (poly_deg + 3) / 6;
+        if (poly_deg < 0 || m > MAX_POLY)
+            return fail(c, MP_ERR_ARG, "mp_weights_assemble: poly_deg out of range");
+        if (!exps) return fail(c, MP_ERR_ARG, "mp_weights_assemble: exps is NULL");
+        int minlen = MAX_STENCIL + 1;
+        for (int i = 0; i < c->nv; ++i) minlen = std::min(minlen, c->h_rowptr[i + 1] - c->h_rowptr[i]);
+        if (minlen < m)
+            return fail(c, MP_ERR_ARG,
+                        "mp_weights_assemble: a stencil has fewer points than monomials");
+        if (precision_mode != MP_PREC_FP64 && precision_mode != MP_PREC_MIXED)
+            return fail(c, MP_ERR_ARG, "mp_weights_assemble: unknown precision_mode");
+        DevBuf<int> de;
+        de.alloc((size_t)m * c->dim);
+        MP_HIP(hipMemcpy(de.p, exps, sizeof(int) * m * c->dim, hipMemcpyHostToDevice));
+        DevBuf<double> dcond;
+        if (cond_out) dcond.alloc(c->npad);
+        c->timers[4].begin(c->stream);
+        launch_weights_fp64(c->stream, c->dim, phs_deg, m, de.p, c->sv, c->xyz.p, c->npad,
+                            c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p,
+                            cond_out ? dcond.p : nullptr, c->max_len);
+        c->timers[4].end(c->stream);
+        MP_HIP(hipStreamSynchronize(c->stream));
+        c->timers[4].harvest();
+        {
+            double M = (double)(c->max_len + m);
+            c->bytes[4] = (2.0 / 3.0) * M * M * M;  // flops per stencil (largest M)
+        }
+        c->has_weights = true;
+        if (gx) slots_to_csr(c, c->w[0], gx);
+        if (gy) slots_to_csr(c, c->w[1], gy);
+        if (gz && c->dim == 3) slots_to_csr(c, c->w[2], gz);
+        if (lap) slots_to_csr(c, c->w[3], lap);
+        if (cond_out) {
+            std::vector<double> h(c->npad);
+            MP_HIP(hipMemcpy(h.data(), dcond.p, sizeof(double) * c->npad, hipMemcpyDeviceToHost));
+            for (int d = 0; d < c->nv; ++d) cond_out[c->perm[d]] = h[d];
+        }
+        return MP_OK;
+    });
+}
+
+int mp_weights_set(mp_ctx* c, const double* gx, const double* gy, const double* gz,
+                   const double* lap) {
+    return guarded(c, "mp_weights_set", [&]() -> int {
+        if (!c->has_cloud) return fail(c, MP_ERR_STATE, "mp_weights_set: no cloud uploaded");
+        if (!gx || !gy || !lap || (c->dim == 3 && !gz))
+            return fail(c, MP_ERR_ARG, "mp_weights_set: missing operator");
+        csr_to_slots(c, gx, c->w[0]);
+        csr_to_slots(c, gy, c->w[1]);
+        if (c->dim == 3) csr_to_slots(c, gz, c->w[2]);
+        csr_to_slots(c, lap, c->w[3]);
+        c->has_weights = true;
+        return MP_OK;
+    });
+}
+
+int mp_operator_apply(mp_ctx* c, int op, const double* x, double* y) {
+    return guarded(c, "mp_operator_apply", [&]() -> int {
+        if (!c->has_weights) return fail(c, MP_ERR_STATE, "mp_operator_apply: no weights");
+        if (op < 0 || op > 3 || (op == MP_GZ && c->dim != 3) || !x || !y)
+            return fail(c, MP_ERR_ARG, "mp_operator_apply: bad operator or pointer");
+        std::vector<double> h;
+        to_device_order(c, x, h, c->nv);
+        upload(c, c->vec_a.p, h);
+        launch_spmv(c->stream, c->sv, c->w[op].p, c->vec_a.p, c->vec_b.p);
+        download(c, c->vec_b.p, h);
+        from_device_order(c, h, y, c->nv);
+        return MP_OK;
+    });
+}
+
+int mp_solver_init(mp_ctx* c, const uint8_t* dirichlet, double unsteady, double conv, double diff,
+                   double tolerance, int n_iter, int precision) {
+    return guarded(c, "mp_solver_init", [&]() -> int {
+        if (!c->has_weights) return fail(c, MP_ERR_STATE, "mp_solver_init: no weights");
+        if (!dirichlet || n_iter < 1 || !(tolerance > 0) ||
+            (precision != MP_PREC_FP64 && precision != MP_PREC_MIXED))
+            return fail(c, MP_ERR_ARG, "mp_solver_init: bad arguments");
+        bool bordered = true;
+        for (int i = 0; i < c->nv; ++i)
+            if (c->h_bnd[c->iperm[i]] && dirichlet[i]) {
+                bordered = false;
+                break;
+            }
+        upload_flags(c, dirichlet, c->solver_dir);
+        c->solver.reset(new PpeSolver());
+        c->solver->setup(c->stream, c->sv, c->dim, c->npad, c->w[0].p, c->w[1].p, c->w[2].p,
+                         c->w[3].p, c->normal.p, c->bnd.p, c->solver_dir.p, bordered, unsteady,
+                         conv, diff);
+        c->solver->tol = tolerance;
+        c->solver->maxit = n_iter;
+        c->solver->precision = precision;
+        c->solver->spmv_timer.enabled = c->timers[0].enabled;
+        return MP_OK;
+    });
+}
+
+int mp_solver_system_size(mp_ctx* c, int* n) {
+    return guarded(c, "mp_solver_system_size", [&]() -> int {
+        PpeSolver* s = c->solver ? c->solver.get() : c->ppe.get();
+        if (!s || !n) return fail(c, MP_ERR_STATE, "mp_solver_system_size: no solver");
+        *n = s->n;
+        return MP_OK;
+    });
+}
+
+int mp_solver_solve(mp_ctx* c, const double* rhs, double* field_new, double* field_old) {
+    return guarded(c, "mp_solver_solve", [&]() -> int {
+        if (!c->solver) return fail(c, MP_ERR_STATE, "mp_solver_solve: mp_solver_init not called");
+        if (!rhs || !field_new || !field_old) return fail(c, MP_ERR_ARG, "mp_solver_solve: NULL");
+        PpeSolver& S = *c->solver;
+        std::vector<double> hb, hx;
+        to_device_order(c, rhs, hb, S.n);
+        to_device_order(c, field_old, hx, S.n);
+        upload(c, c->vec_a.p, hb);
+        upload(c, c->vec_b.p, hx);
+        SolveStats st = S.solve(c->stream, c->vec_a.p, c->vec_b.p);
+        download(c, c->vec_b.p, hx);
+        from_device_order(c, hx, field_new, S.n);
+        if (S.bordered) field_old[c->nv] = field_new[c->nv];
+        c->stats.iterations = st.iterations;
+        c->stats.outer_cycles = st.outer_cycles;
+        c->stats.restarts = st.restarts;
+        c->stats.converged = st.converged;
+        c->stats.rel_res = st.rel_res;
+        c->stats.abs_res = st.abs_res;
+        c->stats.alpha = S.bordered ? field_new[c->nv] : 0.0;
+        c->stats.guess_rel_res = st.guess_rel_res;
+        c->stats.total_iterations += st.iterations;
+        c->stats.total_solves += 1;
+        return MP_OK;
+    });
+}
+
+int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const uint8_t* v_dir,
+                 const uint8_t* w_dir, const uint8_t* p_dir) {
+    return guarded(c, "mp_fs_create", [&]() -> int {
+        if (!c->has_weights) return fail(c, MP_ERR_STATE, "mp_fs_create: no weights");
+        if (!prm || !u_dir || !v_dir || !p_dir || (c->dim == 3 && !w_dir))
+            return fail(c, MP_ERR_ARG, "mp_fs_create: NULL argument");
+        // FRACTIONAL_STEP_1::check_bc, fractional_step_1.cpp:78-102
+        const uint8_t* dirs[3] = {u_dir, v_dir, w_dir};
+        for (int q = 0; q < c->dim; ++q) {
+            long sum = 0;
+            for (int i = 0; i < c->nv; ++i) sum += dirs[q][i] ? 1 : 0;
+            if (sum == 0)
+                return fail(c, MP_ERR_ARG,
+                            "mp_fs_create: full Neumann velocity BC is not permitted");
+        }
+        if (prm->rho < 0 || prm->mu < 0)
+            return fail(c, MP_ERR_ARG, "mp_fs_create: rho and mu must be set (>= 0)");
+        if (prm->temporal_order != 1 && prm->temporal_order != 2)
+            return fail(c, MP_ERR_ARG, "mp_fs_create: temporal_order must be 1 or 2");
+        if (!(prm->dt > 0) || !(prm->rho > 0) || !(prm->solver_tolerance > 0) || prm->n_iter < 1)
+            return fail(c, MP_ERR_ARG, "mp_fs_create: dt, rho, tolerance and n_iter must be > 0");
+        if (prm->precision != MP_PREC_FP64 && prm->precision != MP_PREC_MIXED)
+            return fail(c, MP_ERR_ARG, "mp_fs_create: unknown precision");
+        c->prm = *prm;
+        for (int q = 0; q < c->dim; ++q) upload_flags(c, dirs[q], c->udir[q]);
+        if (c->dim == 2) upload_flags(c, nullptr, c->udir[2]);
+        upload_flags(c, p_dir, c->pdir);
+        bool bordered = true;
+        for (int i = 0; i < c->nv; ++i)
+            if (c->h_bnd[c->iperm[i]] && p_dir[i]) {
+                bordered = false;
+                break;
+            }
+        c->ppe.reset(new PpeSolver());
+        c->ppe->setup(c->stream, c->sv, c->dim, c->npad, c->w[0].p, c->w[1].p, c->w[2].p,
+                      c->w[3].p, c->normal.p, c->bnd.p, c->pdir.p, bordered, 0.0, 0.0, 1.0);
+        c->ppe->tol = prm->solver_tolerance;
+        c->ppe->maxit = prm->n_iter;
+        c->ppe->precision = prm->precision;
+        c->ppe->inner_tol = (prm->inner_tolerance > 0) ? prm->inner_tolerance : 1e-3;
+        c->ppe->check_every = (prm->check_every > 0) ? prm->check_every : 8;
+        c->ppe->spmv_timer.enabled = c->timers[0].enabled;
+        for (int k = 0; k < 2; ++k)
+            for (int q = 0; q < 3; ++q) alloc_vec(c, c->U[k][q]);
+        alloc_vec(c, c->P);
+        alloc_vec(c, c->b);
+        for (int q = 0; q < 3; ++q) alloc_vec(c, c->uh[q]);
+        for (int k = 0; k < 2; ++k) {
+            c->S[k].alloc((size_t)3 * c->npad + 64);
+            MP_HIP(hipMemsetAsync(c->S[k].p, 0, sizeof(double) * c->S[k].n, c->stream));
+        }
+        MP_HIP(hipStreamSynchronize(c->stream));
+        c->cur = 0;
+        c->s_cur = 0;
+        c->stats = mp_step_stats{};
+        // algorithmic bytes per launch (real entries only)
+        const double W = (double)c->nnz, N = (double)c->nv, D = (double)c->dim;
+        c->bytes[1] = W * (4.0 + 8.0 * (D + 1.0)) + N * (4.0 + 8.0 * (D /*own*/ + D /*gather*/ +
+                                                                  D /*uh*/ + D /*S*/ + D /*S_oo*/));
+        c->bytes[2] = W * (4.0 + 8.0 * D) + N * (4.0 + 8.0 * (D /*uh*/ + 1.0 /*b*/));
+        c->bytes[3] = W * (4.0 + 8.0 * D) + N * (4.0 + 8.0 * (1.0 + D /*uh*/ + D /*un*/));
+        c->has_fs = true;
+        return MP_OK;
+    });
+}
+
+int mp_fields_set(mp_ctx* c, int which, const double* values) {
+    return guarded(c, "mp_fields_set", [&]() -> int {
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_fields_set: mp_fs_create not called");
+        if (!values || which < 0 || which > 3 || (which == MP_W && c->dim != 3))
+            return fail(c, MP_ERR_ARG, "mp_fields_set: bad field");
+        std::vector<double> h;
+        if (which == MP_P) {
+            to_device_order(c, values, h, c->ppe->n);
+            upload(c, c->P.p, h);
+        } else {
+            to_device_order(c, values, h, c->nv);
+            upload(c, c->U[c->cur][which].p, h);
+        }
+        return MP_OK;
+    });
+}
+
+int mp_fields_get(mp_ctx* c, int which, double* values) {
+    return guarded(c, "mp_fields_get", [&]() -> int {
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_fields_get: mp_fs_create not called");
+        if (!values || which < 0 || which > 3 || (which == MP_W && c->dim != 3))
+            return fail(c, MP_ERR_ARG, "mp_fields_get: bad field");
+        std::vector<double> h;
+        if (which == MP_P) {
+            download(c, c->P.p, h);
+            from_device_order(c, h, values, c->ppe->n);
+        } else {
+            download(c, c->U[c->cur][which].p, h);
+            from_device_order(c, h, values, c->nv);
+        }
+        return MP_OK;
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+template <int DIM>
+void fs_step_one(mp_ctx* c, int it) {
+    const mp_fs_params& q = c->prm;
+    const int o = c->cur, nw = c->cur ^ 1;
+    DevBuf<double>(&Uo)[3] = c->U[o];
+    DevBuf<double>(&Un)[3] = c->U[nw];
+    const int euler = (q.temporal_order == 1 || it == 0) ? 1 : 0;
+    double* S_new = c->S[c->s_cur ^ 1].p;
+    const double* S_oo = c->S[c->s_cur].p;
+    // calc_vel_hat
+    c->timers[1].begin(c->stream);
+    launch_predictor<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p, Uo[0].p,
+                          Uo[1].p, Uo[2].p, c->uh[0].p, c->uh[1].p, c->uh[2].p, S_new, S_oo,
+                          c->npad, q.rho, q.mu, q.dt, euler);
+    c->timers[1].end(c->stream);
+    // calc_pressure
+    c->timers[2].begin(c->stream);
+    launch_ppe_rhs<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->uh[0].p, c->uh[1].p,
+                        c->uh[2].p, Uo[0].p, Uo[1].p, Uo[2].p, c->P.p, c->bnd.p, c->pdir.p,
+                        c->normal.p, c->npad, q.rho, q.dt, c->b.p, c->ppe->bordered);
+    c->timers[2].end(c->stream);
+    SolveStats st = c->ppe->solve(c->stream, c->b.p, c->P.p);
+    // calc_vel_corr
+    c->timers[3].begin(c->stream);
+    launch_corrector<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->P.p, c->uh[0].p,
+                          c->uh[1].p, c->uh[2].p, Un[0].p, Un[1].p, Un[2].p, c->npad, q.rho, q.dt);
+    c->timers[3].end(c->stream);
+    launch_boundary_fix<DIM>(c->stream, c->sv, c->brows.p, c->nb, c->w[0].p, c->w[1].p, c->w[2].p,
+                             c->normal.p, c->npad, c->pdir.p, c->udir[0].p, c->udir[1].p,
+                             c->udir[2].p, Uo[0].p, Uo[1].p, Uo[2].p, Un[0].p, Un[1].p, Un[2].p);
+    // extras (AB2 history) and the driver's old <- new
+    if (q.temporal_order == 2) c->s_cur ^= 1;
+    c->cur = nw;
+    c->stats.iterations = st.iterations;
+    c->stats.outer_cycles = st.outer_cycles;
+    c->stats.restarts = st.restarts;
+    c->stats.converged = st.converged;
+    c->stats.rel_res = st.rel_res;
+    c->stats.abs_res = st.abs_res;
+    c->stats.alpha = st.alpha;
+    c->stats.guess_rel_res = st.guess_rel_res;
+    c->stats.total_iterations += st.iterations;
+    c->stats.total_solves += 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mp_fs_step(mp_ctx* c, int n_steps, int it0) {
+    return guarded(c, "mp_fs_step", [&]() -> int {
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_fs_step: mp_fs_create not called");
+        if (n_steps < 0 || it0 < 0) return fail(c, MP_ERR_ARG, "mp_fs_step: negative count");
+        if (c->guess_mode == MP_GUESS_LEARNED)
+            return fail(c, MP_ERR_STATE, "mp_fs_step: learned guess not trained");
+        for (int k = 0; k < n_steps; ++k) {
+            if (c->dim == 2)
+                fs_step_one<2>(c, it0 + k);
+            else
+                fs_step_one<3>(c, it0 + k);
+        }
+        MP_HIP(hipStreamSynchronize(c->stream));
+        for (int t = 1; t <= 3; ++t) c->timers[t].harvest();
+        return MP_OK;
+    });
+}
+
+int mp_stats_get(mp_ctx* c, mp_step_stats* stats) {
+    return guarded(c, "mp_stats_get", [&]() -> int {
+        if (!stats) return fail(c, MP_ERR_ARG, "mp_stats_get: NULL");
+        *stats = c->stats;
+        return MP_OK;
+    });
+}
+
+int mp_guess_mode(mp_ctx* c, int mode) {
+    return guarded(c, "mp_guess_mode", [&]() -> int {
+        if (mode != MP_GUESS_PREVIOUS && mode != MP_GUESS_LEARNED)
+            return fail(c, MP_ERR_ARG, "mp_guess_mode: unknown mode");
+        c->guess_mode = mode;
+        return MP_OK;
+    });
+}
+
+int mp_divergence(mp_ctx* c, double* out) {
+    return guarded(c, "mp_divergence", [&]() -> int {
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_divergence: mp_fs_create not called");
+        if (!out) return fail(c, MP_ERR_ARG, "mp_divergence: NULL");
+        const int g = (c->nv + BLOCK - 1) / BLOCK;
+        DevBuf<double>(&U)[3] = c->U[c->cur];
+        if (c->dim == 2)
+            launch_divergence<2>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, U[0].p, U[1].p,
+                                 U[2].p, c->bnd.p, c->div_partial.p, g);
+        else
+            launch_divergence<3>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, U[0].p, U[1].p,
+                                 U[2].p, c->bnd.p, c->div_partial.p, g);
+        std::vector<double> h((size_t)g * 4);
+        MP_HIP(hipMemcpyAsync(h.data(), c->div_partial.p, sizeof(double) * h.size(),
+                              hipMemcpyDeviceToHost, c->stream));
+        MP_HIP(hipStreamSynchronize(c->stream));
+        double mi = 0, si = 0, mb = 0, sb = 0;
+        for (int k = 0; k < g; ++k) {
+            mi = std::max(mi, h[k * 4 + 0]);
+            si += h[k * 4 + 1];
+            mb = std::max(mb, h[k * 4 + 2]);
+            sb += h[k * 4 + 3];
+        }
+        int nb = c->nb, ni = c->nv - c->nb;
+        out[0] = mi;
+        out[1] = ni ? si / ni : 0.0;
+        out[2] = mb;
+        out[3] = nb ? sb / nb : 0.0;
+        out[4] = std::max(mi, mb);
+        out[5] = (si + sb) / c->nv;
+        return MP_OK;
+    });
+}
+
+int mp_kernel_timing(mp_ctx* c, int enable) {
+    return guarded(c, "mp_kernel_timing", [&]() -> int {
+        for (auto& t : c->timers) {
+            t.enabled = enable != 0;
+            t.launches = 0;
+            t.total_ms = 0.0;
+        }
+        for (PpeSolver* s : {c->solver.get(), c->ppe.get()})
+            if (s) {
+                s->spmv_timer.enabled = enable != 0;
+                s->spmv_timer.launches = 0;
+                s->spmv_timer.total_ms = 0.0;
+            }
+        return MP_OK;
+    });
+}
+
+int mp_kernel_timing_get(mp_ctx* c, int which, long* launches, double* mean_ms,
+                         double* bytes_per_launch) {
+    return guarded(c, "mp_kernel_timing_get", [&]() -> int {
+        if (which < 0 || which > 4 || !launches || !mean_ms || !bytes_per_launch)
+            return fail(c, MP_ERR_ARG, "mp_kernel_timing_get: bad arguments");
+        if (which == 0) {
+            PpeSolver* s = c->ppe ? c->ppe.get() : c->solver.get();
+            if (!s) return fail(c, MP_ERR_STATE, "mp_kernel_timing_get: no solver");
+            *launches = s->spmv_timer.launches;
+            *mean_ms = s->spmv_timer.launches ? s->spmv_timer.total_ms / s->spmv_timer.launches : 0;
+            *bytes_per_launch = s->spmv_bytes(s->precision == MP_PREC_MIXED ? 1 : 0);
+        } else {
+            KernelTimer& t = c->timers[which];
+            *launches = t.launches;
+            *mean_ms = t.launches ? t.total_ms / t.launches : 0.0;
+            *bytes_per_launch = c->bytes[which];
+        }
+        return MP_OK;
+    });
+}
+
+}  // extern "C"

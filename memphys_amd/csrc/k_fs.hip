@@ -1,0 +1,389 @@
+// Parts (b) and (d): predictor, pressure right-hand side, corrector, boundary
+// fix-up and continuity residual over SELL-64 stencil operators.
+//
+// Layout: device rows are grouped into slices of 64 (one wavefront); the
+// k-th neighbour of the rows of a slice sits at slot sptr[slice] + k*64 + lane,
+// so every neighbour step is one coalesced 64-lane load of column ids and of
+// each weight array.  Rows are sorted by stencil length inside windows of
+// 4096 rows, so a slice holds rows of (nearly) one length.  One thread owns
+// one row and accumulates in stencil order with fma, the oracle's order.
+#include "mp_internal.hpp"
+
+namespace mp {
+
+namespace {
+
+__device__ __forceinline__ int row_base(const SellView& sv, int r) {
+    return sv.sptr[r >> 6] + (r & 63);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void predictor_kernel(
+    SellView sv, const double* __restrict__ gx, const double* __restrict__ gy,
+    const double* __restrict__ gz, const double* __restrict__ lap, const double* __restrict__ u,
+    const double* __restrict__ v, const double* __restrict__ w, double* __restrict__ uh,
+    double* __restrict__ vh, double* __restrict__ wh, double* __restrict__ S_new,
+    const double* __restrict__ S_oo, int npad, double rho, double mu, double dt, int euler) {
+    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r >= sv.nv) return;
+    const int base = row_base(sv, r);
+    const int len = sv.len[r];
+    double acc[DIM][4];
+#pragma unroll
+    for (int f = 0; f < DIM; ++f)
+#pragma unroll
+        for (int o = 0; o < 4; ++o) acc[f][o] = 0.0;
+    for (int j = 0; j < len; ++j) {
+        const int slot = base + j * SLICE;
+        const int c = sv.col[slot];
+        const double a0 = gx[slot], a1 = gy[slot], a3 = lap[slot];
+        const double a2 = (DIM == 3) ? gz[slot] : 0.0;
+        double f[DIM];
+        f[0] = u[c];
+        f[1] = v[c];
+        if (DIM == 3) f[DIM - 1] = w[c];
+#pragma unroll
+        for (int q = 0; q < DIM; ++q) {
+            acc[q][0] = fma(a0, f[q], acc[q][0]);
+            acc[q][1] = fma(a1, f[q], acc[q][1]);
+            if (DIM == 3) acc[q][2] = fma(a2, f[q], acc[q][2]);
+            acc[q][3] = fma(a3, f[q], acc[q][3]);
+        }
+    }
+    double own[DIM];
+    own[0] = u[r];
+    own[1] = v[r];
+    if (DIM == 3) own[DIM - 1] = w[r];
+    double* out[3] = {uh, vh, wh};
+#pragma unroll
+    for (int q = 0; q < DIM; ++q) {
+        double conv = -(own[0] * acc[q][0]) - own[1] * acc[q][1];
+        if (DIM == 3) conv = conv - own[DIM - 1] * acc[q][2];
+        const double S = rho * conv + mu * acc[q][3];
+        S_new[(size_t)q * npad + r] = S;
+        double res;
+        if (euler)
+            res = own[q] + (dt / rho) * S;
+        else
+            res = own[q] + (dt / rho) * (1.5 * S - 0.5 * S_oo[(size_t)q * npad + r]);
+        out[q][r] = res;
+    }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void ppe_rhs_kernel(
+    SellView sv, const double* __restrict__ gx, const double* __restrict__ gy,
+    const double* __restrict__ gz, const double* __restrict__ uh, const double* __restrict__ vh,
+    const double* __restrict__ wh, const double* __restrict__ u, const double* __restrict__ v,
+    const double* __restrict__ w, const double* __restrict__ p_old,
+    const uint8_t* __restrict__ bnd, const uint8_t* __restrict__ pdir,
+    const double* __restrict__ nrm, int npad, double rho, double dt, double* __restrict__ b,
+    int bordered) {
+    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    if (bordered && r == 0) b[sv.nv] = 0.0;
+    if (r >= sv.nv) return;
+    double out;
+    if (bnd[r]) {
+        if (pdir[r]) {
+            out = p_old[r];
+        } else {
+            double s = (u[r] - uh[r]) * nrm[r] + (v[r] - vh[r]) * nrm[(size_t)npad + r];
+            if (DIM == 3) s = s + (w[r] - wh[r]) * nrm[2 * (size_t)npad + r];
+            out = -rho * s / dt;
+        }
+    } else {
+        const int base = row_base(sv, r);
+        const int len = sv.len[r];
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int j = 0; j < len; ++j) {
+            const int slot = base + j * SLICE;
+            const int c = sv.col[slot];
+            ax = fma(gx[slot], uh[c], ax);
+            ay = fma(gy[slot], vh[c], ay);
+            if (DIM == 3) az = fma(gz[slot], wh[c], az);
+        }
+        out = ax + ay;
+        if (DIM == 3) out = out + az;
+        out = out * rho / dt;
+    }
+    b[r] = out;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void corrector_kernel(
+    SellView sv, const double* __restrict__ gx, const double* __restrict__ gy,
+    const double* __restrict__ gz, const double* __restrict__ p, const double* __restrict__ uh,
+    const double* __restrict__ vh, const double* __restrict__ wh, double* __restrict__ un,
+    double* __restrict__ vn, double* __restrict__ wn, double rho, double dt) {
+    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r >= sv.nv) return;
+    const int base = row_base(sv, r);
+    const int len = sv.len[r];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int j = 0; j < len; ++j) {
+        const int slot = base + j * SLICE;
+        const double pc = p[sv.col[slot]];
+        ax = fma(gx[slot], pc, ax);
+        ay = fma(gy[slot], pc, ay);
+        if (DIM == 3) az = fma(gz[slot], pc, az);
+    }
+    un[r] = uh[r] - (dt * ax) / rho;
+    vn[r] = vh[r] - (dt * ay) / rho;
+    if (DIM == 3) wn[r] = wh[r] - (dt * az) / rho;
+}
+
+// Boundary rows with Neumann pressure: Dirichlet velocity keeps its old value,
+// Neumann velocity solves the n.grad row for the diagonal entry
+// (fractional_step_1.cpp:159-208).  Boundary stencils hold only interior
+// neighbours besides the node itself, so rows are independent.
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void boundary_fix_kernel(
+    SellView sv, const int* __restrict__ brows, int nb, const double* __restrict__ gx,
+    const double* __restrict__ gy, const double* __restrict__ gz, const double* __restrict__ nrm,
+    int npad, const uint8_t* __restrict__ pdir, const uint8_t* __restrict__ udir,
+    const uint8_t* __restrict__ vdir, const uint8_t* __restrict__ wdir,
+    const double* __restrict__ u_old, const double* __restrict__ v_old,
+    const double* __restrict__ w_old, double* __restrict__ un, double* __restrict__ vn,
+    double* __restrict__ wn) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= nb) return;
+    const int r = brows[t];
+    if (pdir[r]) return;
+    const uint8_t* dirs[3] = {udir, vdir, wdir};
+    const double* olds[3] = {u_old, v_old, w_old};
+    double* news[3] = {un, vn, wn};
+    const double nx = nrm[r], ny = nrm[(size_t)npad + r];
+    const double nz = (DIM == 3) ? nrm[2 * (size_t)npad + r] : 0.0;
+    const int base = row_base(sv, r);
+    const int len = sv.len[r];
+    for (int q = 0; q < DIM; ++q) {
+        if (dirs[q][r]) {
+            news[q][r] = olds[q][r];
+        } else {
+            double rhs = 0.0, diag = 1.0;
+            for (int j = 0; j < len; ++j) {
+                const int slot = base + j * SLICE;
+                const int c = sv.col[slot];
+                double cf = gx[slot] * nx + gy[slot] * ny;
+                if (DIM == 3) cf = cf + gz[slot] * nz;
+                if (c == r)
+                    diag = cf;
+                else
+                    rhs = rhs - cf * news[q][c];
+            }
+            news[q][r] = rhs / diag;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void spmv_kernel(SellView sv, const double* __restrict__ val,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ y) {
+    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r >= sv.nv) return;
+    const int base = row_base(sv, r);
+    const int len = sv.len[r];
+    double s = 0.0;
+    for (int j = 0; j < len; ++j) {
+        const int slot = base + j * SLICE;
+        s = fma(val[slot], x[sv.col[slot]], s);
+    }
+    y[r] = s;
+}
+
+// partial[blk*4 + {0,1,2,3}] = {max |r| interior, sum |r| interior, max boundary, sum boundary}
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void divergence_kernel(
+    SellView sv, const double* __restrict__ gx, const double* __restrict__ gy,
+    const double* __restrict__ gz, const double* __restrict__ u, const double* __restrict__ v,
+    const double* __restrict__ w, const uint8_t* __restrict__ bnd, double* __restrict__ partial) {
+    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    double vals[4] = {0.0, 0.0, 0.0, 0.0};
+    if (r < sv.nv) {
+        const int base = row_base(sv, r);
+        const int len = sv.len[r];
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int j = 0; j < len; ++j) {
+            const int slot = base + j * SLICE;
+            const int c = sv.col[slot];
+            ax = fma(gx[slot], u[c], ax);
+            ay = fma(gy[slot], v[c], ay);
+            if (DIM == 3) az = fma(gz[slot], w[c], az);
+        }
+        double res = ax + ay;
+        if (DIM == 3) res = res + az;
+        const double e = fabs(res);
+        if (bnd[r]) {
+            vals[2] = e;
+            vals[3] = e;
+        } else {
+            vals[0] = e;
+            vals[1] = e;
+        }
+    }
+    __shared__ double sh[4][BLOCK / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        vals[0] = fmax(vals[0], __shfl_xor(vals[0], off));
+        vals[1] += __shfl_xor(vals[1], off);
+        vals[2] = fmax(vals[2], __shfl_xor(vals[2], off));
+        vals[3] += __shfl_xor(vals[3], off);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q < 4; ++q) sh[q][wid] = vals[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double o[4] = {sh[0][0], sh[1][0], sh[2][0], sh[3][0]};
+        for (int t = 1; t < BLOCK / 64; ++t) {
+            o[0] = fmax(o[0], sh[0][t]);
+            o[1] += sh[1][t];
+            o[2] = fmax(o[2], sh[2][t]);
+            o[3] += sh[3][t];
+        }
+        for (int q = 0; q < 4; ++q) partial[blockIdx.x * 4 + q] = o[q];
+    }
+}
+
+// Implicit operator values in SELL slots (solver.cpp:165-214) and its diagonal.
+__global__ __launch_bounds__(BLOCK) void build_operator_kernel(
+    SellView sv, int dim, const double* __restrict__ gx, const double* __restrict__ gy,
+    const double* __restrict__ gz, const double* __restrict__ lap, const double* __restrict__ nrm,
+    int npad, const uint8_t* __restrict__ bnd, const uint8_t* __restrict__ dir, double unsteady,
+    double conv, double diff, double* __restrict__ aval, double* __restrict__ diag) {
+    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r >= sv.nv) return;
+    const int base = row_base(sv, r);
+    const int len = sv.len[r];
+    double dg = 0.0;
+    for (int j = 0; j < len; ++j) {
+        const int slot = base + j * SLICE;
+        const int c = sv.col[slot];
+        double val;
+        if (bnd[r]) {
+            if (dir[r]) {
+                val = (c == r) ? 1.0 : 0.0;
+            } else {
+                val = nrm[r] * gx[slot] + nrm[(size_t)npad + r] * gy[slot];
+                if (dim == 3) val += nrm[2 * (size_t)npad + r] * gz[slot];
+            }
+        } else {
+            val = diff * lap[slot];
+            val += conv * gx[slot];
+            val += conv * gy[slot];
+            if (dim == 3) val += conv * gz[slot];
+            if (c == r) val += unsteady;
+        }
+        aval[slot] = val;
+        if (c == r) dg = val;
+    }
+    diag[r] = dg;
+}
+
+}  // namespace
+
+template <int DIM>
+void launch_predictor(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
+                      const double* gz, const double* lap, const double* u, const double* v,
+                      const double* w, double* uh, double* vh, double* wh, double* S_new,
+                      const double* S_oo, int npad, double rho, double mu, double dt, int euler) {
+    const int g = (sv.nv + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(predictor_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, lap, u,
+                       v, w, uh, vh, wh, S_new, S_oo, npad, rho, mu, dt, euler);
+    MP_HIP(hipGetLastError());
+}
+
+template <int DIM>
+void launch_ppe_rhs(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
+                    const double* gz, const double* uh, const double* vh, const double* wh,
+                    const double* u, const double* v, const double* w, const double* p_old,
+                    const uint8_t* bnd, const uint8_t* pdir, const double* normal_soa, int npad,
+                    double rho, double dt, double* b, int bordered) {
+    const int g = (sv.nv + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(ppe_rhs_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, uh, vh, wh,
+                       u, v, w, p_old, bnd, pdir, normal_soa, npad, rho, dt, b, bordered);
+    MP_HIP(hipGetLastError());
+}
+
+template <int DIM>
+void launch_corrector(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
+                      const double* gz, const double* p, const double* uh, const double* vh,
+                      const double* wh, double* un, double* vn, double* wn, int npad, double rho,
+                      double dt) {
+    (void)npad;
+    const int g = (sv.nv + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(corrector_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, p, uh,
+                       vh, wh, un, vn, wn, rho, dt);
+    MP_HIP(hipGetLastError());
+}
+
+template <int DIM>
+void launch_boundary_fix(hipStream_t s, const SellView& sv, const int* brows, int nb,
+                         const double* gx, const double* gy, const double* gz,
+                         const double* normal_soa, int npad, const uint8_t* pdir,
+                         const uint8_t* udir, const uint8_t* vdir, const uint8_t* wdir,
+                         const double* u_old, const double* v_old, const double* w_old,
+                         double* un, double* vn, double* wn) {
+    if (nb == 0) return;
+    const int g = (nb + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(boundary_fix_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, brows, nb, gx, gy,
+                       gz, normal_soa, npad, pdir, udir, vdir, wdir, u_old, v_old, w_old, un, vn,
+                       wn);
+    MP_HIP(hipGetLastError());
+}
+
+void launch_spmv(hipStream_t s, const SellView& sv, const double* val, const double* x,
+                 double* y) {
+    const int g = (sv.nv + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(spmv_kernel, dim3(g), dim3(BLOCK), 0, s, sv, val, x, y);
+    MP_HIP(hipGetLastError());
+}
+
+template <int DIM>
+void launch_divergence(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
+                       const double* gz, const double* u, const double* v, const double* w,
+                       const uint8_t* bnd, double* partial, int nblocks) {
+    hipLaunchKernelGGL(divergence_kernel<DIM>, dim3(nblocks), dim3(BLOCK), 0, s, sv, gx, gy, gz, u,
+                       v, w, bnd, partial);
+    MP_HIP(hipGetLastError());
+}
+
+void launch_build_operator(hipStream_t s, const SellView& sv, int dim, const double* gx,
+                           const double* gy, const double* gz, const double* lap,
+                           const double* normal_soa, int npad, const uint8_t* bnd,
+                           const uint8_t* dir, double unsteady, double conv, double diff,
+                           double* aval, double* diag) {
+    const int g = (sv.nv + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(build_operator_kernel, dim3(g), dim3(BLOCK), 0, s, sv, dim, gx, gy, gz, lap,
+                       normal_soa, npad, bnd, dir, unsteady, conv, diff, aval, diag);
+    MP_HIP(hipGetLastError());
+}
+
+#define MP_INST(D)                                                                              \
+    template void launch_predictor<D>(hipStream_t, const SellView&, const double*, const double*, \
+                                      const double*, const double*, const double*, const double*, \
+                                      const double*, double*, double*, double*, double*,        \
+                                      const double*, int, double, double, double, int);         \
+    template void launch_ppe_rhs<D>(hipStream_t, const SellView&, const double*, const double*,   \
+                                    const double*, const double*, const double*, const double*,   \
+                                    const double*, const double*, const double*, const double*,   \
+                                    const uint8_t*, const uint8_t*, const double*, int, double,   \
+                                    double, double*, int);                                        \
+    template void launch_corrector<D>(hipStream_t, const SellView&, const double*, const double*, \
+                                      const double*, const double*, const double*, const double*, \
+                                      const double*, double*, double*, double*, int, double,      \
+                                      double);                                                    \
+    template void launch_boundary_fix<D>(hipStream_t, const SellView&, const int*, int,           \
+                                         const double*, const double*, const double*,             \
+                                         const double*, int, const uint8_t*, const uint8_t*,      \
+                                         const uint8_t*, const uint8_t*, const double*,           \
+                                         const double*, const double*, double*, double*,          \
+                                         double*);                                                \
+    template void launch_divergence<D>(hipStream_t, const SellView&, const double*,               \
+                                       const double*, const double*, const double*,               \
+                                       const double*, const double*, const uint8_t*, double*,     \
+                                       int);
+MP_INST(2)
+MP_INST(3)
+#undef MP_INST
+
+}  // namespace mp

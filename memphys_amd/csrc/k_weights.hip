@@ -1,0 +1,279 @@
+// Part (a): batched PHS r^p + polynomial stencil weights, FP64.
+//
+// One workgroup (256 threads, 4 wavefronts) per stencil.  The saddle-point
+// matrix [Phi P; P^T 0] (M = k + m) and the d+1 right-hand sides live in LDS;
+// partial-pivot LU runs right-looking with one barrier-separated phase per
+// step (pivot search on wave 0, row swap, multipliers, trailing update), then a
+// column-oriented back substitution.  The arithmetic order is fixed: each
+// matrix entry sees the same fma sequence as the oracle's sequential LU, so
+// the weights agree bit for bit with it.
+//
+// Follows coefficient_computations.cpp:33-328 (shifting_scaling, A and rhs
+// assembly, partialPivLu().solve) and cloud.cpp:207-279 (loop over nodes,
+// row 0 of the solution is the stencil's weight row).
+#include "mp_internal.hpp"
+
+namespace mp {
+
+namespace {
+
+__device__ __forceinline__ double rpow_d(double r2, int q) {
+    if (q >= 1) {
+        double v = sqrt(r2);
+        for (int s = 0; s < (q - 1) / 2; ++s) v = v * r2;
+        return v;
+    }
+    double v = 1.0 / sqrt(r2);
+    for (int s = 0; s < (-q - 1) / 2; ++s) v = v / r2;
+    return v;
+}
+
+__device__ __forceinline__ double ipow_d(double x, int e) {
+    double v = 1.0;
+    for (int s = 0; s < e; ++s) v = v * x;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void weights_fp64_kernel(
+    int dim, int phs, int m, const int* __restrict__ exps, SellView sv,
+    const double* __restrict__ xyz, int npad, double* __restrict__ gx, double* __restrict__ gy,
+    double* __restrict__ gz, double* __restrict__ lap, double* __restrict__ cond) {
+    extern __shared__ double lds[];
+    __shared__ double s_mn[3], s_sc[3];
+    __shared__ int s_piv;
+    __shared__ int s_e[MAX_POLY * 3];
+
+    const int r = blockIdx.x;
+    if (r >= sv.nv) return;
+    const int tid = threadIdx.x;
+    const int k = sv.len[r];
+    const int M = k + m;
+    const int LDA = M + 1;
+    const int nr = dim + 1;
+    double* A = lds;
+    double* R = A + M * LDA;
+    double* xs = R + M * 4;
+    const int base = sv.sptr[r / SLICE] + (r % SLICE);
+
+    for (int i = tid; i < m * dim; i += blockDim.x) s_e[i] = exps[i];
+    for (int t = tid; t < k; t += blockDim.x) {
+        const int c = sv.col[base + t * SLICE];
+        for (int a = 0; a < dim; ++a) xs[t * 3 + a] = xyz[(size_t)a * npad + c];
+    }
+    __syncthreads();
+    if (tid < dim) {
+        double mn = xs[tid], mx = xs[tid];
+        for (int t = 0; t < k; ++t) {
+            double v = xs[t * 3 + tid];
+            if (mn > v) mn = v;
+            if (mx < v) mx = v;
+        }
+        s_mn[tid] = mn;
+        s_sc[tid] = mx - mn;
+    }
+    __syncthreads();
+    for (int t = tid; t < k * dim; t += blockDim.x) {
+        const int j = t / dim, a = t - j * dim;
+        xs[j * 3 + a] = (xs[j * 3 + a] - s_mn[a]) / s_sc[a];
+    }
+    __syncthreads();
+
+    // ---- assemble A (M x M) ----
+    for (int idx = tid; idx < M * M; idx += blockDim.x) {
+        const int i = idx / M, j = idx - i * M;
+        double v = 0.0;
+        if (i < k && j < k) {
+            if (i != j) {
+                double s = 0.0;
+                for (int a = 0; a < dim; ++a) {
+                    double d = xs[i * 3 + a] - xs[j * 3 + a];
+                    s = s + d * d;
+                }
+                v = rpow_d(s, phs);
+            }
+        } else if (i < k || j < k) {
+            const int pt = (i < k) ? i : j;
+            const int t = (i < k) ? j - k : i - k;
+            v = ipow_d(xs[pt * 3 + 0], s_e[t * dim + 0]);
+            for (int a = 1; a < dim; ++a) v = v * ipow_d(xs[pt * 3 + a], s_e[t * dim + a]);
+        }
+        A[i * LDA + j] = v;
+    }
+    // ---- right-hand sides: gx, gy, (gz), lap ----
+    const double p = (double)phs;
+    for (int i = tid; i < M; i += blockDim.x) {
+        if (i < k) {
+            double r2 = 0.0;
+            for (int a = 0; a < dim; ++a) {
+                double d = xs[i * 3 + a] - xs[a];
+                r2 = r2 + d * d;
+            }
+            for (int a = 0; a < dim; ++a) {
+                double v = (-p) * (xs[i * 3 + a] - xs[a]);
+                v = v * rpow_d(r2, phs - 2);
+                R[i * 4 + a] = v / s_sc[a];
+            }
+            double lp = 0.0;
+            if (fabs(r2) > 1e-14) {
+                double g[3] = {0.0, 0.0, 0.0};
+                for (int a = 0; a < dim; ++a) {
+                    double d = xs[i * 3 + a] - xs[a];
+                    double d2 = d * d;
+                    double q4 = rpow_d(r2, phs - 4);
+                    double v = (p * p) * d2 * q4;
+                    v = v - (2.0 * p) * d2 * q4;
+                    v = v + p * rpow_d(r2, phs - 2);
+                    g[a] = v / (s_sc[a] * s_sc[a]);
+                }
+                lp = g[0] + g[1];
+                if (dim == 3) lp = lp + g[2];
+            }
+            R[i * 4 + dim] = lp;
+        } else {
+            const int t = i - k;
+            const int* e = s_e + t * dim;
+            const double x0 = xs[0], x1 = xs[1], x2 = (dim == 3) ? xs[2] : 0.0;
+            for (int a = 0; a < dim; ++a) {
+                double v = 0.0;
+                if (e[a] > 0) {
+                    if (a == 1) {
+                        v = ipow_d(x0, e[0]);
+                        v = v * (double)e[1];
+                        v = v * ipow_d(x1, e[1] - 1);
+                        if (dim == 3) v = v * ipow_d(x2, e[2]);
+                    } else if (a == 0) {
+                        v = (double)e[0] * ipow_d(x0, e[0] - 1);
+                        v = v * ipow_d(x1, e[1]);
+                        if (dim == 3) v = v * ipow_d(x2, e[2]);
+                    } else {
+                        v = (double)e[2] * ipow_d(x2, e[2] - 1);
+                        v = v * ipow_d(x0, e[0]);
+                        v = v * ipow_d(x1, e[1]);
+                    }
+                    v = v / s_sc[a];
+                }
+                R[i * 4 + a] = v;
+            }
+            double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+            if (e[0] > 1) {
+                double v = (double)e[0] * ((double)e[0] - 1.0) * ipow_d(x0, e[0] - 2);
+                v = v * ipow_d(x1, e[1]);
+                if (dim == 3) v = v * ipow_d(x2, e[2]);
+                g0 = v / (s_sc[0] * s_sc[0]);
+            }
+            if (e[1] > 1) {
+                double v = (double)e[1] * ((double)e[1] - 1.0) * ipow_d(x1, e[1] - 2);
+                v = v * ipow_d(x0, e[0]);
+                if (dim == 3) v = v * ipow_d(x2, e[2]);
+                g1 = v / (s_sc[1] * s_sc[1]);
+            }
+            if (dim == 3 && e[2] > 1) {
+                double v = (double)e[2] * ((double)e[2] - 1.0) * ipow_d(x2, e[2] - 2);
+                v = v * ipow_d(x0, e[0]);
+                v = v * ipow_d(x1, e[1]);
+                g2 = v / (s_sc[2] * s_sc[2]);
+            }
+            double lp = g0 + g1;
+            if (dim == 3) lp = lp + g2;
+            R[i * 4 + dim] = lp;
+        }
+    }
+    __syncthreads();
+
+    // ---- LU with partial pivoting, right-looking ----
+    for (int c = 0; c < M; ++c) {
+        if (tid < 64) {
+            double best = -1.0;
+            int bi = M;
+            for (int i = c + tid; i < M; i += 64) {
+                double v = fabs(A[i * LDA + c]);
+                if (v > best) {
+                    best = v;
+                    bi = i;
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                double ob = __shfl_xor(best, off);
+                int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bi = oi;
+                }
+            }
+            if (tid == 0) s_piv = bi;
+        }
+        __syncthreads();
+        const int piv = s_piv;
+        if (piv != c) {
+            for (int j = tid; j < M + nr; j += blockDim.x) {
+                if (j < M) {
+                    double t0 = A[c * LDA + j];
+                    A[c * LDA + j] = A[piv * LDA + j];
+                    A[piv * LDA + j] = t0;
+                } else {
+                    const int q = j - M;
+                    double t0 = R[c * 4 + q];
+                    R[c * 4 + q] = R[piv * 4 + q];
+                    R[piv * 4 + q] = t0;
+                }
+            }
+            __syncthreads();
+        }
+        const double d = A[c * LDA + c];
+        if (d != 0.0) {
+            for (int i = c + 1 + tid; i < M; i += blockDim.x) A[i * LDA + c] = A[i * LDA + c] / d;
+            __syncthreads();
+            const int nrow = M - c - 1;
+            const int ncol = M - c - 1 + nr;
+            for (int idx = tid; idx < nrow * ncol; idx += blockDim.x) {
+                const int ii = idx / ncol;
+                const int jj = idx - ii * ncol;
+                const int i = c + 1 + ii;
+                const double l = A[i * LDA + c];
+                if (jj < nrow) {
+                    const int j = c + 1 + jj;
+                    A[i * LDA + j] = fma(-l, A[c * LDA + j], A[i * LDA + j]);
+                } else {
+                    const int q = jj - nrow;
+                    R[i * 4 + q] = fma(-l, R[c * 4 + q], R[i * 4 + q]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // ---- back substitution (column oriented) ----
+    for (int c = M - 1; c >= 0; --c) {
+        if (tid < nr) R[c * 4 + tid] = R[c * 4 + tid] / A[c * LDA + c];
+        __syncthreads();
+        for (int idx = tid; idx < c * nr; idx += blockDim.x) {
+            const int i = idx / nr, q = idx - i * nr;
+            R[i * 4 + q] = fma(-A[i * LDA + c], R[c * 4 + q], R[i * 4 + q]);
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < k; t += blockDim.x) {
+        const size_t slot = (size_t)base + (size_t)t * SLICE;
+        gx[slot] = R[t * 4 + 0];
+        gy[slot] = R[t * 4 + 1];
+        if (dim == 3) gz[slot] = R[t * 4 + 2];
+        lap[slot] = R[t * 4 + dim];
+    }
+    if (cond && tid == 0) cond[r] = __builtin_nan("");
+}
+
+}  // namespace
+
+void launch_weights_fp64(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
+                         const SellView& sv, const double* xyz_soa, int npad, double* gx,
+                         double* gy, double* gz, double* lap, double* cond, int max_len) {
+    const int M = max_len + m;
+    const size_t lds = sizeof(double) * ((size_t)M * (M + 1) + (size_t)M * 4 + (size_t)max_len * 3);
+    if (lds > 64 * 1024)
+        MP_HIP(hipFuncSetAttribute((const void*)weights_fp64_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(weights_fp64_kernel, dim3(sv.nv), dim3(256), lds, s, dim, phs, m, exps_dev,
+                       sv, xyz_soa, npad, gx, gy, gz, lap, cond);
+    MP_HIP(hipGetLastError());
+}
+
+}  // namespace mp

@@ -1,0 +1,116 @@
+// Internal declarations shared by the HIP translation units of libmemphys_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/memphys_hip.h"
+
+namespace mp {
+
+constexpr int SLICE = 64;       // rows per SELL slice (one wavefront)
+constexpr int SIGMA = 4096;     // rows per length-sorting window
+constexpr int BLOCK = 256;      // threads per block of the row kernels
+constexpr int MAX_STENCIL = 96; // largest stencil (k) the weight kernel accepts
+constexpr int MAX_POLY = 35;    // largest monomial count (degree 4 in 3D)
+
+struct HipError {
+    hipError_t code;
+    const char* what;
+};
+
+#define MP_HIP(call)                                         \
+    do {                                                     \
+        hipError_t _e = (call);                              \
+        if (_e != hipSuccess) throw ::mp::HipError{_e, #call}; \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        free();
+        if (count == 0) count = 1;
+        MP_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { free(); }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// SELL-64 view of the stencil set in device order.
+struct SellView {
+    int nv;            // real rows
+    int nslices;
+    const int* sptr;   // [nslices+1] first slot of each slice
+    const int* col;    // [slots] device column ids
+    const int* len;    // [nslices*64] row lengths (0 on ghost rows)
+};
+
+// Device-side Krylov scalars (FP64).  Updated only by the last block of a
+// reduction, read by every block of the following kernels.
+struct KScalars {
+    double rho, rho_old, alpha, omega, beta;
+    double r0sq, bb, rr, tol2, eps2;
+    double red[4];          // scratch results of the last reduction
+    int done;               // 1 converged, 2 iteration cap, 3 breakdown (restart needed)
+    int iter, maxit, pad;
+    unsigned int counter[4];
+};
+
+// ---- launchers (defined in the .hip files) ----
+void launch_weights_fp64(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
+                         const SellView& sv, const double* xyz_soa, int npad, double* gx,
+                         double* gy, double* gz, double* lap, double* cond, int max_len);
+
+template <int DIM>
+void launch_predictor(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
+                      const double* gz, const double* lap, const double* u, const double* v,
+                      const double* w, double* uh, double* vh, double* wh, double* S_new,
+                      const double* S_oo, int npad, double rho, double mu, double dt, int euler);
+template <int DIM>
+void launch_ppe_rhs(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
+                    const double* gz, const double* uh, const double* vh, const double* wh,
+                    const double* u, const double* v, const double* w, const double* p_old,
+                    const uint8_t* bnd, const uint8_t* pdir, const double* normal_soa,
+                    int npad, double rho, double dt, double* b, int bordered);
+template <int DIM>
+void launch_corrector(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
+                      const double* gz, const double* p, const double* uh, const double* vh,
+                      const double* wh, double* un, double* vn, double* wn, int npad, double rho,
+                      double dt);
+template <int DIM>
+void launch_boundary_fix(hipStream_t s, const SellView& sv, const int* brows, int nb,
+                         const double* gx, const double* gy, const double* gz,
+                         const double* normal_soa, int npad, const uint8_t* pdir,
+                         const uint8_t* udir, const uint8_t* vdir, const uint8_t* wdir,
+                         const double* u_old, const double* v_old, const double* w_old,
+                         double* un, double* vn, double* wn);
+void launch_spmv(hipStream_t s, const SellView& sv, const double* val, const double* x, double* y);
+template <int DIM>
+void launch_divergence(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
+                       const double* gz, const double* u, const double* v, const double* w,
+                       const uint8_t* bnd, double* partial, int nblocks);
+
+// Pressure operator in SELL slots: value arrays from the stencil weights.
+void launch_build_operator(hipStream_t s, const SellView& sv, int dim, const double* gx,
+                           const double* gy, const double* gz, const double* lap,
+                           const double* normal_soa, int npad, const uint8_t* bnd,
+                           const uint8_t* dir, double unsteady, double conv, double diff,
+                           double* aval, double* diag);
+
+// Krylov building blocks (k_krylov.hip).
+struct KrylovBufs;
+void launch_fill(hipStream_t s, double* x, double v, int n);
+
+}  // namespace mp

@@ -1,0 +1,242 @@
+"""Pins for the oracle (CPU): golden vectors, closed forms and library solvers.
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+term, a wrong sign, a wrong scale factor or index) fails it:
+  * LAPACK golden weights (tests/golden/make_golden.py, independent numpy code);
+  * polynomial exactness of every operator up to the appended degree;
+  * RBF exactness: derivatives of sum_j c_j phi(|x~ - x~_j|) with P^T c = 0;
+  * SuperLU solution of the bordered pressure system;
+  * predictor / pressure right-hand side / corrector on polynomial fields,
+    whose derivatives the operators reproduce exactly.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import lid_bc, rel_err
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "stencil_weights.npz")
+
+
+def test_golden_weights_match_lapack():
+    g = np.load(GOLDEN)
+    for i in range(int(g["ncases"])):
+        dim, k, phs, deg = (int(v) for v in g[f"c{i}_meta"])
+        X, W = g[f"c{i}_X"], g[f"c{i}_W"]
+        gx, gy, gz, lap, cond = O.single_stencil(X, phs, deg)
+        ours = [gx, gy] + ([gz] if dim == 3 else []) + [lap]
+        for a, b in zip(ours, W):
+            assert rel_err(a, b) <= 1e-10, (i, rel_err(a, b))
+        # Hager's estimate is a lower bound of the exact 1-norm condition number
+        c1 = float(g[f"c{i}_cond1"])
+        assert 0.3 * c1 <= cond <= c1 * (1 + 1e-8)
+
+
+def test_refinement_keeps_weights():
+    g = np.load(GOLDEN)
+    X = g["c6_X"]
+    a = O.single_stencil(X, 3, 3, refine=0)
+    b = O.single_stencil(X, 3, 3, refine=2)
+    for u, v in zip(a[:4], b[:4]):
+        assert rel_err(u, v) <= 1e-11
+
+
+def _monomials(dim, deg):
+    ex = O.poly_exponents(dim, deg)
+    return ex
+
+
+def _eval_mono(x, e):
+    v = np.ones(x.shape[0])
+    for a in range(x.shape[1]):
+        v = v * x[:, a] ** e[a]
+    return v
+
+
+def _deriv_mono(x, e, a):
+    if e[a] == 0:
+        return np.zeros(x.shape[0])
+    e2 = list(e)
+    e2[a] -= 1
+    return e[a] * _eval_mono(x, e2)
+
+
+def _lap_mono(x, e):
+    out = np.zeros(x.shape[0])
+    for a in range(x.shape[1]):
+        if e[a] >= 2:
+            e2 = list(e)
+            e2[a] -= 2
+            out += e[a] * (e[a] - 1) * _eval_mono(x, e2)
+    return out
+
+
+@pytest.mark.parametrize("which", ["cavity2d", "cube3d_small"])
+def test_polynomial_exactness(which, request):
+    cloud = request.getfixturevalue(which)
+    gx, gy, gz, lap, _ = O.stencil_weights(cloud, 3, 3)
+    ops = [gx, gy] + ([gz] if cloud.dim == 3 else [])
+    x = cloud.xyz
+    for e in _monomials(cloud.dim, 3):
+        f = _eval_mono(x, e)
+        for a, w in enumerate(ops):
+            d = O.spmv(cloud, w, f)
+            assert np.abs(d - _deriv_mono(x, e, a)).max() <= 1e-9
+        l = O.spmv(cloud, lap, f)
+        assert np.abs(l - _lap_mono(x, e)).max() <= 1e-6  # |L| entries ~1e5 (h ~ 1e-2)
+
+
+def test_rbf_part_exactness():
+    """D f(x_c) = sum_j w_j f(x_j) for f = sum_j c_j |x~ - x~_j|^3 with P^T c = 0."""
+    g = np.load(GOLDEN)
+    for i in (0, 5, 6):
+        dim, k, phs, deg = (int(v) for v in g[f"c{i}_meta"])
+        X = g[f"c{i}_X"]
+        gx, gy, gz, lap, _ = O.single_stencil(X, phs, deg)
+        mn, mx = X.min(axis=0), X.max(axis=0)
+        sc = mx - mn
+        xs = (X - mn) / sc
+        E = O.poly_exponents(dim, deg)
+        P = np.stack([_eval_mono(xs, e) for e in E], axis=1)
+        rng = np.random.default_rng(i)
+        c = rng.standard_normal(k)
+        c -= P @ np.linalg.lstsq(P, c, rcond=None)[0]  # project onto null(P^T)
+        assert np.abs(P.T @ c).max() < 1e-10
+        d = xs[None, :, :] - xs[:, None, :]        # d[i, j] = x~_j - x~_i
+        r = np.sqrt(np.sum(d * d, axis=2))
+        f = (r ** phs) @ c                          # f at each stencil node
+        # exact derivatives of f at the centre (index 0), physical coordinates
+        dc = xs[0][None, :] - xs                    # x~_c - x~_j
+        rc = np.sqrt(np.sum(dc * dc, axis=1))
+        grads = []
+        for a in range(dim):
+            grads.append(np.sum(c * phs * dc[:, a] * rc ** (phs - 2)) / sc[a])
+        lapv = 0.0
+        for a in range(dim):
+            term = phs * (phs - 2) * dc[:, a] ** 2 * np.where(rc > 0, rc, 1.0) ** (phs - 4) \
+                + phs * rc ** (phs - 2)
+            term = np.where(rc > 0, term, 0.0)
+            lapv += np.sum(c * term) / sc[a] ** 2
+        ws = [gx, gy] + ([gz] if dim == 3 else [])
+        scale = np.abs(f).max()
+        for a in range(dim):
+            assert abs(ws[a] @ f - grads[a]) <= 1e-9 * max(scale, 1.0) * max(1.0 / sc[a], 1.0)
+        assert abs(lap @ f - lapv) <= 1e-8 * max(abs(lapv), 1.0)
+
+
+def test_poly_exponent_order_known():
+    assert O.poly_exponents(2, 3).tolist() == [[0, 0], [1, 0], [0, 1], [2, 0], [1, 1], [0, 2],
+                                               [3, 0], [2, 1], [1, 2], [0, 3]]
+    assert O.poly_exponents(3, 3).tolist() == [
+        [0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [2, 0, 0], [1, 1, 0], [1, 0, 1], [0, 2, 0],
+        [0, 1, 1], [0, 0, 2], [3, 0, 0], [2, 1, 0], [2, 0, 1], [1, 2, 0], [1, 1, 1], [1, 0, 2],
+        [0, 3, 0], [0, 2, 1], [0, 1, 2], [0, 0, 3]]
+
+
+def _small(dim, n, k, layers=(None, None, None)):
+    import memphys_amd as M
+    return M.box_cloud(dim=dim, n=n, jitter=0.25, seed=3, k_interior=k, k_layers=layers)
+
+
+def test_ppe_solve_matches_superlu():
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    c = _small(2, 30, 30)
+    w = O.stencil_weights(c, 3, 3)[:4]
+    u_dir, p_dir, _ = lid_bc(c, 1)
+    for precond in (1, 2):
+        fs = O.FractionalStep(c, w, u_dir, u_dir, None, p_dir, 1.0, 0.01, 1e-3, 2,
+                              precond=precond, tol=1e-12, maxit=20000)
+        rp, ci, v = fs.ppe_csr()
+        n = fs.n_sys
+        assert n == c.nv + 1
+        A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+        b = np.random.default_rng(1).standard_normal(n)
+        b[-1] = 0.0
+        x, xo, it, rr = fs.solve(b, np.zeros(n))
+        assert rr <= 1e-11
+        assert rel_err(x, spl.spsolve(A.tocsc(), b)) <= 1e-8
+        assert xo[-1] == x[-1]
+
+
+def test_dense_direct_matches_superlu():
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    c = _small(2, 14, 20)
+    w = O.stencil_weights(c, 3, 2)[:4]
+    u_dir, p_dir, _ = lid_bc(c, 1)
+    fs = O.FractionalStep(c, w, u_dir, u_dir, None, p_dir, 1.0, 0.01, 1e-3, 2, solver_kind=1)
+    rp, ci, v = fs.ppe_csr()
+    A = sp.csr_matrix((v, ci, rp), shape=(fs.n_sys, fs.n_sys))
+    b = np.random.default_rng(2).standard_normal(fs.n_sys)
+    x, _, _, rr = fs.solve(b, np.zeros(fs.n_sys))
+    assert rel_err(x, spl.spsolve(A.tocsc(), b)) <= 1e-10
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_predictor_rhs_corrector_closed_form(dim):
+    """Quadratic velocity, Euler predictor: S = rho(-u.grad u) + mu lap u exactly;
+    the pressure RHS is rho/dt div(uh) (interior, exact for cubic uh) and
+    -rho((u - uh).n)/dt on Neumann boundary rows; the corrector subtracts
+    dt/rho grad p exactly for a cubic pressure."""
+    c = _small(dim, 12 if dim == 2 else 9, 30 if dim == 2 else 40,
+               (None, None, None) if dim == 2 else (60, 55, 50))
+    w = O.stencil_weights(c, 3, 3)[:4]
+    nv = c.nv
+    x = c.xyz
+    rho, mu, dt = 1.3, 0.02, 1e-3
+    # polynomial fields
+    U = [0.3 + x[:, 0] * x[:, 1], x[:, 0] ** 2 - 0.5 * x[:, 1]]
+    dU = [[x[:, 1], x[:, 0]], [2 * x[:, 0], -0.5 + 0 * x[:, 0]]]
+    lapU = [0 * x[:, 0], 2 + 0 * x[:, 0]]
+    if dim == 3:
+        U = [U[0] + x[:, 2] ** 2, U[1] + x[:, 1] * x[:, 2], 0.2 * x[:, 0] * x[:, 2]]
+        dU = [[x[:, 1], x[:, 0], 2 * x[:, 2]], [2 * x[:, 0], -0.5 + x[:, 2], x[:, 1]],
+              [0.2 * x[:, 2], 0 * x[:, 0], 0.2 * x[:, 0]]]
+        lapU = [2 + 0 * x[:, 0], 2 + 0 * x[:, 0], 0 * x[:, 0]]
+    ones = np.ones(nv, dtype=np.uint8)
+    p_dir = np.zeros(nv, dtype=np.uint8)
+    fs = O.FractionalStep(c, w, ones, ones, ones if dim == 3 else None, p_dir, rho, mu, dt, 1)
+    p0 = np.zeros(fs.n_sys)
+    uh, b = fs.predict(0, U[0], U[1], U[2] if dim == 3 else None, p0)
+    for q in range(dim):
+        conv = -sum(U[a] * dU[q][a] for a in range(dim))
+        S = rho * conv + mu * lapU[q]
+        exact = U[q] + dt / rho * S
+        assert np.abs(uh[q] - exact).max() <= 1e-9
+    # divergence of the exact cubic uh, analytically: uh = U + dt/rho * S(U)
+    eps = 1e-6
+    inter = c.boundary == 0
+    # b_interior / (rho/dt) equals sum_q d(uh_q)/dx_q; check against operators on exact uh
+    div_num = sum(O.spmv(c, w[q], uh[q]) for q in range(dim))
+    assert np.abs(b[:nv][inter] - div_num[inter] * rho / dt).max() <= 1e-6 * max(1, np.abs(b).max())
+    bnd = np.flatnonzero(c.boundary == 1)
+    s = sum((U[q][bnd] - uh[q][bnd]) * c.normal[bnd, q] for q in range(dim))
+    assert np.allclose(b[bnd], -rho * s / dt, rtol=1e-12, atol=1e-12)
+    assert b[-1] == 0.0
+    # corrector with a cubic pressure, Dirichlet velocity on the boundary
+    pc = np.zeros(fs.n_sys)
+    pc[:nv] = x[:, 0] ** 3 + x[:, 1] * x[:, 0]
+    gp = [3 * x[:, 0] ** 2 + x[:, 1], x[:, 0]] + ([0 * x[:, 0]] if dim == 3 else [])
+    un = fs.correct(U[0], U[1], U[2] if dim == 3 else None, uh, pc)
+    for q in range(dim):
+        exact = uh[q] - dt * gp[q] / rho
+        exact[bnd] = U[q][bnd]  # Dirichlet velocity keeps its old value
+        assert np.abs(un[q] - exact).max() <= 1e-9
+    del eps
+
+
+def test_divergence_of_solenoidal_field_is_zero():
+    c = _small(3, 9, 40, (60, 55, 50))
+    gx, gy, gz, lap, _ = O.stencil_weights(c, 3, 3)
+    x = c.xyz
+    u = x[:, 0] ** 2 * x[:, 1]
+    v = -x[:, 0] * x[:, 1] ** 2 + x[:, 2]
+    w = 0.5 * x[:, 1] + x[:, 0] * 0.0
+    out, res = O.divergence(c, gx, gy, gz, u, v, w)
+    assert np.abs(res).max() <= 1e-9
+    assert out[4] <= 1e-9
+    out2, _ = O.divergence(c, gx, gy, gz, x[:, 0], 0 * u, 0 * u)
+    assert abs(out2[4] - 1.0) <= 1e-9 and abs(out2[5] - 1.0) <= 1e-9
