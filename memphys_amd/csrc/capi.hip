@@ -212,9 +212,36 @@ int mp_cloud_upload(mp_ctx* c, int nv, int dim, const double* xyz, const uint8_t
         c->npad = c->nslices * SLICE;
         c->h_rowptr.assign(row_ptr, row_ptr + nv + 1);
         c->nnz = row_ptr[nv];
-        // device order: stable sort by decreasing stencil length inside windows
+        // device order: Morton (Z-order) of the coordinates, so that 64
+        // consecutive rows are a compact cluster whose stencils overlap, then a
+        // stable sort by decreasing stencil length inside windows of SIGMA rows
         c->perm.resize(nv);
         std::iota(c->perm.begin(), c->perm.end(), 0);
+        {
+            double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            for (int a = 0; a < dim; ++a) {
+                lo[a] = hi[a] = xyz[a];
+                for (int i = 0; i < nv; ++i) {
+                    lo[a] = std::min(lo[a], xyz[(size_t)i * dim + a]);
+                    hi[a] = std::max(hi[a], xyz[(size_t)i * dim + a]);
+                }
+            }
+            const int bits = (dim == 3) ? 10 : 16;
+            const double levels = (double)(1u << bits);
+            std::vector<uint64_t> key(nv, 0);
+            for (int i = 0; i < nv; ++i) {
+                uint64_t k = 0;
+                for (int a = 0; a < dim; ++a) {
+                    double span = hi[a] - lo[a];
+                    double t = span > 0 ? (xyz[(size_t)i * dim + a] - lo[a]) / span : 0.0;
+                    uint64_t q = (uint64_t)std::min(levels - 1.0, std::floor(t * levels));
+                    for (int bt = 0; bt < bits; ++bt) k |= ((q >> bt) & 1ull) << (bt * dim + a);
+                }
+                key[i] = k;
+            }
+            std::stable_sort(c->perm.begin(), c->perm.end(),
+                             [&](int a, int b) { return key[a] < key[b]; });
+        }
         for (int w0 = 0; w0 < nv; w0 += SIGMA) {
             int w1 = std::min(nv, w0 + SIGMA);
             std::stable_sort(c->perm.begin() + w0, c->perm.begin() + w1, [&](int a, int b) {

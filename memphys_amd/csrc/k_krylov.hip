@@ -5,16 +5,18 @@
 // scaling is folded into the stored operator (A D^-1, column scaling) so the
 // preconditioner costs nothing per iteration and x = D^-1 x~ at the end.
 //
-// Per iteration five launches: p-update, SpMV + r0.v, s-update, SpMV + (t.s,
-// t.t), x/r-update + (r.r, r0.r).  Reductions are two-stage and
-// deterministic: each block stores its partial, the last block to finish
-// (atomic ticket) sums the partials in block order and updates the scalars
+// Per iteration: p-update, SpMV + partial r0.v, finish(alpha), s-update,
+// SpMV + partial (t.s, t.t), finish(omega), x/r-update + partial (r.r, r0.r),
+// finish(loop head).  Reductions are two-stage and deterministic: every block
+// stores its partial sums (plain stores, no same-address atomics), then a
+// single 1024-thread block sums them in block order and updates the scalars
 // (alpha, omega, beta, convergence flag) in device memory.  Every kernel
-// returns at entry once the flag is set, so the host can launch iterations
-// in chunks and poll the flag asynchronously.
+// returns at entry once the flag is set, so the host launches iterations in
+// chunks and polls the flag asynchronously.
 //
 // The full-Neumann border (solver.cpp:205-213) is applied as a rank-one
-// term: rows i < nv add x[nv]; row nv is sum_j D^-1_j x_j, formed by the last block.
+// term: rows i < nv add x[nv]; row nv is sum_j D^-1_j x_j, formed by the
+// finishing kernel.
 #include "krylov.hpp"
 
 #include <algorithm>
@@ -25,19 +27,14 @@ namespace mp {
 
 namespace {
 
-__device__ __forceinline__ double coherent_load(const double* p) {
-    unsigned long long v = __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)v);
-}
+constexpr int FIN = 1024;  // threads of the single-block finishing kernels
 
-// Block-reduces NV sums; returns true in the last block to finish, with the
-// grid totals in tot (identical in every thread of that block).
+// Block sums of NV values stored at partial[q * nb + lb] (lb: logical block)
+// (plain stores: no atomics, the finishing kernel runs after the grid).
 template <int NV>
-__device__ bool reduce_last(double (&v)[NV], double* partial, unsigned int* counter,
-                            double (&tot)[NV]) {
+__device__ __forceinline__ void block_store(double (&v)[NV], double* __restrict__ partial, int lb,
+                                            int nb) {
     __shared__ double sh[NV][BLOCK / 64];
-    __shared__ int s_last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; ++q)
@@ -46,30 +43,30 @@ __device__ bool reduce_last(double (&v)[NV], double* partial, unsigned int* coun
 #pragma unroll
         for (int q = 0; q < NV; ++q) sh[q][wid] = v[q];
     __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            double s = sh[q][0];
-            for (int t = 1; t < BLOCK / 64; ++t) s += sh[q][t];
-            partial[(size_t)blockIdx.x * NV + q] = s;
-        }
-        __threadfence();
-        unsigned int prev = atomicAdd(counter, 1u);
-        s_last = (prev == gridDim.x - 1) ? 1 : 0;
+    if (threadIdx.x < NV) {
+        const int q = threadIdx.x;
+        double s = sh[q][0];
+        for (int t = 1; t < BLOCK / 64; ++t) s += sh[q][t];
+        partial[(size_t)q * nb + lb] = s;
     }
-    __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
+}
+
+// Sum over nb block partials per value, in a fixed order (deterministic);
+// every thread of the (single) block receives the totals.
+template <int NV>
+__device__ __forceinline__ void finish_sum(const double* __restrict__ partial, int nb,
+                                           double (&tot)[NV]) {
+    __shared__ double sh[NV][FIN / 64];
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLOCK)
+    for (int b = threadIdx.x; b < nb; b += FIN)
 #pragma unroll
-        for (int q = 0; q < NV; ++q) acc[q] += coherent_load(partial + (size_t)b * NV + q);
+        for (int q = 0; q < NV; ++q) acc[q] += partial[(size_t)q * nb + b];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; ++q)
         for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off);
-    __syncthreads();
     if (lane == 0)
 #pragma unroll
         for (int q = 0; q < NV; ++q) sh[q][wid] = acc[q];
@@ -77,23 +74,30 @@ __device__ bool reduce_last(double (&v)[NV], double* partial, unsigned int* coun
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         double s = sh[q][0];
-        for (int t = 1; t < BLOCK / 64; ++t) s += sh[q][t];
+        for (int t = 1; t < FIN / 64; ++t) s += sh[q][t];
         tot[q] = s;
     }
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
 }
 
 // Loop head of Eigen's BiCGSTAB: convergence / iteration-cap test, then
-// rho_old = rho, rho = r0.r, restart test, beta.
+// rho_old = rho, rho = r0.r, restart test, beta.  A non-finite residual or a
+// zero omega (which would make beta infinite) stops with done = 5.
 __device__ void loop_top(KScalars* sc, double rr, double r0r) {
     sc->rr = rr;
+    if (!(rr <= 1.0e300)) {
+        sc->done = 5;
+        return;
+    }
     if (rr <= sc->tol2) {
         sc->done = 1;
         return;
     }
     if (sc->iter >= sc->maxit) {
         sc->done = 2;
+        return;
+    }
+    if (sc->omega == 0.0) {
+        sc->done = 5;
         return;
     }
     sc->rho_old = sc->rho;
@@ -105,83 +109,128 @@ __device__ void loop_top(KScalars* sc, double rr, double r0r) {
     sc->beta = (sc->rho / sc->rho_old) * (sc->alpha / sc->omega);
 }
 
+// Row r of the SELL-64 product A x: the slice width is uniform across the
+// wavefront, so the neighbour loop runs in lockstep, unrolled by four with
+// the column/value loads issued ahead of the gathers.  Padding slots hold a
+// zero value and a valid column, which leaves the fma chain unchanged.
+// Matrix streams use non-temporal loads so the gathered vector stays cached.
 template <typename T>
-__device__ __forceinline__ T sell_row(const SellView& sv, const T* __restrict__ a,
-                                      const T* __restrict__ x, int r) {
-    const int base = sv.sptr[r >> 6] + (r & 63);
-    const int len = sv.len[r];
-    T s = (T)0;
-    for (int j = 0; j < len; ++j) {
-        const int slot = base + j * SLICE;
-        s = fma(a[slot], x[sv.col[slot]], s);
+__device__ __forceinline__ T sell_dot(const int* __restrict__ sptr, const int* __restrict__ col,
+                                      const T* __restrict__ a, const T* __restrict__ x, int s,
+                                      int lane) {
+    const int b0 = sptr[s];
+    const int w = (sptr[s + 1] - b0) >> 6;
+    const int* cp = col + b0 + lane;
+    const T* ap = a + b0 + lane;
+    T acc = (T)0;
+    int j = 0;
+    for (; j + 8 <= w; j += 8) {
+        int c[8];
+        T av[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = __builtin_nontemporal_load(cp + u * SLICE);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) av[u] = __builtin_nontemporal_load(ap + u * SLICE);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = fma(av[u], xv[u], acc);
+        cp += 8 * SLICE;
+        ap += 8 * SLICE;
     }
-    return s;
+    for (; j < w; ++j) {
+        const int c0 = __builtin_nontemporal_load(cp);
+        const T a0 = __builtin_nontemporal_load(ap);
+        acc = fma(a0, x[c0], acc);
+        cp += SLICE;
+        ap += SLICE;
+    }
+    return acc;
 }
 
-// r = rhs - A~ x; r0 = r; optionally p = v = 0.  mode 0: fresh start, mode 1: restart.
+// Slice of this wavefront under the XCD-aware numbering (lb: logical block).
+__device__ __forceinline__ int wave_slice(int lb) {
+    return __builtin_amdgcn_readfirstlane(lb * (BLOCK / SLICE) + (int)(threadIdx.x >> 6));
+}
+
+// r = rhs - A~ x; r0 = r; p = v = 0 on a fresh start (mode 0).
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_residual_init(
     SellView sv, int bordered, const T* __restrict__ a, const T* __restrict__ dinv,
     const T* __restrict__ rhs, const T* __restrict__ x, T* __restrict__ r, T* __restrict__ r0,
-    T* __restrict__ p, T* __restrict__ v, double* partial, KScalars* sc, int mode, int rel_to_rhs,
-    double tol_rel) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    T* __restrict__ p, T* __restrict__ v, double* __restrict__ partial, int mode) {
+    const int nbk = (sv.nv + BLOCK - 1) / BLOCK;
+    const int lb = xcd_block(nbk);
+    if (lb < 0) return;
+    const int i = lb * BLOCK + threadIdx.x;
+    const int s = wave_slice(lb);
     const int nv = sv.nv;
-    double vals[3] = {0.0, 0.0, 0.0};  // r.r, rhs.rhs, sum dinv x
-    if (i < nv) {
-        T ax = sell_row(sv, a, x, i);
-        if (bordered) ax = ax + x[nv];
-        const T ri = rhs[i] - ax;
-        r[i] = ri;
-        r0[i] = ri;
-        if (mode == 0) {
-            p[i] = (T)0;
-            v[i] = (T)0;
-        }
-        vals[0] = (double)ri * (double)ri;
-        vals[1] = (double)rhs[i] * (double)rhs[i];
-        if (bordered) vals[2] = (double)dinv[i] * (double)x[i];
-    }
-    double tot[3];
-    if (!reduce_last<3>(vals, partial, &sc->counter[0], tot)) return;
-    if (threadIdx.x == 0) {
-        double rr = tot[0], bb = tot[1];
-        if (bordered) {
-            const T ax = (T)tot[2];
-            const T rn = rhs[nv] - ax;
-            r[nv] = rn;
-            r0[nv] = rn;
+    double vals[3] = {0.0, 0.0, 0.0};  // r.r, rhs.rhs, sum D^-1 x
+    if (s < sv.nslices) {
+        T ax = sell_dot(sv.sptr, sv.col, a, x, s, threadIdx.x & 63);
+        if (i < nv) {
+            if (bordered) ax = ax + x[nv];
+            const T ri = rhs[i] - ax;
+            r[i] = ri;
+            r0[i] = ri;
             if (mode == 0) {
-                p[nv] = (T)0;
-                v[nv] = (T)0;
+                p[i] = (T)0;
+                v[i] = (T)0;
             }
-            rr += (double)rn * (double)rn;
-            bb += (double)rhs[nv] * (double)rhs[nv];
+            vals[0] = (double)ri * (double)ri;
+            vals[1] = (double)rhs[i] * (double)rhs[i];
+            if (bordered) vals[2] = (double)dinv[i] * (double)x[i];
         }
+    }
+    block_store<3>(vals, partial, lb, nbk);
+}
+
+template <typename T>
+__global__ __launch_bounds__(FIN) void k_finish_init(int nb, int nv, int bordered,
+                                                     const T* __restrict__ rhs, T* __restrict__ r,
+                                                     T* __restrict__ r0, T* __restrict__ p,
+                                                     T* __restrict__ v,
+                                                     const double* __restrict__ partial,
+                                                     KScalars* sc, int mode, int rel_to_rhs,
+                                                     double tol_rel) {
+    double tot[3];
+    finish_sum<3>(partial, nb, tot);
+    if (threadIdx.x != 0) return;
+    double rr = tot[0], bb = tot[1];
+    if (bordered) {
+        const T rn = rhs[nv] - (T)tot[2];
+        r[nv] = rn;
+        r0[nv] = rn;
         if (mode == 0) {
-            sc->r0sq = rr;
-            sc->bb = bb;
-            sc->tol2 = tol_rel * tol_rel * (rel_to_rhs ? bb : rr);
-            sc->eps2 = (sizeof(T) == 4) ? 1.4210854715202004e-14 : 4.930380657631324e-32;
-            sc->rho = 1.0;
-            sc->alpha = 1.0;
-            sc->omega = 1.0;
-            sc->iter = 0;
-            sc->done = 0;
-            sc->red[0] = rr;
-            sc->red[1] = bb;
-            if (rel_to_rhs && bb == 0.0) {
-                sc->done = 4;  // zero right-hand side: x = 0
-                return;
-            }
-            loop_top(sc, rr, rr);
-        } else {
-            sc->r0sq = rr;
-            sc->rho = rr;
-            sc->rr = rr;
-            sc->beta = (sc->rho / sc->rho_old) * (sc->alpha / sc->omega);
-            sc->done = 0;
+            p[nv] = (T)0;
+            v[nv] = (T)0;
         }
+        rr += (double)rn * (double)rn;
+        bb += (double)rhs[nv] * (double)rhs[nv];
+    }
+    if (mode == 0) {
+        sc->r0sq = rr;
+        sc->bb = bb;
+        sc->tol2 = tol_rel * tol_rel * (rel_to_rhs ? bb : rr);
+        sc->eps2 = (sizeof(T) == 4) ? 1.4210854715202004e-14 : 4.930380657631324e-32;
+        sc->rho = 1.0;
+        sc->alpha = 1.0;
+        sc->omega = 1.0;
+        sc->iter = 0;
+        sc->done = 0;
+        sc->red[0] = rr;
+        sc->red[1] = bb;
+        if (rel_to_rhs && bb == 0.0) {
+            sc->done = 4;  // zero right-hand side: x = 0
+            return;
+        }
+        loop_top(sc, rr, rr);
+    } else {
+        sc->r0sq = rr;
+        sc->rho = rr;
+        sc->rr = rr;
+        sc->beta = (sc->rho / sc->rho_old) * (sc->alpha / sc->omega);
+        sc->done = 0;
     }
 }
 
@@ -196,36 +245,52 @@ __global__ __launch_bounds__(BLOCK) void k_update_p(int n, const T* __restrict__
     p[i] = r[i] + beta * (p[i] - w * v[i]);
 }
 
-// v = A~ p and r0.v; last block: border row, alpha = rho / r0.v
+// v = A~ p; partial sums of r0.v and of D^-1 p (border row)
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_spmv_alpha(SellView sv, int bordered,
                                                       const T* __restrict__ a,
                                                       const T* __restrict__ dinv,
                                                       const T* __restrict__ p, T* __restrict__ v,
-                                                      const T* __restrict__ r0, double* partial,
-                                                      KScalars* sc) {
+                                                      const T* __restrict__ r0,
+                                                      double* __restrict__ partial,
+                                                      const KScalars* sc) {
     if (sc->done) return;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int nbk = (sv.nv + BLOCK - 1) / BLOCK;
+    const int lb = xcd_block(nbk);
+    if (lb < 0) return;
+    const int i = lb * BLOCK + threadIdx.x;
+    const int s = wave_slice(lb);
     const int nv = sv.nv;
     double vals[2] = {0.0, 0.0};
-    if (i < nv) {
-        T y = sell_row(sv, a, p, i);
-        if (bordered) y = y + p[nv];
-        v[i] = y;
-        vals[0] = (double)r0[i] * (double)y;
-        if (bordered) vals[1] = (double)dinv[i] * (double)p[i];
-    }
-    double tot[2];
-    if (!reduce_last<2>(vals, partial, &sc->counter[0], tot)) return;
-    if (threadIdx.x == 0) {
-        double r0v = tot[0];
-        if (bordered) {
-            const T yn = (T)tot[1];
-            v[nv] = yn;
-            r0v += (double)r0[nv] * (double)yn;
+    if (s < sv.nslices) {
+        T y = sell_dot(sv.sptr, sv.col, a, p, s, threadIdx.x & 63);
+        if (i < nv) {
+            if (bordered) y = y + p[nv];
+            v[i] = y;
+            vals[0] = (double)r0[i] * (double)y;
+            if (bordered) vals[1] = (double)dinv[i] * (double)p[i];
         }
-        sc->alpha = sc->rho / r0v;
     }
+    block_store<2>(vals, partial, lb, nbk);
+}
+
+template <typename T>
+__global__ __launch_bounds__(FIN) void k_finish_alpha(int nb, int nv, int bordered,
+                                                      T* __restrict__ v, const T* __restrict__ r0,
+                                                      const double* __restrict__ partial,
+                                                      KScalars* sc) {
+    if (sc->done) return;
+    double tot[2];
+    finish_sum<2>(partial, nb, tot);
+    if (threadIdx.x != 0) return;
+    double r0v = tot[0];
+    if (bordered) {
+        const T yn = (T)tot[1];
+        v[nv] = yn;
+        r0v += (double)r0[nv] * (double)yn;
+    }
+    sc->alpha = sc->rho / r0v;
+    if (!(fabs(sc->alpha) <= 1.0e300)) sc->done = 5;
 }
 
 template <typename T>
@@ -238,47 +303,63 @@ __global__ __launch_bounds__(BLOCK) void k_update_s(int n, const T* __restrict__
     s[i] = r[i] - (T)sc->alpha * v[i];
 }
 
-// t = A~ s, t.s, t.t; last block: border row, omega
+// t = A~ s; partial sums of t.s, t.t and D^-1 s
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_spmv_omega(SellView sv, int bordered,
                                                       const T* __restrict__ a,
                                                       const T* __restrict__ dinv,
                                                       const T* __restrict__ s, T* __restrict__ t,
-                                                      double* partial, KScalars* sc) {
+                                                      double* __restrict__ partial,
+                                                      const KScalars* sc) {
     if (sc->done) return;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int nbk = (sv.nv + BLOCK - 1) / BLOCK;
+    const int lb = xcd_block(nbk);
+    if (lb < 0) return;
+    const int i = lb * BLOCK + threadIdx.x;
+    const int sl = wave_slice(lb);
     const int nv = sv.nv;
     double vals[3] = {0.0, 0.0, 0.0};
-    if (i < nv) {
-        T y = sell_row(sv, a, s, i);
-        if (bordered) y = y + s[nv];
-        t[i] = y;
-        vals[0] = (double)y * (double)s[i];
-        vals[1] = (double)y * (double)y;
-        if (bordered) vals[2] = (double)dinv[i] * (double)s[i];
-    }
-    double tot[3];
-    if (!reduce_last<3>(vals, partial, &sc->counter[0], tot)) return;
-    if (threadIdx.x == 0) {
-        double ts = tot[0], tt = tot[1];
-        if (bordered) {
-            const T yn = (T)tot[2];
-            t[nv] = yn;
-            ts += (double)yn * (double)s[nv];
-            tt += (double)yn * (double)yn;
+    if (sl < sv.nslices) {
+        T y = sell_dot(sv.sptr, sv.col, a, s, sl, threadIdx.x & 63);
+        if (i < nv) {
+            if (bordered) y = y + s[nv];
+            t[i] = y;
+            vals[0] = (double)y * (double)s[i];
+            vals[1] = (double)y * (double)y;
+            if (bordered) vals[2] = (double)dinv[i] * (double)s[i];
         }
-        sc->omega = (tt > 0.0) ? ts / tt : 0.0;
     }
+    block_store<3>(vals, partial, lb, nbk);
 }
 
-// x += alpha p + omega s; r = s - omega t; r.r and r0.r; last block: loop head
+template <typename T>
+__global__ __launch_bounds__(FIN) void k_finish_omega(int nb, int nv, int bordered,
+                                                      T* __restrict__ t, const T* __restrict__ s,
+                                                      const double* __restrict__ partial,
+                                                      KScalars* sc) {
+    if (sc->done) return;
+    double tot[3];
+    finish_sum<3>(partial, nb, tot);
+    if (threadIdx.x != 0) return;
+    double ts = tot[0], tt = tot[1];
+    if (bordered) {
+        const T yn = (T)tot[2];
+        t[nv] = yn;
+        ts += (double)yn * (double)s[nv];
+        tt += (double)yn * (double)yn;
+    }
+    sc->omega = (tt > 0.0) ? ts / tt : 0.0;
+}
+
+// x += alpha p + omega s; r = s - omega t; partial sums of r.r and r0.r
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_update_xr(int n, T* __restrict__ x, T* __restrict__ r,
                                                      const T* __restrict__ p,
                                                      const T* __restrict__ s,
                                                      const T* __restrict__ t,
-                                                     const T* __restrict__ r0, double* partial,
-                                                     KScalars* sc) {
+                                                     const T* __restrict__ r0,
+                                                     double* __restrict__ partial,
+                                                     const KScalars* sc) {
     if (sc->done) return;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     double vals[2] = {0.0, 0.0};
@@ -290,47 +371,65 @@ __global__ __launch_bounds__(BLOCK) void k_update_xr(int n, T* __restrict__ x, T
         vals[0] = (double)ri * (double)ri;
         vals[1] = (double)r0[i] * (double)ri;
     }
-    double tot[2];
-    if (!reduce_last<2>(vals, partial, &sc->counter[0], tot)) return;
-    if (threadIdx.x == 0) {
-        sc->iter += 1;
-        loop_top(sc, tot[0], tot[1]);
-    }
+    block_store<2>(vals, partial, blockIdx.x, gridDim.x);
 }
 
-// out = b - A~ xt (FP64) and |out|^2, |b|^2 into sc->red[0..1]
+__global__ __launch_bounds__(FIN) void k_finish_xr(int nb, const double* __restrict__ partial,
+                                                   KScalars* sc) {
+    if (sc->done) return;
+    double tot[2];
+    finish_sum<2>(partial, nb, tot);
+    if (threadIdx.x != 0) return;
+    sc->iter += 1;
+    loop_top(sc, tot[0], tot[1]);
+}
+
+// out = b - A~ x (FP64); partial sums of out.out, b.b, D^-1 x
 __global__ __launch_bounds__(BLOCK) void k_residual64(SellView sv, int bordered,
                                                       const double* __restrict__ a,
                                                       const double* __restrict__ dinv,
                                                       const double* __restrict__ b,
                                                       const double* __restrict__ x,
-                                                      double* __restrict__ out, double* partial,
-                                                      KScalars* sc) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+                                                      double* __restrict__ out,
+                                                      double* __restrict__ partial) {
+    const int nbk = (sv.nv + BLOCK - 1) / BLOCK;
+    const int lb = xcd_block(nbk);
+    if (lb < 0) return;
+    const int i = lb * BLOCK + threadIdx.x;
+    const int s = wave_slice(lb);
     const int nv = sv.nv;
     double vals[3] = {0.0, 0.0, 0.0};
-    if (i < nv) {
-        double ax = sell_row(sv, a, x, i);
-        if (bordered) ax = ax + x[nv];
-        const double ri = b[i] - ax;
-        out[i] = ri;
-        vals[0] = ri * ri;
-        vals[1] = b[i] * b[i];
-        if (bordered) vals[2] = dinv[i] * x[i];
-    }
-    double tot[3];
-    if (!reduce_last<3>(vals, partial, &sc->counter[0], tot)) return;
-    if (threadIdx.x == 0) {
-        double rr = tot[0], bb = tot[1];
-        if (bordered) {
-            const double rn = b[nv] - tot[2];
-            out[nv] = rn;
-            rr += rn * rn;
-            bb += b[nv] * b[nv];
+    if (s < sv.nslices) {
+        double ax = sell_dot(sv.sptr, sv.col, a, x, s, threadIdx.x & 63);
+        if (i < nv) {
+            if (bordered) ax = ax + x[nv];
+            const double ri = b[i] - ax;
+            out[i] = ri;
+            vals[0] = ri * ri;
+            vals[1] = b[i] * b[i];
+            if (bordered) vals[2] = dinv[i] * x[i];
         }
-        sc->red[0] = rr;
-        sc->red[1] = bb;
     }
+    block_store<3>(vals, partial, lb, nbk);
+}
+
+__global__ __launch_bounds__(FIN) void k_finish_res64(int nb, int nv, int bordered,
+                                                      const double* __restrict__ b,
+                                                      double* __restrict__ out,
+                                                      const double* __restrict__ partial,
+                                                      KScalars* sc) {
+    double tot[3];
+    finish_sum<3>(partial, nb, tot);
+    if (threadIdx.x != 0) return;
+    double rr = tot[0], bb = tot[1];
+    if (bordered) {
+        const double rn = b[nv] - tot[2];
+        out[nv] = rn;
+        rr += rn * rn;
+        bb += b[nv] * b[nv];
+    }
+    sc->red[0] = rr;
+    sc->red[1] = bb;
 }
 
 // y = A x with the unscaled operator: A x = A~ (D x)
@@ -529,44 +628,51 @@ int PpeSolver::run_bicgstab(hipStream_t st, const T* a, const T* di, const T* rh
                             int cap, int* restarts, int* done_out) {
     const int g_rows = nblk(nv), g_n = nblk(n);
     hipLaunchKernelGGL(k_sc_reset, dim3(1), dim3(64), 0, st, sc.p, cap);
-    hipLaunchKernelGGL(k_residual_init<T>, dim3(g_rows), dim3(BLOCK), 0, st, sv, bordered, a, di,
-                       rhs, x, rr, rr0, pp, vv, partial.p, sc.p, 0, rel_to_rhs, tol_rel);
+    hipLaunchKernelGGL(k_residual_init<T>, dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a, di,
+                       rhs, x, rr, rr0, pp, vv, partial.p, 0);
+    hipLaunchKernelGGL(k_finish_init<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered, rhs, rr,
+                       rr0, pp, vv, partial.p, sc.p, 0, rel_to_rhs, tol_rel);
     MP_HIP(hipGetLastError());
-    int chunk = std::max(1, check_every);
-    int launched = 0, polled = 0, slot = 0;
-    int result = 0;
+    const int chunk = std::max(1, check_every);
+    int slot = 0, result = 0;
     bool have_prev = false;
     *restarts = 0;
     while (true) {
         for (int c = 0; c < chunk; ++c) {
             hipLaunchKernelGGL(k_update_p<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, pp, vv, sc.p);
             spmv_timer.begin(st);
-            hipLaunchKernelGGL(k_spmv_alpha<T>, dim3(g_rows), dim3(BLOCK), 0, st, sv, bordered, a,
+            hipLaunchKernelGGL(k_spmv_alpha<T>, dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a,
                                di, pp, vv, rr0, partial.p, sc.p);
             spmv_timer.end(st);
+            hipLaunchKernelGGL(k_finish_alpha<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered,
+                               vv, rr0, partial.p, sc.p);
             hipLaunchKernelGGL(k_update_s<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, vv, ss, sc.p);
             spmv_timer.begin(st);
-            hipLaunchKernelGGL(k_spmv_omega<T>, dim3(g_rows), dim3(BLOCK), 0, st, sv, bordered, a,
+            hipLaunchKernelGGL(k_spmv_omega<T>, dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a,
                                di, ss, tt, partial.p, sc.p);
             spmv_timer.end(st);
+            hipLaunchKernelGGL(k_finish_omega<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered,
+                               tt, ss, partial.p, sc.p);
             hipLaunchKernelGGL(k_update_xr<T>, dim3(g_n), dim3(BLOCK), 0, st, n, x, rr, pp, ss,
                                tt, rr0, partial.p, sc.p);
+            hipLaunchKernelGGL(k_finish_xr, dim3(1), dim3(FIN), 0, st, g_n, partial.p, sc.p);
         }
         MP_HIP(hipGetLastError());
-        launched += chunk;
         MP_HIP(hipMemcpyAsync(&host_sc[slot], sc.p, sizeof(KScalars), hipMemcpyDeviceToHost, st));
         MP_HIP(hipEventRecord(ev[slot], st));
         if (have_prev) {
             const int ps = slot ^ 1;
             MP_HIP(hipEventSynchronize(ev[ps]));
-            polled++;
             const KScalars& h = host_sc[ps];
             if (h.done == 3) {
-                // breakdown: wait for the in-flight chunk (no-ops), then restart
+                // breakdown of r0.r: wait for the in-flight chunk (no-ops), then
+                // restart with r0 = r = rhs - A x, as Eigen's BiCGSTAB does
                 MP_HIP(hipEventSynchronize(ev[slot]));
-                hipLaunchKernelGGL(k_residual_init<T>, dim3(g_rows), dim3(BLOCK), 0, st, sv,
-                                   bordered, a, di, rhs, x, rr, rr0, pp, vv, partial.p, sc.p, 1,
-                                   rel_to_rhs, tol_rel);
+                hipLaunchKernelGGL(k_residual_init<T>, dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv,
+                                   bordered, a, di, rhs, x, rr, rr0, pp, vv, partial.p, 1);
+                hipLaunchKernelGGL(k_finish_init<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv,
+                                   bordered, rhs, rr, rr0, pp, vv, partial.p, sc.p, 1, rel_to_rhs,
+                                   tol_rel);
                 if ((*restarts)++ == 0) {
                     int zero = 0;
                     MP_HIP(hipMemcpyAsync(&sc.p->iter, &zero, sizeof(int), hipMemcpyHostToDevice,
@@ -585,18 +691,16 @@ int PpeSolver::run_bicgstab(hipStream_t st, const T* a, const T* di, const T* rh
         have_prev = true;
         slot ^= 1;
     }
-    (void)polled;
-    (void)launched;
-    // final scalars (the copy of the last chunk is complete)
-    KScalars fin;
-    std::memcpy(&fin, &host_sc[slot], sizeof(KScalars));
     *done_out = result;
-    return fin.iter;
+    return host_sc[slot].iter;
 }
 
 void PpeSolver::residual64(hipStream_t st, const double* b_dev, double* out_rr, double* out_bb) {
-    hipLaunchKernelGGL(k_residual64, dim3(nblk(nv)), dim3(BLOCK), 0, st, sv, bordered, aval.p,
-                       dinv.p, b_dev, xt.p, tmp.p, partial.p, sc.p);
+    const int g = nblk(nv);
+    hipLaunchKernelGGL(k_residual64, dim3(grid_xcd(g)), dim3(BLOCK), 0, st, sv, bordered, aval.p, dinv.p,
+                       b_dev, xt.p, tmp.p, partial.p);
+    hipLaunchKernelGGL(k_finish_res64, dim3(1), dim3(FIN), 0, st, g, nv, bordered, b_dev, tmp.p,
+                       partial.p, sc.p);
     MP_HIP(hipGetLastError());
     MP_HIP(hipMemcpyAsync(host_sc, sc.p, sizeof(KScalars), hipMemcpyDeviceToHost, st));
     MP_HIP(hipStreamSynchronize(st));
@@ -613,15 +717,16 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
     double rr0 = 0, bb = 0;
     residual64(st, b_dev, &rr0, &bb);
     out.guess_rel_res = (bb > 0) ? std::sqrt(rr0 / bb) : 0.0;
-    if (precision == 0) {
-        // BiCGSTAB stops on its recursively updated residual, which drifts from
-        // b - A x by rounding; when the true residual misses the tolerance the
-        // iteration restarts from the current x (at most 4 times).
-        int total = 0, rs_total = 0, cycles = 0;
+    // FP64 BiCGSTAB from the current x~.  BiCGSTAB stops on its recursively
+    // updated residual, which drifts from b - A x by rounding; when the true
+    // residual misses the tolerance the iteration restarts from the current x.
+    auto fp64_solve = [&](int budget, int& total, int& rs_total, int& cycles) {
+        int used = 0;
         while (true) {
             int restarts = 0, done = 0;
             int it = run_bicgstab<double>(st, aval.p, dinv.p, b_dev, xt.p, r.p, r0.p, p.p, v.p,
-                                          s.p, t.p, tol, 1, maxit - total, &restarts, &done);
+                                          s.p, t.p, tol, 1, budget - used, &restarts, &done);
+            used += it;
             total += it;
             rs_total += restarts;
             cycles++;
@@ -629,21 +734,24 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                 launch_fill(st, xt.p, 0.0, n);
                 break;
             }
-            if (done != 1 || cycles >= 5 || total >= maxit) break;
+            if (!(done == 1 || done == 5) || cycles >= 6 || used >= budget) break;
             double rr_true = 0, bb_true = 0;
             residual64(st, b_dev, &rr_true, &bb_true);
-            if (rr_true <= tol * tol * bb_true) break;
+            if (!(rr_true > tol * tol * bb_true)) break;  // converged, or non-finite
         }
-        out.iterations = total;
-        out.restarts = rs_total;
-        out.outer_cycles = cycles;
+    };
+    int total = 0, cycles = 0, rs_total = 0;
+    if (precision == 0) {
+        fp64_solve(maxit, total, rs_total, cycles);
     } else {
-        // FP64 defect correction around an FP32 inner BiCGSTAB
-        double rr = rr0;
-        int total = 0, cycles = 0, rs_total = 0;
+        // FP64 defect correction around an FP32 inner BiCGSTAB: each cycle
+        // solves A~ e = r / |r| in FP32 to inner_tol (at most inner_maxit
+        // iterations), then x~ += |r| e and r = b - A~ x~ in FP64.  A cycle that
+        // breaks down or stalls hands the rest of the solve to FP64 BiCGSTAB.
+        double rr = rr0, rr_prev = rr0;
         const double tol2 = tol * tol * bb;
+        bool fallback = false;
         while (rr > tol2 && total < maxit && bb > 0) {
-            // inner: solve A~ e = r / |r| in FP32
             hipLaunchKernelGGL(k_to32, dim3(g_n), dim3(BLOCK), 0, st, n, tmp.p, rhs32.p, sc.p);
             hipLaunchKernelGGL(k_fill32, dim3(g_n), dim3(BLOCK), 0, st, e32.p, 0.0f, n);
             MP_HIP(hipGetLastError());
@@ -653,21 +761,33 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
             int restarts = 0, done = 0;
             int it = run_bicgstab<float>(st, aval32.p, dinv32.p, rhs32.p, e32.p, r32.p, r032.p,
                                          p32.p, v32.p, s32.p, t32.p, inner_tol, 0,
-                                         maxit - total, &restarts, &done);
+                                         std::min(inner_maxit, maxit - total), &restarts, &done);
             total += it;
             rs_total += restarts;
+            cycles++;
+            if (done == 5 || it == 0) {
+                fallback = true;
+                break;
+            }
             hipLaunchKernelGGL(k_correct, dim3(g_n), dim3(BLOCK), 0, st, n, xt.p, e32.p,
                                partial.p + (partial.n - 1));
             MP_HIP(hipGetLastError());
             residual64(st, b_dev, &rr, &bb);
-            cycles++;
-            if (it == 0) break;  // no progress possible
+            if (!(rr < 0.81 * rr_prev)) {  // less than a 10% reduction of |r|
+                fallback = true;
+                break;
+            }
+            rr_prev = rr;
+        }
+        if (fallback && total < maxit) {
+            fp64_fallbacks++;
+            fp64_solve(maxit - total, total, rs_total, cycles);
         }
         if (bb == 0) launch_fill(st, xt.p, 0.0, n);
-        out.iterations = total;
-        out.restarts = rs_total;
-        out.outer_cycles = cycles;
     }
+    out.iterations = total;
+    out.restarts = rs_total;
+    out.outer_cycles = cycles;
     // x = D^-1 x~ and the logged residual |A x - b| (solver.cpp:342-345)
     double rr = 0;
     residual64(st, b_dev, &rr, &bb);
@@ -684,18 +804,6 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
     total_iterations += out.iterations;
     total_solves += 1;
     return out;
-}
-
-void PpeSolver::apply(hipStream_t st, const double* x_dev, double* y_dev) {
-    // y = A x = A~ (D x): scale into xt, then residual with b = 0 gives -A x
-    hipLaunchKernelGGL(k_scale, dim3(nblk(n)), dim3(BLOCK), 0, st, n, dinv.p, x_dev, xt.p, 1);
-    launch_fill(st, tmp.p, 0.0, (int)tmp.n);
-    hipLaunchKernelGGL(k_residual64, dim3(nblk(nv)), dim3(BLOCK), 0, st, sv, bordered, aval.p,
-                       dinv.p, tmp.p, xt.p, y_dev, partial.p, sc.p);
-    MP_HIP(hipGetLastError());
-    // y currently holds -A x
-    hipLaunchKernelGGL(k_scale, dim3(nblk(n)), dim3(BLOCK), 0, st, n, dinv.p, y_dev, y_dev, 2);
-    MP_HIP(hipGetLastError());
 }
 
 }  // namespace mp

@@ -37,7 +37,8 @@ struct PpeSolver {
     hipEvent_t ev[2] = {nullptr, nullptr};
     int precision = 0;
     double tol = 1e-8, inner_tol = 1e-3;
-    int maxit = 1000, check_every = 8;
+    int maxit = 1000, check_every = 8, inner_maxit = 1000;
+    long fp64_fallbacks = 0;
     KernelTimer spmv_timer;
     long total_iterations = 0, total_solves = 0;
 
@@ -49,8 +50,6 @@ struct PpeSolver {
                double conv, double diff);
     // b_dev, x_dev: device order, length n (guess in x_dev, solution out).
     SolveStats solve(hipStream_t s, const double* b_dev, double* x_dev);
-    // y = A x (unscaled operator incl. border), device vectors of length n
-    void apply(hipStream_t s, const double* x_dev, double* y_dev);
     double spmv_bytes(int fp32) const;
 
   private:

@@ -48,6 +48,19 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
 };
 
+// Number of XCDs of the MI355X: the dispatcher hands workgroup b to XCD b % 8.
+constexpr int NXCD = 8;
+
+// XCD-aware block numbering: launch grid_xcd(G) blocks; block b works on
+// logical block (b % 8) * ceil(G/8) + b / 8, so each XCD (and its L2) owns a
+// contiguous range of rows.  Returns -1 for the padding blocks.
+__host__ __device__ inline int grid_xcd(int g) { return ((g + NXCD - 1) / NXCD) * NXCD; }
+__device__ __forceinline__ int xcd_block(int g) {
+    const int per = (g + NXCD - 1) / NXCD;
+    const int lb = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+    return lb < g ? lb : -1;
+}
+
 // SELL-64 view of the stencil set in device order.
 struct SellView {
     int nv;            // real rows
