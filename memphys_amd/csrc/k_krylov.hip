@@ -114,14 +114,14 @@ __device__ void loop_top(KScalars* sc, double rr, double r0r) {
 // the column/value loads issued ahead of the gathers.  Padding slots hold a
 // zero value and a valid column, which leaves the fma chain unchanged.
 // Matrix streams use non-temporal loads so the gathered vector stays cached.
-template <typename T>
+template <typename TM, typename T>
 __device__ __forceinline__ T sell_dot(const int* __restrict__ sptr, const int* __restrict__ col,
-                                      const T* __restrict__ a, const T* __restrict__ x, int s,
+                                      const TM* __restrict__ a, const T* __restrict__ x, int s,
                                       int lane) {
     const int b0 = sptr[s];
     const int w = (sptr[s + 1] - b0) >> 6;
     const int* cp = col + b0 + lane;
-    const T* ap = a + b0 + lane;
+    const TM* ap = a + b0 + lane;
     T acc = (T)0;
     int j = 0;
     for (; j + 8 <= w; j += 8) {
@@ -130,7 +130,7 @@ __device__ __forceinline__ T sell_dot(const int* __restrict__ sptr, const int* _
 #pragma unroll
         for (int u = 0; u < 8; ++u) c[u] = __builtin_nontemporal_load(cp + u * SLICE);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) av[u] = __builtin_nontemporal_load(ap + u * SLICE);
+        for (int u = 0; u < 8; ++u) av[u] = (T)__builtin_nontemporal_load(ap + u * SLICE);
 #pragma unroll
         for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
 #pragma unroll
@@ -140,7 +140,7 @@ __device__ __forceinline__ T sell_dot(const int* __restrict__ sptr, const int* _
     }
     for (; j < w; ++j) {
         const int c0 = __builtin_nontemporal_load(cp);
-        const T a0 = __builtin_nontemporal_load(ap);
+        const T a0 = (T)__builtin_nontemporal_load(ap);
         acc = fma(a0, x[c0], acc);
         cp += SLICE;
         ap += SLICE;
@@ -154,9 +154,9 @@ __device__ __forceinline__ int wave_slice(int lb) {
 }
 
 // r = rhs - A~ x; r0 = r; p = v = 0 on a fresh start (mode 0).
-template <typename T>
+template <typename TM, typename T>
 __global__ __launch_bounds__(BLOCK) void k_residual_init(
-    SellView sv, int bordered, const T* __restrict__ a, const T* __restrict__ dinv,
+    SellView sv, int bordered, const TM* __restrict__ a, const T* __restrict__ dinv,
     const T* __restrict__ rhs, const T* __restrict__ x, T* __restrict__ r, T* __restrict__ r0,
     T* __restrict__ p, T* __restrict__ v, double* __restrict__ partial, int mode) {
     const int nbk = (sv.nv + BLOCK - 1) / BLOCK;
@@ -246,9 +246,9 @@ __global__ __launch_bounds__(BLOCK) void k_update_p(int n, const T* __restrict__
 }
 
 // v = A~ p; partial sums of r0.v and of D^-1 p (border row)
-template <typename T>
+template <typename TM, typename T>
 __global__ __launch_bounds__(BLOCK) void k_spmv_alpha(SellView sv, int bordered,
-                                                      const T* __restrict__ a,
+                                                      const TM* __restrict__ a,
                                                       const T* __restrict__ dinv,
                                                       const T* __restrict__ p, T* __restrict__ v,
                                                       const T* __restrict__ r0,
@@ -304,9 +304,9 @@ __global__ __launch_bounds__(BLOCK) void k_update_s(int n, const T* __restrict__
 }
 
 // t = A~ s; partial sums of t.s, t.t and D^-1 s
-template <typename T>
+template <typename TM, typename T>
 __global__ __launch_bounds__(BLOCK) void k_spmv_omega(SellView sv, int bordered,
-                                                      const T* __restrict__ a,
+                                                      const TM* __restrict__ a,
                                                       const T* __restrict__ dinv,
                                                       const T* __restrict__ s, T* __restrict__ t,
                                                       double* __restrict__ partial,
@@ -467,6 +467,13 @@ __global__ __launch_bounds__(BLOCK) void k_fill32(float* __restrict__ x, float v
     if (i < n) x[i] = v;
 }
 
+// y += a x
+__global__ __launch_bounds__(BLOCK) void k_axpy(int n, double a, const double* __restrict__ x,
+                                                double* __restrict__ y) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) y[i] = y[i] + a * x[i];
+}
+
 // r32 = r64 / |r64| (scaled copy for the FP32 inner solve)
 __global__ __launch_bounds__(BLOCK) void k_to32(int n, const double* __restrict__ src,
                                                 float* __restrict__ dst, const KScalars* sc) {
@@ -593,7 +600,7 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, int dim_, int npad_,
     hipLaunchKernelGGL(k_scale_columns<float>, dim3(nblk(nv)), dim3(BLOCK), 0, st, sv, raw.p,
                        dinv.p, aval32.p);
     MP_HIP(hipGetLastError());
-    for (DevBuf<double>* b : {&xt, &r, &r0, &p, &v, &s, &t, &tmp}) {
+    for (DevBuf<double>* b : {&xt, &r, &r0, &p, &v, &s, &t, &tmp, &e64, &rhs64}) {
         b->alloc(vlen);
         MP_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * vlen, st));
     }
@@ -622,13 +629,13 @@ double PpeSolver::spmv_bytes(int fp32) const {
     return (double)nnz * (vb + 4.0) + (double)nv * (vb * (bordered ? 4.0 : 3.0) + 4.0);
 }
 
-template <typename T>
-int PpeSolver::run_bicgstab(hipStream_t st, const T* a, const T* di, const T* rhs, T* x, T* rr,
+template <typename TM, typename T>
+int PpeSolver::run_bicgstab(hipStream_t st, const TM* a, const T* di, const T* rhs, T* x, T* rr,
                             T* rr0, T* pp, T* vv, T* ss, T* tt, double tol_rel, int rel_to_rhs,
                             int cap, int* restarts, int* done_out) {
     const int g_rows = nblk(nv), g_n = nblk(n);
     hipLaunchKernelGGL(k_sc_reset, dim3(1), dim3(64), 0, st, sc.p, cap);
-    hipLaunchKernelGGL(k_residual_init<T>, dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a, di,
+    hipLaunchKernelGGL((k_residual_init<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a, di,
                        rhs, x, rr, rr0, pp, vv, partial.p, 0);
     hipLaunchKernelGGL(k_finish_init<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered, rhs, rr,
                        rr0, pp, vv, partial.p, sc.p, 0, rel_to_rhs, tol_rel);
@@ -641,14 +648,14 @@ int PpeSolver::run_bicgstab(hipStream_t st, const T* a, const T* di, const T* rh
         for (int c = 0; c < chunk; ++c) {
             hipLaunchKernelGGL(k_update_p<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, pp, vv, sc.p);
             spmv_timer.begin(st);
-            hipLaunchKernelGGL(k_spmv_alpha<T>, dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a,
+            hipLaunchKernelGGL((k_spmv_alpha<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a,
                                di, pp, vv, rr0, partial.p, sc.p);
             spmv_timer.end(st);
             hipLaunchKernelGGL(k_finish_alpha<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered,
                                vv, rr0, partial.p, sc.p);
             hipLaunchKernelGGL(k_update_s<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, vv, ss, sc.p);
             spmv_timer.begin(st);
-            hipLaunchKernelGGL(k_spmv_omega<T>, dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a,
+            hipLaunchKernelGGL((k_spmv_omega<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a,
                                di, ss, tt, partial.p, sc.p);
             spmv_timer.end(st);
             hipLaunchKernelGGL(k_finish_omega<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered,
@@ -668,7 +675,7 @@ int PpeSolver::run_bicgstab(hipStream_t st, const T* a, const T* di, const T* rh
                 // breakdown of r0.r: wait for the in-flight chunk (no-ops), then
                 // restart with r0 = r = rhs - A x, as Eigen's BiCGSTAB does
                 MP_HIP(hipEventSynchronize(ev[slot]));
-                hipLaunchKernelGGL(k_residual_init<T>, dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv,
+                hipLaunchKernelGGL((k_residual_init<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv,
                                    bordered, a, di, rhs, x, rr, rr0, pp, vv, partial.p, 1);
                 hipLaunchKernelGGL(k_finish_init<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv,
                                    bordered, rhs, rr, rr0, pp, vv, partial.p, sc.p, 1, rel_to_rhs,
@@ -724,7 +731,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
         int used = 0;
         while (true) {
             int restarts = 0, done = 0;
-            int it = run_bicgstab<double>(st, aval.p, dinv.p, b_dev, xt.p, r.p, r0.p, p.p, v.p,
+            int it = run_bicgstab<double, double>(st, aval.p, dinv.p, b_dev, xt.p, r.p, r0.p, p.p, v.p,
                                           s.p, t.p, tol, 1, budget - used, &restarts, &done);
             used += it;
             total += it;
@@ -744,24 +751,23 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
     if (precision == 0) {
         fp64_solve(maxit, total, rs_total, cycles);
     } else {
-        // FP64 defect correction around an FP32 inner BiCGSTAB: each cycle
-        // solves A~ e = r / |r| in FP32 to inner_tol (at most inner_maxit
-        // iterations), then x~ += |r| e and r = b - A~ x~ in FP64.  A cycle that
-        // breaks down or stalls hands the rest of the solve to FP64 BiCGSTAB.
+        // FP64 defect correction around an inner BiCGSTAB that streams the FP32
+        // copy of the operator (8 instead of 12 bytes per entry) while keeping
+        // FP64 vectors and reductions: each cycle solves A32 e = r with
+        // r = b - A x~ from the FP64 operator, to a relative tolerance
+        // max(inner_tol, tol |b| / 2|r|), then x~ += e.  A cycle that breaks
+        // down or stalls hands the rest of the solve to FP64 BiCGSTAB.
         double rr = rr0, rr_prev = rr0;
         const double tol2 = tol * tol * bb;
         bool fallback = false;
         while (rr > tol2 && total < maxit && bb > 0) {
-            hipLaunchKernelGGL(k_to32, dim3(g_n), dim3(BLOCK), 0, st, n, tmp.p, rhs32.p, sc.p);
-            hipLaunchKernelGGL(k_fill32, dim3(g_n), dim3(BLOCK), 0, st, e32.p, 0.0f, n);
-            MP_HIP(hipGetLastError());
-            // keep |r|^2 of this cycle for the correction scale (sc->red[0] is overwritten)
-            MP_HIP(hipMemcpyAsync(partial.p + (partial.n - 1), &sc.p->red[0], sizeof(double),
-                                  hipMemcpyDeviceToDevice, st));
+            MP_HIP(hipMemcpyAsync(rhs64.p, tmp.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+            launch_fill(st, e64.p, 0.0, n);
+            const double tau = std::max(inner_tol, 0.5 * std::sqrt(tol2 / rr));
             int restarts = 0, done = 0;
-            int it = run_bicgstab<float>(st, aval32.p, dinv32.p, rhs32.p, e32.p, r32.p, r032.p,
-                                         p32.p, v32.p, s32.p, t32.p, inner_tol, 0,
-                                         std::min(inner_maxit, maxit - total), &restarts, &done);
+            int it = run_bicgstab<float, double>(st, aval32.p, dinv.p, rhs64.p, e64.p, r.p, r0.p,
+                                                 p.p, v.p, s.p, t.p, tau, 1, maxit - total,
+                                                 &restarts, &done);
             total += it;
             rs_total += restarts;
             cycles++;
@@ -769,8 +775,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                 fallback = true;
                 break;
             }
-            hipLaunchKernelGGL(k_correct, dim3(g_n), dim3(BLOCK), 0, st, n, xt.p, e32.p,
-                               partial.p + (partial.n - 1));
+            hipLaunchKernelGGL(k_axpy, dim3(g_n), dim3(BLOCK), 0, st, n, 1.0, e64.p, xt.p);
             MP_HIP(hipGetLastError());
             residual64(st, b_dev, &rr, &bb);
             if (!(rr < 0.81 * rr_prev)) {  // less than a 10% reduction of |r|

@@ -40,7 +40,8 @@ __global__ __launch_bounds__(256) void weights_fp64_kernel(
     double* __restrict__ gz, double* __restrict__ lap, double* __restrict__ cond) {
     extern __shared__ double lds[];
     __shared__ double s_mn[3], s_sc[3];
-    __shared__ int s_piv;
+    __shared__ double s_anorm, s_red[4];
+    __shared__ int s_piv, s_jm;
     __shared__ int s_e[MAX_POLY * 3];
 
     const int r = blockIdx.x;
@@ -180,6 +181,21 @@ __global__ __launch_bounds__(256) void weights_fp64_kernel(
     }
     __syncthreads();
 
+    // ---- 1-norm of A (for the condition estimate) ----
+    int* perm = (int*)xs;  // the scaled coordinates are no longer needed
+    if (cond) {
+        if (tid == 0) s_anorm = 0.0;
+        __syncthreads();
+        for (int j = tid; j < M; j += blockDim.x) {
+            double sum = 0.0;
+            for (int i = 0; i < M; ++i) sum += fabs(A[i * LDA + j]);
+            atomicMax((unsigned long long*)&s_anorm, (unsigned long long)__double_as_longlong(sum));
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < M; i += blockDim.x) perm[i] = i;
+    __syncthreads();
+
     // ---- LU with partial pivoting, right-looking ----
     for (int c = 0; c < M; ++c) {
         if (tid < 64) {
@@ -205,6 +221,11 @@ __global__ __launch_bounds__(256) void weights_fp64_kernel(
         __syncthreads();
         const int piv = s_piv;
         if (piv != c) {
+            if (tid == 0) {
+                const int tp = perm[c];
+                perm[c] = perm[piv];
+                perm[piv] = tp;
+            }
             for (int j = tid; j < M + nr; j += blockDim.x) {
                 if (j < M) {
                     double t0 = A[c * LDA + j];
@@ -258,7 +279,93 @@ __global__ __launch_bounds__(256) void weights_fp64_kernel(
         if (dim == 3) gz[slot] = R[t * 4 + 2];
         lap[slot] = R[t * 4 + dim];
     }
-    if (cond && tid == 0) cond[r] = __builtin_nan("");
+    if (!cond) return;
+    __syncthreads();
+    // ---- Hager (1984) / Higham (1988) estimate of ||A^-1||_1 from the LU factors ----
+    double* w = R;          // work vectors reuse the right-hand-side storage
+    double* xv = R + M;
+    double* yv = R + 2 * M;
+    bool singular = false;
+    for (int c = 0; c < M; ++c) singular |= (A[c * LDA + c] == 0.0);
+    if (singular) {
+        if (tid == 0) cond[r] = __builtin_inf();
+        return;
+    }
+    // w = A^-1 v  (v in xv): w = P v, forward with unit L, backward with U
+    auto solve = [&](const double* v, double* out) {
+        for (int i = tid; i < M; i += blockDim.x) out[i] = v[perm[i]];
+        __syncthreads();
+        for (int c = 0; c < M; ++c) {
+            for (int i = c + 1 + tid; i < M; i += blockDim.x) out[i] = fma(-A[i * LDA + c], out[c], out[i]);
+            __syncthreads();
+        }
+        for (int c = M - 1; c >= 0; --c) {
+            if (tid == 0) out[c] = out[c] / A[c * LDA + c];
+            __syncthreads();
+            for (int i = tid; i < c; i += blockDim.x) out[i] = fma(-A[i * LDA + c], out[c], out[i]);
+            __syncthreads();
+        }
+    };
+    // out = A^-T v: U^T then L^T (column oriented), then un-permute
+    auto solve_t = [&](const double* v, double* tmpv, double* out) {
+        for (int i = tid; i < M; i += blockDim.x) tmpv[i] = v[i];
+        __syncthreads();
+        for (int c = 0; c < M; ++c) {
+            if (tid == 0) tmpv[c] = tmpv[c] / A[c * LDA + c];
+            __syncthreads();
+            for (int i = c + 1 + tid; i < M; i += blockDim.x) tmpv[i] = fma(-A[c * LDA + i], tmpv[c], tmpv[i]);
+            __syncthreads();
+        }
+        for (int c = M - 1; c >= 0; --c) {
+            for (int i = tid; i < c; i += blockDim.x) tmpv[i] = fma(-A[c * LDA + i], tmpv[c], tmpv[i]);
+            __syncthreads();
+        }
+        for (int i = tid; i < M; i += blockDim.x) out[perm[i]] = tmpv[i];
+        __syncthreads();
+    };
+    auto l1 = [&](const double* v) {  // sequential sum on thread 0, broadcast
+        if (tid == 0) {
+            double sum = 0.0;
+            for (int i = 0; i < M; ++i) sum += fabs(v[i]);
+            s_red[0] = sum;
+        }
+        __syncthreads();
+        double res = s_red[0];
+        __syncthreads();
+        return res;
+    };
+    for (int i = tid; i < M; i += blockDim.x) xv[i] = 1.0 / (double)M;
+    __syncthreads();
+    double est = 0.0;
+    int jlast = -1;
+    for (int itn = 0; itn < 5; ++itn) {
+        solve(xv, yv);
+        est = fmax(est, l1(yv));
+        for (int i = tid; i < M; i += blockDim.x) w[i] = (yv[i] >= 0.0) ? 1.0 : -1.0;
+        __syncthreads();
+        solve_t(w, yv, w);
+        if (tid == 0) {
+            int jm = 0;
+            double zx = 0.0;
+            for (int i = 0; i < M; ++i) {
+                if (fabs(w[i]) > fabs(w[jm])) jm = i;
+                zx += w[i] * xv[i];
+            }
+            s_jm = (fabs(w[jm]) <= zx || jm == jlast) ? -1 : jm;
+        }
+        __syncthreads();
+        const int jm = s_jm;
+        if (jm < 0) break;
+        jlast = jm;
+        for (int i = tid; i < M; i += blockDim.x) xv[i] = (i == jm) ? 1.0 : 0.0;
+        __syncthreads();
+    }
+    for (int i = tid; i < M; i += blockDim.x)
+        xv[i] = ((i % 2) ? -1.0 : 1.0) * (1.0 + (double)i / (double)(M > 1 ? M - 1 : 1));
+    __syncthreads();
+    solve(xv, yv);
+    est = fmax(est, 2.0 * l1(yv) / (3.0 * (double)M));
+    if (tid == 0) cond[r] = s_anorm * est;
 }
 
 }  // namespace

@@ -29,7 +29,7 @@ struct PpeSolver {
     SellView sv{};
     DevBuf<double> aval, dinv;
     DevBuf<float> aval32, dinv32;
-    DevBuf<double> xt, r, r0, p, v, s, t, tmp;
+    DevBuf<double> xt, r, r0, p, v, s, t, tmp, e64, rhs64;
     DevBuf<float> e32, rhs32, r32, r032, p32, v32, s32, t32;
     DevBuf<double> partial;
     DevBuf<KScalars> sc;
@@ -53,8 +53,8 @@ struct PpeSolver {
     double spmv_bytes(int fp32) const;
 
   private:
-    template <typename T>
-    int run_bicgstab(hipStream_t s, const T* aval_t, const T* dinv_t, const T* rhs, T* x, T* rr,
+    template <typename TM, typename T>
+    int run_bicgstab(hipStream_t s, const TM* aval_t, const T* dinv_t, const T* rhs, T* x, T* rr,
                      T* rr0, T* pp, T* vv, T* ss, T* tt, double tol_rel, int rel_to_rhs, int maxit,
                      int* restarts, int* done_out);
     void residual64(hipStream_t s, const double* b_dev, double* out_rr, double* out_bb);

@@ -298,18 +298,14 @@ double stencil_weights(int dim, int phs, int m, const int* exps, int k, const do
                 for (int i = 0; i < c; ++i) w[i] = std::fma(-A[i * M + c], w[c], w[i]);
             }
             v = w;
-        } else {  // A^T = U^T L^T P
+        } else {  // A^T = U^T L^T P, both triangles column oriented
             std::vector<double> w = v;
-            for (int i = 0; i < M; ++i) {
-                double s = w[i];
-                for (int t = 0; t < i; ++t) s = std::fma(-A[t * M + i], w[t], s);
-                w[i] = s / A[i * M + i];
+            for (int c = 0; c < M; ++c) {
+                w[c] = w[c] / A[c * M + c];
+                for (int i = c + 1; i < M; ++i) w[i] = std::fma(-A[c * M + i], w[c], w[i]);
             }
-            for (int i = M - 1; i >= 0; --i) {
-                double s = w[i];
-                for (int t = i + 1; t < M; ++t) s = std::fma(-A[t * M + i], w[t], s);
-                w[i] = s;
-            }
+            for (int c = M - 1; c >= 0; --c)
+                for (int i = 0; i < c; ++i) w[i] = std::fma(-A[c * M + i], w[c], w[i]);
             std::vector<double> o(M);
             for (int i = 0; i < M; ++i) o[perm[i]] = w[i];
             v = o;

@@ -166,3 +166,52 @@ def test_fs_3d_cube_matches_oracle(ctx, cube3d_small):
 def test_fs_3d_cube_mixed_matches_oracle(ctx, cube3d_small):
     import memphys_amd as M
     _fs_parity(ctx, cube3d_small, 8, 2, 1.0, 0.0025, 5e-4, M.MP_PREC_MIXED)
+
+
+def test_condition_estimate_matches_oracle(ctx, cube3d_small):
+    """cond = ||A||_1 * Hager/Higham estimate of ||A^-1||_1 (the estimator behind
+    Eigen's rcond, coefficient_computations.cpp:312), same arithmetic order."""
+    import oracle as O
+    c = cube3d_small
+    ctx.upload(c)
+    _, cond = ctx.weights(3, 3, cond=True)
+    ocond = O.stencil_weights(c, 3, 3, want_cond=True)[4]
+    _print("cond max/min/avg", cond.max(), cond.min(), cond.mean(),
+           "bit-exact fraction", np.mean(cond == ocond))
+    assert rel_err(cond, ocond) <= 1e-12
+
+
+def test_class_api_driver_matches_oracle(tmp_path):
+    """examples/lid_driven_cavity (PARAMETERS -> POINTS -> CLOUD ->
+    FRACTIONAL_STEP_1 -> time loop) against the oracle on the same cloud."""
+    import os
+    import subprocess
+
+    import memphys_amd as M
+    import oracle as O
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "examples", "lid_driven_cavity")
+    out = str(tmp_path / "fields.bin")
+    steps, dt, Re, n = 10, 1e-3, 100.0, 30
+    r = subprocess.run([exe, os.path.join(root, "examples", "parameters_cavity.csv"), "2", str(n),
+                        str(steps), str(dt), str(Re), out], capture_output=True, text=True,
+                       timeout=120)
+    _print(r.stdout.strip())
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = open(out, "rb").read()
+    nv, dim, nsys = np.frombuffer(raw[:12], dtype=np.int32)
+    data = np.frombuffer(raw[12:], dtype=np.float64)
+    xyz = data[:nv * dim].reshape(nv, dim)
+    u, v, p = data[nv * dim:nv * dim + nv], data[nv * dim + nv:nv * dim + 2 * nv], data[nv * dim + 2 * nv:]
+    cloud = M.box_cloud(dim=2, n=n, jitter=0.25, seed=20240601, k_interior=30)
+    assert np.array_equal(cloud.xyz, xyz)  # same generator, kNN and RCM order
+    w = O.stencil_weights(cloud, 3, 3)[:4]
+    u_dir, p_dir, u0 = lid_bc(cloud, 1)
+    fs = O.FractionalStep(cloud, w, u_dir, u_dir, None, p_dir, 1.0, 1.0 / Re, dt, 2, precond=2,
+                          tol=1e-14, maxit=20000)
+    uo, vo, po = u0.copy(), np.zeros(nv), np.zeros(fs.n_sys)
+    for it in range(steps):
+        fs.step(it, uo, vo, None, po)
+    errs = (rel_err(u, uo), rel_err(v, vo), rel_err(p, po))
+    _print("class API driver errs", errs)
+    assert max(errs) <= 1e-8
