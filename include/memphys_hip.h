@@ -37,8 +37,11 @@ enum mp_op { MP_GX = 0, MP_GY = 1, MP_GZ = 2, MP_LAP = 3 };
 enum mp_precision { MP_PREC_FP64 = 0, MP_PREC_MIXED = 1 };
 
 /* Initial guess for each pressure solve: the previous pressure (reference
- * behaviour, solveWithGuess(rhs, p_old)) or the learned message-passing guess. */
-enum mp_guess { MP_GUESS_PREVIOUS = 0, MP_GUESS_LEARNED = 1 };
+ * behaviour, solveWithGuess(rhs, p_old)), the learned message-passing guess
+ * (linear extrapolation 2 p(t-1) - p(t-2) plus the network's correction), or
+ * the linear extrapolation alone.  The last two fall back to the previous
+ * pressure until two pressures are in the history. */
+enum mp_guess { MP_GUESS_PREVIOUS = 0, MP_GUESS_LEARNED = 1, MP_GUESS_EXTRAPOLATE = 2 };
 
 typedef struct {
     double rho, mu, dt;
@@ -118,6 +121,24 @@ int mp_fields_get(mp_ctx* ctx, int which, double* values);
 int mp_fs_step(mp_ctx* ctx, int n_steps, int it0);
 int mp_stats_get(mp_ctx* ctx, mp_step_stats* stats);
 int mp_guess_mode(mp_ctx* ctx, int mode);
+
+/* Learned guess: a 3-layer message-passing network over the stencil graph
+ * (width 64, BF16 weights, FP32 accumulation).  The parameters are a packed
+ * FP32 vector of mp_guess_param_count() entries.
+ * mp_guess_train: while enabled, every pressure solve of mp_fs_step trains the
+ * network on (features before the solve -> converged pressure) with one Adam
+ * step; seed != 0 re-initialises the parameters first.  last_loss receives the
+ * mean squared error of the last training step. */
+int mp_guess_param_count(void);
+int mp_guess_train(mp_ctx* ctx, int enable, double learning_rate, unsigned long long seed);
+int mp_guess_params(mp_ctx* ctx, int set, float* params);
+int mp_guess_last_loss(mp_ctx* ctx, double* loss);
+/* One evaluation on caller-order inputs (needs mp_fs_create): guess (nv
+ * entries) from p1 = p(t-1), p2 = p(t-2) and the right-hand side b (system
+ * size); with p_true != NULL also the loss and its gradient (packed like the
+ * parameters).  No parameter update. */
+int mp_guess_eval(mp_ctx* ctx, const double* p1, const double* p2, const double* b,
+                  const double* p_true, double* guess, float* grad, double* loss);
 
 /* Continuity residual Gx u + Gy v (+ Gz w) of the current fields
  * (postprocessing_functions.cpp:192-195, 248-251):

@@ -23,7 +23,7 @@ HOST_LIB_PATH = os.path.join(_HERE, "libmemphys_host.so")
 MP_U, MP_V, MP_W, MP_P = 0, 1, 2, 3
 MP_GX, MP_GY, MP_GZ, MP_LAP = 0, 1, 2, 3
 MP_PREC_FP64, MP_PREC_MIXED = 0, 1
-MP_GUESS_PREVIOUS, MP_GUESS_LEARNED = 0, 1
+MP_GUESS_PREVIOUS, MP_GUESS_LEARNED, MP_GUESS_EXTRAPOLATE = 0, 1, 2
 
 # Every symbol include/memphys_hip.h declares.
 HIP_SYMBOLS = (
@@ -31,7 +31,8 @@ HIP_SYMBOLS = (
     "mp_weights_assemble", "mp_weights_set", "mp_operator_apply", "mp_solver_init",
     "mp_solver_system_size", "mp_solver_solve", "mp_fs_create", "mp_fields_set",
     "mp_fields_get", "mp_fs_step", "mp_stats_get", "mp_guess_mode", "mp_divergence",
-    "mp_kernel_timing", "mp_kernel_timing_get",
+    "mp_kernel_timing", "mp_kernel_timing_get", "mp_guess_param_count", "mp_guess_train",
+    "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval",
 )
 
 
@@ -96,6 +97,12 @@ def hip_lib() -> C.CDLL:
         lib.mp_kernel_timing.argtypes = [vp, C.c_int]
         lib.mp_kernel_timing_get.argtypes = [vp, C.c_int, C.POINTER(C.c_long),
                                              C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.mp_guess_param_count.argtypes = []
+        lib.mp_guess_train.argtypes = [vp, C.c_int, C.c_double, C.c_ulonglong]
+        lib.mp_guess_params.argtypes = [vp, C.c_int, np.ctypeslib.ndpointer(dtype=np.float32,
+                                                                            flags="C_CONTIGUOUS")]
+        lib.mp_guess_last_loss.argtypes = [vp, C.POINTER(C.c_double)]
+        lib.mp_guess_eval.argtypes = [vp, _dp, _dp, _dp, vp, _dp, vp, C.POINTER(C.c_double)]
         for name in HIP_SYMBOLS:
             getattr(lib, name).restype = C.c_char_p if name == "mp_last_error" else C.c_int
         _hip = lib
@@ -307,6 +314,40 @@ class Context:
     def guess_mode(self, mode: int):
         self._check(self.lib.mp_guess_mode(self.h, mode), "mp_guess_mode")
 
+    def guess_train(self, enable: bool, lr: float = 1e-3, seed: int = 0):
+        self._check(self.lib.mp_guess_train(self.h, 1 if enable else 0, lr, seed), "mp_guess_train")
+
+    def guess_params(self, values=None) -> np.ndarray:
+        n = self.lib.mp_guess_param_count()
+        if values is not None:
+            v = np.ascontiguousarray(values, dtype=np.float32)
+            assert v.size == n
+            self._check(self.lib.mp_guess_params(self.h, 1, v), "mp_guess_params")
+            return v
+        out = np.zeros(n, dtype=np.float32)
+        self._check(self.lib.mp_guess_params(self.h, 0, out), "mp_guess_params")
+        return out
+
+    def guess_last_loss(self) -> float:
+        v = C.c_double()
+        self._check(self.lib.mp_guess_last_loss(self.h, C.byref(v)), "mp_guess_last_loss")
+        return v.value
+
+    def guess_eval(self, p1, p2, b, p_true=None):
+        nv = self.cloud.nv
+        g = np.zeros(nv)
+        grad = np.zeros(self.lib.mp_guess_param_count(), dtype=np.float32) if p_true is not None \
+            else None
+        loss = C.c_double()
+        pt = np.ascontiguousarray(p_true, dtype=np.float64) if p_true is not None else None
+        self._check(self.lib.mp_guess_eval(
+            self.h, np.ascontiguousarray(p1, dtype=np.float64),
+            np.ascontiguousarray(p2, dtype=np.float64), np.ascontiguousarray(b, dtype=np.float64),
+            pt.ctypes.data_as(C.c_void_p) if pt is not None else None, g,
+            grad.ctypes.data_as(C.c_void_p) if grad is not None else None, C.byref(loss)),
+            "mp_guess_eval")
+        return g, grad, loss.value
+
     def sync(self):
         self._check(self.lib.mp_sync(self.h), "mp_sync")
 
@@ -320,3 +361,49 @@ class Context:
         self._check(self.lib.mp_kernel_timing_get(self.h, which, C.byref(n), C.byref(ms),
                                                   C.byref(b)), "mp_kernel_timing_get")
         return n.value, ms.value, b.value
+
+
+# ----------------------------------------------------------------------------- learned guess
+def lid_conditions(cloud: Cloud, lid_speed: float = 1.0):
+    """Cavity boundary conditions: velocity Dirichlet on every wall, the lid
+    (last coordinate = 1) moving along x at lid_speed; pressure Neumann."""
+    nv = cloud.nv
+    u_dir = np.ones(nv, dtype=np.uint8)
+    p_dir = np.zeros(nv, dtype=np.uint8)
+    lid = (cloud.boundary == 1) & (np.abs(cloud.xyz[:, cloud.dim - 1] - 1.0) < 1e-12)
+    u0 = np.where(lid, lid_speed, 0.0)
+    return u_dir, p_dir, u0
+
+
+def train_guess(cloud: Cloud, rho: float, mu: float, dt: float, tol: float, episodes: int,
+                steps_per_episode: int, lid_range=(0.5, 1.5), seed: int = 1, lr: float = 1e-3,
+                device: int = 0, log=None) -> np.ndarray:
+    """Trains the learned pressure guess on solver-generated pairs: `episodes`
+    cavity runs from rest with seeded random lid speeds in lid_range, one Adam
+    step per pressure solve (forward/backward/Adam run in the HIP library).
+    Returns the packed FP32 parameters."""
+    rng = np.random.default_rng(seed)
+    with Context(device) as ctx:
+        ctx.upload(cloud)
+        ctx.weights(3, 3, download=False)
+        u_dir, p_dir, u0 = lid_conditions(cloud)
+        w_dir = u_dir if cloud.dim == 3 else None
+        ctx.fs_create(rho, mu, dt, 2, tol, 20000, u_dir, u_dir, w_dir, p_dir)
+        ctx.guess_mode(MP_GUESS_LEARNED)
+        ctx.guess_train(True, lr, seed)
+        nv = cloud.nv
+        for ep in range(episodes):
+            speed = float(rng.uniform(*lid_range))
+            ctx.set_field(MP_U, u0 * speed)
+            ctx.set_field(MP_V, np.zeros(nv))
+            if cloud.dim == 3:
+                ctx.set_field(MP_W, np.zeros(nv))
+            ctx.set_field(MP_P, np.zeros(ctx.n_sys))
+            iters = 0
+            for it in range(steps_per_episode):
+                ctx.step(1, it)
+                iters += ctx.stats().iterations
+            if log:
+                log(f"guess training episode {ep}: lid {speed:.3f}, mean iterations "
+                    f"{iters / steps_per_episode:.1f}, last loss {ctx.guess_last_loss():.3e}")
+        return ctx.guess_params()

@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "gnn.hpp"
 #include "krylov.hpp"
 #include "mp_internal.hpp"
 
@@ -58,6 +59,13 @@ struct mp_ctx {
     mp_step_stats stats{};
     int guess_mode = MP_GUESS_PREVIOUS;
     DevBuf<double> div_partial;
+    // pressure history and the learned guess
+    DevBuf<double> P1, P2, Ptmp;
+    int hist = 0;
+    std::unique_ptr<GuessNet> net;
+    bool train = false;
+    double last_loss = 0.0;
+    float lr = 1e-3f;
 
     KernelTimer timers[5];
     double bytes[5] = {0, 0, 0, 0, 0};
@@ -135,6 +143,40 @@ void upload_flags(const mp_ctx* c, const uint8_t* src, DevBuf<uint8_t>& dev) {
         for (int d = 0; d < c->nv; ++d) h[d] = src[c->perm[d]] ? 1 : 0;
     dev.alloc(h.size());
     MP_HIP(hipMemcpy(dev.p, h.data(), h.size(), hipMemcpyHostToDevice));
+}
+
+// out = 2 p1 - p2 on the node block; the border unknown keeps p1's value
+__global__ __launch_bounds__(256) void k_extrapolate(int nv, int n, const double* __restrict__ p1,
+                                                     const double* __restrict__ p2,
+                                                     double* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < nv)
+        out[i] = 2.0 * p1[i] - p2[i];
+    else if (i < n)
+        out[i] = p1[i];
+}
+
+__global__ __launch_bounds__(256) void k_copy(int n, const double* __restrict__ a,
+                                              double* __restrict__ b) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// (Re)builds the network for the current cloud and pressure operator, keeping
+// its parameters when it already exists.
+void ensure_net(mp_ctx* c) {
+    std::vector<float> keep;
+    if (c->net) {
+        keep.resize(GNN_NPARAM);
+        c->net->get_params(c->stream, keep.data());
+    }
+    c->net.reset(new GuessNet());
+    c->net->setup(c->stream, c->sv, c->ppe->aval32.p, c->total_slots);
+    c->net->lr = c->lr;
+    if (keep.empty())
+        c->net->init_params(c->stream, 20240611ull);
+    else
+        c->net->set_params(c->stream, keep.data());
 }
 
 }  // namespace
@@ -515,6 +557,11 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
             for (int q = 0; q < 3; ++q) alloc_vec(c, c->U[k][q]);
         alloc_vec(c, c->P);
         alloc_vec(c, c->b);
+        alloc_vec(c, c->P1);
+        alloc_vec(c, c->P2);
+        alloc_vec(c, c->Ptmp);
+        c->hist = 0;
+        if (c->net) ensure_net(c);
         for (int q = 0; q < 3; ++q) alloc_vec(c, c->uh[q]);
         for (int k = 0; k < 2; ++k) {
             c->S[k].alloc((size_t)3 * c->npad + 64);
@@ -544,6 +591,7 @@ int mp_fields_set(mp_ctx* c, int which, const double* values) {
         if (which == MP_P) {
             to_device_order(c, values, h, c->ppe->n);
             upload(c, c->P.p, h);
+            c->hist = 0;  // the pressure history no longer matches
         } else {
             to_device_order(c, values, h, c->nv);
             upload(c, c->U[c->cur][which].p, h);
@@ -594,7 +642,35 @@ void fs_step_one(mp_ctx* c, int it) {
                         c->uh[2].p, Uo[0].p, Uo[1].p, Uo[2].p, c->P.p, c->bnd.p, c->pdir.p,
                         c->normal.p, c->npad, q.rho, q.dt, c->b.p, c->ppe->bordered);
     c->timers[2].end(c->stream);
+    const int n_sys = c->ppe->n;
+    const int nv = c->nv;
+    bool learned = false;
+    if (c->guess_mode != MP_GUESS_PREVIOUS && c->hist >= 2) {
+        hipLaunchKernelGGL(k_extrapolate, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, nv,
+                           n_sys, c->P1.p, c->P2.p, c->P.p);
+        MP_HIP(hipGetLastError());
+        if (c->guess_mode == MP_GUESS_LEARNED) {
+            const double* r = c->ppe->residual_of(c->stream, c->b.p, c->P.p);
+            c->net->features(c->stream, c->P1.p, c->P2.p, r, c->ppe->dinv.p, c->xyz.p, c->npad,
+                             c->dim);
+            c->net->forward(c->stream);
+            c->net->guess(c->stream, c->P1.p, c->P2.p, c->P.p);
+            learned = true;
+        }
+    }
     SolveStats st = c->ppe->solve(c->stream, c->b.p, c->P.p);
+    if (learned && c->train) {
+        c->net->backward(c->stream, c->P1.p, c->P2.p, c->P.p);
+        c->net->adam_step(c->stream);
+        c->last_loss = c->net->loss_host(c->stream);
+    }
+    // pressure history for the extrapolated / learned guesses
+    hipLaunchKernelGGL(k_copy, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, n_sys, c->P1.p,
+                       c->P2.p);
+    hipLaunchKernelGGL(k_copy, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, n_sys, c->P.p,
+                       c->P1.p);
+    MP_HIP(hipGetLastError());
+    c->hist = std::min(c->hist + 1, 2);
     // calc_vel_corr
     c->timers[3].begin(c->stream);
     launch_corrector<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->P.p, c->uh[0].p,
@@ -626,8 +702,6 @@ int mp_fs_step(mp_ctx* c, int n_steps, int it0) {
     return guarded(c, "mp_fs_step", [&]() -> int {
         if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_fs_step: mp_fs_create not called");
         if (n_steps < 0 || it0 < 0) return fail(c, MP_ERR_ARG, "mp_fs_step: negative count");
-        if (c->guess_mode == MP_GUESS_LEARNED)
-            return fail(c, MP_ERR_STATE, "mp_fs_step: learned guess not trained");
         for (int k = 0; k < n_steps; ++k) {
             if (c->dim == 2)
                 fs_step_one<2>(c, it0 + k);
@@ -650,9 +724,88 @@ int mp_stats_get(mp_ctx* c, mp_step_stats* stats) {
 
 int mp_guess_mode(mp_ctx* c, int mode) {
     return guarded(c, "mp_guess_mode", [&]() -> int {
-        if (mode != MP_GUESS_PREVIOUS && mode != MP_GUESS_LEARNED)
+        if (mode != MP_GUESS_PREVIOUS && mode != MP_GUESS_LEARNED && mode != MP_GUESS_EXTRAPOLATE)
             return fail(c, MP_ERR_ARG, "mp_guess_mode: unknown mode");
+        if (mode == MP_GUESS_LEARNED) {
+            if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_mode: mp_fs_create not called");
+            if (!c->net) ensure_net(c);
+        }
         c->guess_mode = mode;
+        return MP_OK;
+    });
+}
+
+int mp_guess_param_count(void) { return GNN_NPARAM; }
+
+int mp_guess_train(mp_ctx* c, int enable, double learning_rate, unsigned long long seed) {
+    return guarded(c, "mp_guess_train", [&]() -> int {
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_train: mp_fs_create not called");
+        if (!(learning_rate > 0)) return fail(c, MP_ERR_ARG, "mp_guess_train: learning rate <= 0");
+        c->lr = (float)learning_rate;
+        if (!c->net) ensure_net(c);
+        c->net->lr = c->lr;
+        if (seed) c->net->init_params(c->stream, seed);
+        c->train = enable != 0;
+        return MP_OK;
+    });
+}
+
+int mp_guess_params(mp_ctx* c, int set, float* params) {
+    return guarded(c, "mp_guess_params", [&]() -> int {
+        if (!params) return fail(c, MP_ERR_ARG, "mp_guess_params: NULL");
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_params: mp_fs_create not called");
+        if (!c->net) ensure_net(c);
+        if (set)
+            c->net->set_params(c->stream, params);
+        else
+            c->net->get_params(c->stream, params);
+        return MP_OK;
+    });
+}
+
+int mp_guess_last_loss(mp_ctx* c, double* loss) {
+    return guarded(c, "mp_guess_last_loss", [&]() -> int {
+        if (!loss) return fail(c, MP_ERR_ARG, "mp_guess_last_loss: NULL");
+        *loss = c->last_loss;
+        return MP_OK;
+    });
+}
+
+int mp_guess_eval(mp_ctx* c, const double* p1, const double* p2, const double* b,
+                  const double* p_true, double* guess, float* grad, double* loss) {
+    return guarded(c, "mp_guess_eval", [&]() -> int {
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_eval: mp_fs_create not called");
+        if (!p1 || !p2 || !b || !guess) return fail(c, MP_ERR_ARG, "mp_guess_eval: NULL");
+        if (!c->net) ensure_net(c);
+        const int n_sys = c->ppe->n;
+        std::vector<double> h;
+        DevBuf<double> d1, d2, db, dt, dg;
+        for (DevBuf<double>* q : {&d1, &d2, &db, &dt, &dg}) alloc_vec(c, *q);
+        to_device_order(c, p1, h, n_sys);
+        upload(c, d1.p, h);
+        to_device_order(c, p2, h, n_sys);
+        upload(c, d2.p, h);
+        to_device_order(c, b, h, n_sys);
+        upload(c, db.p, h);
+        hipLaunchKernelGGL(k_extrapolate, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, c->nv,
+                           n_sys, d1.p, d2.p, dg.p);
+        const double* r = c->ppe->residual_of(c->stream, db.p, dg.p);
+        c->net->features(c->stream, d1.p, d2.p, r, c->ppe->dinv.p, c->xyz.p, c->npad, c->dim);
+        c->net->forward(c->stream);
+        c->net->guess(c->stream, d1.p, d2.p, dg.p);
+        download(c, dg.p, h);
+        from_device_order(c, h, guess, c->nv);
+        if (p_true) {
+            to_device_order(c, p_true, h, n_sys);
+            upload(c, dt.p, h);
+            c->net->backward(c->stream, d1.p, d2.p, dt.p);
+            double ls = c->net->loss_host(c->stream);
+            if (loss) *loss = ls;
+            if (grad) {
+                MP_HIP(hipMemcpy(grad, c->net->grad.p, sizeof(float) * GNN_NPARAM,
+                                 hipMemcpyDeviceToHost));
+            }
+        }
         return MP_OK;
     });
 }

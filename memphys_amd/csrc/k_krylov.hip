@@ -623,10 +623,11 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, int dim_, int npad_,
 }
 
 double PpeSolver::spmv_bytes(int fp32) const {
-    // val + col per entry; per row: gathered x once, y written, r0 read, len read,
-    // D^-1 read for the border row
+    // per entry: value (8 B, or 4 B for the FP32 operator) + column (4 B);
+    // per row: gathered x once, y written, r0 read (FP64 vectors), length,
+    // D^-1 for the border row
     const double vb = fp32 ? 4.0 : 8.0;
-    return (double)nnz * (vb + 4.0) + (double)nv * (vb * (bordered ? 4.0 : 3.0) + 4.0);
+    return (double)nnz * (vb + 4.0) + (double)nv * (8.0 * (bordered ? 4.0 : 3.0) + 4.0);
 }
 
 template <typename TM, typename T>
@@ -713,6 +714,18 @@ void PpeSolver::residual64(hipStream_t st, const double* b_dev, double* out_rr, 
     MP_HIP(hipStreamSynchronize(st));
     *out_rr = host_sc[0].red[0];
     *out_bb = host_sc[0].red[1];
+}
+
+double* PpeSolver::residual_of(hipStream_t st, const double* b_dev, const double* x_dev) {
+    const int g_n = nblk(n);
+    hipLaunchKernelGGL(k_scale, dim3(g_n), dim3(BLOCK), 0, st, n, dinv.p, x_dev, xt.p, 1);
+    const int g = nblk(nv);
+    hipLaunchKernelGGL(k_residual64, dim3(grid_xcd(g)), dim3(BLOCK), 0, st, sv, bordered, aval.p,
+                       dinv.p, b_dev, xt.p, tmp.p, partial.p);
+    hipLaunchKernelGGL(k_finish_res64, dim3(1), dim3(FIN), 0, st, g, nv, bordered, b_dev, tmp.p,
+                       partial.p, sc.p);
+    MP_HIP(hipGetLastError());
+    return tmp.p;
 }
 
 SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) {

@@ -51,6 +51,8 @@ struct PpeSolver {
     // b_dev, x_dev: device order, length n (guess in x_dev, solution out).
     SolveStats solve(hipStream_t s, const double* b_dev, double* x_dev);
     double spmv_bytes(int fp32) const;
+    // r = b - A x (FP64, unscaled operator incl. border) into `tmp`; returns tmp.p
+    double* residual_of(hipStream_t s, const double* b_dev, const double* x_dev);
 
   private:
     template <typename TM, typename T>

@@ -1,0 +1,87 @@
+"""Learned pressure guess on the GPU: forward, loss and gradient against the
+float64 oracle; training effect against the previous-pressure guess."""
+import numpy as np
+import pytest
+
+from conftest import lid_bc, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _fs(ctx, cloud, precision=0, tol=1e-10):
+    u_dir, p_dir, u0 = lid_bc(cloud, cloud.dim - 1)
+    ctx.upload(cloud)
+    (gx, gy, gz, lap), _ = ctx.weights(3, 3)
+    ctx.fs_create(1.0, 0.0025, 5e-4, 2, tol, 20000, u_dir, u_dir,
+                  u_dir if cloud.dim == 3 else None, p_dir, precision=precision)
+    return (gx, gy, gz, lap), u_dir, p_dir, u0
+
+
+def test_guess_forward_and_gradient_match_oracle(cube3d_small):
+    import scipy.sparse as sp
+
+    import memphys_amd as M
+    import oracle as O
+    from oracle import gnn as G
+    c = cube3d_small
+    with M.Context(0) as ctx:
+        w, u_dir, p_dir, u0 = _fs(ctx, c)
+        rng = np.random.default_rng(5)
+        theta = (rng.standard_normal(G.NPARAM) * 0.1).astype(np.float32)
+        ctx.guess_params(theta)
+        n = ctx.n_sys
+        x = c.xyz
+        p1 = np.r_[np.sin(3 * x[:, 0]) * x[:, 1] + 0.1 * rng.standard_normal(c.nv), 0.3]
+        p2 = np.r_[0.9 * p1[:c.nv] + 0.05 * np.cos(2 * x[:, 2]), 0.2]
+        b = np.r_[rng.standard_normal(c.nv) * 1e3, 0.0]
+        pt = np.r_[1.1 * p1[:c.nv] - 0.1 * p2[:c.nv] + 0.01 * rng.standard_normal(c.nv), 0.0]
+        g_gpu, grad_gpu, loss_gpu = ctx.guess_eval(p1, p2, b, pt)
+    fs = O.FractionalStep(c, w, u_dir, u_dir, u_dir, p_dir, 1.0, 0.0025, 5e-4, 2)
+    rp, ci, v = fs.ppe_csr()
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    g_or, grad_or, loss_or = G.evaluate(theta, A, c.nv, p1, p2, b, x, pt)
+    _print("guess rel", rel_err(g_gpu, g_or), "loss", loss_gpu, loss_or,
+           "grad rel", rel_err(grad_gpu, grad_or))
+    assert rel_err(g_gpu, g_or) <= 1e-5
+    assert abs(loss_gpu - loss_or) <= 1e-4 * abs(loss_or)
+    # per parameter block (FP32 accumulation on the device, float64 here)
+    blocks = [(G.W0, G.B0), (G.B0, G.LAYER)] + \
+        [(G.LAYER + l * G.LAYER_SIZE, G.LAYER + (l + 1) * G.LAYER_SIZE) for l in range(G.L)] + \
+        [(G.WO, G.NPARAM)]
+    for a, bnd in blocks:
+        assert rel_err(grad_gpu[a:bnd], grad_or[a:bnd]) <= 1e-3, (a, bnd)
+
+
+def test_trained_guess_saves_iterations():
+    """Train on episodes with random lid speeds, then run a fresh cavity with
+    the previous-pressure guess (the reference's solveWithGuess(rhs, p_old))
+    and with the trained network: the network must cut the Krylov iterations."""
+    import memphys_amd as M
+    c = M.box_cloud(dim=3, n=16, jitter=0.3, seed=11, k_interior=40, k_layers=(60, 55, 50))
+    params = M.train_guess(c, 1.0, 0.0025, 5e-4, 1e-8, episodes=4, steps_per_episode=100,
+                           seed=3, log=_print)
+    res = {}
+    for mode in (M.MP_GUESS_PREVIOUS, M.MP_GUESS_EXTRAPOLATE, M.MP_GUESS_LEARNED):
+        with M.Context(0) as ctx:
+            _fs(ctx, c, tol=1e-8)
+            u_dir, p_dir, u0 = M.lid_conditions(c, 1.0)
+            ctx.set_field(M.MP_U, u0)
+            ctx.set_field(M.MP_V, np.zeros(c.nv))
+            ctx.set_field(M.MP_W, np.zeros(c.nv))
+            ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
+            ctx.guess_mode(mode)
+            if mode == M.MP_GUESS_LEARNED:
+                ctx.guess_params(params)
+            its = []
+            for it in range(120):
+                ctx.step(1, it)
+                its.append(ctx.stats().iterations)
+                assert ctx.stats().rel_res <= 1e-8
+            res[mode] = np.mean(its[20:])
+    _print("mean iterations: previous", res[M.MP_GUESS_PREVIOUS], "extrapolate",
+           res[M.MP_GUESS_EXTRAPOLATE], "learned", res[M.MP_GUESS_LEARNED])
+    assert res[M.MP_GUESS_LEARNED] <= 0.85 * res[M.MP_GUESS_PREVIOUS]
