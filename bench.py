@@ -161,6 +161,29 @@ def main():
     wk = ctx.kernel_timing_get(4)
     ctx.kernel_timing(False)
     log(f"weights: {t_w:.3f}s wall incl. download, kernel {wk[1]:.2f} ms")
+    # FP32 LU + FP64 refinement of the same weights (config: batched assembly,
+    # FP64 vs FP32+FP64-refined); the FP64 weights stay in use for the steps
+    ctx.kernel_timing(True)
+    (mx, my, mz, ml), _ = ctx.weights(3, 3, precision=M.MP_PREC_MIXED, download=not args.no_cpu)
+    wkm = ctx.kernel_timing_get(4)
+    ctx.kernel_timing(False)
+    weights_check = {"fp64_ms": wk[1], "fp32_refined_ms": wkm[1]}
+    if not args.no_cpu:
+        import oracle as O
+        rows = np.random.default_rng(3).choice(cloud.nv, 3000, replace=False)
+        ox, oy, oz, ol, _ = O.stencil_weights(cloud, 3, 3, rows=rows, threads=0)
+        worst64 = worst32 = 0.0
+        for r in rows:
+            s_ = slice(cloud.row_ptr[r], cloud.row_ptr[r + 1])
+            for a, b, o in ((gx, mx, ox), (gy, my, oy), (gz, mz, oz), (lap, ml, ol)):
+                den = np.linalg.norm(o[s_])
+                worst64 = max(worst64, np.linalg.norm(a[s_] - o[s_]) / den)
+                worst32 = max(worst32, np.linalg.norm(b[s_] - o[s_]) / den)
+        weights_check.update({"sample_stencils": 3000, "fp64_worst_rel_vs_oracle": worst64,
+                              "fp32_refined_worst_rel_vs_oracle": worst32})
+        del mx, my, mz, ml
+    ctx.weights(3, 3, download=False)  # FP64 weights back for the time steps
+    log(f"weights check: {weights_check}")
 
     forms = args.forms.split(",")
     fp64 = run_form(M, ctx, cloud, prm, M.MP_PREC_FP64, args.steps, args.warmup, "fp64")
@@ -216,6 +239,7 @@ def main():
         "fp64_spmv_roofline": {"achieved": ach64, "frac": ach64 / HBM_PEAK_GBS,
                                "bytes_per_launch": b64, "mean_ms": ms64},
         "weights_kernel_ms": wk[1],
+        "weights_fp64_vs_fp32_refined": weights_check,
         "divergence_fp64": fp64["divergence"],
     }
     if not args.no_cpu:

@@ -69,6 +69,7 @@ struct mp_ctx {
 
     KernelTimer timers[5];
     double bytes[5] = {0, 0, 0, 0, 0};
+    int weights_fp64_rows = 0;  // stencils the last assembly solved in FP64
 };
 
 namespace {
@@ -394,9 +395,49 @@ int mp_weights_assemble(mp_ctx* c, int phs_deg, int poly_deg, const int* exps, i
         DevBuf<double> dcond;
         if (cond_out) dcond.alloc(c->npad);
         c->timers[4].begin(c->stream);
-        launch_weights_fp64(c->stream, c->dim, phs_deg, m, de.p, c->sv, c->xyz.p, c->npad,
-                            c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p,
-                            cond_out ? dcond.p : nullptr, c->max_len);
+        c->weights_fp64_rows = c->nv;
+        if (precision_mode == MP_PREC_FP64 || cond_out) {
+            launch_weights_fp64(c->stream, c->dim, phs_deg, m, de.p, c->sv, c->xyz.p, c->npad,
+                                c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p,
+                                cond_out ? dcond.p : nullptr, c->max_len);
+        } else {
+            // FP32 LU + FP64 refinement for the shortest (interior) stencils; the
+            // longer boundary / near-boundary stencils and any stencil whose
+            // refinement stalls are solved in FP64.
+            int kmin = MAX_STENCIL + 1;
+            for (int d = 0; d < c->nv; ++d)
+                kmin = std::min(kmin, c->h_rowptr[c->perm[d] + 1] - c->h_rowptr[c->perm[d]]);
+            std::vector<int> mixed_rows, fp64_rows;
+            for (int d = 0; d < c->nv; ++d) {
+                const int len = c->h_rowptr[c->perm[d] + 1] - c->h_rowptr[c->perm[d]];
+                (len == kmin ? mixed_rows : fp64_rows).push_back(d);
+            }
+            DevBuf<int> dm, df, dflag;
+            dm.alloc(mixed_rows.size());
+            df.alloc(std::max<size_t>(1, fp64_rows.size() + mixed_rows.size()));
+            dflag.alloc(c->npad);
+            MP_HIP(hipMemsetAsync(dflag.p, 0, sizeof(int) * c->npad, c->stream));
+            if (!mixed_rows.empty())
+                MP_HIP(hipMemcpy(dm.p, mixed_rows.data(), sizeof(int) * mixed_rows.size(),
+                                 hipMemcpyHostToDevice));
+            launch_weights_mixed(c->stream, c->dim, phs_deg, m, de.p, c->sv, c->xyz.p, c->npad,
+                                 c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p, dm.p,
+                                 (int)mixed_rows.size(), dflag.p, kmin, 8);
+            std::vector<int> hflag(c->npad);
+            MP_HIP(hipMemcpyAsync(hflag.data(), dflag.p, sizeof(int) * c->npad,
+                                  hipMemcpyDeviceToHost, c->stream));
+            MP_HIP(hipStreamSynchronize(c->stream));
+            for (int d : mixed_rows)
+                if (hflag[d]) fp64_rows.push_back(d);
+            c->weights_fp64_rows = (int)fp64_rows.size();
+            if (!fp64_rows.empty()) {
+                MP_HIP(hipMemcpy(df.p, fp64_rows.data(), sizeof(int) * fp64_rows.size(),
+                                 hipMemcpyHostToDevice));
+                launch_weights_fp64(c->stream, c->dim, phs_deg, m, de.p, c->sv, c->xyz.p,
+                                    c->npad, c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p, nullptr,
+                                    c->max_len, df.p, (int)fp64_rows.size());
+            }
+        }
         c->timers[4].end(c->stream);
         MP_HIP(hipStreamSynchronize(c->stream));
         c->timers[4].harvest();

@@ -82,9 +82,16 @@ struct KScalars {
 };
 
 // ---- launchers (defined in the .hip files) ----
+// rows == nullptr: every row; otherwise the listed device rows.
 void launch_weights_fp64(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
                          const SellView& sv, const double* xyz_soa, int npad, double* gx,
-                         double* gy, double* gz, double* lap, double* cond, int max_len);
+                         double* gy, double* gz, double* lap, double* cond, int max_len,
+                         const int* rows = nullptr, int nrows = 0);
+// FP32 LU + FP64 refinement for the listed rows; flag[r] = 1 where it did not converge.
+void launch_weights_mixed(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
+                          const SellView& sv, const double* xyz_soa, int npad, double* gx,
+                          double* gy, double* gz, double* lap, const int* rows, int nrows,
+                          int* flag, int max_len, int max_ref);
 
 template <int DIM>
 void launch_predictor(hipStream_t s, const SellView& sv, const double* gx, const double* gy,

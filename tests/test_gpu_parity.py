@@ -215,3 +215,23 @@ def test_class_api_driver_matches_oracle(tmp_path):
     errs = (rel_err(u, uo), rel_err(v, vo), rel_err(p, po))
     _print("class API driver errs", errs)
     assert max(errs) <= 1e-8
+
+
+def test_weights_mixed_precision_match_oracle(ctx, cube3d_small, cavity2d):
+    """FP32 LU + FP64 refinement (interior stencils) and FP64 (boundary-layer
+    stencils): weights within 1e-10 relative of the FP64 oracle."""
+    import memphys_amd as M
+    import oracle as O
+    for c in (cube3d_small, cavity2d):
+        ctx.upload(c)
+        (gx, gy, gz, lap), _ = ctx.weights(3, 3, precision=M.MP_PREC_MIXED)
+        ox, oy, oz, ol, _ = O.stencil_weights(c, 3, 3)
+        pairs = [(gx, ox), (gy, oy), (lap, ol)] + ([(gz, oz)] if c.dim == 3 else [])
+        worst = 0.0
+        for r in range(0, c.nv, 7):
+            s = slice(c.row_ptr[r], c.row_ptr[r + 1])
+            worst = max(worst, max(rel_err(a[s], b[s]) for a, b in pairs))
+        _print(f"mixed weights {c.dim}D worst per-stencil rel {worst:.2e}")
+        assert worst <= 1e-10
+        for a, b in pairs:
+            assert rel_err(a, b) <= 1e-10
