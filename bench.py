@@ -47,11 +47,15 @@ def lid_bc(cloud):
     return u_dir, p_dir, u0
 
 
-def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None):
+def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, precond=0):
     u_dir, p_dir, u0 = lid_bc(cloud)
+    t = time.perf_counter()
     ctx.fs_create(prm["rho"], prm["mu"], prm["dt"], 2, prm["tol"], prm["n_iter"], u_dir, u_dir,
                   u_dir, p_dir, precision=precision, inner_tol=prm["inner_tol"],
-                  check_every=prm["check_every"])
+                  check_every=prm["check_every"], precond=precond)
+    t_create = time.perf_counter() - t
+    hier = ctx.precond_info()
+    log(f"[{label}] fs_create {t_create:.2f}s, hierarchy {hier}")
     if guess is None:
         ctx.guess_mode(M.MP_GUESS_PREVIOUS)
     else:
@@ -88,7 +92,7 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None):
     log(f"[{label}] {steps} steps in {t1 - t0:.3f}s, iters/step {np.mean(iters):.1f}, "
         f"max rel_res {max(res):.2e}, spmv {spmv}")
     return dict(seconds=t1 - t0, iters=iters, rel_res=res, spmv=spmv, predictor=pred,
-                fields=fields, divergence=div.tolist())
+                fields=fields, divergence=div.tolist(), hierarchy=hier, create_s=t_create)
 
 
 def cpu_baseline(cloud, w, prm, iters_per_step, max_iter_sample=10):
@@ -130,8 +134,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--train-steps", type=int, default=2000,
                     help="solver steps of guess training (10 episodes)")
-    ap.add_argument("--forms", default="fp64,headline",
-                    help="comma list of forms to time (fp64, headline)")
+    ap.add_argument("--forms", default="fp64,mixed,headline",
+                    help="comma list of forms to time besides fp64: mixed (FP32-inner, "
+                         "Jacobi), headline (FP32-inner, multigrid, learned guess)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
@@ -187,9 +192,13 @@ def main():
 
     forms = args.forms.split(",")
     fp64 = run_form(M, ctx, cloud, prm, M.MP_PREC_FP64, args.steps, args.warmup, "fp64")
+    extra = {}
+    if "mixed" in forms:
+        extra["mixed_jacobi"] = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps,
+                                         args.warmup, "mixed")
     if "headline" in forms:
         head = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps, args.warmup,
-                        "mixed+learned", guess=guess)
+                        "mixed+multigrid+learned", guess=guess, precond=M.MP_PRECOND_AMG)
     else:
         head = fp64
     vdiff = max(np.linalg.norm(a - b) / max(np.linalg.norm(a), 1e-300)
@@ -220,7 +229,8 @@ def main():
                          f"({args.n}^3 lattice, +-0.3h), PHS r^3 + deg-3 poly, k=40/50/55/60, "
                          f"dt=5e-4, AB2, PPE rel. residual {args.tol:g}"),
             "nv": cloud.nv, "nnz": cloud.nnz,
-            "form": "FP32-operator inner BiCGSTAB in FP64 defect correction + learned guess",
+            "form": ("FP32-operator inner BiCGSTAB with FP32 smoothed-aggregation multigrid "
+                     "V-cycle, in FP64 defect correction, + learned guess"),
             "guess_training": f"{args.train_steps} solver steps on {small.nv} nodes, "
                               f"lid speeds U[0.5,1.5], {t_train:.1f}s",
         },
@@ -234,6 +244,12 @@ def main():
         "speedup_vs_fp64": sps_head / sps_fp64,
         "ppe_iters_per_step": {"fp64": float(np.mean(fp64["iters"])),
                                "headline": float(np.mean(head["iters"]))},
+        "multigrid": {"levels": head["hierarchy"]["levels"], "rows": head["hierarchy"]["rows"],
+                      "nnz": head["hierarchy"]["nnz"],
+                      "host_setup_s": head["hierarchy"]["setup_s"]},
+        "other_forms": {k: {"timesteps_per_s": args.steps / f["seconds"],
+                            "ppe_iters_per_step": float(np.mean(f["iters"])),
+                            "max_ppe_rel_res": max(f["rel_res"])} for k, f in extra.items()},
         "max_ppe_rel_res": {"fp64": max(fp64["rel_res"]), "headline": max(head["rel_res"])},
         "headline_vs_fp64_rel_diff": {"velocity": vdiff, "pressure": pdiff},
         "fp64_spmv_roofline": {"achieved": ach64, "frac": ach64 / HBM_PEAK_GBS,

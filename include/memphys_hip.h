@@ -43,6 +43,11 @@ enum mp_precision { MP_PREC_FP64 = 0, MP_PREC_MIXED = 1 };
  * pressure until two pressures are in the history. */
 enum mp_guess { MP_GUESS_PREVIOUS = 0, MP_GUESS_LEARNED = 1, MP_GUESS_EXTRAPOLATE = 2 };
 
+/* PPE preconditioner: Jacobi (right diagonal scaling, the plain path), or a
+ * smoothed-aggregation algebraic multigrid V(1,1)-cycle in FP32 (set up on the
+ * host from the assembled operator, applied on the device). */
+enum mp_precond { MP_PRECOND_JACOBI = 0, MP_PRECOND_AMG = 1 };
+
 typedef struct {
     double rho, mu, dt;
     int temporal_order;       /* 1 = Euler, 2 = AB2 after the first step */
@@ -51,6 +56,7 @@ typedef struct {
     int precision;            /* enum mp_precision */
     double inner_tolerance;   /* MIXED: inner FP32 relative reduction per cycle */
     int check_every;          /* iterations launched between convergence polls (0 = default) */
+    int preconditioner;       /* enum mp_precond */
 } mp_fs_params;
 
 typedef struct {
@@ -144,6 +150,11 @@ int mp_guess_eval(mp_ctx* ctx, const double* p1, const double* p2, const double*
  * (postprocessing_functions.cpp:192-195, 248-251):
  * out = {max_int, mean_int, max_bnd, mean_bnd, max_all, mean_all}. */
 int mp_divergence(mp_ctx* ctx, double* out);
+
+/* Multigrid hierarchy of the pressure operator (MP_PRECOND_AMG): number of
+ * levels (<= max_levels), rows and stored entries per level, host set-up time. */
+int mp_precond_info(mp_ctx* ctx, int max_levels, int* levels, int* rows, long* nnz,
+                    double* setup_seconds);
 
 /* Per-kernel timing with HIP events on the library's stream.
  * which: 0 = PPE SpMV, 1 = predictor, 2 = pressure RHS, 3 = corrector,

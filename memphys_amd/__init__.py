@@ -24,6 +24,7 @@ MP_U, MP_V, MP_W, MP_P = 0, 1, 2, 3
 MP_GX, MP_GY, MP_GZ, MP_LAP = 0, 1, 2, 3
 MP_PREC_FP64, MP_PREC_MIXED = 0, 1
 MP_GUESS_PREVIOUS, MP_GUESS_LEARNED, MP_GUESS_EXTRAPOLATE = 0, 1, 2
+MP_PRECOND_JACOBI, MP_PRECOND_AMG = 0, 1
 
 # Every symbol include/memphys_hip.h declares.
 HIP_SYMBOLS = (
@@ -32,7 +33,7 @@ HIP_SYMBOLS = (
     "mp_solver_system_size", "mp_solver_solve", "mp_fs_create", "mp_fields_set",
     "mp_fields_get", "mp_fs_step", "mp_stats_get", "mp_guess_mode", "mp_divergence",
     "mp_kernel_timing", "mp_kernel_timing_get", "mp_guess_param_count", "mp_guess_train",
-    "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval",
+    "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
 )
 
 
@@ -45,7 +46,7 @@ class FsParams(C.Structure):
         ("rho", C.c_double), ("mu", C.c_double), ("dt", C.c_double),
         ("temporal_order", C.c_int), ("solver_tolerance", C.c_double),
         ("n_iter", C.c_int), ("precision", C.c_int), ("inner_tolerance", C.c_double),
-        ("check_every", C.c_int),
+        ("check_every", C.c_int), ("preconditioner", C.c_int),
     ]
 
 
@@ -103,6 +104,8 @@ def hip_lib() -> C.CDLL:
                                                                             flags="C_CONTIGUOUS")]
         lib.mp_guess_last_loss.argtypes = [vp, C.POINTER(C.c_double)]
         lib.mp_guess_eval.argtypes = [vp, _dp, _dp, _dp, vp, _dp, vp, C.POINTER(C.c_double)]
+        lib.mp_precond_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), vp, vp,
+                                        C.POINTER(C.c_double)]
         for name in HIP_SYMBOLS:
             getattr(lib, name).restype = C.c_char_p if name == "mp_last_error" else C.c_int
         _hip = lib
@@ -124,6 +127,10 @@ def host_lib() -> C.CDLL:
         lib.mph_rcm.argtypes = [C.c_int, _ip, _ip, _ip]
         lib.mph_apply_order.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp, _u8, C.c_void_p, _ip, _ip]
         lib.mph_poly_exponents.argtypes = [C.c_int, C.c_int, _ip]
+        lib.mph_amg_first_level.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_double, _ip,
+                                            C.POINTER(C.c_int), _ip, _ip, _dp, C.c_long, _ip,
+                                            _ip, _dp, C.c_long]
+        lib.mph_amg_bordered_inverse.argtypes = [C.c_int, _ip, _ip, _dp, _dp]
         _host = lib
     return _host
 
@@ -274,8 +281,9 @@ class Context:
         return xn, xo
 
     def fs_create(self, rho, mu, dt, order, tol, n_iter, u_dir, v_dir, w_dir, p_dir,
-                  precision=MP_PREC_FP64, inner_tol=1e-3, check_every=8):
-        p = FsParams(rho, mu, dt, order, tol, n_iter, precision, inner_tol, check_every)
+                  precision=MP_PREC_FP64, inner_tol=1e-3, check_every=8,
+                  precond=MP_PRECOND_JACOBI):
+        p = FsParams(rho, mu, dt, order, tol, n_iter, precision, inner_tol, check_every, precond)
         wd = np.ascontiguousarray(w_dir, dtype=np.uint8) if w_dir is not None else None
         self._wd = wd
         self._check(self.lib.mp_fs_create(
@@ -310,6 +318,20 @@ class Context:
         out = np.zeros(6)
         self._check(self.lib.mp_divergence(self.h, out), "mp_divergence")
         return out
+
+    def precond_info(self) -> dict:
+        """Multigrid hierarchy: rows and stored entries per level, set-up seconds."""
+        lv = C.c_int()
+        rows = np.zeros(16, dtype=np.int32)
+        nnz = np.zeros(16, dtype=np.int64)
+        secs = C.c_double()
+        self._check(self.lib.mp_precond_info(self.h, 16, C.byref(lv),
+                                             rows.ctypes.data_as(C.c_void_p),
+                                             nnz.ctypes.data_as(C.c_void_p), C.byref(secs)),
+                    "mp_precond_info")
+        L = min(lv.value, 16)
+        return {"levels": lv.value, "rows": rows[:L].tolist(), "nnz": nnz[:L].tolist(),
+                "setup_s": secs.value}
 
     def guess_mode(self, mode: int):
         self._check(self.lib.mp_guess_mode(self.h, mode), "mp_guess_mode")
@@ -407,3 +429,47 @@ def train_guess(cloud: Cloud, rho: float, mu: float, dt: float, tol: float, epis
                 log(f"guess training episode {ep}: lid {speed:.3f}, mean iterations "
                     f"{iters / steps_per_episode:.1f}, last loss {ctx.guess_last_loss():.3e}")
         return ctx.guess_params()
+
+
+def amg_first_level(A, k: int = 6, omega_p: float = 0.66):
+    """First coarsening step of the library's multigrid set-up on a scipy CSR
+    matrix: (aggregate per row, P, R A P) as computed by the C++ builder."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    n = A.shape[0]
+    ptr = np.ascontiguousarray(A.indptr, dtype=np.int32)
+    col = np.ascontiguousarray(A.indices, dtype=np.int32)
+    val = np.ascontiguousarray(A.data, dtype=np.float64)
+    agg = np.zeros(n, dtype=np.int32)
+    na = C.c_int()
+    pcap = int(A.nnz + n)
+    p_ptr = np.zeros(n + 1, dtype=np.int32)
+    p_col = np.zeros(pcap, dtype=np.int32)
+    p_val = np.zeros(pcap)
+    ac_cap = int(8 * A.nnz)
+    ac_ptr = np.zeros(n + 1, dtype=np.int32)
+    ac_col = np.zeros(ac_cap, dtype=np.int32)
+    ac_val = np.zeros(ac_cap)
+    rc = host_lib().mph_amg_first_level(n, ptr, col, val, k, omega_p, agg, C.byref(na), p_ptr,
+                                        p_col, p_val, pcap, ac_ptr, ac_col, ac_val, ac_cap)
+    if rc != 0:
+        raise MemphysError(f"mph_amg_first_level failed ({rc})")
+    m = na.value
+    P = sp.csr_matrix((p_val[:p_ptr[n]], p_col[:p_ptr[n]], p_ptr), shape=(n, m))
+    Ac = sp.csr_matrix((ac_val[:ac_ptr[m]], ac_col[:ac_ptr[m]], ac_ptr[:m + 1]), shape=(m, m))
+    return agg, P, Ac
+
+
+def amg_bordered_inverse(A) -> np.ndarray:
+    """The library's coarsest-level solve: top-left block of inv([[A, 1], [1^T, 0]])."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    G = np.zeros(n * n)
+    rc = host_lib().mph_amg_bordered_inverse(n, np.ascontiguousarray(A.indptr, dtype=np.int32),
+                                             np.ascontiguousarray(A.indices, dtype=np.int32),
+                                             np.ascontiguousarray(A.data, dtype=np.float64), G)
+    if rc != 0:
+        raise MemphysError("mph_amg_bordered_inverse failed")
+    return G.reshape(n, n)

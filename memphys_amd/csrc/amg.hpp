@@ -1,0 +1,55 @@
+// Device side of the smoothed-aggregation preconditioner: one V(1,1)-cycle
+// with damped-Jacobi smoothing, in FP32 (host set-up in amg_build.hpp).
+#pragma once
+#include <string>
+
+#include "amg_build.hpp"
+#include "mp_internal.hpp"
+
+namespace mp {
+
+// A sparse matrix of the hierarchy in one of two layouts, chosen per matrix
+// by timing at set-up:
+//   g == 0: SELL-64, one lane per row (short rows; ptr = slice pointers);
+//   g  > 0: CSR with g lanes per row and a g-lane reduction (long rows of
+//           the coarse operators and restrictions; ptr = row pointers).
+struct AmgMat {
+    int n = 0, nslices = 0, g = 0;
+    long nnz = 0;
+    DevBuf<int> ptr_b, col_b;
+    DevBuf<float> val_b;
+    const int* ptr = nullptr;
+    const int* col = nullptr;
+    const float* val = nullptr;
+};
+
+struct AmgLevelDev {
+    int n = 0;
+    AmgMat A;      // operator of this level
+    AmgMat P, R;   // prolongation from / restriction to the next level
+    DevBuf<float> dinv;  // omega / a_ii
+    DevBuf<float> b, x, res, y;
+};
+
+struct AmgDevice {
+    double omega = 0.7;  // Jacobi smoothing weight
+    std::vector<AmgLevelDev> lv;
+    int nc = 0;
+    DevBuf<float> cinv;  // dense inverse on the coarsest level
+    std::vector<int> rows;
+    std::vector<long> nnz;
+    std::string layout;  // chosen layout per matrix, for reports
+    double setup_seconds = 0.0;
+
+    // Builds the hierarchy from the finest operator, given in SELL-64 slots
+    // (device order) by the solver: sv and val32 stay owned by the solver and
+    // serve as level 0.
+    void setup(hipStream_t s, const SellView& sv, const double* val64_dev,
+               const float* val32_dev, const mp_amg::Options& opt);
+    // out = M^-1 in for the bordered system of size nv + bordered: the V-cycle
+    // on the node block, identity on the border unknown.  Every kernel returns
+    // at once when done_flag (may be null) is non-zero.
+    void apply(hipStream_t s, const double* in, double* out, int bordered, const int* done_flag);
+};
+
+}  // namespace mp

@@ -172,7 +172,7 @@ void ensure_net(mp_ctx* c) {
         c->net->get_params(c->stream, keep.data());
     }
     c->net.reset(new GuessNet());
-    c->net->setup(c->stream, c->sv, c->ppe->aval32.p, c->total_slots);
+    c->net->setup(c->stream, c->sv, c->ppe->scaled32, c->total_slots);
     c->net->lr = c->lr;
     if (keep.empty())
         c->net->init_params(c->stream, 20240611ull);
@@ -575,6 +575,8 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
             return fail(c, MP_ERR_ARG, "mp_fs_create: dt, rho, tolerance and n_iter must be > 0");
         if (prm->precision != MP_PREC_FP64 && prm->precision != MP_PREC_MIXED)
             return fail(c, MP_ERR_ARG, "mp_fs_create: unknown precision");
+        if (prm->preconditioner != MP_PRECOND_JACOBI && prm->preconditioner != MP_PRECOND_AMG)
+            return fail(c, MP_ERR_ARG, "mp_fs_create: unknown preconditioner");
         c->prm = *prm;
         for (int q = 0; q < c->dim; ++q) upload_flags(c, dirs[q], c->udir[q]);
         if (c->dim == 2) upload_flags(c, nullptr, c->udir[2]);
@@ -587,7 +589,8 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
             }
         c->ppe.reset(new PpeSolver());
         c->ppe->setup(c->stream, c->sv, c->dim, c->npad, c->w[0].p, c->w[1].p, c->w[2].p,
-                      c->w[3].p, c->normal.p, c->bnd.p, c->pdir.p, bordered, 0.0, 0.0, 1.0);
+                      c->w[3].p, c->normal.p, c->bnd.p, c->pdir.p, bordered, 0.0, 0.0, 1.0,
+                      prm->preconditioner);
         c->ppe->tol = prm->solver_tolerance;
         c->ppe->maxit = prm->n_iter;
         c->ppe->precision = prm->precision;
@@ -692,7 +695,7 @@ void fs_step_one(mp_ctx* c, int it) {
         MP_HIP(hipGetLastError());
         if (c->guess_mode == MP_GUESS_LEARNED) {
             const double* r = c->ppe->residual_of(c->stream, c->b.p, c->P.p);
-            c->net->features(c->stream, c->P1.p, c->P2.p, r, c->ppe->dinv.p, c->xyz.p, c->npad,
+            c->net->features(c->stream, c->P1.p, c->P2.p, r, c->ppe->diag_inv, c->xyz.p, c->npad,
                              c->dim);
             c->net->forward(c->stream);
             c->net->guess(c->stream, c->P1.p, c->P2.p, c->P.p);
@@ -831,7 +834,7 @@ int mp_guess_eval(mp_ctx* c, const double* p1, const double* p2, const double* b
         hipLaunchKernelGGL(k_extrapolate, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, c->nv,
                            n_sys, d1.p, d2.p, dg.p);
         const double* r = c->ppe->residual_of(c->stream, db.p, dg.p);
-        c->net->features(c->stream, d1.p, d2.p, r, c->ppe->dinv.p, c->xyz.p, c->npad, c->dim);
+        c->net->features(c->stream, d1.p, d2.p, r, c->ppe->diag_inv, c->xyz.p, c->npad, c->dim);
         c->net->forward(c->stream);
         c->net->guess(c->stream, d1.p, d2.p, dg.p);
         download(c, dg.p, h);
@@ -881,6 +884,23 @@ int mp_divergence(mp_ctx* c, double* out) {
         out[3] = nb ? sb / nb : 0.0;
         out[4] = std::max(mi, mb);
         out[5] = (si + sb) / c->nv;
+        return MP_OK;
+    });
+}
+
+int mp_precond_info(mp_ctx* c, int max_levels, int* levels, int* rows, long* nnz,
+                    double* setup_seconds) {
+    return guarded(c, "mp_precond_info", [&]() -> int {
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_precond_info: mp_fs_create not called");
+        if (!levels || max_levels < 0) return fail(c, MP_ERR_ARG, "mp_precond_info: bad arguments");
+        const AmgDevice* a = c->ppe->amg.get();
+        const int L = a ? (int)a->rows.size() : 0;
+        *levels = L;
+        for (int l = 0; l < L && l < max_levels; ++l) {
+            if (rows) rows[l] = a->rows[l];
+            if (nnz) nnz[l] = a->nnz[l];
+        }
+        if (setup_seconds) *setup_seconds = a ? a->setup_seconds : 0.0;
         return MP_OK;
     });
 }

@@ -4,6 +4,9 @@
 // branch, solver.cpp:337-350), right-preconditioned by Jacobi.  The Jacobi
 // scaling is folded into the stored operator (A D^-1, column scaling) so the
 // preconditioner costs nothing per iteration and x = D^-1 x~ at the end.
+// With the multigrid preconditioner the stored operator is A (scaling vector
+// of ones) and p, s pass through one V-cycle each (k_amg.hip) before their
+// products, as in Eigen's preconditioned loop (y = M^-1 p, z = M^-1 s).
 //
 // Per iteration: p-update, SpMV + partial r0.v, finish(alpha), s-update,
 // SpMV + partial (t.s, t.t), finish(omega), x/r-update + partial (r.r, r0.r),
@@ -303,11 +306,13 @@ __global__ __launch_bounds__(BLOCK) void k_update_s(int n, const T* __restrict__
     s[i] = r[i] - (T)sc->alpha * v[i];
 }
 
-// t = A~ s; partial sums of t.s, t.t and D^-1 s
+// t = A~ z (z = s, or M^-1 s with the multigrid preconditioner); partial sums
+// of t.s, t.t and D^-1 z
 template <typename TM, typename T>
 __global__ __launch_bounds__(BLOCK) void k_spmv_omega(SellView sv, int bordered,
                                                       const TM* __restrict__ a,
                                                       const T* __restrict__ dinv,
+                                                      const T* __restrict__ z,
                                                       const T* __restrict__ s, T* __restrict__ t,
                                                       double* __restrict__ partial,
                                                       const KScalars* sc) {
@@ -320,13 +325,13 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_omega(SellView sv, int bordered,
     const int nv = sv.nv;
     double vals[3] = {0.0, 0.0, 0.0};
     if (sl < sv.nslices) {
-        T y = sell_dot(sv.sptr, sv.col, a, s, sl, threadIdx.x & 63);
+        T y = sell_dot(sv.sptr, sv.col, a, z, sl, threadIdx.x & 63);
         if (i < nv) {
-            if (bordered) y = y + s[nv];
+            if (bordered) y = y + z[nv];
             t[i] = y;
             vals[0] = (double)y * (double)s[i];
             vals[1] = (double)y * (double)y;
-            if (bordered) vals[2] = (double)dinv[i] * (double)s[i];
+            if (bordered) vals[2] = (double)dinv[i] * (double)z[i];
         }
     }
     block_store<3>(vals, partial, lb, nbk);
@@ -351,10 +356,12 @@ __global__ __launch_bounds__(FIN) void k_finish_omega(int nb, int nv, int border
     sc->omega = (tt > 0.0) ? ts / tt : 0.0;
 }
 
-// x += alpha p + omega s; r = s - omega t; partial sums of r.r and r0.r
+// x += alpha y + omega z (y = p, z = s without the multigrid preconditioner);
+// r = s - omega t; partial sums of r.r and r0.r
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_update_xr(int n, T* __restrict__ x, T* __restrict__ r,
-                                                     const T* __restrict__ p,
+                                                     const T* __restrict__ y,
+                                                     const T* __restrict__ z,
                                                      const T* __restrict__ s,
                                                      const T* __restrict__ t,
                                                      const T* __restrict__ r0,
@@ -365,7 +372,7 @@ __global__ __launch_bounds__(BLOCK) void k_update_xr(int n, T* __restrict__ x, T
     double vals[2] = {0.0, 0.0};
     if (i < n) {
         const T al = (T)sc->alpha, w = (T)sc->omega;
-        x[i] = x[i] + (al * p[i] + w * s[i]);
+        x[i] = x[i] + (al * y[i] + w * z[i]);
         const T ri = s[i] - w * t[i];
         r[i] = ri;
         vals[0] = (double)ri * (double)ri;
@@ -462,34 +469,11 @@ __global__ __launch_bounds__(BLOCK) void k_fill(double* __restrict__ x, double v
     if (i < n) x[i] = v;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_fill32(float* __restrict__ x, float v, int n) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) x[i] = v;
-}
-
 // y += a x
 __global__ __launch_bounds__(BLOCK) void k_axpy(int n, double a, const double* __restrict__ x,
                                                 double* __restrict__ y) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) y[i] = y[i] + a * x[i];
-}
-
-// r32 = r64 / |r64| (scaled copy for the FP32 inner solve)
-__global__ __launch_bounds__(BLOCK) void k_to32(int n, const double* __restrict__ src,
-                                                float* __restrict__ dst, const KScalars* sc) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const double sc_inv = 1.0 / sqrt(sc->red[0]);
-    dst[i] = (float)(src[i] * sc_inv);
-}
-
-// x~ += |r64| * e32
-__global__ __launch_bounds__(BLOCK) void k_correct(int n, double* __restrict__ x,
-                                                   const float* __restrict__ e,
-                                                   const double* __restrict__ scale_src) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    x[i] = x[i] + sqrt(*scale_src) * (double)e[i];
 }
 
 // A~ = A D^-1 (column scaling) for the SELL slots; D^-1 of the border unknown is 1.
@@ -509,14 +493,12 @@ __global__ __launch_bounds__(BLOCK) void k_scale_columns(SellView sv, const doub
 
 __global__ __launch_bounds__(BLOCK) void k_diag_inverse(int nv, int npad,
                                                         const double* __restrict__ diag,
-                                                        double* __restrict__ dinv,
-                                                        float* __restrict__ dinv32) {
+                                                        double* __restrict__ dinv) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= npad) return;
     double d = 1.0;
     if (i < nv && diag[i] != 0.0) d = 1.0 / diag[i];
     dinv[i] = d;
-    dinv32[i] = (float)d;
 }
 
 inline int nblk(int n) { return (n + BLOCK - 1) / BLOCK; }
@@ -568,7 +550,7 @@ PpeSolver::~PpeSolver() {
 void PpeSolver::setup(hipStream_t st, const SellView& view, int dim_, int npad_,
                       const double* gx, const double* gy, const double* gz, const double* lap,
                       const double* normal_soa, const uint8_t* bnd_dev, const uint8_t* dir_dev,
-                      bool bord, double unsteady, double conv, double diff) {
+                      bool bord, double unsteady, double conv, double diff, int precond_) {
     sv = view;
     nv = view.nv;
     dim = dim_;
@@ -586,7 +568,6 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, int dim_, int npad_,
     aval.alloc(total_slots);
     aval32.alloc(total_slots);
     dinv.alloc(vlen);
-    dinv32.alloc(vlen);
     MP_HIP(hipMemsetAsync(raw.p, 0, sizeof(double) * total_slots, st));
     MP_HIP(hipMemsetAsync(aval.p, 0, sizeof(double) * total_slots, st));
     MP_HIP(hipMemsetAsync(aval32.p, 0, sizeof(float) * total_slots, st));
@@ -594,19 +575,42 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, int dim_, int npad_,
     launch_build_operator(st, sv, dim, gx, gy, gz, lap, normal_soa, npad, bnd_dev, dir_dev,
                           unsteady, conv, diff, raw.p, diag.p);
     hipLaunchKernelGGL(k_diag_inverse, dim3(nblk((int)vlen)), dim3(BLOCK), 0, st, nv, (int)vlen,
-                       diag.p, dinv.p, dinv32.p);
+                       diag.p, dinv.p);
     hipLaunchKernelGGL(k_scale_columns<double>, dim3(nblk(nv)), dim3(BLOCK), 0, st, sv, raw.p,
                        dinv.p, aval.p);
     hipLaunchKernelGGL(k_scale_columns<float>, dim3(nblk(nv)), dim3(BLOCK), 0, st, sv, raw.p,
                        dinv.p, aval32.p);
     MP_HIP(hipGetLastError());
+    precond = precond_;
+    amg.reset();
+    diag_inv = dinv.p;
+    scaled32 = aval32.p;
+    if (precond == MP_PRECOND_AMG) {
+        // keep A D^-1 (FP32) and D^-1 for the learned guess, then switch the
+        // Krylov operator to the unscaled A with a scaling vector of ones
+        jac_dinv.alloc(vlen);
+        jac_a32.alloc(total_slots);
+        MP_HIP(hipMemcpyAsync(jac_dinv.p, dinv.p, sizeof(double) * vlen, hipMemcpyDeviceToDevice, st));
+        MP_HIP(hipMemcpyAsync(jac_a32.p, aval32.p, sizeof(float) * total_slots,
+                              hipMemcpyDeviceToDevice, st));
+        launch_fill(st, dinv.p, 1.0, (int)vlen);
+        hipLaunchKernelGGL(k_scale_columns<double>, dim3(nblk(nv)), dim3(BLOCK), 0, st, sv, raw.p,
+                           dinv.p, aval.p);
+        hipLaunchKernelGGL(k_scale_columns<float>, dim3(nblk(nv)), dim3(BLOCK), 0, st, sv, raw.p,
+                           dinv.p, aval32.p);
+        MP_HIP(hipGetLastError());
+        diag_inv = jac_dinv.p;
+        scaled32 = jac_a32.p;
+        amg.reset(new AmgDevice());
+        amg->setup(st, sv, aval.p, aval32.p, mp_amg::Options{});
+        for (DevBuf<double>* b : {&yv, &zv}) {
+            b->alloc(vlen);
+            MP_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * vlen, st));
+        }
+    }
     for (DevBuf<double>* b : {&xt, &r, &r0, &p, &v, &s, &t, &tmp, &e64, &rhs64}) {
         b->alloc(vlen);
         MP_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * vlen, st));
-    }
-    for (DevBuf<float>* b : {&e32, &rhs32, &r32, &r032, &p32, &v32, &s32, &t32}) {
-        b->alloc(vlen);
-        MP_HIP(hipMemsetAsync(b->p, 0, sizeof(float) * vlen, st));
     }
     partial.alloc((size_t)(nblk(n) + 1) * 4);
     sc.alloc(1);
@@ -630,6 +634,13 @@ double PpeSolver::spmv_bytes(int fp32) const {
     return (double)nnz * (vb + 4.0) + (double)nv * (8.0 * (bordered ? 4.0 : 3.0) + 4.0);
 }
 
+// y = M^-1 p with the multigrid V-cycle; without it the direction itself.
+const double* PpeSolver::precondition(hipStream_t st, const double* in, double* out) {
+    if (!amg) return in;
+    amg->apply(st, in, out, bordered, &sc.p->done);
+    return out;
+}
+
 template <typename TM, typename T>
 int PpeSolver::run_bicgstab(hipStream_t st, const TM* a, const T* di, const T* rhs, T* x, T* rr,
                             T* rr0, T* pp, T* vv, T* ss, T* tt, double tol_rel, int rel_to_rhs,
@@ -648,21 +659,23 @@ int PpeSolver::run_bicgstab(hipStream_t st, const TM* a, const T* di, const T* r
     while (true) {
         for (int c = 0; c < chunk; ++c) {
             hipLaunchKernelGGL(k_update_p<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, pp, vv, sc.p);
+            const T* yin = precondition(st, pp, yv.p);
             spmv_timer.begin(st);
             hipLaunchKernelGGL((k_spmv_alpha<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a,
-                               di, pp, vv, rr0, partial.p, sc.p);
+                               di, yin, vv, rr0, partial.p, sc.p);
             spmv_timer.end(st);
             hipLaunchKernelGGL(k_finish_alpha<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered,
                                vv, rr0, partial.p, sc.p);
             hipLaunchKernelGGL(k_update_s<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, vv, ss, sc.p);
+            const T* zin = precondition(st, ss, zv.p);
             spmv_timer.begin(st);
             hipLaunchKernelGGL((k_spmv_omega<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a,
-                               di, ss, tt, partial.p, sc.p);
+                               di, zin, ss, tt, partial.p, sc.p);
             spmv_timer.end(st);
             hipLaunchKernelGGL(k_finish_omega<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered,
                                tt, ss, partial.p, sc.p);
-            hipLaunchKernelGGL(k_update_xr<T>, dim3(g_n), dim3(BLOCK), 0, st, n, x, rr, pp, ss,
-                               tt, rr0, partial.p, sc.p);
+            hipLaunchKernelGGL(k_update_xr<T>, dim3(g_n), dim3(BLOCK), 0, st, n, x, rr, yin, zin,
+                               ss, tt, rr0, partial.p, sc.p);
             hipLaunchKernelGGL(k_finish_xr, dim3(1), dim3(FIN), 0, st, g_n, partial.p, sc.p);
         }
         MP_HIP(hipGetLastError());

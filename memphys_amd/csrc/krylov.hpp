@@ -1,5 +1,8 @@
 // Pressure-Poisson solver state (part c).  Host orchestration in k_krylov.hip.
 #pragma once
+#include <memory>
+
+#include "amg.hpp"
 #include "mp_internal.hpp"
 
 namespace mp {
@@ -22,15 +25,23 @@ struct KernelTimer {
 };
 
 // BiCGSTAB (as Eigen 3.3.7's BiCGSTAB.h) on the right-Jacobi-scaled operator
-// A D^-1, bordered with a ones row/column in the full-Neumann case.
+// A D^-1, bordered with a ones row/column in the full-Neumann case.  With the
+// multigrid preconditioner the operator is A itself (the scaling vector is
+// all ones) and every p and s is preconditioned by one V-cycle.
 struct PpeSolver {
     int nv = 0, npad = 0, n = 0, bordered = 0, dim = 0;
     long nnz = 0;  // real stencil entries of the operator
     SellView sv{};
     DevBuf<double> aval, dinv;
-    DevBuf<float> aval32, dinv32;
+    DevBuf<float> aval32;
     DevBuf<double> xt, r, r0, p, v, s, t, tmp, e64, rhs64;
-    DevBuf<float> e32, rhs32, r32, r032, p32, v32, s32, t32;
+    DevBuf<double> yv, zv;            // preconditioned directions (multigrid)
+    DevBuf<double> jac_dinv;          // D^-1 when the operator above is unscaled
+    DevBuf<float> jac_a32;            // A D^-1 in FP32 when the operator above is unscaled
+    const double* diag_inv = nullptr; // D^-1 (the learned guess's features)
+    const float* scaled32 = nullptr;  // A D^-1 in FP32 (the learned guess's aggregation)
+    int precond = 0;                  // enum mp_precond
+    std::unique_ptr<AmgDevice> amg;
     DevBuf<double> partial;
     DevBuf<KScalars> sc;
     KScalars* host_sc = nullptr;  // pinned, 2 slots
@@ -47,7 +58,7 @@ struct PpeSolver {
     void setup(hipStream_t s, const SellView& view, int dim, int npad, const double* gx,
                const double* gy, const double* gz, const double* lap, const double* normal_soa,
                const uint8_t* bnd_dev, const uint8_t* dir_dev, bool bordered, double unsteady,
-               double conv, double diff);
+               double conv, double diff, int precond = 0);
     // b_dev, x_dev: device order, length n (guess in x_dev, solution out).
     SolveStats solve(hipStream_t s, const double* b_dev, double* x_dev);
     double spmv_bytes(int fp32) const;
@@ -60,6 +71,7 @@ struct PpeSolver {
                      T* rr0, T* pp, T* vv, T* ss, T* tt, double tol_rel, int rel_to_rhs, int maxit,
                      int* restarts, int* done_out);
     void residual64(hipStream_t s, const double* b_dev, double* out_rr, double* out_bb);
+    const double* precondition(hipStream_t s, const double* in, double* out);
 };
 
 }  // namespace mp

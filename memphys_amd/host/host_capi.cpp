@@ -6,6 +6,7 @@
 #include <exception>
 #include <vector>
 
+#include "../csrc/amg_build.hpp"
 #include "cloud_setup.hpp"
 
 extern "C" {
@@ -98,6 +99,61 @@ int mph_poly_exponents(int dim, int poly_deg, int* out) {
         std::vector<int> e = memphys::polynomial_exponents(dim, poly_deg);
         std::memcpy(out, e.data(), sizeof(int) * e.size());
         return (int)(e.size() / dim);
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// First coarsening step of the multigrid set-up the HIP library runs
+// (amg_build.hpp), for CPU checks: aggregates (n entries), the smoothed
+// prolongator P (n x n_agg) and the Galerkin operator R A P (n_agg square) as
+// CSR into caller buffers of the given capacities.  Returns 0, -1 on error, or
+// 1 when a capacity is too small (p_ptr[n] / ac_ptr[n_agg] then hold the sizes
+// needed when they fit).
+int mph_amg_first_level(int n, const int* ptr, const int* col, const double* val, int k,
+                        double omega_p, int* agg, int* n_agg, int* p_ptr, int* p_col,
+                        double* p_val, long p_cap, int* ac_ptr, int* ac_col, double* ac_val,
+                        long ac_cap) {
+    try {
+        mp_amg::Csr A;
+        A.nrows = A.ncols = n;
+        A.ptr.assign(ptr, ptr + n + 1);
+        A.col.assign(col, col + ptr[n]);
+        A.val.assign(val, val + ptr[n]);
+        std::vector<int> a;
+        int na = 0;
+        mp_amg::aggregate(A, k, a, na);
+        mp_amg::Csr P = mp_amg::smoothed_prolongator(A, a, na, omega_p);
+        mp_amg::Csr R = mp_amg::transpose(P);
+        mp_amg::Csr Ac = mp_amg::multiply(R, mp_amg::multiply(A, P, 0), 0);
+        std::memcpy(agg, a.data(), sizeof(int) * n);
+        *n_agg = na;
+        std::memcpy(p_ptr, P.ptr.data(), sizeof(int) * (n + 1));
+        std::memcpy(ac_ptr, Ac.ptr.data(), sizeof(int) * (na + 1));
+        if (P.nnz() > p_cap || Ac.nnz() > ac_cap) return 1;
+        std::memcpy(p_col, P.col.data(), sizeof(int) * P.nnz());
+        std::memcpy(p_val, P.val.data(), sizeof(double) * P.nnz());
+        std::memcpy(ac_col, Ac.col.data(), sizeof(int) * Ac.nnz());
+        std::memcpy(ac_val, Ac.val.data(), sizeof(double) * Ac.nnz());
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// Top-left n x n block of the inverse of [[A, 1], [1^T, 0]] (the coarsest
+// multigrid solve), row-major into G.
+int mph_amg_bordered_inverse(int n, const int* ptr, const int* col, const double* val,
+                             double* G) {
+    try {
+        mp_amg::Csr A;
+        A.nrows = A.ncols = n;
+        A.ptr.assign(ptr, ptr + n + 1);
+        A.col.assign(col, col + ptr[n]);
+        A.val.assign(val, val + ptr[n]);
+        std::vector<double> g = mp_amg::bordered_inverse(A);
+        std::memcpy(G, g.data(), sizeof(double) * g.size());
+        return 0;
     } catch (const std::exception&) {
         return -1;
     }

@@ -1,0 +1,124 @@
+"""CPU checks of the multigrid set-up (host C++ builder) on the 3D cube's
+pressure operator.
+
+The builder's output is pinned by what the method fixes independently of
+its code: the aggregates partition the nodes; the prolongator equals
+(I - w D^-1 A) P_tent formed here with scipy from the aggregates; the coarse
+operator equals P^T A P formed with scipy; the aggregation follows the rule
+it states (restated below on the same strength graph); the coarsest-level
+solve is the inverse of the ones-bordered matrix (numpy LAPACK inverse).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import lid_bc
+
+
+@pytest.fixture(scope="module")
+def ppe_block(cube3d_small):
+    import oracle as O
+    c = cube3d_small
+    w = O.stencil_weights(c, 3, 3, threads=8)[:4]
+    u_dir, p_dir, _ = lid_bc(c, 2)
+    fs = O.FractionalStep(c, w, u_dir, u_dir, u_dir, p_dir, 1.0, 0.0025, 5e-4, 2)
+    rp, ci, v = fs.ppe_csr()
+    A = sp.csr_matrix((v, ci, rp), shape=(fs.n_sys, fs.n_sys))[:c.nv, :c.nv].tocsr()
+    A.sort_indices()
+    return A
+
+
+def _aggregate_restated(A, k):
+    n = A.shape[0]
+    rows, cols = [], []
+    for i in range(n):
+        idx = A.indices[A.indptr[i]:A.indptr[i + 1]]
+        val = np.abs(A.data[A.indptr[i]:A.indptr[i + 1]])
+        m = idx != i
+        idx, val = idx[m], val[m]
+        order = np.lexsort((idx, -val))[:k]  # largest |a|, ties to the smaller column
+        rows += [i] * len(order)
+        cols += list(idx[order])
+    S = sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n))
+    S = ((S + S.T) > 0).tocsr()
+    S.sort_indices()
+    agg = -np.ones(n, dtype=np.int64)
+    na = 0
+    for i in range(n):
+        if agg[i] >= 0:
+            continue
+        nb = S.indices[S.indptr[i]:S.indptr[i + 1]]
+        if np.all(agg[nb] < 0):
+            agg[i] = na
+            agg[nb] = na
+            na += 1
+    for i in range(n):
+        if agg[i] >= 0:
+            continue
+        nb = S.indices[S.indptr[i]:S.indptr[i + 1]]
+        a = agg[nb][agg[nb] >= 0]
+        if len(a):
+            agg[i] = a[0]
+        else:
+            agg[i] = na
+            na += 1
+    return agg, na
+
+
+def test_aggregates_partition_and_follow_rule(ppe_block):
+    import memphys_amd as M
+    A = ppe_block
+    agg, P, Ac = M.amg_first_level(A, k=6)
+    n, na = A.shape[0], Ac.shape[0]
+    assert agg.min() == 0 and agg.max() == na - 1
+    assert np.all(np.bincount(agg, minlength=na) > 0)
+    assert na < n / 4, "coarsening ratio"
+    ref, ref_na = _aggregate_restated(A, 6)
+    assert ref_na == na
+    assert np.array_equal(ref, agg)
+
+
+def test_prolongator_and_galerkin_operator(ppe_block):
+    import memphys_amd as M
+    A = ppe_block
+    omega = 0.66
+    agg, P, Ac = M.amg_first_level(A, k=6, omega_p=omega)
+    n, na = A.shape[0], Ac.shape[0]
+    Pt = sp.csr_matrix((np.ones(n), (np.arange(n), agg)), shape=(n, na))
+    P_ref = (Pt - omega * sp.diags(1.0 / A.diagonal()) @ (A @ Pt)).tocsr()
+    dP = abs(P - P_ref).max()
+    assert dP <= 1e-13 * abs(P_ref).max()
+    # a planted transposition (A^T in place of A) must fail the same check
+    P_bad = (Pt - omega * sp.diags(1.0 / A.diagonal()) @ (A.T @ Pt)).tocsr()
+    assert abs(P - P_bad).max() > 1e-6 * abs(P_ref).max()
+    Ac_ref = (P_ref.T @ A @ P_ref).tocsr()
+    dA = abs(Ac - Ac_ref).max()
+    assert dA <= 1e-12 * abs(Ac_ref).max()
+    # the constant mode survives: row sums of the Neumann Laplacian are ~0 and
+    # P reproduces constants, so the coarse operator annihilates constants too
+    ones_c = np.ones(na)
+    assert np.abs(P @ ones_c - 1.0).max() <= 1e-8
+    assert np.abs(Ac @ ones_c).max() <= 1e-6 * abs(Ac).max()
+
+
+def test_bordered_inverse(ppe_block):
+    import memphys_amd as M
+    A = ppe_block
+    _, _, Ac = M.amg_first_level(A, k=6)
+    _, _, A2 = M.amg_first_level(Ac, k=6)
+    n = A2.shape[0]
+    assert 10 <= n <= 700
+    G = M.amg_bordered_inverse(A2)
+    B = np.zeros((n + 1, n + 1))
+    B[:n, :n] = A2.toarray()
+    B[:n, n] = 1.0
+    B[n, :n] = 1.0
+    G_ref = np.linalg.inv(B)[:n, :n]
+    assert np.abs(G - G_ref).max() <= 1e-9 * np.abs(G_ref).max()
+    # x = G b solves A x + lambda 1 = b, sum(x) = 0: the residual is constant
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(n)
+    x = G @ b
+    assert abs(x.sum()) <= 1e-9 * np.abs(x).max()
+    r = A2 @ x - b
+    assert np.abs(r - r.mean()).max() <= 1e-8 * np.linalg.norm(b)

@@ -114,7 +114,7 @@ def test_ppe_solve_mixed_matches_superlu(ctx, cube3d_small):
     _ppe_case(ctx, cube3d_small, M.MP_PREC_MIXED)
 
 
-def _fs_parity(ctx, cloud, steps, lid_axis, rho, mu, dt, precision):
+def _fs_parity(ctx, cloud, steps, lid_axis, rho, mu, dt, precision, precond=0):
     import memphys_amd as M
     import oracle as O
     ctx.upload(cloud)
@@ -122,7 +122,7 @@ def _fs_parity(ctx, cloud, steps, lid_axis, rho, mu, dt, precision):
     w = (gx, gy, gz, lap)
     u_dir, p_dir, u0 = lid_bc(cloud, lid_axis)
     ctx.fs_create(rho, mu, dt, 2, 1e-12, 20000, u_dir, u_dir, u_dir if cloud.dim == 3 else None,
-                  p_dir, precision=precision)
+                  p_dir, precision=precision, precond=precond)
     nv = cloud.nv
     ctx.set_field(M.MP_U, u0)
     ctx.set_field(M.MP_V, np.zeros(nv))
@@ -142,13 +142,14 @@ def _fs_parity(ctx, cloud, steps, lid_axis, rho, mu, dt, precision):
             rel_err(ctx.get_field(M.MP_P), p)]
     if cloud.dim == 3:
         errs.append(rel_err(ctx.get_field(M.MP_W), ww))
-    _print(f"FS {cloud.dim}D steps {steps} prec {precision}: errs {errs} iters {st.iterations}"
-           f" rel_res {st.rel_res}")
+    _print(f"FS {cloud.dim}D steps {steps} prec {precision} precond {precond}: errs {errs}"
+           f" iters {st.iterations} total {st.total_iterations} rel_res {st.rel_res}")
     assert max(errs) <= 1e-8
     div = ctx.divergence()
     div_o, _ = O.divergence(cloud, gx, gy, gz, ctx.get_field(M.MP_U), ctx.get_field(M.MP_V),
                             ctx.get_field(M.MP_W) if cloud.dim == 3 else None)
     assert np.allclose(div, div_o, rtol=1e-12, atol=1e-14)
+    return st.total_iterations
 
 
 def test_fs_2d_cavity_matches_oracle(ctx, cavity2d):
@@ -166,6 +167,27 @@ def test_fs_3d_cube_matches_oracle(ctx, cube3d_small):
 def test_fs_3d_cube_mixed_matches_oracle(ctx, cube3d_small):
     import memphys_amd as M
     _fs_parity(ctx, cube3d_small, 8, 2, 1.0, 0.0025, 5e-4, M.MP_PREC_MIXED)
+
+
+def test_fs_3d_cube_multigrid_matches_oracle(ctx, cube3d_small):
+    """Multigrid-preconditioned BiCGSTAB (FP64 and FP32-inner) reaches the
+    same fields as the oracle, in far fewer iterations than Jacobi."""
+    import memphys_amd as M
+    it_jac = _fs_parity(ctx, cube3d_small, 8, 2, 1.0, 0.0025, 5e-4, M.MP_PREC_FP64)
+    it_mg = _fs_parity(ctx, cube3d_small, 8, 2, 1.0, 0.0025, 5e-4, M.MP_PREC_FP64,
+                       M.MP_PRECOND_AMG)
+    info = ctx.precond_info()
+    _print("hierarchy", info)
+    assert info["levels"] >= 2 and info["rows"][0] == cube3d_small.nv
+    it_mx = _fs_parity(ctx, cube3d_small, 8, 2, 1.0, 0.0025, 5e-4, M.MP_PREC_MIXED,
+                       M.MP_PRECOND_AMG)
+    _print("iterations jacobi", it_jac, "multigrid fp64", it_mg, "multigrid mixed", it_mx)
+    assert it_mg <= 0.5 * it_jac
+
+
+def test_fs_2d_cavity_multigrid_matches_oracle(ctx, cavity2d):
+    import memphys_amd as M
+    _fs_parity(ctx, cavity2d, 10, 1, 1.0, 0.01, 2e-4, M.MP_PREC_MIXED, M.MP_PRECOND_AMG)
 
 
 def test_condition_estimate_matches_oracle(ctx, cube3d_small):
