@@ -47,6 +47,19 @@ def lid_bc(cloud):
     return u_dir, p_dir, u0
 
 
+def measured_traffic(n: int, kernel_index: int = 0):
+    """HBM bytes per launch of the PPE SpMV measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs on the 100^3 workload (index 0: FP32 operator, 1: FP64 operator),
+    read from the committed summary; None for other workloads."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "ppe_spmv_traffic_cavity3d_1M.md")
+    if n != 100 or not os.path.exists(path):
+        return None
+    vals = [float(line.split("=")[1]) for line in open(path)
+            if line.startswith("traffic_bytes_per_launch")]
+    return vals[kernel_index] if len(vals) > kernel_index else None
+
+
 def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, precond=0):
     u_dir, p_dir, u0 = lid_bc(cloud)
     t = time.perf_counter()
@@ -134,6 +147,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--train-steps", type=int, default=2000,
                     help="solver steps of guess training (10 episodes)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device three times (a plain copy kernel "
+                         "of known size, the scale for HBM counter runs)")
     ap.add_argument("--forms", default="fp64,mixed,headline",
                     help="comma list of forms to time besides fp64: mixed (FP32-inner, "
                          "Jacobi), headline (FP32-inner, multigrid, learned guess)")
@@ -143,6 +159,17 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import memphys_amd as M
+
+    if args.copy_calib > 0:
+        import torch
+        nbytes = args.copy_calib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):  # elementwise kernel: reads and writes nbytes once each
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {nbytes} bytes read + written (torch.neg)")
+        del src, dst
 
     prm = dict(rho=1.0, mu=0.0025, dt=5e-4, tol=args.tol, n_iter=20000, inner_tol=1e-5,
                check_every=8)
@@ -236,7 +263,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.n, 0),
             "kernel": "k_spmv_alpha/k_spmv_omega<float,double> (PPE SpMV, FP32 operator)",
             "bytes_per_launch": bytes_l, "mean_ms": ms, "launches": n_l,
         },
@@ -253,7 +280,8 @@ def main():
         "max_ppe_rel_res": {"fp64": max(fp64["rel_res"]), "headline": max(head["rel_res"])},
         "headline_vs_fp64_rel_diff": {"velocity": vdiff, "pressure": pdiff},
         "fp64_spmv_roofline": {"achieved": ach64, "frac": ach64 / HBM_PEAK_GBS,
-                               "bytes_per_launch": b64, "mean_ms": ms64},
+                               "bytes_per_launch": b64, "mean_ms": ms64,
+                               "traffic": measured_traffic(args.n, 1)},
         "weights_kernel_ms": wk[1],
         "weights_fp64_vs_fp32_refined": weights_check,
         "divergence_fp64": fp64["divergence"],
