@@ -156,6 +156,12 @@ int mp_divergence(mp_ctx* ctx, double* out);
 int mp_precond_info(mp_ctx* ctx, int max_levels, int* levels, int* rows, long* nnz,
                     double* setup_seconds);
 
+/* Krylov iterations between two convergence polls are replayed from a
+ * hipGraph captured on first use (enable = 1, the default) or launched one by
+ * one (enable = 0).  Results are identical; kernel timing (below) always
+ * launches one by one. */
+int mp_graph_mode(mp_ctx* ctx, int enable);
+
 /* Per-kernel timing with HIP events on the library's stream.
  * which: 0 = PPE SpMV, 1 = predictor, 2 = pressure RHS, 3 = corrector,
  *        4 = stencil weights.  Reports launches and mean milliseconds. */

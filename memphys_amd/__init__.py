@@ -34,6 +34,7 @@ HIP_SYMBOLS = (
     "mp_fields_get", "mp_fs_step", "mp_stats_get", "mp_guess_mode", "mp_divergence",
     "mp_kernel_timing", "mp_kernel_timing_get", "mp_guess_param_count", "mp_guess_train",
     "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
+    "mp_graph_mode",
 )
 
 
@@ -104,6 +105,7 @@ def hip_lib() -> C.CDLL:
                                                                             flags="C_CONTIGUOUS")]
         lib.mp_guess_last_loss.argtypes = [vp, C.POINTER(C.c_double)]
         lib.mp_guess_eval.argtypes = [vp, _dp, _dp, _dp, vp, _dp, vp, C.POINTER(C.c_double)]
+        lib.mp_graph_mode.argtypes = [vp, C.c_int]
         lib.mp_precond_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), vp, vp,
                                         C.POINTER(C.c_double)]
         for name in HIP_SYMBOLS:
@@ -318,6 +320,10 @@ class Context:
         out = np.zeros(6)
         self._check(self.lib.mp_divergence(self.h, out), "mp_divergence")
         return out
+
+    def graph_mode(self, enable: bool):
+        """Replay Krylov chunks from hipGraphs (default) or launch them one by one."""
+        self._check(self.lib.mp_graph_mode(self.h, 1 if enable else 0), "mp_graph_mode")
 
     def precond_info(self) -> dict:
         """Multigrid hierarchy: rows and stored entries per level, set-up seconds."""

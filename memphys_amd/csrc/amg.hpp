@@ -49,7 +49,12 @@ struct AmgDevice {
     // out = M^-1 in for the bordered system of size nv + bordered: the V-cycle
     // on the node block, identity on the border unknown.  Every kernel returns
     // at once when done_flag (may be null) is non-zero.
-    void apply(hipStream_t s, const double* in, double* out, int bordered, const int* done_flag);
+    // prescaled: the level-0 smoother input w D^-1 in (FP32) is already in
+    // prescaled_input() (written by the Krylov update kernels).
+    void apply(hipStream_t s, const double* in, double* out, int bordered, const int* done_flag,
+               bool prescaled = false);
+    float* prescaled_input() { return lv.size() > 1 ? lv[0].x.p : nullptr; }
+    const float* level0_dinv() const { return lv.empty() ? nullptr : lv[0].dinv.p; }
 };
 
 }  // namespace mp

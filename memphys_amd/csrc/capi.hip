@@ -64,6 +64,7 @@ struct mp_ctx {
     int hist = 0;
     std::unique_ptr<GuessNet> net;
     bool train = false;
+    bool use_graphs = true;
     double last_loss = 0.0;
     float lr = 1e-3f;
 
@@ -511,6 +512,7 @@ int mp_solver_init(mp_ctx* c, const uint8_t* dirichlet, double unsteady, double 
         c->solver->maxit = n_iter;
         c->solver->precision = precision;
         c->solver->spmv_timer.enabled = c->timers[0].enabled;
+        c->solver->use_graphs = c->use_graphs;
         return MP_OK;
     });
 }
@@ -597,6 +599,7 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
         c->ppe->inner_tol = (prm->inner_tolerance > 0) ? prm->inner_tolerance : 1e-3;
         c->ppe->check_every = (prm->check_every > 0) ? prm->check_every : 8;
         c->ppe->spmv_timer.enabled = c->timers[0].enabled;
+        c->ppe->use_graphs = c->use_graphs;
         for (int k = 0; k < 2; ++k)
             for (int q = 0; q < 3; ++q) alloc_vec(c, c->U[k][q]);
         alloc_vec(c, c->P);
@@ -901,6 +904,15 @@ int mp_precond_info(mp_ctx* c, int max_levels, int* levels, int* rows, long* nnz
             if (nnz) nnz[l] = a->nnz[l];
         }
         if (setup_seconds) *setup_seconds = a ? a->setup_seconds : 0.0;
+        return MP_OK;
+    });
+}
+
+int mp_graph_mode(mp_ctx* c, int enable) {
+    return guarded(c, "mp_graph_mode", [&]() -> int {
+        c->use_graphs = enable != 0;
+        for (PpeSolver* s : {c->solver.get(), c->ppe.get()})
+            if (s) s->use_graphs = c->use_graphs;
         return MP_OK;
     });
 }

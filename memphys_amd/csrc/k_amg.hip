@@ -19,7 +19,7 @@ namespace mp {
 
 namespace {
 
-enum { OP_PRE = 0, OP_SPMV = 1, OP_POST = 2 };
+enum { OP_PRE = 0, OP_SPMV = 1, OP_POST = 2, OP_PRES = 3 };
 
 __host__ __device__ inline int nblk_rows(int nslices) { return (nslices * SLICE + BLOCK - 1) / BLOCK; }
 
@@ -46,6 +46,7 @@ __device__ __forceinline__ float gathered(int c, const TB* __restrict__ b, const
 // One multigrid product with its epilogue, in either layout (G == 0: SELL-64,
 // else CSR with G lanes per row):
 //   OP_PRE : xout = D^-1 b,  res = b - A (D^-1 b)
+//   OP_PRES: res = b - A xin, with xin = D^-1 b already formed
 //   OP_SPMV: out = base + M xin (base may be null and may alias out)
 //   OP_POST: out = xin + D^-1 (b - A xin)
 // Level 0 also copies the border unknown (border_in -> border_out).
@@ -114,6 +115,8 @@ __global__ __launch_bounds__(BLOCK) void k_amg_op(MatView m, const float* __rest
         const float bi = (float)b[i];
         xout[i] = dinv[i] * bi;
         res[i] = bi - acc;
+    } else if constexpr (OP == OP_PRES) {
+        res[i] = (float)b[i] - acc;
     } else if constexpr (OP == OP_SPMV) {
         out[i] = (TO)(base ? base[i] + acc : acc);
     } else {
@@ -394,7 +397,7 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
 }
 
 void AmgDevice::apply(hipStream_t st, const double* in, double* out, int bordered,
-                      const int* done) {
+                      const int* done, bool prescaled) {
     const int L = (int)lv.size();
     const int nv = lv[0].n;
     const double* bin = bordered ? in + nv : nullptr;
@@ -406,8 +409,12 @@ void AmgDevice::apply(hipStream_t st, const double* in, double* out, int bordere
         return;
     }
     // downward: level 0 from the FP64 input
-    launch_op<OP_PRE, double, float>(st, lv[0].A, lv[0].dinv.p, in, nullptr, nullptr, lv[0].x.p,
-                                     lv[0].res.p, (float*)nullptr, bin, bout, done);
+    if (prescaled)
+        launch_op<OP_PRES, double, float>(st, lv[0].A, nullptr, in, lv[0].x.p, nullptr, nullptr,
+                                          lv[0].res.p, (float*)nullptr, bin, bout, done);
+    else
+        launch_op<OP_PRE, double, float>(st, lv[0].A, lv[0].dinv.p, in, nullptr, nullptr,
+                                         lv[0].x.p, lv[0].res.p, (float*)nullptr, bin, bout, done);
     for (int l = 0; l + 1 < L; ++l) {
         AmgLevelDev& f = lv[l];
         AmgLevelDev& c = lv[l + 1];

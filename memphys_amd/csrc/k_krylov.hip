@@ -240,12 +240,16 @@ __global__ __launch_bounds__(FIN) void k_finish_init(int nb, int nv, int bordere
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_update_p(int n, const T* __restrict__ r,
                                                     T* __restrict__ p, const T* __restrict__ v,
-                                                    const KScalars* sc) {
+                                                    const KScalars* sc, int nv,
+                                                    const float* __restrict__ d0,
+                                                    float* __restrict__ xs) {
     if (sc->done) return;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const T beta = (T)sc->beta, w = (T)sc->omega;
-    p[i] = r[i] + beta * (p[i] - w * v[i]);
+    const T pi = r[i] + beta * (p[i] - w * v[i]);
+    p[i] = pi;
+    if (xs && i < nv) xs[i] = d0[i] * (float)pi;  // smoother input of the V-cycle
 }
 
 // v = A~ p; partial sums of r0.v and of D^-1 p (border row)
@@ -299,11 +303,15 @@ __global__ __launch_bounds__(FIN) void k_finish_alpha(int nb, int nv, int border
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_update_s(int n, const T* __restrict__ r,
                                                     const T* __restrict__ v, T* __restrict__ s,
-                                                    const KScalars* sc) {
+                                                    const KScalars* sc, int nv,
+                                                    const float* __restrict__ d0,
+                                                    float* __restrict__ xs) {
     if (sc->done) return;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    s[i] = r[i] - (T)sc->alpha * v[i];
+    const T si = r[i] - (T)sc->alpha * v[i];
+    s[i] = si;
+    if (xs && i < nv) xs[i] = d0[i] * (float)si;  // smoother input of the V-cycle
 }
 
 // t = A~ z (z = s, or M^-1 s with the multigrid preconditioner); partial sums
@@ -541,6 +549,7 @@ void KernelTimer::release() {
 }
 
 PpeSolver::~PpeSolver() {
+    release_graphs();
     if (host_sc) (void)hipHostFree(host_sc);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
@@ -551,6 +560,7 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, int dim_, int npad_,
                       const double* gx, const double* gy, const double* gz, const double* lap,
                       const double* normal_soa, const uint8_t* bnd_dev, const uint8_t* dir_dev,
                       bool bord, double unsteady, double conv, double diff, int precond_) {
+    release_graphs();
     sv = view;
     nv = view.nv;
     dim = dim_;
@@ -635,10 +645,86 @@ double PpeSolver::spmv_bytes(int fp32) const {
 }
 
 // y = M^-1 p with the multigrid V-cycle; without it the direction itself.
+// The preceding update kernel has written the V-cycle's scaled FP32 input.
 const double* PpeSolver::precondition(hipStream_t st, const double* in, double* out) {
     if (!amg) return in;
-    amg->apply(st, in, out, bordered, &sc.p->done);
+    amg->apply(st, in, out, bordered, &sc.p->done, amg->prescaled_input() != nullptr);
     return out;
+}
+
+// The chunk of iterations between two convergence polls, launched on the
+// stream (captured into a graph by run_bicgstab).
+template <typename TM, typename T>
+void PpeSolver::launch_chunk(hipStream_t st, int chunk, const TM* a, const T* di, T* x, T* rr,
+                             T* rr0, T* pp, T* vv, T* ss, T* tt) {
+    const int g_rows = nblk(nv), g_n = nblk(n);
+    float* xs = amg ? amg->prescaled_input() : nullptr;
+    const float* d0 = amg ? amg->level0_dinv() : nullptr;
+    for (int c = 0; c < chunk; ++c) {
+        hipLaunchKernelGGL(k_update_p<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, pp, vv, sc.p, nv,
+                           d0, xs);
+        const T* yin = precondition(st, pp, yv.p);
+        spmv_timer.begin(st);
+        hipLaunchKernelGGL((k_spmv_alpha<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv,
+                           bordered, a, di, yin, vv, rr0, partial.p, sc.p);
+        spmv_timer.end(st);
+        hipLaunchKernelGGL(k_finish_alpha<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered, vv,
+                           rr0, partial.p, sc.p);
+        hipLaunchKernelGGL(k_update_s<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, vv, ss, sc.p, nv,
+                           d0, xs);
+        const T* zin = precondition(st, ss, zv.p);
+        spmv_timer.begin(st);
+        hipLaunchKernelGGL((k_spmv_omega<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv,
+                           bordered, a, di, zin, ss, tt, partial.p, sc.p);
+        spmv_timer.end(st);
+        hipLaunchKernelGGL(k_finish_omega<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered, tt,
+                           ss, partial.p, sc.p);
+        hipLaunchKernelGGL(k_update_xr<T>, dim3(g_n), dim3(BLOCK), 0, st, n, x, rr, yin, zin, ss,
+                           tt, rr0, partial.p, sc.p);
+        hipLaunchKernelGGL(k_finish_xr, dim3(1), dim3(FIN), 0, st, g_n, partial.p, sc.p);
+    }
+    MP_HIP(hipGetLastError());
+}
+
+// Replays the chunk from a cached graph (captured on first use for this set of
+// operands); per-launch event timing needs the plain launches instead.
+template <typename TM, typename T>
+void PpeSolver::run_chunk(hipStream_t st, int chunk, const TM* a, const T* di, T* x, T* rr,
+                          T* rr0, T* pp, T* vv, T* ss, T* tt) {
+    if (!use_graphs || spmv_timer.enabled) {
+        launch_chunk<TM, T>(st, chunk, a, di, x, rr, rr0, pp, vv, ss, tt);
+        return;
+    }
+    const void* key[9] = {a, di, x, rr, rr0, pp, vv, ss, tt};
+    for (ChunkGraph& g : graphs)
+        if (g.chunk == chunk && std::memcmp(g.key, key, sizeof(key)) == 0) {
+            MP_HIP(hipGraphLaunch(g.exec, st));
+            return;
+        }
+    hipGraph_t graph = nullptr;
+    MP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        launch_chunk<TM, T>(st, chunk, a, di, x, rr, rr0, pp, vv, ss, tt);
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    MP_HIP(hipStreamEndCapture(st, &graph));
+    ChunkGraph g{};
+    std::memcpy(g.key, key, sizeof(key));
+    g.chunk = chunk;
+    const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MP_HIP(e);
+    graphs.push_back(g);
+    MP_HIP(hipGraphLaunch(g.exec, st));
+}
+
+void PpeSolver::release_graphs() {
+    for (ChunkGraph& g : graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
 }
 
 template <typename TM, typename T>
@@ -657,27 +743,7 @@ int PpeSolver::run_bicgstab(hipStream_t st, const TM* a, const T* di, const T* r
     bool have_prev = false;
     *restarts = 0;
     while (true) {
-        for (int c = 0; c < chunk; ++c) {
-            hipLaunchKernelGGL(k_update_p<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, pp, vv, sc.p);
-            const T* yin = precondition(st, pp, yv.p);
-            spmv_timer.begin(st);
-            hipLaunchKernelGGL((k_spmv_alpha<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a,
-                               di, yin, vv, rr0, partial.p, sc.p);
-            spmv_timer.end(st);
-            hipLaunchKernelGGL(k_finish_alpha<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered,
-                               vv, rr0, partial.p, sc.p);
-            hipLaunchKernelGGL(k_update_s<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, vv, ss, sc.p);
-            const T* zin = precondition(st, ss, zv.p);
-            spmv_timer.begin(st);
-            hipLaunchKernelGGL((k_spmv_omega<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a,
-                               di, zin, ss, tt, partial.p, sc.p);
-            spmv_timer.end(st);
-            hipLaunchKernelGGL(k_finish_omega<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered,
-                               tt, ss, partial.p, sc.p);
-            hipLaunchKernelGGL(k_update_xr<T>, dim3(g_n), dim3(BLOCK), 0, st, n, x, rr, yin, zin,
-                               ss, tt, rr0, partial.p, sc.p);
-            hipLaunchKernelGGL(k_finish_xr, dim3(1), dim3(FIN), 0, st, g_n, partial.p, sc.p);
-        }
+        run_chunk<TM, T>(st, chunk, a, di, x, rr, rr0, pp, vv, ss, tt);
         MP_HIP(hipGetLastError());
         MP_HIP(hipMemcpyAsync(&host_sc[slot], sc.p, sizeof(KScalars), hipMemcpyDeviceToHost, st));
         MP_HIP(hipEventRecord(ev[slot], st));

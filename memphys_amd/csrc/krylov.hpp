@@ -24,6 +24,14 @@ struct KernelTimer {
     void release();
 };
 
+// An instantiated graph of one chunk of BiCGSTAB iterations for one set of
+// operand pointers.
+struct ChunkGraph {
+    const void* key[9];
+    int chunk;
+    hipGraphExec_t exec;
+};
+
 // BiCGSTAB (as Eigen 3.3.7's BiCGSTAB.h) on the right-Jacobi-scaled operator
 // A D^-1, bordered with a ones row/column in the full-Neumann case.  With the
 // multigrid preconditioner the operator is A itself (the scaling vector is
@@ -51,6 +59,8 @@ struct PpeSolver {
     int maxit = 1000, check_every = 8, inner_maxit = 1000;
     long fp64_fallbacks = 0;
     KernelTimer spmv_timer;
+    bool use_graphs = true;           // replay each chunk of iterations from a hipGraph
+    std::vector<ChunkGraph> graphs;
     long total_iterations = 0, total_solves = 0;
 
     ~PpeSolver();
@@ -72,6 +82,13 @@ struct PpeSolver {
                      int* restarts, int* done_out);
     void residual64(hipStream_t s, const double* b_dev, double* out_rr, double* out_bb);
     const double* precondition(hipStream_t s, const double* in, double* out);
+    template <typename TM, typename T>
+    void launch_chunk(hipStream_t s, int chunk, const TM* a, const T* di, T* x, T* rr, T* rr0,
+                      T* pp, T* vv, T* ss, T* tt);
+    template <typename TM, typename T>
+    void run_chunk(hipStream_t s, int chunk, const TM* a, const T* di, T* x, T* rr, T* rr0,
+                   T* pp, T* vv, T* ss, T* tt);
+    void release_graphs();
 };
 
 }  // namespace mp

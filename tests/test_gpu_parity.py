@@ -190,6 +190,34 @@ def test_fs_2d_cavity_multigrid_matches_oracle(ctx, cavity2d):
     _fs_parity(ctx, cavity2d, 10, 1, 1.0, 0.01, 2e-4, M.MP_PREC_MIXED, M.MP_PRECOND_AMG)
 
 
+@pytest.mark.parametrize("precision,precond", [(0, 0), (1, 1)])
+def test_graph_replay_matches_plain_launches(ctx, cube3d_small, precision, precond):
+    """Krylov chunks replayed from hipGraphs (many replays per solve, eager
+    predictor/corrector work and host polls between them) give bit-identical
+    fields to the same chunks launched one by one."""
+    import memphys_amd as M
+    cloud = cube3d_small
+    ctx.upload(cloud)
+    ctx.weights(3, 3, download=False)
+    u_dir, p_dir, u0 = lid_bc(cloud, 2)
+    out = []
+    for graphs in (True, False, True):
+        ctx.graph_mode(graphs)
+        ctx.fs_create(1.0, 0.0025, 5e-4, 2, 1e-10, 20000, u_dir, u_dir, u_dir, p_dir,
+                      precision=precision, precond=precond, check_every=4)
+        ctx.set_field(M.MP_U, u0)
+        for w in (M.MP_V, M.MP_W):
+            ctx.set_field(w, np.zeros(cloud.nv))
+        ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
+        ctx.step(6, 0)
+        out.append([ctx.get_field(w) for w in (M.MP_U, M.MP_V, M.MP_W, M.MP_P)])
+        _print(f"graphs {graphs} prec {precision} precond {precond}: total iterations",
+               ctx.stats().total_iterations)
+    ctx.graph_mode(True)
+    for a, b, c in zip(*out):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
 def test_condition_estimate_matches_oracle(ctx, cube3d_small):
     """cond = ||A||_1 * Hager/Higham estimate of ||A^-1||_1 (the estimator behind
     Eigen's rcond, coefficient_computations.cpp:312), same arithmetic order."""
