@@ -85,6 +85,21 @@ int mp_sync(mp_ctx* ctx);
 int mp_cloud_upload(mp_ctx* ctx, int nv, int dim, const double* xyz, const uint8_t* boundary,
                     const double* normal, const int* row_ptr, const int* col_idx);
 
+/* Batch of independent clouds (no reference counterpart: the reference runs
+ * one cloud per process).  nprob clouds given back to back in caller order:
+ * problem q owns caller nodes [off_q, off_q + nv_per_problem[q]) with off_q the
+ * sum of the earlier sizes; row_ptr/col_idx index the concatenation and a
+ * stencil may only reference nodes of its own problem.  Every later call
+ * (weights, mp_fs_create, mp_fs_step, fields) then serves all problems at once:
+ * one kernel launch per phase covers every problem, and each pressure system
+ * keeps its own Krylov scalars and stopping test.  Node fields are the
+ * concatenation of the problems' fields; a pressure array is the nodes
+ * followed by the border unknowns of the full-Neumann problems in problem
+ * order.  mp_cloud_upload is the one-problem case. */
+int mp_batch_upload(mp_ctx* ctx, int nprob, const int* nv_per_problem, int dim, const double* xyz,
+                    const uint8_t* boundary, const double* normal, const int* row_ptr,
+                    const int* col_idx);
+
 /* Batched PHS + polynomial stencil weights (replaces CLOUD::calc_grad_laplace_coeffs,
  * cloud.cpp:207-279, and calc_PHS_RBF_grad_laplace_single_vert,
  * coefficient_computations.cpp:294-328).  exps: m*dim monomial exponents.
@@ -150,6 +165,25 @@ int mp_guess_eval(mp_ctx* ctx, const double* p1, const double* p2, const double*
  * (postprocessing_functions.cpp:192-195, 248-251):
  * out = {max_int, mean_int, max_bnd, mean_bnd, max_all, mean_all}. */
 int mp_divergence(mp_ctx* ctx, double* out);
+
+/* Per-problem rho, mu, dt of a batch (after mp_fs_create; arrays of nprob);
+ * replaces the scalars of mp_fs_params for the time steps. */
+int mp_batch_params(mp_ctx* ctx, const double* rho, const double* mu, const double* dt);
+/* Statistics of one problem's last pressure solve (total_iterations: since
+ * mp_fs_create). */
+int mp_batch_stats(mp_ctx* ctx, int problem, mp_step_stats* stats);
+
+/* Autotuner (no reference counterpart): after at least one mp_fs_step,
+ * (1) with the multigrid preconditioner, picks the layout of every hierarchy
+ * matrix (SELL-64 or CSR with 4..64 lanes per row) by timing its product; the
+ * layouts sum rows in different orders, so later V-cycles differ from the
+ * set-up's rule-of-thumb layouts at the rounding level;
+ * (2) times the last step's pressure solve for convergence-poll intervals of
+ * 4, 8, 16, 32 and 64 iterations (each from the same starting pressure; the
+ * results do not depend on the interval) and keeps the fastest.  chosen
+ * receives it, ms_per_candidate (may be NULL) the 5 timings.  Fields and the
+ * pressure history are left unchanged. */
+int mp_autotune(mp_ctx* ctx, int* chosen, double* ms_per_candidate);
 
 /* Multigrid hierarchy of the pressure operator (MP_PRECOND_AMG): number of
  * levels (<= max_levels), rows and stored entries per level, host set-up time. */

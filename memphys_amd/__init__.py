@@ -34,7 +34,7 @@ HIP_SYMBOLS = (
     "mp_fields_get", "mp_fs_step", "mp_stats_get", "mp_guess_mode", "mp_divergence",
     "mp_kernel_timing", "mp_kernel_timing_get", "mp_guess_param_count", "mp_guess_train",
     "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
-    "mp_graph_mode",
+    "mp_graph_mode", "mp_batch_upload", "mp_batch_params", "mp_batch_stats", "mp_autotune",
 )
 
 
@@ -106,6 +106,10 @@ def hip_lib() -> C.CDLL:
         lib.mp_guess_last_loss.argtypes = [vp, C.POINTER(C.c_double)]
         lib.mp_guess_eval.argtypes = [vp, _dp, _dp, _dp, vp, _dp, vp, C.POINTER(C.c_double)]
         lib.mp_graph_mode.argtypes = [vp, C.c_int]
+        lib.mp_batch_upload.argtypes = [vp, C.c_int, _ip, C.c_int, _dp, _u8, _dp, _ip, _ip]
+        lib.mp_batch_params.argtypes = [vp, _dp, _dp, _dp]
+        lib.mp_batch_stats.argtypes = [vp, C.c_int, C.POINTER(StepStats)]
+        lib.mp_autotune.argtypes = [vp, C.POINTER(C.c_int), _dp]
         lib.mp_precond_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), vp, vp,
                                         C.POINTER(C.c_double)]
         for name in HIP_SYMBOLS:
@@ -238,6 +242,35 @@ class Context:
             np.ascontiguousarray(cloud.boundary), np.ascontiguousarray(cloud.normal.ravel()),
             np.ascontiguousarray(cloud.row_ptr), np.ascontiguousarray(cloud.col)),
             "mp_cloud_upload")
+
+    def upload_batch(self, clouds):
+        """Uploads independent clouds as one batch (concatenated in order)."""
+        cat = concat_clouds(clouds)
+        sizes = np.ascontiguousarray([c.nv for c in clouds], dtype=np.int32)
+        self.cloud = cat
+        self.batch_sizes = sizes
+        self._check(self.lib.mp_batch_upload(
+            self.h, len(clouds), sizes, cat.dim, np.ascontiguousarray(cat.xyz.ravel()),
+            np.ascontiguousarray(cat.boundary), np.ascontiguousarray(cat.normal.ravel()),
+            np.ascontiguousarray(cat.row_ptr), np.ascontiguousarray(cat.col)), "mp_batch_upload")
+        return cat
+
+    def batch_params(self, rho, mu, dt):
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        self._check(self.lib.mp_batch_params(self.h, f(rho), f(mu), f(dt)), "mp_batch_params")
+
+    def autotune(self):
+        """Picks the fastest convergence-poll interval on the last step's solve;
+        returns (interval, {interval: ms})."""
+        chosen = C.c_int()
+        ms = np.zeros(5)
+        self._check(self.lib.mp_autotune(self.h, C.byref(chosen), ms), "mp_autotune")
+        return chosen.value, dict(zip((4, 8, 16, 32, 64), ms.tolist()))
+
+    def batch_stats(self, problem: int) -> StepStats:
+        st = StepStats()
+        self._check(self.lib.mp_batch_stats(self.h, problem, C.byref(st)), "mp_batch_stats")
+        return st
 
     def weights(self, phs_deg: int, poly_deg: int, precision: int = MP_PREC_FP64,
                 download: bool = True, cond: bool = False):
@@ -435,6 +468,53 @@ def train_guess(cloud: Cloud, rho: float, mu: float, dt: float, tol: float, epis
                 log(f"guess training episode {ep}: lid {speed:.3f}, mean iterations "
                     f"{iters / steps_per_episode:.1f}, last loss {ctx.guess_last_loss():.3e}")
         return ctx.guess_params()
+
+
+def rng_uniform(seed: int, count: int) -> np.ndarray:
+    """First `count` values of the library's seeded generator (std::mt19937_64,
+    53-bit mapping (x >> 11) * 2^-53)."""
+    out = np.zeros(count)
+    host_lib().mph_rng_uniform(seed, count, out.ctypes.data_as(C.c_void_p), None)
+    return out
+
+
+def cavity_batch(nprob: int = 64, seed: int = 5, n_min: int = 2000, n_max: int = 40000,
+                 re_range=(50.0, 400.0), k: int = 30, jitter: float = 0.25):
+    """The batched-cavities workload: nprob 2D lid-driven cavities with node
+    counts log-uniform on [n_min, n_max] (lattice n x n minus the 4 corners),
+    Reynolds numbers uniform on re_range (rho = 1, mu = 1/Re, lid speed 1),
+    interior jitter +-jitter*h, PHS r^3 + degree-3 polynomial, k-node stencils.
+    The time step of each problem is half its explicit AB2 limit,
+    dt = min(0.5 h^2 Re / 33.7, 0.2 h) (33.7 h^-2 bounds |lambda(L)| on these
+    clouds).  Returns [(cloud, Re, dt)]."""
+    u = rng_uniform(seed, 2 * nprob)
+    out = []
+    for q in range(nprob):
+        nodes = np.exp(np.log(n_min) + u[2 * q] * (np.log(n_max) - np.log(n_min)))
+        n = max(8, int(round(np.sqrt(nodes + 4.0))))
+        re = re_range[0] + u[2 * q + 1] * (re_range[1] - re_range[0])
+        h = 1.0 / (n - 1)
+        dt = min(0.5 * h * h * re / 33.7, 0.2 * h)
+        cloud = box_cloud(dim=2, n=n, jitter=jitter, seed=seed * 1000 + q, k_interior=k)
+        out.append((cloud, re, dt))
+    return out
+
+
+def concat_clouds(clouds) -> "Cloud":
+    """Disjoint union of clouds (node indices of each shifted past the earlier ones)."""
+    dim = clouds[0].dim
+    if any(c.dim != dim for c in clouds):
+        raise MemphysError("concat_clouds: mixed dimensions")
+    offs = np.cumsum([0] + [c.nv for c in clouds])
+    nnz_offs = np.cumsum([0] + [c.nnz for c in clouds])
+    row_ptr = np.concatenate([c.row_ptr[:-1] + nnz_offs[k] for k, c in enumerate(clouds)] +
+                             [np.array([nnz_offs[-1]])]).astype(np.int32)
+    col = np.concatenate([c.col + offs[k] for k, c in enumerate(clouds)]).astype(np.int32)
+    return Cloud(dim=dim, xyz=np.concatenate([c.xyz for c in clouds]),
+                 boundary=np.concatenate([c.boundary for c in clouds]),
+                 normal=np.concatenate([c.normal for c in clouds]), row_ptr=row_ptr, col=col,
+                 ksize=np.concatenate([c.ksize for c in clouds]),
+                 layer=np.concatenate([c.layer for c in clouds]))
 
 
 def amg_first_level(A, k: int = 6, omega_p: float = 0.66):

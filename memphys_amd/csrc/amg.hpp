@@ -40,12 +40,19 @@ struct AmgDevice {
     std::vector<long> nnz;
     std::string layout;  // chosen layout per matrix, for reports
     double setup_seconds = 0.0;
+    // host copies of the hierarchy (A[0] stays empty), for re-tuning layouts
+    std::vector<mp_amg::Csr> hA, hP, hR;
 
     // Builds the hierarchy from the finest operator, given in SELL-64 slots
     // (device order) by the solver: sv and val32 stay owned by the solver and
     // serve as level 0.
     void setup(hipStream_t s, const SellView& sv, const double* val64_dev,
                const float* val32_dev, const mp_amg::Options& opt);
+    // Layout of every hierarchy matrix chosen by timing each candidate (SELL-64,
+    // CSR with 4..64 lanes per row) instead of the set-up's rule of thumb.  The
+    // layouts sum a row in different orders, so the V-cycle then changes at the
+    // rounding level; callers holding graphs over these buffers must drop them.
+    void retune(hipStream_t s);
     // out = M^-1 in for the bordered system of size nv + bordered: the V-cycle
     // on the node block, identity on the border unknown.  Every kernel returns
     // at once when done_flag (may be null) is non-zero.

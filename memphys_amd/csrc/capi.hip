@@ -1,11 +1,16 @@
 // C ABI of the meshless fractional-step path (include/memphys_hip.h).
 //
-// Device order.  The caller's node order (RCM, as CLOUD produces) is
-// re-sorted by stencil length inside windows of SIGMA rows, then cut into
-// slices of 64 rows; each slice stores its neighbours column-major (SELL-64).
-// Every field, weight and operator lives on the device in that order; host
-// arrays crossing the ABI are permuted on the way in and out.
+// Device order.  Each point cloud (one, or every problem of a batch) gets a
+// run of device rows starting on a BLOCK boundary and padded to whole blocks
+// with one row to spare (the border unknown of a full-Neumann pressure
+// system); padding rows have no stencil.  Inside a cloud the caller's node
+// order (RCM, as CLOUD produces) is re-sorted into Morton (Z-order) of the
+// coordinates and then by stencil length inside windows of SIGMA rows, and
+// cut into slices of 64 rows; each slice stores its neighbours column-major
+// (SELL-64).  Every field, weight and operator lives on the device in that
+// order; host arrays crossing the ABI are permuted on the way in and out.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -26,11 +31,19 @@ struct mp_ctx {
     hipStream_t stream = nullptr;
     std::string err;
 
-    // cloud
+    // cloud: nv real nodes in nprob problems; npad device rows (multiple of
+    // BLOCK); perm: device row -> caller node (-1 on padding rows); iperm:
+    // caller node -> device row
     bool has_cloud = false;
     int nv = 0, dim = 0, nslices = 0, npad = 0, total_slots = 0, max_len = 0;
     long nnz = 0;
     std::vector<int> perm, iperm, csr_slot, h_rowptr;
+    int nprob = 1;
+    std::vector<int> prob_off, prob_nv, prob_row0, prob_rows;  // caller offset, nodes, rows
+    DevBuf<int> blk_prob;     // problem of each row block
+    DevBuf<double> prob_prm;  // rho, mu, dt per problem (batch parameters)
+    bool per_problem_prm = false;
+    std::vector<long> prob_total_iters;  // Krylov iterations per problem since mp_fs_create
     std::vector<uint8_t> h_bnd;
     DevBuf<int> sptr, col, len, brows;
     int nb = 0;
@@ -94,15 +107,48 @@ int guarded(mp_ctx* c, const char* fn, F&& body) {
     }
 }
 
-void to_device_order(const mp_ctx* c, const double* src, std::vector<double>& dst, int n_sys) {
+// Caller order -> device rows.  Node fields have nv entries; a pressure
+// system (S != null) has nv node entries followed by the border unknowns of
+// its bordered problems, in problem order.
+void to_device_order(const mp_ctx* c, const double* src, std::vector<double>& dst,
+                     const PpeSolver* S = nullptr) {
     dst.assign((size_t)c->npad + 64, 0.0);
-    for (int d = 0; d < c->nv; ++d) dst[d] = src[c->perm[d]];
-    if (n_sys > c->nv) dst[c->nv] = src[c->nv];
+    for (int o = 0; o < c->nv; ++o) dst[c->iperm[o]] = src[o];
+    if (S) {
+        int k = c->nv;
+        for (const Seg& g : S->segs)
+            if (g.border) dst[g.row0 + g.nv] = src[k++];
+    }
 }
 
-void from_device_order(const mp_ctx* c, const std::vector<double>& src, double* dst, int n_sys) {
-    for (int d = 0; d < c->nv; ++d) dst[c->perm[d]] = src[d];
-    if (n_sys > c->nv) dst[c->nv] = src[c->nv];
+void from_device_order(const mp_ctx* c, const std::vector<double>& src, double* dst,
+                       const PpeSolver* S = nullptr) {
+    for (int o = 0; o < c->nv; ++o) dst[o] = src[c->iperm[o]];
+    if (S) {
+        int k = c->nv;
+        for (const Seg& g : S->segs)
+            if (g.border) dst[k++] = src[g.row0 + g.nv];
+    }
+}
+
+// The systems of a pressure operator with Dirichlet flags dir (caller order):
+// a problem without a Dirichlet boundary node is bordered (solver.cpp:205-213).
+std::vector<Seg> make_systems(const mp_ctx* c, const uint8_t* dir) {
+    std::vector<Seg> out(c->nprob);
+    for (int q = 0; q < c->nprob; ++q) {
+        Seg& g = out[q];
+        g.row0 = c->prob_row0[q];
+        g.nv = c->prob_nv[q];
+        g.blk0 = g.row0 / BLOCK;
+        g.nblk = c->prob_rows[q] / BLOCK;
+        g.border = 1;
+        for (int o = c->prob_off[q]; o < c->prob_off[q] + c->prob_nv[q]; ++o)
+            if (c->h_bnd[c->iperm[o]] && dir[o]) {
+                g.border = 0;
+                break;
+            }
+    }
+    return out;
 }
 
 void upload(const mp_ctx* c, double* dev, const std::vector<double>& h) {
@@ -142,7 +188,7 @@ void csr_to_slots(const mp_ctx* c, const double* in, DevBuf<double>& dev) {
 void upload_flags(const mp_ctx* c, const uint8_t* src, DevBuf<uint8_t>& dev) {
     std::vector<uint8_t> h((size_t)c->npad, 0);
     if (src)
-        for (int d = 0; d < c->nv; ++d) h[d] = src[c->perm[d]] ? 1 : 0;
+        for (int o = 0; o < c->nv; ++o) h[c->iperm[o]] = src[o] ? 1 : 0;
     dev.alloc(h.size());
     MP_HIP(hipMemcpy(dev.p, h.data(), h.size(), hipMemcpyHostToDevice));
 }
@@ -173,7 +219,10 @@ void ensure_net(mp_ctx* c) {
         c->net->get_params(c->stream, keep.data());
     }
     c->net.reset(new GuessNet());
-    c->net->setup(c->stream, c->sv, c->ppe->scaled32, c->total_slots);
+    SellView gv = c->sv;  // the node rows of the (single) problem
+    gv.nv = c->nv;
+    gv.nslices = (c->nv + SLICE - 1) / SLICE;
+    c->net->setup(c->stream, gv, c->ppe->scaled32, c->total_slots);
     c->net->lr = c->lr;
     if (keep.empty())
         c->net->init_params(c->stream, 20240611ull);
@@ -223,152 +272,217 @@ int mp_sync(mp_ctx* c) {
     });
 }
 
-int mp_cloud_upload(mp_ctx* c, int nv, int dim, const double* xyz, const uint8_t* boundary,
-                    const double* normal, const int* row_ptr, const int* col_idx) {
-    return guarded(c, "mp_cloud_upload", [&]() -> int {
-        if (nv <= 0 || (dim != 2 && dim != 3) || !xyz || !boundary || !normal || !row_ptr ||
-            !col_idx)
-            return fail(c, MP_ERR_ARG, "mp_cloud_upload: bad arguments");
-        if (row_ptr[0] != 0) return fail(c, MP_ERR_ARG, "mp_cloud_upload: row_ptr[0] != 0");
-        int maxlen = 0;
+}  // extern "C"
+
+namespace {
+
+// Uploads nprob clouds given back to back in caller order (problem q owns
+// caller nodes [off_q, off_q + nvp[q]); its stencils may only reference its
+// own nodes).  One problem is the ordinary single-cloud case.
+int upload_clouds(mp_ctx* c, const char* fn, int nprob, const int* nvp, int dim,
+                  const double* xyz, const uint8_t* boundary, const double* normal,
+                  const int* row_ptr, const int* col_idx) {
+    const std::string f(fn);
+    if (nprob < 1 || !nvp || (dim != 2 && dim != 3) || !xyz || !boundary || !normal || !row_ptr ||
+        !col_idx)
+        return fail(c, MP_ERR_ARG, f + ": bad arguments");
+    long nvl = 0;
+    for (int q = 0; q < nprob; ++q) {
+        if (nvp[q] <= 0) return fail(c, MP_ERR_ARG, f + ": a problem has no nodes");
+        nvl += nvp[q];
+    }
+    if (nvl > (1L << 30)) return fail(c, MP_ERR_ARG, f + ": too many nodes");
+    const int nv = (int)nvl;
+    if (row_ptr[0] != 0) return fail(c, MP_ERR_ARG, f + ": row_ptr[0] != 0");
+    int maxlen = 0;
+    {
+        int q = 0, lo = 0;
         for (int i = 0; i < nv; ++i) {
-            int l = row_ptr[i + 1] - row_ptr[i];
+            while (i >= lo + nvp[q]) lo += nvp[q++];
+            const int l = row_ptr[i + 1] - row_ptr[i];
             if (l < 1 || l > MAX_STENCIL)
                 return fail(c, MP_ERR_ARG,
-                            "mp_cloud_upload: row " + std::to_string(i) + " has " +
-                                std::to_string(l) + " entries (allowed 1.." +
-                                std::to_string(MAX_STENCIL) + ")");
+                            f + ": row " + std::to_string(i) + " has " + std::to_string(l) +
+                                " entries (allowed 1.." + std::to_string(MAX_STENCIL) + ")");
             if (col_idx[row_ptr[i]] != i)
-                return fail(c, MP_ERR_ARG, "mp_cloud_upload: entry 0 of row " +
-                                               std::to_string(i) + " is not the node itself");
+                return fail(c, MP_ERR_ARG,
+                            f + ": entry 0 of row " + std::to_string(i) + " is not the node itself");
             for (int t = row_ptr[i]; t < row_ptr[i + 1]; ++t)
-                if (col_idx[t] < 0 || col_idx[t] >= nv)
-                    return fail(c, MP_ERR_ARG, "mp_cloud_upload: column index out of range");
+                if (col_idx[t] < lo || col_idx[t] >= lo + nvp[q])
+                    return fail(c, MP_ERR_ARG,
+                                f + ": column index out of range (row " + std::to_string(i) + ")");
             maxlen = std::max(maxlen, l);
         }
-        c->has_cloud = c->has_weights = c->has_fs = false;
-        c->solver.reset();
-        c->ppe.reset();
-        c->nv = nv;
-        c->dim = dim;
-        c->max_len = maxlen;
-        c->nslices = (nv + SLICE - 1) / SLICE;
-        c->npad = c->nslices * SLICE;
-        c->h_rowptr.assign(row_ptr, row_ptr + nv + 1);
-        c->nnz = row_ptr[nv];
-        // device order: Morton (Z-order) of the coordinates, so that 64
-        // consecutive rows are a compact cluster whose stencils overlap, then a
-        // stable sort by decreasing stencil length inside windows of SIGMA rows
-        c->perm.resize(nv);
-        std::iota(c->perm.begin(), c->perm.end(), 0);
-        {
-            double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-            for (int a = 0; a < dim; ++a) {
-                lo[a] = hi[a] = xyz[a];
-                for (int i = 0; i < nv; ++i) {
-                    lo[a] = std::min(lo[a], xyz[(size_t)i * dim + a]);
-                    hi[a] = std::max(hi[a], xyz[(size_t)i * dim + a]);
-                }
+    }
+    c->has_cloud = c->has_weights = c->has_fs = false;
+    c->solver.reset();
+    c->ppe.reset();
+    c->net.reset();
+    c->nv = nv;
+    c->dim = dim;
+    c->max_len = maxlen;
+    c->nprob = nprob;
+    c->prob_off.assign(nprob, 0);
+    c->prob_nv.assign(nvp, nvp + nprob);
+    c->prob_row0.assign(nprob, 0);
+    c->prob_rows.assign(nprob, 0);
+    int rows = 0;
+    for (int q = 0; q < nprob; ++q) {
+        c->prob_off[q] = q ? c->prob_off[q - 1] + nvp[q - 1] : 0;
+        c->prob_row0[q] = rows;
+        c->prob_rows[q] = ((nvp[q] + 1 + BLOCK - 1) / BLOCK) * BLOCK;
+        rows += c->prob_rows[q];
+    }
+    c->npad = rows;
+    c->nslices = rows / SLICE;
+    c->h_rowptr.assign(row_ptr, row_ptr + nv + 1);
+    c->nnz = row_ptr[nv];
+    c->per_problem_prm = false;
+    // device order per problem: Morton (Z-order) of the coordinates, so that 64
+    // consecutive rows are a compact cluster whose stencils overlap, then a
+    // stable sort by decreasing stencil length inside windows of SIGMA rows
+    c->perm.assign(rows, -1);
+    c->iperm.assign(nv, 0);
+    for (int q = 0; q < nprob; ++q) {
+        const int o0 = c->prob_off[q], n = nvp[q];
+        std::vector<int> ord(n);
+        std::iota(ord.begin(), ord.end(), o0);
+        double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (int a = 0; a < dim; ++a) {
+            lo[a] = hi[a] = xyz[(size_t)o0 * dim + a];
+            for (int i = o0; i < o0 + n; ++i) {
+                lo[a] = std::min(lo[a], xyz[(size_t)i * dim + a]);
+                hi[a] = std::max(hi[a], xyz[(size_t)i * dim + a]);
             }
-            const int bits = (dim == 3) ? 10 : 16;
-            const double levels = (double)(1u << bits);
-            std::vector<uint64_t> key(nv, 0);
-            for (int i = 0; i < nv; ++i) {
-                uint64_t k = 0;
-                for (int a = 0; a < dim; ++a) {
-                    double span = hi[a] - lo[a];
-                    double t = span > 0 ? (xyz[(size_t)i * dim + a] - lo[a]) / span : 0.0;
-                    uint64_t q = (uint64_t)std::min(levels - 1.0, std::floor(t * levels));
-                    for (int bt = 0; bt < bits; ++bt) k |= ((q >> bt) & 1ull) << (bt * dim + a);
-                }
-                key[i] = k;
-            }
-            std::stable_sort(c->perm.begin(), c->perm.end(),
-                             [&](int a, int b) { return key[a] < key[b]; });
         }
-        for (int w0 = 0; w0 < nv; w0 += SIGMA) {
-            int w1 = std::min(nv, w0 + SIGMA);
-            std::stable_sort(c->perm.begin() + w0, c->perm.begin() + w1, [&](int a, int b) {
+        const int bits = (dim == 3) ? 10 : 16;
+        const double levels = (double)(1u << bits);
+        std::vector<uint64_t> key(n, 0);
+        for (int i = 0; i < n; ++i) {
+            uint64_t k = 0;
+            for (int a = 0; a < dim; ++a) {
+                const double span = hi[a] - lo[a];
+                const double t = span > 0 ? (xyz[(size_t)(o0 + i) * dim + a] - lo[a]) / span : 0.0;
+                const uint64_t g = (uint64_t)std::min(levels - 1.0, std::floor(t * levels));
+                for (int bt = 0; bt < bits; ++bt) k |= ((g >> bt) & 1ull) << (bt * dim + a);
+            }
+            key[i] = k;
+        }
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](int a, int b) { return key[a - o0] < key[b - o0]; });
+        for (int w0 = 0; w0 < n; w0 += SIGMA) {
+            const int w1 = std::min(n, w0 + SIGMA);
+            std::stable_sort(ord.begin() + w0, ord.begin() + w1, [&](int a, int b) {
                 return (row_ptr[a + 1] - row_ptr[a]) > (row_ptr[b + 1] - row_ptr[b]);
             });
         }
-        c->iperm.assign(nv, 0);
-        for (int d = 0; d < nv; ++d) c->iperm[c->perm[d]] = d;
-        std::vector<int> hsptr(c->nslices + 1, 0), hlen(c->npad, 0);
-        for (int d = 0; d < nv; ++d) hlen[d] = row_ptr[c->perm[d] + 1] - row_ptr[c->perm[d]];
-        for (int s = 0; s < c->nslices; ++s) {
-            int wmax = 0;
-            for (int l = 0; l < SLICE; ++l) wmax = std::max(wmax, hlen[s * SLICE + l]);
-            hsptr[s + 1] = hsptr[s] + wmax * SLICE;
+        for (int i = 0; i < n; ++i) {
+            c->perm[c->prob_row0[q] + i] = ord[i];
+            c->iperm[ord[i]] = c->prob_row0[q] + i;
         }
-        c->total_slots = hsptr[c->nslices];
-        std::vector<int> hcol(c->total_slots, 0);
-        c->csr_slot.assign(row_ptr[nv], 0);
-        for (int s = 0; s < c->nslices; ++s) {
-            int wmax = (hsptr[s + 1] - hsptr[s]) / SLICE;
-            for (int l = 0; l < SLICE; ++l) {
-                int d = s * SLICE + l;
-                for (int j = 0; j < wmax; ++j) {
-                    int slot = hsptr[s] + j * SLICE + l;
-                    if (d < nv && j < hlen[d]) {
-                        int t = row_ptr[c->perm[d]] + j;
-                        hcol[slot] = c->iperm[col_idx[t]];
-                        c->csr_slot[t] = slot;
-                    } else {
-                        hcol[slot] = (d < nv) ? d : 0;
-                    }
+    }
+    std::vector<int> hsptr(c->nslices + 1, 0), hlen(c->npad, 0);
+    for (int d = 0; d < rows; ++d)
+        if (c->perm[d] >= 0) hlen[d] = row_ptr[c->perm[d] + 1] - row_ptr[c->perm[d]];
+    for (int sl = 0; sl < c->nslices; ++sl) {
+        int wmax = 0;
+        for (int l = 0; l < SLICE; ++l) wmax = std::max(wmax, hlen[sl * SLICE + l]);
+        hsptr[sl + 1] = hsptr[sl] + wmax * SLICE;
+    }
+    c->total_slots = hsptr[c->nslices];
+    std::vector<int> hcol(c->total_slots, 0);
+    c->csr_slot.assign(row_ptr[nv], 0);
+    for (int sl = 0; sl < c->nslices; ++sl) {
+        const int wmax = (hsptr[sl + 1] - hsptr[sl]) / SLICE;
+        for (int l = 0; l < SLICE; ++l) {
+            const int d = sl * SLICE + l;
+            for (int j = 0; j < wmax; ++j) {
+                const int slot = hsptr[sl] + j * SLICE + l;
+                if (j < hlen[d]) {
+                    const int t = row_ptr[c->perm[d]] + j;
+                    hcol[slot] = c->iperm[col_idx[t]];
+                    c->csr_slot[t] = slot;
+                } else {
+                    hcol[slot] = d;  // padding slot: own row, value 0
                 }
             }
         }
-        std::vector<double> hx((size_t)dim * c->npad, 0.0), hn((size_t)dim * c->npad, 0.0);
-        c->h_bnd.assign(c->npad, 0);
-        std::vector<int> hb;
-        for (int d = 0; d < nv; ++d) {
-            int o = c->perm[d];
-            for (int a = 0; a < dim; ++a) {
-                hx[(size_t)a * c->npad + d] = xyz[(size_t)o * dim + a];
-                hn[(size_t)a * c->npad + d] = normal[(size_t)o * dim + a];
-            }
-            c->h_bnd[d] = boundary[o] ? 1 : 0;
-            if (boundary[o]) hb.push_back(d);
+    }
+    std::vector<double> hx((size_t)dim * c->npad, 0.0), hn((size_t)dim * c->npad, 0.0);
+    c->h_bnd.assign(c->npad, 0);
+    std::vector<int> hb;
+    for (int d = 0; d < rows; ++d) {
+        const int o = c->perm[d];
+        if (o < 0) continue;
+        for (int a = 0; a < dim; ++a) {
+            hx[(size_t)a * c->npad + d] = xyz[(size_t)o * dim + a];
+            hn[(size_t)a * c->npad + d] = normal[(size_t)o * dim + a];
         }
-        for (int d : hb)
-            for (int t = row_ptr[c->perm[d]] + 1; t < row_ptr[c->perm[d] + 1]; ++t)
-                if (boundary[col_idx[t]])
-                    return fail(c, MP_ERR_ARG,
-                                "mp_cloud_upload: boundary node " + std::to_string(c->perm[d]) +
-                                    " couples to boundary node " + std::to_string(col_idx[t]));
-        c->nb = (int)hb.size();
-        c->sptr.alloc(hsptr.size());
-        c->col.alloc(hcol.size());
-        c->len.alloc(hlen.size());
-        c->brows.alloc(std::max<size_t>(1, hb.size()));
-        c->xyz.alloc(hx.size());
-        c->normal.alloc(hn.size());
-        c->bnd.alloc(c->h_bnd.size());
-        MP_HIP(hipMemcpy(c->sptr.p, hsptr.data(), sizeof(int) * hsptr.size(), hipMemcpyHostToDevice));
-        MP_HIP(hipMemcpy(c->col.p, hcol.data(), sizeof(int) * hcol.size(), hipMemcpyHostToDevice));
-        MP_HIP(hipMemcpy(c->len.p, hlen.data(), sizeof(int) * hlen.size(), hipMemcpyHostToDevice));
-        if (!hb.empty())
-            MP_HIP(hipMemcpy(c->brows.p, hb.data(), sizeof(int) * hb.size(), hipMemcpyHostToDevice));
-        MP_HIP(hipMemcpy(c->xyz.p, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice));
-        MP_HIP(hipMemcpy(c->normal.p, hn.data(), sizeof(double) * hn.size(), hipMemcpyHostToDevice));
-        MP_HIP(hipMemcpy(c->bnd.p, c->h_bnd.data(), c->h_bnd.size(), hipMemcpyHostToDevice));
-        c->sv.nv = nv;
-        c->sv.nslices = c->nslices;
-        c->sv.sptr = c->sptr.p;
-        c->sv.col = c->col.p;
-        c->sv.len = c->len.p;
-        for (auto& wb : c->w) {
-            wb.alloc(c->total_slots);
-            MP_HIP(hipMemsetAsync(wb.p, 0, sizeof(double) * wb.n, c->stream));
-        }
-        alloc_vec(c, c->vec_a);
-        alloc_vec(c, c->vec_b);
-        c->div_partial.alloc((size_t)((nv + BLOCK - 1) / BLOCK) * 4);
-        MP_HIP(hipStreamSynchronize(c->stream));
-        c->has_cloud = true;
-        return MP_OK;
+        c->h_bnd[d] = boundary[o] ? 1 : 0;
+        if (boundary[o]) hb.push_back(d);
+    }
+    for (int d : hb)
+        for (int t = row_ptr[c->perm[d]] + 1; t < row_ptr[c->perm[d] + 1]; ++t)
+            if (boundary[col_idx[t]])
+                return fail(c, MP_ERR_ARG,
+                            f + ": boundary node " + std::to_string(c->perm[d]) +
+                                " couples to boundary node " + std::to_string(col_idx[t]));
+    std::vector<int> hbp(rows / BLOCK, 0);
+    for (int q = 0; q < nprob; ++q)
+        for (int b = 0; b < c->prob_rows[q] / BLOCK; ++b) hbp[c->prob_row0[q] / BLOCK + b] = q;
+    c->nb = (int)hb.size();
+    c->sptr.alloc(hsptr.size());
+    c->col.alloc(hcol.size());
+    c->len.alloc(hlen.size());
+    c->brows.alloc(std::max<size_t>(1, hb.size()));
+    c->xyz.alloc(hx.size());
+    c->normal.alloc(hn.size());
+    c->bnd.alloc(c->h_bnd.size());
+    c->blk_prob.alloc(hbp.size());
+    MP_HIP(hipMemcpy(c->sptr.p, hsptr.data(), sizeof(int) * hsptr.size(), hipMemcpyHostToDevice));
+    MP_HIP(hipMemcpy(c->col.p, hcol.data(), sizeof(int) * hcol.size(), hipMemcpyHostToDevice));
+    MP_HIP(hipMemcpy(c->len.p, hlen.data(), sizeof(int) * hlen.size(), hipMemcpyHostToDevice));
+    if (!hb.empty())
+        MP_HIP(hipMemcpy(c->brows.p, hb.data(), sizeof(int) * hb.size(), hipMemcpyHostToDevice));
+    MP_HIP(hipMemcpy(c->xyz.p, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice));
+    MP_HIP(hipMemcpy(c->normal.p, hn.data(), sizeof(double) * hn.size(), hipMemcpyHostToDevice));
+    MP_HIP(hipMemcpy(c->bnd.p, c->h_bnd.data(), c->h_bnd.size(), hipMemcpyHostToDevice));
+    MP_HIP(hipMemcpy(c->blk_prob.p, hbp.data(), sizeof(int) * hbp.size(), hipMemcpyHostToDevice));
+    c->sv.nv = rows;
+    c->sv.nslices = c->nslices;
+    c->sv.sptr = c->sptr.p;
+    c->sv.col = c->col.p;
+    c->sv.len = c->len.p;
+    for (auto& wb : c->w) {
+        wb.alloc(c->total_slots);
+        MP_HIP(hipMemsetAsync(wb.p, 0, sizeof(double) * wb.n, c->stream));
+    }
+    alloc_vec(c, c->vec_a);
+    alloc_vec(c, c->vec_b);
+    c->div_partial.alloc((size_t)(rows / BLOCK) * 4);
+    MP_HIP(hipStreamSynchronize(c->stream));
+    c->has_cloud = true;
+    return MP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mp_cloud_upload(mp_ctx* c, int nv, int dim, const double* xyz, const uint8_t* boundary,
+                    const double* normal, const int* row_ptr, const int* col_idx) {
+    return guarded(c, "mp_cloud_upload", [&]() -> int {
+        return upload_clouds(c, "mp_cloud_upload", 1, &nv, dim, xyz, boundary, normal, row_ptr,
+                             col_idx);
+    });
+}
+
+int mp_batch_upload(mp_ctx* c, int nprob, const int* nv_per_problem, int dim, const double* xyz,
+                    const uint8_t* boundary, const double* normal, const int* row_ptr,
+                    const int* col_idx) {
+    return guarded(c, "mp_batch_upload", [&]() -> int {
+        return upload_clouds(c, "mp_batch_upload", nprob, nv_per_problem, dim, xyz, boundary,
+                             normal, row_ptr, col_idx);
     });
 }
 
@@ -406,10 +520,11 @@ int mp_weights_assemble(mp_ctx* c, int phs_deg, int poly_deg, const int* exps, i
             // longer boundary / near-boundary stencils and any stencil whose
             // refinement stalls are solved in FP64.
             int kmin = MAX_STENCIL + 1;
-            for (int d = 0; d < c->nv; ++d)
-                kmin = std::min(kmin, c->h_rowptr[c->perm[d] + 1] - c->h_rowptr[c->perm[d]]);
+            for (int o = 0; o < c->nv; ++o)
+                kmin = std::min(kmin, c->h_rowptr[o + 1] - c->h_rowptr[o]);
             std::vector<int> mixed_rows, fp64_rows;
-            for (int d = 0; d < c->nv; ++d) {
+            for (int d = 0; d < c->npad; ++d) {
+                if (c->perm[d] < 0) continue;
                 const int len = c->h_rowptr[c->perm[d] + 1] - c->h_rowptr[c->perm[d]];
                 (len == kmin ? mixed_rows : fp64_rows).push_back(d);
             }
@@ -454,7 +569,8 @@ int mp_weights_assemble(mp_ctx* c, int phs_deg, int poly_deg, const int* exps, i
         if (cond_out) {
             std::vector<double> h(c->npad);
             MP_HIP(hipMemcpy(h.data(), dcond.p, sizeof(double) * c->npad, hipMemcpyDeviceToHost));
-            for (int d = 0; d < c->nv; ++d) cond_out[c->perm[d]] = h[d];
+            for (int d = 0; d < c->npad; ++d)
+                if (c->perm[d] >= 0) cond_out[c->perm[d]] = h[d];
         }
         return MP_OK;
     });
@@ -481,11 +597,11 @@ int mp_operator_apply(mp_ctx* c, int op, const double* x, double* y) {
         if (op < 0 || op > 3 || (op == MP_GZ && c->dim != 3) || !x || !y)
             return fail(c, MP_ERR_ARG, "mp_operator_apply: bad operator or pointer");
         std::vector<double> h;
-        to_device_order(c, x, h, c->nv);
+        to_device_order(c, x, h);
         upload(c, c->vec_a.p, h);
         launch_spmv(c->stream, c->sv, c->w[op].p, c->vec_a.p, c->vec_b.p);
         download(c, c->vec_b.p, h);
-        from_device_order(c, h, y, c->nv);
+        from_device_order(c, h, y);
         return MP_OK;
     });
 }
@@ -497,17 +613,11 @@ int mp_solver_init(mp_ctx* c, const uint8_t* dirichlet, double unsteady, double 
         if (!dirichlet || n_iter < 1 || !(tolerance > 0) ||
             (precision != MP_PREC_FP64 && precision != MP_PREC_MIXED))
             return fail(c, MP_ERR_ARG, "mp_solver_init: bad arguments");
-        bool bordered = true;
-        for (int i = 0; i < c->nv; ++i)
-            if (c->h_bnd[c->iperm[i]] && dirichlet[i]) {
-                bordered = false;
-                break;
-            }
         upload_flags(c, dirichlet, c->solver_dir);
         c->solver.reset(new PpeSolver());
-        c->solver->setup(c->stream, c->sv, c->dim, c->npad, c->w[0].p, c->w[1].p, c->w[2].p,
-                         c->w[3].p, c->normal.p, c->bnd.p, c->solver_dir.p, bordered, unsteady,
-                         conv, diff);
+        c->solver->setup(c->stream, c->sv, make_systems(c, dirichlet), c->dim, c->w[0].p,
+                         c->w[1].p, c->w[2].p, c->w[3].p, c->normal.p, c->bnd.p,
+                         c->solver_dir.p, unsteady, conv, diff);
         c->solver->tol = tolerance;
         c->solver->maxit = n_iter;
         c->solver->precision = precision;
@@ -521,7 +631,7 @@ int mp_solver_system_size(mp_ctx* c, int* n) {
     return guarded(c, "mp_solver_system_size", [&]() -> int {
         PpeSolver* s = c->solver ? c->solver.get() : c->ppe.get();
         if (!s || !n) return fail(c, MP_ERR_STATE, "mp_solver_system_size: no solver");
-        *n = s->n;
+        *n = s->n_sys;
         return MP_OK;
     });
 }
@@ -532,21 +642,21 @@ int mp_solver_solve(mp_ctx* c, const double* rhs, double* field_new, double* fie
         if (!rhs || !field_new || !field_old) return fail(c, MP_ERR_ARG, "mp_solver_solve: NULL");
         PpeSolver& S = *c->solver;
         std::vector<double> hb, hx;
-        to_device_order(c, rhs, hb, S.n);
-        to_device_order(c, field_old, hx, S.n);
+        to_device_order(c, rhs, hb, &S);
+        to_device_order(c, field_old, hx, &S);
         upload(c, c->vec_a.p, hb);
         upload(c, c->vec_b.p, hx);
         SolveStats st = S.solve(c->stream, c->vec_a.p, c->vec_b.p);
         download(c, c->vec_b.p, hx);
-        from_device_order(c, hx, field_new, S.n);
-        if (S.bordered) field_old[c->nv] = field_new[c->nv];
+        from_device_order(c, hx, field_new, &S);
+        for (int k = c->nv; k < S.n_sys; ++k) field_old[k] = field_new[k];  // solver.cpp:361-362
         c->stats.iterations = st.iterations;
         c->stats.outer_cycles = st.outer_cycles;
         c->stats.restarts = st.restarts;
         c->stats.converged = st.converged;
         c->stats.rel_res = st.rel_res;
         c->stats.abs_res = st.abs_res;
-        c->stats.alpha = S.bordered ? field_new[c->nv] : 0.0;
+        c->stats.alpha = st.alpha;
         c->stats.guess_rel_res = st.guess_rel_res;
         c->stats.total_iterations += st.iterations;
         c->stats.total_solves += 1;
@@ -562,13 +672,18 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
             return fail(c, MP_ERR_ARG, "mp_fs_create: NULL argument");
         // FRACTIONAL_STEP_1::check_bc, fractional_step_1.cpp:78-102
         const uint8_t* dirs[3] = {u_dir, v_dir, w_dir};
-        for (int q = 0; q < c->dim; ++q) {
-            long sum = 0;
-            for (int i = 0; i < c->nv; ++i) sum += dirs[q][i] ? 1 : 0;
-            if (sum == 0)
-                return fail(c, MP_ERR_ARG,
-                            "mp_fs_create: full Neumann velocity BC is not permitted");
-        }
+        for (int q = 0; q < c->dim; ++q)
+            for (int pr = 0; pr < c->nprob; ++pr) {
+                long sum = 0;
+                for (int i = c->prob_off[pr]; i < c->prob_off[pr] + c->prob_nv[pr]; ++i)
+                    sum += dirs[q][i] ? 1 : 0;
+                if (sum == 0)
+                    return fail(c, MP_ERR_ARG,
+                                "mp_fs_create: full Neumann velocity BC is not permitted");
+            }
+        if (prm->preconditioner == MP_PRECOND_AMG && c->nprob > 1)
+            return fail(c, MP_ERR_ARG,
+                        "mp_fs_create: the multigrid preconditioner takes a single problem");
         if (prm->rho < 0 || prm->mu < 0)
             return fail(c, MP_ERR_ARG, "mp_fs_create: rho and mu must be set (>= 0)");
         if (prm->temporal_order != 1 && prm->temporal_order != 2)
@@ -583,16 +698,11 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
         for (int q = 0; q < c->dim; ++q) upload_flags(c, dirs[q], c->udir[q]);
         if (c->dim == 2) upload_flags(c, nullptr, c->udir[2]);
         upload_flags(c, p_dir, c->pdir);
-        bool bordered = true;
-        for (int i = 0; i < c->nv; ++i)
-            if (c->h_bnd[c->iperm[i]] && p_dir[i]) {
-                bordered = false;
-                break;
-            }
         c->ppe.reset(new PpeSolver());
-        c->ppe->setup(c->stream, c->sv, c->dim, c->npad, c->w[0].p, c->w[1].p, c->w[2].p,
-                      c->w[3].p, c->normal.p, c->bnd.p, c->pdir.p, bordered, 0.0, 0.0, 1.0,
+        c->ppe->setup(c->stream, c->sv, make_systems(c, p_dir), c->dim, c->w[0].p, c->w[1].p,
+                      c->w[2].p, c->w[3].p, c->normal.p, c->bnd.p, c->pdir.p, 0.0, 0.0, 1.0,
                       prm->preconditioner);
+        c->per_problem_prm = false;
         c->ppe->tol = prm->solver_tolerance;
         c->ppe->maxit = prm->n_iter;
         c->ppe->precision = prm->precision;
@@ -608,7 +718,7 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
         alloc_vec(c, c->P2);
         alloc_vec(c, c->Ptmp);
         c->hist = 0;
-        if (c->net) ensure_net(c);
+        if (c->net && c->nprob == 1) ensure_net(c);
         for (int q = 0; q < 3; ++q) alloc_vec(c, c->uh[q]);
         for (int k = 0; k < 2; ++k) {
             c->S[k].alloc((size_t)3 * c->npad + 64);
@@ -618,6 +728,7 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
         c->cur = 0;
         c->s_cur = 0;
         c->stats = mp_step_stats{};
+        c->prob_total_iters.assign(c->nprob, 0);
         // algorithmic bytes per launch (real entries only)
         const double W = (double)c->nnz, N = (double)c->nv, D = (double)c->dim;
         c->bytes[1] = W * (4.0 + 8.0 * (D + 1.0)) + N * (4.0 + 8.0 * (D /*own*/ + D /*gather*/ +
@@ -636,11 +747,11 @@ int mp_fields_set(mp_ctx* c, int which, const double* values) {
             return fail(c, MP_ERR_ARG, "mp_fields_set: bad field");
         std::vector<double> h;
         if (which == MP_P) {
-            to_device_order(c, values, h, c->ppe->n);
+            to_device_order(c, values, h, c->ppe.get());
             upload(c, c->P.p, h);
             c->hist = 0;  // the pressure history no longer matches
         } else {
-            to_device_order(c, values, h, c->nv);
+            to_device_order(c, values, h);
             upload(c, c->U[c->cur][which].p, h);
         }
         return MP_OK;
@@ -655,10 +766,10 @@ int mp_fields_get(mp_ctx* c, int which, double* values) {
         std::vector<double> h;
         if (which == MP_P) {
             download(c, c->P.p, h);
-            from_device_order(c, h, values, c->ppe->n);
+            from_device_order(c, h, values, c->ppe.get());
         } else {
             download(c, c->U[c->cur][which].p, h);
-            from_device_order(c, h, values, c->nv);
+            from_device_order(c, h, values);
         }
         return MP_OK;
     });
@@ -679,17 +790,19 @@ void fs_step_one(mp_ctx* c, int it) {
     const double* S_oo = c->S[c->s_cur].p;
     // calc_vel_hat
     c->timers[1].begin(c->stream);
+    const int* bp = c->per_problem_prm ? c->blk_prob.p : nullptr;
+    const double* pp = c->per_problem_prm ? c->prob_prm.p : nullptr;
     launch_predictor<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p, Uo[0].p,
                           Uo[1].p, Uo[2].p, c->uh[0].p, c->uh[1].p, c->uh[2].p, S_new, S_oo,
-                          c->npad, q.rho, q.mu, q.dt, euler);
+                          c->npad, q.rho, q.mu, q.dt, euler, bp, pp);
     c->timers[1].end(c->stream);
     // calc_pressure
     c->timers[2].begin(c->stream);
     launch_ppe_rhs<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->uh[0].p, c->uh[1].p,
                         c->uh[2].p, Uo[0].p, Uo[1].p, Uo[2].p, c->P.p, c->bnd.p, c->pdir.p,
-                        c->normal.p, c->npad, q.rho, q.dt, c->b.p, c->ppe->bordered);
+                        c->normal.p, c->npad, q.rho, q.dt, c->b.p, bp, pp);
     c->timers[2].end(c->stream);
-    const int n_sys = c->ppe->n;
+    const int n_sys = c->ppe->n_sys;  // single problem: border unknown at row nv
     const int nv = c->nv;
     bool learned = false;
     if (c->guess_mode != MP_GUESS_PREVIOUS && c->hist >= 2) {
@@ -712,16 +825,17 @@ void fs_step_one(mp_ctx* c, int it) {
         c->last_loss = c->net->loss_host(c->stream);
     }
     // pressure history for the extrapolated / learned guesses
-    hipLaunchKernelGGL(k_copy, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, n_sys, c->P1.p,
+    hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad, c->P1.p,
                        c->P2.p);
-    hipLaunchKernelGGL(k_copy, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, n_sys, c->P.p,
+    hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad, c->P.p,
                        c->P1.p);
     MP_HIP(hipGetLastError());
     c->hist = std::min(c->hist + 1, 2);
     // calc_vel_corr
     c->timers[3].begin(c->stream);
     launch_corrector<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->P.p, c->uh[0].p,
-                          c->uh[1].p, c->uh[2].p, Un[0].p, Un[1].p, Un[2].p, c->npad, q.rho, q.dt);
+                          c->uh[1].p, c->uh[2].p, Un[0].p, Un[1].p, Un[2].p, c->npad, q.rho, q.dt,
+                          bp, pp);
     c->timers[3].end(c->stream);
     launch_boundary_fix<DIM>(c->stream, c->sv, c->brows.p, c->nb, c->w[0].p, c->w[1].p, c->w[2].p,
                              c->normal.p, c->npad, c->pdir.p, c->udir[0].p, c->udir[1].p,
@@ -739,6 +853,8 @@ void fs_step_one(mp_ctx* c, int it) {
     c->stats.guess_rel_res = st.guess_rel_res;
     c->stats.total_iterations += st.iterations;
     c->stats.total_solves += 1;
+    c->prob_total_iters.resize(c->nprob, 0);
+    for (int pr = 0; pr < c->nprob; ++pr) c->prob_total_iters[pr] += c->ppe->seg_stats[pr].iterations;
 }
 
 }  // namespace
@@ -773,6 +889,8 @@ int mp_guess_mode(mp_ctx* c, int mode) {
     return guarded(c, "mp_guess_mode", [&]() -> int {
         if (mode != MP_GUESS_PREVIOUS && mode != MP_GUESS_LEARNED && mode != MP_GUESS_EXTRAPOLATE)
             return fail(c, MP_ERR_ARG, "mp_guess_mode: unknown mode");
+        if (mode != MP_GUESS_PREVIOUS && c->nprob > 1)
+            return fail(c, MP_ERR_ARG, "mp_guess_mode: batches use the previous pressure");
         if (mode == MP_GUESS_LEARNED) {
             if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_mode: mp_fs_create not called");
             if (!c->net) ensure_net(c);
@@ -787,6 +905,7 @@ int mp_guess_param_count(void) { return GNN_NPARAM; }
 int mp_guess_train(mp_ctx* c, int enable, double learning_rate, unsigned long long seed) {
     return guarded(c, "mp_guess_train", [&]() -> int {
         if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_train: mp_fs_create not called");
+        if (c->nprob > 1) return fail(c, MP_ERR_ARG, "mp_guess_train: single problems only");
         if (!(learning_rate > 0)) return fail(c, MP_ERR_ARG, "mp_guess_train: learning rate <= 0");
         c->lr = (float)learning_rate;
         if (!c->net) ensure_net(c);
@@ -801,6 +920,7 @@ int mp_guess_params(mp_ctx* c, int set, float* params) {
     return guarded(c, "mp_guess_params", [&]() -> int {
         if (!params) return fail(c, MP_ERR_ARG, "mp_guess_params: NULL");
         if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_params: mp_fs_create not called");
+        if (c->nprob > 1) return fail(c, MP_ERR_ARG, "mp_guess_params: single problems only");
         if (!c->net) ensure_net(c);
         if (set)
             c->net->set_params(c->stream, params);
@@ -824,15 +944,17 @@ int mp_guess_eval(mp_ctx* c, const double* p1, const double* p2, const double* b
         if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_eval: mp_fs_create not called");
         if (!p1 || !p2 || !b || !guess) return fail(c, MP_ERR_ARG, "mp_guess_eval: NULL");
         if (!c->net) ensure_net(c);
-        const int n_sys = c->ppe->n;
+        if (c->nprob > 1) return fail(c, MP_ERR_ARG, "mp_guess_eval: single problems only");
+        const int n_sys = c->ppe->n_sys;
+        const PpeSolver* S = c->ppe.get();
         std::vector<double> h;
         DevBuf<double> d1, d2, db, dt, dg;
         for (DevBuf<double>* q : {&d1, &d2, &db, &dt, &dg}) alloc_vec(c, *q);
-        to_device_order(c, p1, h, n_sys);
+        to_device_order(c, p1, h, S);
         upload(c, d1.p, h);
-        to_device_order(c, p2, h, n_sys);
+        to_device_order(c, p2, h, S);
         upload(c, d2.p, h);
-        to_device_order(c, b, h, n_sys);
+        to_device_order(c, b, h, S);
         upload(c, db.p, h);
         hipLaunchKernelGGL(k_extrapolate, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, c->nv,
                            n_sys, d1.p, d2.p, dg.p);
@@ -841,9 +963,9 @@ int mp_guess_eval(mp_ctx* c, const double* p1, const double* p2, const double* b
         c->net->forward(c->stream);
         c->net->guess(c->stream, d1.p, d2.p, dg.p);
         download(c, dg.p, h);
-        from_device_order(c, h, guess, c->nv);
+        from_device_order(c, h, guess);
         if (p_true) {
-            to_device_order(c, p_true, h, n_sys);
+            to_device_order(c, p_true, h, S);
             upload(c, dt.p, h);
             c->net->backward(c->stream, d1.p, d2.p, dt.p);
             double ls = c->net->loss_host(c->stream);
@@ -861,7 +983,7 @@ int mp_divergence(mp_ctx* c, double* out) {
     return guarded(c, "mp_divergence", [&]() -> int {
         if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_divergence: mp_fs_create not called");
         if (!out) return fail(c, MP_ERR_ARG, "mp_divergence: NULL");
-        const int g = (c->nv + BLOCK - 1) / BLOCK;
+        const int g = c->npad / BLOCK;  // every row block (padding rows add zeros)
         DevBuf<double>(&U)[3] = c->U[c->cur];
         if (c->dim == 2)
             launch_divergence<2>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, U[0].p, U[1].p,
@@ -887,6 +1009,90 @@ int mp_divergence(mp_ctx* c, double* out) {
         out[3] = nb ? sb / nb : 0.0;
         out[4] = std::max(mi, mb);
         out[5] = (si + sb) / c->nv;
+        return MP_OK;
+    });
+}
+
+int mp_batch_params(mp_ctx* c, const double* rho, const double* mu, const double* dt) {
+    return guarded(c, "mp_batch_params", [&]() -> int {
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_batch_params: mp_fs_create not called");
+        if (!rho || !mu || !dt) return fail(c, MP_ERR_ARG, "mp_batch_params: NULL");
+        std::vector<double> h((size_t)3 * c->nprob);
+        for (int q = 0; q < c->nprob; ++q) {
+            if (!(rho[q] > 0) || !(mu[q] >= 0) || !(dt[q] > 0))
+                return fail(c, MP_ERR_ARG, "mp_batch_params: rho, dt must be > 0, mu >= 0");
+            h[3 * q] = rho[q];
+            h[3 * q + 1] = mu[q];
+            h[3 * q + 2] = dt[q];
+        }
+        c->prob_prm.alloc(h.size());
+        MP_HIP(hipMemcpy(c->prob_prm.p, h.data(), sizeof(double) * h.size(),
+                         hipMemcpyHostToDevice));
+        c->per_problem_prm = true;
+        return MP_OK;
+    });
+}
+
+int mp_batch_stats(mp_ctx* c, int problem, mp_step_stats* stats) {
+    return guarded(c, "mp_batch_stats", [&]() -> int {
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_batch_stats: mp_fs_create not called");
+        if (!stats || problem < 0 || problem >= c->nprob)
+            return fail(c, MP_ERR_ARG, "mp_batch_stats: bad problem index");
+        const SolveStats& g = c->ppe->seg_stats[problem];
+        mp_step_stats o{};
+        o.iterations = g.iterations;
+        o.outer_cycles = g.outer_cycles;
+        o.restarts = g.restarts;
+        o.converged = g.converged;
+        o.rel_res = g.rel_res;
+        o.abs_res = g.abs_res;
+        o.alpha = g.alpha;
+        o.guess_rel_res = g.guess_rel_res;
+        o.total_iterations = c->prob_total_iters.empty() ? 0 : c->prob_total_iters[problem];
+        o.total_solves = c->stats.total_solves;
+        *stats = o;
+        return MP_OK;
+    });
+}
+
+int mp_autotune(mp_ctx* c, int* chosen, double* ms_per_candidate) {
+    return guarded(c, "mp_autotune", [&]() -> int {
+        if (!c->has_fs || c->stats.total_solves == 0)
+            return fail(c, MP_ERR_STATE, "mp_autotune: needs a context that has stepped");
+        // the pressure solve of the last step (right-hand side b, started from
+        // the pressure before that step, kept in P2), repeated into scratch for
+        // each poll interval; results do not depend on the interval, since a
+        // system that has stopped turns the rest of its chunk into no-ops
+        static const int cand[5] = {4, 8, 16, 32, 64};
+        PpeSolver& S = *c->ppe;
+        const int n = c->npad;
+        if (S.amg) {  // multigrid layouts by timing (rounding-level change of the V-cycle)
+            S.release_graphs();
+            S.amg->retune(c->stream);
+        }
+        double best_ms = 1e300;
+        int best = S.check_every;
+        for (int k = 0; k < 5; ++k) {
+            S.check_every = cand[k];
+            double ms = 0.0;
+            for (int rep = 0; rep < 2; ++rep) {  // the first run also captures the graphs
+                hipLaunchKernelGGL(k_copy, dim3(n / 256), dim3(256), 0, c->stream, n, c->P2.p,
+                                   c->vec_a.p);
+                MP_HIP(hipGetLastError());
+                MP_HIP(hipStreamSynchronize(c->stream));
+                const auto t0 = std::chrono::steady_clock::now();
+                S.solve(c->stream, c->b.p, c->vec_a.p);
+                ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                         .count();
+            }
+            if (ms_per_candidate) ms_per_candidate[k] = ms;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = cand[k];
+            }
+        }
+        S.check_every = best;
+        if (chosen) *chosen = best;
         return MP_OK;
     });
 }

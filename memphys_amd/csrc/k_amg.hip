@@ -282,6 +282,16 @@ void upload_tuned(AmgMat& d, const mp_amg::Csr& M, hipStream_t st, float* xin, f
     report += buf;
 }
 
+// Rule-of-thumb layout (deterministic, used at set-up): SELL-64 for short
+// rows, else CSR with about one lane per 5 entries of the mean row (4..64).
+int rule_layout(const mp_amg::Csr& M) {
+    const double avg = (double)M.nnz() / std::max(1, M.nrows);
+    if (avg <= 24.0) return 0;
+    int g = 4;
+    while (g < 64 && g * 5 < avg) g *= 2;
+    return g;
+}
+
 std::vector<float> scaled_diag_inverse(const mp_amg::Csr& A, double omega) {
     std::vector<float> d(A.nrows, 0.0f);
     for (int i = 0; i < A.nrows; ++i) {
@@ -337,17 +347,7 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
     rows.assign(L, 0);
     nnz.assign(L, 0);
     layout.clear();
-    // scratch for the layout timings
-    size_t big = 64;
-    for (int l = 0; l < L; ++l) big = std::max(big, (size_t)((l == 0 ? nv : h.A[l].nrows) + 64));
-    DevBuf<float> tx, ty;
-    alloc_zero(tx, big, st);
-    alloc_zero(ty, big, st);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    MP_HIP(hipEventCreate(&e0));
-    MP_HIP(hipEventCreate(&e1));
-    try {
-        for (int l = 0; l < L; ++l) {
+    for (int l = 0; l < L; ++l) {
             const mp_amg::Csr& A = (l == 0) ? A0 : h.A[l];
             AmgLevelDev& d = lv[l];
             d.n = A.nrows;
@@ -363,8 +363,7 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
                 d.A.col = sv.col;
                 d.A.val = val32_dev;
             } else if (l + 1 < L) {
-                upload_tuned(d.A, A, st, tx.p, ty.p, e0, e1, layout,
-                             ("A" + std::to_string(l)).c_str());
+                upload_mat(d.A, A, rule_layout(A), st);
             }
             const size_t vl = (size_t)A.nrows + 64;
             alloc_zero(d.x, vl, st);
@@ -374,11 +373,49 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
                 alloc_zero(d.y, vl, st);
             }
             if (l + 1 < L) {
-                upload_tuned(d.P, h.P[l], st, tx.p, ty.p, e0, e1, layout,
-                             ("P" + std::to_string(l)).c_str());
-                upload_tuned(d.R, h.R[l], st, tx.p, ty.p, e0, e1, layout,
-                             ("R" + std::to_string(l)).c_str());
+                upload_mat(d.P, h.P[l], rule_layout(h.P[l]), st);
+                upload_mat(d.R, h.R[l], rule_layout(h.R[l]), st);
             }
+    }
+    nc = h.nc;
+    std::vector<float> g(h.coarse_inv.size());
+    for (size_t t = 0; t < g.size(); ++t) g[t] = (float)h.coarse_inv[t];
+    put(cinv, g, st);
+    MP_HIP(hipStreamSynchronize(st));
+    hA = std::move(h.A);
+    hP = std::move(h.P);
+    hR = std::move(h.R);
+    for (int l = 0; l + 1 < L; ++l) {
+        char buf[64];
+        std::snprintf(buf, sizeof(buf), "A%d:%d P%d:%d R%d:%d ", l, lv[l].A.g, l, lv[l].P.g, l,
+                      lv[l].R.g);
+        layout += buf;
+    }
+    setup_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void AmgDevice::retune(hipStream_t st) {
+    const int L = (int)lv.size();
+    if (L < 2) return;
+    size_t big = 64;
+    for (int l = 0; l < L; ++l) big = std::max(big, (size_t)lv[l].n + 64);
+    DevBuf<float> tx, ty;
+    alloc_zero(tx, big, st);
+    alloc_zero(ty, big, st);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MP_HIP(hipEventCreate(&e0));
+    MP_HIP(hipEventCreate(&e1));
+    layout.clear();
+    try {
+        for (int l = 0; l + 1 < L; ++l) {
+            AmgLevelDev& d = lv[l];
+            if (l > 0) upload_tuned(d.A, hA[l], st, tx.p, ty.p, e0, e1, layout,
+                                    ("A" + std::to_string(l)).c_str());
+            upload_tuned(d.P, hP[l], st, tx.p, ty.p, e0, e1, layout,
+                         ("P" + std::to_string(l)).c_str());
+            upload_tuned(d.R, hR[l], st, tx.p, ty.p, e0, e1, layout,
+                         ("R" + std::to_string(l)).c_str());
         }
     } catch (...) {
         (void)hipEventDestroy(e0);
@@ -387,13 +424,7 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
     }
     MP_HIP(hipEventDestroy(e0));
     MP_HIP(hipEventDestroy(e1));
-    nc = h.nc;
-    std::vector<float> g(h.coarse_inv.size());
-    for (size_t t = 0; t < g.size(); ++t) g[t] = (float)h.coarse_inv[t];
-    put(cinv, g, st);
     MP_HIP(hipStreamSynchronize(st));
-    setup_seconds =
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 void AmgDevice::apply(hipStream_t st, const double* in, double* out, int bordered,

@@ -23,9 +23,17 @@ __global__ __launch_bounds__(BLOCK) void predictor_kernel(
     const double* __restrict__ gz, const double* __restrict__ lap, const double* __restrict__ u,
     const double* __restrict__ v, const double* __restrict__ w, double* __restrict__ uh,
     double* __restrict__ vh, double* __restrict__ wh, double* __restrict__ S_new,
-    const double* __restrict__ S_oo, int npad, double rho, double mu, double dt, int euler) {
+    const double* __restrict__ S_oo, int npad, double rho_all, double mu_all, double dt_all,
+    int euler, const int* __restrict__ bprob, const double* __restrict__ prm) {
     const int r = blockIdx.x * BLOCK + threadIdx.x;
     if (r >= sv.nv) return;
+    double rho = rho_all, mu = mu_all, dt = dt_all;
+    if (bprob) {  // batch: parameters of this block's problem
+        const int q = bprob[blockIdx.x];
+        rho = prm[3 * q];
+        mu = prm[3 * q + 1];
+        dt = prm[3 * q + 2];
+    }
     const int base = row_base(sv, r);
     const int len = sv.len[r];
     double acc[DIM][4];
@@ -77,11 +85,16 @@ __global__ __launch_bounds__(BLOCK) void ppe_rhs_kernel(
     const double* __restrict__ wh, const double* __restrict__ u, const double* __restrict__ v,
     const double* __restrict__ w, const double* __restrict__ p_old,
     const uint8_t* __restrict__ bnd, const uint8_t* __restrict__ pdir,
-    const double* __restrict__ nrm, int npad, double rho, double dt, double* __restrict__ b,
-    int bordered) {
+    const double* __restrict__ nrm, int npad, double rho_all, double dt_all,
+    double* __restrict__ b, const int* __restrict__ bprob, const double* __restrict__ prm) {
     const int r = blockIdx.x * BLOCK + threadIdx.x;
-    if (bordered && r == 0) b[sv.nv] = 0.0;
-    if (r >= sv.nv) return;
+    if (r >= sv.nv) return;  // rows without a stencil (border unknowns, padding) get b = 0
+    double rho = rho_all, dt = dt_all;
+    if (bprob) {
+        const int q = bprob[blockIdx.x];
+        rho = prm[3 * q];
+        dt = prm[3 * q + 2];
+    }
     double out;
     if (bnd[r]) {
         if (pdir[r]) {
@@ -114,9 +127,16 @@ __global__ __launch_bounds__(BLOCK) void corrector_kernel(
     SellView sv, const double* __restrict__ gx, const double* __restrict__ gy,
     const double* __restrict__ gz, const double* __restrict__ p, const double* __restrict__ uh,
     const double* __restrict__ vh, const double* __restrict__ wh, double* __restrict__ un,
-    double* __restrict__ vn, double* __restrict__ wn, double rho, double dt) {
+    double* __restrict__ vn, double* __restrict__ wn, double rho_all, double dt_all,
+    const int* __restrict__ bprob, const double* __restrict__ prm) {
     const int r = blockIdx.x * BLOCK + threadIdx.x;
     if (r >= sv.nv) return;
+    double rho = rho_all, dt = dt_all;
+    if (bprob) {
+        const int q = bprob[blockIdx.x];
+        rho = prm[3 * q];
+        dt = prm[3 * q + 2];
+    }
     const int base = row_base(sv, r);
     const int len = sv.len[r];
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -285,10 +305,11 @@ template <int DIM>
 void launch_predictor(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
                       const double* gz, const double* lap, const double* u, const double* v,
                       const double* w, double* uh, double* vh, double* wh, double* S_new,
-                      const double* S_oo, int npad, double rho, double mu, double dt, int euler) {
+                      const double* S_oo, int npad, double rho, double mu, double dt, int euler,
+                      const int* bprob, const double* prm) {
     const int g = (sv.nv + BLOCK - 1) / BLOCK;
     hipLaunchKernelGGL(predictor_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, lap, u,
-                       v, w, uh, vh, wh, S_new, S_oo, npad, rho, mu, dt, euler);
+                       v, w, uh, vh, wh, S_new, S_oo, npad, rho, mu, dt, euler, bprob, prm);
     MP_HIP(hipGetLastError());
 }
 
@@ -297,10 +318,10 @@ void launch_ppe_rhs(hipStream_t s, const SellView& sv, const double* gx, const d
                     const double* gz, const double* uh, const double* vh, const double* wh,
                     const double* u, const double* v, const double* w, const double* p_old,
                     const uint8_t* bnd, const uint8_t* pdir, const double* normal_soa, int npad,
-                    double rho, double dt, double* b, int bordered) {
+                    double rho, double dt, double* b, const int* bprob, const double* prm) {
     const int g = (sv.nv + BLOCK - 1) / BLOCK;
     hipLaunchKernelGGL(ppe_rhs_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, uh, vh, wh,
-                       u, v, w, p_old, bnd, pdir, normal_soa, npad, rho, dt, b, bordered);
+                       u, v, w, p_old, bnd, pdir, normal_soa, npad, rho, dt, b, bprob, prm);
     MP_HIP(hipGetLastError());
 }
 
@@ -308,11 +329,11 @@ template <int DIM>
 void launch_corrector(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
                       const double* gz, const double* p, const double* uh, const double* vh,
                       const double* wh, double* un, double* vn, double* wn, int npad, double rho,
-                      double dt) {
+                      double dt, const int* bprob, const double* prm) {
     (void)npad;
     const int g = (sv.nv + BLOCK - 1) / BLOCK;
     hipLaunchKernelGGL(corrector_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, p, uh,
-                       vh, wh, un, vn, wn, rho, dt);
+                       vh, wh, un, vn, wn, rho, dt, bprob, prm);
     MP_HIP(hipGetLastError());
 }
 
@@ -362,16 +383,17 @@ void launch_build_operator(hipStream_t s, const SellView& sv, int dim, const dou
     template void launch_predictor<D>(hipStream_t, const SellView&, const double*, const double*, \
                                       const double*, const double*, const double*, const double*, \
                                       const double*, double*, double*, double*, double*,        \
-                                      const double*, int, double, double, double, int);         \
+                                      const double*, int, double, double, double, int,          \
+                                      const int*, const double*);                               \
     template void launch_ppe_rhs<D>(hipStream_t, const SellView&, const double*, const double*,   \
                                     const double*, const double*, const double*, const double*,   \
                                     const double*, const double*, const double*, const double*,   \
                                     const uint8_t*, const uint8_t*, const double*, int, double,   \
-                                    double, double*, int);                                        \
+                                    double, double*, const int*, const double*);                  \
     template void launch_corrector<D>(hipStream_t, const SellView&, const double*, const double*, \
                                       const double*, const double*, const double*, const double*, \
                                       const double*, double*, double*, double*, int, double,      \
-                                      double);                                                    \
+                                      double, const int*, const double*);                         \
     template void launch_boundary_fix<D>(hipStream_t, const SellView&, const int*, int,           \
                                          const double*, const double*, const double*,             \
                                          const double*, int, const uint8_t*, const uint8_t*,      \

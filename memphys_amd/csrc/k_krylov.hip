@@ -8,18 +8,24 @@
 // of ones) and p, s pass through one V-cycle each (k_amg.hip) before their
 // products, as in Eigen's preconditioned loop (y = M^-1 p, z = M^-1 s).
 //
+// Batches.  One launch serves every independent system of a batch (the
+// block-diagonal union of several point clouds): each system owns whole row
+// blocks, carries its own scalars (alpha, omega, convergence flag ...) and
+// stops on its own; a block looks up its system once and returns at entry
+// when that system is done.  A single system is the one-system batch.
+//
 // Per iteration: p-update, SpMV + partial r0.v, finish(alpha), s-update,
 // SpMV + partial (t.s, t.t), finish(omega), x/r-update + partial (r.r, r0.r),
 // finish(loop head).  Reductions are two-stage and deterministic: every block
-// stores its partial sums (plain stores, no same-address atomics), then a
-// single 1024-thread block sums them in block order and updates the scalars
-// (alpha, omega, beta, convergence flag) in device memory.  Every kernel
-// returns at entry once the flag is set, so the host launches iterations in
-// chunks and polls the flag asynchronously.
+// stores its partial sums (plain stores, no same-address atomics), then one
+// 1024-thread block per system sums that system's partials in block order and
+// updates its scalars in device memory.  The host launches iterations in
+// chunks (replayed from a hipGraph) and polls the flags asynchronously.
 //
 // The full-Neumann border (solver.cpp:205-213) is applied as a rank-one
-// term: rows i < nv add x[nv]; row nv is sum_j D^-1_j x_j, formed by the
-// finishing kernel.
+// term: the rows of a system add x[b] (b = row0 + nv, its border unknown);
+// row b is sum_j D^-1_j x_j over the system's rows, formed by the finishing
+// kernel.
 #include "krylov.hpp"
 
 #include <algorithm>
@@ -30,7 +36,7 @@ namespace mp {
 
 namespace {
 
-constexpr int FIN = 1024;  // threads of the single-block finishing kernels
+constexpr int FIN = 1024;  // threads of the finishing kernels (one block per system)
 
 // Block sums of NV values stored at partial[q * nb + lb] (lb: logical block)
 // (plain stores: no atomics, the finishing kernel runs after the grid).
@@ -54,18 +60,18 @@ __device__ __forceinline__ void block_store(double (&v)[NV], double* __restrict_
     }
 }
 
-// Sum over nb block partials per value, in a fixed order (deterministic);
-// every thread of the (single) block receives the totals.
+// Sum of the partials of blocks [b0, b0 + cnt) per value (stride nb), in a
+// fixed order; every thread of the block receives the totals.
 template <int NV>
-__device__ __forceinline__ void finish_sum(const double* __restrict__ partial, int nb,
-                                           double (&tot)[NV]) {
+__device__ __forceinline__ void finish_sum(const double* __restrict__ partial, int nb, int b0,
+                                           int cnt, double (&tot)[NV]) {
     __shared__ double sh[NV][FIN / 64];
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += FIN)
+    for (int b = threadIdx.x; b < cnt; b += FIN)
 #pragma unroll
-        for (int q = 0; q < NV; ++q) acc[q] += partial[(size_t)q * nb + b];
+        for (int q = 0; q < NV; ++q) acc[q] += partial[(size_t)q * nb + b0 + b];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; ++q)
@@ -113,7 +119,7 @@ __device__ void loop_top(KScalars* sc, double rr, double r0r) {
 }
 
 // Row r of the SELL-64 product A x: the slice width is uniform across the
-// wavefront, so the neighbour loop runs in lockstep, unrolled by four with
+// wavefront, so the neighbour loop runs in lockstep, unrolled by eight with
 // the column/value loads issued ahead of the gathers.  Padding slots hold a
 // zero value and a valid column, which leaves the fma chain unchanged.
 // Matrix streams use non-temporal loads so the gathered vector stays cached.
@@ -156,23 +162,34 @@ __device__ __forceinline__ int wave_slice(int lb) {
     return __builtin_amdgcn_readfirstlane(lb * (BLOCK / SLICE) + (int)(threadIdx.x >> 6));
 }
 
-// r = rhs - A~ x; r0 = r; p = v = 0 on a fresh start (mode 0).
+// System of logical block lb (block-uniform scalar load).
+__device__ __forceinline__ int block_seg(const SegView& sg, int lb) {
+    return __builtin_amdgcn_readfirstlane(sg.bseg[lb]);
+}
+
+// r = rhs - A~ x; r0 = r; p = v = 0 on a fresh start (mode 0, every system
+// taking part); mode 1 restarts the systems flagged with done == 3.
 template <typename TM, typename T>
 __global__ __launch_bounds__(BLOCK) void k_residual_init(
-    SellView sv, int bordered, const TM* __restrict__ a, const T* __restrict__ dinv,
+    SellView sv, SegView sg, const TM* __restrict__ a, const T* __restrict__ dinv,
     const T* __restrict__ rhs, const T* __restrict__ x, T* __restrict__ r, T* __restrict__ r0,
-    T* __restrict__ p, T* __restrict__ v, double* __restrict__ partial, int mode) {
-    const int nbk = (sv.nv + BLOCK - 1) / BLOCK;
+    T* __restrict__ p, T* __restrict__ v, double* __restrict__ partial, const KScalars* sc,
+    int mode) {
+    const int nbk = sg.nrows / BLOCK;
     const int lb = xcd_block(nbk);
     if (lb < 0) return;
+    const int q = block_seg(sg, lb);
+    const int dn = sc[q].done;
+    if (mode == 0 ? dn != 0 : dn != 3) return;
+    const Seg S = sg.seg[q];
+    const int iend = S.row0 + S.nv;
     const int i = lb * BLOCK + threadIdx.x;
     const int s = wave_slice(lb);
-    const int nv = sv.nv;
     double vals[3] = {0.0, 0.0, 0.0};  // r.r, rhs.rhs, sum D^-1 x
     if (s < sv.nslices) {
         T ax = sell_dot(sv.sptr, sv.col, a, x, s, threadIdx.x & 63);
-        if (i < nv) {
-            if (bordered) ax = ax + x[nv];
+        if (i < iend) {
+            if (S.border) ax = ax + x[iend];
             const T ri = rhs[i] - ax;
             r[i] = ri;
             r0[i] = ri;
@@ -182,45 +199,48 @@ __global__ __launch_bounds__(BLOCK) void k_residual_init(
             }
             vals[0] = (double)ri * (double)ri;
             vals[1] = (double)rhs[i] * (double)rhs[i];
-            if (bordered) vals[2] = (double)dinv[i] * (double)x[i];
+            if (S.border) vals[2] = (double)dinv[i] * (double)x[i];
         }
     }
     block_store<3>(vals, partial, lb, nbk);
 }
 
 template <typename T>
-__global__ __launch_bounds__(FIN) void k_finish_init(int nb, int nv, int bordered,
-                                                     const T* __restrict__ rhs, T* __restrict__ r,
-                                                     T* __restrict__ r0, T* __restrict__ p,
-                                                     T* __restrict__ v,
+__global__ __launch_bounds__(FIN) void k_finish_init(SegView sg, const T* __restrict__ rhs,
+                                                     T* __restrict__ r, T* __restrict__ r0,
+                                                     T* __restrict__ p, T* __restrict__ v,
                                                      const double* __restrict__ partial,
-                                                     KScalars* sc, int mode, int rel_to_rhs,
-                                                     double tol_rel) {
+                                                     KScalars* scs, int mode, int rel_to_rhs) {
+    KScalars* sc = scs + blockIdx.x;
+    const int dn = sc->done;
+    if (mode == 0 ? dn != 0 : dn != 3) return;
+    const Seg S = sg.seg[blockIdx.x];
     double tot[3];
-    finish_sum<3>(partial, nb, tot);
+    finish_sum<3>(partial, sg.nrows / BLOCK, S.blk0, S.nblk, tot);
     if (threadIdx.x != 0) return;
     double rr = tot[0], bb = tot[1];
-    if (bordered) {
-        const T rn = rhs[nv] - (T)tot[2];
-        r[nv] = rn;
-        r0[nv] = rn;
+    if (S.border) {
+        const int nb_ = S.row0 + S.nv;
+        const T rn = rhs[nb_] - (T)tot[2];
+        r[nb_] = rn;
+        r0[nb_] = rn;
         if (mode == 0) {
-            p[nv] = (T)0;
-            v[nv] = (T)0;
+            p[nb_] = (T)0;
+            v[nb_] = (T)0;
         }
         rr += (double)rn * (double)rn;
-        bb += (double)rhs[nv] * (double)rhs[nv];
+        bb += (double)rhs[nb_] * (double)rhs[nb_];
     }
     if (mode == 0) {
         sc->r0sq = rr;
         sc->bb = bb;
-        sc->tol2 = tol_rel * tol_rel * (rel_to_rhs ? bb : rr);
-        sc->eps2 = (sizeof(T) == 4) ? 1.4210854715202004e-14 : 4.930380657631324e-32;
+        sc->tol2 = sc->tol_rel * sc->tol_rel * (rel_to_rhs ? bb : rr);
+        sc->eps2 = 4.930380657631324e-32;  // Eigen: epsilon^2 of double
         sc->rho = 1.0;
         sc->alpha = 1.0;
         sc->omega = 1.0;
         sc->iter = 0;
-        sc->done = 0;
+        sc->restarts = 0;
         sc->red[0] = rr;
         sc->red[1] = bb;
         if (rel_to_rhs && bb == 0.0) {
@@ -229,136 +249,152 @@ __global__ __launch_bounds__(FIN) void k_finish_init(int nb, int nv, int bordere
         }
         loop_top(sc, rr, rr);
     } else {
+        // restart with r0 = r = rhs - A x (Eigen: the first restart also resets
+        // the iteration count)
         sc->r0sq = rr;
         sc->rho = rr;
         sc->rr = rr;
         sc->beta = (sc->rho / sc->rho_old) * (sc->alpha / sc->omega);
+        if (sc->restarts++ == 0) sc->iter = 0;
         sc->done = 0;
     }
 }
 
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_update_p(int n, const T* __restrict__ r,
+__global__ __launch_bounds__(BLOCK) void k_update_p(SegView sg, const T* __restrict__ r,
                                                     T* __restrict__ p, const T* __restrict__ v,
-                                                    const KScalars* sc, int nv,
+                                                    const KScalars* scs,
                                                     const float* __restrict__ d0,
                                                     float* __restrict__ xs) {
+    const int q = block_seg(sg, blockIdx.x);
+    const KScalars* sc = scs + q;
     if (sc->done) return;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
     const T beta = (T)sc->beta, w = (T)sc->omega;
     const T pi = r[i] + beta * (p[i] - w * v[i]);
     p[i] = pi;
-    if (xs && i < nv) xs[i] = d0[i] * (float)pi;  // smoother input of the V-cycle
+    if (xs && i < sg.seg[q].row0 + sg.seg[q].nv) xs[i] = d0[i] * (float)pi;  // V-cycle input
 }
 
-// v = A~ p; partial sums of r0.v and of D^-1 p (border row)
+// v = A~ y (y = p, or M^-1 p with the multigrid preconditioner); partial sums
+// of r0.v and of D^-1 y (border row)
 template <typename TM, typename T>
-__global__ __launch_bounds__(BLOCK) void k_spmv_alpha(SellView sv, int bordered,
+__global__ __launch_bounds__(BLOCK) void k_spmv_alpha(SellView sv, SegView sg,
                                                       const TM* __restrict__ a,
                                                       const T* __restrict__ dinv,
-                                                      const T* __restrict__ p, T* __restrict__ v,
+                                                      const T* __restrict__ y, T* __restrict__ v,
                                                       const T* __restrict__ r0,
                                                       double* __restrict__ partial,
                                                       const KScalars* sc) {
-    if (sc->done) return;
-    const int nbk = (sv.nv + BLOCK - 1) / BLOCK;
+    const int nbk = sg.nrows / BLOCK;
     const int lb = xcd_block(nbk);
     if (lb < 0) return;
+    const int q = block_seg(sg, lb);
+    if (sc[q].done) return;
+    const Seg S = sg.seg[q];
+    const int iend = S.row0 + S.nv;
     const int i = lb * BLOCK + threadIdx.x;
     const int s = wave_slice(lb);
-    const int nv = sv.nv;
     double vals[2] = {0.0, 0.0};
     if (s < sv.nslices) {
-        T y = sell_dot(sv.sptr, sv.col, a, p, s, threadIdx.x & 63);
-        if (i < nv) {
-            if (bordered) y = y + p[nv];
-            v[i] = y;
-            vals[0] = (double)r0[i] * (double)y;
-            if (bordered) vals[1] = (double)dinv[i] * (double)p[i];
+        T yv = sell_dot(sv.sptr, sv.col, a, y, s, threadIdx.x & 63);
+        if (i < iend) {
+            if (S.border) yv = yv + y[iend];
+            v[i] = yv;
+            vals[0] = (double)r0[i] * (double)yv;
+            if (S.border) vals[1] = (double)dinv[i] * (double)y[i];
         }
     }
     block_store<2>(vals, partial, lb, nbk);
 }
 
 template <typename T>
-__global__ __launch_bounds__(FIN) void k_finish_alpha(int nb, int nv, int bordered,
-                                                      T* __restrict__ v, const T* __restrict__ r0,
+__global__ __launch_bounds__(FIN) void k_finish_alpha(SegView sg, T* __restrict__ v,
+                                                      const T* __restrict__ r0,
                                                       const double* __restrict__ partial,
-                                                      KScalars* sc) {
+                                                      KScalars* scs) {
+    KScalars* sc = scs + blockIdx.x;
     if (sc->done) return;
+    const Seg S = sg.seg[blockIdx.x];
     double tot[2];
-    finish_sum<2>(partial, nb, tot);
+    finish_sum<2>(partial, sg.nrows / BLOCK, S.blk0, S.nblk, tot);
     if (threadIdx.x != 0) return;
     double r0v = tot[0];
-    if (bordered) {
+    if (S.border) {
+        const int nb_ = S.row0 + S.nv;
         const T yn = (T)tot[1];
-        v[nv] = yn;
-        r0v += (double)r0[nv] * (double)yn;
+        v[nb_] = yn;
+        r0v += (double)r0[nb_] * (double)yn;
     }
     sc->alpha = sc->rho / r0v;
     if (!(fabs(sc->alpha) <= 1.0e300)) sc->done = 5;
 }
 
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_update_s(int n, const T* __restrict__ r,
+__global__ __launch_bounds__(BLOCK) void k_update_s(SegView sg, const T* __restrict__ r,
                                                     const T* __restrict__ v, T* __restrict__ s,
-                                                    const KScalars* sc, int nv,
+                                                    const KScalars* scs,
                                                     const float* __restrict__ d0,
                                                     float* __restrict__ xs) {
+    const int q = block_seg(sg, blockIdx.x);
+    const KScalars* sc = scs + q;
     if (sc->done) return;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
     const T si = r[i] - (T)sc->alpha * v[i];
     s[i] = si;
-    if (xs && i < nv) xs[i] = d0[i] * (float)si;  // smoother input of the V-cycle
+    if (xs && i < sg.seg[q].row0 + sg.seg[q].nv) xs[i] = d0[i] * (float)si;  // V-cycle input
 }
 
 // t = A~ z (z = s, or M^-1 s with the multigrid preconditioner); partial sums
 // of t.s, t.t and D^-1 z
 template <typename TM, typename T>
-__global__ __launch_bounds__(BLOCK) void k_spmv_omega(SellView sv, int bordered,
+__global__ __launch_bounds__(BLOCK) void k_spmv_omega(SellView sv, SegView sg,
                                                       const TM* __restrict__ a,
                                                       const T* __restrict__ dinv,
                                                       const T* __restrict__ z,
                                                       const T* __restrict__ s, T* __restrict__ t,
                                                       double* __restrict__ partial,
                                                       const KScalars* sc) {
-    if (sc->done) return;
-    const int nbk = (sv.nv + BLOCK - 1) / BLOCK;
+    const int nbk = sg.nrows / BLOCK;
     const int lb = xcd_block(nbk);
     if (lb < 0) return;
+    const int q = block_seg(sg, lb);
+    if (sc[q].done) return;
+    const Seg S = sg.seg[q];
+    const int iend = S.row0 + S.nv;
     const int i = lb * BLOCK + threadIdx.x;
     const int sl = wave_slice(lb);
-    const int nv = sv.nv;
     double vals[3] = {0.0, 0.0, 0.0};
     if (sl < sv.nslices) {
         T y = sell_dot(sv.sptr, sv.col, a, z, sl, threadIdx.x & 63);
-        if (i < nv) {
-            if (bordered) y = y + z[nv];
+        if (i < iend) {
+            if (S.border) y = y + z[iend];
             t[i] = y;
             vals[0] = (double)y * (double)s[i];
             vals[1] = (double)y * (double)y;
-            if (bordered) vals[2] = (double)dinv[i] * (double)z[i];
+            if (S.border) vals[2] = (double)dinv[i] * (double)z[i];
         }
     }
     block_store<3>(vals, partial, lb, nbk);
 }
 
 template <typename T>
-__global__ __launch_bounds__(FIN) void k_finish_omega(int nb, int nv, int bordered,
-                                                      T* __restrict__ t, const T* __restrict__ s,
+__global__ __launch_bounds__(FIN) void k_finish_omega(SegView sg, T* __restrict__ t,
+                                                      const T* __restrict__ s,
                                                       const double* __restrict__ partial,
-                                                      KScalars* sc) {
+                                                      KScalars* scs) {
+    KScalars* sc = scs + blockIdx.x;
     if (sc->done) return;
+    const Seg S = sg.seg[blockIdx.x];
     double tot[3];
-    finish_sum<3>(partial, nb, tot);
+    finish_sum<3>(partial, sg.nrows / BLOCK, S.blk0, S.nblk, tot);
     if (threadIdx.x != 0) return;
     double ts = tot[0], tt = tot[1];
-    if (bordered) {
+    if (S.border) {
+        const int nb_ = S.row0 + S.nv;
         const T yn = (T)tot[2];
-        t[nv] = yn;
-        ts += (double)yn * (double)s[nv];
+        t[nb_] = yn;
+        ts += (double)yn * (double)s[nb_];
         tt += (double)yn * (double)yn;
     }
     sc->omega = (tt > 0.0) ? ts / tt : 0.0;
@@ -367,81 +403,83 @@ __global__ __launch_bounds__(FIN) void k_finish_omega(int nb, int nv, int border
 // x += alpha y + omega z (y = p, z = s without the multigrid preconditioner);
 // r = s - omega t; partial sums of r.r and r0.r
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_update_xr(int n, T* __restrict__ x, T* __restrict__ r,
-                                                     const T* __restrict__ y,
+__global__ __launch_bounds__(BLOCK) void k_update_xr(SegView sg, T* __restrict__ x,
+                                                     T* __restrict__ r, const T* __restrict__ y,
                                                      const T* __restrict__ z,
                                                      const T* __restrict__ s,
                                                      const T* __restrict__ t,
                                                      const T* __restrict__ r0,
                                                      double* __restrict__ partial,
-                                                     const KScalars* sc) {
+                                                     const KScalars* scs) {
+    const KScalars* sc = scs + block_seg(sg, blockIdx.x);
     if (sc->done) return;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
-    double vals[2] = {0.0, 0.0};
-    if (i < n) {
-        const T al = (T)sc->alpha, w = (T)sc->omega;
-        x[i] = x[i] + (al * y[i] + w * z[i]);
-        const T ri = s[i] - w * t[i];
-        r[i] = ri;
-        vals[0] = (double)ri * (double)ri;
-        vals[1] = (double)r0[i] * (double)ri;
-    }
+    const T al = (T)sc->alpha, w = (T)sc->omega;
+    x[i] = x[i] + (al * y[i] + w * z[i]);
+    const T ri = s[i] - w * t[i];
+    r[i] = ri;
+    double vals[2] = {(double)ri * (double)ri, (double)r0[i] * (double)ri};
     block_store<2>(vals, partial, blockIdx.x, gridDim.x);
 }
 
-__global__ __launch_bounds__(FIN) void k_finish_xr(int nb, const double* __restrict__ partial,
-                                                   KScalars* sc) {
+__global__ __launch_bounds__(FIN) void k_finish_xr(SegView sg, const double* __restrict__ partial,
+                                                   KScalars* scs) {
+    KScalars* sc = scs + blockIdx.x;
     if (sc->done) return;
+    const Seg S = sg.seg[blockIdx.x];
     double tot[2];
-    finish_sum<2>(partial, nb, tot);
+    finish_sum<2>(partial, sg.nrows / BLOCK, S.blk0, S.nblk, tot);
     if (threadIdx.x != 0) return;
     sc->iter += 1;
     loop_top(sc, tot[0], tot[1]);
 }
 
-// out = b - A~ x (FP64); partial sums of out.out, b.b, D^-1 x
-__global__ __launch_bounds__(BLOCK) void k_residual64(SellView sv, int bordered,
+// out = b - A~ x (FP64) for every system; partial sums of out.out, b.b, D^-1 x
+__global__ __launch_bounds__(BLOCK) void k_residual64(SellView sv, SegView sg,
                                                       const double* __restrict__ a,
                                                       const double* __restrict__ dinv,
                                                       const double* __restrict__ b,
                                                       const double* __restrict__ x,
                                                       double* __restrict__ out,
                                                       double* __restrict__ partial) {
-    const int nbk = (sv.nv + BLOCK - 1) / BLOCK;
+    const int nbk = sg.nrows / BLOCK;
     const int lb = xcd_block(nbk);
     if (lb < 0) return;
+    const Seg S = sg.seg[block_seg(sg, lb)];
+    const int iend = S.row0 + S.nv;
     const int i = lb * BLOCK + threadIdx.x;
     const int s = wave_slice(lb);
-    const int nv = sv.nv;
     double vals[3] = {0.0, 0.0, 0.0};
     if (s < sv.nslices) {
         double ax = sell_dot(sv.sptr, sv.col, a, x, s, threadIdx.x & 63);
-        if (i < nv) {
-            if (bordered) ax = ax + x[nv];
+        if (i < iend) {
+            if (S.border) ax = ax + x[iend];
             const double ri = b[i] - ax;
             out[i] = ri;
             vals[0] = ri * ri;
             vals[1] = b[i] * b[i];
-            if (bordered) vals[2] = dinv[i] * x[i];
+            if (S.border) vals[2] = dinv[i] * x[i];
         }
     }
     block_store<3>(vals, partial, lb, nbk);
 }
 
-__global__ __launch_bounds__(FIN) void k_finish_res64(int nb, int nv, int bordered,
-                                                      const double* __restrict__ b,
+__global__ __launch_bounds__(FIN) void k_finish_res64(SegView sg, const double* __restrict__ b,
                                                       double* __restrict__ out,
                                                       const double* __restrict__ partial,
-                                                      KScalars* sc) {
+                                                      KScalars* scs) {
+    KScalars* sc = scs + blockIdx.x;
+    const Seg S = sg.seg[blockIdx.x];
     double tot[3];
-    finish_sum<3>(partial, nb, tot);
+    finish_sum<3>(partial, sg.nrows / BLOCK, S.blk0, S.nblk, tot);
     if (threadIdx.x != 0) return;
     double rr = tot[0], bb = tot[1];
-    if (bordered) {
-        const double rn = b[nv] - tot[2];
-        out[nv] = rn;
+    if (S.border) {
+        const int nb_ = S.row0 + S.nv;
+        const double rn = b[nb_] - tot[2];
+        out[nb_] = rn;
         rr += rn * rn;
-        bb += b[nv] * b[nv];
+        bb += b[nb_] * b[nb_];
     }
     sc->red[0] = rr;
     sc->red[1] = bb;
@@ -462,14 +500,17 @@ __global__ __launch_bounds__(BLOCK) void k_scale(int n, const double* __restrict
         y[i] = -x[i];
 }
 
-// Clears the Krylov scalars (a kernel rather than a memset so the solve can
-// be captured in a graph) and sets the iteration cap.
-__global__ void k_sc_reset(KScalars* sc, int maxit) {
-    if (threadIdx.x == 0) {
-        KScalars z{};
-        z.maxit = maxit;
-        *sc = z;
-    }
+// Resets the scalars of every system for a new run (a kernel rather than a
+// memset, so the solve may sit in a captured graph): systems not taking part
+// are marked done = 6.
+__global__ void k_sc_reset(int nseg, KScalars* sc, const SegCtl* ctl) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nseg) return;
+    KScalars z{};
+    z.maxit = ctl[q].maxit;
+    z.tol_rel = ctl[q].tol_rel;
+    z.done = ctl[q].active ? 0 : 6;
+    sc[q] = z;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_fill(double* __restrict__ x, double v, int n) {
@@ -499,14 +540,14 @@ __global__ __launch_bounds__(BLOCK) void k_scale_columns(SellView sv, const doub
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_diag_inverse(int nv, int npad,
-                                                        const double* __restrict__ diag,
+// D^-1 of the operator's diagonal; rows without a diagonal (border unknowns,
+// padding) get 1.
+__global__ __launch_bounds__(BLOCK) void k_diag_inverse(int n, const double* __restrict__ diag,
                                                         double* __restrict__ dinv) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= npad) return;
-    double d = 1.0;
-    if (i < nv && diag[i] != 0.0) d = 1.0 / diag[i];
-    dinv[i] = d;
+    if (i >= n) return;
+    const double d = diag[i];
+    dinv[i] = (d != 0.0) ? 1.0 / d : 1.0;
 }
 
 inline int nblk(int n) { return (n + BLOCK - 1) / BLOCK; }
@@ -514,6 +555,7 @@ inline int nblk(int n) { return (n + BLOCK - 1) / BLOCK; }
 }  // namespace
 
 void launch_fill(hipStream_t s, double* x, double v, int n) {
+    if (n <= 0) return;
     hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(BLOCK), 0, s, x, v, n);
     MP_HIP(hipGetLastError());
 }
@@ -551,28 +593,49 @@ void KernelTimer::release() {
 PpeSolver::~PpeSolver() {
     release_graphs();
     if (host_sc) (void)hipHostFree(host_sc);
+    if (host_ctl) (void)hipHostFree(host_ctl);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
     spmv_timer.release();
 }
 
-void PpeSolver::setup(hipStream_t st, const SellView& view, int dim_, int npad_,
-                      const double* gx, const double* gy, const double* gz, const double* lap,
-                      const double* normal_soa, const uint8_t* bnd_dev, const uint8_t* dir_dev,
-                      bool bord, double unsteady, double conv, double diff, int precond_) {
+void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Seg>& systems,
+                      int dim_, const double* gx, const double* gy, const double* gz,
+                      const double* lap, const double* normal_soa, const uint8_t* bnd_dev,
+                      const uint8_t* dir_dev, double unsteady, double conv, double diff,
+                      int precond_) {
     release_graphs();
     sv = view;
-    nv = view.nv;
+    nrows = view.nv;
     dim = dim_;
-    npad = npad_;
-    bordered = bord ? 1 : 0;
-    n = nv + bordered;
-    const size_t vlen = (size_t)npad + 64;
+    segs = systems;
+    nseg = (int)segs.size();
+    if (nrows % BLOCK != 0 || nseg < 1) throw std::runtime_error("PpeSolver: bad row layout");
+    nv = 0;
+    n_sys = 0;
+    bordered = 0;
+    std::vector<int> hb(nrows / BLOCK, 0);
+    for (int q = 0; q < nseg; ++q) {
+        const Seg& S = segs[q];
+        if (S.blk0 * BLOCK != S.row0 || S.row0 + S.nv + S.border > (S.blk0 + S.nblk) * BLOCK)
+            throw std::runtime_error("PpeSolver: system not aligned to its row blocks");
+        for (int b = 0; b < S.nblk; ++b) hb[S.blk0 + b] = q;
+        nv += S.nv;
+        n_sys += S.nv + S.border;
+        bordered |= S.border;
+    }
+    seg_d.alloc(nseg);
+    bseg_d.alloc(hb.size());
+    MP_HIP(hipMemcpyAsync(seg_d.p, segs.data(), sizeof(Seg) * nseg, hipMemcpyHostToDevice, st));
+    MP_HIP(hipMemcpyAsync(bseg_d.p, hb.data(), sizeof(int) * hb.size(), hipMemcpyHostToDevice, st));
+    sg = SegView{nseg, nrows, seg_d.p, bseg_d.p};
+    const size_t vlen = (size_t)nrows + 64;
     DevBuf<double> raw, diag;
-    // slot count is the last entry of sptr; fetch it
-    int nsl = (nv + SLICE - 1) / SLICE;
-    int total_slots = 0;
-    MP_HIP(hipMemcpy(&total_slots, view.sptr + nsl, sizeof(int), hipMemcpyDeviceToHost));
+    const int total_slots = [&]() {
+        int t = 0;
+        MP_HIP(hipMemcpy(&t, view.sptr + view.nslices, sizeof(int), hipMemcpyDeviceToHost));
+        return t;
+    }();
     raw.alloc(total_slots);
     diag.alloc(vlen);
     aval.alloc(total_slots);
@@ -582,13 +645,13 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, int dim_, int npad_,
     MP_HIP(hipMemsetAsync(aval.p, 0, sizeof(double) * total_slots, st));
     MP_HIP(hipMemsetAsync(aval32.p, 0, sizeof(float) * total_slots, st));
     MP_HIP(hipMemsetAsync(diag.p, 0, sizeof(double) * vlen, st));
-    launch_build_operator(st, sv, dim, gx, gy, gz, lap, normal_soa, npad, bnd_dev, dir_dev,
+    launch_build_operator(st, sv, dim, gx, gy, gz, lap, normal_soa, nrows, bnd_dev, dir_dev,
                           unsteady, conv, diff, raw.p, diag.p);
-    hipLaunchKernelGGL(k_diag_inverse, dim3(nblk((int)vlen)), dim3(BLOCK), 0, st, nv, (int)vlen,
+    hipLaunchKernelGGL(k_diag_inverse, dim3(nblk((int)vlen)), dim3(BLOCK), 0, st, (int)vlen,
                        diag.p, dinv.p);
-    hipLaunchKernelGGL(k_scale_columns<double>, dim3(nblk(nv)), dim3(BLOCK), 0, st, sv, raw.p,
+    hipLaunchKernelGGL(k_scale_columns<double>, dim3(nblk(nrows)), dim3(BLOCK), 0, st, sv, raw.p,
                        dinv.p, aval.p);
-    hipLaunchKernelGGL(k_scale_columns<float>, dim3(nblk(nv)), dim3(BLOCK), 0, st, sv, raw.p,
+    hipLaunchKernelGGL(k_scale_columns<float>, dim3(nblk(nrows)), dim3(BLOCK), 0, st, sv, raw.p,
                        dinv.p, aval32.p);
     MP_HIP(hipGetLastError());
     precond = precond_;
@@ -596,6 +659,7 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, int dim_, int npad_,
     diag_inv = dinv.p;
     scaled32 = aval32.p;
     if (precond == MP_PRECOND_AMG) {
+        if (nseg != 1) throw std::runtime_error("PpeSolver: multigrid needs a single system");
         // keep A D^-1 (FP32) and D^-1 for the learned guess, then switch the
         // Krylov operator to the unscaled A with a scaling vector of ones
         jac_dinv.alloc(vlen);
@@ -604,15 +668,18 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, int dim_, int npad_,
         MP_HIP(hipMemcpyAsync(jac_a32.p, aval32.p, sizeof(float) * total_slots,
                               hipMemcpyDeviceToDevice, st));
         launch_fill(st, dinv.p, 1.0, (int)vlen);
-        hipLaunchKernelGGL(k_scale_columns<double>, dim3(nblk(nv)), dim3(BLOCK), 0, st, sv, raw.p,
-                           dinv.p, aval.p);
-        hipLaunchKernelGGL(k_scale_columns<float>, dim3(nblk(nv)), dim3(BLOCK), 0, st, sv, raw.p,
-                           dinv.p, aval32.p);
+        hipLaunchKernelGGL(k_scale_columns<double>, dim3(nblk(nrows)), dim3(BLOCK), 0, st, sv,
+                           raw.p, dinv.p, aval.p);
+        hipLaunchKernelGGL(k_scale_columns<float>, dim3(nblk(nrows)), dim3(BLOCK), 0, st, sv,
+                           raw.p, dinv.p, aval32.p);
         MP_HIP(hipGetLastError());
         diag_inv = jac_dinv.p;
         scaled32 = jac_a32.p;
+        SellView s0 = sv;
+        s0.nv = segs[0].nv;  // the real rows: the hierarchy is built on the node block
+        s0.nslices = (s0.nv + SLICE - 1) / SLICE;
         amg.reset(new AmgDevice());
-        amg->setup(st, sv, aval.p, aval32.p, mp_amg::Options{});
+        amg->setup(st, s0, aval.p, aval32.p, mp_amg::Options{});
         for (DevBuf<double>* b : {&yv, &zv}) {
             b->alloc(vlen);
             MP_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * vlen, st));
@@ -622,18 +689,25 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, int dim_, int npad_,
         b->alloc(vlen);
         MP_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * vlen, st));
     }
-    partial.alloc((size_t)(nblk(n) + 1) * 4);
-    sc.alloc(1);
-    MP_HIP(hipMemsetAsync(sc.p, 0, sizeof(KScalars), st));
-    if (!host_sc) MP_HIP(hipHostMalloc((void**)&host_sc, 2 * sizeof(KScalars), 0));
+    partial.alloc((size_t)(nrows / BLOCK) * 4);
+    sc.alloc(nseg);
+    MP_HIP(hipMemsetAsync(sc.p, 0, sizeof(KScalars) * nseg, st));
+    ctl_d.alloc(nseg);
+    if (host_sc) (void)hipHostFree(host_sc);
+    if (host_ctl) (void)hipHostFree(host_ctl);
+    host_sc = nullptr;
+    host_ctl = nullptr;
+    MP_HIP(hipHostMalloc((void**)&host_sc, 2 * sizeof(KScalars) * nseg, 0));
+    MP_HIP(hipHostMalloc((void**)&host_ctl, sizeof(SegCtl) * nseg, 0));
     for (auto& e : ev)
         if (!e) MP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // nnz of the operator (real entries): sum of row lengths
-    std::vector<int> hl(nv);
-    MP_HIP(hipMemcpyAsync(hl.data(), sv.len, sizeof(int) * nv, hipMemcpyDeviceToHost, st));
+    std::vector<int> hl(nrows);
+    MP_HIP(hipMemcpyAsync(hl.data(), sv.len, sizeof(int) * nrows, hipMemcpyDeviceToHost, st));
     MP_HIP(hipStreamSynchronize(st));
     nnz = 0;
-    for (int i = 0; i < nv; ++i) nnz += hl[i];
+    for (int i = 0; i < nrows; ++i) nnz += hl[i];
+    seg_stats.assign(nseg, SolveStats{});
 }
 
 double PpeSolver::spmv_bytes(int fp32) const {
@@ -648,40 +722,40 @@ double PpeSolver::spmv_bytes(int fp32) const {
 // The preceding update kernel has written the V-cycle's scaled FP32 input.
 const double* PpeSolver::precondition(hipStream_t st, const double* in, double* out) {
     if (!amg) return in;
-    amg->apply(st, in, out, bordered, &sc.p->done, amg->prescaled_input() != nullptr);
+    amg->apply(st, in, out, segs[0].border, &sc.p->done, amg->prescaled_input() != nullptr);
     return out;
 }
 
 // The chunk of iterations between two convergence polls, launched on the
-// stream (captured into a graph by run_bicgstab).
+// stream (captured into a graph by run_chunk).
 template <typename TM, typename T>
 void PpeSolver::launch_chunk(hipStream_t st, int chunk, const TM* a, const T* di, T* x, T* rr,
                              T* rr0, T* pp, T* vv, T* ss, T* tt) {
-    const int g_rows = nblk(nv), g_n = nblk(n);
+    const int nbk = nrows / BLOCK;
     float* xs = amg ? amg->prescaled_input() : nullptr;
     const float* d0 = amg ? amg->level0_dinv() : nullptr;
     for (int c = 0; c < chunk; ++c) {
-        hipLaunchKernelGGL(k_update_p<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, pp, vv, sc.p, nv,
-                           d0, xs);
+        hipLaunchKernelGGL(k_update_p<T>, dim3(nbk), dim3(BLOCK), 0, st, sg, rr, pp, vv, sc.p, d0,
+                           xs);
         const T* yin = precondition(st, pp, yv.p);
         spmv_timer.begin(st);
-        hipLaunchKernelGGL((k_spmv_alpha<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv,
-                           bordered, a, di, yin, vv, rr0, partial.p, sc.p);
+        hipLaunchKernelGGL((k_spmv_alpha<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, sv, sg,
+                           a, di, yin, vv, rr0, partial.p, sc.p);
         spmv_timer.end(st);
-        hipLaunchKernelGGL(k_finish_alpha<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered, vv,
-                           rr0, partial.p, sc.p);
-        hipLaunchKernelGGL(k_update_s<T>, dim3(g_n), dim3(BLOCK), 0, st, n, rr, vv, ss, sc.p, nv,
-                           d0, xs);
+        hipLaunchKernelGGL(k_finish_alpha<T>, dim3(nseg), dim3(FIN), 0, st, sg, vv, rr0, partial.p,
+                           sc.p);
+        hipLaunchKernelGGL(k_update_s<T>, dim3(nbk), dim3(BLOCK), 0, st, sg, rr, vv, ss, sc.p, d0,
+                           xs);
         const T* zin = precondition(st, ss, zv.p);
         spmv_timer.begin(st);
-        hipLaunchKernelGGL((k_spmv_omega<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv,
-                           bordered, a, di, zin, ss, tt, partial.p, sc.p);
+        hipLaunchKernelGGL((k_spmv_omega<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, sv, sg,
+                           a, di, zin, ss, tt, partial.p, sc.p);
         spmv_timer.end(st);
-        hipLaunchKernelGGL(k_finish_omega<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered, tt,
-                           ss, partial.p, sc.p);
-        hipLaunchKernelGGL(k_update_xr<T>, dim3(g_n), dim3(BLOCK), 0, st, n, x, rr, yin, zin, ss,
+        hipLaunchKernelGGL(k_finish_omega<T>, dim3(nseg), dim3(FIN), 0, st, sg, tt, ss, partial.p,
+                           sc.p);
+        hipLaunchKernelGGL(k_update_xr<T>, dim3(nbk), dim3(BLOCK), 0, st, sg, x, rr, yin, zin, ss,
                            tt, rr0, partial.p, sc.p);
-        hipLaunchKernelGGL(k_finish_xr, dim3(1), dim3(FIN), 0, st, g_n, partial.p, sc.p);
+        hipLaunchKernelGGL(k_finish_xr, dim3(nseg), dim3(FIN), 0, st, sg, partial.p, sc.p);
     }
     MP_HIP(hipGetLastError());
 }
@@ -727,180 +801,271 @@ void PpeSolver::release_graphs() {
     graphs.clear();
 }
 
+// One BiCGSTAB run for the systems with ctl[q].active (controls already in
+// host_ctl): fills done[q], iters[q], restarts[q] of the taking part systems.
 template <typename TM, typename T>
-int PpeSolver::run_bicgstab(hipStream_t st, const TM* a, const T* di, const T* rhs, T* x, T* rr,
-                            T* rr0, T* pp, T* vv, T* ss, T* tt, double tol_rel, int rel_to_rhs,
-                            int cap, int* restarts, int* done_out) {
-    const int g_rows = nblk(nv), g_n = nblk(n);
-    hipLaunchKernelGGL(k_sc_reset, dim3(1), dim3(64), 0, st, sc.p, cap);
-    hipLaunchKernelGGL((k_residual_init<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv, bordered, a, di,
-                       rhs, x, rr, rr0, pp, vv, partial.p, 0);
-    hipLaunchKernelGGL(k_finish_init<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv, bordered, rhs, rr,
-                       rr0, pp, vv, partial.p, sc.p, 0, rel_to_rhs, tol_rel);
+void PpeSolver::run_bicgstab(hipStream_t st, const TM* a, const T* di, const T* rhs, T* x, T* rr,
+                             T* rr0, T* pp, T* vv, T* ss, T* tt, int rel_to_rhs,
+                             std::vector<int>& done, std::vector<int>& iters,
+                             std::vector<int>& restarts) {
+    const int nbk = nrows / BLOCK;
+    MP_HIP(hipMemcpyAsync(ctl_d.p, host_ctl, sizeof(SegCtl) * nseg, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_sc_reset, dim3((nseg + 63) / 64), dim3(64), 0, st, nseg, sc.p, ctl_d.p);
+    hipLaunchKernelGGL((k_residual_init<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, sv, sg,
+                       a, di, rhs, x, rr, rr0, pp, vv, partial.p, sc.p, 0);
+    hipLaunchKernelGGL(k_finish_init<T>, dim3(nseg), dim3(FIN), 0, st, sg, rhs, rr, rr0, pp, vv,
+                       partial.p, sc.p, 0, rel_to_rhs);
     MP_HIP(hipGetLastError());
     const int chunk = std::max(1, check_every);
-    int slot = 0, result = 0;
+    int slot = 0;
     bool have_prev = false;
-    *restarts = 0;
+    KScalars* hs[2] = {host_sc, host_sc + nseg};
     while (true) {
         run_chunk<TM, T>(st, chunk, a, di, x, rr, rr0, pp, vv, ss, tt);
-        MP_HIP(hipGetLastError());
-        MP_HIP(hipMemcpyAsync(&host_sc[slot], sc.p, sizeof(KScalars), hipMemcpyDeviceToHost, st));
+        MP_HIP(hipMemcpyAsync(hs[slot], sc.p, sizeof(KScalars) * nseg, hipMemcpyDeviceToHost, st));
         MP_HIP(hipEventRecord(ev[slot], st));
         if (have_prev) {
             const int ps = slot ^ 1;
             MP_HIP(hipEventSynchronize(ev[ps]));
-            const KScalars& h = host_sc[ps];
-            if (h.done == 3) {
-                // breakdown of r0.r: wait for the in-flight chunk (no-ops), then
-                // restart with r0 = r = rhs - A x, as Eigen's BiCGSTAB does
+            bool restart = false, running = false;
+            for (int q = 0; q < nseg; ++q) {
+                restart |= hs[ps][q].done == 3;
+                running |= hs[ps][q].done == 0;
+            }
+            if (restart) {
+                // breakdown of r0.r in some system: once the in-flight chunk has
+                // run, restart those systems with r0 = r = rhs - A x, as Eigen's
+                // BiCGSTAB does (the kernels act on the systems still at done == 3)
                 MP_HIP(hipEventSynchronize(ev[slot]));
-                hipLaunchKernelGGL((k_residual_init<TM, T>), dim3(grid_xcd(g_rows)), dim3(BLOCK), 0, st, sv,
-                                   bordered, a, di, rhs, x, rr, rr0, pp, vv, partial.p, 1);
-                hipLaunchKernelGGL(k_finish_init<T>, dim3(1), dim3(FIN), 0, st, g_rows, nv,
-                                   bordered, rhs, rr, rr0, pp, vv, partial.p, sc.p, 1, rel_to_rhs,
-                                   tol_rel);
-                if ((*restarts)++ == 0) {
-                    int zero = 0;
-                    MP_HIP(hipMemcpyAsync(&sc.p->iter, &zero, sizeof(int), hipMemcpyHostToDevice,
-                                          st));
-                }
+                hipLaunchKernelGGL((k_residual_init<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0,
+                                   st, sv, sg, a, di, rhs, x, rr, rr0, pp, vv, partial.p, sc.p, 1);
+                hipLaunchKernelGGL(k_finish_init<T>, dim3(nseg), dim3(FIN), 0, st, sg, rhs, rr,
+                                   rr0, pp, vv, partial.p, sc.p, 1, rel_to_rhs);
+                MP_HIP(hipGetLastError());
                 have_prev = false;
                 slot ^= 1;
                 continue;
             }
-            if (h.done) {
+            if (!running) {
                 MP_HIP(hipEventSynchronize(ev[slot]));
-                result = host_sc[slot].done ? host_sc[slot].done : h.done;
                 break;
             }
         }
         have_prev = true;
         slot ^= 1;
     }
-    *done_out = result;
-    return host_sc[slot].iter;
+    for (int q = 0; q < nseg; ++q) {
+        const KScalars& h = hs[slot][q];
+        if (!host_ctl[q].active) continue;
+        done[q] = h.done;
+        iters[q] = h.iter;
+        restarts[q] = h.restarts;
+    }
 }
 
-void PpeSolver::residual64(hipStream_t st, const double* b_dev, double* out_rr, double* out_bb) {
-    const int g = nblk(nv);
-    hipLaunchKernelGGL(k_residual64, dim3(grid_xcd(g)), dim3(BLOCK), 0, st, sv, bordered, aval.p, dinv.p,
-                       b_dev, xt.p, tmp.p, partial.p);
-    hipLaunchKernelGGL(k_finish_res64, dim3(1), dim3(FIN), 0, st, g, nv, bordered, b_dev, tmp.p,
-                       partial.p, sc.p);
+// Per-system FP64 residual |b - A x~|^2 and |b|^2 of xt into tmp; waits.
+void PpeSolver::residual64(hipStream_t st, const double* b_dev, std::vector<double>& rr,
+                           std::vector<double>& bb) {
+    const int nbk = nrows / BLOCK;
+    hipLaunchKernelGGL(k_residual64, dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, sv, sg, aval.p,
+                       dinv.p, b_dev, xt.p, tmp.p, partial.p);
+    hipLaunchKernelGGL(k_finish_res64, dim3(nseg), dim3(FIN), 0, st, sg, b_dev, tmp.p, partial.p,
+                       sc.p);
     MP_HIP(hipGetLastError());
-    MP_HIP(hipMemcpyAsync(host_sc, sc.p, sizeof(KScalars), hipMemcpyDeviceToHost, st));
+    MP_HIP(hipMemcpyAsync(host_sc, sc.p, sizeof(KScalars) * nseg, hipMemcpyDeviceToHost, st));
     MP_HIP(hipStreamSynchronize(st));
-    *out_rr = host_sc[0].red[0];
-    *out_bb = host_sc[0].red[1];
+    rr.resize(nseg);
+    bb.resize(nseg);
+    for (int q = 0; q < nseg; ++q) {
+        rr[q] = host_sc[q].red[0];
+        bb[q] = host_sc[q].red[1];
+    }
 }
 
 double* PpeSolver::residual_of(hipStream_t st, const double* b_dev, const double* x_dev) {
-    const int g_n = nblk(n);
-    hipLaunchKernelGGL(k_scale, dim3(g_n), dim3(BLOCK), 0, st, n, dinv.p, x_dev, xt.p, 1);
-    const int g = nblk(nv);
-    hipLaunchKernelGGL(k_residual64, dim3(grid_xcd(g)), dim3(BLOCK), 0, st, sv, bordered, aval.p,
+    const int nbk = nrows / BLOCK;
+    hipLaunchKernelGGL(k_scale, dim3(nbk), dim3(BLOCK), 0, st, nrows, dinv.p, x_dev, xt.p, 1);
+    hipLaunchKernelGGL(k_residual64, dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, sv, sg, aval.p,
                        dinv.p, b_dev, xt.p, tmp.p, partial.p);
-    hipLaunchKernelGGL(k_finish_res64, dim3(1), dim3(FIN), 0, st, g, nv, bordered, b_dev, tmp.p,
-                       partial.p, sc.p);
+    hipLaunchKernelGGL(k_finish_res64, dim3(nseg), dim3(FIN), 0, st, sg, b_dev, tmp.p, partial.p,
+                       sc.p);
     MP_HIP(hipGetLastError());
     return tmp.p;
 }
 
+void PpeSolver::zero_system(hipStream_t st, double* x, int q) {
+    launch_fill(st, x + segs[q].row0, 0.0, segs[q].nv + segs[q].border);
+}
+
 SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) {
-    SolveStats out;
-    const int g_n = nblk(n);
+    const int nbk = nrows / BLOCK;
+    std::vector<SolveStats> out(nseg);
     // x~ = D x (guess)
-    hipLaunchKernelGGL(k_scale, dim3(g_n), dim3(BLOCK), 0, st, n, dinv.p, x_dev, xt.p, 1);
+    hipLaunchKernelGGL(k_scale, dim3(nbk), dim3(BLOCK), 0, st, nrows, dinv.p, x_dev, xt.p, 1);
     MP_HIP(hipGetLastError());
-    double rr0 = 0, bb = 0;
-    residual64(st, b_dev, &rr0, &bb);
-    out.guess_rel_res = (bb > 0) ? std::sqrt(rr0 / bb) : 0.0;
-    // FP64 BiCGSTAB from the current x~.  BiCGSTAB stops on its recursively
-    // updated residual, which drifts from b - A x by rounding; when the true
-    // residual misses the tolerance the iteration restarts from the current x.
-    auto fp64_solve = [&](int budget, int& total, int& rs_total, int& cycles) {
-        int used = 0;
+    std::vector<double> rr0, bb;
+    residual64(st, b_dev, rr0, bb);
+    for (int q = 0; q < nseg; ++q)
+        out[q].guess_rel_res = (bb[q] > 0) ? std::sqrt(rr0[q] / bb[q]) : 0.0;
+    std::vector<int> total(nseg, 0), cycles(nseg, 0), rs_total(nseg, 0);
+    std::vector<int> done(nseg, 0), its(nseg, 0), rst(nseg, 0);
+    // FP64 BiCGSTAB from the current x~ for the systems in `todo` (budget:
+    // iterations each).  BiCGSTAB stops on its recursively updated residual,
+    // which drifts from b - A x by rounding; a system whose true residual misses
+    // the tolerance restarts from its current x (at most 6 runs).
+    auto fp64_solve = [&](std::vector<char> todo, const std::vector<int>& budget) {
+        std::vector<int> used(nseg, 0), runs(nseg, 0);
         while (true) {
-            int restarts = 0, done = 0;
-            int it = run_bicgstab<double, double>(st, aval.p, dinv.p, b_dev, xt.p, r.p, r0.p, p.p, v.p,
-                                          s.p, t.p, tol, 1, budget - used, &restarts, &done);
-            used += it;
-            total += it;
-            rs_total += restarts;
-            cycles++;
-            if (done == 4) {
-                launch_fill(st, xt.p, 0.0, n);
-                break;
+            bool any = false;
+            for (int q = 0; q < nseg; ++q) {
+                host_ctl[q].active = todo[q] ? 1 : 0;
+                host_ctl[q].maxit = budget[q] - used[q];
+                host_ctl[q].tol_rel = tol;
+                any |= todo[q] != 0;
             }
-            if (!(done == 1 || done == 5) || cycles >= 6 || used >= budget) break;
-            double rr_true = 0, bb_true = 0;
-            residual64(st, b_dev, &rr_true, &bb_true);
-            if (!(rr_true > tol * tol * bb_true)) break;  // converged, or non-finite
+            if (!any) break;
+            run_bicgstab<double, double>(st, aval.p, dinv.p, b_dev, xt.p, r.p, r0.p, p.p, v.p,
+                                         s.p, t.p, 1, done, its, rst);
+            bool check = false;
+            for (int q = 0; q < nseg; ++q) {
+                if (!todo[q]) continue;
+                used[q] += its[q];
+                total[q] += its[q];
+                rs_total[q] += rst[q];
+                cycles[q]++;
+                runs[q]++;
+                if (done[q] == 4) {
+                    zero_system(st, xt.p, q);
+                    todo[q] = 0;
+                } else if (!(done[q] == 1 || done[q] == 5) || runs[q] >= 6 ||
+                           used[q] >= budget[q]) {
+                    todo[q] = 0;
+                } else {
+                    check = true;
+                }
+            }
+            if (!check) break;
+            std::vector<double> rt, bt;
+            residual64(st, b_dev, rt, bt);
+            for (int q = 0; q < nseg; ++q)
+                if (todo[q] && !(rt[q] > tol * tol * bt[q])) todo[q] = 0;  // converged, or non-finite
         }
     };
-    int total = 0, cycles = 0, rs_total = 0;
     if (precision == 0) {
-        fp64_solve(maxit, total, rs_total, cycles);
+        fp64_solve(std::vector<char>(nseg, 1), std::vector<int>(nseg, maxit));
     } else {
         // FP64 defect correction around an inner BiCGSTAB that streams the FP32
         // copy of the operator (8 instead of 12 bytes per entry) while keeping
         // FP64 vectors and reductions: each cycle solves A32 e = r with
         // r = b - A x~ from the FP64 operator, to a relative tolerance
-        // max(inner_tol, tol |b| / 2|r|), then x~ += e.  A cycle that breaks
-        // down or stalls hands the rest of the solve to FP64 BiCGSTAB.
-        double rr = rr0, rr_prev = rr0;
-        const double tol2 = tol * tol * bb;
-        bool fallback = false;
-        while (rr > tol2 && total < maxit && bb > 0) {
-            MP_HIP(hipMemcpyAsync(rhs64.p, tmp.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-            launch_fill(st, e64.p, 0.0, n);
-            const double tau = std::max(inner_tol, 0.5 * std::sqrt(tol2 / rr));
-            int restarts = 0, done = 0;
-            int it = run_bicgstab<float, double>(st, aval32.p, dinv.p, rhs64.p, e64.p, r.p, r0.p,
-                                                 p.p, v.p, s.p, t.p, tau, 1, maxit - total,
-                                                 &restarts, &done);
-            total += it;
-            rs_total += restarts;
-            cycles++;
-            if (done == 5 || it == 0) {
-                fallback = true;
-                break;
+        // max(inner_tol, tol |b| / 2|r|), then x~ += e.  A system whose cycle
+        // breaks down or stalls hands the rest of its solve to FP64 BiCGSTAB.
+        std::vector<double> rr = rr0, rr_prev = rr0, tol2(nseg);
+        std::vector<char> act(nseg, 0), fallback(nseg, 0);
+        for (int q = 0; q < nseg; ++q) {
+            tol2[q] = tol * tol * bb[q];
+            act[q] = (rr[q] > tol2[q] && bb[q] > 0) ? 1 : 0;
+        }
+        while (true) {
+            bool any = false;
+            for (int q = 0; q < nseg; ++q) {
+                act[q] = act[q] && total[q] < maxit;
+                host_ctl[q].active = act[q];
+                host_ctl[q].maxit = maxit - total[q];
+                host_ctl[q].tol_rel = act[q] ? std::max(inner_tol, 0.5 * std::sqrt(tol2[q] / rr[q]))
+                                             : 1.0;
+                any |= act[q] != 0;
             }
-            hipLaunchKernelGGL(k_axpy, dim3(g_n), dim3(BLOCK), 0, st, n, 1.0, e64.p, xt.p);
+            if (!any) break;
+            MP_HIP(hipMemcpyAsync(rhs64.p, tmp.p, sizeof(double) * nrows, hipMemcpyDeviceToDevice,
+                                  st));
+            launch_fill(st, e64.p, 0.0, nrows);
+            run_bicgstab<float, double>(st, aval32.p, dinv.p, rhs64.p, e64.p, r.p, r0.p, p.p, v.p,
+                                        s.p, t.p, 1, done, its, rst);
+            for (int q = 0; q < nseg; ++q) {
+                if (!act[q]) continue;
+                total[q] += its[q];
+                rs_total[q] += rst[q];
+                cycles[q]++;
+                if (done[q] == 5 || its[q] == 0) {
+                    fallback[q] = 1;  // the correction of this cycle is dropped
+                    act[q] = 0;
+                    zero_system(st, e64.p, q);
+                }
+            }
+            // e is zero on the systems that took no part
+            hipLaunchKernelGGL(k_axpy, dim3(nbk), dim3(BLOCK), 0, st, nrows, 1.0, e64.p, xt.p);
             MP_HIP(hipGetLastError());
-            residual64(st, b_dev, &rr, &bb);
-            if (!(rr < 0.81 * rr_prev)) {  // less than a 10% reduction of |r|
-                fallback = true;
-                break;
+            residual64(st, b_dev, rr, bb);
+            for (int q = 0; q < nseg; ++q) {
+                if (!act[q]) continue;
+                if (!(rr[q] < 0.81 * rr_prev[q])) {  // less than a 10% reduction of |r|
+                    fallback[q] = 1;
+                    act[q] = 0;
+                    continue;
+                }
+                rr_prev[q] = rr[q];
+                act[q] = rr[q] > tol2[q];
             }
-            rr_prev = rr;
         }
-        if (fallback && total < maxit) {
-            fp64_fallbacks++;
-            fp64_solve(maxit - total, total, rs_total, cycles);
+        std::vector<char> fb(nseg, 0);
+        std::vector<int> budget(nseg, 0);
+        bool anyfb = false;
+        for (int q = 0; q < nseg; ++q) {
+            fb[q] = fallback[q] && total[q] < maxit;
+            budget[q] = maxit - total[q];
+            if (fb[q]) {
+                fp64_fallbacks++;
+                anyfb = true;
+            }
         }
-        if (bb == 0) launch_fill(st, xt.p, 0.0, n);
+        if (anyfb) fp64_solve(fb, budget);
+        for (int q = 0; q < nseg; ++q)
+            if (bb[q] == 0) zero_system(st, xt.p, q);
     }
-    out.iterations = total;
-    out.restarts = rs_total;
-    out.outer_cycles = cycles;
     // x = D^-1 x~ and the logged residual |A x - b| (solver.cpp:342-345)
-    double rr = 0;
-    residual64(st, b_dev, &rr, &bb);
-    hipLaunchKernelGGL(k_scale, dim3(g_n), dim3(BLOCK), 0, st, n, dinv.p, xt.p, x_dev, 0);
+    std::vector<double> rr;
+    residual64(st, b_dev, rr, bb);
+    hipLaunchKernelGGL(k_scale, dim3(nbk), dim3(BLOCK), 0, st, nrows, dinv.p, xt.p, x_dev, 0);
     MP_HIP(hipGetLastError());
-    out.abs_res = std::sqrt(rr);
-    out.rel_res = (bb > 0) ? std::sqrt(rr / bb) : 0.0;
-    out.converged = (out.rel_res <= tol) ? 1 : 0;
-    if (bordered) {
-        MP_HIP(hipMemcpyAsync(&out.alpha, x_dev + nv, sizeof(double), hipMemcpyDeviceToHost, st));
-    }
+    std::vector<double> alpha(nseg, 0.0);
+    for (int q = 0; q < nseg; ++q)
+        if (segs[q].border)
+            MP_HIP(hipMemcpyAsync(&alpha[q], x_dev + segs[q].row0 + segs[q].nv, sizeof(double),
+                                  hipMemcpyDeviceToHost, st));
     MP_HIP(hipStreamSynchronize(st));
     if (spmv_timer.enabled) spmv_timer.harvest();
-    total_iterations += out.iterations;
+    SolveStats agg;
+    double rr_all = 0, bb_all = 0;
+    agg.converged = 1;
+    for (int q = 0; q < nseg; ++q) {
+        SolveStats& o = out[q];
+        o.iterations = total[q];
+        o.restarts = rs_total[q];
+        o.outer_cycles = cycles[q];
+        o.abs_res = std::sqrt(rr[q]);
+        o.rel_res = (bb[q] > 0) ? std::sqrt(rr[q] / bb[q]) : 0.0;
+        o.converged = (o.rel_res <= tol) ? 1 : 0;
+        o.alpha = alpha[q];
+        agg.iterations = std::max(agg.iterations, o.iterations);
+        agg.restarts += o.restarts;
+        agg.outer_cycles = std::max(agg.outer_cycles, o.outer_cycles);
+        agg.converged &= o.converged;
+        agg.rel_res = std::max(agg.rel_res, o.rel_res);
+        agg.guess_rel_res = std::max(agg.guess_rel_res, o.guess_rel_res);
+        rr_all += rr[q];
+        bb_all += bb[q];
+        total_iterations += o.iterations;
+    }
+    if (nseg == 1) {
+        agg = out[0];
+    } else {
+        agg.abs_res = std::sqrt(rr_all);
+        agg.alpha = 0.0;
+    }
     total_solves += 1;
-    return out;
+    seg_stats = out;
+    return agg;
 }
 
 }  // namespace mp

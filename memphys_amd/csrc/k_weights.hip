@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void weights_fp64_kernel(
     __shared__ int s_e[MAX_POLY * 3];
 
     const int r = rows ? rows[blockIdx.x] : (int)blockIdx.x;
-    if (r >= sv.nv) return;
+    if (r >= sv.nv || sv.len[r] == 0) return;  // padding rows carry no stencil
     const int tid = threadIdx.x;
     const int k = sv.len[r];
     const int M = k + m;

@@ -33,13 +33,22 @@ struct ChunkGraph {
 };
 
 // BiCGSTAB (as Eigen 3.3.7's BiCGSTAB.h) on the right-Jacobi-scaled operator
-// A D^-1, bordered with a ones row/column in the full-Neumann case.  With the
-// multigrid preconditioner the operator is A itself (the scaling vector is
-// all ones) and every p and s is preconditioned by one V-cycle.
+// A D^-1, bordered with a ones row/column in the full-Neumann case, for one
+// system or a batch of independent systems (the block-diagonal union of
+// several clouds, each with its own scalars and stopping test).  With the
+// multigrid preconditioner (one system) the operator is A itself (the scaling
+// vector is all ones) and every p and s is preconditioned by one V-cycle.
 struct PpeSolver {
-    int nv = 0, npad = 0, n = 0, bordered = 0, dim = 0;
-    long nnz = 0;  // real stencil entries of the operator
+    int nrows = 0;   // rows of all systems incl. padding (multiple of BLOCK)
+    int nv = 0;      // real rows of all systems
+    int n_sys = 0;   // unknowns: real rows + border unknowns
+    int nseg = 1, bordered = 0, dim = 0;
+    long nnz = 0;    // real stencil entries of the operator
     SellView sv{};
+    std::vector<Seg> segs;
+    DevBuf<Seg> seg_d;
+    DevBuf<int> bseg_d;
+    SegView sg{};
     DevBuf<double> aval, dinv;
     DevBuf<float> aval32;
     DevBuf<double> xt, r, r0, p, v, s, t, tmp, e64, rhs64;
@@ -51,36 +60,45 @@ struct PpeSolver {
     int precond = 0;                  // enum mp_precond
     std::unique_ptr<AmgDevice> amg;
     DevBuf<double> partial;
-    DevBuf<KScalars> sc;
-    KScalars* host_sc = nullptr;  // pinned, 2 slots
+    DevBuf<KScalars> sc;              // [nseg]
+    DevBuf<SegCtl> ctl_d;             // [nseg]
+    KScalars* host_sc = nullptr;      // pinned, 2 x nseg
+    SegCtl* host_ctl = nullptr;       // pinned, nseg
     hipEvent_t ev[2] = {nullptr, nullptr};
     int precision = 0;
     double tol = 1e-8, inner_tol = 1e-3;
-    int maxit = 1000, check_every = 8, inner_maxit = 1000;
+    int maxit = 1000, check_every = 8;
     long fp64_fallbacks = 0;
     KernelTimer spmv_timer;
     bool use_graphs = true;           // replay each chunk of iterations from a hipGraph
     std::vector<ChunkGraph> graphs;
     long total_iterations = 0, total_solves = 0;
+    std::vector<SolveStats> seg_stats;  // per system, of the last solve
 
     ~PpeSolver();
-    // Builds the operator from weights in SELL slots.
-    void setup(hipStream_t s, const SellView& view, int dim, int npad, const double* gx,
-               const double* gy, const double* gz, const double* lap, const double* normal_soa,
-               const uint8_t* bnd_dev, const uint8_t* dir_dev, bool bordered, double unsteady,
-               double conv, double diff, int precond = 0);
-    // b_dev, x_dev: device order, length n (guess in x_dev, solution out).
+    // Builds the operator from weights in SELL slots; view.nv = nrows.
+    void setup(hipStream_t s, const SellView& view, const std::vector<Seg>& systems, int dim,
+               const double* gx, const double* gy, const double* gz, const double* lap,
+               const double* normal_soa, const uint8_t* bnd_dev, const uint8_t* dir_dev,
+               double unsteady, double conv, double diff, int precond = 0);
+    // b_dev, x_dev: device rows (guess in x_dev, solution out).  Returns the
+    // statistics of a single system, or the worst over a batch (per system in
+    // seg_stats).
     SolveStats solve(hipStream_t s, const double* b_dev, double* x_dev);
     double spmv_bytes(int fp32) const;
     // r = b - A x (FP64, unscaled operator incl. border) into `tmp`; returns tmp.p
     double* residual_of(hipStream_t s, const double* b_dev, const double* x_dev);
+    // Drops the captured chunk graphs (their kernel arguments point into the
+    // operator and preconditioner buffers).
+    void release_graphs();
 
   private:
     template <typename TM, typename T>
-    int run_bicgstab(hipStream_t s, const TM* aval_t, const T* dinv_t, const T* rhs, T* x, T* rr,
-                     T* rr0, T* pp, T* vv, T* ss, T* tt, double tol_rel, int rel_to_rhs, int maxit,
-                     int* restarts, int* done_out);
-    void residual64(hipStream_t s, const double* b_dev, double* out_rr, double* out_bb);
+    void run_bicgstab(hipStream_t s, const TM* aval_t, const T* dinv_t, const T* rhs, T* x,
+                      T* rr, T* rr0, T* pp, T* vv, T* ss, T* tt, int rel_to_rhs,
+                      std::vector<int>& done, std::vector<int>& iters, std::vector<int>& restarts);
+    void residual64(hipStream_t s, const double* b_dev, std::vector<double>& rr,
+                    std::vector<double>& bb);
     const double* precondition(hipStream_t s, const double* in, double* out);
     template <typename TM, typename T>
     void launch_chunk(hipStream_t s, int chunk, const TM* a, const T* di, T* x, T* rr, T* rr0,
@@ -88,7 +106,7 @@ struct PpeSolver {
     template <typename TM, typename T>
     void run_chunk(hipStream_t s, int chunk, const TM* a, const T* di, T* x, T* rr, T* rr0,
                    T* pp, T* vv, T* ss, T* tt);
-    void release_graphs();
+    void zero_system(hipStream_t s, double* x, int q);
 };
 
 }  // namespace mp

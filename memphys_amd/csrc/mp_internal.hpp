@@ -84,15 +84,36 @@ struct SellView {
     const int* len;    // [nslices*64] row lengths (0 on ghost rows)
 };
 
-// Device-side Krylov scalars (FP64).  Updated only by the last block of a
-// reduction, read by every block of the following kernels.
+// Device-side Krylov scalars (FP64) of one system.  Written only by the
+// single-block finishing kernels, read by every block of the following kernels.
 struct KScalars {
     double rho, rho_old, alpha, omega, beta;
-    double r0sq, bb, rr, tol2, eps2;
-    double red[4];          // scratch results of the last reduction
-    int done;               // 1 converged, 2 iteration cap, 3 breakdown (restart needed)
-    int iter, maxit, pad;
-    unsigned int counter[4];
+    double r0sq, bb, rr, tol2, eps2, tol_rel;
+    double red[4];  // results of the last residual reduction
+    // 0 running, 1 converged, 2 iteration cap, 3 restart needed, 4 zero
+    // right-hand side, 5 breakdown / non-finite, 6 not part of this solve
+    int done;
+    int iter, maxit, restarts;
+};
+
+// One independent system of a batch: real rows [row0, row0 + nv), its border
+// unknown (full-Neumann Lagrange multiplier) at row row0 + nv when border != 0,
+// and the row blocks [blk0, blk0 + nblk) of BLOCK rows it owns.  Systems start
+// on block boundaries, so a block never mixes two systems.
+struct Seg {
+    int row0, nv, blk0, nblk, border;
+};
+
+struct SegView {
+    int nseg, nrows;    // systems; rows of all systems incl. padding (multiple of BLOCK)
+    const Seg* seg;     // [nseg]
+    const int* bseg;    // [nrows / BLOCK] system of each row block
+};
+
+// Per-system controls of one Krylov run (uploaded before it starts).
+struct SegCtl {
+    int active, maxit;
+    double tol_rel;
 };
 
 // ---- launchers (defined in the .hip files) ----
@@ -111,18 +132,20 @@ template <int DIM>
 void launch_predictor(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
                       const double* gz, const double* lap, const double* u, const double* v,
                       const double* w, double* uh, double* vh, double* wh, double* S_new,
-                      const double* S_oo, int npad, double rho, double mu, double dt, int euler);
+                      const double* S_oo, int npad, double rho, double mu, double dt, int euler,
+                      const int* bprob = nullptr, const double* prm = nullptr);
 template <int DIM>
 void launch_ppe_rhs(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
                     const double* gz, const double* uh, const double* vh, const double* wh,
                     const double* u, const double* v, const double* w, const double* p_old,
                     const uint8_t* bnd, const uint8_t* pdir, const double* normal_soa,
-                    int npad, double rho, double dt, double* b, int bordered);
+                    int npad, double rho, double dt, double* b, const int* bprob = nullptr,
+                    const double* prm = nullptr);
 template <int DIM>
 void launch_corrector(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
                       const double* gz, const double* p, const double* uh, const double* vh,
                       const double* wh, double* un, double* vn, double* wn, int npad, double rho,
-                      double dt);
+                      double dt, const int* bprob = nullptr, const double* prm = nullptr);
 template <int DIM>
 void launch_boundary_fix(hipStream_t s, const SellView& sv, const int* brows, int nb,
                          const double* gx, const double* gy, const double* gz,
@@ -144,7 +167,6 @@ void launch_build_operator(hipStream_t s, const SellView& sv, int dim, const dou
                            double* aval, double* diag);
 
 // Krylov building blocks (k_krylov.hip).
-struct KrylovBufs;
 void launch_fill(hipStream_t s, double* x, double v, int n);
 
 }  // namespace mp

@@ -1,0 +1,125 @@
+"""GPU: batched independent cavities in one context against the same problems
+solved one context each (and one of them against the oracle).
+
+Each problem of a batch owns whole row blocks and its own Krylov scalars;
+its reductions run over its own blocks in the same order as when it is alone,
+so a batched problem must reproduce its stand-alone run bit for bit: fields,
+pressure border unknown and iteration counts.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _problems():
+    import memphys_amd as M
+    out = []
+    for n, re, seed in ((24, 60.0, 11), (31, 200.0, 12), (40, 350.0, 13)):
+        cloud = M.box_cloud(dim=2, n=n, jitter=0.25, seed=seed, k_interior=30)
+        h = 1.0 / (n - 1)
+        out.append((cloud, re, min(0.5 * h * h * re / 33.7, 0.2 * h)))
+    return out
+
+
+def _run_single(cloud, re, dt, steps, precision):
+    import memphys_amd as M
+    with M.Context(0) as ctx:
+        ctx.upload(cloud)
+        ctx.weights(3, 3, download=False)
+        u_dir, p_dir, u0 = M.lid_conditions(cloud)
+        ctx.fs_create(1.0, 1.0 / re, dt, 2, 1e-10, 20000, u_dir, u_dir, None, p_dir,
+                      precision=precision)
+        ctx.set_field(M.MP_U, u0)
+        ctx.set_field(M.MP_V, np.zeros(cloud.nv))
+        ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
+        iters = []
+        for it in range(steps):
+            ctx.step(1, it)
+            iters.append(ctx.stats().iterations)
+        return ([ctx.get_field(w) for w in (M.MP_U, M.MP_V, M.MP_P)], iters)
+
+
+def _run_batch(probs, steps, precision):
+    import memphys_amd as M
+    with M.Context(0) as ctx:
+        cat = ctx.upload_batch([p[0] for p in probs])
+        ctx.weights(3, 3, download=False)
+        u_dir, p_dir, u0 = M.lid_conditions(cat)
+        ctx.fs_create(1.0, 0.01, 1e-4, 2, 1e-10, 20000, u_dir, u_dir, None, p_dir,
+                      precision=precision)
+        ctx.batch_params([1.0] * len(probs), [1.0 / p[1] for p in probs], [p[2] for p in probs])
+        assert ctx.n_sys == cat.nv + len(probs)  # one border unknown per cavity
+        ctx.set_field(M.MP_U, u0)
+        ctx.set_field(M.MP_V, np.zeros(cat.nv))
+        ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
+        iters = []
+        for it in range(steps):
+            ctx.step(1, it)
+            iters.append([ctx.batch_stats(q).iterations for q in range(len(probs))])
+        u, v, p = (ctx.get_field(w) for w in (M.MP_U, M.MP_V, M.MP_P))
+    out = []
+    off = 0
+    for q, (cloud, _, _) in enumerate(probs):
+        sl = slice(off, off + cloud.nv)
+        out.append(([u[sl], v[sl], np.append(p[sl], p[cat.nv + q])],
+                    [row[q] for row in iters]))
+        off += cloud.nv
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_reproduces_single_problems(precision):
+    probs = _problems()
+    steps = 5
+    batch = _run_batch(probs, steps, precision)
+    for q, (cloud, re, dt) in enumerate(probs):
+        fields, iters = _run_single(cloud, re, dt, steps, precision)
+        bf, bi = batch[q]
+        _print(f"precision {precision} problem {q} nv {cloud.nv} Re {re} dt {dt:.2e}: "
+               f"iterations single {iters} batch {bi}")
+        assert bi == iters
+        for a, b in zip(fields, bf):
+            assert np.array_equal(a, b)
+
+
+def test_batch_problem_matches_oracle():
+    import memphys_amd as M
+    import oracle as O
+    probs = _problems()
+    steps = 4
+    batch = _run_batch(probs, steps, 1)
+    cloud, re, dt = probs[1]
+    w = O.stencil_weights(cloud, 3, 3)[:4]
+    u_dir, p_dir, u0 = M.lid_conditions(cloud)
+    fs = O.FractionalStep(cloud, w, u_dir, u_dir, None, p_dir, 1.0, 1.0 / re, dt, 2,
+                          solver_kind=0, precond=2, tol=1e-14, maxit=20000)
+    u, v = u0.copy(), np.zeros(cloud.nv)
+    p = np.zeros(fs.n_sys)
+    for it in range(steps):
+        fs.step(it, u, v, None, p)
+    (bu, bv, bp), _ = batch[1]
+    errs = [rel_err(bu, u), rel_err(bv, v), rel_err(bp, p)]
+    _print("batched problem 1 vs oracle:", errs)
+    assert max(errs) <= 1e-8
+
+
+def test_batch_rejects_single_problem_features():
+    import memphys_amd as M
+    probs = _problems()[:2]
+    with M.Context(0) as ctx:
+        cat = ctx.upload_batch([p[0] for p in probs])
+        ctx.weights(3, 3, download=False)
+        u_dir, p_dir, _ = M.lid_conditions(cat)
+        with pytest.raises(M.MemphysError):
+            ctx.fs_create(1.0, 0.01, 1e-4, 2, 1e-10, 100, u_dir, u_dir, None, p_dir,
+                          precond=M.MP_PRECOND_AMG)
+        ctx.fs_create(1.0, 0.01, 1e-4, 2, 1e-10, 100, u_dir, u_dir, None, p_dir)
+        with pytest.raises(M.MemphysError):
+            ctx.guess_mode(M.MP_GUESS_LEARNED)
