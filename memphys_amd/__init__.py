@@ -137,6 +137,8 @@ def host_lib() -> C.CDLL:
                                             C.POINTER(C.c_int), _ip, _ip, _dp, C.c_long, _ip,
                                             _ip, _dp, C.c_long]
         lib.mph_amg_bordered_inverse.argtypes = [C.c_int, _ip, _ip, _dp, _dp]
+        lib.mph_gmsh_read.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _host = lib
     return _host
 
@@ -195,6 +197,38 @@ def box_cloud(dim: int, n: int, jitter: float, seed: int, k_interior: int,
         layer = extra
         ks = np.diff(rp).astype(np.int32)
     return Cloud(dim, xyz.reshape(nv, dim), bnd, nrm.reshape(nv, dim), rp, col, ks, layer)
+
+
+def gmsh_cloud(path: str, k: int, knn_threads: int = 0, rcm: bool = True) -> Cloud:
+    """A Gmsh 2.2 mesh read as POINTS(PARAMETERS&) reads it (boundary flags,
+    averaged outward normals, corner/edge and hanging nodes deleted), with k-node
+    stencils (CLOUD::calc_cloud_points_fast) and, optionally, RCM order."""
+    lib = host_lib()
+    dim, nv = C.c_int(), C.c_int()
+    if lib.mph_gmsh_read(path.encode(), C.byref(dim), C.byref(nv), None, None, None, None) != 0:
+        raise MemphysError(f"gmsh_cloud: cannot read {path}")
+    d, n = dim.value, nv.value
+    xyz = np.zeros(n * d)
+    nrm = np.zeros(n * d)
+    bnd = np.zeros(n, dtype=np.uint8)
+    orig = np.zeros(n, dtype=np.int32)
+    lib.mph_gmsh_read(path.encode(), None, None, xyz.ctypes.data_as(C.c_void_p),
+                      bnd.ctypes.data_as(C.c_void_p), nrm.ctypes.data_as(C.c_void_p),
+                      orig.ctypes.data_as(C.c_void_p))
+    ks = np.full(n, k, dtype=np.int32)
+    rp = np.zeros(n + 1, dtype=np.int32)
+    col = np.zeros(n * k, dtype=np.int32)
+    if lib.mph_knn(n, d, xyz, bnd, ks, rp, col, int(knn_threads)) != 0:
+        raise MemphysError("gmsh_cloud: kNN failed")
+    extra = orig.copy()
+    if rcm:
+        order = np.zeros(n, dtype=np.int32)
+        if lib.mph_rcm(n, rp, col, order) != 0:
+            raise MemphysError("gmsh_cloud: RCM failed")
+        if lib.mph_apply_order(n, d, order, xyz, nrm, bnd, extra.ctypes.data_as(C.c_void_p),
+                               rp, col) != 0:
+            raise MemphysError("gmsh_cloud: reorder failed")
+    return Cloud(d, xyz.reshape(n, d), bnd, nrm.reshape(n, d), rp, col, ks, extra)
 
 
 def poly_exponents(dim: int, poly_deg: int) -> np.ndarray:

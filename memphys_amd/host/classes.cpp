@@ -202,9 +202,40 @@ POINTS POINTS::box(PARAMETERS& parameters, int n, double jitter, uint64_t seed,
     return p;
 }
 
+// points.cpp:27-72: read the Gmsh 2.2 mesh, boundary flags, vertex normals,
+// delete corner/edge vertices, bounding box.
 POINTS::POINTS(PARAMETERS& parameters) {
-    (void)parameters;
-    fail("POINTS::POINTS", "the Gmsh reader is not part of this build; use POINTS::box");
+    clock_t t1 = clock();
+    memphys::GmshCloud g;
+    try {
+        g = memphys::read_gmsh22(parameters.meshfile);
+    } catch (const std::exception& e) {
+        fail("POINTS::POINTS", e.what());
+    }
+    if (g.dim != parameters.dimension)
+        fail("POINTS::POINTS", "mesh dimension differs from PARAMETERS::dimension");
+    nv = g.nv;
+    nv_original = g.nv_original;
+    xyz = g.xyz;
+    normal = g.normal;
+    boundary_flag.assign(g.boundary.begin(), g.boundary.end());
+    corner_edge_vertices.assign(g.nv_original, true);
+    for (int o : g.original) corner_edge_vertices[o] = false;
+    bc_tag.assign(nv, -1);
+    const int d = g.dim;
+    xyz_min.assign(d, 0.0);
+    xyz_max.assign(d, 0.0);
+    for (int k = 0; k < d; ++k) {
+        xyz_min[k] = xyz_max[k] = xyz[k];
+        for (int i = 0; i < nv; ++i) {
+            xyz_min[k] = std::min(xyz_min[k], xyz[(size_t)i * d + k]);
+            xyz_max[k] = std::max(xyz_max[k], xyz[(size_t)i * d + k]);
+        }
+    }
+    xyz_length.resize(d);
+    for (int k = 0; k < d; ++k) xyz_length[k] = xyz_max[k] - xyz_min[k];
+    parameters.points_timer = (double)(clock() - t1) / CLOCKS_PER_SEC;
+    printf("POINTS::POINTS original mesh nv: %d, after deleting corners nv: %d\n", nv_original, nv);
 }
 
 // ---------------------------------------------------------------- CLOUD

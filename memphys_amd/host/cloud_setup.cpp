@@ -3,6 +3,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <fstream>
+#include <sstream>
+#include <unordered_map>
 #include <numeric>
 #include <queue>
 #include <random>
@@ -381,6 +384,197 @@ std::vector<int> polynomial_exponents(int dim, int poly_deg) {
         }
     }
     return e;
+}
+
+// ---------------------------------------------------------------- Gmsh 2.2
+
+namespace {
+
+bool is_volume_type(int t) {
+    return (t >= 4 && t <= 7) || (t >= 11 && t <= 14) || (t >= 17 && t <= 19) ||
+           (t >= 29 && t <= 31);
+}
+
+int nodes_of_type(int t) {
+    switch (t) {
+        case 15: return 1;  // point
+        case 1: return 2;   // line
+        case 2: return 3;   // triangle
+        case 3: return 4;   // quadrangle
+        case 4: return 4;   // tetrahedron
+        default: return -1;
+    }
+}
+
+}  // namespace
+
+GmshCloud read_gmsh22(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("read_gmsh22: cannot open " + path);
+    std::string tok;
+    while (f >> tok && tok != "$MeshFormat") {}
+    if (tok != "$MeshFormat") throw std::runtime_error("read_gmsh22: $MeshFormat not found");
+    std::string v1, v2, v3;
+    f >> v1 >> v2 >> v3;
+    if (v1 != "2.2" || v2 != "0" || v3 != "8")
+        throw std::runtime_error("read_gmsh22: MeshFormat should be 2.2 0 8");
+    while (f >> tok && tok != "$Nodes") {}
+    if (tok != "$Nodes") throw std::runtime_error("read_gmsh22: $Nodes not found");
+    int nn = 0;
+    f >> nn;
+    if (nn <= 0) throw std::runtime_error("read_gmsh22: no nodes");
+    std::vector<double> x3((size_t)nn * 3);
+    std::vector<int> ids(nn);
+    std::unordered_map<int, int> index_of;
+    for (int i = 0; i < nn; ++i) {
+        f >> ids[i] >> x3[3 * i] >> x3[3 * i + 1] >> x3[3 * i + 2];
+        index_of[ids[i]] = i;
+    }
+    if (!f) throw std::runtime_error("read_gmsh22: truncated $Nodes");
+    while (f >> tok && tok != "$Elements") {}
+    if (tok != "$Elements") throw std::runtime_error("read_gmsh22: $Elements not found");
+    int ne = 0;
+    f >> ne;
+    struct Elem {
+        int type;
+        std::vector<int> v;
+    };
+    std::vector<Elem> elems;
+    elems.reserve(ne);
+    int dim = 2;
+    std::string line;
+    std::getline(f, line);
+    for (int e = 0; e < ne; ++e) {
+        if (!std::getline(f, line)) throw std::runtime_error("read_gmsh22: truncated $Elements");
+        std::istringstream ls(line);
+        int id = 0, type = 0, ntags = 0;
+        ls >> id >> type >> ntags;
+        for (int t = 0; t < ntags; ++t) {
+            int tag;
+            ls >> tag;
+        }
+        Elem el{type, {}};
+        const int k = nodes_of_type(type);
+        for (int j = 0; j < std::max(0, k); ++j) {
+            int nid;
+            ls >> nid;
+            auto it = index_of.find(nid);
+            if (it == index_of.end()) throw std::runtime_error("read_gmsh22: unknown node id");
+            el.v.push_back(it->second);
+        }
+        if (is_volume_type(type)) dim = 3;
+        elems.push_back(std::move(el));
+    }
+    // boundary and corner/edge flags (read_points_flag_msh)
+    std::vector<uint8_t> bflag(nn, 0), drop(nn, 0);
+    for (const Elem& el : elems) {
+        if (dim == 2) {
+            if (el.type == 1)
+                for (int v : el.v) bflag[v] = 1;
+            else if (el.type == 15)
+                drop[el.v[0]] = 1;
+        } else {
+            if (el.type == 2 || el.type == 3)
+                for (int v : el.v) bflag[v] = 1;
+            else if (el.type == 1)
+                for (int v : el.v) drop[v] = 1;
+        }
+    }
+    // elements of the normal computation (read_elem_vert_complete_msh):
+    // 2D: lines (boundary) and triangles (volume); 3D: triangles (boundary) and
+    // tetrahedra (volume); points and (in 3D) lines are skipped
+    std::vector<int> el_idx;
+    std::vector<uint8_t> el_bnd;
+    for (int e = 0; e < ne; ++e) {
+        const int t = elems[e].type;
+        if (t == 15 || (t == 1 && dim == 3)) continue;
+        if (t == 1 || t == 2 || t == 4) {
+            el_idx.push_back(e);
+            el_bnd.push_back((t == 1 && dim == 2) || (t == 2 && dim == 3) ? 1 : 0);
+        } else {
+            throw std::runtime_error("read_gmsh22: unsupported element type " + std::to_string(t));
+        }
+    }
+    const int ncv = (int)el_idx.size();
+    std::vector<std::vector<int>> vert_cv(nn);
+    for (int c = 0; c < ncv; ++c)
+        for (int v : elems[el_idx[c]].v) vert_cv[v].push_back(c);
+    for (int v = 0; v < nn; ++v)
+        if (vert_cv[v].empty()) drop[v] = 1;  // hanging node
+    auto X = [&](int v, int a) { return x3[3 * v + a]; };
+    std::vector<double> en((size_t)ncv * 3, 0.0);
+    for (int c = 0; c < ncv; ++c) {
+        if (!el_bnd[c]) continue;
+        const std::vector<int>& v = elems[el_idx[c]].v;
+        double nrm[3] = {0, 0, 0};
+        if (dim == 2) {
+            nrm[0] = X(v[1], 1) - X(v[0], 1);
+            nrm[1] = -(X(v[1], 0) - X(v[0], 0));
+        } else {
+            double a[3], b[3];
+            for (int k = 0; k < 3; ++k) {
+                a[k] = X(v[1], k) - X(v[0], k);
+                b[k] = X(v[2], k) - X(v[1], k);
+            }
+            nrm[0] = a[1] * b[2] - a[2] * b[1];
+            nrm[1] = a[2] * b[0] - a[0] * b[2];
+            nrm[2] = a[0] * b[1] - a[1] * b[0];
+        }
+        int inner = -1;
+        for (int c2 : vert_cv[v[0]])
+            if (!el_bnd[c2]) {
+                inner = c2;
+                break;
+            }
+        if (inner < 0)
+            throw std::runtime_error("read_gmsh22: boundary element without a volume neighbour");
+        const std::vector<int>& iv = elems[el_idx[inner]].v;
+        double cen[3] = {0, 0, 0}, dir[3], dot = 0.0, mag = 0.0;
+        for (int k = 0; k < dim; ++k) {
+            for (int u : iv) cen[k] += X(u, k);
+            cen[k] = cen[k] / (double)iv.size();
+            dir[k] = X(v[1], k) - cen[k];
+            dot += dir[k] * nrm[k];
+        }
+        if (dot < 0)
+            for (int k = 0; k < dim; ++k) nrm[k] = -nrm[k];
+        for (int k = 0; k < dim; ++k) mag += nrm[k] * nrm[k];
+        mag = std::sqrt(mag);
+        for (int k = 0; k < dim; ++k) en[3 * c + k] = nrm[k] / mag;
+    }
+    // node normals: mean of the boundary elements' normals, normalised
+    std::vector<double> vn((size_t)nn * 3, 0.0);
+    for (int v = 0; v < nn; ++v) {
+        if (!bflag[v]) continue;
+        int count = 0;
+        for (int c : vert_cv[v])
+            if (el_bnd[c]) {
+                for (int k = 0; k < dim; ++k) vn[3 * v + k] += en[3 * c + k];
+                count++;
+            }
+        double mag = 0.0;
+        for (int k = 0; k < dim; ++k) {
+            vn[3 * v + k] /= (double)count;
+            mag += vn[3 * v + k] * vn[3 * v + k];
+        }
+        mag = std::sqrt(mag);
+        for (int k = 0; k < dim; ++k) vn[3 * v + k] /= mag;
+    }
+    GmshCloud out;
+    out.dim = dim;
+    out.nv_original = nn;
+    out.nelem = ncv;
+    for (int v = 0; v < nn; ++v) {
+        if (drop[v]) continue;
+        for (int k = 0; k < dim; ++k) {
+            out.xyz.push_back(X(v, k));
+            out.normal.push_back(vn[3 * v + k]);
+        }
+        out.boundary.push_back(bflag[v]);
+        out.original.push_back(v);
+    }
+    out.nv = (int)out.boundary.size();
+    return out;
 }
 
 }  // namespace memphys

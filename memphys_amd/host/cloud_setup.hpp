@@ -68,6 +68,24 @@ void apply_order(int nv, int dim, const std::vector<int>& order,
                  std::vector<uint8_t>& boundary, std::vector<int>& extra_int,
                  std::vector<int>& row_ptr, std::vector<int>& col);
 
+// A Gmsh 2.2 ASCII mesh read as POINTS does (points.cpp:27-72, 323-406,
+// 498-794): node coordinates (z dropped in 2D); boundary nodes = nodes of the
+// boundary elements (2D: 2-node lines; 3D: 3-node triangles, 4-node quads);
+// outward unit normals averaged over a node's boundary elements, each element
+// normal oriented away from the centroid of the first volume element at its
+// first node; corner points (2D: 1-node elements), edge nodes (3D: nodes of
+// 2-node lines) and nodes of no element are deleted.  The dimension is 3
+// when any volume element type is present (parameters.cpp:132-215).
+struct GmshCloud {
+    int dim = 0;
+    int nv = 0, nv_original = 0, nelem = 0;
+    std::vector<double> xyz;        // nv*dim, AoS
+    std::vector<uint8_t> boundary;  // nv
+    std::vector<double> normal;     // nv*dim (zero on interior nodes)
+    std::vector<int> original;      // nv: index of each kept node in the file's node list
+};
+GmshCloud read_gmsh22(const std::string& path);
+
 // Graded monomial table of PARAMETERS::calc_polynomial_term_exponents
 // (parameters.cpp:217-267), row-major [m x dim].
 std::vector<int> polynomial_exponents(int dim, int poly_deg);

@@ -104,6 +104,27 @@ int mph_poly_exponents(int dim, int poly_deg, int* out) {
     }
 }
 
+// Gmsh 2.2 mesh read as POINTS(PARAMETERS&) does: first call with xyz == NULL
+// returns the node count and dimension; the second fills xyz/normal (nv*dim),
+// boundary (nv) and original (nv, index in the file's node list; may be NULL).
+int mph_gmsh_read(const char* path, int* dim, int* nv, double* xyz, uint8_t* boundary,
+                  double* normal, int* original) {
+    try {
+        memphys::GmshCloud g = memphys::read_gmsh22(path);
+        if (dim) *dim = g.dim;
+        if (nv) *nv = g.nv;
+        if (xyz) {
+            std::memcpy(xyz, g.xyz.data(), sizeof(double) * g.xyz.size());
+            std::memcpy(boundary, g.boundary.data(), g.boundary.size());
+            std::memcpy(normal, g.normal.data(), sizeof(double) * g.normal.size());
+            if (original) std::memcpy(original, g.original.data(), sizeof(int) * g.original.size());
+        }
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
 // First coarsening step of the multigrid set-up the HIP library runs
 // (amg_build.hpp), for CPU checks: aggregates (n entries), the smoothed
 // prolongator P (n x n_agg) and the Galerkin operator R A P (n_agg square) as

@@ -285,3 +285,21 @@ def test_weights_mixed_precision_match_oracle(ctx, cube3d_small, cavity2d):
         assert worst <= 1e-10
         for a, b in pairs:
             assert rel_err(a, b) <= 1e-10
+
+
+def test_gmsh_annulus_matches_reference_sample_output(ctx):
+    """GPU weights on the reference's own mesh reproduce the operator
+    eigenvalues and RBF condition numbers the reference wrote to its sample
+    simulation_details.csv."""
+    import memphys_amd as M
+    from test_gmsh_known_answers import KA, MESH, _check_eig, _largest_eig
+    cloud = M.gmsh_cloud(MESH, KA["cloud_size"])
+    ctx.upload(cloud)
+    (gx, gy, _, lap), cond = ctx.weights(KA["phs_deg"], KA["poly_deg"], cond=True)
+    _print("annulus: nv", cloud.nv, "cond max", cond.max())
+    _check_eig(_largest_eig(gx, cloud), KA["grad_x_eigval"], 1e-10)
+    _check_eig(_largest_eig(gy, cloud), KA["grad_y_eigval"], 1e-10)
+    _check_eig(_largest_eig(lap, cloud), KA["laplace_eigval"], 1e-10)
+    assert cond.max() == pytest.approx(KA["rbf_cond_num_max"], rel=1e-12)
+    assert cond.min() == pytest.approx(KA["rbf_cond_num_min"], rel=1e-12)
+    assert cond.mean() == pytest.approx(KA["rbf_cond_num_avg"], rel=1e-12)
