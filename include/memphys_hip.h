@@ -118,6 +118,14 @@ int mp_weights_set(mp_ctx* ctx, const double* gx, const double* gy, const double
 /* y = Op x for one stencil operator (replaces points.*_matrix_EIGEN * x). */
 int mp_operator_apply(mp_ctx* ctx, int op, const double* x, double* y);
 
+/* Largest-magnitude eigenvalue of one stencil operator (replaces
+ * calc_largest_magnitude_eigenvalue, general_functions.cpp:4-24, as used by
+ * PARAMETERS::calc_dt, parameters.cpp:269-296): restarted Arnoldi on the GPU
+ * until the Ritz residual is below tol relative; im >= 0 (a conjugate pair is
+ * reported once). */
+int mp_operator_eigenvalue(mp_ctx* ctx, int op, double tol, int max_restarts, double* re,
+                           double* im);
+
 /* Implicit operator and Krylov solve (replaces SOLVER::init, solver.cpp:14-79 with
  * calc_coeff_matrix 165-214, and SOLVER::general_solve 283-363).
  * Rows: interior diff*L + conv*(Gx+Gy+Gz) + unsteady*I; boundary Dirichlet

@@ -35,6 +35,7 @@ HIP_SYMBOLS = (
     "mp_kernel_timing", "mp_kernel_timing_get", "mp_guess_param_count", "mp_guess_train",
     "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
     "mp_graph_mode", "mp_batch_upload", "mp_batch_params", "mp_batch_stats", "mp_autotune",
+    "mp_operator_eigenvalue",
 )
 
 
@@ -110,6 +111,8 @@ def hip_lib() -> C.CDLL:
         lib.mp_batch_params.argtypes = [vp, _dp, _dp, _dp]
         lib.mp_batch_stats.argtypes = [vp, C.c_int, C.POINTER(StepStats)]
         lib.mp_autotune.argtypes = [vp, C.POINTER(C.c_int), _dp]
+        lib.mp_operator_eigenvalue.argtypes = [vp, C.c_int, C.c_double, C.c_int,
+                                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.mp_precond_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), vp, vp,
                                         C.POINTER(C.c_double)]
         for name in HIP_SYMBOLS:
@@ -137,6 +140,7 @@ def host_lib() -> C.CDLL:
                                             C.POINTER(C.c_int), _ip, _ip, _dp, C.c_long, _ip,
                                             _ip, _dp, C.c_long]
         lib.mph_amg_bordered_inverse.argtypes = [C.c_int, _ip, _ip, _dp, _dp]
+        lib.mph_hessenberg_eig.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp]
         lib.mph_gmsh_read.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _host = lib
@@ -292,6 +296,13 @@ class Context:
     def batch_params(self, rho, mu, dt):
         f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         self._check(self.lib.mp_batch_params(self.h, f(rho), f(mu), f(dt)), "mp_batch_params")
+
+    def eigenvalue(self, op: int, tol: float = 1e-12, max_restarts: int = 200) -> complex:
+        """Largest-magnitude eigenvalue of a stencil operator (imaginary part >= 0)."""
+        re, im = C.c_double(), C.c_double()
+        self._check(self.lib.mp_operator_eigenvalue(self.h, op, tol, max_restarts, C.byref(re),
+                                                    C.byref(im)), "mp_operator_eigenvalue")
+        return complex(re.value, im.value)
 
     def autotune(self):
         """Picks the fastest convergence-poll interval on the last step's solve;
@@ -593,3 +604,14 @@ def amg_bordered_inverse(A) -> np.ndarray:
     if rc != 0:
         raise MemphysError("mph_amg_bordered_inverse failed")
     return G.reshape(n, n)
+
+
+def hessenberg_eig(H, which: int = 0):
+    """Eigenvalues of an upper Hessenberg matrix and the unit eigenvector of
+    eigenvalue number `which`, by the routines of the GPU Arnoldi iteration."""
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    m = H.shape[0]
+    re, im, vr, vi = (np.zeros(m) for _ in range(4))
+    if host_lib().mph_hessenberg_eig(m, H.ravel(), re, im, which, vr, vi) != 0:
+        raise MemphysError("hessenberg_eig failed")
+    return re + 1j * im, vr + 1j * vi

@@ -606,6 +606,18 @@ int mp_operator_apply(mp_ctx* c, int op, const double* x, double* y) {
     });
 }
 
+int mp_operator_eigenvalue(mp_ctx* c, int op, double tol, int max_restarts, double* re,
+                           double* im) {
+    return guarded(c, "mp_operator_eigenvalue", [&]() -> int {
+        if (!c->has_weights) return fail(c, MP_ERR_STATE, "mp_operator_eigenvalue: no weights");
+        if (op < 0 || op > 3 || (op == MP_GZ && c->dim != 3) || !re || !im || !(tol > 0) ||
+            max_restarts < 0)
+            return fail(c, MP_ERR_ARG, "mp_operator_eigenvalue: bad arguments");
+        largest_eigenvalue(c->stream, c->sv, c->w[op].p, 40, tol, max_restarts, re, im);
+        return MP_OK;
+    });
+}
+
 int mp_solver_init(mp_ctx* c, const uint8_t* dirichlet, double unsteady, double conv, double diff,
                    double tolerance, int n_iter, int precision) {
     return guarded(c, "mp_solver_init", [&]() -> int {

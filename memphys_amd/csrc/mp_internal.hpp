@@ -166,6 +166,11 @@ void launch_build_operator(hipStream_t s, const SellView& sv, int dim, const dou
                            const uint8_t* dir, double unsteady, double conv, double diff,
                            double* aval, double* diag);
 
+// Largest-magnitude eigenvalue of a stencil operator (k_eig.hip); returns the
+// Arnoldi restarts used.
+int largest_eigenvalue(hipStream_t s, const SellView& sv, const double* val, int m, double tol,
+                       int max_restarts, double* re, double* im);
+
 // Krylov building blocks (k_krylov.hip).
 void launch_fill(hipStream_t s, double* x, double v, int n);
 

@@ -178,6 +178,40 @@ void PARAMETERS::calc_polynomial_term_exponents() {
         for (int c = 0; c < dimension; ++c) polynomial_term_exponents(r, c) = e[r * dimension + c];
 }
 
+// parameters.cpp:269-296: dt = Courant * 2 / (u0 |l_x| + v0 |l_y| (+ w0 |l_z|) + alpha |l_L|)
+// with l_* the largest-magnitude eigenvalue of each operator (forward Euler).
+void PARAMETERS::calc_dt(memphys::SparseMatrixCSR& grad_x, memphys::SparseMatrixCSR& grad_y,
+                         memphys::SparseMatrixCSR& grad_z, memphys::SparseMatrixCSR& laplacian,
+                         double u0, double v0, double w0, double alpha) {
+    auto eig = [&](memphys::SparseMatrixCSR& m, double& re, double& im) {
+        if (!m.ctx || m.op < 0 || m.rows_mask != 0)
+            fail("PARAMETERS::calc_dt", "operator matrix is not bound to the GPU cloud");
+        memphys::check(m.ctx, mp_operator_eigenvalue(m.ctx, m.op, 1e-12, 500, &re, &im),
+                       "PARAMETERS::calc_dt");
+    };
+    eig(grad_x, grad_x_eigval_real, grad_x_eigval_imag);
+    eig(grad_y, grad_y_eigval_real, grad_y_eigval_imag);
+    if (dimension == 3) eig(grad_z, grad_z_eigval_real, grad_z_eigval_imag);
+    eig(laplacian, laplace_eigval_real, laplace_eigval_imag);
+    printf("PARAMETERS::calc_dt Eigenvalue with largest magnitude: grad_x: (%g, %g)\n",
+           grad_x_eigval_real, grad_x_eigval_imag);
+    printf("PARAMETERS::calc_dt Eigenvalue with largest magnitude: grad_y: (%g, %g)\n",
+           grad_y_eigval_real, grad_y_eigval_imag);
+    if (dimension == 3)
+        printf("PARAMETERS::calc_dt Eigenvalue with largest magnitude: grad_z: (%g, %g)\n",
+               grad_z_eigval_real, grad_z_eigval_imag);
+    printf("PARAMETERS::calc_dt Eigenvalue with largest magnitude: laplacian: (%g, %g)\n",
+           laplace_eigval_real, laplace_eigval_imag);
+    auto mag = [](double re, double im) { return std::sqrt(re * re + im * im); };
+    double ev = u0 * mag(grad_x_eigval_real, grad_x_eigval_imag);
+    ev += v0 * mag(grad_y_eigval_real, grad_y_eigval_imag);
+    if (dimension == 3) ev += w0 * mag(grad_z_eigval_real, grad_z_eigval_imag);
+    ev += alpha * mag(laplace_eigval_real, laplace_eigval_imag);
+    dt = 2.0 / ev;
+    dt = dt * Courant;
+    printf("PARAMETERS::calc_dt Courant: %g, dt: %g seconds\n\n", Courant, dt);
+}
+
 // ---------------------------------------------------------------- POINTS
 POINTS POINTS::box(PARAMETERS& parameters, int n, double jitter, uint64_t seed,
                    const int k_layer[3]) {

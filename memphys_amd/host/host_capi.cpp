@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../csrc/amg_build.hpp"
+#include "../csrc/eig_host.hpp"
 #include "cloud_setup.hpp"
 
 extern "C" {
@@ -156,6 +157,31 @@ int mph_amg_first_level(int n, const int* ptr, const int* col, const double* val
         std::memcpy(p_val, P.val.data(), sizeof(double) * P.nnz());
         std::memcpy(ac_col, Ac.col.data(), sizeof(int) * Ac.nnz());
         std::memcpy(ac_val, Ac.val.data(), sizeof(double) * Ac.nnz());
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// Eigenvalues (re, im: m each) of an upper Hessenberg matrix H (row-major
+// m x m) and the unit eigenvector (vre, vim) of eigenvalue number `which`, as
+// the GPU Arnoldi iteration computes them.
+int mph_hessenberg_eig(int m, const double* H, double* re, double* im, int which, double* vre,
+                       double* vim) {
+    try {
+        std::vector<double> h(H, H + (size_t)m * m);
+        std::vector<mp_eig::cplx> ev = mp_eig::hessenberg_eigenvalues(h, m);
+        for (int i = 0; i < m; ++i) {
+            re[i] = ev[i].real();
+            im[i] = ev[i].imag();
+        }
+        if (vre && which >= 0 && which < m) {
+            std::vector<mp_eig::cplx> y = mp_eig::hessenberg_eigenvector(h, m, ev[which]);
+            for (int i = 0; i < m; ++i) {
+                vre[i] = y[i].real();
+                vim[i] = y[i].imag();
+            }
+        }
         return 0;
     } catch (const std::exception&) {
         return -1;

@@ -109,6 +109,10 @@ class PARAMETERS {
     std::vector<double> rel_res_log, abs_res_log, regul_alpha_log, steady_error_log;
     std::vector<int> n_iter_actual;
 
+    double grad_x_eigval_real = 0, grad_x_eigval_imag = 0, grad_y_eigval_real = 0,
+           grad_y_eigval_imag = 0, grad_z_eigval_real = 0, grad_z_eigval_imag = 0,
+           laplace_eigval_real = 0, laplace_eigval_imag = 0;
+
     // GPU-side choices (not in the reference's CSV): PPE and weight precision.
     int precision = 0;  // 0 FP64, 1 FP32-operator inner solve with FP64 defect correction
     int device = 0;
@@ -122,6 +126,11 @@ class PARAMETERS {
     void calc_cloud_num_points();
     void get_problem_dimension_msh();
     void calc_polynomial_term_exponents();
+    // Explicit time step from the largest-magnitude eigenvalues of the operators
+    // (parameters.cpp:269-296), computed on the GPU.
+    void calc_dt(memphys::SparseMatrixCSR& grad_x, memphys::SparseMatrixCSR& grad_y,
+                 memphys::SparseMatrixCSR& grad_z, memphys::SparseMatrixCSR& laplacian, double u0,
+                 double v0, double w0, double alpha);
 };
 
 class POINTS {

@@ -303,3 +303,44 @@ def test_gmsh_annulus_matches_reference_sample_output(ctx):
     assert cond.max() == pytest.approx(KA["rbf_cond_num_max"], rel=1e-12)
     assert cond.min() == pytest.approx(KA["rbf_cond_num_min"], rel=1e-12)
     assert cond.mean() == pytest.approx(KA["rbf_cond_num_avg"], rel=1e-12)
+    # the GPU Arnoldi iteration (PARAMETERS::calc_dt's eigenvalues)
+    for op, key in ((M.MP_GX, "grad_x_eigval"), (M.MP_GY, "grad_y_eigval"),
+                    (M.MP_LAP, "laplace_eigval")):
+        ev = ctx.eigenvalue(op)
+        _print("GPU Arnoldi", key, ev, "reference", KA[key])
+        _check_eig(ev, KA[key], 1e-9)
+
+
+def test_class_api_gmsh_setup_matches_reference_sample_output(tmp_path):
+    """PARAMETERS(csv, msh) -> POINTS -> CLOUD -> calc_dt through the C++ class
+    API reproduces the reference's simulation_details.csv for its own mesh."""
+    import os
+    import subprocess
+    from test_gmsh_known_answers import KA, MESH
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "examples", "annulus_operators")
+    a = KA["calc_dt_args"]
+    out = subprocess.run([exe, os.path.join(root, "examples", "parameters_annulus.csv"), MESH,
+                          str(a["u0"]), str(a["v0"]), str(a["alpha"])], capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    got = {}
+    for line in out.stdout.splitlines():
+        parts = line.split(",")
+        if len(parts) >= 2 and parts[0] in KA or parts[0] in ("RBF_cond_num_max",
+                                                               "RBF_cond_num_min",
+                                                               "RBF_cond_num_avg"):
+            got[parts[0]] = [float(x) for x in parts[1:]]
+    _print("class API annulus:", got)
+    for k in ("nv", "nv_boundary", "nv_internal", "cloud_size"):
+        assert got[k][0] == KA[k]
+    for k in ("dx_max", "dx_min", "dx_avg"):
+        assert got[k][0] == pytest.approx(KA[k], rel=1e-14)
+    for k, ref in (("RBF_cond_num_max", "rbf_cond_num_max"), ("RBF_cond_num_min", "rbf_cond_num_min"),
+                   ("RBF_cond_num_avg", "rbf_cond_num_avg")):
+        assert got[k][0] == pytest.approx(KA[ref], rel=1e-12)
+    for k in ("grad_x_eigval", "grad_y_eigval", "laplace_eigval"):
+        re, im = got[k]
+        assert re == pytest.approx(KA[k][0], rel=1e-9)
+        assert abs(im - abs(KA[k][1])) <= 1e-9 * abs(KA[k][0])
+    assert got["timestep_dt"][0] == pytest.approx(KA["timestep_dt"], rel=1e-9)
