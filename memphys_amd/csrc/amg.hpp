@@ -1,5 +1,6 @@
 // Device side of the smoothed-aggregation preconditioner: one V(1,1)-cycle
-// with damped-Jacobi smoothing, in FP32 (host set-up in amg_build.hpp).
+// with damped-Jacobi smoothing, FP32 arithmetic and vectors over BF16 matrix
+// values (host set-up in amg_build.hpp).
 #pragma once
 #include <string>
 
@@ -17,10 +18,10 @@ struct AmgMat {
     int n = 0, nslices = 0, g = 0;
     long nnz = 0;
     DevBuf<int> ptr_b, col_b;
-    DevBuf<float> val_b;
+    DevBuf<uint16_t> val_b;  // BF16 values (round to nearest even)
     const int* ptr = nullptr;
     const int* col = nullptr;
-    const float* val = nullptr;
+    const uint16_t* val = nullptr;
 };
 
 struct AmgLevelDev {
@@ -44,8 +45,8 @@ struct AmgDevice {
     std::vector<mp_amg::Csr> hA, hP, hR;
 
     // Builds the hierarchy from the finest operator, given in SELL-64 slots
-    // (device order) by the solver: sv and val32 stay owned by the solver and
-    // serve as level 0.
+    // (device order) by the solver: sv stays owned by the solver and serves as
+    // level 0's layout, with a BF16 copy of val32 as its values.
     void setup(hipStream_t s, const SellView& sv, const double* val64_dev,
                const float* val32_dev, const mp_amg::Options& opt);
     // Layout of every hierarchy matrix chosen by timing each candidate (SELL-64,
