@@ -192,33 +192,53 @@ def run_batched_cavities(M, steps, seq_steps, tol, nprob=64):
     }
 
 
-def cpu_baseline(cloud, w, prm, iters_per_step, max_iter_sample=10):
-    """Oracle FP64 step (1 thread) on the same 1M-node cloud: predictor/RHS and
-    corrector timed in full, BiCGSTAB (Jacobi) timed over a few iterations and
-    scaled to the iterations per step the FP64 GPU path needed."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
+def cpu_baseline(cloud, w, prm, iters_per_step, max_iter_sample=10, repeats=3):
+    """Oracle FP64 step on the same 1M-node cloud, on 1 thread and on all the
+    box's cores (OpenMP rows, at most 16): predictor/RHS and corrector timed in
+    full, BiCGSTAB (Jacobi) timed over a few iterations and scaled to the
+    iterations per step the FP64 GPU path needed; median of `repeats` runs."""
     import oracle as O
     u_dir, p_dir, u0 = lid_bc(cloud)
     nv = cloud.nv
     fs = O.FractionalStep(cloud, w, u_dir, u_dir, u_dir, p_dir, prm["rho"], prm["mu"], prm["dt"],
                           2, solver_kind=0, precond=1, tol=1e-30, maxit=max_iter_sample)
-    u, v, ww = u0.copy(), np.zeros(nv), np.zeros(nv)
-    p = np.zeros(fs.n_sys)
+    cores = max(1, min(16, os.cpu_count() or 1))
+    out = {}
+    for nthr in (1, cores):
+        O.set_threads(nthr)
+        per_steps = []
+        for _ in range(repeats):
+            u, v, ww = u0.copy(), np.zeros(nv), np.zeros(nv)
+            p = np.zeros(fs.n_sys)
+            t0 = time.perf_counter()
+            uh, b = fs.predict(0, u, v, ww, p)
+            t1 = time.perf_counter()
+            x, _, it, _ = fs.solve(b, p)
+            t2 = time.perf_counter()
+            fs.correct(u, v, ww, uh, x)
+            t3 = time.perf_counter()
+            t_iter = (t2 - t1) / max(it, 1)
+            per_steps.append(((t1 - t0) + (t3 - t2)) + iters_per_step * t_iter)
+        out[nthr] = (float(np.median(per_steps)), t1 - t0, t3 - t2, t_iter, it)
+    O.set_threads(cores)
+    ps, tp, tc, ti, it = out[cores]
+    ps1, tp1, tc1, ti1, _ = out[1]
+    return {"value": 1.0 / ps, "unit": "timesteps/s", "cores": cores, "kind": "port",
+            "single_thread_value": 1.0 / ps1,
+            "sample": (f"oracle FP64 step on the same {nv}-node cloud, median of {repeats}: "
+                       f"predictor+rhs and corrector timed in full ({tp:.3f}s + {tc:.3f}s on "
+                       f"{cores} threads, {tp1:.2f}s + {tc1:.2f}s on 1), Jacobi BiCGSTAB "
+                       f"{it} iterations ({ti * 1e3:.1f} ms/iter on {cores} threads, "
+                       f"{ti1 * 1e3:.1f} on 1) scaled to {iters_per_step:.0f} iterations/step "
+                       f"(the FP64 GPU path's count)")}
+
+
+def cpu_weights_rate(cloud, rows, threads):
+    """Oracle FP64 stencil weights on the sampled rows: stencils per second."""
+    import oracle as O
     t0 = time.perf_counter()
-    uh, b = fs.predict(0, u, v, ww, p)
-    t1 = time.perf_counter()
-    x, _, it, _ = fs.solve(b, p)
-    t2 = time.perf_counter()
-    fs.correct(u, v, ww, uh, x)
-    t3 = time.perf_counter()
-    t_fixed = (t1 - t0) + (t3 - t2)
-    t_iter = (t2 - t1) / max(it, 1)
-    per_step = t_fixed + iters_per_step * t_iter
-    return {"value": 1.0 / per_step, "unit": "timesteps/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle FP64 step on the same {nv}-node cloud, 1 thread: predictor+rhs "
-                       f"{t1 - t0:.2f}s and corrector {t3 - t2:.2f}s timed in full, Jacobi "
-                       f"BiCGSTAB {it} iterations ({t_iter * 1e3:.1f} ms/iter) scaled to "
-                       f"{iters_per_step:.0f} iterations/step (FP64 GPU path's count)")}
+    O.stencil_weights(cloud, 3, 3, rows=rows, threads=threads)
+    return len(rows) / (time.perf_counter() - t0)
 
 
 def main():
@@ -297,8 +317,13 @@ def main():
                 den = np.linalg.norm(o[s_])
                 worst64 = max(worst64, np.linalg.norm(a[s_] - o[s_]) / den)
                 worst32 = max(worst32, np.linalg.norm(b[s_] - o[s_]) / den)
+        cores = max(1, min(16, os.cpu_count() or 1))
         weights_check.update({"sample_stencils": 3000, "fp64_worst_rel_vs_oracle": worst64,
-                              "fp32_refined_worst_rel_vs_oracle": worst32})
+                              "fp32_refined_worst_rel_vs_oracle": worst32,
+                              "gpu_fp64_stencils_per_s": cloud.nv / (wk[1] * 1e-3),
+                              "cpu_oracle_stencils_per_s": {
+                                  "1": cpu_weights_rate(cloud, rows, 1),
+                                  str(cores): cpu_weights_rate(cloud, rows, cores)}})
         del mx, my, mz, ml
     ctx.weights(3, 3, download=False)  # FP64 weights back for the time steps
     log(f"weights check: {weights_check}")

@@ -71,6 +71,7 @@ def lib() -> C.CDLL:
         L.or_divergence.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _vp, _u8, _dp, _dp, _vp,
                                     _vp, _dp]
         L.or_divergence.restype = None
+        L.or_set_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -98,6 +99,11 @@ def single_stencil(X: np.ndarray, phs_deg: int, poly_deg: int, refine: int = 0):
     cond = lib().or_single_stencil(dim, phs_deg, len(ex) // dim, ex, k, X.ravel(), gx, gy,
                                    _ptr(gz), lap, refine)
     return gx, gy, gz, lap, cond
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's OpenMP loops; results do not depend on it."""
+    lib().or_set_threads(int(n))
 
 
 def stencil_weights(cloud, phs_deg: int, poly_deg: int, rows=None, refine: int = 0,

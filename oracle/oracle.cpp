@@ -454,7 +454,10 @@ Csr build_coeff(int nv, int dim, const int* rp, const int* ci, const double* gx,
     return A;
 }
 
+// Row loops run in parallel (OpenMP, or_set_threads); each row keeps its own
+// sequential fma order, so results do not depend on the thread count.
 void spmv(const Csr& A, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < A.n; ++i) {
         double s = 0.0;
         for (int t = A.rp[i]; t < A.rp[i + 1]; ++t) s = std::fma(A.v[t], x[A.ci[t]], s);
@@ -462,9 +465,21 @@ void spmv(const Csr& A, const double* x, double* y) {
     }
 }
 
+// Dot product over 256 fixed contiguous chunks (each summed in order, then
+// the chunk sums in order): the same value for any thread count.
 double dot(const std::vector<double>& a, const std::vector<double>& b) {
+    constexpr int NCH = 256;
+    const long n = (long)a.size();
+    double part[NCH];
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < NCH; ++c) {
+        const long i0 = n * c / NCH, i1 = n * (c + 1) / NCH;
+        double s = 0.0;
+        for (long i = i0; i < i1; ++i) s += a[i] * b[i];
+        part[c] = s;
+    }
     double s = 0.0;
-    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    for (int c = 0; c < NCH; ++c) s += part[c];
     return s;
 }
 
@@ -596,7 +611,11 @@ struct Precond {
     void apply(const std::vector<double>& in, std::vector<double>& out) const {
         out.resize(in.size());
         if (kind == 0) out = in;
-        else if (kind == 1) for (size_t i = 0; i < in.size(); ++i) out[i] = dinv[i] * in[i];
+        else if (kind == 1) {
+            const long n = (long)in.size();
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n; ++i) out[i] = dinv[i] * in[i];
+        }
         else ilut.solve(in.data(), out.data());
     }
 };
@@ -632,17 +651,22 @@ int bicgstab(const Csr& A, const Precond& P, const std::vector<double>& b, std::
             if (restarts++ == 0) i = 0;
         }
         double beta = (rho / rho_old) * (alpha / w);
+#pragma omp parallel for schedule(static)
         for (int q = 0; q < n; ++q) p[q] = r[q] + beta * (p[q] - w * v[q]);
         P.apply(p, y);
         spmv(A, y.data(), v.data());
         alpha = rho / dot(r0, v);
+#pragma omp parallel for schedule(static)
         for (int q = 0; q < n; ++q) s[q] = r[q] - alpha * v[q];
         P.apply(s, z);
         spmv(A, z.data(), t.data());
         double tt = dot(t, t);
         w = (tt > 0.0) ? dot(t, s) / tt : 0.0;
-        for (int q = 0; q < n; ++q) x[q] += alpha * y[q] + w * z[q];
-        for (int q = 0; q < n; ++q) r[q] = s[q] - w * t[q];
+#pragma omp parallel for schedule(static)
+        for (int q = 0; q < n; ++q) {
+            x[q] += alpha * y[q] + w * z[q];
+            r[q] = s[q] - w * t[q];
+        }
         ++i;
     }
     if (err_out) *err_out = std::sqrt(dot(r, r) / rhs_sq);
@@ -706,6 +730,7 @@ struct OrFs {
 namespace {
 void op_apply(const OrFs& F, const std::vector<double>& w, const double* x, double* y,
               bool interior_only) {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < F.nv; ++i) {
         if (interior_only && F.bnd[i]) { y[i] = 0.0; continue; }
         double s = 0.0;
@@ -716,6 +741,11 @@ void op_apply(const OrFs& F, const std::vector<double>& w, const double* x, doub
 }  // namespace
 
 extern "C" {
+
+// Threads of the OpenMP loops (time step and weights); results do not depend on it.
+void or_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+}
 
 // solver_kind: 0 = BiCGSTAB, 1 = dense LU.  precond: 0 none, 1 Jacobi, 2 ILUT.
 OrFs* or_fs_create(int nv, int dim, const int* rp, const int* ci, const double* gx,
