@@ -253,6 +253,9 @@ def main():
                     help="solver steps of guess training (10 episodes)")
     ap.add_argument("--batch-steps", type=int, default=200,
                     help="steps of the 64-cavity batch (0: skip it)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel instead of replaying hipGraphs (same results; "
+                         "for profilers)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device three times (a plain copy kernel "
                          "of known size, the scale for HBM counter runs)")
@@ -266,6 +269,8 @@ def main():
         sys.exit(2)
     import memphys_amd as M
 
+    if args.no_graphs:
+        M.Context.default_graphs = False
     if args.copy_calib > 0:
         import torch
         nbytes = args.copy_calib << 20

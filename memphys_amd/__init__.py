@@ -248,6 +248,10 @@ def poly_exponents(dim: int, poly_deg: int) -> np.ndarray:
 class Context:
     """One HIP context of the C ABI (one per host thread)."""
 
+    # Default of new contexts: replay Krylov chunks from hipGraphs (False
+    # launches every kernel; results are identical).
+    default_graphs = True
+
     def __init__(self, device: int = 0):
         self.lib = hip_lib()
         self.h = C.c_void_p()
@@ -256,6 +260,8 @@ class Context:
             raise MemphysError(f"mp_ctx_create failed with status {rc}")
         self.cloud = None
         self.n_sys = None
+        if not Context.default_graphs:
+            self.graph_mode(False)
 
     def _check(self, rc: int, what: str):
         if rc != 0:

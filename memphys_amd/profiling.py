@@ -1,7 +1,8 @@
 """Text summaries of rocprofv3 result databases (rocpd SQLite).
 
-    python -m memphys_amd.profiling stats DB [--top N]
-        per kernel and grid size: dispatches, total ms, mean/min/max us
+    python -m memphys_amd.profiling stats DB [--top N] [--busy-us T]
+        per kernel and grid size: dispatches, total ms, mean/min/max us, and the
+        mean over launches longer than T us
     python -m memphys_amd.profiling pmc DB [--top N]
         per kernel, grid size and counter: dispatches and mean value per dispatch
     python -m memphys_amd.profiling traffic FETCH_DB WRITE_DB --kernel SUBSTR
@@ -31,18 +32,25 @@ def _short(name: str) -> str:
     return "".join(out).strip()
 
 
-def stats(db: str, top: int) -> str:
+def stats(db: str, top: int, busy_us: float = 3.0) -> str:
+    """busy: launches longer than busy_us (solver kernels launched after their
+    system converged return at entry in ~1-2 us and are left out there)."""
     cur = sqlite3.connect(db).cursor()
     rows = cur.execute(
         "select name, grid_x, count(*), sum(end-start), avg(end-start), min(end-start), "
-        "max(end-start) from kernels group by name, grid_x order by sum(end-start) desc").fetchall()
+        "max(end-start), sum(case when end-start > ? then 1 else 0 end), "
+        "avg(case when end-start > ? then end-start end) from kernels group by name, grid_x "
+        "order by sum(end-start) desc", (busy_us * 1e3, busy_us * 1e3)).fetchall()
     tot = sum(r[3] for r in rows) or 1
-    lines = [f"{'kernel':64s} {'grid':>9s} {'calls':>7s} {'total_ms':>9s} {'mean_us':>8s} "
-             f"{'min_us':>8s} {'max_us':>8s} {'pct':>6s}"]
+    lines = [f"{'kernel':56s} {'grid':>9s} {'calls':>7s} {'busy':>7s} {'total_ms':>9s} "
+             f"{'mean_us':>8s} {'busy_us':>8s} {'min_us':>7s} {'max_us':>8s} {'pct':>6s}"]
     for r in rows[:top]:
-        lines.append(f"{_short(r[0])[:64]:64s} {r[1]:9d} {r[2]:7d} {r[3] / 1e6:9.2f} "
-                     f"{r[4] / 1e3:8.1f} {r[5] / 1e3:8.1f} {r[6] / 1e3:8.1f} {100 * r[3] / tot:6.2f}")
-    lines.append(f"total kernel time {tot / 1e6:.2f} ms over {sum(r[2] for r in rows)} dispatches")
+        busy_mean = (r[8] or 0.0) / 1e3
+        lines.append(f"{_short(r[0])[:56]:56s} {r[1]:9d} {r[2]:7d} {r[7]:7d} {r[3] / 1e6:9.2f} "
+                     f"{r[4] / 1e3:8.1f} {busy_mean:8.1f} {r[5] / 1e3:7.1f} {r[6] / 1e3:8.1f} "
+                     f"{100 * r[3] / tot:6.2f}")
+    lines.append(f"total kernel time {tot / 1e6:.2f} ms over {sum(r[2] for r in rows)} dispatches; "
+                 f"busy = launches longer than {busy_us:g} us")
     return "\n".join(lines)
 
 
@@ -103,6 +111,7 @@ def main(argv=None):
     a = sub.add_parser("stats")
     a.add_argument("db")
     a.add_argument("--top", type=int, default=40)
+    a.add_argument("--busy-us", type=float, default=3.0)
     b = sub.add_parser("pmc")
     b.add_argument("db")
     b.add_argument("--top", type=int, default=40)
@@ -115,7 +124,7 @@ def main(argv=None):
     c.add_argument("--grid", type=int, default=None)
     args = ap.parse_args(argv)
     if args.cmd == "stats":
-        print(stats(args.db, args.top))
+        print(stats(args.db, args.top, args.busy_us))
     elif args.cmd == "pmc":
         print(pmc(args.db, args.top))
     else:
