@@ -11,7 +11,7 @@ namespace mp {
 
 // A sparse matrix of the hierarchy in one of two layouts, chosen per matrix
 // by timing at set-up:
-//   g == 0: SELL-64, one lane per row (short rows; ptr = slice pointers);
+//   g == 0: SELL-64x4, one lane per row (short rows; ptr = slice pointers);
 //   g  > 0: CSR with g lanes per row and a g-lane reduction (long rows of
 //           the coarse operators and restrictions; ptr = row pointers).
 struct AmgMat {
@@ -44,12 +44,12 @@ struct AmgDevice {
     // host copies of the hierarchy (A[0] stays empty), for re-tuning layouts
     std::vector<mp_amg::Csr> hA, hP, hR;
 
-    // Builds the hierarchy from the finest operator, given in SELL-64 slots
+    // Builds the hierarchy from the finest operator, given in SELL-64x4 slots
     // (device order) by the solver: sv stays owned by the solver and serves as
     // level 0's layout, with a BF16 copy of val32 as its values.
     void setup(hipStream_t s, const SellView& sv, const double* val64_dev,
                const float* val32_dev, const mp_amg::Options& opt);
-    // Layout of every hierarchy matrix chosen by timing each candidate (SELL-64,
+    // Layout of every hierarchy matrix chosen by timing each candidate (SELL-64x4,
     // CSR with 4..64 lanes per row) instead of the set-up's rule of thumb.  The
     // layouts sum a row in different orders, so the V-cycle then changes at the
     // rounding level; callers holding graphs over these buffers must drop them.

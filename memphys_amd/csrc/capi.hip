@@ -9,6 +9,7 @@
 // cut into slices of 64 rows; each slice stores its neighbours column-major
 // (SELL-64).  Every field, weight and operator lives on the device in that
 // order; host arrays crossing the ABI are permuted on the way in and out.
+// Slots follow the SELL-64x4 numbering of mp_internal.hpp.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -387,7 +388,7 @@ int upload_clouds(mp_ctx* c, const char* fn, int nprob, const int* nvp, int dim,
     for (int sl = 0; sl < c->nslices; ++sl) {
         int wmax = 0;
         for (int l = 0; l < SLICE; ++l) wmax = std::max(wmax, hlen[sl * SLICE + l]);
-        hsptr[sl + 1] = hsptr[sl] + wmax * SLICE;
+        hsptr[sl + 1] = hsptr[sl] + sell_width4(wmax) * SLICE;
     }
     c->total_slots = hsptr[c->nslices];
     std::vector<int> hcol(c->total_slots, 0);
@@ -397,7 +398,7 @@ int upload_clouds(mp_ctx* c, const char* fn, int nprob, const int* nvp, int dim,
         for (int l = 0; l < SLICE; ++l) {
             const int d = sl * SLICE + l;
             for (int j = 0; j < wmax; ++j) {
-                const int slot = hsptr[sl] + j * SLICE + l;
+                const int slot = hsptr[sl] + sell_off(j) + l * SGROUP;
                 if (j < hlen[d]) {
                     const int t = row_ptr[c->perm[d]] + j;
                     hcol[slot] = c->iperm[col_idx[t]];

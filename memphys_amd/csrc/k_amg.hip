@@ -11,8 +11,9 @@
 //   post-smooth            y_l = x_l + w D^-1 (b_l - A_l x_l)
 // Level 0 reads its right-hand side from the Krylov vector (FP64) and writes
 // the FP64 result; the border unknown of the full-Neumann system passes
-// through unchanged.  Every matrix is SELL-64: one wavefront per slice of 64
-// rows, columns stored slot-major, padding slots with value 0.
+// through unchanged.  A matrix is SELL-64x4 (one wavefront per slice of 64
+// rows, four neighbours of a row side by side, padding slots with value 0)
+// or CSR with several lanes per row, whichever suits its row lengths.
 #include "amg.hpp"
 
 #include <chrono>
@@ -62,7 +63,7 @@ __device__ __forceinline__ float gathered(int c, const TB* __restrict__ b, const
         return xin[c];
 }
 
-// One multigrid product with its epilogue, in either layout (G == 0: SELL-64,
+// One multigrid product with its epilogue, in either layout (G == 0: SELL-64x4,
 // else CSR with G lanes per row):
 //   OP_PRE : xout = D^-1 b,  res = b - A (D^-1 b)
 //   OP_PRES: res = b - A xin, with xin = D^-1 b already formed
@@ -87,32 +88,9 @@ __global__ __launch_bounds__(BLOCK) void k_amg_op(MatView m, const float* __rest
         i = lb * BLOCK + threadIdx.x;
         const int s = __builtin_amdgcn_readfirstlane(i >> 6);
         if (s >= m.nslices) return;
-        const int lane = threadIdx.x & 63;
-        const int b0 = m.ptr[s];
-        const int w = (m.ptr[s + 1] - b0) >> 6;
-        const int* cp = m.col + b0 + lane;
-        const uint16_t* ap = m.val + b0 + lane;
-        int j = 0;
-        for (; j + 4 <= w; j += 4) {
-            int c[4];
-            float av[4], xv[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) c[u] = __builtin_nontemporal_load(cp + u * SLICE);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) av[u] = bf2f(__builtin_nontemporal_load(ap + u * SLICE));
-#pragma unroll
-            for (int u = 0; u < 4; ++u) xv[u] = gathered<OP>(c[u], b, xin, dinv);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) acc = fmaf(av[u], xv[u], acc);
-            cp += 4 * SLICE;
-            ap += 4 * SLICE;
-        }
-        for (; j < w; ++j) {
-            acc = fmaf(bf2f(__builtin_nontemporal_load(ap)),
-                       gathered<OP>(__builtin_nontemporal_load(cp), b, xin, dinv), acc);
-            cp += SLICE;
-            ap += SLICE;
-        }
+        acc = sell4_accumulate<4, float>(
+            m.ptr, m.col, m.val, s, threadIdx.x & 63, [](uint16_t v) { return bf2f(v); },
+            [=](int c) { return gathered<OP>(c, b, xin, dinv); });
         if (i >= m.n) return;
     } else {
         constexpr int RPB = BLOCK / G;
@@ -206,7 +184,7 @@ HostSell to_sell(const mp_amg::Csr& M) {
             const int r = s * SLICE + l;
             if (r < M.nrows) w = std::max(w, M.ptr[r + 1] - M.ptr[r]);
         }
-        h.sptr[s + 1] = h.sptr[s] + w * SLICE;
+        h.sptr[s + 1] = h.sptr[s] + sell_width4(w) * SLICE;
     }
     h.col.assign(h.sptr[h.nslices], 0);
     h.val.assign(h.sptr[h.nslices], 0);
@@ -217,7 +195,7 @@ HostSell to_sell(const mp_amg::Csr& M) {
             if (r >= M.nrows) continue;
             const int len = M.ptr[r + 1] - M.ptr[r];
             for (int j = 0; j < w && j < len; ++j) {
-                const int slot = h.sptr[s] + j * SLICE + l;
+                const int slot = h.sptr[s] + sell_off(j) + l * SGROUP;
                 h.col[slot] = M.col[M.ptr[r] + j];
                 h.val[slot] = f2bf((float)M.val[M.ptr[r] + j]);
             }
@@ -233,7 +211,7 @@ void put(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
         MP_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
 }
 
-// Uploads M in layout g (0: SELL-64, else CSR with g lanes per row).
+// Uploads M in layout g (0: SELL-64x4, else CSR with g lanes per row).
 void upload_mat(AmgMat& d, const mp_amg::Csr& M, int g, hipStream_t st) {
     d.n = M.nrows;
     d.g = g;
@@ -257,7 +235,7 @@ void upload_mat(AmgMat& d, const mp_amg::Csr& M, int g, hipStream_t st) {
     d.val = d.val_b.p;
 }
 
-// Autotuned layout: every candidate (SELL-64 unless its padding more than
+// Autotuned layout: every candidate (SELL-64x4 unless its padding more than
 // triples the stored entries; CSR with 4..64 lanes per row) is uploaded and
 // timed on a product with this matrix; the fastest stays.
 void upload_tuned(AmgMat& d, const mp_amg::Csr& M, hipStream_t st, float* xin, float* out,
@@ -269,7 +247,7 @@ void upload_tuned(AmgMat& d, const mp_amg::Csr& M, hipStream_t st, float* xin, f
             int w = 0;
             for (int l = 0; l < SLICE && s * SLICE + l < M.nrows; ++l)
                 w = std::max(w, M.ptr[s * SLICE + l + 1] - M.ptr[s * SLICE + l]);
-            slots += (long)w * SLICE;
+            slots += (long)sell_width4(w) * SLICE;
         }
         if (slots <= 3 * std::max<long>(1, M.nnz())) cand.push_back(0);
     }
@@ -301,7 +279,7 @@ void upload_tuned(AmgMat& d, const mp_amg::Csr& M, hipStream_t st, float* xin, f
     report += buf;
 }
 
-// Rule-of-thumb layout (deterministic, used at set-up): SELL-64 for short
+// Rule-of-thumb layout (deterministic, used at set-up): SELL-64x4 for short
 // rows, else CSR with about one lane per 5 entries of the mean row (4..64).
 int rule_layout(const mp_amg::Csr& M) {
     const double avg = (double)M.nnz() / std::max(1, M.nrows);
@@ -351,9 +329,9 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
     A0.col.resize(A0.ptr[nv]);
     A0.val.resize(A0.ptr[nv]);
     for (int i = 0; i < nv; ++i) {
-        const int s = i / SLICE, l = i % SLICE;
+        const int base = sell_base(hsptr.data(), i);
         for (int j = 0; j < hlen[i]; ++j) {
-            const int slot = hsptr[s] + j * SLICE + l;
+            const int slot = base + sell_off(j);
             A0.col[A0.ptr[i] + j] = hcol[slot];
             A0.val[A0.ptr[i] + j] = hval[slot];
         }

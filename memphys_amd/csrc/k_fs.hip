@@ -2,9 +2,9 @@
 // fix-up and continuity residual over SELL-64 stencil operators.
 //
 // Layout: device rows are grouped into slices of 64 (one wavefront); the
-// k-th neighbour of the rows of a slice sits at slot sptr[slice] + k*64 + lane,
-// so every neighbour step is one coalesced 64-lane load of column ids and of
-// each weight array.  Rows are sorted by stencil length inside windows of
+// k-th neighbour of slice row l sits at sptr[slice] + 256*(k/4) + 4*l + k%4
+// (SELL-64x4), so every group of four neighbour steps is one coalesced
+// wavefront load of column ids and of each weight array.  Rows are sorted by stencil length inside windows of
 // 4096 rows, so a slice holds rows of (nearly) one length.  One thread owns
 // one row and accumulates in stencil order with fma, the oracle's order.
 #include "mp_internal.hpp"
@@ -14,7 +14,7 @@ namespace mp {
 namespace {
 
 __device__ __forceinline__ int row_base(const SellView& sv, int r) {
-    return sv.sptr[r >> 6] + (r & 63);
+    return sell_base(sv.sptr, r);
 }
 
 template <int DIM>
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(BLOCK) void predictor_kernel(
 #pragma unroll
         for (int o = 0; o < 4; ++o) acc[f][o] = 0.0;
     for (int j = 0; j < len; ++j) {
-        const int slot = base + j * SLICE;
+        const int slot = base + sell_off(j);
         const int c = sv.col[slot];
         const double a0 = gx[slot], a1 = gy[slot], a3 = lap[slot];
         const double a2 = (DIM == 3) ? gz[slot] : 0.0;
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(BLOCK) void ppe_rhs_kernel(
         const int len = sv.len[r];
         double ax = 0.0, ay = 0.0, az = 0.0;
         for (int j = 0; j < len; ++j) {
-            const int slot = base + j * SLICE;
+            const int slot = base + sell_off(j);
             const int c = sv.col[slot];
             ax = fma(gx[slot], uh[c], ax);
             ay = fma(gy[slot], vh[c], ay);
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(BLOCK) void corrector_kernel(
     const int len = sv.len[r];
     double ax = 0.0, ay = 0.0, az = 0.0;
     for (int j = 0; j < len; ++j) {
-        const int slot = base + j * SLICE;
+        const int slot = base + sell_off(j);
         const double pc = p[sv.col[slot]];
         ax = fma(gx[slot], pc, ax);
         ay = fma(gy[slot], pc, ay);
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(BLOCK) void boundary_fix_kernel(
         } else {
             double rhs = 0.0, diag = 1.0;
             for (int j = 0; j < len; ++j) {
-                const int slot = base + j * SLICE;
+                const int slot = base + sell_off(j);
                 const int c = sv.col[slot];
                 double cf = gx[slot] * nx + gy[slot] * ny;
                 if (DIM == 3) cf = cf + gz[slot] * nz;
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(BLOCK) void spmv_kernel(SellView sv, const double* 
     const int len = sv.len[r];
     double s = 0.0;
     for (int j = 0; j < len; ++j) {
-        const int slot = base + j * SLICE;
+        const int slot = base + sell_off(j);
         s = fma(val[slot], x[sv.col[slot]], s);
     }
     y[r] = s;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(BLOCK) void divergence_kernel(
         const int len = sv.len[r];
         double ax = 0.0, ay = 0.0, az = 0.0;
         for (int j = 0; j < len; ++j) {
-            const int slot = base + j * SLICE;
+            const int slot = base + sell_off(j);
             const int c = sv.col[slot];
             ax = fma(gx[slot], u[c], ax);
             ay = fma(gy[slot], v[c], ay);
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(BLOCK) void build_operator_kernel(
     const int len = sv.len[r];
     double dg = 0.0;
     for (int j = 0; j < len; ++j) {
-        const int slot = base + j * SLICE;
+        const int slot = base + sell_off(j);
         const int c = sv.col[slot];
         double val;
         if (bnd[r]) {

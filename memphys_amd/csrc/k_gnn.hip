@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_dense_t(int nv, const float* __restrict
     }
 }
 
-// m[i][:] = sum_j a_ij h[j][:] over a SELL-64 operator; one wavefront per row,
+// m[i][:] = sum_j a_ij h[j][:] over a SELL-64x4 operator; one wavefront per row,
 // lane = feature.  accumulate: m += instead of m =.
 __global__ __launch_bounds__(256) void k_aggregate(SellView sv, const float* __restrict__ val,
                                                    const float* __restrict__ h,
@@ -130,11 +130,11 @@ __global__ __launch_bounds__(256) void k_aggregate(SellView sv, const float* __r
     const int row = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (row >= sv.nv) return;
     const int c = threadIdx.x & 63;
-    const int base = sv.sptr[row >> 6] + (row & 63);
+    const int base = sell_base(sv.sptr, row);
     const int len = sv.len[row];
     float acc = accumulate ? m[(size_t)row * H + c] : 0.0f;
     for (int j = 0; j < len; ++j) {
-        const int slot = base + j * SLICE;
+        const int slot = base + sell_off(j);
         const int col = sv.col[slot];
         acc = fmaf(val[slot], h[(size_t)col * H + c], acc);
     }
@@ -323,7 +323,7 @@ void GuessNet::setup(hipStream_t st, const SellView& view, const float* a32_slot
     MP_HIP(hipMemcpy(hval.data(), a32_slots, sizeof(float) * nslots_total, hipMemcpyDeviceToHost));
     std::vector<int> tcnt(nv + 1, 0);
     for (int r = 0; r < nv; ++r)
-        for (int j = 0; j < hlen[r]; ++j) tcnt[hcol[hsptr[r / SLICE] + j * SLICE + r % SLICE] + 1]++;
+        for (int j = 0; j < hlen[r]; ++j) tcnt[hcol[sell_base(hsptr.data(), r) + sell_off(j)] + 1]++;
     std::vector<int> trp(nv + 1, 0);
     for (int i = 0; i < nv; ++i) trp[i + 1] = trp[i] + tcnt[i + 1];
     std::vector<int> tc(trp[nv]);
@@ -331,7 +331,7 @@ void GuessNet::setup(hipStream_t st, const SellView& view, const float* a32_slot
     std::vector<int> fill(trp.begin(), trp.end() - 1);
     for (int r = 0; r < nv; ++r)
         for (int j = 0; j < hlen[r]; ++j) {
-            const int slot = hsptr[r / SLICE] + j * SLICE + r % SLICE;
+            const int slot = sell_base(hsptr.data(), r) + sell_off(j);
             const int c = hcol[slot];
             tc[fill[c]] = r;
             tv[fill[c]] = hval[slot];
@@ -342,13 +342,13 @@ void GuessNet::setup(hipStream_t st, const SellView& view, const float* a32_slot
     for (int s = 0; s < nsl; ++s) {
         int w = 0;
         for (int l = 0; l < SLICE; ++l) w = std::max(w, tl[s * SLICE + l]);
-        tsp[s + 1] = tsp[s] + w * SLICE;
+        tsp[s + 1] = tsp[s] + sell_width4(w) * SLICE;
     }
     std::vector<int> tcol_h(std::max(1, tsp[nsl]), 0);
     std::vector<float> tval_h(std::max(1, tsp[nsl]), 0.0f);
     for (int i = 0; i < nv; ++i)
         for (int j = 0; j < tl[i]; ++j) {
-            const int slot = tsp[i / SLICE] + j * SLICE + i % SLICE;
+            const int slot = sell_base(tsp.data(), i) + sell_off(j);
             tcol_h[slot] = tc[trp[i] + j];
             tval_h[slot] = tv[trp[i] + j];
         }

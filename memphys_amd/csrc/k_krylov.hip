@@ -118,43 +118,18 @@ __device__ void loop_top(KScalars* sc, double rr, double r0r) {
     sc->beta = (sc->rho / sc->rho_old) * (sc->alpha / sc->omega);
 }
 
-// Row r of the SELL-64 product A x: the slice width is uniform across the
-// wavefront, so the neighbour loop runs in lockstep, unrolled by eight with
-// the column/value loads issued ahead of the gathers.  Padding slots hold a
-// zero value and a valid column, which leaves the fma chain unchanged.
-// Matrix streams use non-temporal loads so the gathered vector stays cached.
+// Row of the SELL-64x4 product A x for slice s, lane = slice row: the slice
+// width is uniform across the wavefront, so the neighbour loop runs in
+// lockstep.  Padding slots hold a zero value and a valid column, which leaves
+// the fma chain unchanged.
 template <typename TM, typename T>
 __device__ __forceinline__ T sell_dot(const int* __restrict__ sptr, const int* __restrict__ col,
                                       const TM* __restrict__ a, const T* __restrict__ x, int s,
                                       int lane) {
-    const int b0 = sptr[s];
-    const int w = (sptr[s + 1] - b0) >> 6;
-    const int* cp = col + b0 + lane;
-    const TM* ap = a + b0 + lane;
-    T acc = (T)0;
-    int j = 0;
-    for (; j + 8 <= w; j += 8) {
-        int c[8];
-        T av[8], xv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) c[u] = __builtin_nontemporal_load(cp + u * SLICE);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) av[u] = (T)__builtin_nontemporal_load(ap + u * SLICE);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) acc = fma(av[u], xv[u], acc);
-        cp += 8 * SLICE;
-        ap += 8 * SLICE;
-    }
-    for (; j < w; ++j) {
-        const int c0 = __builtin_nontemporal_load(cp);
-        const T a0 = (T)__builtin_nontemporal_load(ap);
-        acc = fma(a0, x[c0], acc);
-        cp += SLICE;
-        ap += SLICE;
-    }
-    return acc;
+    // FP64 values: two groups per iteration keep the register count at eight
+    // waves per SIMD; narrower values take four.
+    return sell4_accumulate<(sizeof(TM) == 8 ? 2 : 4), T>(
+        sptr, col, a, s, lane, [](TM v) { return (T)v; }, [x](int c) { return x[c]; });
 }
 
 // Slice of this wavefront under the XCD-aware numbering (lb: logical block).
@@ -532,10 +507,10 @@ __global__ __launch_bounds__(BLOCK) void k_scale_columns(SellView sv, const doub
                                                          T* __restrict__ out) {
     const int r = blockIdx.x * BLOCK + threadIdx.x;
     if (r >= sv.nv) return;
-    const int base = sv.sptr[r >> 6] + (r & 63);
+    const int base = sell_base(sv.sptr, r);
     const int len = sv.len[r];
     for (int j = 0; j < len; ++j) {
-        const int slot = base + j * SLICE;
+        const int slot = base + sell_off(j);
         out[slot] = (T)(a[slot] * dinv[sv.col[slot]]);
     }
 }

@@ -55,11 +55,11 @@ __global__ __launch_bounds__(256) void weights_fp64_kernel(
     double* A = lds;
     double* R = A + M * LDA;
     double* xs = R + M * 4;
-    const int base = sv.sptr[r / SLICE] + (r % SLICE);
+    const int base = sell_base(sv.sptr, r);
 
     for (int i = tid; i < m * dim; i += blockDim.x) s_e[i] = exps[i];
     for (int t = tid; t < k; t += blockDim.x) {
-        const int c = sv.col[base + t * SLICE];
+        const int c = sv.col[base + sell_off(t)];
         for (int a = 0; a < dim; ++a) xs[t * 3 + a] = xyz[(size_t)a * npad + c];
     }
     __syncthreads();
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void weights_fp64_kernel(
         __syncthreads();
     }
     for (int t = tid; t < k; t += blockDim.x) {
-        const size_t slot = (size_t)base + (size_t)t * SLICE;
+        const size_t slot = (size_t)base + (size_t)sell_off(t);
         gx[slot] = R[t * 4 + 0];
         gy[slot] = R[t * 4 + 1];
         if (dim == 3) gz[slot] = R[t * 4 + 2];
@@ -420,12 +420,12 @@ __global__ __launch_bounds__(256) void weights_mixed_kernel(
     float* Af = (float*)(xs + k * 3);      // FP32 LU factors [M][LDA]
     float* W = Af + M * LDA;               // FP32 work [M][4]
     int* perm = (int*)(W + M * 4);         // row permutation [M]
-    const int base = sv.sptr[r / SLICE] + (r % SLICE);
+    const int base = sell_base(sv.sptr, r);
     (void)s_bestf;
 
     for (int i = tid; i < m * dim; i += blockDim.x) s_e[i] = exps[i];
     for (int t = tid; t < k; t += blockDim.x) {
-        const int c = sv.col[base + t * SLICE];
+        const int c = sv.col[base + sell_off(t)];
         for (int a = 0; a < dim; ++a) xs[t * 3 + a] = xyz[(size_t)a * npad + c];
     }
     __syncthreads();
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(256) void weights_mixed_kernel(
         return;
     }
     for (int t = tid; t < k; t += blockDim.x) {
-        const size_t slot = (size_t)base + (size_t)t * SLICE;
+        const size_t slot = (size_t)base + (size_t)sell_off(t);
         gx[slot] = X[t * 4 + 0];
         gy[slot] = X[t * 4 + 1];
         if (dim == 3) gz[slot] = X[t * 4 + 2];

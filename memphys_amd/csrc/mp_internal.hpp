@@ -11,6 +11,7 @@
 namespace mp {
 
 constexpr int SLICE = 64;       // rows per SELL slice (one wavefront)
+constexpr int SGROUP = 4;       // neighbours of a row stored side by side in a slice
 constexpr int SIGMA = 4096;     // rows per length-sorting window
 constexpr int BLOCK = 256;      // threads per block of the row kernels
 constexpr int MAX_STENCIL = 96; // largest stencil (k) the weight kernel accepts
@@ -75,7 +76,91 @@ __device__ __forceinline__ int xcd_block(int g) {
     return lb < g ? lb : -1;
 }
 
-// SELL-64 view of the stencil set in device order.
+// SELL-64x4 slot numbering: slice s holds rows 64s..64s+63; neighbour j of
+// slice row l sits at sptr[s] + 256*(j/4) + 4*l + j%4, so each lane reads its
+// row's neighbours four at a time (one 16-byte load of columns or FP32
+// values) and the wavefront still covers 256 consecutive slots per group.
+// Slice widths are padded to a multiple of 4; padding slots hold the row's
+// own column and value 0.
+__host__ __device__ inline int sell_base(const int* sptr, int r) {
+    return sptr[r >> 6] + ((r & 63) << 2);
+}
+__host__ __device__ inline int sell_off(int j) { return ((j >> 2) << 8) + (j & 3); }
+__host__ __device__ inline int sell_width4(int w) { return (w + 3) & ~3; }
+
+template <typename T>
+struct Vec4 {
+    typedef T type __attribute__((ext_vector_type(4)));
+};
+
+// Sum over the neighbours of one slice row, in neighbour order, of
+// fv(value) * fx(column), as an fma chain: slice s, lane = slice row.  The
+// four neighbours of a group come in one vector load of columns and one of
+// values; U4 groups are loaded before their gathers.  Columns and values up
+// to 4 bytes use non-temporal loads so the gathered vector stays cached; FP64
+// values (32 bytes a lane, fetched as two 16-byte halves) use ordinary loads,
+// which keep the line in cache between the halves.
+template <typename V>
+__device__ __forceinline__ V load_values(const V* p) {
+    if constexpr (sizeof(V) > 16)
+        return *p;
+    else
+        return __builtin_nontemporal_load(p);
+}
+
+template <int U4, typename T, typename TM, typename FV, typename FX>
+__device__ __forceinline__ T sell4_accumulate(const int* __restrict__ sptr,
+                                              const int* __restrict__ col,
+                                              const TM* __restrict__ a, int s, int lane, FV fv,
+                                              FX fx) {
+    typedef typename Vec4<int>::type c4;
+    typedef typename Vec4<TM>::type v4;
+    const int b0 = sptr[s];
+    const int w4 = (sptr[s + 1] - b0) >> 8;
+    const c4* cp = reinterpret_cast<const c4*>(col + b0) + lane;
+    const v4* ap = reinterpret_cast<const v4*>(a + b0) + lane;
+    T acc = (T)0;
+    int g = 0;
+    for (; g + U4 <= w4; g += U4) {
+        c4 c[U4];
+        v4 v[U4];
+        T xv[4 * U4];
+#pragma unroll
+        for (int u = 0; u < U4; ++u) c[u] = __builtin_nontemporal_load(cp + u * SLICE);
+#pragma unroll
+        for (int u = 0; u < U4; ++u) v[u] = load_values(ap + u * SLICE);
+#pragma unroll
+        for (int u = 0; u < U4; ++u) {
+            xv[4 * u + 0] = fx(c[u].x);
+            xv[4 * u + 1] = fx(c[u].y);
+            xv[4 * u + 2] = fx(c[u].z);
+            xv[4 * u + 3] = fx(c[u].w);
+        }
+#pragma unroll
+        for (int u = 0; u < U4; ++u) {
+            acc = fma(fv(v[u].x), xv[4 * u + 0], acc);
+            acc = fma(fv(v[u].y), xv[4 * u + 1], acc);
+            acc = fma(fv(v[u].z), xv[4 * u + 2], acc);
+            acc = fma(fv(v[u].w), xv[4 * u + 3], acc);
+        }
+        cp += U4 * SLICE;
+        ap += U4 * SLICE;
+    }
+    for (; g < w4; ++g) {
+        const c4 c = __builtin_nontemporal_load(cp);
+        const v4 v = load_values(ap);
+        const T x0 = fx(c.x), x1 = fx(c.y), x2 = fx(c.z), x3 = fx(c.w);
+        acc = fma(fv(v.x), x0, acc);
+        acc = fma(fv(v.y), x1, acc);
+        acc = fma(fv(v.z), x2, acc);
+        acc = fma(fv(v.w), x3, acc);
+        cp += SLICE;
+        ap += SLICE;
+    }
+    return acc;
+}
+
+// SELL-64x4 view of the stencil set in device order.
 struct SellView {
     int nv;            // real rows
     int nslices;
