@@ -4,17 +4,21 @@ Workload (one GPU): 100^3 lattice on [0,1]^3, interior jitter +-0.3h (seeded),
 edges/corners removed -> 998,816 nodes; PHS r^3 + degree-3 polynomial (20
 monomials); stencils of 40 nodes, 60/55/50 on the face layer and the two
 layers under it; Re = 400 (rho = 1, mu = 0.0025), lid u = 1 on z = 1,
-dt = 5e-4, AB2; pressure-Poisson relative residual 1e-8.
+dt = 5e-4, AB2; pressure-Poisson relative residual 1e-9 (--tol).
 
-Two forms are timed over the same number of steps from the same warmed-up
-state: the plain FP64 path (FP64 BiCGSTAB, previous pressure as the guess, as
-the reference's solveWithGuess(rhs, p_old)) and the headline form (FP32-operator
-inner BiCGSTAB inside FP64 defect correction, learned guess).  The learned
-guess is trained first, untimed, on a 13,824-node cube with the same physics:
-2000 solver steps in 10 episodes from rest with seeded random lid speeds in
-[0.5, 1.5].  `value` is the headline form's timesteps/s; the FP64 rate, the
-ratio, the residuals and the velocity difference between the two are reported
-beside it.
+Forms timed over the same number of steps from the same warmed-up state: the
+plain FP64 path (FP64 Jacobi-scaled BiCGSTAB, previous pressure as the guess,
+as the reference's solveWithGuess(rhs, p_old)), the mixed form (FP32-operator
+inner BiCGSTAB inside FP64 defect correction) and the headline form (the
+mixed form preconditioned by an FP32 smoothed-aggregation multigrid V-cycle,
+with the learned guess).  The learned guess is trained first, untimed, on a
+13,552-node cube with the same physics: 2000 solver steps in 10 episodes from
+rest with seeded random lid speeds in [0.5, 1.5].  `value` is the headline
+form's timesteps/s; the FP64 rate, the ratio, the residuals and the velocity
+and pressure differences between the two are reported beside it.  The
+roofline entry times the PPE SpMV with HIP events on the library's stream,
+over the launches that did the work (launches of a system that converged
+inside a chunk of iterations return at once and are counted apart).
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -112,6 +116,32 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
     return dict(seconds=t1 - t0, iters=iters, rel_res=res, spmv=spmv, predictor=pred,
                 fields=fields, divergence=div.tolist(), hierarchy=hier, create_s=t_create,
                 poll_interval=interval)
+
+
+def stream_copy_gbps(mib: int = 1024, reps: int = 10) -> float:
+    """HBM rate of a plain elementwise kernel (torch.neg: reads and writes
+    `mib` MiB once each), timed with events on torch's stream it runs on;
+    median of `reps` launches.  A practical streaming ceiling beside the
+    published peak, and the calibration kernel of HBM byte counters (it moves
+    exactly 2 x `mib` MiB)."""
+    import torch
+    n = (mib << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.neg(src, out=dst)
+    times = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        torch.neg(src, out=dst)
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    del src, dst
+    torch.cuda.empty_cache()
+    ms = sorted(times)[len(times) // 2]
+    return 2.0 * (mib << 20) / (ms * 1e-3) / 1e9
 
 
 def run_batched_cavities(M, steps, seq_steps, tol, nprob=64):
@@ -256,9 +286,6 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel instead of replaying hipGraphs (same results; "
                          "for profilers)")
-    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
-                    help="first copy MIB MiB device-to-device three times (a plain copy kernel "
-                         "of known size, the scale for HBM counter runs)")
     ap.add_argument("--forms", default="fp64,mixed,headline",
                     help="comma list of forms to time besides fp64: mixed (FP32-inner, "
                          "Jacobi), headline (FP32-inner, multigrid, learned guess)")
@@ -271,17 +298,9 @@ def main():
 
     if args.no_graphs:
         M.Context.default_graphs = False
-    if args.copy_calib > 0:
-        import torch
-        nbytes = args.copy_calib << 20
-        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):  # elementwise kernel: reads and writes nbytes once each
-            torch.neg(src, out=dst)
-        torch.cuda.synchronize()
-        log(f"copy calibration: 3 x {nbytes} bytes read + written (torch.neg)")
-        del src, dst
-
+    # torch's own HIP runtime initialises only before the library's does
+    copy_gbps = stream_copy_gbps()
+    log(f"streaming ceiling (torch.neg, 1 GiB each way): {copy_gbps:.0f} GB/s")
     prm = dict(rho=1.0, mu=0.0025, dt=5e-4, tol=args.tol, n_iter=20000, inner_tol=1e-5,
                check_every=8)
     t = time.perf_counter()
@@ -350,9 +369,9 @@ def main():
 
     sps_head = args.steps / head["seconds"]
     sps_fp64 = args.steps / fp64["seconds"]
-    n_l, ms, bytes_l = head["spmv"]
+    n_l, ms, bytes_l, all_l = head["spmv"]
     achieved = bytes_l / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    n64, ms64, b64 = fp64["spmv"]
+    n64, ms64, b64, all64 = fp64["spmv"]
     ach64 = b64 / (ms64 * 1e-3) / 1e9 if ms64 > 0 else 0.0
     out = {
         "metric": METRIC,
@@ -382,6 +401,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.n, 0),
             "kernel": "k_spmv_alpha/k_spmv_omega<float,double> (PPE SpMV, FP32 operator)",
             "bytes_per_launch": bytes_l, "mean_ms": ms, "launches": n_l,
+            "launches_returned_early": all_l - n_l,
+            "copy_kernel_GBps": copy_gbps, "frac_of_copy": achieved / copy_gbps,
         },
         "fp64_timesteps_per_s": sps_fp64,
         "speedup_vs_fp64": sps_head / sps_fp64,
@@ -397,7 +418,9 @@ def main():
         "max_ppe_rel_res": {"fp64": max(fp64["rel_res"]), "headline": max(head["rel_res"])},
         "headline_vs_fp64_rel_diff": {"velocity": vdiff, "pressure": pdiff},
         "fp64_spmv_roofline": {"achieved": ach64, "frac": ach64 / HBM_PEAK_GBS,
-                               "bytes_per_launch": b64, "mean_ms": ms64,
+                               "frac_of_copy": ach64 / copy_gbps,
+                               "bytes_per_launch": b64, "mean_ms": ms64, "launches": n64,
+                               "launches_returned_early": all64 - n64,
                                "traffic": measured_traffic(args.n, 1)},
         "weights_kernel_ms": wk[1],
         "weights_fp64_vs_fp32_refined": weights_check,

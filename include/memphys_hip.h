@@ -206,10 +206,15 @@ int mp_graph_mode(mp_ctx* ctx, int enable);
 
 /* Per-kernel timing with HIP events on the library's stream.
  * which: 0 = PPE SpMV, 1 = predictor, 2 = pressure RHS, 3 = corrector,
- *        4 = stencil weights.  Reports launches and mean milliseconds. */
+ *        4 = stencil weights.  Reports the launches that did the work and
+ * their mean milliseconds: Krylov launches of a system that converged inside
+ * a chunk of iterations return at once and are left out (a launch counts
+ * when it takes at least half the median launch's time).  all_launches (may
+ * be NULL) receives every timed launch.  bytes_per_launch: the kernel's
+ * algorithmic HBM bytes per working launch. */
 int mp_kernel_timing(mp_ctx* ctx, int enable);
 int mp_kernel_timing_get(mp_ctx* ctx, int which, long* launches, double* mean_ms,
-                         double* bytes_per_launch);
+                         double* bytes_per_launch, long* all_launches);
 
 #ifdef __cplusplus
 }

@@ -99,7 +99,8 @@ def hip_lib() -> C.CDLL:
         lib.mp_divergence.argtypes = [vp, _dp]
         lib.mp_kernel_timing.argtypes = [vp, C.c_int]
         lib.mp_kernel_timing_get.argtypes = [vp, C.c_int, C.POINTER(C.c_long),
-                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_long)]
         lib.mp_guess_param_count.argtypes = []
         lib.mp_guess_train.argtypes = [vp, C.c_int, C.c_double, C.c_ulonglong]
         lib.mp_guess_params.argtypes = [vp, C.c_int, np.ctypeslib.ndpointer(dtype=np.float32,
@@ -467,12 +468,16 @@ class Context:
         self._check(self.lib.mp_kernel_timing(self.h, 1 if enable else 0), "mp_kernel_timing")
 
     def kernel_timing_get(self, which: int):
+        """(working launches, their mean ms, algorithmic bytes per launch, all launches):
+        launches of a converged system, which return at once, are not working ones."""
         n = C.c_long()
         ms = C.c_double()
         b = C.c_double()
+        a = C.c_long()
         self._check(self.lib.mp_kernel_timing_get(self.h, which, C.byref(n), C.byref(ms),
-                                                  C.byref(b)), "mp_kernel_timing_get")
-        return n.value, ms.value, b.value
+                                                  C.byref(b), C.byref(a)),
+                    "mp_kernel_timing_get")
+        return n.value, ms.value, b.value, a.value
 
 
 # ----------------------------------------------------------------------------- learned guess

@@ -22,6 +22,8 @@ struct AmgMat {
     const int* ptr = nullptr;
     const int* col = nullptr;
     const uint16_t* val = nullptr;
+    const uint16_t* col16 = nullptr;  // compressed columns of level 0 (the solver's)
+    const int* cbase = nullptr;
 };
 
 struct AmgLevelDev {

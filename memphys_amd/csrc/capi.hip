@@ -46,7 +46,8 @@ struct mp_ctx {
     bool per_problem_prm = false;
     std::vector<long> prob_total_iters;  // Krylov iterations per problem since mp_fs_create
     std::vector<uint8_t> h_bnd;
-    DevBuf<int> sptr, col, len, brows;
+    DevBuf<int> sptr, col, len, brows, cbase;
+    DevBuf<uint16_t> col16;
     int nb = 0;
     DevBuf<double> xyz, normal;
     DevBuf<uint8_t> bnd;
@@ -280,6 +281,37 @@ namespace {
 // Uploads nprob clouds given back to back in caller order (problem q owns
 // caller nodes [off_q, off_q + nvp[q]); its stencils may only reference its
 // own nodes).  One problem is the ordinary single-cloud case.
+// 16-bit column form of the SELL slots: per slice, the sorted distinct
+// columns are covered greedily by windows of 2^CW_BITS ids; a slice needing
+// at most CW_COUNT windows stores (window << CW_BITS) | (column - base) per
+// slot, any other keeps its 32-bit columns (first base -1).
+void compress_columns(const std::vector<int>& sptr, const std::vector<int>& col, int nslices,
+                      std::vector<uint16_t>& c16, std::vector<int>& cb) {
+    c16.assign(col.size(), 0);
+    cb.assign((size_t)nslices * CW_COUNT, -1);
+    std::vector<int> u;
+    for (int sl = 0; sl < nslices; ++sl) {
+        u.assign(col.begin() + sptr[sl], col.begin() + sptr[sl + 1]);
+        if (u.empty()) continue;
+        std::sort(u.begin(), u.end());
+        int base[CW_COUNT];
+        int nw = 0;
+        for (size_t i = 0; i < u.size() && nw <= CW_COUNT;) {
+            if (nw < CW_COUNT) base[nw] = u[i];
+            ++nw;
+            const int end = u[i] + (1 << CW_BITS);
+            while (i < u.size() && u[i] < end) ++i;
+        }
+        if (nw > CW_COUNT) continue;
+        for (int w = 0; w < CW_COUNT; ++w) cb[(size_t)sl * CW_COUNT + w] = base[std::min(w, nw - 1)];
+        for (int t = sptr[sl]; t < sptr[sl + 1]; ++t) {
+            int w = nw - 1;
+            while (base[w] > col[t]) --w;
+            c16[t] = (uint16_t)((w << CW_BITS) | (col[t] - base[w]));
+        }
+    }
+}
+
 int upload_clouds(mp_ctx* c, const char* fn, int nprob, const int* nvp, int dim,
                   const double* xyz, const uint8_t* boundary, const double* normal,
                   const int* row_ptr, const int* col_idx) {
@@ -433,7 +465,12 @@ int upload_clouds(mp_ctx* c, const char* fn, int nprob, const int* nvp, int dim,
         for (int b = 0; b < c->prob_rows[q] / BLOCK; ++b) hbp[c->prob_row0[q] / BLOCK + b] = q;
     c->nb = (int)hb.size();
     c->sptr.alloc(hsptr.size());
+    std::vector<uint16_t> hcol16;
+    std::vector<int> hcb;
+    compress_columns(hsptr, hcol, c->nslices, hcol16, hcb);
     c->col.alloc(hcol.size());
+    c->col16.alloc(hcol16.size());
+    c->cbase.alloc(hcb.size());
     c->len.alloc(hlen.size());
     c->brows.alloc(std::max<size_t>(1, hb.size()));
     c->xyz.alloc(hx.size());
@@ -442,6 +479,9 @@ int upload_clouds(mp_ctx* c, const char* fn, int nprob, const int* nvp, int dim,
     c->blk_prob.alloc(hbp.size());
     MP_HIP(hipMemcpy(c->sptr.p, hsptr.data(), sizeof(int) * hsptr.size(), hipMemcpyHostToDevice));
     MP_HIP(hipMemcpy(c->col.p, hcol.data(), sizeof(int) * hcol.size(), hipMemcpyHostToDevice));
+    MP_HIP(hipMemcpy(c->col16.p, hcol16.data(), sizeof(uint16_t) * hcol16.size(),
+                     hipMemcpyHostToDevice));
+    MP_HIP(hipMemcpy(c->cbase.p, hcb.data(), sizeof(int) * hcb.size(), hipMemcpyHostToDevice));
     MP_HIP(hipMemcpy(c->len.p, hlen.data(), sizeof(int) * hlen.size(), hipMemcpyHostToDevice));
     if (!hb.empty())
         MP_HIP(hipMemcpy(c->brows.p, hb.data(), sizeof(int) * hb.size(), hipMemcpyHostToDevice));
@@ -453,6 +493,8 @@ int upload_clouds(mp_ctx* c, const char* fn, int nprob, const int* nvp, int dim,
     c->sv.nslices = c->nslices;
     c->sv.sptr = c->sptr.p;
     c->sv.col = c->col.p;
+    c->sv.col16 = c->col16.p;
+    c->sv.cbase = c->cbase.p;
     c->sv.len = c->len.p;
     for (auto& wb : c->w) {
         wb.alloc(c->total_slots);
@@ -1140,34 +1182,34 @@ int mp_kernel_timing(mp_ctx* c, int enable) {
     return guarded(c, "mp_kernel_timing", [&]() -> int {
         for (auto& t : c->timers) {
             t.enabled = enable != 0;
-            t.launches = 0;
-            t.total_ms = 0.0;
+            t.reset();
         }
         for (PpeSolver* s : {c->solver.get(), c->ppe.get()})
             if (s) {
                 s->spmv_timer.enabled = enable != 0;
-                s->spmv_timer.launches = 0;
-                s->spmv_timer.total_ms = 0.0;
+                s->spmv_timer.reset();
             }
         return MP_OK;
     });
 }
 
 int mp_kernel_timing_get(mp_ctx* c, int which, long* launches, double* mean_ms,
-                         double* bytes_per_launch) {
+                         double* bytes_per_launch, long* all_launches) {
     return guarded(c, "mp_kernel_timing_get", [&]() -> int {
         if (which < 0 || which > 4 || !launches || !mean_ms || !bytes_per_launch)
             return fail(c, MP_ERR_ARG, "mp_kernel_timing_get: bad arguments");
         if (which == 0) {
             PpeSolver* s = c->ppe ? c->ppe.get() : c->solver.get();
             if (!s) return fail(c, MP_ERR_STATE, "mp_kernel_timing_get: no solver");
-            *launches = s->spmv_timer.launches;
-            *mean_ms = s->spmv_timer.launches ? s->spmv_timer.total_ms / s->spmv_timer.launches : 0;
+            long all = 0;
+            s->spmv_timer.summary(launches, mean_ms, &all);
+            if (all_launches) *all_launches = all;
             *bytes_per_launch = s->spmv_bytes(s->precision == MP_PREC_MIXED ? 1 : 0);
         } else {
             KernelTimer& t = c->timers[which];
-            *launches = t.launches;
-            *mean_ms = t.launches ? t.total_ms / t.launches : 0.0;
+            long all = 0;
+            t.summary(launches, mean_ms, &all);
+            if (all_launches) *all_launches = all;
             *bytes_per_launch = c->bytes[which];
         }
         return MP_OK;

@@ -33,7 +33,9 @@ struct MatView {
     int n, nslices;
     const int* ptr;
     const int* col;
-    const uint16_t* val;  // BF16
+    const uint16_t* val;    // BF16
+    const uint16_t* col16;  // compressed columns (SELL-64x4 only; may be null)
+    const int* cbase;
 };
 
 __device__ __forceinline__ float bf2f(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
@@ -89,7 +91,7 @@ __global__ __launch_bounds__(BLOCK) void k_amg_op(MatView m, const float* __rest
         const int s = __builtin_amdgcn_readfirstlane(i >> 6);
         if (s >= m.nslices) return;
         acc = sell4_accumulate<4, float>(
-            m.ptr, m.col, m.val, s, threadIdx.x & 63, [](uint16_t v) { return bf2f(v); },
+            m.ptr, m.col, m.col16, m.cbase, m.val, s, threadIdx.x & 63, [](uint16_t v) { return bf2f(v); },
             [=](int c) { return gathered<OP>(c, b, xin, dinv); });
         if (i >= m.n) return;
     } else {
@@ -144,7 +146,7 @@ template <int G, int OP, typename TB, typename TO>
 void launch_g(hipStream_t st, const AmgMat& M, const float* dinv, const TB* b, const float* xin,
               const float* base, float* xout, float* res, TO* out, const double* bin,
               double* bout, const int* done) {
-    const MatView v{M.n, M.nslices, M.ptr, M.col, M.val};
+    const MatView v{M.n, M.nslices, M.ptr, M.col, M.val, M.col16, M.cbase};
     int nb;
     if constexpr (G == 0)
         nb = nblk_rows(M.nslices);
@@ -358,6 +360,8 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
                 d.A.nnz = A.nnz();
                 d.A.ptr = sv.sptr;
                 d.A.col = sv.col;
+                d.A.col16 = sv.col16;
+                d.A.cbase = sv.cbase;
                 // BF16 copy of the solver's FP32 slots (padding slots stay 0)
                 d.A.val_b.alloc((size_t)slots);
                 hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((slots + BLOCK - 1) / BLOCK)),

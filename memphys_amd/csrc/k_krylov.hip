@@ -123,13 +123,16 @@ __device__ void loop_top(KScalars* sc, double rr, double r0r) {
 // lockstep.  Padding slots hold a zero value and a valid column, which leaves
 // the fma chain unchanged.
 template <typename TM, typename T>
-__device__ __forceinline__ T sell_dot(const int* __restrict__ sptr, const int* __restrict__ col,
-                                      const TM* __restrict__ a, const T* __restrict__ x, int s,
-                                      int lane) {
+__device__ __forceinline__ T sell_dot(const SellView& sv, const TM* __restrict__ a,
+                                      const T* __restrict__ x, int s, int lane) {
     // FP64 values: two groups per iteration keep the register count at eight
-    // waves per SIMD; narrower values take four.
-    return sell4_accumulate<(sizeof(TM) == 8 ? 2 : 4), T>(
-        sptr, col, a, s, lane, [](TM v) { return (T)v; }, [x](int c) { return x[c]; });
+    // waves per SIMD, and the 32-bit columns (decoding 16-bit ones there makes
+    // the compiler serialise the gathers); narrower values take four groups
+    // and the compressed columns.
+    constexpr bool wide = sizeof(TM) == 8;
+    return sell4_accumulate<(wide ? 2 : 4), T>(
+        sv.sptr, sv.col, wide ? nullptr : sv.col16, sv.cbase, a, s, lane,
+        [](TM v) { return (T)v; }, [x](int c) { return x[c]; });
 }
 
 // Slice of this wavefront under the XCD-aware numbering (lb: logical block).
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(BLOCK) void k_residual_init(
     const int s = wave_slice(lb);
     double vals[3] = {0.0, 0.0, 0.0};  // r.r, rhs.rhs, sum D^-1 x
     if (s < sv.nslices) {
-        T ax = sell_dot(sv.sptr, sv.col, a, x, s, threadIdx.x & 63);
+        T ax = sell_dot(sv, a, x, s, threadIdx.x & 63);
         if (i < iend) {
             if (S.border) ax = ax + x[iend];
             const T ri = rhs[i] - ax;
@@ -272,7 +275,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_alpha(SellView sv, SegView sg,
     const int s = wave_slice(lb);
     double vals[2] = {0.0, 0.0};
     if (s < sv.nslices) {
-        T yv = sell_dot(sv.sptr, sv.col, a, y, s, threadIdx.x & 63);
+        T yv = sell_dot(sv, a, y, s, threadIdx.x & 63);
         if (i < iend) {
             if (S.border) yv = yv + y[iend];
             v[i] = yv;
@@ -341,7 +344,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_omega(SellView sv, SegView sg,
     const int sl = wave_slice(lb);
     double vals[3] = {0.0, 0.0, 0.0};
     if (sl < sv.nslices) {
-        T y = sell_dot(sv.sptr, sv.col, a, z, sl, threadIdx.x & 63);
+        T y = sell_dot(sv, a, z, sl, threadIdx.x & 63);
         if (i < iend) {
             if (S.border) y = y + z[iend];
             t[i] = y;
@@ -426,7 +429,7 @@ __global__ __launch_bounds__(BLOCK) void k_residual64(SellView sv, SegView sg,
     const int s = wave_slice(lb);
     double vals[3] = {0.0, 0.0, 0.0};
     if (s < sv.nslices) {
-        double ax = sell_dot(sv.sptr, sv.col, a, x, s, threadIdx.x & 63);
+        double ax = sell_dot(sv, a, x, s, threadIdx.x & 63);
         if (i < iend) {
             if (S.border) ax = ax + x[iend];
             const double ri = b[i] - ax;
@@ -552,12 +555,26 @@ void KernelTimer::end(hipStream_t s) {
 void KernelTimer::harvest() {
     for (size_t i = 0; i + 1 < used; i += 2) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess && ms > 0.0f) {
-            total_ms += ms;
-            launches += 1;
-        }
+        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess && ms > 0.0f)
+            samples.push_back(ms);
     }
     used = 0;
+}
+void KernelTimer::summary(long* working, double* mean_ms, long* all) const {
+    *working = 0;
+    *mean_ms = 0.0;
+    *all = (long)samples.size();
+    if (samples.empty()) return;
+    std::vector<float> v(samples);
+    std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+    const float cut = 0.5f * v[v.size() / 2];
+    double tot = 0.0;
+    for (float ms : samples)
+        if (ms >= cut) {
+            tot += ms;
+            ++*working;
+        }
+    *mean_ms = tot / (double)*working;
 }
 void KernelTimer::release() {
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -682,15 +699,31 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
     MP_HIP(hipStreamSynchronize(st));
     nnz = 0;
     for (int i = 0; i < nrows; ++i) nnz += hl[i];
+    // entries whose columns the products read in 16 bits, and those slices
+    nnz16 = 0;
+    nslices16 = 0;
+    if (sv.col16) {
+        std::vector<int> cb((size_t)sv.nslices * CW_COUNT);
+        MP_HIP(hipMemcpy(cb.data(), sv.cbase, sizeof(int) * cb.size(), hipMemcpyDeviceToHost));
+        for (int sl = 0; sl < sv.nslices; ++sl)
+            if (cb[(size_t)sl * CW_COUNT] >= 0) {
+                ++nslices16;
+                for (int l = 0; l < SLICE; ++l) nnz16 += hl[(size_t)sl * SLICE + l];
+            }
+    }
     seg_stats.assign(nseg, SolveStats{});
 }
 
 double PpeSolver::spmv_bytes(int fp32) const {
-    // per entry: value (8 B, or 4 B for the FP32 operator) + column (4 B);
-    // per row: gathered x once, y written, r0 read (FP64 vectors), length,
-    // D^-1 for the border row
+    // per entry: value (8 B, or 4 B for the FP32 operator) + column (FP32
+    // operator: 2 B in slices with compressed columns, else 4 B); per
+    // compressed slice its window bases; per row: gathered x once, y written, r0 read (FP64
+    // vectors), length, D^-1 for the border row
     const double vb = fp32 ? 4.0 : 8.0;
-    return (double)nnz * (vb + 4.0) + (double)nv * (8.0 * (bordered ? 4.0 : 3.0) + 4.0);
+    const double n16 = fp32 ? (double)nnz16 : 0.0;  // the FP64 product reads 32-bit columns
+    return (double)nnz * vb + 2.0 * n16 + 4.0 * ((double)nnz - n16) +
+           (fp32 ? (double)nslices16 * 4.0 * CW_COUNT : 0.0) +
+           (double)nv * (8.0 * (bordered ? 4.0 : 3.0) + 4.0);
 }
 
 // y = M^-1 p with the multigrid V-cycle; without it the direction itself.

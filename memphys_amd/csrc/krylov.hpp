@@ -12,15 +12,20 @@ struct SolveStats {
     double rel_res = 0, abs_res = 0, alpha = 0, guess_rel_res = 0;
 };
 
+// HIP-event timing of one kernel's launches.  Krylov kernels of a system that
+// has converged inside a chunk return at once; the summary counts only the
+// launches that did the work: duration at least half the median launch's.
 struct KernelTimer {
     bool enabled = false;
     std::vector<hipEvent_t> ev;  // pairs
     size_t used = 0;
-    long launches = 0;
-    double total_ms = 0.0;
+    std::vector<float> samples;  // milliseconds per launch
     void begin(hipStream_t s);
     void end(hipStream_t s);
-    void harvest();  // after a synchronize: accumulate and recycle
+    void harvest();  // after a synchronize: collect and recycle the events
+    void reset() { samples.clear(); }
+    // working launches, their mean duration (ms), and all launches
+    void summary(long* working, double* mean_ms, long* all) const;
     void release();
 };
 
@@ -44,6 +49,8 @@ struct PpeSolver {
     int n_sys = 0;   // unknowns: real rows + border unknowns
     int nseg = 1, bordered = 0, dim = 0;
     long nnz = 0;    // real stencil entries of the operator
+    long nnz16 = 0;  // of these, entries in slices with 16-bit columns
+    int nslices16 = 0;
     SellView sv{};
     std::vector<Seg> segs;
     DevBuf<Seg> seg_d;

@@ -93,13 +93,25 @@ struct Vec4 {
     typedef T type __attribute__((ext_vector_type(4)));
 };
 
-// Sum over the neighbours of one slice row, in neighbour order, of
-// fv(value) * fx(column), as an fma chain: slice s, lane = slice row.  The
-// four neighbours of a group come in one vector load of columns and one of
-// values; U4 groups are loaded before their gathers.  Columns and values up
-// to 4 bytes use non-temporal loads so the gathered vector stays cached; FP64
-// values (32 bytes a lane, fetched as two 16-byte halves) use ordinary loads,
-// which keep the line in cache between the halves.
+// Compressed columns: a slice whose columns fall into at most CW_COUNT
+// windows of 2^CW_BITS consecutive ids stores each slot's column in 16 bits,
+// (window << CW_BITS) | (column - base[window]), with the slice's window bases
+// in cbase[CW_COUNT * s ...]; cbase[CW_COUNT * s] < 0 marks a slice that keeps
+// its 32-bit columns.  The decode is exact, so either form gives the same sums.
+constexpr int CW_BITS = 13;
+constexpr int CW_COUNT = 8;
+
+__device__ __forceinline__ int col_decode(uint32_t e, const int (&b)[CW_COUNT]) {
+    const uint32_t w = e >> CW_BITS;
+    const int t0 = (w & 1u) ? b[1] : b[0];
+    const int t1 = (w & 1u) ? b[3] : b[2];
+    const int t2 = (w & 1u) ? b[5] : b[4];
+    const int t3 = (w & 1u) ? b[7] : b[6];
+    const int u0 = (w & 2u) ? t1 : t0;
+    const int u1 = (w & 2u) ? t3 : t2;
+    return ((w & 4u) ? u1 : u0) + (int)(e & ((1u << CW_BITS) - 1u));
+}
+
 template <typename V>
 __device__ __forceinline__ V load_values(const V* p) {
     if constexpr (sizeof(V) > 16)
@@ -108,33 +120,33 @@ __device__ __forceinline__ V load_values(const V* p) {
         return __builtin_nontemporal_load(p);
 }
 
-template <int U4, typename T, typename TM, typename FV, typename FX>
-__device__ __forceinline__ T sell4_accumulate(const int* __restrict__ sptr,
-                                              const int* __restrict__ col,
-                                              const TM* __restrict__ a, int s, int lane, FV fv,
-                                              FX fx) {
-    typedef typename Vec4<int>::type c4;
+// The neighbour loop of one slice row (groups of four slots from slot b0 +
+// 4 lane, w4 groups): load(g) fetches group g's stored columns and decode()
+// turns them into column ids.  The stored columns and the values of U4
+// groups are all requested before the first decode, so their loads overlap.
+template <int U4, typename T, typename TM, typename LD, typename DC, typename FV, typename FX>
+__device__ __forceinline__ T sell4_loop(int b0, int w4, int lane, const TM* __restrict__ a,
+                                        LD load, DC decode, FV fv, FX fx) {
     typedef typename Vec4<TM>::type v4;
-    const int b0 = sptr[s];
-    const int w4 = (sptr[s + 1] - b0) >> 8;
-    const c4* cp = reinterpret_cast<const c4*>(col + b0) + lane;
+    typedef decltype(load(0)) raw4;
     const v4* ap = reinterpret_cast<const v4*>(a + b0) + lane;
     T acc = (T)0;
     int g = 0;
     for (; g + U4 <= w4; g += U4) {
-        c4 c[U4];
+        raw4 e[U4];
         v4 v[U4];
         T xv[4 * U4];
 #pragma unroll
-        for (int u = 0; u < U4; ++u) c[u] = __builtin_nontemporal_load(cp + u * SLICE);
+        for (int u = 0; u < U4; ++u) e[u] = load(g + u);
 #pragma unroll
-        for (int u = 0; u < U4; ++u) v[u] = load_values(ap + u * SLICE);
+        for (int u = 0; u < U4; ++u) v[u] = load_values(ap + (g + u) * SLICE);
 #pragma unroll
         for (int u = 0; u < U4; ++u) {
-            xv[4 * u + 0] = fx(c[u].x);
-            xv[4 * u + 1] = fx(c[u].y);
-            xv[4 * u + 2] = fx(c[u].z);
-            xv[4 * u + 3] = fx(c[u].w);
+            const typename Vec4<int>::type c = decode(e[u]);
+            xv[4 * u + 0] = fx(c.x);
+            xv[4 * u + 1] = fx(c.y);
+            xv[4 * u + 2] = fx(c.z);
+            xv[4 * u + 3] = fx(c.w);
         }
 #pragma unroll
         for (int u = 0; u < U4; ++u) {
@@ -143,21 +155,61 @@ __device__ __forceinline__ T sell4_accumulate(const int* __restrict__ sptr,
             acc = fma(fv(v[u].z), xv[4 * u + 2], acc);
             acc = fma(fv(v[u].w), xv[4 * u + 3], acc);
         }
-        cp += U4 * SLICE;
-        ap += U4 * SLICE;
     }
     for (; g < w4; ++g) {
-        const c4 c = __builtin_nontemporal_load(cp);
-        const v4 v = load_values(ap);
+        const raw4 e = load(g);
+        const v4 v = load_values(ap + g * SLICE);
+        const typename Vec4<int>::type c = decode(e);
         const T x0 = fx(c.x), x1 = fx(c.y), x2 = fx(c.z), x3 = fx(c.w);
         acc = fma(fv(v.x), x0, acc);
         acc = fma(fv(v.y), x1, acc);
         acc = fma(fv(v.z), x2, acc);
         acc = fma(fv(v.w), x3, acc);
-        cp += SLICE;
-        ap += SLICE;
     }
     return acc;
+}
+
+// Sum over the neighbours of one slice row, in neighbour order, of
+// fv(value) * fx(column), as an fma chain: slice s, lane = slice row.  The
+// four neighbours of a group come in one vector load of columns (16-bit
+// compressed ones when col16 is given and the slice has them) and one of
+// values; U4 groups are loaded before their gathers.  Columns and values up
+// to 4 bytes use non-temporal loads so the gathered vector stays cached; FP64
+// values (32 bytes a lane, fetched as two 16-byte halves) use ordinary loads,
+// which keep the line in cache between the halves.
+template <int U4, typename T, typename TM, typename FV, typename FX>
+__device__ __forceinline__ T sell4_accumulate(const int* __restrict__ sptr,
+                                              const int* __restrict__ col,
+                                              const uint16_t* __restrict__ col16,
+                                              const int* __restrict__ cbase,
+                                              const TM* __restrict__ a, int s, int lane, FV fv,
+                                              FX fx) {
+    typedef typename Vec4<int>::type c4;
+    typedef typename Vec4<uint16_t>::type e4;
+    const int b0 = sptr[s];
+    const int w4 = (sptr[s + 1] - b0) >> 8;
+    if (col16 != nullptr) {
+        const int* bp = cbase + CW_COUNT * s;
+        const int base[CW_COUNT] = {bp[0], bp[1], bp[2], bp[3], bp[4], bp[5], bp[6], bp[7]};
+        if (base[0] >= 0) {
+            const e4* ep = reinterpret_cast<const e4*>(col16 + b0) + lane;
+            return sell4_loop<U4, T>(
+                b0, w4, lane, a, [&](int g) { return __builtin_nontemporal_load(ep + g * SLICE); },
+                [&](e4 e) {
+                    c4 c;
+                    c.x = col_decode(e.x, base);
+                    c.y = col_decode(e.y, base);
+                    c.z = col_decode(e.z, base);
+                    c.w = col_decode(e.w, base);
+                    return c;
+                },
+                fv, fx);
+        }
+    }
+    const c4* cp = reinterpret_cast<const c4*>(col + b0) + lane;
+    return sell4_loop<U4, T>(
+        b0, w4, lane, a, [&](int g) { return __builtin_nontemporal_load(cp + g * SLICE); },
+        [](c4 c) { return c; }, fv, fx);
 }
 
 // SELL-64x4 view of the stencil set in device order.
@@ -167,6 +219,8 @@ struct SellView {
     const int* sptr;   // [nslices+1] first slot of each slice
     const int* col;    // [slots] device column ids
     const int* len;    // [nslices*64] row lengths (0 on ghost rows)
+    const uint16_t* col16 = nullptr;  // [slots] compressed column ids (or null)
+    const int* cbase = nullptr;       // [nslices*CW_COUNT] window bases
 };
 
 // Device-side Krylov scalars (FP64) of one system.  Written only by the

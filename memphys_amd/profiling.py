@@ -1,8 +1,9 @@
 """Text summaries of rocprofv3 result databases (rocpd SQLite).
 
-    python -m memphys_amd.profiling stats DB [--top N] [--busy-us T]
+    python -m memphys_amd.profiling stats DB [--top N]
         per kernel and grid size: dispatches, total ms, mean/min/max us, and the
-        mean over launches longer than T us
+        mean over the launches that did the work (at least half the median
+        duration: Krylov launches of a converged system return at once)
     python -m memphys_amd.profiling pmc DB [--top N]
         per kernel, grid size and counter: dispatches and mean value per dispatch
     python -m memphys_amd.profiling traffic FETCH_DB WRITE_DB --kernel SUBSTR
@@ -32,25 +33,33 @@ def _short(name: str) -> str:
     return "".join(out).strip()
 
 
-def stats(db: str, top: int, busy_us: float = 3.0) -> str:
-    """busy: launches longer than busy_us (solver kernels launched after their
-    system converged return at entry in ~1-2 us and are left out there)."""
+def _working(durs: list) -> list:
+    """Launches that did the work: duration at least half the median launch's
+    (Krylov kernels launched after their system converged return at once).
+    The same rule as the library's HIP-event kernel timer."""
+    srt = sorted(durs)
+    cut = 0.5 * srt[len(srt) // 2]
+    return [d for d in durs if d >= cut]
+
+
+def stats(db: str, top: int) -> str:
     cur = sqlite3.connect(db).cursor()
-    rows = cur.execute(
-        "select name, grid_x, count(*), sum(end-start), avg(end-start), min(end-start), "
-        "max(end-start), sum(case when end-start > ? then 1 else 0 end), "
-        "avg(case when end-start > ? then end-start end) from kernels group by name, grid_x "
-        "order by sum(end-start) desc", (busy_us * 1e3, busy_us * 1e3)).fetchall()
-    tot = sum(r[3] for r in rows) or 1
-    lines = [f"{'kernel':56s} {'grid':>9s} {'calls':>7s} {'busy':>7s} {'total_ms':>9s} "
-             f"{'mean_us':>8s} {'busy_us':>8s} {'min_us':>7s} {'max_us':>8s} {'pct':>6s}"]
-    for r in rows[:top]:
-        busy_mean = (r[8] or 0.0) / 1e3
-        lines.append(f"{_short(r[0])[:56]:56s} {r[1]:9d} {r[2]:7d} {r[7]:7d} {r[3] / 1e6:9.2f} "
-                     f"{r[4] / 1e3:8.1f} {busy_mean:8.1f} {r[5] / 1e3:7.1f} {r[6] / 1e3:8.1f} "
-                     f"{100 * r[3] / tot:6.2f}")
-    lines.append(f"total kernel time {tot / 1e6:.2f} ms over {sum(r[2] for r in rows)} dispatches; "
-                 f"busy = launches longer than {busy_us:g} us")
+    per = {}
+    for name, grid, d in cur.execute("select name, grid_x, end-start from kernels"):
+        per.setdefault((name, grid), []).append(d)
+    rows = sorted(per.items(), key=lambda kv: -sum(kv[1]))
+    tot = sum(sum(v) for _, v in rows) or 1
+    lines = [f"{'kernel':56s} {'grid':>9s} {'calls':>7s} {'working':>7s} {'total_ms':>9s} "
+             f"{'mean_us':>8s} {'work_us':>8s} {'min_us':>7s} {'max_us':>8s} {'pct':>6s}"]
+    for (name, grid), v in rows[:top]:
+        w = _working(v)
+        lines.append(f"{_short(name)[:56]:56s} {grid:9d} {len(v):7d} {len(w):7d} "
+                     f"{sum(v) / 1e6:9.2f} {sum(v) / len(v) / 1e3:8.1f} "
+                     f"{sum(w) / len(w) / 1e3:8.1f} {min(v) / 1e3:7.1f} {max(v) / 1e3:8.1f} "
+                     f"{100 * sum(v) / tot:6.2f}")
+    lines.append(f"total kernel time {tot / 1e6:.2f} ms over {sum(len(v) for _, v in rows)} "
+                 f"dispatches; working = launches of at least half the median duration of "
+                 f"their kernel and grid, work_us their mean")
     return "\n".join(lines)
 
 
@@ -72,15 +81,26 @@ def pmc(db: str, top: int) -> str:
 
 
 def _counter(db: str, counter: str, pattern: str, grid: int | None):
+    """Mean counter value of the largest launch shape of the kernel, over the
+    dispatches that did the work: duration at least half the median dispatch's
+    (Krylov launches of a converged system return at once)."""
     best = None
     for name, g, cname, n, mean, dur in _pmc_rows(db):
         if cname != counter or pattern not in name or (grid is not None and g != grid):
             continue
         if best is None or g > best[1]:  # the largest launch of the kernel
-            best = (name, g, n, mean, dur)
+            best = (name, g)
     if best is None:
         raise SystemExit(f"no dispatch of '{pattern}' with {counter} in {db}")
-    return best
+    cur = sqlite3.connect(db).cursor()
+    rows = cur.execute("select value, duration from counters_collection where kernel_name = ? "
+                       "and grid_size = ? and counter_name = ?",
+                       (best[0], best[1], counter)).fetchall()
+    cut = min(_working([r[1] for r in rows]))
+    work = [r for r in rows if r[1] >= cut]
+    mean = sum(r[0] for r in work) / len(work)
+    dur = sum(r[1] for r in work) / len(work)
+    return best[0], best[1], len(work), mean, dur, len(rows)
 
 
 def traffic(fetch_db: str, write_db: str, kernel: str, copy_kernel: str, copy_bytes: int,
@@ -97,8 +117,10 @@ def traffic(fetch_db: str, write_db: str, kernel: str, copy_kernel: str, copy_by
         f"copy kernel: {_short(cf[0])} grid {cf[1]}: FETCH_SIZE {cf[3]:.1f} KiB, "
         f"WRITE_SIZE {cw[3]:.1f} KiB for {copy_bytes} bytes each way",
         f"scale factors (counter bytes per true byte): fetch {f_scale:.4f}, write {w_scale:.4f}",
-        f"kernel: {_short(kf[0])} grid {kf[1]} ({kf[2]} dispatches): FETCH_SIZE {kf[3]:.1f} KiB, "
-        f"WRITE_SIZE {kw[3]:.1f} KiB per dispatch",
+        f"kernel: {_short(kf[0])} grid {kf[1]}: {kf[2]} of {kf[5]} dispatches did the work "
+        f"(the rest returned at once); over those: FETCH_SIZE {kf[3]:.1f} KiB, "
+        f"WRITE_SIZE {kw[3]:.1f} KiB per dispatch, mean duration under the counters "
+        f"{kf[4] / 1e3:.1f} us",
         f"fetch_bytes_per_launch = {fetch_b:.0f}",
         f"write_bytes_per_launch = {write_b:.0f}",
         f"traffic_bytes_per_launch = {fetch_b + write_b:.0f}",
@@ -111,7 +133,6 @@ def main(argv=None):
     a = sub.add_parser("stats")
     a.add_argument("db")
     a.add_argument("--top", type=int, default=40)
-    a.add_argument("--busy-us", type=float, default=3.0)
     b = sub.add_parser("pmc")
     b.add_argument("db")
     b.add_argument("--top", type=int, default=40)
@@ -124,7 +145,7 @@ def main(argv=None):
     c.add_argument("--grid", type=int, default=None)
     args = ap.parse_args(argv)
     if args.cmd == "stats":
-        print(stats(args.db, args.top, args.busy_us))
+        print(stats(args.db, args.top))
     elif args.cmd == "pmc":
         print(pmc(args.db, args.top))
     else:
