@@ -118,6 +118,17 @@ int mp_weights_set(mp_ctx* ctx, const double* gx, const double* gy, const double
 /* y = Op x for one stencil operator (replaces points.*_matrix_EIGEN * x). */
 int mp_operator_apply(mp_ctx* ctx, int op, const double* x, double* y);
 
+/* Stencil selection on the GPU (replaces CLOUD::calc_cloud_points_fast,
+ * cloud.cpp:570-648): an interior node gets its k nearest nodes (itself
+ * first), a boundary node itself and its k-1 nearest interior nodes; each
+ * row is ordered by (squared distance, index), exact ties going to the lower
+ * index.  Host arrays: xyz nv*dim (AoS), boundary nv, k_per_node nv (1..96).
+ * Output CSR in host memory: row_ptr nv+1 (filled from k_per_node) and col
+ * (row_ptr[nv] entries).  Needs no uploaded cloud; leaves the context's
+ * cloud untouched. */
+int mp_knn_stencils(mp_ctx* ctx, int nv, int dim, const double* xyz, const uint8_t* boundary,
+                    const int* k_per_node, int* row_ptr, int* col);
+
 /* Largest-magnitude eigenvalue of one stencil operator (replaces
  * calc_largest_magnitude_eigenvalue, general_functions.cpp:4-24, as used by
  * PARAMETERS::calc_dt, parameters.cpp:269-296): restarted Arnoldi on the GPU

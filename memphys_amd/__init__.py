@@ -35,7 +35,7 @@ HIP_SYMBOLS = (
     "mp_kernel_timing", "mp_kernel_timing_get", "mp_guess_param_count", "mp_guess_train",
     "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
     "mp_graph_mode", "mp_batch_upload", "mp_batch_params", "mp_batch_stats", "mp_autotune",
-    "mp_operator_eigenvalue",
+    "mp_operator_eigenvalue", "mp_knn_stencils",
 )
 
 
@@ -116,6 +116,7 @@ def hip_lib() -> C.CDLL:
                                                C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.mp_precond_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), vp, vp,
                                         C.POINTER(C.c_double)]
+        lib.mp_knn_stencils.argtypes = [vp, C.c_int, C.c_int, _dp, _u8, _ip, _ip, _ip]
         for name in HIP_SYMBOLS:
             getattr(lib, name).restype = C.c_char_p if name == "mp_last_error" else C.c_int
         _hip = lib
@@ -310,6 +311,18 @@ class Context:
         self._check(self.lib.mp_operator_eigenvalue(self.h, op, tol, max_restarts, C.byref(re),
                                                     C.byref(im)), "mp_operator_eigenvalue")
         return complex(re.value, im.value)
+
+    def knn_stencils(self, xyz: np.ndarray, boundary: np.ndarray, k_per_node: np.ndarray):
+        """Stencils on the GPU (CLOUD::calc_cloud_points_fast): returns (row_ptr, col)."""
+        nv, dim = xyz.shape
+        ks = np.ascontiguousarray(k_per_node, dtype=np.int32)
+        rp = np.zeros(nv + 1, dtype=np.int32)
+        col = np.zeros(int(ks.sum()), dtype=np.int32)
+        self._check(self.lib.mp_knn_stencils(self.h, nv, dim,
+                                             np.ascontiguousarray(xyz, dtype=np.float64).ravel(),
+                                             np.ascontiguousarray(boundary, dtype=np.uint8), ks,
+                                             rp, col), "mp_knn_stencils")
+        return rp, col
 
     def autotune(self):
         """Picks the fastest convergence-poll interval on the last step's solve;

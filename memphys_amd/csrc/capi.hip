@@ -649,6 +649,16 @@ int mp_operator_apply(mp_ctx* c, int op, const double* x, double* y) {
     });
 }
 
+int mp_knn_stencils(mp_ctx* c, int nv, int dim, const double* xyz, const uint8_t* boundary,
+                    const int* k_per_node, int* row_ptr, int* col) {
+    return guarded(c, "mp_knn_stencils", [&]() -> int {
+        if (!xyz || !boundary || !k_per_node || !row_ptr || !col)
+            return fail(c, MP_ERR_ARG, "mp_knn_stencils: null pointer");
+        knn_stencils(c->stream, nv, dim, xyz, boundary, k_per_node, row_ptr, col);
+        return MP_OK;
+    });
+}
+
 int mp_operator_eigenvalue(mp_ctx* c, int op, double tol, int max_restarts, double* re,
                            double* im) {
     return guarded(c, "mp_operator_eigenvalue", [&]() -> int {

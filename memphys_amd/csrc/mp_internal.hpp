@@ -307,6 +307,11 @@ void launch_build_operator(hipStream_t s, const SellView& sv, int dim, const dou
 
 // Largest-magnitude eigenvalue of a stencil operator (k_eig.hip); returns the
 // Arnoldi restarts used.
+// kNN stencils of host points into host CSR (row_ptr is filled from the
+// stencil sizes); throws std::invalid_argument on bad sizes.
+void knn_stencils(hipStream_t st, int nv, int dim, const double* xyz, const uint8_t* boundary,
+                  const int* k_per_node, int* row_ptr, int* col);
+
 int largest_eigenvalue(hipStream_t s, const SellView& sv, const double* val, int m, double tol,
                        int max_restarts, double* re, double* im);
 
