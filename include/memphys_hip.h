@@ -129,6 +129,16 @@ int mp_operator_apply(mp_ctx* ctx, int op, const double* x, double* y);
 int mp_knn_stencils(mp_ctx* ctx, int nv, int dim, const double* xyz, const uint8_t* boundary,
                     const int* k_per_node, int* row_ptr, int* col);
 
+/* Reverse Cuthill-McKee order on the GPU (replaces cuthill_mckee_ordering +
+ * reverse_cuthill_mckee_ordering, general_functions.cpp:807-879, as called by
+ * CLOUD::re_order_points_reverse_cuthill_mckee, cloud.cpp:315-331): start at
+ * the lowest-index node of minimum degree (row length), visit each node's
+ * stencil neighbours, append every newly visited set sorted by (degree,
+ * index), reverse.  Nodes the start cannot reach restart the search at the
+ * lowest-index unvisited node of minimum degree.  Host CSR in (row_ptr nv+1,
+ * col), host order out: order[old] = new. */
+int mp_rcm_order(mp_ctx* ctx, int nv, const int* row_ptr, const int* col, int* order);
+
 /* Largest-magnitude eigenvalue of one stencil operator (replaces
  * calc_largest_magnitude_eigenvalue, general_functions.cpp:4-24, as used by
  * PARAMETERS::calc_dt, parameters.cpp:269-296): restarted Arnoldi on the GPU

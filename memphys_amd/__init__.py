@@ -35,7 +35,7 @@ HIP_SYMBOLS = (
     "mp_kernel_timing", "mp_kernel_timing_get", "mp_guess_param_count", "mp_guess_train",
     "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
     "mp_graph_mode", "mp_batch_upload", "mp_batch_params", "mp_batch_stats", "mp_autotune",
-    "mp_operator_eigenvalue", "mp_knn_stencils",
+    "mp_operator_eigenvalue", "mp_knn_stencils", "mp_rcm_order",
 )
 
 
@@ -117,6 +117,7 @@ def hip_lib() -> C.CDLL:
         lib.mp_precond_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), vp, vp,
                                         C.POINTER(C.c_double)]
         lib.mp_knn_stencils.argtypes = [vp, C.c_int, C.c_int, _dp, _u8, _ip, _ip, _ip]
+        lib.mp_rcm_order.argtypes = [vp, C.c_int, _ip, _ip, _ip]
         for name in HIP_SYMBOLS:
             getattr(lib, name).restype = C.c_char_p if name == "mp_last_error" else C.c_int
         _hip = lib
@@ -323,6 +324,16 @@ class Context:
                                              np.ascontiguousarray(boundary, dtype=np.uint8), ks,
                                              rp, col), "mp_knn_stencils")
         return rp, col
+
+    def rcm_order(self, row_ptr: np.ndarray, col: np.ndarray) -> np.ndarray:
+        """Reverse Cuthill-McKee order on the GPU: order[old] = new."""
+        nv = len(row_ptr) - 1
+        order = np.zeros(nv, dtype=np.int32)
+        self._check(self.lib.mp_rcm_order(self.h, nv,
+                                          np.ascontiguousarray(row_ptr, dtype=np.int32),
+                                          np.ascontiguousarray(col, dtype=np.int32), order),
+                    "mp_rcm_order")
+        return order
 
     def autotune(self):
         """Picks the fastest convergence-poll interval on the last step's solve;

@@ -659,6 +659,14 @@ int mp_knn_stencils(mp_ctx* c, int nv, int dim, const double* xyz, const uint8_t
     });
 }
 
+int mp_rcm_order(mp_ctx* c, int nv, const int* row_ptr, const int* col, int* order) {
+    return guarded(c, "mp_rcm_order", [&]() -> int {
+        if (!row_ptr || !col || !order) return fail(c, MP_ERR_ARG, "mp_rcm_order: null pointer");
+        rcm_order(c->stream, nv, row_ptr, col, order, nullptr);
+        return MP_OK;
+    });
+}
+
 int mp_operator_eigenvalue(mp_ctx* c, int op, double tol, int max_restarts, double* re,
                            double* im) {
     return guarded(c, "mp_operator_eigenvalue", [&]() -> int {

@@ -312,6 +312,11 @@ void launch_build_operator(hipStream_t s, const SellView& sv, int dim, const dou
 void knn_stencils(hipStream_t st, int nv, int dim, const double* xyz, const uint8_t* boundary,
                   const int* k_per_node, int* row_ptr, int* col);
 
+// Reverse Cuthill-McKee order of a host CSR graph (order[old] = new); the
+// number of BFS levels goes to *levels (may be null).
+void rcm_order(hipStream_t st, int nv, const int* row_ptr, const int* col, int* order,
+               int* levels);
+
 int largest_eigenvalue(hipStream_t s, const SellView& sv, const double* val, int m, double tol,
                        int max_restarts, double* re, double* im);
 

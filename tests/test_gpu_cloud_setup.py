@@ -1,9 +1,12 @@
-"""GPU: stencil selection (mp_knn_stencils) against the host kNN and the
-oracle's brute force.
+"""GPU cloud set-up: stencil selection (mp_knn_stencils) against the host kNN
+and the oracle's brute force; reverse Cuthill-McKee order (mp_rcm_order)
+against the host order and the oracle's restatement.
 
-Both sides return the k smallest (squared distance, index) pairs with the same
-squared-distance arithmetic (no contraction), so the CSR must agree entry for
-entry, ties included (an unjittered lattice has many exact ties).
+Both kNN sides return the k smallest (squared distance, index) pairs with the
+same squared-distance arithmetic (no contraction), so the CSR must agree entry
+for entry, ties included (an unjittered lattice has many exact ties).  The
+RCM order is a permutation fixed by the stencil graph alone, so it must agree
+exactly too.
 """
 import numpy as np
 import pytest
@@ -102,3 +105,61 @@ def test_knn_rejects_bad_sizes():
             ctx.knn_stencils(c.xyz, c.boundary, np.full(c.nv, c.nv + 1, dtype=np.int32))
         with pytest.raises(M.MemphysError):
             ctx.knn_stencils(c.xyz, c.boundary, np.zeros(c.nv, dtype=np.int32))
+
+
+def _host_rcm(rp, col):
+    import memphys_amd as M
+    order = np.zeros(len(rp) - 1, dtype=np.int32)
+    assert M.host_lib().mph_rcm(len(rp) - 1, rp, col, order) == 0
+    return order
+
+
+@pytest.mark.parametrize("dim,n,k", [(2, 40, 30), (3, 16, 40), (2, 30, 12)])
+def test_rcm_matches_host(dim, n, k):
+    import memphys_amd as M
+    import oracle as O
+    c = M.box_cloud(dim=dim, n=n, jitter=0.3, seed=9, k_interior=k, rcm=False)
+    nv = c.nv
+    # the graph itself, and two disjoint copies (a restart at the second)
+    rp2 = np.concatenate([c.row_ptr, c.row_ptr[1:] + c.row_ptr[-1]]).astype(np.int32)
+    col2 = np.concatenate([c.col, c.col + nv]).astype(np.int32)
+    with M.Context(0) as ctx:
+        for name, rp, col in (("graph", c.row_ptr, c.col), ("two components", rp2, col2)):
+            g = ctx.rcm_order(rp, col)
+            h = _host_rcm(rp, col)
+            _print(f"RCM dim {dim} n {n} {name}: nv {len(rp) - 1} identical {np.array_equal(g, h)}")
+            assert np.array_equal(g, h)
+            assert np.array_equal(np.sort(g), np.arange(len(rp) - 1))
+        if nv <= 2000:
+            assert np.array_equal(ctx.rcm_order(c.row_ptr, c.col),
+                                  O.rcm_reference(c.row_ptr, c.col))
+
+
+def test_rcm_gmsh_and_full_size():
+    import os
+    import time
+    import memphys_amd as M
+    path = os.path.join(os.path.dirname(__file__), "golden", "ecc_ratio_0.5_r_i_0.6_n_200.msh")
+    a = M.gmsh_cloud(path, 20, rcm=False)
+    c = M.box_cloud(dim=3, n=100, jitter=0.3, seed=7, k_interior=40, k_layers=(60, 55, 50),
+                    rcm=False)
+    with M.Context(0) as ctx:
+        assert np.array_equal(ctx.rcm_order(a.row_ptr, a.col), _host_rcm(a.row_ptr, a.col))
+        t0 = time.perf_counter()
+        g = ctx.rcm_order(c.row_ptr, c.col)
+        t1 = time.perf_counter()
+    h = _host_rcm(c.row_ptr, c.col)
+    t2 = time.perf_counter()
+    _print(f"1M-node RCM: GPU {t1 - t0:.3f}s incl. transfers, host {t2 - t1:.3f}s, "
+           f"identical {np.array_equal(g, h)}")
+    assert np.array_equal(g, h)
+
+
+def test_rcm_rejects_bad_graph():
+    import memphys_amd as M
+    rp = np.array([0, 2, 3], dtype=np.int32)
+    with M.Context(0) as ctx:
+        with pytest.raises(M.MemphysError):
+            ctx.rcm_order(rp, np.array([0, 5, 1], dtype=np.int32))
+        with pytest.raises(M.MemphysError):
+            ctx.rcm_order(rp, np.array([0, -1, 1], dtype=np.int32))
