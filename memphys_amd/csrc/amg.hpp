@@ -47,10 +47,12 @@ struct AmgDevice {
     std::vector<mp_amg::Csr> hA, hP, hR;
 
     // Builds the hierarchy from the finest operator, given in SELL-64x4 slots
-    // (device order) by the solver: sv stays owned by the solver and serves as
-    // level 0's layout, with a BF16 copy of val32 as its values.
+    // (device order) by the solver: FP64 values val64_dev in the slots of sv
+    // for the host set-up; the finest level's products run on sv_apply (owned
+    // by the solver, the same rows, possibly another order of each row's
+    // slots) with a BF16 copy of val32_dev, given in sv_apply's slots.
     void setup(hipStream_t s, const SellView& sv, const double* val64_dev,
-               const float* val32_dev, const mp_amg::Options& opt);
+               const SellView& sv_apply, const float* val32_dev, const mp_amg::Options& opt);
     // Layout of every hierarchy matrix chosen by timing each candidate (SELL-64x4,
     // CSR with 4..64 lanes per row) instead of the set-up's rule of thumb.  The
     // layouts sum a row in different orders, so the V-cycle then changes at the

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -46,12 +47,16 @@ struct mp_ctx {
     bool per_problem_prm = false;
     std::vector<long> prob_total_iters;  // Krylov iterations per problem since mp_fs_create
     std::vector<uint8_t> h_bnd;
-    DevBuf<int> sptr, col, len, brows, cbase;
-    DevBuf<uint16_t> col16;
+    DevBuf<int> sptr, col, len, brows;
+    // the column-sorted slot order of the FP32/BF16 products: 32-bit and
+    // 16-bit columns, window bases, and slot_perm[sorted slot] = stencil slot
+    DevBuf<int> col_s, cbase_s, slot_perm;
+    DevBuf<uint16_t> col16_s;
     int nb = 0;
     DevBuf<double> xyz, normal;
     DevBuf<uint8_t> bnd;
-    SellView sv{};
+    SellView sv{};   // stencil order (weights, FP64 operator)
+    SellView svs{};  // column-sorted order (FP32 operator, multigrid level 0)
 
     // stencil weights (SELL slots)
     bool has_weights = false;
@@ -465,12 +470,43 @@ int upload_clouds(mp_ctx* c, const char* fn, int nprob, const int* nvp, int dim,
         for (int b = 0; b < c->prob_rows[q] / BLOCK; ++b) hbp[c->prob_row0[q] / BLOCK + b] = q;
     c->nb = (int)hb.size();
     c->sptr.alloc(hsptr.size());
+    // column-sorted copy: each row's slots reordered by increasing column,
+    // so that a wavefront's gathers of one neighbour step fall closer
+    // together; the FP32 operator and the multigrid's finest level use it
+    std::vector<int> hcol_s(hcol), hperm(c->total_slots);
+    std::iota(hperm.begin(), hperm.end(), 0);
+    {
+        const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (int tn = 0; tn < nthr; ++tn)
+            pool.emplace_back([&, tn]() {
+                std::vector<std::pair<int, int>> e;
+                for (int d = tn; d < rows; d += nthr) {
+                    const int L = hlen[d];
+                    if (L == 0) continue;
+                    const int base = sell_base(hsptr.data(), d);
+                    e.clear();
+                    for (int j = 0; j < L; ++j)
+                        e.emplace_back(hcol[base + sell_off(j)], base + sell_off(j));
+                    std::stable_sort(e.begin(), e.end(),
+                                     [](const std::pair<int, int>& a,
+                                        const std::pair<int, int>& b) { return a.first < b.first; });
+                    for (int j = 0; j < L; ++j) {
+                        hcol_s[base + sell_off(j)] = e[j].first;
+                        hperm[base + sell_off(j)] = e[j].second;
+                    }
+                }
+            });
+        for (auto& t : pool) t.join();
+    }
     std::vector<uint16_t> hcol16;
     std::vector<int> hcb;
-    compress_columns(hsptr, hcol, c->nslices, hcol16, hcb);
+    compress_columns(hsptr, hcol_s, c->nslices, hcol16, hcb);
     c->col.alloc(hcol.size());
-    c->col16.alloc(hcol16.size());
-    c->cbase.alloc(hcb.size());
+    c->col_s.alloc(hcol_s.size());
+    c->slot_perm.alloc(hperm.size());
+    c->col16_s.alloc(hcol16.size());
+    c->cbase_s.alloc(hcb.size());
     c->len.alloc(hlen.size());
     c->brows.alloc(std::max<size_t>(1, hb.size()));
     c->xyz.alloc(hx.size());
@@ -479,9 +515,13 @@ int upload_clouds(mp_ctx* c, const char* fn, int nprob, const int* nvp, int dim,
     c->blk_prob.alloc(hbp.size());
     MP_HIP(hipMemcpy(c->sptr.p, hsptr.data(), sizeof(int) * hsptr.size(), hipMemcpyHostToDevice));
     MP_HIP(hipMemcpy(c->col.p, hcol.data(), sizeof(int) * hcol.size(), hipMemcpyHostToDevice));
-    MP_HIP(hipMemcpy(c->col16.p, hcol16.data(), sizeof(uint16_t) * hcol16.size(),
+    MP_HIP(hipMemcpy(c->col_s.p, hcol_s.data(), sizeof(int) * hcol_s.size(),
                      hipMemcpyHostToDevice));
-    MP_HIP(hipMemcpy(c->cbase.p, hcb.data(), sizeof(int) * hcb.size(), hipMemcpyHostToDevice));
+    MP_HIP(hipMemcpy(c->slot_perm.p, hperm.data(), sizeof(int) * hperm.size(),
+                     hipMemcpyHostToDevice));
+    MP_HIP(hipMemcpy(c->col16_s.p, hcol16.data(), sizeof(uint16_t) * hcol16.size(),
+                     hipMemcpyHostToDevice));
+    MP_HIP(hipMemcpy(c->cbase_s.p, hcb.data(), sizeof(int) * hcb.size(), hipMemcpyHostToDevice));
     MP_HIP(hipMemcpy(c->len.p, hlen.data(), sizeof(int) * hlen.size(), hipMemcpyHostToDevice));
     if (!hb.empty())
         MP_HIP(hipMemcpy(c->brows.p, hb.data(), sizeof(int) * hb.size(), hipMemcpyHostToDevice));
@@ -493,8 +533,10 @@ int upload_clouds(mp_ctx* c, const char* fn, int nprob, const int* nvp, int dim,
     c->sv.nslices = c->nslices;
     c->sv.sptr = c->sptr.p;
     c->sv.col = c->col.p;
-    c->sv.col16 = c->col16.p;
-    c->sv.cbase = c->cbase.p;
+    c->svs = c->sv;
+    c->svs.col = c->col_s.p;
+    c->svs.col16 = c->col16_s.p;
+    c->svs.cbase = c->cbase_s.p;
     c->sv.len = c->len.p;
     for (auto& wb : c->w) {
         wb.alloc(c->total_slots);
@@ -688,6 +730,8 @@ int mp_solver_init(mp_ctx* c, const uint8_t* dirichlet, double unsteady, double 
             return fail(c, MP_ERR_ARG, "mp_solver_init: bad arguments");
         upload_flags(c, dirichlet, c->solver_dir);
         c->solver.reset(new PpeSolver());
+        c->solver->svs = c->svs;
+        c->solver->slot_perm = c->slot_perm.p;
         c->solver->setup(c->stream, c->sv, make_systems(c, dirichlet), c->dim, c->w[0].p,
                          c->w[1].p, c->w[2].p, c->w[3].p, c->normal.p, c->bnd.p,
                          c->solver_dir.p, unsteady, conv, diff);
@@ -772,6 +816,8 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
         if (c->dim == 2) upload_flags(c, nullptr, c->udir[2]);
         upload_flags(c, p_dir, c->pdir);
         c->ppe.reset(new PpeSolver());
+        c->ppe->svs = c->svs;
+        c->ppe->slot_perm = c->slot_perm.p;
         c->ppe->setup(c->stream, c->sv, make_systems(c, p_dir), c->dim, c->w[0].p, c->w[1].p,
                       c->w[2].p, c->w[3].p, c->normal.p, c->bnd.p, c->pdir.p, 0.0, 0.0, 1.0,
                       prm->preconditioner);

@@ -310,7 +310,8 @@ void alloc_zero(DevBuf<float>& b, size_t n, hipStream_t st) {
 }  // namespace
 
 void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_dev,
-                      const float* val32_dev, const mp_amg::Options& opt) {
+                      const SellView& sv_apply, const float* val32_dev,
+                      const mp_amg::Options& opt) {
     const auto t0 = std::chrono::steady_clock::now();
     // finest operator as CSR in device order (padding slots dropped)
     const int nv = sv.nv;
@@ -355,13 +356,13 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
             put(d.dinv, scaled_diag_inverse(A, omega), st);
             if (l == 0) {
                 d.A.n = nv;
-                d.A.nslices = sv.nslices;
+                d.A.nslices = sv_apply.nslices;
                 d.A.g = 0;
                 d.A.nnz = A.nnz();
-                d.A.ptr = sv.sptr;
-                d.A.col = sv.col;
-                d.A.col16 = sv.col16;
-                d.A.cbase = sv.cbase;
+                d.A.ptr = sv_apply.sptr;
+                d.A.col = sv_apply.col;
+                d.A.col16 = sv_apply.col16;
+                d.A.cbase = sv_apply.cbase;
                 // BF16 copy of the solver's FP32 slots (padding slots stay 0)
                 d.A.val_b.alloc((size_t)slots);
                 hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((slots + BLOCK - 1) / BLOCK)),

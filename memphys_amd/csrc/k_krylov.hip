@@ -125,10 +125,11 @@ __device__ void loop_top(KScalars* sc, double rr, double r0r) {
 template <typename TM, typename T>
 __device__ __forceinline__ T sell_dot(const SellView& sv, const TM* __restrict__ a,
                                       const T* __restrict__ x, int s, int lane) {
-    // FP64 values: two groups per iteration keep the register count at eight
-    // waves per SIMD, and the 32-bit columns (decoding 16-bit ones there makes
-    // the compiler serialise the gathers); narrower values take four groups
-    // and the compressed columns.
+    // FP64 values (stencil order, 32-bit columns): two groups per iteration
+    // keep the register count at eight waves per SIMD, and decoding 16-bit
+    // columns there would make the compiler serialise the gathers.  FP32
+    // values (column-sorted order, whose gathers are cheaper): four groups
+    // and the 16-bit columns of the view.
     constexpr bool wide = sizeof(TM) == 8;
     return sell4_accumulate<(wide ? 2 : 4), T>(
         sv.sptr, sv.col, wide ? nullptr : sv.col16, sv.cbase, a, s, lane,
@@ -518,6 +519,14 @@ __global__ __launch_bounds__(BLOCK) void k_scale_columns(SellView sv, const doub
     }
 }
 
+// out[s] = in[perm[s]] over the slots (values into the column-sorted order)
+__global__ __launch_bounds__(BLOCK) void k_permute_slots(int n, const int* __restrict__ perm,
+                                                         const float* __restrict__ in,
+                                                         float* __restrict__ out) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) out[i] = in[perm[i]];
+}
+
 // D^-1 of the operator's diagonal; rows without a diagonal (border unknowns,
 // padding) get 1.
 __global__ __launch_bounds__(BLOCK) void k_diag_inverse(int n, const double* __restrict__ diag,
@@ -667,11 +676,26 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
         MP_HIP(hipGetLastError());
         diag_inv = jac_dinv.p;
         scaled32 = jac_a32.p;
-        SellView s0 = sv;
-        s0.nv = segs[0].nv;  // the real rows: the hierarchy is built on the node block
-        s0.nslices = (s0.nv + SLICE - 1) / SLICE;
+    }
+    // the FP32 operator of the Krylov products (and of the multigrid's finest
+    // level) in the column-sorted slot order when there is one
+    svk = sv;
+    a32k = aval32.p;
+    if (slot_perm) {
+        aval32s.alloc(total_slots);
+        hipLaunchKernelGGL(k_permute_slots, dim3(nblk(total_slots)), dim3(BLOCK), 0, st,
+                           total_slots, slot_perm, aval32.p, aval32s.p);
+        MP_HIP(hipGetLastError());
+        svk = svs;
+        svk.nv = nrows;
+        a32k = aval32s.p;
+    }
+    if (precond == MP_PRECOND_AMG) {
+        SellView s0 = sv, s0k = svk;
+        s0.nv = s0k.nv = segs[0].nv;  // the real rows: the hierarchy is built on the node block
+        s0.nslices = s0k.nslices = (s0.nv + SLICE - 1) / SLICE;
         amg.reset(new AmgDevice());
-        amg->setup(st, s0, aval.p, aval32.p, mp_amg::Options{});
+        amg->setup(st, s0, aval.p, s0k, a32k, mp_amg::Options{});
         for (DevBuf<double>* b : {&yv, &zv}) {
             b->alloc(vlen);
             MP_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * vlen, st));
@@ -699,13 +723,12 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
     MP_HIP(hipStreamSynchronize(st));
     nnz = 0;
     for (int i = 0; i < nrows; ++i) nnz += hl[i];
-    // entries whose columns the products read in 16 bits, and those slices
     nnz16 = 0;
     nslices16 = 0;
-    if (sv.col16) {
-        std::vector<int> cb((size_t)sv.nslices * CW_COUNT);
-        MP_HIP(hipMemcpy(cb.data(), sv.cbase, sizeof(int) * cb.size(), hipMemcpyDeviceToHost));
-        for (int sl = 0; sl < sv.nslices; ++sl)
+    if (svk.col16) {
+        std::vector<int> cb((size_t)svk.nslices * CW_COUNT);
+        MP_HIP(hipMemcpy(cb.data(), svk.cbase, sizeof(int) * cb.size(), hipMemcpyDeviceToHost));
+        for (int sl = 0; sl < svk.nslices; ++sl)
             if (cb[(size_t)sl * CW_COUNT] >= 0) {
                 ++nslices16;
                 for (int l = 0; l < SLICE; ++l) nnz16 += hl[(size_t)sl * SLICE + l];
@@ -715,12 +738,12 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
 }
 
 double PpeSolver::spmv_bytes(int fp32) const {
-    // per entry: value (8 B, or 4 B for the FP32 operator) + column (FP32
-    // operator: 2 B in slices with compressed columns, else 4 B); per
-    // compressed slice its window bases; per row: gathered x once, y written, r0 read (FP64
-    // vectors), length, D^-1 for the border row
+    // per entry: value (8 B, or 4 B for the FP32 operator) + column (4 B; the
+    // FP32 operator: 2 B in slices with 16-bit columns, plus their window
+    // bases); per row: gathered x once, y written, r0 read (FP64 vectors),
+    // length, D^-1 for the border row
     const double vb = fp32 ? 4.0 : 8.0;
-    const double n16 = fp32 ? (double)nnz16 : 0.0;  // the FP64 product reads 32-bit columns
+    const double n16 = fp32 ? (double)nnz16 : 0.0;
     return (double)nnz * vb + 2.0 * n16 + 4.0 * ((double)nnz - n16) +
            (fp32 ? (double)nslices16 * 4.0 * CW_COUNT : 0.0) +
            (double)nv * (8.0 * (bordered ? 4.0 : 3.0) + 4.0);
@@ -747,7 +770,7 @@ void PpeSolver::launch_chunk(hipStream_t st, int chunk, const TM* a, const T* di
                            xs);
         const T* yin = precondition(st, pp, yv.p);
         spmv_timer.begin(st);
-        hipLaunchKernelGGL((k_spmv_alpha<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, sv, sg,
+        hipLaunchKernelGGL((k_spmv_alpha<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, view_of<TM>(), sg,
                            a, di, yin, vv, rr0, partial.p, sc.p);
         spmv_timer.end(st);
         hipLaunchKernelGGL(k_finish_alpha<T>, dim3(nseg), dim3(FIN), 0, st, sg, vv, rr0, partial.p,
@@ -756,7 +779,7 @@ void PpeSolver::launch_chunk(hipStream_t st, int chunk, const TM* a, const T* di
                            xs);
         const T* zin = precondition(st, ss, zv.p);
         spmv_timer.begin(st);
-        hipLaunchKernelGGL((k_spmv_omega<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, sv, sg,
+        hipLaunchKernelGGL((k_spmv_omega<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, view_of<TM>(), sg,
                            a, di, zin, ss, tt, partial.p, sc.p);
         spmv_timer.end(st);
         hipLaunchKernelGGL(k_finish_omega<T>, dim3(nseg), dim3(FIN), 0, st, sg, tt, ss, partial.p,
@@ -819,7 +842,8 @@ void PpeSolver::run_bicgstab(hipStream_t st, const TM* a, const T* di, const T* 
     const int nbk = nrows / BLOCK;
     MP_HIP(hipMemcpyAsync(ctl_d.p, host_ctl, sizeof(SegCtl) * nseg, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_sc_reset, dim3((nseg + 63) / 64), dim3(64), 0, st, nseg, sc.p, ctl_d.p);
-    hipLaunchKernelGGL((k_residual_init<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, sv, sg,
+    hipLaunchKernelGGL((k_residual_init<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
+                       view_of<TM>(), sg,
                        a, di, rhs, x, rr, rr0, pp, vv, partial.p, sc.p, 0);
     hipLaunchKernelGGL(k_finish_init<T>, dim3(nseg), dim3(FIN), 0, st, sg, rhs, rr, rr0, pp, vv,
                        partial.p, sc.p, 0, rel_to_rhs);
@@ -846,7 +870,8 @@ void PpeSolver::run_bicgstab(hipStream_t st, const TM* a, const T* di, const T* 
                 // BiCGSTAB does (the kernels act on the systems still at done == 3)
                 MP_HIP(hipEventSynchronize(ev[slot]));
                 hipLaunchKernelGGL((k_residual_init<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0,
-                                   st, sv, sg, a, di, rhs, x, rr, rr0, pp, vv, partial.p, sc.p, 1);
+                                   st, view_of<TM>(), sg, a, di, rhs, x, rr, rr0, pp, vv,
+                                   partial.p, sc.p, 1);
                 hipLaunchKernelGGL(k_finish_init<T>, dim3(nseg), dim3(FIN), 0, st, sg, rhs, rr,
                                    rr0, pp, vv, partial.p, sc.p, 1, rel_to_rhs);
                 MP_HIP(hipGetLastError());
@@ -988,7 +1013,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
             MP_HIP(hipMemcpyAsync(rhs64.p, tmp.p, sizeof(double) * nrows, hipMemcpyDeviceToDevice,
                                   st));
             launch_fill(st, e64.p, 0.0, nrows);
-            run_bicgstab<float, double>(st, aval32.p, dinv.p, rhs64.p, e64.p, r.p, r0.p, p.p, v.p,
+            run_bicgstab<float, double>(st, a32k, dinv.p, rhs64.p, e64.p, r.p, r0.p, p.p, v.p,
                                         s.p, t.p, 1, done, its, rst);
             for (int q = 0; q < nseg; ++q) {
                 if (!act[q]) continue;

@@ -49,15 +49,22 @@ struct PpeSolver {
     int n_sys = 0;   // unknowns: real rows + border unknowns
     int nseg = 1, bordered = 0, dim = 0;
     long nnz = 0;    // real stencil entries of the operator
-    long nnz16 = 0;  // of these, entries in slices with 16-bit columns
-    int nslices16 = 0;
     SellView sv{};
+    // column-sorted slot order (set before setup; slot_perm null: none): the
+    // FP32 operator products run on svs with their values permuted into it
+    SellView svs{};
+    const int* slot_perm = nullptr;
     std::vector<Seg> segs;
     DevBuf<Seg> seg_d;
     DevBuf<int> bseg_d;
     SegView sg{};
     DevBuf<double> aval, dinv;
     DevBuf<float> aval32;
+    DevBuf<float> aval32s;            // aval32 in the column-sorted slot order
+    const float* a32k = nullptr;      // FP32 operator of the Krylov products
+    SellView svk{};                   // its slot view
+    long nnz16 = 0;                   // entries of svk's slices with 16-bit columns
+    int nslices16 = 0;
     DevBuf<double> xt, r, r0, p, v, s, t, tmp, e64, rhs64;
     DevBuf<double> yv, zv;            // preconditioned directions (multigrid)
     DevBuf<double> jac_dinv;          // D^-1 when the operator above is unscaled
@@ -100,6 +107,10 @@ struct PpeSolver {
     void release_graphs();
 
   private:
+    template <typename TM>
+    const SellView& view_of() const {  // the slot view of operator values of type TM
+        return sizeof(TM) == 8 ? sv : svk;
+    }
     template <typename TM, typename T>
     void run_bicgstab(hipStream_t s, const TM* aval_t, const T* dinv_t, const T* rhs, T* x,
                       T* rr, T* rr0, T* pp, T* vv, T* ss, T* tt, int rel_to_rhs,

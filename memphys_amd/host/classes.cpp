@@ -290,23 +290,32 @@ CLOUD::~CLOUD() {
     if (ctx) mp_ctx_destroy(ctx);
 }
 
+// cloud.cpp:570-648 on the GPU (mp_knn_stencils)
 void CLOUD::calc_cloud_points_fast(POINTS& points, PARAMETERS& parameters) {
     const int nv = points.nv, dim = parameters.dimension;
     std::vector<int> k(nv, parameters.cloud_size);
     if ((int)points.stencil_size.size() == nv) k = points.stencil_size;
     std::vector<uint8_t> b(nv);
     for (int i = 0; i < nv; ++i) b[i] = points.boundary_flag[i] ? 1 : 0;
-    try {
-        memphys::knn_stencils(nv, dim, points.xyz.data(), b.data(), k.data(), nb_points_row,
-                              nb_points_col, 0);
-    } catch (const std::exception& e) {
-        fail("CLOUD::calc_cloud_points_fast", e.what());
-    }
+    if (!ctx && mp_ctx_create(parameters.device, &ctx) != MP_OK)
+        fail("CLOUD::calc_cloud_points_fast", "cannot create a GPU context");
+    long total = 0;
+    for (int i = 0; i < nv; ++i) total += std::max(k[i], 0);
+    nb_points_row.assign(nv + 1, 0);
+    nb_points_col.assign(total, 0);
+    check(ctx, mp_knn_stencils(ctx, nv, dim, points.xyz.data(), b.data(), k.data(),
+                               nb_points_row.data(), nb_points_col.data()),
+          "CLOUD::calc_cloud_points_fast");
 }
 
+// cloud.cpp:315-331 with the order from the GPU (mp_rcm_order); the arrays are
+// permuted on the host as re_order_points does (cloud.cpp:338-439)
 void CLOUD::re_order_points_reverse_cuthill_mckee(POINTS& points, PARAMETERS& parameters) {
     const int nv = points.nv, dim = parameters.dimension;
-    memphys::rcm_order(nv, nb_points_row.data(), nb_points_col.data(), rcm_points_order);
+    rcm_points_order.assign(nv, 0);
+    check(ctx, mp_rcm_order(ctx, nv, nb_points_row.data(), nb_points_col.data(),
+                            rcm_points_order.data()),
+          "CLOUD::re_order_points_reverse_cuthill_mckee");
     std::vector<uint8_t> b(nv);
     for (int i = 0; i < nv; ++i) b[i] = points.boundary_flag[i] ? 1 : 0;
     std::vector<int> extra = points.bc_tag;
