@@ -263,6 +263,40 @@ def cpu_baseline(cloud, w, prm, iters_per_step, max_iter_sample=10, repeats=3):
                        f"(the FP64 GPU path's count)")}
 
 
+def cloud_setup_timing(M, ctx, cloud, repeats=3):
+    """Stencil selection and RCM order of the cloud's nodes: GPU (host arrays
+    in and out, transfers included) against the host code on all cores;
+    median of `repeats`, and whether both give the same CSR and order."""
+    nv, dim = cloud.xyz.shape
+    lib = M.host_lib()
+    xyz = np.ascontiguousarray(cloud.xyz).ravel()
+    ks = np.ascontiguousarray(cloud.ksize, dtype=np.int32)
+    rp_h = np.zeros(nv + 1, dtype=np.int32)
+    col_h = np.zeros(int(ks.sum()), dtype=np.int32)
+    ord_h = np.zeros(nv, dtype=np.int32)
+    ctx.knn_stencils(cloud.xyz[:64], np.zeros(64, dtype=np.uint8), np.full(64, 8, dtype=np.int32))
+    t = {"knn_gpu": [], "knn_host": [], "rcm_gpu": [], "rcm_host": []}
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        rp_g, col_g = ctx.knn_stencils(cloud.xyz, cloud.boundary, ks)
+        t1 = time.perf_counter()
+        lib.mph_knn(nv, dim, xyz, cloud.boundary, ks, rp_h, col_h, 0)
+        t2 = time.perf_counter()
+        ord_g = ctx.rcm_order(rp_g, col_g)
+        t3 = time.perf_counter()
+        lib.mph_rcm(nv, rp_h, col_h, ord_h)
+        t4 = time.perf_counter()
+        for key, dt in (("knn_gpu", t1 - t0), ("knn_host", t2 - t1), ("rcm_gpu", t3 - t2),
+                        ("rcm_host", t4 - t3)):
+            t[key].append(dt)
+    out = {k + "_s": float(np.median(v)) for k, v in t.items()}
+    out["host_threads"] = os.cpu_count()
+    out["identical"] = bool(np.array_equal(rp_g, rp_h) and np.array_equal(col_g, col_h)
+                            and np.array_equal(ord_g, ord_h))
+    out["note"] = "GPU times include host-device transfers of the inputs and the CSR/order"
+    return out
+
+
 def cpu_weights_rate(cloud, rows, threads):
     """Oracle FP64 stencil weights on the sampled rows: stencils per second."""
     import oracle as O
@@ -315,6 +349,8 @@ def main():
                         k_layers=(60, 55, 50))
     log(f"cloud nv={cloud.nv} nnz={cloud.nnz} setup {time.perf_counter() - t:.2f}s")
     ctx = M.Context(0)
+    setup = cloud_setup_timing(M, ctx, cloud)
+    log(f"cloud set-up: {setup}")
     ctx.upload(cloud)
     ctx.kernel_timing(True)
     t = time.perf_counter()
@@ -422,6 +458,7 @@ def main():
                                "bytes_per_launch": b64, "mean_ms": ms64, "launches": n64,
                                "launches_returned_early": all64 - n64,
                                "traffic": measured_traffic(args.n, 1)},
+        "cloud_setup": setup,
         "weights_kernel_ms": wk[1],
         "weights_fp64_vs_fp32_refined": weights_check,
         "divergence_fp64": fp64["divergence"],
