@@ -17,6 +17,19 @@ __device__ __forceinline__ int row_base(const SellView& sv, int r) {
     return sell_base(sv.sptr, r);
 }
 
+// A row's neighbours come in groups of four adjacent slots (SELL-64x4): slot
+// s4 = base + 256 g holds neighbours 4g..4g+3, read as one 16-byte load of
+// columns and one 32-byte load per weight array.  Entries at or past the
+// row's length are skipped, so every sum keeps the order of the stencil.
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ i4 cols4(const SellView& sv, int s4) {
+    return *reinterpret_cast<const i4*>(sv.col + s4);
+}
+__device__ __forceinline__ d4 vals4(const double* a, int s4) {
+    return *reinterpret_cast<const d4*>(a + s4);
+}
+
 template <int DIM>
 __global__ __launch_bounds__(BLOCK) void predictor_kernel(
     SellView sv, const double* __restrict__ gx, const double* __restrict__ gy,
@@ -41,21 +54,26 @@ __global__ __launch_bounds__(BLOCK) void predictor_kernel(
     for (int f = 0; f < DIM; ++f)
 #pragma unroll
         for (int o = 0; o < 4; ++o) acc[f][o] = 0.0;
-    for (int j = 0; j < len; ++j) {
-        const int slot = base + sell_off(j);
-        const int c = sv.col[slot];
-        const double a0 = gx[slot], a1 = gy[slot], a3 = lap[slot];
-        const double a2 = (DIM == 3) ? gz[slot] : 0.0;
-        double f[DIM];
-        f[0] = u[c];
-        f[1] = v[c];
-        if (DIM == 3) f[DIM - 1] = w[c];
+    for (int j0 = 0; j0 < len; j0 += 4) {
+        const int s4 = base + (j0 << 6);
+        const i4 c4 = cols4(sv, s4);
+        const d4 A0 = vals4(gx, s4), A1 = vals4(gy, s4), A3 = vals4(lap, s4);
+        const d4 A2 = (DIM == 3) ? vals4(gz, s4) : d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < DIM; ++q) {
-            acc[q][0] = fma(a0, f[q], acc[q][0]);
-            acc[q][1] = fma(a1, f[q], acc[q][1]);
-            if (DIM == 3) acc[q][2] = fma(a2, f[q], acc[q][2]);
-            acc[q][3] = fma(a3, f[q], acc[q][3]);
+        for (int e = 0; e < 4; ++e) {
+            if (j0 + e >= len) break;
+            const int c = c4[e];
+            double f[DIM];
+            f[0] = u[c];
+            f[1] = v[c];
+            if (DIM == 3) f[DIM - 1] = w[c];
+#pragma unroll
+            for (int q = 0; q < DIM; ++q) {
+                acc[q][0] = fma(A0[e], f[q], acc[q][0]);
+                acc[q][1] = fma(A1[e], f[q], acc[q][1]);
+                if (DIM == 3) acc[q][2] = fma(A2[e], f[q], acc[q][2]);
+                acc[q][3] = fma(A3[e], f[q], acc[q][3]);
+            }
         }
     }
     double own[DIM];
@@ -108,12 +126,19 @@ __global__ __launch_bounds__(BLOCK) void ppe_rhs_kernel(
         const int base = row_base(sv, r);
         const int len = sv.len[r];
         double ax = 0.0, ay = 0.0, az = 0.0;
-        for (int j = 0; j < len; ++j) {
-            const int slot = base + sell_off(j);
-            const int c = sv.col[slot];
-            ax = fma(gx[slot], uh[c], ax);
-            ay = fma(gy[slot], vh[c], ay);
-            if (DIM == 3) az = fma(gz[slot], wh[c], az);
+        for (int j0 = 0; j0 < len; j0 += 4) {
+            const int s4 = base + (j0 << 6);
+            const i4 c4 = cols4(sv, s4);
+            const d4 X = vals4(gx, s4), Y = vals4(gy, s4);
+            const d4 Z = (DIM == 3) ? vals4(gz, s4) : d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (j0 + e >= len) break;
+                const int c = c4[e];
+                ax = fma(X[e], uh[c], ax);
+                ay = fma(Y[e], vh[c], ay);
+                if (DIM == 3) az = fma(Z[e], wh[c], az);
+            }
         }
         out = ax + ay;
         if (DIM == 3) out = out + az;
@@ -140,12 +165,19 @@ __global__ __launch_bounds__(BLOCK) void corrector_kernel(
     const int base = row_base(sv, r);
     const int len = sv.len[r];
     double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int j = 0; j < len; ++j) {
-        const int slot = base + sell_off(j);
-        const double pc = p[sv.col[slot]];
-        ax = fma(gx[slot], pc, ax);
-        ay = fma(gy[slot], pc, ay);
-        if (DIM == 3) az = fma(gz[slot], pc, az);
+    for (int j0 = 0; j0 < len; j0 += 4) {
+        const int s4 = base + (j0 << 6);
+        const i4 c4 = cols4(sv, s4);
+        const d4 X = vals4(gx, s4), Y = vals4(gy, s4);
+        const d4 Z = (DIM == 3) ? vals4(gz, s4) : d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (j0 + e >= len) break;
+            const double pc = p[c4[e]];
+            ax = fma(X[e], pc, ax);
+            ay = fma(Y[e], pc, ay);
+            if (DIM == 3) az = fma(Z[e], pc, az);
+        }
     }
     un[r] = uh[r] - (dt * ax) / rho;
     vn[r] = vh[r] - (dt * ay) / rho;
@@ -181,15 +213,22 @@ __global__ __launch_bounds__(BLOCK) void boundary_fix_kernel(
             news[q][r] = olds[q][r];
         } else {
             double rhs = 0.0, diag = 1.0;
-            for (int j = 0; j < len; ++j) {
-                const int slot = base + sell_off(j);
-                const int c = sv.col[slot];
-                double cf = gx[slot] * nx + gy[slot] * ny;
-                if (DIM == 3) cf = cf + gz[slot] * nz;
-                if (c == r)
-                    diag = cf;
-                else
-                    rhs = rhs - cf * news[q][c];
+            for (int j0 = 0; j0 < len; j0 += 4) {
+                const int s4 = base + (j0 << 6);
+                const i4 c4 = cols4(sv, s4);
+                const d4 X = vals4(gx, s4), Y = vals4(gy, s4);
+                const d4 Z = (DIM == 3) ? vals4(gz, s4) : d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (j0 + e >= len) break;
+                    const int c = c4[e];
+                    double cf = X[e] * nx + Y[e] * ny;
+                    if (DIM == 3) cf = cf + Z[e] * nz;
+                    if (c == r)
+                        diag = cf;
+                    else
+                        rhs = rhs - cf * news[q][c];
+                }
             }
             news[q][r] = rhs / diag;
         }
@@ -204,9 +243,15 @@ __global__ __launch_bounds__(BLOCK) void spmv_kernel(SellView sv, const double* 
     const int base = row_base(sv, r);
     const int len = sv.len[r];
     double s = 0.0;
-    for (int j = 0; j < len; ++j) {
-        const int slot = base + sell_off(j);
-        s = fma(val[slot], x[sv.col[slot]], s);
+    for (int j0 = 0; j0 < len; j0 += 4) {
+        const int s4 = base + (j0 << 6);
+        const i4 c4 = cols4(sv, s4);
+        const d4 A = vals4(val, s4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (j0 + e >= len) break;
+            s = fma(A[e], x[c4[e]], s);
+        }
     }
     y[r] = s;
 }
@@ -223,12 +268,19 @@ __global__ __launch_bounds__(BLOCK) void divergence_kernel(
         const int base = row_base(sv, r);
         const int len = sv.len[r];
         double ax = 0.0, ay = 0.0, az = 0.0;
-        for (int j = 0; j < len; ++j) {
-            const int slot = base + sell_off(j);
-            const int c = sv.col[slot];
-            ax = fma(gx[slot], u[c], ax);
-            ay = fma(gy[slot], v[c], ay);
-            if (DIM == 3) az = fma(gz[slot], w[c], az);
+        for (int j0 = 0; j0 < len; j0 += 4) {
+            const int s4 = base + (j0 << 6);
+            const i4 c4 = cols4(sv, s4);
+            const d4 X = vals4(gx, s4), Y = vals4(gy, s4);
+            const d4 Z = (DIM == 3) ? vals4(gz, s4) : d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (j0 + e >= len) break;
+                const int c = c4[e];
+                ax = fma(X[e], u[c], ax);
+                ay = fma(Y[e], v[c], ay);
+                if (DIM == 3) az = fma(Z[e], w[c], az);
+            }
         }
         double res = ax + ay;
         if (DIM == 3) res = res + az;
@@ -275,26 +327,38 @@ __global__ __launch_bounds__(BLOCK) void build_operator_kernel(
     const int base = row_base(sv, r);
     const int len = sv.len[r];
     double dg = 0.0;
-    for (int j = 0; j < len; ++j) {
-        const int slot = base + sell_off(j);
-        const int c = sv.col[slot];
-        double val;
-        if (bnd[r]) {
-            if (dir[r]) {
-                val = (c == r) ? 1.0 : 0.0;
+    const bool b = bnd[r] != 0, dr = dir[r] != 0;
+    const double n0 = b ? nrm[r] : 0.0, n1 = b ? nrm[(size_t)npad + r] : 0.0;
+    const double n2 = (b && dim == 3) ? nrm[2 * (size_t)npad + r] : 0.0;
+    for (int j0 = 0; j0 < len; j0 += 4) {
+        const int s4 = base + (j0 << 6);
+        const i4 c4 = cols4(sv, s4);
+        const d4 X = vals4(gx, s4), Y = vals4(gy, s4), L = vals4(lap, s4);
+        const d4 Z = (dim == 3) ? vals4(gz, s4) : d4{0.0, 0.0, 0.0, 0.0};
+        d4 out = {0.0, 0.0, 0.0, 0.0};  // slots past the row's length stay 0
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (j0 + e >= len) break;
+            const int c = c4[e];
+            double val;
+            if (b) {
+                if (dr) {
+                    val = (c == r) ? 1.0 : 0.0;
+                } else {
+                    val = n0 * X[e] + n1 * Y[e];
+                    if (dim == 3) val += n2 * Z[e];
+                }
             } else {
-                val = nrm[r] * gx[slot] + nrm[(size_t)npad + r] * gy[slot];
-                if (dim == 3) val += nrm[2 * (size_t)npad + r] * gz[slot];
+                val = diff * L[e];
+                val += conv * X[e];
+                val += conv * Y[e];
+                if (dim == 3) val += conv * Z[e];
+                if (c == r) val += unsteady;
             }
-        } else {
-            val = diff * lap[slot];
-            val += conv * gx[slot];
-            val += conv * gy[slot];
-            if (dim == 3) val += conv * gz[slot];
-            if (c == r) val += unsteady;
+            out[e] = val;
+            if (c == r) dg = val;
         }
-        aval[slot] = val;
-        if (c == r) dg = val;
+        *reinterpret_cast<d4*>(aval + s4) = out;
     }
     diag[r] = dg;
 }

@@ -511,11 +511,18 @@ __global__ __launch_bounds__(BLOCK) void k_scale_columns(SellView sv, const doub
                                                          T* __restrict__ out) {
     const int r = blockIdx.x * BLOCK + threadIdx.x;
     if (r >= sv.nv) return;
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    typedef T t4 __attribute__((ext_vector_type(4)));
     const int base = sell_base(sv.sptr, r);
     const int len = sv.len[r];
-    for (int j = 0; j < len; ++j) {
-        const int slot = base + sell_off(j);
-        out[slot] = (T)(a[slot] * dinv[sv.col[slot]]);
+    for (int j0 = 0; j0 < len; j0 += 4) {  // groups of four slots; past the length a = 0
+        const int s4 = base + (j0 << 6);
+        const i4 c = *reinterpret_cast<const i4*>(sv.col + s4);
+        const d4 v = *reinterpret_cast<const d4*>(a + s4);
+        const t4 o = {(T)(v.x * dinv[c.x]), (T)(v.y * dinv[c.y]), (T)(v.z * dinv[c.z]),
+                      (T)(v.w * dinv[c.w])};
+        *reinterpret_cast<t4*>(out + s4) = o;
     }
 }
 
