@@ -4,9 +4,17 @@
 // reference writes to its simulation_details.csv
 // (postprocessing_functions.cpp:52-152), one "key,value" per line.
 //
-// usage: annulus_operators <parameters.csv> <mesh.msh> <u0> <v0> <alpha>
+// With an output prefix and a tecplot file it also reads the file's u_new,
+// v_new, p_new and T_new onto the nodes (read_restart_tecplot_file), writes
+// them back (write_tecplot_steady_variables) and writes the
+// simulation_details.csv (write_simulation_details), with the fluid of the
+// reference's sample run (rho = 10, mu = 0.03).
+//
+// usage: annulus_operators <parameters.csv> <mesh.msh> <u0> <v0> <alpha> [prefix tecplot.plt]
 #include <cstdio>
 #include <cstdlib>
+#include <string>
+#include <vector>
 
 #include "../memphys_amd/host/memphys.hpp"
 
@@ -37,6 +45,17 @@ int main(int argc, char** argv) {
         printf("laplace_eigval,%.16g,%.16g\n", parameters.laplace_eigval_real,
                parameters.laplace_eigval_imag);
         printf("timestep_dt,%.16g\n", parameters.dt);
+        if (argc >= 8) {
+            parameters.output_file_prefix = argv[6];
+            parameters.rho = 10.0;
+            parameters.mu = 0.03;
+            VectorXd u(points.nv), v(points.nv), p(points.nv), T(points.nv);
+            std::vector<std::string> names = {"u_new", "v_new", "p_new", "T_new"};
+            std::vector<VectorXd*> fields = {&u, &v, &p, &T};
+            read_restart_tecplot_file(points, parameters, argv[7], names, fields);
+            write_tecplot_steady_variables(points, parameters, names, fields);
+            write_simulation_details(points, cloud, parameters);
+        }
     } catch (const std::exception&) {
         return 1;
     }

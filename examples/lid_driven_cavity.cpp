@@ -73,6 +73,9 @@ int main(int argc, char** argv) {
         if (dim == 3) fwrite(w_new.data(), sizeof(double), nv, f);
         fwrite(p_new.data(), sizeof(double), p_new.size(), f);
         fclose(f);
+        // the solver log of every step (postprocessing_functions.cpp:52-72)
+        parameters.output_file_prefix = argv[7];
+        write_iteration_details(parameters);
         delete fs;
     } catch (const std::exception&) {
         return 1;

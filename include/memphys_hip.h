@@ -29,7 +29,12 @@ enum mp_status {
     MP_ERR_NOCONV = 4    /* Krylov solve stopped above tolerance */
 };
 
-enum mp_field { MP_U = 0, MP_V = 1, MP_W = 2, MP_P = 3 };
+/* Fields of mp_fields_set/get.  MP_SU..MP_SW: the AB2 source history (the
+ * explicit right-hand side S of the last step, S_old of fractional_step_1.cpp
+ * :120-134), which the next AB2 step combines with its own; with the
+ * velocities, the pressure and the step counter it makes a restart exact
+ * (no counterpart in the reference, whose tecplot restart starts AB2 anew). */
+enum mp_field { MP_U = 0, MP_V = 1, MP_W = 2, MP_P = 3, MP_SU = 4, MP_SV = 5, MP_SW = 6 };
 enum mp_op { MP_GX = 0, MP_GY = 1, MP_GZ = 2, MP_LAP = 3 };
 
 /* PPE arithmetic: plain FP64 BiCGSTAB, or FP32 inner BiCGSTAB inside FP64

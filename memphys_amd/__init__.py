@@ -21,6 +21,7 @@ HIP_LIB_PATH = os.path.join(_HERE, "libmemphys_hip.so")
 HOST_LIB_PATH = os.path.join(_HERE, "libmemphys_host.so")
 
 MP_U, MP_V, MP_W, MP_P = 0, 1, 2, 3
+MP_SU, MP_SV, MP_SW = 4, 5, 6  # AB2 source history (exact restart)
 MP_GX, MP_GY, MP_GZ, MP_LAP = 0, 1, 2, 3
 MP_PREC_FP64, MP_PREC_MIXED = 0, 1
 MP_GUESS_PREVIOUS, MP_GUESS_LEARNED, MP_GUESS_EXTRAPOLATE = 0, 1, 2

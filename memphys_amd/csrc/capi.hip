@@ -862,13 +862,18 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
 int mp_fields_set(mp_ctx* c, int which, const double* values) {
     return guarded(c, "mp_fields_set", [&]() -> int {
         if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_fields_set: mp_fs_create not called");
-        if (!values || which < 0 || which > 3 || (which == MP_W && c->dim != 3))
+        if (!values || which < 0 || which > MP_SW ||
+            ((which == MP_W || which == MP_SW) && c->dim != 3))
             return fail(c, MP_ERR_ARG, "mp_fields_set: bad field");
         std::vector<double> h;
         if (which == MP_P) {
             to_device_order(c, values, h, c->ppe.get());
             upload(c, c->P.p, h);
             c->hist = 0;  // the pressure history no longer matches
+        } else if (which >= MP_SU) {  // AB2 source history, component which - MP_SU
+            to_device_order(c, values, h);
+            h.resize(c->npad);
+            upload(c, c->S[c->s_cur].p + (size_t)(which - MP_SU) * c->npad, h);
         } else {
             to_device_order(c, values, h);
             upload(c, c->U[c->cur][which].p, h);
@@ -880,12 +885,16 @@ int mp_fields_set(mp_ctx* c, int which, const double* values) {
 int mp_fields_get(mp_ctx* c, int which, double* values) {
     return guarded(c, "mp_fields_get", [&]() -> int {
         if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_fields_get: mp_fs_create not called");
-        if (!values || which < 0 || which > 3 || (which == MP_W && c->dim != 3))
+        if (!values || which < 0 || which > MP_SW ||
+            ((which == MP_W || which == MP_SW) && c->dim != 3))
             return fail(c, MP_ERR_ARG, "mp_fields_get: bad field");
         std::vector<double> h;
         if (which == MP_P) {
             download(c, c->P.p, h);
             from_device_order(c, h, values, c->ppe.get());
+        } else if (which >= MP_SU) {
+            download(c, c->S[c->s_cur].p + (size_t)(which - MP_SU) * c->npad, h);
+            from_device_order(c, h, values);
         } else {
             download(c, c->U[c->cur][which].p, h);
             from_device_order(c, h, values);

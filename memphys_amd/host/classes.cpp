@@ -251,6 +251,8 @@ POINTS::POINTS(PARAMETERS& parameters) {
     nv = g.nv;
     nv_original = g.nv_original;
     xyz = g.xyz;
+    xyz_original = g.xyz_original;
+    elem_vert_original = g.elem_vert_original;
     normal = g.normal;
     boundary_flag.assign(g.boundary.begin(), g.boundary.end());
     corner_edge_vertices.assign(g.nv_original, true);
@@ -280,7 +282,10 @@ CLOUD::CLOUD(POINTS& points, PARAMETERS& parameters) {
     t1 = clock();
     re_order_points_reverse_cuthill_mckee(points, parameters);
     parameters.rcm_timer = (double)(clock() - t1) / CLOCKS_PER_SEC;
+    t1 = clock();
+    calc_iv_original_nearest_vert(points, parameters);
     calc_charac_dx(points, parameters);
+    parameters.cloud_misc_timer = (double)(clock() - t1) / CLOCKS_PER_SEC;
     calc_grad_laplace_coeffs(points, parameters);
     EIGEN_set_grad_laplace_matrix(points, parameters);
     EIGEN_set_grad_laplace_matrix_separate(points, parameters);

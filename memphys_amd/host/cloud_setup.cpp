@@ -564,6 +564,11 @@ GmshCloud read_gmsh22(const std::string& path) {
     out.dim = dim;
     out.nv_original = nn;
     out.nelem = ncv;
+    out.xyz_original.resize((size_t)nn * dim);
+    for (int v = 0; v < nn; ++v)
+        for (int k = 0; k < dim; ++k) out.xyz_original[(size_t)v * dim + k] = X(v, k);
+    out.elem_vert_original.reserve(ncv);
+    for (int c = 0; c < ncv; ++c) out.elem_vert_original.push_back(elems[el_idx[c]].v);
     for (int v = 0; v < nn; ++v) {
         if (drop[v]) continue;
         for (int k = 0; k < dim; ++k) {

@@ -83,6 +83,10 @@ struct GmshCloud {
     std::vector<uint8_t> boundary;  // nv
     std::vector<double> normal;     // nv*dim (zero on interior nodes)
     std::vector<int> original;      // nv: index of each kept node in the file's node list
+    std::vector<double> xyz_original;  // nv_original*dim: every node of the file, file order
+    // elements as POINTS::read_elem_vert_complete_msh keeps them (points.cpp:568-645):
+    // 2D lines and triangles, 3D triangles and tetrahedra, node indices in file order
+    std::vector<std::vector<int>> elem_vert_original;
 };
 GmshCloud read_gmsh22(const std::string& path);
 

@@ -141,6 +141,8 @@ class POINTS {
     std::vector<double> xyz_min, xyz_max, xyz_length;
     std::vector<double> xyz;
     std::vector<double> xyz_original;
+    std::vector<std::vector<int>> elem_vert_original;  // connectivity of the original mesh
+    std::vector<int> iv_original_nearest_vert;         // [nv_original]: node holding each original vertex
     std::vector<double> normal;
     std::vector<bool> boundary_flag;
     std::vector<int> bc_tag;
@@ -181,6 +183,7 @@ class CLOUD {
     CLOUD& operator=(const CLOUD&) = delete;
     void calc_cloud_points_fast(POINTS& points, PARAMETERS& parameters);
     void re_order_points_reverse_cuthill_mckee(POINTS& points, PARAMETERS& parameters);
+    void calc_iv_original_nearest_vert(POINTS& points, PARAMETERS& parameters);
     void calc_charac_dx(POINTS& points, PARAMETERS& parameters);
     void calc_grad_laplace_coeffs(POINTS& points, PARAMETERS& parameters);
     void EIGEN_set_grad_laplace_matrix(POINTS& points, PARAMETERS& parameters);
@@ -233,6 +236,17 @@ class FRACTIONAL_STEP_1 {
               VectorXd& p_new, VectorXd* U_old[3], VectorXd& p_old, int it1);
     mp_ctx* ctx = nullptr;
 };
+
+// Output and restart files in the reference's formats
+// (postprocessing_functions.cpp:52-152, 597-682, 765-805).
+void write_iteration_details(PARAMETERS& parameters);
+void write_simulation_details(POINTS& points, CLOUD& cloud, PARAMETERS& parameters);
+void write_tecplot_steady_variables(POINTS& points, PARAMETERS& parameters,
+                                    std::vector<std::string>& variable_names,
+                                    std::vector<VectorXd*>& variable_pointers);
+void read_restart_tecplot_file(POINTS& points, PARAMETERS& parameters, std::string filename,
+                               std::vector<std::string>& tecplot_variable_names,
+                               std::vector<VectorXd*>& variable_pointers);
 
 // Continuity residual of postprocessing_functions.cpp:192-195 / 248-251:
 // out = {max_int, mean_int, max_bnd, mean_bnd, max_all, mean_all}.

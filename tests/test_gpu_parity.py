@@ -265,6 +265,17 @@ def test_class_api_driver_matches_oracle(tmp_path):
     errs = (rel_err(u, uo), rel_err(v, vo), rel_err(p, po))
     _print("class API driver errs", errs)
     assert max(errs) <= 1e-8
+    # write_iteration_details (postprocessing_functions.cpp:52-72): one row per step
+    import re
+    lines = open(out + "_iteration_details.csv").read().splitlines()
+    assert lines[0] in ("it,n_iter_actual,rel_res_log,abs_res_log",
+                        "it,n_iter_actual,rel_res_log,abs_res_log,regul_alpha_log")
+    rows = [row.split(",") for row in lines[1:]]
+    assert [int(row[0]) for row in rows] == list(range(steps))
+    m = re.search(r"last PPE iterations (\d+) rel_res (\S+)", r.stdout)
+    assert int(rows[-1][1]) == int(m.group(1))
+    assert float(rows[-1][2]) == pytest.approx(float(m.group(2)), rel=1e-3)
+    assert all(float(row[2]) <= 1e-8 for row in rows)
 
 
 def test_weights_mixed_precision_match_oracle(ctx, cube3d_small, cavity2d):
@@ -320,9 +331,12 @@ def test_class_api_gmsh_setup_matches_reference_sample_output(tmp_path):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     exe = os.path.join(root, "examples", "annulus_operators")
     a = KA["calc_dt_args"]
+    golden = os.path.join(root, "tests", "golden")
+    sample = os.path.join(golden, "ecc_ratio_0.5_r_i_0.6_n_200_polydeg_3")
+    prefix = str(tmp_path / "annulus")
     out = subprocess.run([exe, os.path.join(root, "examples", "parameters_annulus.csv"), MESH,
-                          str(a["u0"]), str(a["v0"]), str(a["alpha"])], capture_output=True,
-                         text=True, timeout=300)
+                          str(a["u0"]), str(a["v0"]), str(a["alpha"]), prefix,
+                          sample + "_tecplot.plt"], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
     got = {}
     for line in out.stdout.splitlines():
@@ -344,3 +358,42 @@ def test_class_api_gmsh_setup_matches_reference_sample_output(tmp_path):
         assert re == pytest.approx(KA[k][0], rel=1e-9)
         assert abs(im - abs(KA[k][1])) <= 1e-9 * abs(KA[k][0])
     assert got["timestep_dt"][0] == pytest.approx(KA["timestep_dt"], rel=1e-9)
+    # read_restart_tecplot_file + write_tecplot_steady_variables: the reference's
+    # own tecplot file, read onto the nodes and written back, byte for byte
+    # (coordinates of every original vertex, the deleted vertices' values from
+    # their nearest node, the triangles)
+    mine = open(prefix + "_tecplot.plt", "rb").read()
+    ref = open(sample + "_tecplot.plt", "rb").read()
+    _print("tecplot round trip:", len(mine), "bytes, identical", mine == ref)
+    assert mine == ref
+    # write_simulation_details against the reference's file: same lines in the
+    # same order; host, user, paths, the step count of its time loop and the
+    # timers differ by nature
+    mine = open(prefix + "_simulation_details.csv").read().splitlines()
+    ref = open(sample + "_simulation_details.csv").read().splitlines()
+    mine = [ln for ln in mine if not ln.startswith("restart,file")]
+    assert len(mine) == len(ref)
+    tol = {"Characteristic": 1e-14, "RBF_cond_num_max": 1e-12, "RBF_cond_num_min": 1e-12,
+           "RBF_cond_num_avg": 1e-12, "grad_x_eigval": 1e-9, "grad_y_eigval": 1e-9,
+           "laplace_eigval": 1e-9, "timestep_dt": 1e-9}
+    skip = ("hostname", "username", "meshfile", "nt_actual")
+    timers = ref.index("Run Times,(seconds)")
+    for i, (m_, r_) in enumerate(zip(mine, ref)):
+        mk, rk = m_.split(","), r_.split(",")
+        if i > timers:
+            assert mk[0] == rk[0]
+            continue
+        if rk[0] in skip:
+            assert mk[0] == rk[0]
+            continue
+        key = rk[0]
+        if key in tol:
+            assert mk[0] == rk[0] and len(mk) == len(rk), (m_, r_)
+            nums_m = [float(x) for x in mk[1:] if x not in ("", "seconds")
+                      and not x.startswith("dx_")]
+            nums_r = [float(x) for x in rk[1:] if x not in ("", "seconds")
+                      and not x.startswith("dx_")]
+            for x, y in zip(nums_m, nums_r):
+                assert x == pytest.approx(y, rel=tol[key], abs=tol[key] * 1e3), (m_, r_)
+        else:
+            assert m_ == r_, (m_, r_)
