@@ -4,7 +4,7 @@
 // (one 256-byte row per node, so a wavefront reads a neighbour's state with one
 // coalesced load).  Weights are read as BF16 and widened to FP32 in LDS; every
 // product accumulates in FP32.  Message passing uses the stencil graph in
-// SELL-64 form with the FP32 values of A~ = A D^-1.
+// SELL-64x4 form with the FP32 values of A~ = A D^-1.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -72,6 +72,10 @@ __global__ __launch_bounds__(256) void k_features(int nv, const double* __restri
 }
 
 // Z = in1 W1 (+ in2 W2) + b; Hout = relu(Z).  in1: [nv][K1], in2: [nv][H].
+// A block takes tiles of DR rows: it stages the tile's inputs in LDS and each
+// thread forms 4 rows x 4 outputs, every output's fma chain in the order
+// bias, in1 columns, in2 columns.
+constexpr int DR = 64;  // rows per tile of the dense kernel
 template <int K1, bool TWO>
 __global__ __launch_bounds__(256) void k_dense(int nv, const float* __restrict__ in1,
                                                const float* __restrict__ in2,
@@ -79,27 +83,68 @@ __global__ __launch_bounds__(256) void k_dense(int nv, const float* __restrict__
                                                const uint16_t* __restrict__ W2,
                                                const uint16_t* __restrict__ bias,
                                                float* __restrict__ Z, float* __restrict__ Hout) {
-    __shared__ float w1[K1 * H];
-    __shared__ float w2[TWO ? H * H : 1];
-    __shared__ float bb[H];
-    for (int t = threadIdx.x; t < K1 * H; t += 256) w1[t] = bf2f(W1[t]);
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    __shared__ f4 w1[K1 * H / 4];
+    __shared__ f4 w2[TWO ? H * H / 4 : 1];
+    __shared__ f4 bb[H / 4];
+    __shared__ float a1[DR][K1 + 1];
+    __shared__ float a2[TWO ? DR : 1][H + 1];
+    float* w1s = reinterpret_cast<float*>(w1);
+    float* w2s = reinterpret_cast<float*>(w2);
+    float* bbs = reinterpret_cast<float*>(bb);
+    for (int t = threadIdx.x; t < K1 * H; t += 256) w1s[t] = bf2f(W1[t]);
     if (TWO)
-        for (int t = threadIdx.x; t < H * H; t += 256) w2[t] = bf2f(W2[t]);
-    if (threadIdx.x < H) bb[threadIdx.x] = bf2f(bias[threadIdx.x]);
-    __syncthreads();
-    const int c = threadIdx.x & 63;
-    for (int row = blockIdx.x * RG + (threadIdx.x >> 6); row < nv; row += gridDim.x * RG) {
-        float acc = bb[c];
-        const float* a = in1 + (size_t)row * K1;
-#pragma unroll 8
-        for (int k = 0; k < K1; ++k) acc = fmaf(a[k], w1[k * H + c], acc);
-        if (TWO) {
-            const float* m = in2 + (size_t)row * H;
-#pragma unroll 8
-            for (int k = 0; k < H; ++k) acc = fmaf(m[k], w2[k * H + c], acc);
+        for (int t = threadIdx.x; t < H * H; t += 256) w2s[t] = bf2f(W2[t]);
+    if (threadIdx.x < H) bbs[threadIdx.x] = bf2f(bias[threadIdx.x]);
+    const int cg = threadIdx.x & 15;  // outputs 4cg..4cg+3
+    const int rg = threadIdx.x >> 4;  // tile rows 4rg..4rg+3
+    for (int r0 = blockIdx.x * DR; r0 < nv; r0 += gridDim.x * DR) {
+        const int nr = min(DR, nv - r0);
+        __syncthreads();  // weights written / previous tile's LDS reads done
+        for (int t = threadIdx.x; t < nr * K1; t += 256)
+            a1[t / K1][t % K1] = in1[(size_t)r0 * K1 + t];
+        if (TWO)
+            for (int t = threadIdx.x; t < nr * H; t += 256) a2[t / H][t % H] = in2[(size_t)r0 * H + t];
+        __syncthreads();
+        f4 acc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = bb[cg];
+#pragma unroll 4
+        for (int k = 0; k < K1; ++k) {
+            const f4 w = w1[k * (H / 4) + cg];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float a = a1[4 * rg + i][k];
+                acc[i].x = fmaf(a, w.x, acc[i].x);
+                acc[i].y = fmaf(a, w.y, acc[i].y);
+                acc[i].z = fmaf(a, w.z, acc[i].z);
+                acc[i].w = fmaf(a, w.w, acc[i].w);
+            }
         }
-        Z[(size_t)row * H + c] = acc;
-        Hout[(size_t)row * H + c] = fmaxf(acc, 0.0f);
+        if (TWO) {
+#pragma unroll 4
+            for (int k = 0; k < H; ++k) {
+                const f4 w = w2[k * (H / 4) + cg];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float a = a2[4 * rg + i][k];
+                    acc[i].x = fmaf(a, w.x, acc[i].x);
+                    acc[i].y = fmaf(a, w.y, acc[i].y);
+                    acc[i].z = fmaf(a, w.z, acc[i].z);
+                    acc[i].w = fmaf(a, w.w, acc[i].w);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4 * rg + i;
+            if (row >= nr) break;
+            const size_t o = (size_t)(r0 + row) * H + 4 * cg;
+            *reinterpret_cast<f4*>(Z + o) = acc[i];
+            const f4 h = {fmaxf(acc[i].x, 0.0f), fmaxf(acc[i].y, 0.0f), fmaxf(acc[i].z, 0.0f),
+                          fmaxf(acc[i].w, 0.0f)};
+            *reinterpret_cast<f4*>(Hout + o) = h;
+        }
     }
 }
 
@@ -123,20 +168,54 @@ __global__ __launch_bounds__(256) void k_dense_t(int nv, const float* __restrict
 }
 
 // m[i][:] = sum_j a_ij h[j][:] over a SELL-64x4 operator; one wavefront per row,
-// lane = feature.  accumulate: m += instead of m =.
+// lane = feature.  accumulate: m += instead of m =.  The row's slots come in
+// groups of four (one uniform load of columns and one of values); the
+// gathers of two groups are issued before their fmas, which keep the order
+// of the row's entries.
 __global__ __launch_bounds__(256) void k_aggregate(SellView sv, const float* __restrict__ val,
                                                    const float* __restrict__ h,
                                                    float* __restrict__ m, int accumulate) {
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    typedef float f4 __attribute__((ext_vector_type(4)));
     const int row = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (row >= sv.nv) return;
     const int c = threadIdx.x & 63;
     const int base = sell_base(sv.sptr, row);
     const int len = sv.len[row];
     float acc = accumulate ? m[(size_t)row * H + c] : 0.0f;
-    for (int j = 0; j < len; ++j) {
-        const int slot = base + sell_off(j);
-        const int col = sv.col[slot];
-        acc = fmaf(val[slot], h[(size_t)col * H + c], acc);
+    int j0 = 0;
+    for (; j0 + 8 <= len; j0 += 8) {
+        const int s4 = base + (j0 << 6);
+        const i4 c0 = *reinterpret_cast<const i4*>(sv.col + s4);
+        const i4 c1 = *reinterpret_cast<const i4*>(sv.col + s4 + 256);
+        const f4 v0 = *reinterpret_cast<const f4*>(val + s4);
+        const f4 v1 = *reinterpret_cast<const f4*>(val + s4 + 256);
+        const float x0 = h[(size_t)c0.x * H + c], x1 = h[(size_t)c0.y * H + c];
+        const float x2 = h[(size_t)c0.z * H + c], x3 = h[(size_t)c0.w * H + c];
+        const float x4 = h[(size_t)c1.x * H + c], x5 = h[(size_t)c1.y * H + c];
+        const float x6 = h[(size_t)c1.z * H + c], x7 = h[(size_t)c1.w * H + c];
+        acc = fmaf(v0.x, x0, acc);
+        acc = fmaf(v0.y, x1, acc);
+        acc = fmaf(v0.z, x2, acc);
+        acc = fmaf(v0.w, x3, acc);
+        acc = fmaf(v1.x, x4, acc);
+        acc = fmaf(v1.y, x5, acc);
+        acc = fmaf(v1.z, x6, acc);
+        acc = fmaf(v1.w, x7, acc);
+    }
+    for (; j0 < len; j0 += 4) {
+        const int s4 = base + (j0 << 6);
+        const i4 cc = *reinterpret_cast<const i4*>(sv.col + s4);
+        const f4 vv = *reinterpret_cast<const f4*>(val + s4);
+        const int n = min(4, len - j0);  // uniform: entries of this group inside the row
+        const float x0 = h[(size_t)cc.x * H + c];
+        const float x1 = n > 1 ? h[(size_t)cc.y * H + c] : 0.0f;
+        const float x2 = n > 2 ? h[(size_t)cc.z * H + c] : 0.0f;
+        const float x3 = n > 3 ? h[(size_t)cc.w * H + c] : 0.0f;
+        acc = fmaf(vv.x, x0, acc);
+        if (n > 1) acc = fmaf(vv.y, x1, acc);
+        if (n > 2) acc = fmaf(vv.z, x2, acc);
+        if (n > 3) acc = fmaf(vv.w, x3, acc);
     }
     m[(size_t)row * H + c] = acc;
 }
@@ -437,7 +516,7 @@ void GuessNet::features(hipStream_t st, const double* p1, const double* p2, cons
 
 void GuessNet::forward(hipStream_t st) {
     const uint16_t* P = theta_bf.p;
-    const int g = std::min(DENSE_GRID, (nv + RG - 1) / RG);
+    const int g = std::min(DENSE_GRID, (nv + DR - 1) / DR);
     hipLaunchKernelGGL((k_dense<F, false>), dim3(g), dim3(256), 0, st, nv, X.p, nullptr,
                        P + GNN_W0, nullptr, P + GNN_B0, Z[0].p, Hh[0].p);
     for (int l = 1; l <= GNN_L; ++l) {
