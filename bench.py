@@ -320,9 +320,10 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel instead of replaying hipGraphs (same results; "
                          "for profilers)")
-    ap.add_argument("--forms", default="fp64,mixed,headline",
+    ap.add_argument("--forms", default="fp64,mixed,multigrid,headline",
                     help="comma list of forms to time besides fp64: mixed (FP32-inner, "
-                         "Jacobi), headline (FP32-inner, multigrid, learned guess)")
+                         "Jacobi), multigrid (FP32-inner, multigrid, previous pressure as "
+                         "the guess), headline (FP32-inner, multigrid, learned guess)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
@@ -394,6 +395,10 @@ def main():
     if "mixed" in forms:
         extra["mixed_jacobi"] = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps,
                                          args.warmup, "mixed")
+    if "multigrid" in forms:  # the headline form without the learned guess
+        extra["mixed_multigrid"] = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps,
+                                            args.warmup, "mixed+multigrid",
+                                            precond=M.MP_PRECOND_AMG)
     if "headline" in forms:
         head = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps, args.warmup,
                         "mixed+multigrid+learned", guess=guess, precond=M.MP_PRECOND_AMG)

@@ -941,7 +941,7 @@ void fs_step_one(mp_ctx* c, int it) {
             const double* r = c->ppe->residual_of(c->stream, c->b.p, c->P.p);
             c->net->features(c->stream, c->P1.p, c->P2.p, r, c->ppe->diag_inv, c->xyz.p, c->npad,
                              c->dim);
-            c->net->forward(c->stream);
+            c->net->forward(c->stream, c->train != 0);
             c->net->guess(c->stream, c->P1.p, c->P2.p, c->P.p);
             learned = true;
         }
@@ -1088,7 +1088,7 @@ int mp_guess_eval(mp_ctx* c, const double* p1, const double* p2, const double* b
                            n_sys, d1.p, d2.p, dg.p);
         const double* r = c->ppe->residual_of(c->stream, db.p, dg.p);
         c->net->features(c->stream, d1.p, d2.p, r, c->ppe->diag_inv, c->xyz.p, c->npad, c->dim);
-        c->net->forward(c->stream);
+        c->net->forward(c->stream, p_true != nullptr);
         c->net->guess(c->stream, d1.p, d2.p, dg.p);
         download(c, dg.p, h);
         from_device_order(c, h, guess);

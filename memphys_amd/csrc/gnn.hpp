@@ -54,7 +54,8 @@ struct GuessNet {
     // features from p1, p2, b and the FP64 residual of p_ext (device order)
     void features(hipStream_t st, const double* p1, const double* p2, const double* resid,
                   const double* dinv, const double* xyz_soa, int npad, int dim);
-    void forward(hipStream_t st);
+    // keep_pre: store the pre-activations, which a following backward() needs
+    void forward(hipStream_t st, bool keep_pre = true);
     // guess = p_ext + s d  (FP64 output, node block only)
     void guess(hipStream_t st, const double* p1, const double* p2, double* out);
     // loss and gradient of the MSE against p_true (sets grad); returns nothing (loss in scal[1])

@@ -71,7 +71,8 @@ __global__ __launch_bounds__(256) void k_features(int nv, const double* __restri
     X[i * F + 4] = (dim == 3) ? (float)xyz[2 * (size_t)npad + i] : 0.0f;
 }
 
-// Z = in1 W1 (+ in2 W2) + b; Hout = relu(Z).  in1: [nv][K1], in2: [nv][H].
+// Z = in1 W1 (+ in2 W2) + b (not stored when Z is null); Hout = relu(Z).
+// in1: [nv][K1], in2: [nv][H].
 // A block takes tiles of DR rows: it stages the tile's inputs in LDS and each
 // thread forms 4 rows x 4 outputs, every output's fma chain in the order
 // bias, in1 columns, in2 columns.
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(256) void k_dense(int nv, const float* __restrict__
             const int row = 4 * rg + i;
             if (row >= nr) break;
             const size_t o = (size_t)(r0 + row) * H + 4 * cg;
-            *reinterpret_cast<f4*>(Z + o) = acc[i];
+            if (Z) *reinterpret_cast<f4*>(Z + o) = acc[i];  // kept for the backward pass
             const f4 h = {fmaxf(acc[i].x, 0.0f), fmaxf(acc[i].y, 0.0f), fmaxf(acc[i].z, 0.0f),
                           fmaxf(acc[i].w, 0.0f)};
             *reinterpret_cast<f4*>(Hout + o) = h;
@@ -514,17 +515,18 @@ void GuessNet::features(hipStream_t st, const double* p1, const double* p2, cons
     MP_HIP(hipGetLastError());
 }
 
-void GuessNet::forward(hipStream_t st) {
+void GuessNet::forward(hipStream_t st, bool keep_pre) {
     const uint16_t* P = theta_bf.p;
     const int g = std::min(DENSE_GRID, (nv + DR - 1) / DR);
     hipLaunchKernelGGL((k_dense<F, false>), dim3(g), dim3(256), 0, st, nv, X.p, nullptr,
-                       P + GNN_W0, nullptr, P + GNN_B0, Z[0].p, Hh[0].p);
+                       P + GNN_W0, nullptr, P + GNN_B0, keep_pre ? Z[0].p : nullptr, Hh[0].p);
     for (int l = 1; l <= GNN_L; ++l) {
         const int o = GNN_LAYER + (l - 1) * GNN_LAYER_SIZE;
         hipLaunchKernelGGL(k_aggregate, dim3((nv + 3) / 4), dim3(256), 0, st, sv, a32, Hh[l - 1].p,
                            Mm[l].p, 0);
         hipLaunchKernelGGL((k_dense<H, true>), dim3(g), dim3(256), 0, st, nv, Hh[l - 1].p, Mm[l].p,
-                           P + o, P + o + H * H, P + o + 2 * H * H, Z[l].p, Hh[l].p);
+                           P + o, P + o + H * H, P + o + 2 * H * H, keep_pre ? Z[l].p : nullptr,
+                           Hh[l].p);
     }
     hipLaunchKernelGGL(k_output, dim3((nv + 3) / 4), dim3(256), 0, st, nv, Hh[GNN_L].p, P + GNN_WO,
                        P + GNN_BO, out.p);
