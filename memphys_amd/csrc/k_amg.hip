@@ -102,8 +102,23 @@ __global__ __launch_bounds__(BLOCK) void k_amg_op(MatView m, const float* __rest
         i = lb * RPB + (int)(threadIdx.x / G);
         const int sub = threadIdx.x % G;
         if (i < m.n) {
+            // each lane sums entries sub, sub + G, ... of the row in order; the
+            // columns and values of four of them are loaded before their gathers
             const int t1 = m.ptr[i + 1];
-            for (int t = m.ptr[i] + sub; t < t1; t += G)
+            int t = m.ptr[i] + sub;
+            for (; t + 3 * G < t1; t += 4 * G) {
+                const int c0 = m.col[t], c1 = m.col[t + G], c2 = m.col[t + 2 * G],
+                          c3 = m.col[t + 3 * G];
+                const uint16_t v0 = m.val[t], v1 = m.val[t + G], v2 = m.val[t + 2 * G],
+                               v3 = m.val[t + 3 * G];
+                const float x0 = gathered<OP>(c0, b, xin, dinv), x1 = gathered<OP>(c1, b, xin, dinv),
+                            x2 = gathered<OP>(c2, b, xin, dinv), x3 = gathered<OP>(c3, b, xin, dinv);
+                acc = fmaf(bf2f(v0), x0, acc);
+                acc = fmaf(bf2f(v1), x1, acc);
+                acc = fmaf(bf2f(v2), x2, acc);
+                acc = fmaf(bf2f(v3), x3, acc);
+            }
+            for (; t < t1; t += G)
                 acc = fmaf(bf2f(m.val[t]), gathered<OP>(m.col[t], b, xin, dinv), acc);
         }
 #pragma unroll
