@@ -63,8 +63,10 @@ struct AmgDevice {
     // at once when done_flag (may be null) is non-zero.
     // prescaled: the level-0 smoother input w D^-1 in (FP32) is already in
     // prescaled_input() (written by the Krylov update kernels).
+    // out32 (may be null): the node rows of out again in FP32 (they are FP32
+    // values; the border unknown passes through only in out).
     void apply(hipStream_t s, const double* in, double* out, int bordered, const int* done_flag,
-               bool prescaled = false);
+               bool prescaled = false, float* out32 = nullptr);
     float* prescaled_input() { return lv.size() > 1 ? lv[0].x.p : nullptr; }
     const float* level0_dinv() const { return lv.empty() ? nullptr : lv[0].dinv.p; }
 };

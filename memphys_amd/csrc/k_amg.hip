@@ -79,7 +79,7 @@ __global__ __launch_bounds__(BLOCK) void k_amg_op(MatView m, const float* __rest
                                                   float* __restrict__ res, TO* out,
                                                   const double* __restrict__ border_in,
                                                   double* __restrict__ border_out,
-                                                  const int* done) {
+                                                  const int* done, float* __restrict__ out32) {
     if (stopped(done)) return;
     float acc = 0.0f;
     int i;
@@ -134,7 +134,9 @@ __global__ __launch_bounds__(BLOCK) void k_amg_op(MatView m, const float* __rest
     } else if constexpr (OP == OP_SPMV) {
         out[i] = (TO)(base ? base[i] + acc : acc);
     } else {
-        out[i] = (TO)(xin[i] + dinv[i] * ((float)b[i] - acc));
+        const float y = xin[i] + dinv[i] * ((float)b[i] - acc);
+        out[i] = (TO)y;
+        if (out32) out32[i] = y;  // the same values in FP32 (gathered by the Krylov products)
     }
 }
 
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(BLOCK) void k_amg_dense(int nc, const float* __rest
                                                      TO* __restrict__ y,
                                                      const double* __restrict__ border_in,
                                                      double* __restrict__ border_out,
-                                                     const int* done) {
+                                                     const int* done, float* __restrict__ y32) {
     if (stopped(done)) return;
     if (border_out && blockIdx.x == 0 && threadIdx.x == 0) *border_out = *border_in;
     const int row = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)));
@@ -154,13 +156,16 @@ __global__ __launch_bounds__(BLOCK) void k_amg_dense(int nc, const float* __rest
     float acc = 0.0f;
     for (int j = lane; j < nc; j += 64) acc = fmaf(G[(size_t)row * nc + j], (float)b[j], acc);
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0) y[row] = (TO)acc;
+    if (lane == 0) {
+        y[row] = (TO)acc;
+        if (y32) y32[row] = acc;
+    }
 }
 
 template <int G, int OP, typename TB, typename TO>
 void launch_g(hipStream_t st, const AmgMat& M, const float* dinv, const TB* b, const float* xin,
               const float* base, float* xout, float* res, TO* out, const double* bin,
-              double* bout, const int* done) {
+              double* bout, const int* done, float* out32) {
     const MatView v{M.n, M.nslices, M.ptr, M.col, M.val, M.col16, M.cbase};
     int nb;
     if constexpr (G == 0)
@@ -168,20 +173,20 @@ void launch_g(hipStream_t st, const AmgMat& M, const float* dinv, const TB* b, c
     else
         nb = (M.n + BLOCK / G - 1) / (BLOCK / G);
     hipLaunchKernelGGL((k_amg_op<G, OP, TB, TO>), dim3(grid_xcd(nb)), dim3(BLOCK), 0, st, v, dinv,
-                       b, xin, base, xout, res, out, bin, bout, done);
+                       b, xin, base, xout, res, out, bin, bout, done, out32);
 }
 
 template <int OP, typename TB, typename TO>
 void launch_op(hipStream_t st, const AmgMat& M, const float* dinv, const TB* b, const float* xin,
                const float* base, float* xout, float* res, TO* out, const double* bin = nullptr,
-               double* bout = nullptr, const int* done = nullptr) {
+               double* bout = nullptr, const int* done = nullptr, float* out32 = nullptr) {
     switch (M.g) {
-        case 0: launch_g<0, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done); break;
-        case 4: launch_g<4, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done); break;
-        case 8: launch_g<8, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done); break;
-        case 16: launch_g<16, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done); break;
-        case 32: launch_g<32, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done); break;
-        default: launch_g<64, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done); break;
+        case 0: launch_g<0, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done, out32); break;
+        case 4: launch_g<4, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done, out32); break;
+        case 8: launch_g<8, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done, out32); break;
+        case 16: launch_g<16, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done, out32); break;
+        case 32: launch_g<32, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done, out32); break;
+        default: launch_g<64, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done, out32); break;
     }
 }
 
@@ -450,14 +455,14 @@ void AmgDevice::retune(hipStream_t st) {
 }
 
 void AmgDevice::apply(hipStream_t st, const double* in, double* out, int bordered,
-                      const int* done, bool prescaled) {
+                      const int* done, bool prescaled, float* out32) {
     const int L = (int)lv.size();
     const int nv = lv[0].n;
     const double* bin = bordered ? in + nv : nullptr;
     double* bout = bordered ? out + nv : nullptr;
     if (L == 1) {
         hipLaunchKernelGGL((k_amg_dense<double, double>), dim3((nc + 3) / 4), dim3(BLOCK), 0, st,
-                           nc, cinv.p, in, out, bin, bout, done);
+                           nc, cinv.p, in, out, bin, bout, done, out32);
         MP_HIP(hipGetLastError());
         return;
     }
@@ -476,7 +481,7 @@ void AmgDevice::apply(hipStream_t st, const double* in, double* out, int bordere
         if (l + 2 == L)
             hipLaunchKernelGGL((k_amg_dense<float, float>), dim3((nc + 3) / 4), dim3(BLOCK), 0, st,
                                nc, cinv.p, c.b.p, c.y.p, (const double*)nullptr, (double*)nullptr,
-                               done);
+                               done, (float*)nullptr);
         else
             launch_op<OP_PRE, float, float>(st, c.A, c.dinv.p, c.b.p, nullptr, nullptr, c.x.p,
                                             c.res.p, (float*)nullptr, nullptr, nullptr, done);
@@ -489,7 +494,7 @@ void AmgDevice::apply(hipStream_t st, const double* in, double* out, int bordere
                                          nullptr, nullptr, f.x.p, nullptr, nullptr, done);
         if (l == 0)
             launch_op<OP_POST, double, double>(st, f.A, f.dinv.p, in, f.x.p, nullptr, nullptr,
-                                               nullptr, out, nullptr, nullptr, done);
+                                               nullptr, out, nullptr, nullptr, done, out32);
         else
             launch_op<OP_POST, float, float>(st, f.A, f.dinv.p, f.b.p, f.x.p, nullptr, nullptr,
                                              nullptr, f.y.p, nullptr, nullptr, done);

@@ -122,18 +122,19 @@ __device__ void loop_top(KScalars* sc, double rr, double r0r) {
 // width is uniform across the wavefront, so the neighbour loop runs in
 // lockstep.  Padding slots hold a zero value and a valid column, which leaves
 // the fma chain unchanged.
-template <typename TM, typename T>
+template <typename T, typename TM, typename TX>
 __device__ __forceinline__ T sell_dot(const SellView& sv, const TM* __restrict__ a,
-                                      const T* __restrict__ x, int s, int lane) {
+                                      const TX* __restrict__ x, int s, int lane) {
     // FP64 values (stencil order, 32-bit columns): two groups per iteration
     // keep the register count at eight waves per SIMD, and decoding 16-bit
     // columns there would make the compiler serialise the gathers.  FP32
     // values (column-sorted order, whose gathers are cheaper): four groups
-    // and the 16-bit columns of the view.
+    // and the 16-bit columns of the view.  x may be an FP32 copy of a vector
+    // whose values are FP32 numbers (the V-cycle's output): same sums.
     constexpr bool wide = sizeof(TM) == 8;
     return sell4_accumulate<(wide ? 2 : 4), T>(
         sv.sptr, sv.col, wide ? nullptr : sv.col16, sv.cbase, a, s, lane,
-        [](TM v) { return (T)v; }, [x](int c) { return x[c]; });
+        [](TM v) { return (T)v; }, [x](int c) { return (T)x[c]; });
 }
 
 // Slice of this wavefront under the XCD-aware numbering (lb: logical block).
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(BLOCK) void k_residual_init(
     const int s = wave_slice(lb);
     double vals[3] = {0.0, 0.0, 0.0};  // r.r, rhs.rhs, sum D^-1 x
     if (s < sv.nslices) {
-        T ax = sell_dot(sv, a, x, s, threadIdx.x & 63);
+        T ax = sell_dot<T>(sv, a, x, s, threadIdx.x & 63);
         if (i < iend) {
             if (S.border) ax = ax + x[iend];
             const T ri = rhs[i] - ax;
@@ -256,12 +257,14 @@ __global__ __launch_bounds__(BLOCK) void k_update_p(SegView sg, const T* __restr
 }
 
 // v = A~ y (y = p, or M^-1 p with the multigrid preconditioner); partial sums
-// of r0.v and of D^-1 y (border row)
-template <typename TM, typename T>
+// of r0.v and of D^-1 y (border row).  yg: y's node rows, gathered (y itself
+// or its FP32 copy); y: the border unknown.
+template <typename TM, typename T, typename TX>
 __global__ __launch_bounds__(BLOCK) void k_spmv_alpha(SellView sv, SegView sg,
                                                       const TM* __restrict__ a,
                                                       const T* __restrict__ dinv,
-                                                      const T* __restrict__ y, T* __restrict__ v,
+                                                      const T* __restrict__ y,
+                                                      const TX* __restrict__ yg, T* __restrict__ v,
                                                       const T* __restrict__ r0,
                                                       double* __restrict__ partial,
                                                       const KScalars* sc) {
@@ -276,12 +279,12 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_alpha(SellView sv, SegView sg,
     const int s = wave_slice(lb);
     double vals[2] = {0.0, 0.0};
     if (s < sv.nslices) {
-        T yv = sell_dot(sv, a, y, s, threadIdx.x & 63);
+        T yv = sell_dot<T>(sv, a, yg, s, threadIdx.x & 63);
         if (i < iend) {
             if (S.border) yv = yv + y[iend];
             v[i] = yv;
             vals[0] = (double)r0[i] * (double)yv;
-            if (S.border) vals[1] = (double)dinv[i] * (double)y[i];
+            if (S.border) vals[1] = (double)dinv[i] * (double)yg[i];
         }
     }
     block_store<2>(vals, partial, lb, nbk);
@@ -326,11 +329,12 @@ __global__ __launch_bounds__(BLOCK) void k_update_s(SegView sg, const T* __restr
 
 // t = A~ z (z = s, or M^-1 s with the multigrid preconditioner); partial sums
 // of t.s, t.t and D^-1 z
-template <typename TM, typename T>
+template <typename TM, typename T, typename TX>
 __global__ __launch_bounds__(BLOCK) void k_spmv_omega(SellView sv, SegView sg,
                                                       const TM* __restrict__ a,
                                                       const T* __restrict__ dinv,
                                                       const T* __restrict__ z,
+                                                      const TX* __restrict__ zg,
                                                       const T* __restrict__ s, T* __restrict__ t,
                                                       double* __restrict__ partial,
                                                       const KScalars* sc) {
@@ -345,13 +349,13 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_omega(SellView sv, SegView sg,
     const int sl = wave_slice(lb);
     double vals[3] = {0.0, 0.0, 0.0};
     if (sl < sv.nslices) {
-        T y = sell_dot(sv, a, z, sl, threadIdx.x & 63);
+        T y = sell_dot<T>(sv, a, zg, sl, threadIdx.x & 63);
         if (i < iend) {
             if (S.border) y = y + z[iend];
             t[i] = y;
             vals[0] = (double)y * (double)s[i];
             vals[1] = (double)y * (double)y;
-            if (S.border) vals[2] = (double)dinv[i] * (double)z[i];
+            if (S.border) vals[2] = (double)dinv[i] * (double)zg[i];
         }
     }
     block_store<3>(vals, partial, lb, nbk);
@@ -430,7 +434,7 @@ __global__ __launch_bounds__(BLOCK) void k_residual64(SellView sv, SegView sg,
     const int s = wave_slice(lb);
     double vals[3] = {0.0, 0.0, 0.0};
     if (s < sv.nslices) {
-        double ax = sell_dot(sv, a, x, s, threadIdx.x & 63);
+        double ax = sell_dot<double>(sv, a, x, s, threadIdx.x & 63);
         if (i < iend) {
             if (S.border) ax = ax + x[iend];
             const double ri = b[i] - ax;
@@ -707,6 +711,10 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
             b->alloc(vlen);
             MP_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * vlen, st));
         }
+        for (DevBuf<float>* b : {&y32, &z32}) {
+            b->alloc(vlen);
+            MP_HIP(hipMemsetAsync(b->p, 0, sizeof(float) * vlen, st));
+        }
     }
     for (DevBuf<double>* b : {&xt, &r, &r0, &p, &v, &s, &t, &tmp, &e64, &rhs64}) {
         b->alloc(vlen);
@@ -747,20 +755,23 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
 double PpeSolver::spmv_bytes(int fp32) const {
     // per entry: value (8 B, or 4 B for the FP32 operator) + column (4 B; the
     // FP32 operator: 2 B in slices with 16-bit columns, plus their window
-    // bases); per row: gathered x once, y written, r0 read (FP64 vectors),
-    // length, D^-1 for the border row
+    // bases); per row: gathered x once (FP32 with the V-cycle, else FP64),
+    // v written, r0 read (FP64), length, D^-1 for the border row
     const double vb = fp32 ? 4.0 : 8.0;
     const double n16 = fp32 ? (double)nnz16 : 0.0;
+    const double xb = amg ? 4.0 : 8.0;  // the V-cycle's output is gathered in FP32
     return (double)nnz * vb + 2.0 * n16 + 4.0 * ((double)nnz - n16) +
            (fp32 ? (double)nslices16 * 4.0 * CW_COUNT : 0.0) +
-           (double)nv * (8.0 * (bordered ? 4.0 : 3.0) + 4.0);
+           (double)nv * (xb + 8.0 * (bordered ? 3.0 : 2.0) + 4.0);
 }
 
 // y = M^-1 p with the multigrid V-cycle; without it the direction itself.
 // The preceding update kernel has written the V-cycle's scaled FP32 input.
-const double* PpeSolver::precondition(hipStream_t st, const double* in, double* out) {
+const double* PpeSolver::precondition(hipStream_t st, const double* in, double* out,
+                                      float* out32) {
     if (!amg) return in;
-    amg->apply(st, in, out, segs[0].border, &sc.p->done, amg->prescaled_input() != nullptr);
+    amg->apply(st, in, out, segs[0].border, &sc.p->done, amg->prescaled_input() != nullptr,
+               out32);
     return out;
 }
 
@@ -775,19 +786,30 @@ void PpeSolver::launch_chunk(hipStream_t st, int chunk, const TM* a, const T* di
     for (int c = 0; c < chunk; ++c) {
         hipLaunchKernelGGL(k_update_p<T>, dim3(nbk), dim3(BLOCK), 0, st, sg, rr, pp, vv, sc.p, d0,
                            xs);
-        const T* yin = precondition(st, pp, yv.p);
+        // with the V-cycle, the products gather its FP32 output (the same values)
+        const T* yin = precondition(st, pp, yv.p, y32.p);
         spmv_timer.begin(st);
-        hipLaunchKernelGGL((k_spmv_alpha<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, view_of<TM>(), sg,
-                           a, di, yin, vv, rr0, partial.p, sc.p);
+        if (amg)
+            hipLaunchKernelGGL((k_spmv_alpha<TM, T, float>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0,
+                               st, view_of<TM>(), sg, a, di, yin, (const float*)y32.p, vv, rr0,
+                               partial.p, sc.p);
+        else
+            hipLaunchKernelGGL((k_spmv_alpha<TM, T, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
+                               view_of<TM>(), sg, a, di, yin, yin, vv, rr0, partial.p, sc.p);
         spmv_timer.end(st);
         hipLaunchKernelGGL(k_finish_alpha<T>, dim3(nseg), dim3(FIN), 0, st, sg, vv, rr0, partial.p,
                            sc.p);
         hipLaunchKernelGGL(k_update_s<T>, dim3(nbk), dim3(BLOCK), 0, st, sg, rr, vv, ss, sc.p, d0,
                            xs);
-        const T* zin = precondition(st, ss, zv.p);
+        const T* zin = precondition(st, ss, zv.p, z32.p);
         spmv_timer.begin(st);
-        hipLaunchKernelGGL((k_spmv_omega<TM, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, view_of<TM>(), sg,
-                           a, di, zin, ss, tt, partial.p, sc.p);
+        if (amg)
+            hipLaunchKernelGGL((k_spmv_omega<TM, T, float>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0,
+                               st, view_of<TM>(), sg, a, di, zin, (const float*)z32.p, ss, tt,
+                               partial.p, sc.p);
+        else
+            hipLaunchKernelGGL((k_spmv_omega<TM, T, T>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
+                               view_of<TM>(), sg, a, di, zin, zin, ss, tt, partial.p, sc.p);
         spmv_timer.end(st);
         hipLaunchKernelGGL(k_finish_omega<T>, dim3(nseg), dim3(FIN), 0, st, sg, tt, ss, partial.p,
                            sc.p);

@@ -67,6 +67,7 @@ struct PpeSolver {
     int nslices16 = 0;
     DevBuf<double> xt, r, r0, p, v, s, t, tmp, e64, rhs64;
     DevBuf<double> yv, zv;            // preconditioned directions (multigrid)
+    DevBuf<float> y32, z32;           // their node rows in FP32 (the values are FP32)
     DevBuf<double> jac_dinv;          // D^-1 when the operator above is unscaled
     DevBuf<float> jac_a32;            // A D^-1 in FP32 when the operator above is unscaled
     const double* diag_inv = nullptr; // D^-1 (the learned guess's features)
@@ -117,7 +118,7 @@ struct PpeSolver {
                       std::vector<int>& done, std::vector<int>& iters, std::vector<int>& restarts);
     void residual64(hipStream_t s, const double* b_dev, std::vector<double>& rr,
                     std::vector<double>& bb);
-    const double* precondition(hipStream_t s, const double* in, double* out);
+    const double* precondition(hipStream_t s, const double* in, double* out, float* out32);
     template <typename TM, typename T>
     void launch_chunk(hipStream_t s, int chunk, const TM* a, const T* di, T* x, T* rr, T* rr0,
                       T* pp, T* vv, T* ss, T* tt);
