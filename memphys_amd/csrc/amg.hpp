@@ -3,6 +3,8 @@
 // values (host set-up in amg_build.hpp).
 #pragma once
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "amg_build.hpp"
 #include "mp_internal.hpp"
@@ -38,7 +40,11 @@ struct AmgDevice {
     double omega = 0.7;  // Jacobi smoothing weight
     std::vector<AmgLevelDev> lv;
     int nc = 0;
-    DevBuf<float> cinv;  // dense inverse on the coarsest level
+    DevBuf<float> cinv;  // dense inverse(s) on the coarsest level
+    bool blocked = false;            // a batch: block-diagonal coarse inverses
+    DevBuf<int> cfirst, csize;       // per coarsest row: its block's first row and size
+    DevBuf<long> cdata;              // per coarsest row: offset of its row of the inverse
+    DevBuf<int> border_rows;
     std::vector<int> rows;
     std::vector<long> nnz;
     std::string layout;  // chosen layout per matrix, for reports
@@ -49,10 +55,16 @@ struct AmgDevice {
     // Builds the hierarchy from the finest operator, given in SELL-64x4 slots
     // (device order) by the solver: FP64 values val64_dev in the slots of sv
     // for the host set-up; the finest level's products run on sv_apply (owned
-    // by the solver, the same rows, possibly another order of each row's
+    // by the solver, sv_apply.nv rows, possibly another order of each row's
     // slots) with a BF16 copy of val32_dev, given in sv_apply's slots.
+    // blocks: (first device row, rows) of each problem.  A batch gets one
+    // hierarchy per problem, all of one depth, stored block-diagonally, so
+    // every level's products run over all problems at once.
     void setup(hipStream_t s, const SellView& sv, const double* val64_dev,
-               const SellView& sv_apply, const float* val32_dev, const mp_amg::Options& opt);
+               const SellView& sv_apply, const float* val32_dev,
+               const std::vector<std::pair<int, int>>& blocks, const mp_amg::Options& opt);
+    // device rows of a batch's border unknowns (copied through the V-cycle)
+    void set_border_rows(hipStream_t s, const std::vector<int>& r);
     // Layout of every hierarchy matrix chosen by timing each candidate (SELL-64x4,
     // CSR with 4..64 lanes per row) instead of the set-up's rule of thumb.  The
     // layouts sum a row in different orders, so the V-cycle then changes at the

@@ -798,9 +798,6 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
                     return fail(c, MP_ERR_ARG,
                                 "mp_fs_create: full Neumann velocity BC is not permitted");
             }
-        if (prm->preconditioner == MP_PRECOND_AMG && c->nprob > 1)
-            return fail(c, MP_ERR_ARG,
-                        "mp_fs_create: the multigrid preconditioner takes a single problem");
         if (prm->rho < 0 || prm->mu < 0)
             return fail(c, MP_ERR_ARG, "mp_fs_create: rho and mu must be set (>= 0)");
         if (prm->temporal_order != 1 && prm->temporal_order != 2)

@@ -18,6 +18,7 @@
 
 #include <chrono>
 #include <cstring>
+#include <type_traits>
 
 namespace mp {
 
@@ -160,6 +161,38 @@ __global__ __launch_bounds__(BLOCK) void k_amg_dense(int nc, const float* __rest
         y[row] = (TO)acc;
         if (y32) y32[row] = acc;
     }
+}
+
+// y = blockdiag(G_q) b on the coarsest level of a batch: one wavefront per
+// row; row r's block starts at row first[r], has size[r] rows, and its row of
+// the block's dense inverse starts at G + data[r].
+template <typename TB, typename TO>
+__global__ __launch_bounds__(BLOCK) void k_amg_blockdense(int nc, const float* __restrict__ G,
+                                                          const int* __restrict__ first,
+                                                          const int* __restrict__ size,
+                                                          const long* __restrict__ data,
+                                                          const TB* __restrict__ b,
+                                                          TO* __restrict__ y,
+                                                          float* __restrict__ y32) {
+    const int row = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)));
+    if (row >= nc) return;
+    const int lane = threadIdx.x & 63;
+    const int f = first[row], m = size[row];
+    const float* g = G + data[row];
+    float acc = 0.0f;
+    for (int j = lane; j < m; j += 64) acc = fmaf(g[j], (float)b[f + j], acc);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) {
+        y[row] = (TO)acc;
+        if (y32) y32[row] = acc;
+    }
+}
+
+// out[border row] = in[border row] for every bordered problem of a batch
+__global__ void k_border_copy(int n, const int* __restrict__ rows, const double* __restrict__ in,
+                              double* __restrict__ out) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t < n) out[rows[t]] = in[rows[t]];
 }
 
 template <int G, int OP, typename TB, typename TO>
@@ -327,14 +360,40 @@ void alloc_zero(DevBuf<float>& b, size_t n, hipStream_t st) {
     MP_HIP(hipMemsetAsync(b.p, 0, sizeof(float) * b.n, st));
 }
 
+// Block-diagonal union of per-problem matrices: block q's rows start at
+// roff[q] and its columns at coff[q].
+mp_amg::Csr block_diag(const std::vector<const mp_amg::Csr*>& m, const std::vector<int>& roff,
+                       const std::vector<int>& coff, int nrows, int ncols) {
+    mp_amg::Csr out;
+    out.nrows = nrows;
+    out.ncols = ncols;
+    std::vector<int> len(nrows, 0);
+    for (size_t q = 0; q < m.size(); ++q)
+        for (int i = 0; i < m[q]->nrows; ++i) len[roff[q] + i] = m[q]->ptr[i + 1] - m[q]->ptr[i];
+    out.ptr.assign(nrows + 1, 0);
+    for (int i = 0; i < nrows; ++i) out.ptr[i + 1] = out.ptr[i] + len[i];
+    out.col.resize(out.ptr[nrows]);
+    out.val.resize(out.ptr[nrows]);
+    for (size_t q = 0; q < m.size(); ++q)
+        for (int i = 0; i < m[q]->nrows; ++i) {
+            int t = out.ptr[roff[q] + i];
+            for (int e = m[q]->ptr[i]; e < m[q]->ptr[i + 1]; ++e, ++t) {
+                out.col[t] = m[q]->col[e] + coff[q];
+                out.val[t] = m[q]->val[e];
+            }
+        }
+    return out;
+}
+
 }  // namespace
 
 void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_dev,
                       const SellView& sv_apply, const float* val32_dev,
-                      const mp_amg::Options& opt) {
+                      const std::vector<std::pair<int, int>>& blocks, const mp_amg::Options& opt) {
     const auto t0 = std::chrono::steady_clock::now();
-    // finest operator as CSR in device order (padding slots dropped)
-    const int nv = sv.nv;
+    if (blocks.empty()) throw std::runtime_error("amg: no problems");
+    // finest operator of each problem as CSR in device order, local indices
+    // (padding slots dropped)
     std::vector<int> hsptr(sv.nslices + 1), hlen((size_t)sv.nslices * SLICE);
     MP_HIP(hipMemcpyAsync(hsptr.data(), sv.sptr, sizeof(int) * hsptr.size(), hipMemcpyDeviceToHost, st));
     MP_HIP(hipMemcpyAsync(hlen.data(), sv.len, sizeof(int) * hlen.size(), hipMemcpyDeviceToHost, st));
@@ -345,73 +404,163 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
     MP_HIP(hipMemcpyAsync(hcol.data(), sv.col, sizeof(int) * slots, hipMemcpyDeviceToHost, st));
     MP_HIP(hipMemcpyAsync(hval.data(), val64_dev, sizeof(double) * slots, hipMemcpyDeviceToHost, st));
     MP_HIP(hipStreamSynchronize(st));
-    mp_amg::Csr A0;
-    A0.nrows = A0.ncols = nv;
-    A0.ptr.assign(nv + 1, 0);
-    for (int i = 0; i < nv; ++i) A0.ptr[i + 1] = A0.ptr[i] + hlen[i];
-    A0.col.resize(A0.ptr[nv]);
-    A0.val.resize(A0.ptr[nv]);
-    for (int i = 0; i < nv; ++i) {
-        const int base = sell_base(hsptr.data(), i);
-        for (int j = 0; j < hlen[i]; ++j) {
-            const int slot = base + sell_off(j);
-            A0.col[A0.ptr[i] + j] = hcol[slot];
-            A0.val[A0.ptr[i] + j] = hval[slot];
+    const int nb = (int)blocks.size();
+    std::vector<mp_amg::Csr> A0s(nb);
+    for (int q = 0; q < nb; ++q) {
+        const int r0 = blocks[q].first, n = blocks[q].second;
+        mp_amg::Csr& A0 = A0s[q];
+        A0.nrows = A0.ncols = n;
+        A0.ptr.assign(n + 1, 0);
+        for (int i = 0; i < n; ++i) A0.ptr[i + 1] = A0.ptr[i] + hlen[r0 + i];
+        A0.col.resize(A0.ptr[n]);
+        A0.val.resize(A0.ptr[n]);
+        for (int i = 0; i < n; ++i) {
+            const int base = sell_base(hsptr.data(), r0 + i);
+            for (int j = 0; j < hlen[r0 + i]; ++j) {
+                const int slot = base + sell_off(j);
+                const int c = hcol[slot] - r0;
+                if (c < 0 || c >= n) throw std::runtime_error("amg: a stencil leaves its problem");
+                A0.col[A0.ptr[i] + j] = c;
+                A0.val[A0.ptr[i] + j] = hval[slot];
+            }
         }
     }
     { std::vector<int>().swap(hcol); std::vector<double>().swap(hval); }
-    mp_amg::Hierarchy h = mp_amg::build(A0, opt);
-    const int L = h.levels();
+    // one hierarchy per problem; in a batch all of the same depth (shallower
+    // ones coarsen further, to a smaller dense coarse solve).  At least two
+    // levels: the bordered dense inverse alone maps onto mean-free vectors,
+    // which would leave the bordered system's constraint row of the
+    // preconditioned operator zero; smoothing on the finest level keeps it.
+    std::vector<mp_amg::Hierarchy> hs(nb);
+    int L = 2;
+    for (int q = 0; q < nb; ++q) {
+        hs[q] = mp_amg::build(A0s[q], opt);
+        L = std::max(L, hs[q].levels());
+    }
+    for (int q = 0; q < nb; ++q)
+        if (hs[q].levels() < L) {
+            mp_amg::Options o = opt;
+            o.coarse_max = 0;
+            o.max_levels = L;
+            hs[q] = mp_amg::build(A0s[q], o);
+            if (hs[q].levels() != L)
+                throw std::runtime_error("amg: problem " + std::to_string(q) +
+                                         " cannot be coarsened to " + std::to_string(L) +
+                                         " levels");
+        }
+    // row offsets per level: level 0 is the device rows, levels >= 1 stack
+    // the problems' rows in problem order
+    const int n0 = sv_apply.nv;
+    std::vector<std::vector<int>> off(L, std::vector<int>(nb, 0));
+    std::vector<int> ntot(L, 0);
+    for (int q = 0; q < nb; ++q) off[0][q] = blocks[q].first;
+    ntot[0] = n0;
+    for (int l = 1; l < L; ++l)
+        for (int q = 0; q < nb; ++q) {
+            off[l][q] = ntot[l];
+            ntot[l] += hs[q].A[l].nrows;
+        }
+    // host copies of the block-diagonal hierarchy (A[0] stays empty)
+    std::vector<mp_amg::Csr> Ab(L), Pb(L > 1 ? L - 1 : 0), Rb(L > 1 ? L - 1 : 0);
+    for (int l = 0; l < L; ++l) {
+        std::vector<const mp_amg::Csr*> ma(nb), mp_(nb), mr(nb);
+        for (int q = 0; q < nb; ++q) {
+            ma[q] = l == 0 ? &A0s[q] : &hs[q].A[l];
+            if (l + 1 < L) {
+                mp_[q] = &hs[q].P[l];
+                mr[q] = &hs[q].R[l];
+            }
+        }
+        if (l > 0) Ab[l] = block_diag(ma, off[l], off[l], ntot[l], ntot[l]);
+        if (l + 1 < L) {
+            Pb[l] = block_diag(mp_, off[l], off[l + 1], ntot[l], ntot[l + 1]);
+            Rb[l] = block_diag(mr, off[l + 1], off[l], ntot[l + 1], ntot[l]);
+        }
+    }
     lv.clear();
     lv.resize(L);
     rows.assign(L, 0);
     nnz.assign(L, 0);
     layout.clear();
     for (int l = 0; l < L; ++l) {
-            const mp_amg::Csr& A = (l == 0) ? A0 : h.A[l];
-            AmgLevelDev& d = lv[l];
-            d.n = A.nrows;
-            rows[l] = A.nrows;
-            nnz[l] = A.nnz();
-            put(d.dinv, scaled_diag_inverse(A, omega), st);
-            if (l == 0) {
-                d.A.n = nv;
-                d.A.nslices = sv_apply.nslices;
-                d.A.g = 0;
-                d.A.nnz = A.nnz();
-                d.A.ptr = sv_apply.sptr;
-                d.A.col = sv_apply.col;
-                d.A.col16 = sv_apply.col16;
-                d.A.cbase = sv_apply.cbase;
-                // BF16 copy of the solver's FP32 slots (padding slots stay 0)
-                d.A.val_b.alloc((size_t)slots);
-                hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((slots + BLOCK - 1) / BLOCK)),
-                                   dim3(BLOCK), 0, st, (long)slots, val32_dev, d.A.val_b.p);
-                MP_HIP(hipGetLastError());
-                d.A.val = d.A.val_b.p;
-            } else if (l + 1 < L) {
-                upload_mat(d.A, A, rule_layout(A), st);
+        AmgLevelDev& d = lv[l];
+        d.n = ntot[l];
+        long nz = 0;
+        if (l == 0) {
+            int nreal = 0;
+            for (int q = 0; q < nb; ++q) {
+                nz += A0s[q].nnz();
+                nreal += A0s[q].nrows;
             }
-            const size_t vl = (size_t)A.nrows + 64;
-            alloc_zero(d.x, vl, st);
-            alloc_zero(d.res, vl, st);
-            if (l > 0) {
-                alloc_zero(d.b, vl, st);
-                alloc_zero(d.y, vl, st);
-            }
-            if (l + 1 < L) {
-                upload_mat(d.P, h.P[l], rule_layout(h.P[l]), st);
-                upload_mat(d.R, h.R[l], rule_layout(h.R[l]), st);
-            }
+            rows[0] = nreal;
+        } else {
+            nz = Ab[l].nnz();
+            rows[l] = ntot[l];
+        }
+        nnz[l] = nz;
+        std::vector<float> dv(ntot[l], 0.0f);  // level 0: rows outside the problems get 0
+        for (int q = 0; q < nb; ++q) {
+            const std::vector<float> dq =
+                scaled_diag_inverse(l == 0 ? A0s[q] : hs[q].A[l], omega);
+            std::copy(dq.begin(), dq.end(), dv.begin() + off[l][q]);
+        }
+        put(d.dinv, dv, st);
+        if (l == 0) {
+            d.A.n = n0;
+            d.A.nslices = sv_apply.nslices;
+            d.A.g = 0;
+            d.A.nnz = nz;
+            d.A.ptr = sv_apply.sptr;
+            d.A.col = sv_apply.col;
+            d.A.col16 = sv_apply.col16;
+            d.A.cbase = sv_apply.cbase;
+            // BF16 copy of the solver's FP32 slots (padding slots stay 0)
+            d.A.val_b.alloc((size_t)slots);
+            hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((slots + BLOCK - 1) / BLOCK)),
+                               dim3(BLOCK), 0, st, (long)slots, val32_dev, d.A.val_b.p);
+            MP_HIP(hipGetLastError());
+            d.A.val = d.A.val_b.p;
+        } else if (l + 1 < L) {
+            upload_mat(d.A, Ab[l], rule_layout(Ab[l]), st);
+        }
+        const size_t vl = (size_t)ntot[l] + 64;
+        alloc_zero(d.x, vl, st);
+        alloc_zero(d.res, vl, st);
+        if (l > 0) {
+            alloc_zero(d.b, vl, st);
+            alloc_zero(d.y, vl, st);
+        }
+        if (l + 1 < L) {
+            upload_mat(d.P, Pb[l], rule_layout(Pb[l]), st);
+            upload_mat(d.R, Rb[l], rule_layout(Rb[l]), st);
+        }
     }
-    nc = h.nc;
-    std::vector<float> g(h.coarse_inv.size());
-    for (size_t t = 0; t < g.size(); ++t) g[t] = (float)h.coarse_inv[t];
+    // coarsest level: the dense inverses, one block per problem
+    nc = ntot[L - 1];
+    std::vector<float> g;
+    std::vector<int> hfirst(nc), hsize(nc);
+    std::vector<long> hdata(nc);
+    for (int q = 0; q < nb; ++q) {
+        const int m = hs[q].nc, first = off[L - 1][q];
+        const long d0 = (long)g.size();
+        for (double v : hs[q].coarse_inv) g.push_back((float)v);
+        for (int r = 0; r < m; ++r) {
+            hfirst[first + r] = first;
+            hsize[first + r] = m;
+            hdata[first + r] = d0 + (long)r * m;
+        }
+    }
     put(cinv, g, st);
+    blocked = nb > 1;
+    if (blocked) {
+        put(cfirst, hfirst, st);
+        put(csize, hsize, st);
+        put(cdata, hdata, st);
+    }
     MP_HIP(hipStreamSynchronize(st));
-    hA = std::move(h.A);
-    hP = std::move(h.P);
-    hR = std::move(h.R);
+    hA = std::move(Ab);
+    hP = std::move(Pb);
+    hR = std::move(Rb);
     for (int l = 0; l + 1 < L; ++l) {
         char buf[64];
         std::snprintf(buf, sizeof(buf), "A%d:%d P%d:%d R%d:%d ", l, lv[l].A.g, l, lv[l].P.g, l,
@@ -420,6 +569,12 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
     }
     setup_seconds =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void AmgDevice::set_border_rows(hipStream_t st, const std::vector<int>& r) {
+    if (r.empty()) return;
+    put(border_rows, r, st);
+    MP_HIP(hipStreamSynchronize(st));
 }
 
 void AmgDevice::retune(hipStream_t st) {
@@ -458,47 +613,61 @@ void AmgDevice::apply(hipStream_t st, const double* in, double* out, int bordere
                       const int* done, bool prescaled, float* out32) {
     const int L = (int)lv.size();
     const int nv = lv[0].n;
-    const double* bin = bordered ? in + nv : nullptr;
-    double* bout = bordered ? out + nv : nullptr;
-    if (L == 1) {
-        hipLaunchKernelGGL((k_amg_dense<double, double>), dim3((nc + 3) / 4), dim3(BLOCK), 0, st,
-                           nc, cinv.p, in, out, bin, bout, done, out32);
-        MP_HIP(hipGetLastError());
-        return;
-    }
-    // downward: level 0 from the FP64 input
-    if (prescaled)
-        launch_op<OP_PRES, double, float>(st, lv[0].A, nullptr, in, lv[0].x.p, nullptr, nullptr,
-                                          lv[0].res.p, (float*)nullptr, bin, bout, done);
-    else
-        launch_op<OP_PRE, double, float>(st, lv[0].A, lv[0].dinv.p, in, nullptr, nullptr,
-                                         lv[0].x.p, lv[0].res.p, (float*)nullptr, bin, bout, done);
-    for (int l = 0; l + 1 < L; ++l) {
-        AmgLevelDev& f = lv[l];
-        AmgLevelDev& c = lv[l + 1];
-        launch_op<OP_SPMV, float, float>(st, f.R, nullptr, (const float*)nullptr, f.res.p, nullptr,
-                                         nullptr, nullptr, c.b.p, nullptr, nullptr, done);
-        if (l + 2 == L)
-            hipLaunchKernelGGL((k_amg_dense<float, float>), dim3((nc + 3) / 4), dim3(BLOCK), 0, st,
-                               nc, cinv.p, c.b.p, c.y.p, (const double*)nullptr, (double*)nullptr,
-                               done, (float*)nullptr);
+    // one problem: its border unknown (row nv) passes through in the kernels;
+    // a batch copies every border unknown at the end
+    const double* bin = (bordered && !blocked) ? in + nv : nullptr;
+    double* bout = (bordered && !blocked) ? out + nv : nullptr;
+    auto coarse = [&](const auto* b, auto* y, float* y32, const double* cbin, double* cbout) {
+        using TB = std::remove_cv_t<std::remove_pointer_t<decltype(b)>>;
+        using TO = std::remove_pointer_t<decltype(y)>;
+        if (blocked)
+            hipLaunchKernelGGL((k_amg_blockdense<TB, TO>), dim3((nc + 3) / 4), dim3(BLOCK), 0, st,
+                               nc, cinv.p, cfirst.p, csize.p, cdata.p, b, y, y32);
         else
-            launch_op<OP_PRE, float, float>(st, c.A, c.dinv.p, c.b.p, nullptr, nullptr, c.x.p,
-                                            c.res.p, (float*)nullptr, nullptr, nullptr, done);
-    }
-    // upward
-    for (int l = L - 2; l >= 0; --l) {
-        AmgLevelDev& f = lv[l];
-        AmgLevelDev& c = lv[l + 1];
-        launch_op<OP_SPMV, float, float>(st, f.P, nullptr, (const float*)nullptr, c.y.p, f.x.p,
-                                         nullptr, nullptr, f.x.p, nullptr, nullptr, done);
-        if (l == 0)
-            launch_op<OP_POST, double, double>(st, f.A, f.dinv.p, in, f.x.p, nullptr, nullptr,
-                                               nullptr, out, nullptr, nullptr, done, out32);
+            hipLaunchKernelGGL((k_amg_dense<TB, TO>), dim3((nc + 3) / 4), dim3(BLOCK), 0, st, nc,
+                               cinv.p, b, y, cbin, cbout, done, y32);
+    };
+    {  // (setup builds at least two levels)
+        // downward: level 0 from the FP64 input
+        if (prescaled)
+            launch_op<OP_PRES, double, float>(st, lv[0].A, nullptr, in, lv[0].x.p, nullptr,
+                                              nullptr, lv[0].res.p, (float*)nullptr, bin, bout,
+                                              done);
         else
-            launch_op<OP_POST, float, float>(st, f.A, f.dinv.p, f.b.p, f.x.p, nullptr, nullptr,
-                                             nullptr, f.y.p, nullptr, nullptr, done);
+            launch_op<OP_PRE, double, float>(st, lv[0].A, lv[0].dinv.p, in, nullptr, nullptr,
+                                             lv[0].x.p, lv[0].res.p, (float*)nullptr, bin, bout,
+                                             done);
+        for (int l = 0; l + 1 < L; ++l) {
+            AmgLevelDev& f = lv[l];
+            AmgLevelDev& c = lv[l + 1];
+            launch_op<OP_SPMV, float, float>(st, f.R, nullptr, (const float*)nullptr, f.res.p,
+                                             nullptr, nullptr, nullptr, c.b.p, nullptr, nullptr,
+                                             done);
+            if (l + 2 == L) {  // (the border unknown was copied at level 0)
+                coarse((const float*)c.b.p, c.y.p, (float*)nullptr, (const double*)nullptr,
+                       (double*)nullptr);
+            } else {
+                launch_op<OP_PRE, float, float>(st, c.A, c.dinv.p, c.b.p, nullptr, nullptr, c.x.p,
+                                                c.res.p, (float*)nullptr, nullptr, nullptr, done);
+            }
+        }
+        // upward
+        for (int l = L - 2; l >= 0; --l) {
+            AmgLevelDev& f = lv[l];
+            AmgLevelDev& c = lv[l + 1];
+            launch_op<OP_SPMV, float, float>(st, f.P, nullptr, (const float*)nullptr, c.y.p, f.x.p,
+                                             nullptr, nullptr, f.x.p, nullptr, nullptr, done);
+            if (l == 0)
+                launch_op<OP_POST, double, double>(st, f.A, f.dinv.p, in, f.x.p, nullptr, nullptr,
+                                                   nullptr, out, nullptr, nullptr, done, out32);
+            else
+                launch_op<OP_POST, float, float>(st, f.A, f.dinv.p, f.b.p, f.x.p, nullptr,
+                                                 nullptr, nullptr, f.y.p, nullptr, nullptr, done);
+        }
     }
+    if (blocked && border_rows.n > 0)
+        hipLaunchKernelGGL(k_border_copy, dim3((border_rows.n + 63) / 64), dim3(64), 0, st,
+                           (int)border_rows.n, border_rows.p, in, out);
     MP_HIP(hipGetLastError());
 }
 

@@ -671,7 +671,6 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
     diag_inv = dinv.p;
     scaled32 = aval32.p;
     if (precond == MP_PRECOND_AMG) {
-        if (nseg != 1) throw std::runtime_error("PpeSolver: multigrid needs a single system");
         // keep A D^-1 (FP32) and D^-1 for the learned guess, then switch the
         // Krylov operator to the unscaled A with a scaling vector of ones
         jac_dinv.alloc(vlen);
@@ -702,11 +701,20 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
         a32k = aval32s.p;
     }
     if (precond == MP_PRECOND_AMG) {
+        // one problem: the hierarchy is built on its node block (rows 0..nv);
+        // a batch: one hierarchy per problem over all device rows
         SellView s0 = sv, s0k = svk;
-        s0.nv = s0k.nv = segs[0].nv;  // the real rows: the hierarchy is built on the node block
+        s0.nv = s0k.nv = nseg == 1 ? segs[0].nv : nrows;
         s0.nslices = s0k.nslices = (s0.nv + SLICE - 1) / SLICE;
+        std::vector<std::pair<int, int>> blocks;
+        std::vector<int> border_rows;
+        for (const Seg& S : segs) {
+            blocks.emplace_back(S.row0, S.nv);
+            if (S.border) border_rows.push_back(S.row0 + S.nv);
+        }
         amg.reset(new AmgDevice());
-        amg->setup(st, s0, aval.p, s0k, a32k, mp_amg::Options{});
+        amg->setup(st, s0, aval.p, s0k, a32k, blocks, mp_amg::Options{});
+        if (nseg > 1) amg->set_border_rows(st, border_rows);
         for (DevBuf<double>* b : {&yv, &zv}) {
             b->alloc(vlen);
             MP_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * vlen, st));
@@ -770,8 +778,10 @@ double PpeSolver::spmv_bytes(int fp32) const {
 const double* PpeSolver::precondition(hipStream_t st, const double* in, double* out,
                                       float* out32) {
     if (!amg) return in;
-    amg->apply(st, in, out, segs[0].border, &sc.p->done, amg->prescaled_input() != nullptr,
-               out32);
+    // one problem: its kernels stop once it has converged; a batch runs the
+    // V-cycle while any problem still iterates (the others ignore it)
+    amg->apply(st, in, out, nseg == 1 ? segs[0].border : 1, nseg == 1 ? &sc.p->done : nullptr,
+               amg->prescaled_input() != nullptr, out32);
     return out;
 }
 

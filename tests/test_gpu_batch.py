@@ -1,5 +1,6 @@
 """GPU: batched independent cavities in one context against the same problems
-solved one context each (and one of them against the oracle).
+solved one context each (and one of them against the oracle), with the
+Jacobi-preconditioned and the multigrid-preconditioned pressure solve.
 
 Each problem of a batch owns whole row blocks and its own Krylov scalars;
 its reductions run over its own blocks in the same order as when it is alone,
@@ -28,14 +29,14 @@ def _problems():
     return out
 
 
-def _run_single(cloud, re, dt, steps, precision):
+def _run_single(cloud, re, dt, steps, precision, precond=0):
     import memphys_amd as M
     with M.Context(0) as ctx:
         ctx.upload(cloud)
         ctx.weights(3, 3, download=False)
         u_dir, p_dir, u0 = M.lid_conditions(cloud)
         ctx.fs_create(1.0, 1.0 / re, dt, 2, 1e-10, 20000, u_dir, u_dir, None, p_dir,
-                      precision=precision)
+                      precision=precision, precond=precond)
         ctx.set_field(M.MP_U, u0)
         ctx.set_field(M.MP_V, np.zeros(cloud.nv))
         ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
@@ -46,14 +47,14 @@ def _run_single(cloud, re, dt, steps, precision):
         return ([ctx.get_field(w) for w in (M.MP_U, M.MP_V, M.MP_P)], iters)
 
 
-def _run_batch(probs, steps, precision):
+def _run_batch(probs, steps, precision, precond=0):
     import memphys_amd as M
     with M.Context(0) as ctx:
         cat = ctx.upload_batch([p[0] for p in probs])
         ctx.weights(3, 3, download=False)
         u_dir, p_dir, u0 = M.lid_conditions(cat)
         ctx.fs_create(1.0, 0.01, 1e-4, 2, 1e-10, 20000, u_dir, u_dir, None, p_dir,
-                      precision=precision)
+                      precision=precision, precond=precond)
         ctx.batch_params([1.0] * len(probs), [1.0 / p[1] for p in probs], [p[2] for p in probs])
         assert ctx.n_sys == cat.nv + len(probs)  # one border unknown per cavity
         ctx.set_field(M.MP_U, u0)
@@ -110,6 +111,42 @@ def test_batch_problem_matches_oracle():
     assert max(errs) <= 1e-8
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_multigrid_matches_single_and_oracle(precision):
+    """One multigrid hierarchy per cavity, stored block-diagonally: each
+    cavity's fields match its stand-alone multigrid run and the oracle (the
+    hierarchies of a batch share one depth, so they may differ from the
+    stand-alone ones, and the fields agree to the solver tolerance, not bit
+    for bit)."""
+    import memphys_amd as M
+    import oracle as O
+    probs = _problems()
+    steps = 4
+    batch = _run_batch(probs, steps, precision, precond=M.MP_PRECOND_AMG)
+    jac = _run_batch(probs, steps, precision)
+    for q, (cloud, re, dt) in enumerate(probs):
+        fields, iters = _run_single(cloud, re, dt, steps, precision, precond=M.MP_PRECOND_AMG)
+        bf, bi = batch[q]
+        _print(f"precision {precision} problem {q}: multigrid iterations batch {bi} "
+               f"single {iters}, Jacobi batch {jac[q][1]}")
+        errs = [rel_err(b, a) for a, b in zip(fields, bf)]
+        assert max(errs) <= 1e-8, errs
+        assert sum(bi) < sum(jac[q][1])
+    cloud, re, dt = probs[1]
+    w = O.stencil_weights(cloud, 3, 3)[:4]
+    u_dir, p_dir, u0 = M.lid_conditions(cloud)
+    fs = O.FractionalStep(cloud, w, u_dir, u_dir, None, p_dir, 1.0, 1.0 / re, dt, 2,
+                          solver_kind=0, precond=2, tol=1e-14, maxit=20000)
+    u, v = u0.copy(), np.zeros(cloud.nv)
+    p = np.zeros(fs.n_sys)
+    for it in range(steps):
+        fs.step(it, u, v, None, p)
+    (bu, bv, bp), _ = batch[1]
+    errs = [rel_err(bu, u), rel_err(bv, v), rel_err(bp, p)]
+    _print("batched multigrid problem 1 vs oracle:", errs)
+    assert max(errs) <= 1e-8
+
+
 def test_batch_rejects_single_problem_features():
     import memphys_amd as M
     probs = _problems()[:2]
@@ -117,9 +154,6 @@ def test_batch_rejects_single_problem_features():
         cat = ctx.upload_batch([p[0] for p in probs])
         ctx.weights(3, 3, download=False)
         u_dir, p_dir, _ = M.lid_conditions(cat)
-        with pytest.raises(M.MemphysError):
-            ctx.fs_create(1.0, 0.01, 1e-4, 2, 1e-10, 100, u_dir, u_dir, None, p_dir,
-                          precond=M.MP_PRECOND_AMG)
         ctx.fs_create(1.0, 0.01, 1e-4, 2, 1e-10, 100, u_dir, u_dir, None, p_dir)
         with pytest.raises(M.MemphysError):
             ctx.guess_mode(M.MP_GUESS_LEARNED)
