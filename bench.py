@@ -144,12 +144,13 @@ def stream_copy_gbps(mib: int = 1024, reps: int = 10) -> float:
     return 2.0 * (mib << 20) / (ms * 1e-3) / 1e9
 
 
-def run_batched_cavities(M, steps, seq_steps, tol, nprob=64):
+def run_batched_cavities(M, steps, seq_steps, tol, precond, nprob=64):
     """64 independent 2D cavities (N log-uniform on [2k, 40k], Re ~ U[50, 400])
     stepped together in one context: one launch per phase serves every cavity,
-    each pressure system (FP32-operator BiCGSTAB in FP64 defect correction)
-    stops on its own.  Beside it, the same cavities one context each, one after
-    another, for seq_steps steps."""
+    each pressure system (FP32-operator BiCGSTAB in FP64 defect correction,
+    Jacobi or one multigrid hierarchy per cavity) stops on its own.  Beside
+    it, the same cavities one context each, one after another, for seq_steps
+    steps, with the same preconditioner."""
     probs = M.cavity_batch(nprob)
     clouds = [p[0] for p in probs]
     ctx = M.Context(0)
@@ -158,7 +159,7 @@ def run_batched_cavities(M, steps, seq_steps, tol, nprob=64):
     ctx.weights(3, 3, precision=M.MP_PREC_MIXED, download=False)
     u_dir, p_dir, u0 = M.lid_conditions(cat)
     ctx.fs_create(1.0, 0.01, 1e-4, 2, tol, 20000, u_dir, u_dir, None, p_dir,
-                  precision=M.MP_PREC_MIXED, inner_tol=1e-5)
+                  precision=M.MP_PREC_MIXED, inner_tol=1e-5, precond=precond)
     ctx.batch_params([1.0] * nprob, [1.0 / p[1] for p in probs], [p[2] for p in probs])
     ctx.set_field(M.MP_U, u0)
     ctx.set_field(M.MP_V, np.zeros(cat.nv))
@@ -185,6 +186,7 @@ def run_batched_cavities(M, steps, seq_steps, tol, nprob=64):
     log(f"[batch] {steps} steps in {t_batch:.2f}s; slowest system {it_sum / steps:.0f} "
         f"iterations/step; max rel_res {worst:.2e}")
     # the same cavities one after another, one context each
+    pname = "multigrid V-cycle per cavity" if precond == M.MP_PRECOND_AMG else "Jacobi"
     t_seq = 0.0
     for q, (cloud, re, dt) in enumerate(probs):
         with M.Context(0) as c1:
@@ -192,7 +194,7 @@ def run_batched_cavities(M, steps, seq_steps, tol, nprob=64):
             c1.weights(3, 3, precision=M.MP_PREC_MIXED, download=False)
             ud, pd, uz = M.lid_conditions(cloud)
             c1.fs_create(1.0, 1.0 / re, dt, 2, tol, 20000, ud, ud, None, pd,
-                         precision=M.MP_PREC_MIXED, inner_tol=1e-5)
+                         precision=M.MP_PREC_MIXED, inner_tol=1e-5, precond=precond)
             c1.set_field(M.MP_U, uz)
             c1.set_field(M.MP_V, np.zeros(cloud.nv))
             c1.set_field(M.MP_P, np.zeros(c1.n_sys))
@@ -206,7 +208,8 @@ def run_batched_cavities(M, steps, seq_steps, tol, nprob=64):
     return {
         "workload": (f"{nprob} 2D lid-driven cavities, N log-uniform on [2000, 40000] "
                      f"({cat.nv} nodes total), Re ~ U[50, 400], jitter +-0.25h, k=30, "
-                     f"FP32-operator BiCGSTAB in FP64 defect correction, rel. residual {tol:g}"),
+                     f"FP32-operator BiCGSTAB ({pname}) in FP64 defect correction, "
+                     f"rel. residual {tol:g}"),
         "steps": steps,
         "cavity_steps_per_s": nprob * steps / t_batch,
         "batch_timesteps_per_s": steps / t_batch,
@@ -317,6 +320,8 @@ def main():
                     help="solver steps of guess training (10 episodes)")
     ap.add_argument("--batch-steps", type=int, default=200,
                     help="steps of the 64-cavity batch (0: skip it)")
+    ap.add_argument("--batch-precond", choices=["multigrid", "jacobi"], default="multigrid",
+                    help="preconditioner of the 64-cavity batch and its sequential comparison")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel instead of replaying hipGraphs (same results; "
                          "for profilers)")
@@ -470,7 +475,8 @@ def main():
     }
     if args.batch_steps > 0:
         ctx.close()
-        out["batched_cavities"] = run_batched_cavities(M, args.batch_steps, 5, args.tol)
+        bp = M.MP_PRECOND_AMG if args.batch_precond == "multigrid" else M.MP_PRECOND_JACOBI
+        out["batched_cavities"] = run_batched_cavities(M, args.batch_steps, 5, args.tol, bp)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(cloud, (gx, gy, gz, lap), prm,
