@@ -64,7 +64,8 @@ def measured_traffic(n: int, kernel_index: int = 0):
     return vals[kernel_index] if len(vals) > kernel_index else None
 
 
-def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, precond=0):
+def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, precond=0,
+             tune=True):
     u_dir, p_dir, u0 = lid_bc(cloud)
     t = time.perf_counter()
     ctx.fs_create(prm["rho"], prm["mu"], prm["dt"], 2, prm["tol"], prm["n_iter"], u_dir, u_dir,
@@ -88,8 +89,11 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
         ctx.step(1, it)
         st = ctx.stats()
         log(f"[{label}] warmup step {it}: iters {st.iterations} rel_res {st.rel_res:.2e}")
-    interval, tune_ms = ctx.autotune()
-    log(f"[{label}] poll interval {interval} (ms per solve {tune_ms})")
+    if tune:
+        interval, tune_ms = ctx.autotune()
+        log(f"[{label}] poll interval {interval} (ms per solve {tune_ms})")
+    else:
+        interval = prm["check_every"]
     it0 = warmup
     iters = []
     res = []
@@ -325,6 +329,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel instead of replaying hipGraphs (same results; "
                          "for profilers)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default poll interval and multigrid layouts (for counter "
+                         "runs, under which the tuning's timed solves take minutes)")
     ap.add_argument("--forms", default="fp64,mixed,multigrid,headline",
                     help="comma list of forms to time besides fp64: mixed (FP32-inner, "
                          "Jacobi), multigrid (FP32-inner, multigrid, previous pressure as "
@@ -395,18 +402,21 @@ def main():
     log(f"weights check: {weights_check}")
 
     forms = args.forms.split(",")
-    fp64 = run_form(M, ctx, cloud, prm, M.MP_PREC_FP64, args.steps, args.warmup, "fp64")
+    tune = not args.no_autotune
+    fp64 = run_form(M, ctx, cloud, prm, M.MP_PREC_FP64, args.steps, args.warmup, "fp64",
+                    tune=tune)
     extra = {}
     if "mixed" in forms:
         extra["mixed_jacobi"] = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps,
-                                         args.warmup, "mixed")
+                                         args.warmup, "mixed", tune=tune)
     if "multigrid" in forms:  # the headline form without the learned guess
         extra["mixed_multigrid"] = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps,
                                             args.warmup, "mixed+multigrid",
-                                            precond=M.MP_PRECOND_AMG)
+                                            precond=M.MP_PRECOND_AMG, tune=tune)
     if "headline" in forms:
         head = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps, args.warmup,
-                        "mixed+multigrid+learned", guess=guess, precond=M.MP_PRECOND_AMG)
+                        "mixed+multigrid+learned", guess=guess, precond=M.MP_PRECOND_AMG,
+                        tune=tune)
     else:
         head = fp64
     vdiff = max(np.linalg.norm(a - b) / max(np.linalg.norm(a), 1e-300)

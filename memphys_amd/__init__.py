@@ -140,7 +140,8 @@ def host_lib() -> C.CDLL:
         lib.mph_rcm.argtypes = [C.c_int, _ip, _ip, _ip]
         lib.mph_apply_order.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp, _u8, C.c_void_p, _ip, _ip]
         lib.mph_poly_exponents.argtypes = [C.c_int, C.c_int, _ip]
-        lib.mph_amg_first_level.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_double, _ip,
+        lib.mph_amg_first_level.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_double,
+                                            C.c_double, C.c_double, _ip,
                                             C.POINTER(C.c_int), _ip, _ip, _dp, C.c_long, _ip,
                                             _ip, _dp, C.c_long]
         lib.mph_amg_bordered_inverse.argtypes = [C.c_int, _ip, _ip, _dp, _dp]
@@ -598,9 +599,12 @@ def concat_clouds(clouds) -> "Cloud":
                  layer=np.concatenate([c.layer for c in clouds]))
 
 
-def amg_first_level(A, k: int = 6, omega_p: float = 0.66):
+def amg_first_level(A, k: int = 6, omega_p: float = 0.66, trunc: float = 0.0,
+                    drop: float = 0.0):
     """First coarsening step of the library's multigrid set-up on a scipy CSR
-    matrix: (aggregate per row, P, R A P) as computed by the C++ builder."""
+    matrix: (aggregate per row, P, R A P) as computed by the C++ builder, with
+    P truncated at `trunc` and R A P sparsified at `drop` (the library's
+    hierarchy uses 0.15 and 0.01; 0 turns either off)."""
     import scipy.sparse as sp
     A = sp.csr_matrix(A)
     A.sort_indices()
@@ -618,8 +622,9 @@ def amg_first_level(A, k: int = 6, omega_p: float = 0.66):
     ac_ptr = np.zeros(n + 1, dtype=np.int32)
     ac_col = np.zeros(ac_cap, dtype=np.int32)
     ac_val = np.zeros(ac_cap)
-    rc = host_lib().mph_amg_first_level(n, ptr, col, val, k, omega_p, agg, C.byref(na), p_ptr,
-                                        p_col, p_val, pcap, ac_ptr, ac_col, ac_val, ac_cap)
+    rc = host_lib().mph_amg_first_level(n, ptr, col, val, k, omega_p, trunc, drop, agg,
+                                        C.byref(na), p_ptr, p_col, p_val, pcap, ac_ptr, ac_col,
+                                        ac_val, ac_cap)
     if rc != 0:
         raise MemphysError(f"mph_amg_first_level failed ({rc})")
     m = na.value

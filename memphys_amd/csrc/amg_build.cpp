@@ -99,7 +99,8 @@ void aggregate(const Csr& A, int k, std::vector<int>& agg, int& n_agg) {
     }
 }
 
-Csr smoothed_prolongator(const Csr& A, const std::vector<int>& agg, int n_agg, double omega) {
+Csr smoothed_prolongator(const Csr& A, const std::vector<int>& agg, int n_agg, double omega,
+                         double trunc) {
     const int n = A.nrows;
     Csr P;
     P.nrows = n;
@@ -118,16 +119,65 @@ Csr smoothed_prolongator(const Csr& A, const std::vector<int>& agg, int n_agg, d
                          [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
                              return a.first < b.first;
                          });
+        const size_t first = P.col.size();
+        double amax = 0.0;
         for (size_t q = 0; q < row.size();) {
             const int c = row[q].first;
             double s = 0.0;
             while (q < row.size() && row[q].first == c) s += row[q++].second;
             P.col.push_back(c);
             P.val.push_back(s);
+            amax = std::max(amax, std::fabs(s));
+        }
+        if (trunc > 0.0) {
+            double all = 0.0, kept = 0.0;
+            size_t w = first;
+            for (size_t q = first; q < P.col.size(); ++q) {
+                all += P.val[q];
+                if (std::fabs(P.val[q]) < trunc * amax) continue;
+                kept += P.val[q];
+                P.col[w] = P.col[q];
+                P.val[w++] = P.val[q];
+            }
+            P.col.resize(w);
+            P.val.resize(w);
+            if (std::fabs(kept) > 1e-300)
+                for (size_t q = first; q < w; ++q) P.val[q] *= all / kept;
         }
         P.ptr[i + 1] = (int)P.col.size();
     }
     return P;
+}
+
+Csr sparsify(const Csr& A, double drop) {
+    const int n = A.nrows;
+    std::vector<double> d(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int t = A.ptr[i]; t < A.ptr[i + 1]; ++t)
+            if (A.col[t] == i) d[i] = std::fabs(A.val[t]);
+    Csr S;
+    S.nrows = n;
+    S.ncols = A.ncols;
+    S.ptr.assign(n + 1, 0);
+    S.col.reserve(A.col.size());
+    S.val.reserve(A.val.size());
+    for (int i = 0; i < n; ++i) {
+        double lumped = 0.0;
+        long diag = -1;
+        for (int t = A.ptr[i]; t < A.ptr[i + 1]; ++t) {
+            const int j = A.col[t];
+            if (j != i && std::fabs(A.val[t]) < drop * std::sqrt(d[i] * d[j])) {
+                lumped += A.val[t];
+                continue;
+            }
+            if (j == i) diag = (long)S.col.size();
+            S.col.push_back(j);
+            S.val.push_back(A.val[t]);
+        }
+        if (diag >= 0) S.val[diag] += lumped;
+        S.ptr[i + 1] = (int)S.col.size();
+    }
+    return S;
 }
 
 Csr transpose(const Csr& A) {
@@ -258,10 +308,11 @@ Hierarchy build(const Csr& A0, const Options& opt) {
         int na = 0;
         aggregate(*cur, opt.strength_k, agg, na);
         if (na >= cur->nrows) break;  // no coarsening possible
-        Csr P = smoothed_prolongator(*cur, agg, na, opt.omega_p);
+        Csr P = smoothed_prolongator(*cur, agg, na, opt.omega_p, opt.trunc);
         Csr R = transpose(P);
         Csr AP = multiply(*cur, P, opt.threads);
         Csr Ac = multiply(R, AP, opt.threads);
+        if (opt.drop > 0.0) Ac = sparsify(Ac, opt.drop);
         h.P.push_back(std::move(P));
         h.R.push_back(std::move(R));
         h.A.push_back(std::move(Ac));

@@ -7,8 +7,12 @@
 // neighbours are all free seeds an aggregate with them; leftovers join the
 // aggregate of their first aggregated neighbour, else stay alone); tentative
 // prolongator = aggregate indicator (the constant near-null space of the
-// Neumann Laplacian); smoothed P = (I - omega D^-1 A) P_tent; R = P^T;
-// A_{l+1} = R A_l P (Galerkin).  The coarsest level is solved with the dense
+// Neumann Laplacian); smoothed P = (I - omega D^-1 A) P_tent, optionally
+// truncated (entries below trunc x the row's largest |p| dropped, the rest
+// rescaled to keep the row sum, so constants are still reproduced); R = P^T;
+// A_{l+1} = R A_l P (Galerkin), optionally sparsified (off-diagonal entries
+// below drop x sqrt|a_ii a_jj| moved onto the diagonal, which keeps row sums
+// and so the constant null vector).  The coarsest level is solved with the dense
 // inverse of its matrix bordered by a ones row and column (the constant mode
 // is singular there, as in the full-Neumann pressure system).
 #pragma once
@@ -26,6 +30,8 @@ struct Csr {
 struct Options {
     int strength_k = 6;      // strongest off-diagonal couplings kept per row
     double omega_p = 0.66;   // prolongation smoothing weight
+    double trunc = 0.15;     // prolongator truncation (relative to the row's max)
+    double drop = 0.01;      // coarse-operator sparsification threshold
     int coarse_max = 700;    // a level with at most this many rows is solved densely
     int max_levels = 12;
     int threads = 0;         // 0: hardware concurrency
@@ -41,7 +47,10 @@ struct Hierarchy {
 };
 
 void aggregate(const Csr& A, int k, std::vector<int>& agg, int& n_agg);
-Csr smoothed_prolongator(const Csr& A, const std::vector<int>& agg, int n_agg, double omega);
+Csr smoothed_prolongator(const Csr& A, const std::vector<int>& agg, int n_agg, double omega,
+                         double trunc = 0.0);
+// Moves off-diagonal entries below drop * sqrt(|a_ii a_jj|) onto the diagonal.
+Csr sparsify(const Csr& A, double drop);
 Csr transpose(const Csr& A);
 Csr multiply(const Csr& A, const Csr& B, int threads);
 // Top-left n x n block of the inverse of [[A, 1], [1^T, 0]] (A dense n x n).

@@ -128,12 +128,14 @@ int mph_gmsh_read(const char* path, int* dim, int* nv, double* xyz, uint8_t* bou
 
 // First coarsening step of the multigrid set-up the HIP library runs
 // (amg_build.hpp), for CPU checks: aggregates (n entries), the smoothed
-// prolongator P (n x n_agg) and the Galerkin operator R A P (n_agg square) as
+// prolongator P (n x n_agg, truncated at `trunc`) and the Galerkin operator
+// R A P (n_agg square, sparsified at `drop`; 0 keeps every entry) as
 // CSR into caller buffers of the given capacities.  Returns 0, -1 on error, or
 // 1 when a capacity is too small (p_ptr[n] / ac_ptr[n_agg] then hold the sizes
 // needed when they fit).
 int mph_amg_first_level(int n, const int* ptr, const int* col, const double* val, int k,
-                        double omega_p, int* agg, int* n_agg, int* p_ptr, int* p_col,
+                        double omega_p, double trunc, double drop, int* agg, int* n_agg,
+                        int* p_ptr, int* p_col,
                         double* p_val, long p_cap, int* ac_ptr, int* ac_col, double* ac_val,
                         long ac_cap) {
     try {
@@ -145,9 +147,10 @@ int mph_amg_first_level(int n, const int* ptr, const int* col, const double* val
         std::vector<int> a;
         int na = 0;
         mp_amg::aggregate(A, k, a, na);
-        mp_amg::Csr P = mp_amg::smoothed_prolongator(A, a, na, omega_p);
+        mp_amg::Csr P = mp_amg::smoothed_prolongator(A, a, na, omega_p, trunc);
         mp_amg::Csr R = mp_amg::transpose(P);
         mp_amg::Csr Ac = mp_amg::multiply(R, mp_amg::multiply(A, P, 0), 0);
+        if (drop > 0.0) Ac = mp_amg::sparsify(Ac, drop);
         std::memcpy(agg, a.data(), sizeof(int) * n);
         *n_agg = na;
         std::memcpy(p_ptr, P.ptr.data(), sizeof(int) * (n + 1));

@@ -4,6 +4,11 @@
         per kernel and grid size: dispatches, total ms, mean/min/max us, and the
         mean over the launches that did the work (at least half the median
         duration: Krylov launches of a converged system return at once)
+    python -m memphys_amd.profiling iteration DB
+        one preconditioned BiCGSTAB iteration of the largest system: the
+        iterations between consecutive k_update_p launches followed by a
+        multigrid launch, with a working SpMV; their median wall and busy time,
+        and the kernel list of the iteration of median wall time
     python -m memphys_amd.profiling pmc DB [--top N]
         per kernel, grid size and counter: dispatches and mean value per dispatch
     python -m memphys_amd.profiling traffic FETCH_DB WRITE_DB --kernel SUBSTR
@@ -60,6 +65,40 @@ def stats(db: str, top: int) -> str:
     lines.append(f"total kernel time {tot / 1e6:.2f} ms over {sum(len(v) for _, v in rows)} "
                  f"dispatches; working = launches of at least half the median duration of "
                  f"their kernel and grid, work_us their mean")
+    return "\n".join(lines)
+
+
+def iteration(db: str, first: str = "k_update_p<double>", marker: str = "k_amg_op",
+              product: str = "k_spmv_alpha") -> str:
+    cur = sqlite3.connect(db).cursor()
+    rows = cur.execute("select name, start, end, grid_x from kernels order by start").fetchall()
+    grid = max((r[3] for r in rows if first in r[0]), default=None)
+    if grid is None:
+        raise SystemExit(f"no launch of '{first}' in {db}")
+    heads = [i for i, r in enumerate(rows[:-1])
+             if first in r[0] and r[3] == grid and marker in rows[i + 1][0]]
+    prod = [r[2] - r[1] for r in rows if product in r[0] and r[3] >= grid]
+    if not heads or not prod:
+        raise SystemExit(f"no preconditioned iteration at grid {grid} in {db}")
+    cut = min(_working(prod))
+    its = []
+    for a, b in zip(heads[:-1], heads[1:]):
+        seg = rows[a:b]
+        if b - a > 80 or not any(product in r[0] and r[3] >= grid and r[2] - r[1] >= cut
+                                 for r in seg):
+            continue  # a solve's last iteration, or a converged system's chunk
+        its.append((rows[b][1] - rows[a][1], sum(r[2] - r[1] for r in seg), a, b))
+    if not its:
+        raise SystemExit("no complete iteration found")
+    its.sort()
+    wall, _, a, b = its[len(its) // 2]
+    busy = sorted(x[1] for x in its)[len(its) // 2]
+    lines = [f"iterations {len(its)}, median wall {wall / 1e3:.1f} us, "
+             f"median busy {busy / 1e3:.1f} us (grid of {first}: {grid})"]
+    t0 = rows[a][1]
+    for name, s, e, g in rows[a:b]:
+        lines.append(f"{_short(name)[:46]:46s} grid {g:9d} start {(s - t0) / 1e3:8.1f} us "
+                     f"dur {(e - s) / 1e3:7.1f} us")
     return "\n".join(lines)
 
 
@@ -133,6 +172,8 @@ def main(argv=None):
     a = sub.add_parser("stats")
     a.add_argument("db")
     a.add_argument("--top", type=int, default=40)
+    it = sub.add_parser("iteration")
+    it.add_argument("db")
     b = sub.add_parser("pmc")
     b.add_argument("db")
     b.add_argument("--top", type=int, default=40)
@@ -146,6 +187,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(stats(args.db, args.top))
+    elif args.cmd == "iteration":
+        print(iteration(args.db))
     elif args.cmd == "pmc":
         print(pmc(args.db, args.top))
     else:

@@ -4,7 +4,9 @@ pressure operator.
 The builder's output is pinned by what the method fixes independently of
 its code: the aggregates partition the nodes; the prolongator equals
 (I - w D^-1 A) P_tent formed here with scipy from the aggregates; the coarse
-operator equals P^T A P formed with scipy; the aggregation follows the rule
+operator equals P^T A P formed with scipy; the truncated prolongator and the
+sparsified coarse operator equal their rules applied here to those, and keep
+the constant mode (a truncation that skips the rescale does not); the aggregation follows the rule
 it states (restated below on the same strength graph); the coarsest-level
 solve is the inverse of the ones-bordered matrix (numpy LAPACK inverse).
 """
@@ -122,3 +124,55 @@ def test_bordered_inverse(ppe_block):
     assert abs(x.sum()) <= 1e-9 * np.abs(x).max()
     r = A2 @ x - b
     assert np.abs(r - r.mean()).max() <= 1e-8 * np.linalg.norm(b)
+
+
+def _truncate_restated(P0, trunc, rescale=True):
+    """Each row: entries below trunc x its largest |p| dropped, the rest scaled
+    to the full row's sum."""
+    P0 = P0.tocsr()
+    rows, cols, vals = [], [], []
+    for i in range(P0.shape[0]):
+        c = P0.indices[P0.indptr[i]:P0.indptr[i + 1]]
+        v = P0.data[P0.indptr[i]:P0.indptr[i + 1]]
+        keep = np.abs(v) >= trunc * np.abs(v).max()
+        s = v[keep].sum()
+        f = v.sum() / s if (rescale and abs(s) > 1e-300) else 1.0
+        rows += [i] * int(keep.sum())
+        cols += list(c[keep])
+        vals += list(v[keep] * f)
+    return sp.csr_matrix((vals, (rows, cols)), shape=P0.shape)
+
+
+def _sparsify_restated(Ac, drop):
+    """Off-diagonal entries below drop x sqrt(|a_ii a_jj|) moved onto the diagonal."""
+    Ac = Ac.tocoo()
+    d = np.abs(Ac.tocsr().diagonal())
+    weak = (Ac.row != Ac.col) & (np.abs(Ac.data) < drop * np.sqrt(d[Ac.row] * d[Ac.col]))
+    lump = np.bincount(Ac.row[weak], weights=Ac.data[weak], minlength=Ac.shape[0])
+    S = sp.csr_matrix((Ac.data[~weak], (Ac.row[~weak], Ac.col[~weak])), shape=Ac.shape)
+    return (S + sp.diags(lump)).tocsr()
+
+
+def test_truncated_prolongator_and_sparsified_operator(ppe_block):
+    import memphys_amd as M
+    A = ppe_block
+    trunc, drop = 0.15, 0.01
+    _, P0, _ = M.amg_first_level(A, k=6)
+    agg, P, Ac = M.amg_first_level(A, k=6, trunc=trunc, drop=drop)
+    n, na = P.shape
+    # P: the untruncated prolongator, truncated and rescaled row by row
+    P_ref = _truncate_restated(P0, trunc)
+    assert P.nnz == P_ref.nnz and P.nnz < 0.6 * P0.nnz
+    assert abs(P - P_ref).max() <= 1e-13 * abs(P_ref).max()
+    # row sums survive the truncation, so constants are still reproduced; a
+    # truncation without the rescale does not reproduce them
+    assert np.abs(P @ np.ones(na) - P0 @ np.ones(na)).max() <= 1e-12
+    assert np.abs(P @ np.ones(na) - 1.0).max() <= 1e-8
+    bad = _truncate_restated(P0, trunc, rescale=False)
+    assert np.abs(bad @ np.ones(na) - 1.0).max() > 1e-3
+    # R A P with the weak entries lumped onto the diagonal
+    Ac_ref = _sparsify_restated((P_ref.T @ A @ P_ref).tocsr(), drop)
+    Ac_ref.eliminate_zeros()
+    assert abs(Ac - Ac_ref).max() <= 1e-12 * abs(Ac_ref).max()
+    assert Ac.nnz < 0.5 * (P_ref.T @ A @ P_ref).nnz
+    assert np.abs(Ac @ np.ones(na)).max() <= 1e-6 * abs(Ac).max()
