@@ -149,6 +149,86 @@ __global__ __launch_bounds__(256) void k_dense(int nv, const float* __restrict__
     }
 }
 
+// Hidden layer on the matrix cores: Z = in1 W1 + in2 W2 + b, Hout = relu(Z)
+// (Z not stored when null); in1, in2: [nv][H], W1, W2: [H][H] (BF16, k-major).
+// v_mfma_f32_16x16x4f32 keeps every product and sum in FP32.  A wavefront
+// takes tiles of 16 rows; the 128 reduction indices k (in1's 64, then in2's)
+// are walked in 32 steps of 4, and lane l holds, for step (j, e), row
+// l % 16 of the tile at k = 16 j + 4 (l / 16) + e: one 16-byte load of four
+// consecutive inputs per lane and j.  The weights sit in LDS in the same
+// fragment order (wf[j][n-block][lane][e]), so each (j, n-block) is one
+// 16-byte LDS read.  Accumulators start at the bias.
+constexpr int MROWS = 16;  // rows per tile of the matrix-core dense kernel
+__global__ __launch_bounds__(256) void k_dense_mfma(int nv, const float* __restrict__ in1,
+                                                    const float* __restrict__ in2,
+                                                    const uint16_t* __restrict__ W1,
+                                                    const uint16_t* __restrict__ W2,
+                                                    const uint16_t* __restrict__ bias,
+                                                    float* __restrict__ Z,
+                                                    float* __restrict__ Hout) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    static_assert(H == 64, "fragment layout assumes 64 hidden units");
+    __shared__ f4 wf[8 * 4 * 64];
+    for (int t = threadIdx.x; t < 8 * 4 * 64; t += 256) {
+        const int lane = t & 63, nb = (t >> 6) & 3, j = t >> 8;
+        const int n = 16 * nb + (lane & 15);
+        f4 v;
+        for (int e = 0; e < 4; ++e) {
+            const int k = 16 * j + 4 * (lane >> 4) + e;
+            v[e] = bf2f(k < H ? W1[k * H + n] : W2[(k - H) * H + n]);
+        }
+        wf[t] = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int q = lane >> 4, c = lane & 15;
+    float bcol[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) bcol[nb] = bf2f(bias[16 * nb + c]);
+    const int ntiles = (nv + MROWS - 1) / MROWS;
+    for (int tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += gridDim.x * 4) {
+        const int r0 = tile * MROWS;
+        const int row = r0 + c;
+        f4 a[8];
+        if (row < nv) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                a[j] = *reinterpret_cast<const f4*>(in1 + (size_t)row * H + 16 * j + 4 * q);
+                a[j + 4] = *reinterpret_cast<const f4*>(in2 + (size_t)row * H + 16 * j + 4 * q);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a[j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        f4 acc[4];
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) acc[nb] = f4{bcol[nb], bcol[nb], bcol[nb], bcol[nb]};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const f4 b = wf[(j * 4 + nb) * 64 + lane];
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j].x, b.x, acc[nb], 0, 0, 0);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j].y, b.y, acc[nb], 0, 0, 0);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j].z, b.z, acc[nb], 0, 0, 0);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j].w, b.w, acc[nb], 0, 0, 0);
+            }
+        }
+        // lane holds rows r0 + 4q + i, columns 16 nb + c
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = r0 + 4 * q + i;
+            if (r >= nv) break;
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const size_t o = (size_t)r * H + 16 * nb + c;
+                if (Z) Z[o] = acc[nb][i];
+                Hout[o] = fmaxf(acc[nb][i], 0.0f);
+            }
+        }
+    }
+}
+
 // out[i][k] = sum_c dZ[i][c] W[k][c] (+ out[i][k] if accumulate); W: [K][H]
 template <int K>
 __global__ __launch_bounds__(256) void k_dense_t(int nv, const float* __restrict__ dZ,
@@ -518,13 +598,14 @@ void GuessNet::features(hipStream_t st, const double* p1, const double* p2, cons
 void GuessNet::forward(hipStream_t st, bool keep_pre) {
     const uint16_t* P = theta_bf.p;
     const int g = std::min(DENSE_GRID, (nv + DR - 1) / DR);
+    const int gm = std::min(DENSE_GRID, (nv + 4 * MROWS - 1) / (4 * MROWS));
     hipLaunchKernelGGL((k_dense<F, false>), dim3(g), dim3(256), 0, st, nv, X.p, nullptr,
                        P + GNN_W0, nullptr, P + GNN_B0, keep_pre ? Z[0].p : nullptr, Hh[0].p);
     for (int l = 1; l <= GNN_L; ++l) {
         const int o = GNN_LAYER + (l - 1) * GNN_LAYER_SIZE;
         hipLaunchKernelGGL(k_aggregate, dim3((nv + 3) / 4), dim3(256), 0, st, sv, a32, Hh[l - 1].p,
                            Mm[l].p, 0);
-        hipLaunchKernelGGL((k_dense<H, true>), dim3(g), dim3(256), 0, st, nv, Hh[l - 1].p, Mm[l].p,
+        hipLaunchKernelGGL(k_dense_mfma, dim3(gm), dim3(256), 0, st, nv, Hh[l - 1].p, Mm[l].p,
                            P + o, P + o + H * H, P + o + 2 * H * H, keep_pre ? Z[l].p : nullptr,
                            Hh[l].p);
     }
