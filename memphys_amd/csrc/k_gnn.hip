@@ -248,22 +248,34 @@ __global__ __launch_bounds__(256) void k_dense_t(int nv, const float* __restrict
     }
 }
 
-// m[i][:] = sum_j a_ij h[j][:] over a SELL-64x4 operator; one wavefront per row,
-// lane = feature.  accumulate: m += instead of m =.  The row's slots come in
-// groups of four (one uniform load of columns and one of values); the
-// gathers of two groups are issued before their fmas, which keep the order
-// of the row's entries.
+// m[i][:] = sum_j a_ij h[j][:] over a SELL-64x4 operator.  A wavefront takes
+// four consecutive rows, sixteen lanes per row, each lane four features (one
+// 16-byte gather per neighbour), so one gather instruction fetches four
+// neighbour rows.  accumulate: m += instead of m =.  A row's slots come in
+// groups of four (one 16-byte load of columns and one of values, the row's
+// own lanes of the slice); the gathers of two groups are issued before their
+// fmas, which keep the order of the row's entries.
 __global__ __launch_bounds__(256) void k_aggregate(SellView sv, const float* __restrict__ val,
                                                    const float* __restrict__ h,
                                                    float* __restrict__ m, int accumulate) {
     typedef int i4 __attribute__((ext_vector_type(4)));
     typedef float f4 __attribute__((ext_vector_type(4)));
-    const int row = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    static_assert(H == 64, "sixteen lanes of four features per row");
+    const int row = (blockIdx.x * 4 + (int)(threadIdx.x >> 6)) * 4 + (int)((threadIdx.x >> 4) & 3);
     if (row >= sv.nv) return;
-    const int c = threadIdx.x & 63;
+    const int fq = 4 * (threadIdx.x & 15);  // features fq..fq+3
     const int base = sell_base(sv.sptr, row);
     const int len = sv.len[row];
-    float acc = accumulate ? m[(size_t)row * H + c] : 0.0f;
+    f4 acc = accumulate ? *reinterpret_cast<const f4*>(m + (size_t)row * H + fq)
+                        : f4{0.0f, 0.0f, 0.0f, 0.0f};
+    auto gather = [&](int c) { return *reinterpret_cast<const f4*>(h + (size_t)c * H + fq); };
+    auto fma4 = [](float a, const f4& x, f4 y) {
+        y.x = fmaf(a, x.x, y.x);
+        y.y = fmaf(a, x.y, y.y);
+        y.z = fmaf(a, x.z, y.z);
+        y.w = fmaf(a, x.w, y.w);
+        return y;
+    };
     int j0 = 0;
     for (; j0 + 8 <= len; j0 += 8) {
         const int s4 = base + (j0 << 6);
@@ -271,34 +283,28 @@ __global__ __launch_bounds__(256) void k_aggregate(SellView sv, const float* __r
         const i4 c1 = *reinterpret_cast<const i4*>(sv.col + s4 + 256);
         const f4 v0 = *reinterpret_cast<const f4*>(val + s4);
         const f4 v1 = *reinterpret_cast<const f4*>(val + s4 + 256);
-        const float x0 = h[(size_t)c0.x * H + c], x1 = h[(size_t)c0.y * H + c];
-        const float x2 = h[(size_t)c0.z * H + c], x3 = h[(size_t)c0.w * H + c];
-        const float x4 = h[(size_t)c1.x * H + c], x5 = h[(size_t)c1.y * H + c];
-        const float x6 = h[(size_t)c1.z * H + c], x7 = h[(size_t)c1.w * H + c];
-        acc = fmaf(v0.x, x0, acc);
-        acc = fmaf(v0.y, x1, acc);
-        acc = fmaf(v0.z, x2, acc);
-        acc = fmaf(v0.w, x3, acc);
-        acc = fmaf(v1.x, x4, acc);
-        acc = fmaf(v1.y, x5, acc);
-        acc = fmaf(v1.z, x6, acc);
-        acc = fmaf(v1.w, x7, acc);
+        const f4 x0 = gather(c0.x), x1 = gather(c0.y), x2 = gather(c0.z), x3 = gather(c0.w);
+        const f4 x4 = gather(c1.x), x5 = gather(c1.y), x6 = gather(c1.z), x7 = gather(c1.w);
+        acc = fma4(v0.x, x0, acc);
+        acc = fma4(v0.y, x1, acc);
+        acc = fma4(v0.z, x2, acc);
+        acc = fma4(v0.w, x3, acc);
+        acc = fma4(v1.x, x4, acc);
+        acc = fma4(v1.y, x5, acc);
+        acc = fma4(v1.z, x6, acc);
+        acc = fma4(v1.w, x7, acc);
     }
     for (; j0 < len; j0 += 4) {
         const int s4 = base + (j0 << 6);
         const i4 cc = *reinterpret_cast<const i4*>(sv.col + s4);
         const f4 vv = *reinterpret_cast<const f4*>(val + s4);
-        const int n = min(4, len - j0);  // uniform: entries of this group inside the row
-        const float x0 = h[(size_t)cc.x * H + c];
-        const float x1 = n > 1 ? h[(size_t)cc.y * H + c] : 0.0f;
-        const float x2 = n > 2 ? h[(size_t)cc.z * H + c] : 0.0f;
-        const float x3 = n > 3 ? h[(size_t)cc.w * H + c] : 0.0f;
-        acc = fmaf(vv.x, x0, acc);
-        if (n > 1) acc = fmaf(vv.y, x1, acc);
-        if (n > 2) acc = fmaf(vv.z, x2, acc);
-        if (n > 3) acc = fmaf(vv.w, x3, acc);
+        const int n = min(4, len - j0);  // entries of this group inside the row
+        acc = fma4(vv.x, gather(cc.x), acc);
+        if (n > 1) acc = fma4(vv.y, gather(cc.y), acc);
+        if (n > 2) acc = fma4(vv.z, gather(cc.z), acc);
+        if (n > 3) acc = fma4(vv.w, gather(cc.w), acc);
     }
-    m[(size_t)row * H + c] = acc;
+    *reinterpret_cast<f4*>(m + (size_t)row * H + fq) = acc;
 }
 
 // d_i = h_i . wo + bo; one wavefront per row
@@ -603,7 +609,7 @@ void GuessNet::forward(hipStream_t st, bool keep_pre) {
                        P + GNN_W0, nullptr, P + GNN_B0, keep_pre ? Z[0].p : nullptr, Hh[0].p);
     for (int l = 1; l <= GNN_L; ++l) {
         const int o = GNN_LAYER + (l - 1) * GNN_LAYER_SIZE;
-        hipLaunchKernelGGL(k_aggregate, dim3((nv + 3) / 4), dim3(256), 0, st, sv, a32, Hh[l - 1].p,
+        hipLaunchKernelGGL(k_aggregate, dim3((nv + 15) / 16), dim3(256), 0, st, sv, a32, Hh[l - 1].p,
                            Mm[l].p, 0);
         hipLaunchKernelGGL(k_dense_mfma, dim3(gm), dim3(256), 0, st, nv, Hh[l - 1].p, Mm[l].p,
                            P + o, P + o + H * H, P + o + 2 * H * H, keep_pre ? Z[l].p : nullptr,
@@ -649,7 +655,7 @@ void GuessNet::backward(hipStream_t st, const double* p1, const double* p2, cons
         hipLaunchKernelGGL((k_dense_t<H>), dim3(g), dim3(256), 0, st, nv, dZ.p, P + o + H * H,
                            dTmp.p, 0);
         hipLaunchKernelGGL((k_dense_t<H>), dim3(g), dim3(256), 0, st, nv, dZ.p, P + o, dH.p, 0);
-        hipLaunchKernelGGL(k_aggregate, dim3((nv + 3) / 4), dim3(256), 0, st, tsv, tval.p, dTmp.p,
+        hipLaunchKernelGGL(k_aggregate, dim3((nv + 15) / 16), dim3(256), 0, st, tsv, tval.p, dTmp.p,
                            dH.p, 1);
     }
     hipLaunchKernelGGL(k_relu_back, dim3(blocks(nh)), dim3(256), 0, st, nh, dH.p, Z[0].p, dZ.p);
