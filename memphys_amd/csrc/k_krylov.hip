@@ -978,7 +978,8 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
     std::vector<double> rr0, bb;
     residual64(st, b_dev, rr0, bb);
     for (int q = 0; q < nseg; ++q)
-        out[q].guess_rel_res = (bb[q] > 0) ? std::sqrt(rr0[q] / bb[q]) : 0.0;
+        out[q].guess_rel_res = !(std::isfinite(rr0[q]) && std::isfinite(bb[q])) ? std::nan("")
+                               : (bb[q] > 0) ? std::sqrt(rr0[q] / bb[q]) : 0.0;
     std::vector<int> total(nseg, 0), cycles(nseg, 0), rs_total(nseg, 0);
     std::vector<int> done(nseg, 0), its(nseg, 0), rst(nseg, 0);
     // FP64 BiCGSTAB from the current x~ for the systems in `todo` (budget:
@@ -1065,20 +1066,37 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                     zero_system(st, e64.p, q);
                 }
             }
-            // e is zero on the systems that took no part
+            // x~ before the correction, kept in rhs64 (free once the inner
+            // solve is done); e is zero on the systems that took no part
+            MP_HIP(hipMemcpyAsync(rhs64.p, xt.p, sizeof(double) * nrows, hipMemcpyDeviceToDevice,
+                                  st));
             hipLaunchKernelGGL(k_axpy, dim3(nbk), dim3(BLOCK), 0, st, nrows, 1.0, e64.p, xt.p);
             MP_HIP(hipGetLastError());
             residual64(st, b_dev, rr, bb);
+            // a cycle that reduced |r| by less than 10% (or left it non-finite)
+            // is undone, and FP64 BiCGSTAB continues from the x~ before it
+            std::vector<char> undo(nseg, 0);
+            bool any_undo = false;
             for (int q = 0; q < nseg; ++q) {
                 if (!act[q]) continue;
-                if (!(rr[q] < 0.81 * rr_prev[q])) {  // less than a 10% reduction of |r|
+                if (!(rr[q] < 0.81 * rr_prev[q])) {
                     fallback[q] = 1;
                     act[q] = 0;
+                    undo[q] = 1;
+                    any_undo = true;
                     continue;
                 }
                 rr_prev[q] = rr[q];
                 act[q] = rr[q] > tol2[q];
             }
+            if (any_undo)
+                for (int q = 0; q < nseg; ++q)
+                    if (undo[q]) {
+                        const size_t n = (size_t)(segs[q].nv + segs[q].border);
+                        MP_HIP(hipMemcpyAsync(xt.p + segs[q].row0, rhs64.p + segs[q].row0,
+                                              sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                        rr[q] = rr_prev[q];
+                    }
         }
         std::vector<char> fb(nseg, 0);
         std::vector<int> budget(nseg, 0);
@@ -1115,16 +1133,21 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
         o.iterations = total[q];
         o.restarts = rs_total[q];
         o.outer_cycles = cycles[q];
+        // a non-finite right-hand side or solution reports NaN, not converged
+        const bool finite = std::isfinite(rr[q]) && std::isfinite(bb[q]);
         o.abs_res = std::sqrt(rr[q]);
-        o.rel_res = (bb[q] > 0) ? std::sqrt(rr[q] / bb[q]) : 0.0;
-        o.converged = (o.rel_res <= tol) ? 1 : 0;
+        o.rel_res = !finite ? std::nan("") : (bb[q] > 0) ? std::sqrt(rr[q] / bb[q]) : 0.0;
+        o.converged = (finite && o.rel_res <= tol) ? 1 : 0;
         o.alpha = alpha[q];
         agg.iterations = std::max(agg.iterations, o.iterations);
         agg.restarts += o.restarts;
         agg.outer_cycles = std::max(agg.outer_cycles, o.outer_cycles);
         agg.converged &= o.converged;
-        agg.rel_res = std::max(agg.rel_res, o.rel_res);
-        agg.guess_rel_res = std::max(agg.guess_rel_res, o.guess_rel_res);
+        // the worst system; NaN if any is NaN
+        agg.rel_res = std::isnan(o.rel_res) ? o.rel_res : std::max(agg.rel_res, o.rel_res);
+        agg.guess_rel_res = std::isnan(o.guess_rel_res)
+                                ? o.guess_rel_res
+                                : std::max(agg.guess_rel_res, o.guess_rel_res);
         rr_all += rr[q];
         bb_all += bb[q];
         total_iterations += o.iterations;

@@ -114,6 +114,33 @@ def test_ppe_solve_mixed_matches_superlu(ctx, cube3d_small):
     _ppe_case(ctx, cube3d_small, M.MP_PREC_MIXED)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_ppe_solve_nonfinite_rhs_is_not_converged(ctx, cavity2d, prec):
+    """A NaN in the right-hand side reports a NaN residual and no convergence
+    (not a zero residual); the same solver then solves a finite system."""
+    import memphys_amd as M
+    import oracle as O
+    ctx.upload(cavity2d)
+    w = O.stencil_weights(cavity2d, 3, 3)[:4]
+    ctx.set_weights(*w)
+    p_dir = np.zeros(cavity2d.nv, dtype=np.uint8)
+    n = ctx.solver_init(p_dir, 0.0, 0.0, 1.0, tol=1e-10, n_iter=2000,
+                        precision=M.MP_PREC_MIXED if prec else M.MP_PREC_FP64)
+    rng = np.random.default_rng(12)
+    b = rng.standard_normal(n)
+    b[-1] = 0.0
+    bad = b.copy()
+    bad[n // 3] = np.nan
+    _print("case: NaN right-hand side, precision", prec)
+    ctx.solve(bad, np.zeros(n))
+    st = ctx.stats()
+    _print("iters", st.iterations, "rel_res", st.rel_res, "converged", st.converged)
+    assert st.converged == 0 and np.isnan(st.rel_res)
+    x, _ = ctx.solve(b, np.zeros(n))
+    st = ctx.stats()
+    assert st.converged == 1 and st.rel_res <= 1e-10 and np.all(np.isfinite(x))
+
+
 def _fs_parity(ctx, cloud, steps, lid_axis, rho, mu, dt, precision, precond=0):
     import memphys_amd as M
     import oracle as O
