@@ -455,7 +455,8 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.n, 0),
-            "kernel": "k_spmv_alpha/k_spmv_omega<float,double> (PPE SpMV, FP32 operator)",
+            "kernel": ("k_spmv_alpha/k_spmv_omega<float,double,float> (PPE SpMV, FP32 "
+                       "operator, FP32 gathers of the V-cycle output)"),
             "bytes_per_launch": bytes_l, "mean_ms": ms, "launches": n_l,
             "launches_returned_early": all_l - n_l,
             "copy_kernel_GBps": copy_gbps, "frac_of_copy": achieved / copy_gbps,
