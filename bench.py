@@ -65,12 +65,12 @@ def measured_traffic(n: int, kernel_index: int = 0):
 
 
 def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, precond=0,
-             tune=True):
+             tune=True, krylov=0):
     u_dir, p_dir, u0 = lid_bc(cloud)
     t = time.perf_counter()
     ctx.fs_create(prm["rho"], prm["mu"], prm["dt"], 2, prm["tol"], prm["n_iter"], u_dir, u_dir,
                   u_dir, p_dir, precision=precision, inner_tol=prm["inner_tol"],
-                  check_every=prm["check_every"], precond=precond)
+                  check_every=prm["check_every"], precond=precond, krylov=krylov)
     t_create = time.perf_counter() - t
     hier = ctx.precond_info()
     log(f"[{label}] fs_create {t_create:.2f}s, hierarchy {hier}")
@@ -329,6 +329,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel instead of replaying hipGraphs (same results; "
                          "for profilers)")
+    ap.add_argument("--krylov", choices=["gmres", "bicgstab"], default="gmres",
+                    help="Krylov method of the multigrid and headline forms (the FP64 and "
+                         "mixed Jacobi forms use BiCGSTAB, the reference's)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default poll interval and multigrid layouts (for counter "
                          "runs, under which the tuning's timed solves take minutes)")
@@ -403,6 +406,7 @@ def main():
 
     forms = args.forms.split(",")
     tune = not args.no_autotune
+    kry = M.MP_KRYLOV_GMRES if args.krylov == "gmres" else M.MP_KRYLOV_BICGSTAB
     fp64 = run_form(M, ctx, cloud, prm, M.MP_PREC_FP64, args.steps, args.warmup, "fp64",
                     tune=tune)
     extra = {}
@@ -412,11 +416,11 @@ def main():
     if "multigrid" in forms:  # the headline form without the learned guess
         extra["mixed_multigrid"] = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps,
                                             args.warmup, "mixed+multigrid",
-                                            precond=M.MP_PRECOND_AMG, tune=tune)
+                                            precond=M.MP_PRECOND_AMG, tune=tune, krylov=kry)
     if "headline" in forms:
         head = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps, args.warmup,
                         "mixed+multigrid+learned", guess=guess, precond=M.MP_PRECOND_AMG,
-                        tune=tune)
+                        tune=tune, krylov=kry)
     else:
         head = fp64
     vdiff = max(np.linalg.norm(a - b) / max(np.linalg.norm(a), 1e-300)

@@ -53,6 +53,12 @@ enum mp_guess { MP_GUESS_PREVIOUS = 0, MP_GUESS_LEARNED = 1, MP_GUESS_EXTRAPOLAT
  * host from the assembled operator, applied on the device). */
 enum mp_precond { MP_PRECOND_JACOBI = 0, MP_PRECOND_AMG = 1 };
 
+/* PPE Krylov method: BiCGSTAB as Eigen's (the reference's eigen_ilu_bicgstab,
+ * solver.cpp:337-350), or restarted GMRES (the reference's hypre_ilu_gmres,
+ * solver.cpp:41-53 and 140-144, with gmres_kdim as the restart length),
+ * right-preconditioned, classical Gram-Schmidt, Givens rotations. */
+enum mp_krylov { MP_KRYLOV_BICGSTAB = 0, MP_KRYLOV_GMRES = 1 };
+
 typedef struct {
     double rho, mu, dt;
     int temporal_order;       /* 1 = Euler, 2 = AB2 after the first step */
@@ -62,6 +68,8 @@ typedef struct {
     double inner_tolerance;   /* MIXED: inner FP32 relative reduction per cycle */
     int check_every;          /* iterations launched between convergence polls (0 = default) */
     int preconditioner;       /* enum mp_precond */
+    int krylov;               /* enum mp_krylov, 0 = BiCGSTAB */
+    int restart;              /* GMRES columns per cycle, 1..31 (0 = 30) */
 } mp_fs_params;
 
 typedef struct {

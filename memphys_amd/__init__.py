@@ -26,6 +26,7 @@ MP_GX, MP_GY, MP_GZ, MP_LAP = 0, 1, 2, 3
 MP_PREC_FP64, MP_PREC_MIXED = 0, 1
 MP_GUESS_PREVIOUS, MP_GUESS_LEARNED, MP_GUESS_EXTRAPOLATE = 0, 1, 2
 MP_PRECOND_JACOBI, MP_PRECOND_AMG = 0, 1
+MP_KRYLOV_BICGSTAB, MP_KRYLOV_GMRES = 0, 1
 
 # Every symbol include/memphys_hip.h declares.
 HIP_SYMBOLS = (
@@ -50,6 +51,7 @@ class FsParams(C.Structure):
         ("temporal_order", C.c_int), ("solver_tolerance", C.c_double),
         ("n_iter", C.c_int), ("precision", C.c_int), ("inner_tolerance", C.c_double),
         ("check_every", C.c_int), ("preconditioner", C.c_int),
+        ("krylov", C.c_int), ("restart", C.c_int),
     ]
 
 
@@ -395,8 +397,9 @@ class Context:
 
     def fs_create(self, rho, mu, dt, order, tol, n_iter, u_dir, v_dir, w_dir, p_dir,
                   precision=MP_PREC_FP64, inner_tol=1e-3, check_every=8,
-                  precond=MP_PRECOND_JACOBI):
-        p = FsParams(rho, mu, dt, order, tol, n_iter, precision, inner_tol, check_every, precond)
+                  precond=MP_PRECOND_JACOBI, krylov=MP_KRYLOV_BICGSTAB, restart=0):
+        p = FsParams(rho, mu, dt, order, tol, n_iter, precision, inner_tol, check_every, precond,
+                     krylov, restart)
         wd = np.ascontiguousarray(w_dir, dtype=np.uint8) if w_dir is not None else None
         self._wd = wd
         self._check(self.lib.mp_fs_create(

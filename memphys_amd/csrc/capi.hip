@@ -808,6 +808,12 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
             return fail(c, MP_ERR_ARG, "mp_fs_create: unknown precision");
         if (prm->preconditioner != MP_PRECOND_JACOBI && prm->preconditioner != MP_PRECOND_AMG)
             return fail(c, MP_ERR_ARG, "mp_fs_create: unknown preconditioner");
+        if (prm->krylov != MP_KRYLOV_BICGSTAB && prm->krylov != MP_KRYLOV_GMRES)
+            return fail(c, MP_ERR_ARG, "mp_fs_create: unknown Krylov method");
+        if (prm->restart < 0 || prm->restart > GM_MAX)
+            return fail(c, MP_ERR_ARG,
+                        "mp_fs_create: GMRES restart length must be 1.." + std::to_string(GM_MAX) +
+                            " (0 = 30)");
         c->prm = *prm;
         for (int q = 0; q < c->dim; ++q) upload_flags(c, dirs[q], c->udir[q]);
         if (c->dim == 2) upload_flags(c, nullptr, c->udir[2]);
@@ -824,6 +830,8 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
         c->ppe->precision = prm->precision;
         c->ppe->inner_tol = (prm->inner_tolerance > 0) ? prm->inner_tolerance : 1e-3;
         c->ppe->check_every = (prm->check_every > 0) ? prm->check_every : 8;
+        c->ppe->krylov = prm->krylov;
+        c->ppe->restart_m = (prm->restart > 0) ? prm->restart : 30;
         c->ppe->spmv_timer.enabled = c->timers[0].enabled;
         c->ppe->use_graphs = c->use_graphs;
         for (int k = 0; k < 2; ++k)

@@ -550,6 +550,350 @@ __global__ __launch_bounds__(BLOCK) void k_diag_inverse(int n, const double* __r
 
 inline int nblk(int n) { return (n + BLOCK - 1) / BLOCK; }
 
+// ---- GMRES(m), right-preconditioned ----
+// Per system: Arnoldi with classical Gram-Schmidt over FP64 basis vectors V
+// (node rows and the border unknown), the Hessenberg column's Givens
+// rotations and the residual estimate |g_j| on the device (one thread per
+// system), the preconditioned directions Z_j = M v_j kept so that a cycle
+// ends with x += Z y (no further V-cycle).  A system's kernels return at
+// entry once it has stopped: converged, at its iteration cap, broken down,
+// or with a full cycle waiting for its restart.  Partials: rows 0..GM_MAX-1
+// of the work array hold the dot products, row GM_MAX the border row of A z.
+
+__global__ void k_gm_reset(int nseg, GmState* gm, GmFlags* fl, const SegCtl* ctl) {
+    const int q = blockIdx.x * 64 + threadIdx.x;
+    if (q >= nseg) return;
+    GmState* g = gm + q;
+    g->done = ctl[q].active ? 0 : 6;
+    g->stop = ctl[q].active ? 0 : 1;
+    g->maxit = ctl[q].maxit;
+    g->tol_rel = ctl[q].tol_rel;
+    g->iter = 0;
+    g->cycles = 0;
+    g->j = 0;
+    g->full = 0;
+    g->apply = 0;
+    g->napply = 0;
+    fl[q] = GmFlags{g->done, 0, 0, 0};
+}
+
+// v0 = rhs - A x (unnormalised) for the systems starting (mode 0) or
+// restarting (mode 1: a full cycle, still running); partial sums of v0.v0,
+// rhs.rhs and the border row's D^-1 x.
+template <typename TM>
+__global__ __launch_bounds__(BLOCK) void k_gm_residual(SellView sv, SegView sg,
+                                                       const TM* __restrict__ a,
+                                                       const double* __restrict__ dinv,
+                                                       const double* __restrict__ rhs,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ v0,
+                                                       double* __restrict__ partial,
+                                                       const GmState* gm, int mode) {
+    const int nbk = sg.nrows / BLOCK;
+    const int lb = xcd_block(nbk);
+    if (lb < 0) return;
+    const int q = block_seg(sg, lb);
+    const GmState* g = gm + q;
+    if (mode == 0 ? g->done != 0 : !(g->full && g->done == 0)) return;
+    const Seg S = sg.seg[q];
+    const int iend = S.row0 + S.nv;
+    const int i = lb * BLOCK + threadIdx.x;
+    const int s = wave_slice(lb);
+    double vals[3] = {0.0, 0.0, 0.0};
+    if (s < sv.nslices) {
+        double ax = sell_dot<double>(sv, a, x, s, threadIdx.x & 63);
+        if (i < iend) {
+            if (S.border) ax = ax + x[iend];
+            const double ri = rhs[i] - ax;
+            v0[i] = ri;
+            vals[0] = ri * ri;
+            vals[1] = rhs[i] * rhs[i];
+            if (S.border) vals[2] = dinv[i] * x[i];
+        }
+    }
+    block_store<3>(vals, partial, lb, nbk);
+}
+
+__global__ __launch_bounds__(FIN) void k_gm_residual_finish(SegView sg,
+                                                            const double* __restrict__ rhs,
+                                                            double* __restrict__ v0,
+                                                            const double* __restrict__ partial,
+                                                            GmState* gm, GmFlags* fl, int mode,
+                                                            int rel_to_rhs) {
+    GmState* g = gm + blockIdx.x;
+    if (mode == 0 ? g->done != 0 : !(g->full && g->done == 0)) return;
+    const Seg S = sg.seg[blockIdx.x];
+    double tot[3];
+    finish_sum<3>(partial, sg.nrows / BLOCK, S.blk0, S.nblk, tot);
+    if (threadIdx.x != 0) return;
+    double rr = tot[0], bb = tot[1];
+    if (S.border) {
+        const int ib = S.row0 + S.nv;
+        const double rn = rhs[ib] - tot[2];
+        v0[ib] = rn;
+        rr += rn * rn;
+        bb += rhs[ib] * rhs[ib];
+    }
+    if (mode == 0) {
+        g->bb = bb;
+        g->tol2 = g->tol_rel * g->tol_rel * (rel_to_rhs ? bb : rr);
+        if (rel_to_rhs && bb == 0.0) g->done = 4;  // zero right-hand side: x = 0
+    } else {
+        g->cycles += 1;
+    }
+    g->g[0] = sqrt(rr);
+    g->rr = rr;
+    g->inv_eta = 1.0 / g->g[0];
+    g->j = 0;
+    g->full = 0;
+    g->apply = 0;
+    if (g->done == 0) {
+        if (!(rr <= 1.0e300))
+            g->done = 5;
+        else if (rr <= g->tol2)
+            g->done = 1;
+    }
+    g->stop = g->done != 0;
+    fl[blockIdx.x] = GmFlags{g->done, 0, g->iter, g->cycles};
+}
+
+// w = A z: node rows (plus the border unknown's column), partial sums of the
+// border row sum D^-1 z; zg: the gathered copy of z (FP32 when z holds FP32
+// values, the V-cycle's output).
+template <typename TM, typename TX>
+__global__ __launch_bounds__(BLOCK) void k_gm_spmv(SellView sv, SegView sg,
+                                                   const TM* __restrict__ a,
+                                                   const double* __restrict__ dinv,
+                                                   const double* __restrict__ z,
+                                                   const TX* __restrict__ zg,
+                                                   double* __restrict__ w,
+                                                   double* __restrict__ partial,
+                                                   const GmState* gm) {
+    const int nbk = sg.nrows / BLOCK;
+    const int lb = xcd_block(nbk);
+    if (lb < 0) return;
+    const int q = block_seg(sg, lb);
+    if (gm[q].stop) return;
+    const Seg S = sg.seg[q];
+    const int iend = S.row0 + S.nv;
+    const int i = lb * BLOCK + threadIdx.x;
+    const int s = wave_slice(lb);
+    double vals[1] = {0.0};
+    if (s < sv.nslices) {
+        double y = sell_dot<double>(sv, a, zg, s, threadIdx.x & 63);
+        if (i < iend) {
+            if (S.border) y = y + z[iend];
+            w[i] = y;
+            if (S.border) vals[0] = dinv[i] * (double)zg[i];
+        }
+    }
+    block_store<1>(vals, partial + (size_t)GM_MAX * nbk, lb, nbk);
+}
+
+// partial sums of v_k . w over the node rows, k < nk (the border row is added
+// by the finishing kernel once w's border entry is known)
+__global__ __launch_bounds__(BLOCK) void k_gm_dots(SegView sg, const double* __restrict__ w,
+                                                   const double* __restrict__ V, size_t ld,
+                                                   int nk, double* __restrict__ partial,
+                                                   const GmState* gm) {
+    const int nbk = sg.nrows / BLOCK;
+    const int lb = blockIdx.x;
+    const int q = block_seg(sg, lb);
+    if (gm[q].stop) return;
+    const Seg S = sg.seg[q];
+    const int i = lb * BLOCK + threadIdx.x;
+    const bool row = i < S.row0 + S.nv;
+    const double wi = row ? w[i] : 0.0;
+    double vals[GM_MAX];
+#pragma unroll
+    for (int k = 0; k < GM_MAX; ++k) vals[k] = (row && k < nk) ? V[k * ld + i] * wi : 0.0;
+    __shared__ double sh[GM_MAX][BLOCK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < GM_MAX; ++k)
+        if (k < nk) {
+            double v = vals[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) sh[k][wid] = v;
+        }
+    __syncthreads();
+    if ((int)threadIdx.x < nk) {
+        double t = sh[threadIdx.x][0];
+        for (int u = 1; u < BLOCK / 64; ++u) t += sh[threadIdx.x][u];
+        partial[(size_t)threadIdx.x * nbk + lb] = t;
+    }
+}
+
+// Hessenberg column: h_k = sum of the partials of v_k . w plus the border
+// terms; w's border entry from its partial sums.  Thread (k, sub) of the
+// block sums every 32nd block partial of row k, then a 32-lane shuffle tree.
+__global__ __launch_bounds__(FIN) void k_gm_finish_h(SegView sg, double* __restrict__ w,
+                                                     const double* __restrict__ V, size_t ld,
+                                                     int nk, const double* __restrict__ partial,
+                                                     GmState* gm) {
+    static_assert(FIN == 32 * (GM_MAX + 1), "one 32-thread group per partial row");
+    GmState* g = gm + blockIdx.x;
+    if (g->stop) return;
+    const Seg S = sg.seg[blockIdx.x];
+    const int nb = sg.nrows / BLOCK;
+    const int k = threadIdx.x >> 5, sub = threadIdx.x & 31;
+    __shared__ double tot[GM_MAX + 1];
+    double acc = 0.0;
+    if (k < nk || k == GM_MAX)
+        for (int b = sub; b < S.nblk; b += 32) acc += partial[(size_t)k * nb + S.blk0 + b];
+    for (int off = 16; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (sub == 0) tot[k] = acc;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int ib = S.row0 + S.nv;
+    const double wb = S.border ? tot[GM_MAX] : 0.0;
+    if (S.border) w[ib] = wb;
+    for (int kk = 0; kk < nk; ++kk)
+        g->h[kk] = S.border ? tot[kk] + V[kk * ld + ib] * wb : tot[kk];
+}
+
+// w -= sum_k h_k v_k over the node rows and the border unknown; partial sums
+// of the new w.w
+__global__ __launch_bounds__(BLOCK) void k_gm_orth(SegView sg, double* __restrict__ w,
+                                                   const double* __restrict__ V, size_t ld,
+                                                   int nk, double* __restrict__ partial,
+                                                   const GmState* gm) {
+    const int nbk = sg.nrows / BLOCK;
+    const int lb = blockIdx.x;
+    const int q = block_seg(sg, lb);
+    const GmState* g = gm + q;
+    if (g->stop) return;
+    const Seg S = sg.seg[q];
+    const int i = lb * BLOCK + threadIdx.x;
+    double vals[1] = {0.0};
+    if (i < S.row0 + S.nv + S.border) {
+        double wi = w[i];
+        for (int k = 0; k < nk; ++k) wi = fma(-g->h[k], V[k * ld + i], wi);
+        w[i] = wi;
+        vals[0] = wi * wi;
+    }
+    block_store<1>(vals, partial, lb, nbk);
+}
+
+// Givens rotation (c, s) with c a + s b = r, -s a + c b = 0
+__device__ __forceinline__ void gm_givens(double a, double b, double& c, double& s, double& r) {
+    if (b == 0.0) {
+        c = 1.0;
+        s = 0.0;
+        r = a;
+    } else {
+        r = hypot(a, b);
+        c = a / r;
+        s = b / r;
+    }
+}
+
+// h_{j+1,j} = |w|; the column's rotations, the new rotation and the residual
+// estimate; then the stopping tests.
+__global__ __launch_bounds__(FIN) void k_gm_finish_norm(SegView sg,
+                                                        const double* __restrict__ partial,
+                                                        GmState* gm, GmFlags* fl, int m) {
+    GmState* g = gm + blockIdx.x;
+    if (g->stop) return;
+    const Seg S = sg.seg[blockIdx.x];
+    double tot[1];
+    finish_sum<1>(partial, sg.nrows / BLOCK, S.blk0, S.nblk, tot);
+    if (threadIdx.x != 0) return;
+    const int j = g->j;
+    const double eta = sqrt(tot[0]);
+    double* h = g->h;
+    for (int i = 0; i < j; ++i) {
+        const double a = h[i], b = h[i + 1];
+        h[i] = g->cs[i] * a + g->sn[i] * b;
+        h[i + 1] = -g->sn[i] * a + g->cs[i] * b;
+    }
+    double c, s, r;
+    gm_givens(h[j], eta, c, s, r);
+    g->cs[j] = c;
+    g->sn[j] = s;
+    h[j] = r;
+    for (int i = 0; i <= j; ++i) g->R[j * GM_MAX + i] = h[i];
+    g->g[j + 1] = -s * g->g[j];
+    g->g[j] = c * g->g[j];
+    g->j = j + 1;
+    g->iter += 1;
+    const double res2 = g->g[j + 1] * g->g[j + 1];
+    g->rr = res2;
+    g->inv_eta = 1.0 / eta;
+    if (!(res2 <= 1.0e300) || !(fabs(r) <= 1.0e300))
+        g->done = 5;
+    else if (res2 <= g->tol2)
+        g->done = 1;
+    else if (r == 0.0)
+        g->done = 5;  // singular Hessenberg column
+    else if (g->iter >= g->maxit)
+        g->done = 2;
+    else if (g->j >= m)
+        g->full = 1;
+    g->stop = (g->done != 0 || g->full) ? 1 : 0;
+    fl[blockIdx.x] = GmFlags{g->done, g->full, g->iter, g->cycles};
+}
+
+// v_next = w / |w| over the node rows and the border unknown, and the
+// V-cycle's FP32 input (d0 v, or v without the scaling) over the node rows
+__global__ __launch_bounds__(BLOCK) void k_gm_scale(SegView sg, const double* __restrict__ w,
+                                                    double* __restrict__ vn,
+                                                    const float* __restrict__ d0,
+                                                    float* __restrict__ xs, const GmState* gm) {
+    const int q = block_seg(sg, blockIdx.x);
+    const GmState* g = gm + q;
+    if (g->stop) return;
+    const Seg S = sg.seg[q];
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int iend = S.row0 + S.nv;
+    if (i >= iend + S.border) return;
+    const double v = w[i] * g->inv_eta;
+    vn[i] = v;
+    if (xs && i < iend) xs[i] = d0 ? d0[i] * (float)v : (float)v;
+}
+
+// end of a cycle: y = R^-1 g over the cycle's columns (back substitution)
+__global__ void k_gm_solve(int nseg, GmState* gm, GmFlags* fl) {
+    const int q = blockIdx.x * 64 + threadIdx.x;
+    if (q >= nseg) return;
+    GmState* g = gm + q;
+    const int j = g->j;
+    g->apply = 0;
+    g->napply = 0;
+    g->j = 0;  // the cycle's columns are consumed here, once
+    if (j == 0 || g->done >= 4) return;
+    bool finite = true;
+    for (int i = j - 1; i >= 0; --i) {
+        double t = g->g[i];
+        for (int k = i + 1; k < j; ++k) t -= g->R[k * GM_MAX + i] * g->y[k];
+        g->y[i] = t / g->R[i * GM_MAX + i];
+        finite = finite && fabs(g->y[i]) <= 1.0e300;
+    }
+    if (!finite) {
+        g->done = 5;
+        g->stop = 1;
+        fl[q].done = 5;
+        return;
+    }
+    g->apply = 1;
+    g->napply = j;
+}
+
+// x += sum_k y_k z_k over the node rows and the border unknown
+__global__ __launch_bounds__(BLOCK) void k_gm_update_x(SegView sg, double* __restrict__ x,
+                                                       const double* __restrict__ Z, size_t ld,
+                                                       const GmState* gm) {
+    const int q = block_seg(sg, blockIdx.x);
+    const GmState* g = gm + q;
+    if (!g->apply) return;
+    const Seg S = sg.seg[q];
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= S.row0 + S.nv + S.border) return;
+    double xi = x[i];
+    for (int k = 0; k < g->napply; ++k) xi = fma(g->y[k], Z[k * ld + i], xi);
+    x[i] = xi;
+}
+
 }  // namespace
 
 void launch_fill(hipStream_t s, double* x, double v, int n) {
@@ -606,6 +950,7 @@ PpeSolver::~PpeSolver() {
     release_graphs();
     if (host_sc) (void)hipHostFree(host_sc);
     if (host_ctl) (void)hipHostFree(host_ctl);
+    if (host_gmf) (void)hipHostFree(host_gmf);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
     spmv_timer.release();
@@ -764,13 +1109,15 @@ double PpeSolver::spmv_bytes(int fp32) const {
     // per entry: value (8 B, or 4 B for the FP32 operator) + column (4 B; the
     // FP32 operator: 2 B in slices with 16-bit columns, plus their window
     // bases); per row: gathered x once (FP32 with the V-cycle, else FP64),
-    // v written, r0 read (FP64), length, D^-1 for the border row
+    // the product written, r0 read (BiCGSTAB's fused dot; FP64), length,
+    // D^-1 for the border row
     const double vb = fp32 ? 4.0 : 8.0;
     const double n16 = fp32 ? (double)nnz16 : 0.0;
     const double xb = amg ? 4.0 : 8.0;  // the V-cycle's output is gathered in FP32
+    const double vec = (krylov == 1 ? 1.0 : 2.0) + (bordered ? 1.0 : 0.0);
     return (double)nnz * vb + 2.0 * n16 + 4.0 * ((double)nnz - n16) +
            (fp32 ? (double)nslices16 * 4.0 * CW_COUNT : 0.0) +
-           (double)nv * (xb + 8.0 * (bordered ? 3.0 : 2.0) + 4.0);
+           (double)nv * (xb + 8.0 * vec + 4.0);
 }
 
 // y = M^-1 p with the multigrid V-cycle; without it the direction itself.
@@ -965,6 +1312,138 @@ double* PpeSolver::residual_of(hipStream_t st, const double* b_dev, const double
     return tmp.p;
 }
 
+void PpeSolver::gmres_alloc(hipStream_t st, int m) {
+    if (gm_cols == m && gm_d.p) return;
+    const size_t ld = (size_t)nrows;
+    gm_v.alloc(ld * (m + 1));
+    MP_HIP(hipMemsetAsync(gm_v.p, 0, sizeof(double) * ld * (m + 1), st));
+    if (amg) {
+        gm_z.alloc(ld * m);
+        MP_HIP(hipMemsetAsync(gm_z.p, 0, sizeof(double) * ld * m, st));
+    }
+    gm_w.alloc(ld);
+    MP_HIP(hipMemsetAsync(gm_w.p, 0, sizeof(double) * ld, st));
+    gm_partial.alloc((size_t)(GM_MAX + 1) * (nrows / BLOCK));
+    gm_d.alloc(nseg);
+    MP_HIP(hipMemsetAsync(gm_d.p, 0, sizeof(GmState) * nseg, st));
+    gmf_d.alloc(nseg);
+    if (host_gmf) (void)hipHostFree(host_gmf);
+    host_gmf = nullptr;
+    MP_HIP(hipHostMalloc((void**)&host_gmf, 2 * sizeof(GmFlags) * nseg, 0));
+    gm_cols = m;
+}
+
+// Iteration k of a GMRES cycle: z_k = M v_k, w = A z_k, Gram-Schmidt against
+// v_0..v_k, the Hessenberg column's rotations, v_{k+1} = w / |w|.
+template <typename TM>
+void PpeSolver::gmres_iteration(hipStream_t st, int k, int m, const TM* a, const double* di) {
+    const int nbk = nrows / BLOCK;
+    const size_t ld = (size_t)nrows;
+    double* vk = gm_v.p + ld * k;
+    double* vk1 = gm_v.p + ld * (k + 1);
+    const double* zk = vk;
+    if (amg) {
+        double* zo = gm_z.p + ld * k;
+        // one system: its V-cycle stops with it; a batch runs it while any
+        // system iterates (the others ignore the result)
+        amg->apply(st, vk, zo, nseg == 1 ? segs[0].border : 1,
+                   nseg == 1 ? &gm_d.p[0].stop : nullptr, true, y32.p);
+        zk = zo;
+    }
+    spmv_timer.begin(st);
+    if (amg)
+        hipLaunchKernelGGL((k_gm_spmv<TM, float>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
+                           view_of<TM>(), sg, a, di, zk, (const float*)y32.p, gm_w.p,
+                           gm_partial.p, gm_d.p);
+    else
+        hipLaunchKernelGGL((k_gm_spmv<TM, double>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
+                           view_of<TM>(), sg, a, di, zk, zk, gm_w.p, gm_partial.p, gm_d.p);
+    spmv_timer.end(st);
+    hipLaunchKernelGGL(k_gm_dots, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_w.p, gm_v.p, ld, k + 1,
+                       gm_partial.p, gm_d.p);
+    hipLaunchKernelGGL(k_gm_finish_h, dim3(nseg), dim3(FIN), 0, st, sg, gm_w.p, gm_v.p, ld, k + 1,
+                       gm_partial.p, gm_d.p);
+    hipLaunchKernelGGL(k_gm_orth, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_w.p, gm_v.p, ld, k + 1,
+                       gm_partial.p, gm_d.p);
+    hipLaunchKernelGGL(k_gm_finish_norm, dim3(nseg), dim3(FIN), 0, st, sg, gm_partial.p, gm_d.p,
+                       gmf_d.p, m);
+    hipLaunchKernelGGL(k_gm_scale, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_w.p, vk1,
+                       amg ? amg->level0_dinv() : nullptr, amg ? amg->prescaled_input() : nullptr,
+                       gm_d.p);
+    MP_HIP(hipGetLastError());
+}
+
+template <typename TM>
+void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const double* rhs,
+                          double* x, int rel_to_rhs, std::vector<int>& done,
+                          std::vector<int>& iters, std::vector<int>& restarts) {
+    const int nbk = nrows / BLOCK;
+    const int m = std::max(1, std::min(GM_MAX, restart_m));
+    gmres_alloc(st, m);
+    const size_t ld = (size_t)nrows;
+    const float* d0 = amg ? amg->level0_dinv() : nullptr;
+    float* xs = amg ? amg->prescaled_input() : nullptr;
+    MP_HIP(hipMemcpyAsync(ctl_d.p, host_ctl, sizeof(SegCtl) * nseg, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_gm_reset, dim3((nseg + 63) / 64), dim3(64), 0, st, nseg, gm_d.p, gmf_d.p,
+                       ctl_d.p);
+    // v0 = (rhs - A x) / |rhs - A x| for the systems taking part (mode 0), or
+    // for those whose full cycle restarts (mode 1)
+    auto start = [&](int mode) {
+        hipLaunchKernelGGL((k_gm_residual<TM>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
+                           view_of<TM>(), sg, a, di, rhs, x, gm_v.p, gm_partial.p, gm_d.p, mode);
+        hipLaunchKernelGGL(k_gm_residual_finish, dim3(nseg), dim3(FIN), 0, st, sg, rhs, gm_v.p,
+                           gm_partial.p, gm_d.p, gmf_d.p, mode, rel_to_rhs);
+        hipLaunchKernelGGL(k_gm_scale, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_v.p, gm_v.p, d0, xs,
+                           gm_d.p);
+        MP_HIP(hipGetLastError());
+    };
+    start(0);
+    const int chunk = std::max(1, check_every);
+    GmFlags* hf[2] = {host_gmf, host_gmf + nseg};
+    auto any_running = [&](const GmFlags* f) {
+        for (int q = 0; q < nseg; ++q)
+            if (f[q].done == 0 && !f[q].full) return true;
+        return false;
+    };
+    while (true) {
+        // one cycle: chunks of iterations, the flags polled one chunk behind
+        int slot = 0;
+        bool have_prev = false;
+        for (int j = 0; j < m;) {
+            const int c = std::min(chunk, m - j);
+            for (int k = j; k < j + c; ++k) gmres_iteration<TM>(st, k, m, a, di);
+            j += c;
+            MP_HIP(hipMemcpyAsync(hf[slot], gmf_d.p, sizeof(GmFlags) * nseg,
+                                  hipMemcpyDeviceToHost, st));
+            MP_HIP(hipEventRecord(ev[slot], st));
+            if (have_prev) {
+                MP_HIP(hipEventSynchronize(ev[slot ^ 1]));
+                if (!any_running(hf[slot ^ 1])) break;
+            }
+            have_prev = true;
+            slot ^= 1;
+        }
+        hipLaunchKernelGGL(k_gm_solve, dim3((nseg + 63) / 64), dim3(64), 0, st, nseg, gm_d.p,
+                           gmf_d.p);
+        hipLaunchKernelGGL(k_gm_update_x, dim3(nbk), dim3(BLOCK), 0, st, sg, x,
+                           amg ? gm_z.p : gm_v.p, ld, gm_d.p);
+        MP_HIP(hipGetLastError());
+        MP_HIP(hipMemcpyAsync(host_gmf, gmf_d.p, sizeof(GmFlags) * nseg, hipMemcpyDeviceToHost,
+                              st));
+        MP_HIP(hipStreamSynchronize(st));
+        bool restart = false;
+        for (int q = 0; q < nseg; ++q) restart |= host_gmf[q].full && host_gmf[q].done == 0;
+        if (!restart) break;
+        start(1);
+    }
+    for (int q = 0; q < nseg; ++q) {
+        if (!host_ctl[q].active) continue;
+        done[q] = host_gmf[q].done;
+        iters[q] = host_gmf[q].iter;
+        restarts[q] = host_gmf[q].cycles;
+    }
+}
+
 void PpeSolver::zero_system(hipStream_t st, double* x, int q) {
     launch_fill(st, x + segs[q].row0, 0.0, segs[q].nv + segs[q].border);
 }
@@ -997,8 +1476,11 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                 any |= todo[q] != 0;
             }
             if (!any) break;
-            run_bicgstab<double, double>(st, aval.p, dinv.p, b_dev, xt.p, r.p, r0.p, p.p, v.p,
-                                         s.p, t.p, 1, done, its, rst);
+            if (krylov == 1)
+                run_gmres<double>(st, aval.p, dinv.p, b_dev, xt.p, 1, done, its, rst);
+            else
+                run_bicgstab<double, double>(st, aval.p, dinv.p, b_dev, xt.p, r.p, r0.p, p.p,
+                                             v.p, s.p, t.p, 1, done, its, rst);
             bool check = false;
             for (int q = 0; q < nseg; ++q) {
                 if (!todo[q]) continue;
@@ -1053,8 +1535,11 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
             MP_HIP(hipMemcpyAsync(rhs64.p, tmp.p, sizeof(double) * nrows, hipMemcpyDeviceToDevice,
                                   st));
             launch_fill(st, e64.p, 0.0, nrows);
-            run_bicgstab<float, double>(st, a32k, dinv.p, rhs64.p, e64.p, r.p, r0.p, p.p, v.p,
-                                        s.p, t.p, 1, done, its, rst);
+            if (krylov == 1)
+                run_gmres<float>(st, a32k, dinv.p, rhs64.p, e64.p, 1, done, its, rst);
+            else
+                run_bicgstab<float, double>(st, a32k, dinv.p, rhs64.p, e64.p, r.p, r0.p, p.p,
+                                            v.p, s.p, t.p, 1, done, its, rst);
             for (int q = 0; q < nseg; ++q) {
                 if (!act[q]) continue;
                 total[q] += its[q];

@@ -29,6 +29,26 @@ struct KernelTimer {
     void release();
 };
 
+// GMRES(m) state of one system, in device memory (m <= GM_MAX columns per
+// cycle).  R holds the Givens-rotated Hessenberg matrix column by column
+// (column j at R[j * GM_MAX], rows 0..j), g the rotated residual vector.
+constexpr int GM_MAX = 31;
+struct GmState {
+    double g[GM_MAX + 1], cs[GM_MAX], sn[GM_MAX], h[GM_MAX + 1], y[GM_MAX];
+    double R[GM_MAX * GM_MAX];
+    double bb, tol2, tol_rel, rr, inv_eta;
+    int j;       // columns of the current cycle
+    int done;    // as KScalars::done (0 running, 1 converged, 2 cap, 4 zero rhs, 5 non-finite / breakdown, 6 inactive)
+    int full;    // the cycle holds m columns: restart pending
+    int stop;    // done != 0 or full: the iteration kernels return at entry
+    int apply;   // x += Z y pending with napply columns
+    int napply;
+    int iter, maxit, cycles;
+};
+struct GmFlags {
+    int done, full, iter, cycles;
+};
+
 // An instantiated graph of one chunk of BiCGSTAB iterations for one set of
 // operand pointers.
 struct ChunkGraph {
@@ -73,6 +93,13 @@ struct PpeSolver {
     const double* diag_inv = nullptr; // D^-1 (the learned guess's features)
     const float* scaled32 = nullptr;  // A D^-1 in FP32 (the learned guess's aggregation)
     int precond = 0;                  // enum mp_precond
+    int krylov = 0;                   // enum mp_krylov
+    int restart_m = 30;               // GMRES columns per cycle
+    DevBuf<double> gm_v, gm_z, gm_w, gm_partial;  // basis, preconditioned basis, work, partials
+    DevBuf<GmState> gm_d;             // [nseg]
+    DevBuf<GmFlags> gmf_d;            // [nseg]
+    GmFlags* host_gmf = nullptr;      // pinned, 2 x nseg
+    int gm_cols = 0;                  // columns allocated
     std::unique_ptr<AmgDevice> amg;
     DevBuf<double> partial;
     DevBuf<KScalars> sc;              // [nseg]
@@ -116,6 +143,16 @@ struct PpeSolver {
     void run_bicgstab(hipStream_t s, const TM* aval_t, const T* dinv_t, const T* rhs, T* x,
                       T* rr, T* rr0, T* pp, T* vv, T* ss, T* tt, int rel_to_rhs,
                       std::vector<int>& done, std::vector<int>& iters, std::vector<int>& restarts);
+    // GMRES(restart_m) on operator values aval_t (FP64 vectors), right-
+    // preconditioned by the V-cycle when there is one; same controls and
+    // outcome codes as run_bicgstab (restarts = GMRES cycles after the first).
+    template <typename TM>
+    void run_gmres(hipStream_t s, const TM* aval_t, const double* dinv_t, const double* rhs,
+                   double* x, int rel_to_rhs, std::vector<int>& done, std::vector<int>& iters,
+                   std::vector<int>& restarts);
+    template <typename TM>
+    void gmres_iteration(hipStream_t s, int k, int m, const TM* aval_t, const double* dinv_t);
+    void gmres_alloc(hipStream_t s, int m);
     void residual64(hipStream_t s, const double* b_dev, std::vector<double>& rr,
                     std::vector<double>& bb);
     const double* precondition(hipStream_t s, const double* in, double* out, float* out32);

@@ -576,6 +576,10 @@ void FRACTIONAL_STEP_1::create(POINTS& points, CLOUD& cloud, PARAMETERS& paramet
     prm.precision = parameters.precision;
     prm.inner_tolerance = 1e-3;
     prm.check_every = 8;
+    if (parameters.solver_type == "hypre_ilu_gmres") {  // GMRES(gmres_kdim), solver.cpp:140-144
+        prm.krylov = MP_KRYLOV_GMRES;
+        prm.restart = std::max(1, std::min(parameters.gmres_kdim, 31));
+    }
     clock_t t1 = clock();
     check(ctx, mp_fs_create(ctx, &prm, fl[0].data(), fl[1].data(),
                             parameters.dimension == 3 ? fl[2].data() : nullptr, fl[3].data()),

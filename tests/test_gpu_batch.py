@@ -1,6 +1,7 @@
 """GPU: batched independent cavities in one context against the same problems
 solved one context each (and one of them against the oracle), with the
-Jacobi-preconditioned and the multigrid-preconditioned pressure solve.
+Jacobi-preconditioned and the multigrid-preconditioned pressure solve, by
+BiCGSTAB and by restarted GMRES.
 
 Each problem of a batch owns whole row blocks and its own Krylov scalars;
 its reductions run over its own blocks in the same order as when it is alone,
@@ -29,14 +30,14 @@ def _problems():
     return out
 
 
-def _run_single(cloud, re, dt, steps, precision, precond=0):
+def _run_single(cloud, re, dt, steps, precision, precond=0, krylov=0):
     import memphys_amd as M
     with M.Context(0) as ctx:
         ctx.upload(cloud)
         ctx.weights(3, 3, download=False)
         u_dir, p_dir, u0 = M.lid_conditions(cloud)
         ctx.fs_create(1.0, 1.0 / re, dt, 2, 1e-10, 20000, u_dir, u_dir, None, p_dir,
-                      precision=precision, precond=precond)
+                      precision=precision, precond=precond, krylov=krylov)
         ctx.set_field(M.MP_U, u0)
         ctx.set_field(M.MP_V, np.zeros(cloud.nv))
         ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
@@ -47,14 +48,14 @@ def _run_single(cloud, re, dt, steps, precision, precond=0):
         return ([ctx.get_field(w) for w in (M.MP_U, M.MP_V, M.MP_P)], iters)
 
 
-def _run_batch(probs, steps, precision, precond=0):
+def _run_batch(probs, steps, precision, precond=0, krylov=0):
     import memphys_amd as M
     with M.Context(0) as ctx:
         cat = ctx.upload_batch([p[0] for p in probs])
         ctx.weights(3, 3, download=False)
         u_dir, p_dir, u0 = M.lid_conditions(cat)
         ctx.fs_create(1.0, 0.01, 1e-4, 2, 1e-10, 20000, u_dir, u_dir, None, p_dir,
-                      precision=precision, precond=precond)
+                      precision=precision, precond=precond, krylov=krylov)
         ctx.batch_params([1.0] * len(probs), [1.0 / p[1] for p in probs], [p[2] for p in probs])
         assert ctx.n_sys == cat.nv + len(probs)  # one border unknown per cavity
         ctx.set_field(M.MP_U, u0)
@@ -75,16 +76,16 @@ def _run_batch(probs, steps, precision, precond=0):
     return out
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_batch_reproduces_single_problems(precision):
+@pytest.mark.parametrize("precision,krylov", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_batch_reproduces_single_problems(precision, krylov):
     probs = _problems()
     steps = 5
-    batch = _run_batch(probs, steps, precision)
+    batch = _run_batch(probs, steps, precision, krylov=krylov)
     for q, (cloud, re, dt) in enumerate(probs):
-        fields, iters = _run_single(cloud, re, dt, steps, precision)
+        fields, iters = _run_single(cloud, re, dt, steps, precision, krylov=krylov)
         bf, bi = batch[q]
-        _print(f"precision {precision} problem {q} nv {cloud.nv} Re {re} dt {dt:.2e}: "
-               f"iterations single {iters} batch {bi}")
+        _print(f"precision {precision} krylov {krylov} problem {q} nv {cloud.nv} Re {re} "
+               f"dt {dt:.2e}: iterations single {iters} batch {bi}")
         assert bi == iters
         for a, b in zip(fields, bf):
             assert np.array_equal(a, b)

@@ -141,7 +141,8 @@ def test_ppe_solve_nonfinite_rhs_is_not_converged(ctx, cavity2d, prec):
     assert st.converged == 1 and st.rel_res <= 1e-10 and np.all(np.isfinite(x))
 
 
-def _fs_parity(ctx, cloud, steps, lid_axis, rho, mu, dt, precision, precond=0):
+def _fs_parity(ctx, cloud, steps, lid_axis, rho, mu, dt, precision, precond=0, krylov=0,
+               restart=0):
     import memphys_amd as M
     import oracle as O
     ctx.upload(cloud)
@@ -149,7 +150,7 @@ def _fs_parity(ctx, cloud, steps, lid_axis, rho, mu, dt, precision, precond=0):
     w = (gx, gy, gz, lap)
     u_dir, p_dir, u0 = lid_bc(cloud, lid_axis)
     ctx.fs_create(rho, mu, dt, 2, 1e-12, 20000, u_dir, u_dir, u_dir if cloud.dim == 3 else None,
-                  p_dir, precision=precision, precond=precond)
+                  p_dir, precision=precision, precond=precond, krylov=krylov, restart=restart)
     nv = cloud.nv
     ctx.set_field(M.MP_U, u0)
     ctx.set_field(M.MP_V, np.zeros(nv))
@@ -169,7 +170,8 @@ def _fs_parity(ctx, cloud, steps, lid_axis, rho, mu, dt, precision, precond=0):
             rel_err(ctx.get_field(M.MP_P), p)]
     if cloud.dim == 3:
         errs.append(rel_err(ctx.get_field(M.MP_W), ww))
-    _print(f"FS {cloud.dim}D steps {steps} prec {precision} precond {precond}: errs {errs}"
+    _print(f"FS {cloud.dim}D steps {steps} prec {precision} precond {precond} krylov {krylov}"
+           f" restart {restart}: errs {errs}"
            f" iters {st.iterations} total {st.total_iterations} rel_res {st.rel_res}")
     assert max(errs) <= 1e-8
     div = ctx.divergence()
@@ -215,6 +217,33 @@ def test_fs_3d_cube_multigrid_matches_oracle(ctx, cube3d_small):
 def test_fs_2d_cavity_multigrid_matches_oracle(ctx, cavity2d):
     import memphys_amd as M
     _fs_parity(ctx, cavity2d, 10, 1, 1.0, 0.01, 2e-4, M.MP_PREC_MIXED, M.MP_PRECOND_AMG)
+
+
+@pytest.mark.parametrize("precision,precond,restart", [(0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 1, 4)])
+def test_fs_3d_cube_gmres_matches_oracle(ctx, cube3d_small, precision, precond, restart):
+    """Restarted GMRES (FP64 or FP32-inner, Jacobi or multigrid, the default
+    30 columns or 4, which restarts many times) reaches the oracle's fields."""
+    import memphys_amd as M
+    _fs_parity(ctx, cube3d_small, 8, 2, 1.0, 0.0025, 5e-4, precision, precond,
+               krylov=M.MP_KRYLOV_GMRES, restart=restart)
+
+
+def test_gmres_multigrid_needs_fewer_vcycles_than_bicgstab(ctx, cube3d_small):
+    """With the multigrid preconditioner a GMRES iteration applies one V-cycle
+    and a BiCGSTAB iteration two: GMRES needs fewer V-cycles per solve."""
+    import memphys_amd as M
+    it_b = _fs_parity(ctx, cube3d_small, 8, 2, 1.0, 0.0025, 5e-4, M.MP_PREC_MIXED,
+                      M.MP_PRECOND_AMG)
+    it_g = _fs_parity(ctx, cube3d_small, 8, 2, 1.0, 0.0025, 5e-4, M.MP_PREC_MIXED,
+                      M.MP_PRECOND_AMG, krylov=M.MP_KRYLOV_GMRES)
+    _print("V-cycles: BiCGSTAB", 2 * it_b, "GMRES", it_g)
+    assert it_g < 2 * it_b
+
+
+def test_fs_2d_cavity_gmres_multigrid_matches_oracle(ctx, cavity2d):
+    import memphys_amd as M
+    _fs_parity(ctx, cavity2d, 10, 1, 1.0, 0.01, 2e-4, M.MP_PREC_MIXED, M.MP_PRECOND_AMG,
+               krylov=M.MP_KRYLOV_GMRES)
 
 
 @pytest.mark.parametrize("precision,precond", [(0, 0), (1, 1)])
