@@ -725,31 +725,55 @@ __global__ __launch_bounds__(BLOCK) void k_gm_dots(SegView sg, const double* __r
 }
 
 // Hessenberg column: h_k = sum of the partials of v_k . w plus the border
-// terms; w's border entry from its partial sums.  Thread (k, sub) of the
-// block sums every 32nd block partial of row k, then a 32-lane shuffle tree.
-__global__ __launch_bounds__(FIN) void k_gm_finish_h(SegView sg, double* __restrict__ w,
+// term; w's border entry from its partial sums.  One block per (system, k):
+// each sums its row of block partials and the border row (every 256th
+// partial per thread, four in flight, then a fixed reduction tree).
+constexpr int GMF = 256;  // threads of the Hessenberg-column kernel
+__device__ __forceinline__ double gm_row_sum(const double* __restrict__ p, int cnt) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int b = threadIdx.x;
+    for (; b + 3 * GMF < cnt; b += 4 * GMF) {
+        a0 += p[b];
+        a1 += p[b + GMF];
+        a2 += p[b + 2 * GMF];
+        a3 += p[b + 3 * GMF];
+    }
+    for (; b < cnt; b += GMF) a0 += p[b];
+    double acc = (a0 + a1) + (a2 + a3);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    return acc;  // per wavefront
+}
+
+__global__ __launch_bounds__(GMF) void k_gm_finish_h(SegView sg, double* __restrict__ w,
                                                      const double* __restrict__ V, size_t ld,
                                                      int nk, const double* __restrict__ partial,
                                                      GmState* gm) {
-    static_assert(FIN == 32 * (GM_MAX + 1), "one 32-thread group per partial row");
-    GmState* g = gm + blockIdx.x;
-    if (g->stop) return;
-    const Seg S = sg.seg[blockIdx.x];
+    const int q = blockIdx.x / GM_MAX, k = blockIdx.x % GM_MAX;
+    GmState* g = gm + q;
+    if (k >= nk || g->stop) return;
+    const Seg S = sg.seg[q];
     const int nb = sg.nrows / BLOCK;
-    const int k = threadIdx.x >> 5, sub = threadIdx.x & 31;
-    __shared__ double tot[GM_MAX + 1];
-    double acc = 0.0;
-    if (k < nk || k == GM_MAX)
-        for (int b = sub; b < S.nblk; b += 32) acc += partial[(size_t)k * nb + S.blk0 + b];
-    for (int off = 16; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (sub == 0) tot[k] = acc;
+    __shared__ double sh[2][GMF / 64];
+    const double dk = gm_row_sum(partial + (size_t)k * nb + S.blk0, S.nblk);
+    const double db = S.border ? gm_row_sum(partial + (size_t)GM_MAX * nb + S.blk0, S.nblk) : 0.0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][wid] = dk;
+        sh[1][wid] = db;
+    }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const int ib = S.row0 + S.nv;
-    const double wb = S.border ? tot[GM_MAX] : 0.0;
-    if (S.border) w[ib] = wb;
-    for (int kk = 0; kk < nk; ++kk)
-        g->h[kk] = S.border ? tot[kk] + V[kk * ld + ib] * wb : tot[kk];
+    double tk = sh[0][0], wb = sh[1][0];
+    for (int t = 1; t < GMF / 64; ++t) {
+        tk += sh[0][t];
+        wb += sh[1][t];
+    }
+    if (S.border) {
+        const int ib = S.row0 + S.nv;
+        if (k == 0) w[ib] = wb;
+        tk += V[k * ld + ib] * wb;
+    }
+    g->h[k] = tk;
 }
 
 // w -= sum_k h_k v_k over the node rows and the border unknown; partial sums
@@ -798,21 +822,32 @@ __global__ __launch_bounds__(FIN) void k_gm_finish_norm(SegView sg,
     const Seg S = sg.seg[blockIdx.x];
     double tot[1];
     finish_sum<1>(partial, sg.nrows / BLOCK, S.blk0, S.nblk, tot);
-    if (threadIdx.x != 0) return;
+    // the column and the previous rotations into LDS in one parallel load;
+    // the rotations themselves are sequential (thread 0)
+    __shared__ double h[GM_MAX + 1], cs[GM_MAX], sn[GM_MAX];
     const int j = g->j;
-    const double eta = sqrt(tot[0]);
-    double* h = g->h;
-    for (int i = 0; i < j; ++i) {
-        const double a = h[i], b = h[i + 1];
-        h[i] = g->cs[i] * a + g->sn[i] * b;
-        h[i + 1] = -g->sn[i] * a + g->cs[i] * b;
+    if ((int)threadIdx.x <= j) h[threadIdx.x] = g->h[threadIdx.x];
+    if ((int)threadIdx.x < j) {
+        cs[threadIdx.x] = g->cs[threadIdx.x];
+        sn[threadIdx.x] = g->sn[threadIdx.x];
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < j; ++i) {
+            const double a = h[i], b = h[i + 1];
+            h[i] = cs[i] * a + sn[i] * b;
+            h[i + 1] = -sn[i] * a + cs[i] * b;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < j) g->R[j * GM_MAX + threadIdx.x] = h[threadIdx.x];
+    if (threadIdx.x != 0) return;
+    const double eta = sqrt(tot[0]);
     double c, s, r;
     gm_givens(h[j], eta, c, s, r);
     g->cs[j] = c;
     g->sn[j] = s;
-    h[j] = r;
-    for (int i = 0; i <= j; ++i) g->R[j * GM_MAX + i] = h[i];
+    g->R[j * GM_MAX + j] = r;
     g->g[j + 1] = -s * g->g[j];
     g->g[j] = c * g->g[j];
     g->j = j + 1;
@@ -1361,8 +1396,8 @@ void PpeSolver::gmres_iteration(hipStream_t st, int k, int m, const TM* a, const
     spmv_timer.end(st);
     hipLaunchKernelGGL(k_gm_dots, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_w.p, gm_v.p, ld, k + 1,
                        gm_partial.p, gm_d.p);
-    hipLaunchKernelGGL(k_gm_finish_h, dim3(nseg), dim3(FIN), 0, st, sg, gm_w.p, gm_v.p, ld, k + 1,
-                       gm_partial.p, gm_d.p);
+    hipLaunchKernelGGL(k_gm_finish_h, dim3(nseg * GM_MAX), dim3(GMF), 0, st, sg, gm_w.p, gm_v.p,
+                       ld, k + 1, gm_partial.p, gm_d.p);
     hipLaunchKernelGGL(k_gm_orth, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_w.p, gm_v.p, ld, k + 1,
                        gm_partial.p, gm_d.p);
     hipLaunchKernelGGL(k_gm_finish_norm, dim3(nseg), dim3(FIN), 0, st, sg, gm_partial.p, gm_d.p,

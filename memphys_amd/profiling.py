@@ -4,11 +4,14 @@
         per kernel and grid size: dispatches, total ms, mean/min/max us, and the
         mean over the launches that did the work (at least half the median
         duration: Krylov launches of a converged system return at once)
-    python -m memphys_amd.profiling iteration DB
-        one preconditioned BiCGSTAB iteration of the largest system: the
-        iterations between consecutive k_update_p launches followed by a
-        multigrid launch, with a working SpMV; their median wall and busy time,
-        and the kernel list of the iteration of median wall time
+    python -m memphys_amd.profiling iteration DB [--first K] [--marker K] [--product K]
+        one preconditioned Krylov iteration of the largest system: the
+        stretches between consecutive launches of `first` (default
+        k_update_p<double>, BiCGSTAB's; k_gm_scale for GMRES) followed by a
+        `marker` launch (default k_amg_op, the V-cycle), with a working
+        `product` launch (default k_spmv_alpha; k_gm_spmv for GMRES); their
+        median wall and busy time, and the kernel list of the iteration of
+        median wall time
     python -m memphys_amd.profiling pmc DB [--top N]
         per kernel, grid size and counter: dispatches and mean value per dispatch
     python -m memphys_amd.profiling traffic FETCH_DB WRITE_DB --kernel SUBSTR
@@ -174,6 +177,9 @@ def main(argv=None):
     a.add_argument("--top", type=int, default=40)
     it = sub.add_parser("iteration")
     it.add_argument("db")
+    it.add_argument("--first", default="k_update_p<double>")
+    it.add_argument("--marker", default="k_amg_op")
+    it.add_argument("--product", default="k_spmv_alpha")
     b = sub.add_parser("pmc")
     b.add_argument("db")
     b.add_argument("--top", type=int, default=40)
@@ -188,7 +194,7 @@ def main(argv=None):
     if args.cmd == "stats":
         print(stats(args.db, args.top))
     elif args.cmd == "iteration":
-        print(iteration(args.db))
+        print(iteration(args.db, args.first, args.marker, args.product))
     elif args.cmd == "pmc":
         print(pmc(args.db, args.top))
     else:
