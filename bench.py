@@ -148,7 +148,7 @@ def stream_copy_gbps(mib: int = 1024, reps: int = 10) -> float:
     return 2.0 * (mib << 20) / (ms * 1e-3) / 1e9
 
 
-def run_batched_cavities(M, steps, seq_steps, tol, precond, nprob=64):
+def run_batched_cavities(M, steps, seq_steps, tol, precond, krylov=0, nprob=64):
     """64 independent 2D cavities (N log-uniform on [2k, 40k], Re ~ U[50, 400])
     stepped together in one context: one launch per phase serves every cavity,
     each pressure system (FP32-operator BiCGSTAB in FP64 defect correction,
@@ -163,7 +163,7 @@ def run_batched_cavities(M, steps, seq_steps, tol, precond, nprob=64):
     ctx.weights(3, 3, precision=M.MP_PREC_MIXED, download=False)
     u_dir, p_dir, u0 = M.lid_conditions(cat)
     ctx.fs_create(1.0, 0.01, 1e-4, 2, tol, 20000, u_dir, u_dir, None, p_dir,
-                  precision=M.MP_PREC_MIXED, inner_tol=1e-5, precond=precond)
+                  precision=M.MP_PREC_MIXED, inner_tol=1e-5, precond=precond, krylov=krylov)
     ctx.batch_params([1.0] * nprob, [1.0 / p[1] for p in probs], [p[2] for p in probs])
     ctx.set_field(M.MP_U, u0)
     ctx.set_field(M.MP_V, np.zeros(cat.nv))
@@ -190,7 +190,8 @@ def run_batched_cavities(M, steps, seq_steps, tol, precond, nprob=64):
     log(f"[batch] {steps} steps in {t_batch:.2f}s; slowest system {it_sum / steps:.0f} "
         f"iterations/step; max rel_res {worst:.2e}")
     # the same cavities one after another, one context each
-    pname = "multigrid V-cycle per cavity" if precond == M.MP_PRECOND_AMG else "Jacobi"
+    pname = ("GMRES(30)" if krylov == M.MP_KRYLOV_GMRES else "BiCGSTAB") + ", " + \
+        ("multigrid V-cycle per cavity" if precond == M.MP_PRECOND_AMG else "Jacobi")
     t_seq = 0.0
     for q, (cloud, re, dt) in enumerate(probs):
         with M.Context(0) as c1:
@@ -198,7 +199,8 @@ def run_batched_cavities(M, steps, seq_steps, tol, precond, nprob=64):
             c1.weights(3, 3, precision=M.MP_PREC_MIXED, download=False)
             ud, pd, uz = M.lid_conditions(cloud)
             c1.fs_create(1.0, 1.0 / re, dt, 2, tol, 20000, ud, ud, None, pd,
-                         precision=M.MP_PREC_MIXED, inner_tol=1e-5, precond=precond)
+                         precision=M.MP_PREC_MIXED, inner_tol=1e-5, precond=precond,
+                         krylov=krylov)
             c1.set_field(M.MP_U, uz)
             c1.set_field(M.MP_V, np.zeros(cloud.nv))
             c1.set_field(M.MP_P, np.zeros(c1.n_sys))
@@ -212,7 +214,7 @@ def run_batched_cavities(M, steps, seq_steps, tol, precond, nprob=64):
     return {
         "workload": (f"{nprob} 2D lid-driven cavities, N log-uniform on [2000, 40000] "
                      f"({cat.nv} nodes total), Re ~ U[50, 400], jitter +-0.25h, k=30, "
-                     f"FP32-operator BiCGSTAB ({pname}) in FP64 defect correction, "
+                     f"FP32-operator Krylov ({pname}) in FP64 defect correction, "
                      f"rel. residual {tol:g}"),
         "steps": steps,
         "cavity_steps_per_s": nprob * steps / t_batch,
@@ -330,8 +332,8 @@ def main():
                     help="launch every kernel instead of replaying hipGraphs (same results; "
                          "for profilers)")
     ap.add_argument("--krylov", choices=["gmres", "bicgstab"], default="gmres",
-                    help="Krylov method of the multigrid and headline forms (the FP64 and "
-                         "mixed Jacobi forms use BiCGSTAB, the reference's)")
+                    help="Krylov method of the multigrid and headline forms and of the batch "
+                         "(the FP64 and mixed Jacobi forms use BiCGSTAB, the reference's)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default poll interval and multigrid layouts (for counter "
                          "runs, under which the tuning's timed solves take minutes)")
@@ -491,7 +493,7 @@ def main():
     if args.batch_steps > 0:
         ctx.close()
         bp = M.MP_PRECOND_AMG if args.batch_precond == "multigrid" else M.MP_PRECOND_JACOBI
-        out["batched_cavities"] = run_batched_cavities(M, args.batch_steps, 5, args.tol, bp)
+        out["batched_cavities"] = run_batched_cavities(M, args.batch_steps, 5, args.tol, bp, kry)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(cloud, (gx, gy, gz, lap), prm,

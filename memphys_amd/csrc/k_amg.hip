@@ -290,9 +290,10 @@ void upload_mat(AmgMat& d, const mp_amg::Csr& M, int g, hipStream_t st) {
     d.val = d.val_b.p;
 }
 
-// Autotuned layout: every candidate (SELL-64x4 unless its padding more than
-// triples the stored entries; CSR with 4..64 lanes per row) is uploaded and
-// timed on a product with this matrix; the fastest stays.
+// Autotuned layout: every candidate (SELL-64x4 for rows of at most 24 entries
+// on average unless its padding more than triples the stored entries; CSR
+// with 4..64 lanes per row) is uploaded and timed on a product with this
+// matrix; the fastest stays.
 void upload_tuned(AmgMat& d, const mp_amg::Csr& M, hipStream_t st, float* xin, float* out,
                   hipEvent_t e0, hipEvent_t e1, std::string& report, const char* label) {
     std::vector<int> cand;
@@ -304,7 +305,10 @@ void upload_tuned(AmgMat& d, const mp_amg::Csr& M, hipStream_t st, float* xin, f
                 w = std::max(w, M.ptr[s * SLICE + l + 1] - M.ptr[s * SLICE + l]);
             slots += (long)sell_width4(w) * SLICE;
         }
-        if (slots <= 3 * std::max<long>(1, M.nnz())) cand.push_back(0);
+        // SELL-64x4 walks a row on one lane: only for short rows (the set-up
+        // rule's bound), where its padding stays small
+        const double avg = (double)M.nnz() / std::max(1, M.nrows);
+        if (avg <= 24.0 && slots <= 3 * std::max<long>(1, M.nnz())) cand.push_back(0);
     }
     for (int g : {4, 8, 16, 32, 64}) cand.push_back(g);
     float best_ms = 1e30f;

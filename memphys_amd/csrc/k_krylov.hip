@@ -580,13 +580,13 @@ __global__ void k_gm_reset(int nseg, GmState* gm, GmFlags* fl, const SegCtl* ctl
 // v0 = rhs - A x (unnormalised) for the systems starting (mode 0) or
 // restarting (mode 1: a full cycle, still running); partial sums of v0.v0,
 // rhs.rhs and the border row's D^-1 x.
-template <typename TM>
+template <typename TM, typename TV>
 __global__ __launch_bounds__(BLOCK) void k_gm_residual(SellView sv, SegView sg,
                                                        const TM* __restrict__ a,
                                                        const double* __restrict__ dinv,
                                                        const double* __restrict__ rhs,
                                                        const double* __restrict__ x,
-                                                       double* __restrict__ v0,
+                                                       TV* __restrict__ v0,
                                                        double* __restrict__ partial,
                                                        const GmState* gm, int mode) {
     const int nbk = sg.nrows / BLOCK;
@@ -605,7 +605,7 @@ __global__ __launch_bounds__(BLOCK) void k_gm_residual(SellView sv, SegView sg,
         if (i < iend) {
             if (S.border) ax = ax + x[iend];
             const double ri = rhs[i] - ax;
-            v0[i] = ri;
+            v0[i] = (TV)ri;
             vals[0] = ri * ri;
             vals[1] = rhs[i] * rhs[i];
             if (S.border) vals[2] = dinv[i] * x[i];
@@ -614,9 +614,11 @@ __global__ __launch_bounds__(BLOCK) void k_gm_residual(SellView sv, SegView sg,
     block_store<3>(vals, partial, lb, nbk);
 }
 
+// (v0 holds the residual rounded to the basis type; its norm is the FP64 one)
+template <typename TV>
 __global__ __launch_bounds__(FIN) void k_gm_residual_finish(SegView sg,
                                                             const double* __restrict__ rhs,
-                                                            double* __restrict__ v0,
+                                                            TV* __restrict__ v0,
                                                             const double* __restrict__ partial,
                                                             GmState* gm, GmFlags* fl, int mode,
                                                             int rel_to_rhs) {
@@ -630,7 +632,7 @@ __global__ __launch_bounds__(FIN) void k_gm_residual_finish(SegView sg,
     if (S.border) {
         const int ib = S.row0 + S.nv;
         const double rn = rhs[ib] - tot[2];
-        v0[ib] = rn;
+        v0[ib] = (TV)rn;
         rr += rn * rn;
         bb += rhs[ib] * rhs[ib];
     }
@@ -660,11 +662,11 @@ __global__ __launch_bounds__(FIN) void k_gm_residual_finish(SegView sg,
 // w = A z: node rows (plus the border unknown's column), partial sums of the
 // border row sum D^-1 z; zg: the gathered copy of z (FP32 when z holds FP32
 // values, the V-cycle's output).
-template <typename TM, typename TX>
+template <typename TM, typename TX, typename TB>
 __global__ __launch_bounds__(BLOCK) void k_gm_spmv(SellView sv, SegView sg,
                                                    const TM* __restrict__ a,
                                                    const double* __restrict__ dinv,
-                                                   const double* __restrict__ z,
+                                                   const TB* __restrict__ z,
                                                    const TX* __restrict__ zg,
                                                    double* __restrict__ w,
                                                    double* __restrict__ partial,
@@ -682,7 +684,7 @@ __global__ __launch_bounds__(BLOCK) void k_gm_spmv(SellView sv, SegView sg,
     if (s < sv.nslices) {
         double y = sell_dot<double>(sv, a, zg, s, threadIdx.x & 63);
         if (i < iend) {
-            if (S.border) y = y + z[iend];
+            if (S.border) y = y + (double)z[iend];
             w[i] = y;
             if (S.border) vals[0] = dinv[i] * (double)zg[i];
         }
@@ -692,8 +694,9 @@ __global__ __launch_bounds__(BLOCK) void k_gm_spmv(SellView sv, SegView sg,
 
 // partial sums of v_k . w over the node rows, k < nk (the border row is added
 // by the finishing kernel once w's border entry is known)
+template <typename TV>
 __global__ __launch_bounds__(BLOCK) void k_gm_dots(SegView sg, const double* __restrict__ w,
-                                                   const double* __restrict__ V, size_t ld,
+                                                   const TV* __restrict__ V, size_t ld,
                                                    int nk, double* __restrict__ partial,
                                                    const GmState* gm) {
     const int nbk = sg.nrows / BLOCK;
@@ -706,7 +709,7 @@ __global__ __launch_bounds__(BLOCK) void k_gm_dots(SegView sg, const double* __r
     const double wi = row ? w[i] : 0.0;
     double vals[GM_MAX];
 #pragma unroll
-    for (int k = 0; k < GM_MAX; ++k) vals[k] = (row && k < nk) ? V[k * ld + i] * wi : 0.0;
+    for (int k = 0; k < GM_MAX; ++k) vals[k] = (row && k < nk) ? (double)V[k * ld + i] * wi : 0.0;
     __shared__ double sh[GM_MAX][BLOCK / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -744,8 +747,9 @@ __device__ __forceinline__ double gm_row_sum(const double* __restrict__ p, int c
     return acc;  // per wavefront
 }
 
+template <typename TV>
 __global__ __launch_bounds__(GMF) void k_gm_finish_h(SegView sg, double* __restrict__ w,
-                                                     const double* __restrict__ V, size_t ld,
+                                                     const TV* __restrict__ V, size_t ld,
                                                      int nk, const double* __restrict__ partial,
                                                      GmState* gm) {
     const int q = blockIdx.x / GM_MAX, k = blockIdx.x % GM_MAX;
@@ -771,15 +775,16 @@ __global__ __launch_bounds__(GMF) void k_gm_finish_h(SegView sg, double* __restr
     if (S.border) {
         const int ib = S.row0 + S.nv;
         if (k == 0) w[ib] = wb;
-        tk += V[k * ld + ib] * wb;
+        tk += (double)V[k * ld + ib] * wb;
     }
     g->h[k] = tk;
 }
 
 // w -= sum_k h_k v_k over the node rows and the border unknown; partial sums
 // of the new w.w
+template <typename TV>
 __global__ __launch_bounds__(BLOCK) void k_gm_orth(SegView sg, double* __restrict__ w,
-                                                   const double* __restrict__ V, size_t ld,
+                                                   const TV* __restrict__ V, size_t ld,
                                                    int nk, double* __restrict__ partial,
                                                    const GmState* gm) {
     const int nbk = sg.nrows / BLOCK;
@@ -792,7 +797,7 @@ __global__ __launch_bounds__(BLOCK) void k_gm_orth(SegView sg, double* __restric
     double vals[1] = {0.0};
     if (i < S.row0 + S.nv + S.border) {
         double wi = w[i];
-        for (int k = 0; k < nk; ++k) wi = fma(-g->h[k], V[k * ld + i], wi);
+        for (int k = 0; k < nk; ++k) wi = fma(-g->h[k], (double)V[k * ld + i], wi);
         w[i] = wi;
         vals[0] = wi * wi;
     }
@@ -869,10 +874,14 @@ __global__ __launch_bounds__(FIN) void k_gm_finish_norm(SegView sg,
     fl[blockIdx.x] = GmFlags{g->done, g->full, g->iter, g->cycles};
 }
 
-// v_next = w / |w| over the node rows and the border unknown, and the
-// V-cycle's FP32 input (d0 v, or v without the scaling) over the node rows
-__global__ __launch_bounds__(BLOCK) void k_gm_scale(SegView sg, const double* __restrict__ w,
-                                                    double* __restrict__ vn,
+// v_next = w / |w| (rounded to the basis type) over the node rows and the
+// border unknown; its FP64 copy vd (when the basis is FP32 and the V-cycle
+// reads FP64: the same values), and the V-cycle's FP32 input (d0 v, or v
+// without the scaling) over the node rows.  w may alias vn (the start of a
+// cycle normalises v0 in place).
+template <typename TW, typename TV>
+__global__ __launch_bounds__(BLOCK) void k_gm_scale(SegView sg, const TW* w, TV* vn,
+                                                    double* __restrict__ vd,
                                                     const float* __restrict__ d0,
                                                     float* __restrict__ xs, const GmState* gm) {
     const int q = block_seg(sg, blockIdx.x);
@@ -882,8 +891,9 @@ __global__ __launch_bounds__(BLOCK) void k_gm_scale(SegView sg, const double* __
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     const int iend = S.row0 + S.nv;
     if (i >= iend + S.border) return;
-    const double v = w[i] * g->inv_eta;
+    const TV v = (TV)((double)w[i] * g->inv_eta);
     vn[i] = v;
+    if (vd) vd[i] = (double)v;
     if (xs && i < iend) xs[i] = d0 ? d0[i] * (float)v : (float)v;
 }
 
@@ -914,18 +924,25 @@ __global__ void k_gm_solve(int nseg, GmState* gm, GmFlags* fl) {
     g->napply = j;
 }
 
-// x += sum_k y_k z_k over the node rows and the border unknown
+// x += sum_k y_k z_k over the node rows and the border unknown (z_k's border
+// entry is v_k's: the V-cycle passes it through)
+template <typename TZ, typename TV>
 __global__ __launch_bounds__(BLOCK) void k_gm_update_x(SegView sg, double* __restrict__ x,
-                                                       const double* __restrict__ Z, size_t ld,
+                                                       const TZ* __restrict__ Z,
+                                                       const TV* __restrict__ V, size_t ld,
                                                        const GmState* gm) {
     const int q = block_seg(sg, blockIdx.x);
     const GmState* g = gm + q;
     if (!g->apply) return;
     const Seg S = sg.seg[q];
     const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= S.row0 + S.nv + S.border) return;
+    const int iend = S.row0 + S.nv;
+    if (i >= iend + S.border) return;
     double xi = x[i];
-    for (int k = 0; k < g->napply; ++k) xi = fma(g->y[k], Z[k * ld + i], xi);
+    if (i < iend)
+        for (int k = 0; k < g->napply; ++k) xi = fma(g->y[k], (double)Z[k * ld + i], xi);
+    else
+        for (int k = 0; k < g->napply; ++k) xi = fma(g->y[k], (double)V[k * ld + i], xi);
     x[i] = xi;
 }
 
@@ -1148,7 +1165,8 @@ double PpeSolver::spmv_bytes(int fp32) const {
     // D^-1 for the border row
     const double vb = fp32 ? 4.0 : 8.0;
     const double n16 = fp32 ? (double)nnz16 : 0.0;
-    const double xb = amg ? 4.0 : 8.0;  // the V-cycle's output is gathered in FP32
+    // the V-cycle's output, and an FP32 GMRES basis vector, are gathered in FP32
+    const double xb = (amg || (krylov == 1 && fp32)) ? 4.0 : 8.0;
     const double vec = (krylov == 1 ? 1.0 : 2.0) + (bordered ? 1.0 : 0.0);
     return (double)nnz * vb + 2.0 * n16 + 4.0 * ((double)nnz - n16) +
            (fp32 ? (double)nslices16 * 4.0 * CW_COUNT : 0.0) +
@@ -1354,7 +1372,9 @@ void PpeSolver::gmres_alloc(hipStream_t st, int m) {
     MP_HIP(hipMemsetAsync(gm_v.p, 0, sizeof(double) * ld * (m + 1), st));
     if (amg) {
         gm_z.alloc(ld * m);
-        MP_HIP(hipMemsetAsync(gm_z.p, 0, sizeof(double) * ld * m, st));
+        MP_HIP(hipMemsetAsync(gm_z.p, 0, sizeof(float) * ld * m, st));
+        gm_vd.alloc(ld);
+        MP_HIP(hipMemsetAsync(gm_vd.p, 0, sizeof(double) * ld, st));
     }
     gm_w.alloc(ld);
     MP_HIP(hipMemsetAsync(gm_w.p, 0, sizeof(double) * ld, st));
@@ -1370,45 +1390,49 @@ void PpeSolver::gmres_alloc(hipStream_t st, int m) {
 
 // Iteration k of a GMRES cycle: z_k = M v_k, w = A z_k, Gram-Schmidt against
 // v_0..v_k, the Hessenberg column's rotations, v_{k+1} = w / |w|.
-template <typename TM>
+template <typename TM, typename TV>
 void PpeSolver::gmres_iteration(hipStream_t st, int k, int m, const TM* a, const double* di) {
     const int nbk = nrows / BLOCK;
     const size_t ld = (size_t)nrows;
-    double* vk = gm_v.p + ld * k;
-    double* vk1 = gm_v.p + ld * (k + 1);
-    const double* zk = vk;
+    TV* V = reinterpret_cast<TV*>(gm_v.p);
+    TV* vk = V + ld * k;
+    TV* vk1 = V + ld * (k + 1);
+    constexpr bool fp32_basis = sizeof(TV) == 4;
     if (amg) {
-        double* zo = gm_z.p + ld * k;
         // one system: its V-cycle stops with it; a batch runs it while any
-        // system iterates (the others ignore the result)
-        amg->apply(st, vk, zo, nseg == 1 ? segs[0].border : 1,
-                   nseg == 1 ? &gm_d.p[0].stop : nullptr, true, y32.p);
-        zk = zo;
-    }
-    spmv_timer.begin(st);
-    if (amg)
-        hipLaunchKernelGGL((k_gm_spmv<TM, float>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
-                           view_of<TM>(), sg, a, di, zk, (const float*)y32.p, gm_w.p,
+        // system iterates (the others ignore the result).  The V-cycle reads
+        // FP64 (an FP32 basis vector's FP64 copy) and its FP32 output is z_k.
+        const double* vin = fp32_basis ? gm_vd.p : reinterpret_cast<const double*>(vk);
+        float* zk = gm_z.p + ld * k;
+        amg->apply(st, vin, yv.p, nseg == 1 ? segs[0].border : 1,
+                   nseg == 1 ? &gm_d.p[0].stop : nullptr, true, zk);
+        spmv_timer.begin(st);
+        hipLaunchKernelGGL((k_gm_spmv<TM, float, TV>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
+                           view_of<TM>(), sg, a, di, (const TV*)vk, (const float*)zk, gm_w.p,
                            gm_partial.p, gm_d.p);
-    else
-        hipLaunchKernelGGL((k_gm_spmv<TM, double>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
-                           view_of<TM>(), sg, a, di, zk, zk, gm_w.p, gm_partial.p, gm_d.p);
+    } else {
+        spmv_timer.begin(st);
+        hipLaunchKernelGGL((k_gm_spmv<TM, TV, TV>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
+                           view_of<TM>(), sg, a, di, (const TV*)vk, (const TV*)vk, gm_w.p,
+                           gm_partial.p, gm_d.p);
+    }
     spmv_timer.end(st);
-    hipLaunchKernelGGL(k_gm_dots, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_w.p, gm_v.p, ld, k + 1,
-                       gm_partial.p, gm_d.p);
-    hipLaunchKernelGGL(k_gm_finish_h, dim3(nseg * GM_MAX), dim3(GMF), 0, st, sg, gm_w.p, gm_v.p,
-                       ld, k + 1, gm_partial.p, gm_d.p);
-    hipLaunchKernelGGL(k_gm_orth, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_w.p, gm_v.p, ld, k + 1,
-                       gm_partial.p, gm_d.p);
+    hipLaunchKernelGGL(k_gm_dots<TV>, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_w.p, (const TV*)V, ld,
+                       k + 1, gm_partial.p, gm_d.p);
+    hipLaunchKernelGGL(k_gm_finish_h<TV>, dim3(nseg * GM_MAX), dim3(GMF), 0, st, sg, gm_w.p,
+                       (const TV*)V, ld, k + 1, gm_partial.p, gm_d.p);
+    hipLaunchKernelGGL(k_gm_orth<TV>, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_w.p, (const TV*)V, ld,
+                       k + 1, gm_partial.p, gm_d.p);
     hipLaunchKernelGGL(k_gm_finish_norm, dim3(nseg), dim3(FIN), 0, st, sg, gm_partial.p, gm_d.p,
                        gmf_d.p, m);
-    hipLaunchKernelGGL(k_gm_scale, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_w.p, vk1,
+    hipLaunchKernelGGL((k_gm_scale<double, TV>), dim3(nbk), dim3(BLOCK), 0, st, sg,
+                       (const double*)gm_w.p, vk1, (fp32_basis && amg) ? gm_vd.p : nullptr,
                        amg ? amg->level0_dinv() : nullptr, amg ? amg->prescaled_input() : nullptr,
                        gm_d.p);
     MP_HIP(hipGetLastError());
 }
 
-template <typename TM>
+template <typename TM, typename TV>
 void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const double* rhs,
                           double* x, int rel_to_rhs, std::vector<int>& done,
                           std::vector<int>& iters, std::vector<int>& restarts) {
@@ -1416,6 +1440,8 @@ void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const d
     const int m = std::max(1, std::min(GM_MAX, restart_m));
     gmres_alloc(st, m);
     const size_t ld = (size_t)nrows;
+    TV* V = reinterpret_cast<TV*>(gm_v.p);
+    constexpr bool fp32_basis = sizeof(TV) == 4;
     const float* d0 = amg ? amg->level0_dinv() : nullptr;
     float* xs = amg ? amg->prescaled_input() : nullptr;
     MP_HIP(hipMemcpyAsync(ctl_d.p, host_ctl, sizeof(SegCtl) * nseg, hipMemcpyHostToDevice, st));
@@ -1424,12 +1450,12 @@ void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const d
     // v0 = (rhs - A x) / |rhs - A x| for the systems taking part (mode 0), or
     // for those whose full cycle restarts (mode 1)
     auto start = [&](int mode) {
-        hipLaunchKernelGGL((k_gm_residual<TM>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
-                           view_of<TM>(), sg, a, di, rhs, x, gm_v.p, gm_partial.p, gm_d.p, mode);
-        hipLaunchKernelGGL(k_gm_residual_finish, dim3(nseg), dim3(FIN), 0, st, sg, rhs, gm_v.p,
+        hipLaunchKernelGGL((k_gm_residual<TM, TV>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
+                           view_of<TM>(), sg, a, di, rhs, x, V, gm_partial.p, gm_d.p, mode);
+        hipLaunchKernelGGL(k_gm_residual_finish<TV>, dim3(nseg), dim3(FIN), 0, st, sg, rhs, V,
                            gm_partial.p, gm_d.p, gmf_d.p, mode, rel_to_rhs);
-        hipLaunchKernelGGL(k_gm_scale, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_v.p, gm_v.p, d0, xs,
-                           gm_d.p);
+        hipLaunchKernelGGL((k_gm_scale<TV, TV>), dim3(nbk), dim3(BLOCK), 0, st, sg, (const TV*)V,
+                           V, (fp32_basis && amg) ? gm_vd.p : nullptr, d0, xs, gm_d.p);
         MP_HIP(hipGetLastError());
     };
     start(0);
@@ -1446,7 +1472,7 @@ void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const d
         bool have_prev = false;
         for (int j = 0; j < m;) {
             const int c = std::min(chunk, m - j);
-            for (int k = j; k < j + c; ++k) gmres_iteration<TM>(st, k, m, a, di);
+            for (int k = j; k < j + c; ++k) gmres_iteration<TM, TV>(st, k, m, a, di);
             j += c;
             MP_HIP(hipMemcpyAsync(hf[slot], gmf_d.p, sizeof(GmFlags) * nseg,
                                   hipMemcpyDeviceToHost, st));
@@ -1460,8 +1486,12 @@ void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const d
         }
         hipLaunchKernelGGL(k_gm_solve, dim3((nseg + 63) / 64), dim3(64), 0, st, nseg, gm_d.p,
                            gmf_d.p);
-        hipLaunchKernelGGL(k_gm_update_x, dim3(nbk), dim3(BLOCK), 0, st, sg, x,
-                           amg ? gm_z.p : gm_v.p, ld, gm_d.p);
+        if (amg)
+            hipLaunchKernelGGL((k_gm_update_x<float, TV>), dim3(nbk), dim3(BLOCK), 0, st, sg, x,
+                               (const float*)gm_z.p, (const TV*)V, ld, gm_d.p);
+        else
+            hipLaunchKernelGGL((k_gm_update_x<TV, TV>), dim3(nbk), dim3(BLOCK), 0, st, sg, x,
+                               (const TV*)V, (const TV*)V, ld, gm_d.p);
         MP_HIP(hipGetLastError());
         MP_HIP(hipMemcpyAsync(host_gmf, gmf_d.p, sizeof(GmFlags) * nseg, hipMemcpyDeviceToHost,
                               st));
@@ -1512,7 +1542,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
             }
             if (!any) break;
             if (krylov == 1)
-                run_gmres<double>(st, aval.p, dinv.p, b_dev, xt.p, 1, done, its, rst);
+                run_gmres<double, double>(st, aval.p, dinv.p, b_dev, xt.p, 1, done, its, rst);
             else
                 run_bicgstab<double, double>(st, aval.p, dinv.p, b_dev, xt.p, r.p, r0.p, p.p,
                                              v.p, s.p, t.p, 1, done, its, rst);
@@ -1571,7 +1601,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                                   st));
             launch_fill(st, e64.p, 0.0, nrows);
             if (krylov == 1)
-                run_gmres<float>(st, a32k, dinv.p, rhs64.p, e64.p, 1, done, its, rst);
+                run_gmres<float, float>(st, a32k, dinv.p, rhs64.p, e64.p, 1, done, its, rst);
             else
                 run_bicgstab<float, double>(st, a32k, dinv.p, rhs64.p, e64.p, r.p, r0.p, p.p,
                                             v.p, s.p, t.p, 1, done, its, rst);

@@ -95,7 +95,9 @@ struct PpeSolver {
     int precond = 0;                  // enum mp_precond
     int krylov = 0;                   // enum mp_krylov
     int restart_m = 30;               // GMRES columns per cycle
-    DevBuf<double> gm_v, gm_z, gm_w, gm_partial;  // basis, preconditioned basis, work, partials
+    DevBuf<double> gm_v, gm_w, gm_partial;  // basis (FP64 or FP32 in the same bytes), work, partials
+    DevBuf<float> gm_z;               // preconditioned basis: the V-cycle's FP32 outputs
+    DevBuf<double> gm_vd;             // FP64 copy of an FP32 basis vector (the V-cycle's input)
     DevBuf<GmState> gm_d;             // [nseg]
     DevBuf<GmFlags> gmf_d;            // [nseg]
     GmFlags* host_gmf = nullptr;      // pinned, 2 x nseg
@@ -143,14 +145,15 @@ struct PpeSolver {
     void run_bicgstab(hipStream_t s, const TM* aval_t, const T* dinv_t, const T* rhs, T* x,
                       T* rr, T* rr0, T* pp, T* vv, T* ss, T* tt, int rel_to_rhs,
                       std::vector<int>& done, std::vector<int>& iters, std::vector<int>& restarts);
-    // GMRES(restart_m) on operator values aval_t (FP64 vectors), right-
-    // preconditioned by the V-cycle when there is one; same controls and
-    // outcome codes as run_bicgstab (restarts = GMRES cycles after the first).
-    template <typename TM>
+    // GMRES(restart_m) on operator values aval_t (FP64 x and right-hand
+    // side, basis vectors of type TV), right-preconditioned by the V-cycle
+    // when there is one; same controls and outcome codes as run_bicgstab
+    // (restarts = GMRES cycles after the first).
+    template <typename TM, typename TV>
     void run_gmres(hipStream_t s, const TM* aval_t, const double* dinv_t, const double* rhs,
                    double* x, int rel_to_rhs, std::vector<int>& done, std::vector<int>& iters,
                    std::vector<int>& restarts);
-    template <typename TM>
+    template <typename TM, typename TV>
     void gmres_iteration(hipStream_t s, int k, int m, const TM* aval_t, const double* dinv_t);
     void gmres_alloc(hipStream_t s, int m);
     void residual64(hipStream_t s, const double* b_dev, std::vector<double>& rr,
