@@ -693,37 +693,34 @@ __global__ __launch_bounds__(BLOCK) void k_gm_spmv(SellView sv, SegView sg,
 }
 
 // partial sums of v_k . w over the node rows, k < nk (the border row is added
-// by the finishing kernel once w's border entry is known)
+// by the finishing kernel once w's border entry is known).  The block's 256
+// rows are shared by its four wavefronts; wavefront u forms the dots of
+// v_u, v_{u+4}, ... (four rows per lane, then one shuffle tree per vector).
 template <typename TV>
 __global__ __launch_bounds__(BLOCK) void k_gm_dots(SegView sg, const double* __restrict__ w,
                                                    const TV* __restrict__ V, size_t ld,
                                                    int nk, double* __restrict__ partial,
                                                    const GmState* gm) {
+    static_assert(BLOCK == 256, "four rows per lane");
     const int nbk = sg.nrows / BLOCK;
     const int lb = blockIdx.x;
     const int q = block_seg(sg, lb);
     if (gm[q].stop) return;
     const Seg S = sg.seg[q];
-    const int i = lb * BLOCK + threadIdx.x;
-    const bool row = i < S.row0 + S.nv;
-    const double wi = row ? w[i] : 0.0;
-    double vals[GM_MAX];
+    const int iend = S.row0 + S.nv;
+    const int lane = threadIdx.x & 63, u = threadIdx.x >> 6;
+    const int r0 = lb * BLOCK + lane;
+    double wr[4];
 #pragma unroll
-    for (int k = 0; k < GM_MAX; ++k) vals[k] = (row && k < nk) ? (double)V[k * ld + i] * wi : 0.0;
-    __shared__ double sh[GM_MAX][BLOCK / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int t = 0; t < 4; ++t) wr[t] = (r0 + 64 * t < iend) ? w[r0 + 64 * t] : 0.0;
+    for (int k = u; k < nk; k += 4) {
+        const TV* vk = V + k * ld + r0;
+        double acc = 0.0;
 #pragma unroll
-    for (int k = 0; k < GM_MAX; ++k)
-        if (k < nk) {
-            double v = vals[k];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0) sh[k][wid] = v;
-        }
-    __syncthreads();
-    if ((int)threadIdx.x < nk) {
-        double t = sh[threadIdx.x][0];
-        for (int u = 1; u < BLOCK / 64; ++u) t += sh[threadIdx.x][u];
-        partial[(size_t)threadIdx.x * nbk + lb] = t;
+        for (int t = 0; t < 4; ++t)
+            if (r0 + 64 * t < iend) acc = fma((double)vk[64 * t], wr[t], acc);
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane == 0) partial[(size_t)k * nbk + lb] = acc;
     }
 }
 
