@@ -9,11 +9,14 @@ dt = 5e-4, AB2; pressure-Poisson relative residual 1e-9 (--tol).
 Forms timed over the same number of steps from the same warmed-up state: the
 plain FP64 path (FP64 Jacobi-scaled BiCGSTAB, previous pressure as the guess,
 as the reference's solveWithGuess(rhs, p_old)), the mixed form (FP32-operator
-inner BiCGSTAB inside FP64 defect correction) and the headline form (the
-mixed form preconditioned by an FP32 smoothed-aggregation multigrid V-cycle,
-with the learned guess).  The learned guess is trained first, untimed, on a
-63,536-node cube (40^3) with the same physics: 2000 solver steps in 10 episodes from
-rest with seeded random lid speeds in [0.5, 1.5].  `value` is the headline
+inner BiCGSTAB inside FP64 defect correction), the multigrid form (FP32-
+operator inner GMRES(30), the reference's other Krylov option, preconditioned
+by an FP32 smoothed-aggregation multigrid V-cycle; previous pressure as the
+guess) and the headline form (the multigrid form with the learned guess;
+--krylov bicgstab runs these two with BiCGSTAB).  The learned guess is
+trained first, untimed, on a 63,536-node cube (40^3) with the same physics:
+2000 solver steps in 10 episodes from rest with seeded random lid speeds in
+[0.5, 1.5].  `value` is the headline
 form's timesteps/s; the FP64 rate, the ratio, the residuals and the velocity
 and pressure differences between the two are reported beside it.  The
 roofline entry times the PPE SpMV with HIP events on the library's stream,
@@ -53,8 +56,9 @@ def lid_bc(cloud):
 
 def measured_traffic(n: int, kernel_index: int = 0):
     """HBM bytes per launch of the PPE SpMV measured with rocprofv3 FETCH_SIZE and
-    WRITE_SIZE runs on the 100^3 workload (index 0: FP32 operator, 1: FP64 operator),
-    read from the committed summary; None for other workloads."""
+    WRITE_SIZE runs on the 100^3 workload (index 0: the headline form's FP32-operator
+    GMRES product, 1: the FP64 form's product), read from the committed summary; None
+    for other workloads."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "ppe_spmv_traffic_cavity3d_1M.md")
     if n != 100 or not os.path.exists(path):
@@ -453,15 +457,19 @@ def main():
                          f"({args.n}^3 lattice, +-0.3h), PHS r^3 + deg-3 poly, k=40/50/55/60, "
                          f"dt=5e-4, AB2, PPE rel. residual {args.tol:g}"),
             "nv": cloud.nv, "nnz": cloud.nnz,
-            "form": ("FP32-operator inner BiCGSTAB with FP32 smoothed-aggregation multigrid "
-                     "V-cycle, in FP64 defect correction, + learned guess"),
+            "form": (f"FP32-operator inner {'GMRES(30)' if kry else 'BiCGSTAB'} with FP32 "
+                     "smoothed-aggregation multigrid V-cycle, in FP64 defect correction, + "
+                     "learned guess"),
             "guess_training": f"{args.train_steps} solver steps on {small.nv} nodes, "
                               f"lid speeds U[0.5,1.5], {t_train:.1f}s",
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.n, 0),
-            "kernel": ("k_spmv_alpha/k_spmv_omega<float,double,float> (PPE SpMV, FP32 "
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(args.n, 0) if kry else None,
+            "kernel": ("k_gm_spmv<float,float,float> (PPE SpMV of GMRES, FP32 operator, FP32 "
+                       "gathers of the V-cycle output)" if kry else
+                       "k_spmv_alpha/k_spmv_omega<float,double,float> (PPE SpMV, FP32 "
                        "operator, FP32 gathers of the V-cycle output)"),
             "bytes_per_launch": bytes_l, "mean_ms": ms, "launches": n_l,
             "launches_returned_early": all_l - n_l,
