@@ -58,9 +58,10 @@ int main(int argc, char** argv) {
         double res[6];
         calc_continuity_residual(points, cloud, parameters, u_new, v_new,
                                  dim == 3 ? &w_new : nullptr, res);
-        printf("nv %d steps %d last PPE iterations %d rel_res %.3e continuity max %.3e mean %.3e "
-               "cond max %.3e dx avg %.5f\n",
-               nv, steps, parameters.n_iter_actual.empty() ? 0 : parameters.n_iter_actual.back(),
+        printf("nv %d steps %d solver %s last PPE iterations %d rel_res %.3e continuity max %.3e "
+               "mean %.3e cond max %.3e dx avg %.5f\n",
+               nv, steps, parameters.solver_type.c_str(),
+               parameters.n_iter_actual.empty() ? 0 : parameters.n_iter_actual.back(),
                parameters.rel_res_log.empty() ? 0.0 : parameters.rel_res_log.back(), res[4], res[5],
                cloud.cond_num_RBF_max, parameters.avg_dx);
         FILE* f = fopen(argv[7], "wb");

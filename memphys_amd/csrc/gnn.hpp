@@ -39,6 +39,7 @@ struct GuessNet {
     DevBuf<float> theta, grad, adam_m, adam_v;
     DevBuf<uint16_t> theta_bf;       // BF16 copy used by the passes
     DevBuf<float> X, Z[GNN_L + 1], Hh[GNN_L + 1], Mm[GNN_L + 1], out, target, dbuf;
+    DevBuf<uint16_t> Hb[GNN_L];      // BF16 copies of the hidden states the next layer aggregates
     DevBuf<float> dH, dZ, dTmp;
     DevBuf<double> scal;             // [0] s, [1] loss, [2..] scratch
     DevBuf<double> partial;

@@ -71,7 +71,8 @@ __global__ __launch_bounds__(256) void k_features(int nv, const double* __restri
     X[i * F + 4] = (dim == 3) ? (float)xyz[2 * (size_t)npad + i] : 0.0f;
 }
 
-// Z = in1 W1 (+ in2 W2) + b (not stored when Z is null); Hout = relu(Z).
+// Z = in1 W1 (+ in2 W2) + b (not stored when Z is null); Hout = relu(Z), and
+// its BF16 copy (round to nearest even) when Hbf is not null.
 // in1: [nv][K1], in2: [nv][H].
 // A block takes tiles of DR rows: it stages the tile's inputs in LDS and each
 // thread forms 4 rows x 4 outputs, every output's fma chain in the order
@@ -83,7 +84,8 @@ __global__ __launch_bounds__(256) void k_dense(int nv, const float* __restrict__
                                                const uint16_t* __restrict__ W1,
                                                const uint16_t* __restrict__ W2,
                                                const uint16_t* __restrict__ bias,
-                                               float* __restrict__ Z, float* __restrict__ Hout) {
+                                               float* __restrict__ Z, float* __restrict__ Hout,
+                                               uint16_t* __restrict__ Hbf) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     __shared__ f4 w1[K1 * H / 4];
     __shared__ f4 w2[TWO ? H * H / 4 : 1];
@@ -145,12 +147,16 @@ __global__ __launch_bounds__(256) void k_dense(int nv, const float* __restrict__
             const f4 h = {fmaxf(acc[i].x, 0.0f), fmaxf(acc[i].y, 0.0f), fmaxf(acc[i].z, 0.0f),
                           fmaxf(acc[i].w, 0.0f)};
             *reinterpret_cast<f4*>(Hout + o) = h;
+            if (Hbf) {
+                typedef uint16_t u4 __attribute__((ext_vector_type(4)));
+                *reinterpret_cast<u4*>(Hbf + o) = u4{f2bf(h.x), f2bf(h.y), f2bf(h.z), f2bf(h.w)};
+            }
         }
     }
 }
 
 // Hidden layer on the matrix cores: Z = in1 W1 + in2 W2 + b, Hout = relu(Z)
-// (Z not stored when null); in1, in2: [nv][H], W1, W2: [H][H] (BF16, k-major).
+// and its BF16 copy (Z and the copy not stored when null); in1, in2: [nv][H], W1, W2: [H][H] (BF16, k-major).
 // v_mfma_f32_16x16x4f32 keeps every product and sum in FP32.  A wavefront
 // takes tiles of 16 rows; the 128 reduction indices k (in1's 64, then in2's)
 // are walked in 32 steps of 4, and lane l holds, for step (j, e), row
@@ -165,7 +171,8 @@ __global__ __launch_bounds__(256) void k_dense_mfma(int nv, const float* __restr
                                                     const uint16_t* __restrict__ W2,
                                                     const uint16_t* __restrict__ bias,
                                                     float* __restrict__ Z,
-                                                    float* __restrict__ Hout) {
+                                                    float* __restrict__ Hout,
+                                                    uint16_t* __restrict__ Hbf) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     static_assert(H == 64, "fragment layout assumes 64 hidden units");
     __shared__ f4 wf[8 * 4 * 64];
@@ -223,7 +230,9 @@ __global__ __launch_bounds__(256) void k_dense_mfma(int nv, const float* __restr
             for (int nb = 0; nb < 4; ++nb) {
                 const size_t o = (size_t)r * H + 16 * nb + c;
                 if (Z) Z[o] = acc[nb][i];
-                Hout[o] = fmaxf(acc[nb][i], 0.0f);
+                const float h = fmaxf(acc[nb][i], 0.0f);
+                Hout[o] = h;
+                if (Hbf) Hbf[o] = f2bf(h);
             }
         }
     }
@@ -248,15 +257,17 @@ __global__ __launch_bounds__(256) void k_dense_t(int nv, const float* __restrict
     }
 }
 
-// m[i][:] = sum_j a_ij h[j][:] over a SELL-64x4 operator.  A wavefront takes
-// four consecutive rows, sixteen lanes per row, each lane four features (one
-// 16-byte gather per neighbour), so one gather instruction fetches four
-// neighbour rows.  accumulate: m += instead of m =.  A row's slots come in
+// m[i][:] = sum_j a_ij h[j][:] over a SELL-64x4 operator, h in FP32 or as
+// BF16 numbers (TH = uint16_t; the products and sums stay FP32).  A wavefront
+// takes four consecutive rows, sixteen lanes per row, each lane four features
+// (one 16- or 8-byte gather per neighbour), so one gather instruction fetches
+// four neighbour rows.  accumulate: m += instead of m =.  A row's slots come in
 // groups of four (one 16-byte load of columns and one of values, the row's
 // own lanes of the slice); the gathers of two groups are issued before their
 // fmas, which keep the order of the row's entries.
+template <typename TH>
 __global__ __launch_bounds__(256) void k_aggregate(SellView sv, const float* __restrict__ val,
-                                                   const float* __restrict__ h,
+                                                   const TH* __restrict__ h,
                                                    float* __restrict__ m, int accumulate) {
     typedef int i4 __attribute__((ext_vector_type(4)));
     typedef float f4 __attribute__((ext_vector_type(4)));
@@ -268,7 +279,15 @@ __global__ __launch_bounds__(256) void k_aggregate(SellView sv, const float* __r
     const int len = sv.len[row];
     f4 acc = accumulate ? *reinterpret_cast<const f4*>(m + (size_t)row * H + fq)
                         : f4{0.0f, 0.0f, 0.0f, 0.0f};
-    auto gather = [&](int c) { return *reinterpret_cast<const f4*>(h + (size_t)c * H + fq); };
+    auto gather = [&](int c) {
+        if constexpr (sizeof(TH) == 2) {
+            typedef uint16_t u4 __attribute__((ext_vector_type(4)));
+            const u4 b = *reinterpret_cast<const u4*>(h + (size_t)c * H + fq);
+            return f4{bf2f(b.x), bf2f(b.y), bf2f(b.z), bf2f(b.w)};
+        } else {
+            return *reinterpret_cast<const f4*>(h + (size_t)c * H + fq);
+        }
+    };
     auto fma4 = [](float a, const f4& x, f4 y) {
         y.x = fmaf(a, x.x, y.x);
         y.y = fmaf(a, x.y, y.y);
@@ -537,6 +556,7 @@ void GuessNet::setup(hipStream_t st, const SellView& view, const float* a32_slot
         Z[l].alloc(nh);
         Hh[l].alloc(nh);
         Mm[l].alloc(nh);
+        if (l < GNN_L) Hb[l].alloc(nh);
     }
     out.alloc(nv);
     dbuf.alloc(nv);
@@ -605,15 +625,18 @@ void GuessNet::forward(hipStream_t st, bool keep_pre) {
     const uint16_t* P = theta_bf.p;
     const int g = std::min(DENSE_GRID, (nv + DR - 1) / DR);
     const int gm = std::min(DENSE_GRID, (nv + 4 * MROWS - 1) / (4 * MROWS));
+    // each hidden state the next layer aggregates also gets a BF16 copy, which
+    // the aggregation gathers (half the bytes of the FP32 rows)
     hipLaunchKernelGGL((k_dense<F, false>), dim3(g), dim3(256), 0, st, nv, X.p, nullptr,
-                       P + GNN_W0, nullptr, P + GNN_B0, keep_pre ? Z[0].p : nullptr, Hh[0].p);
+                       P + GNN_W0, nullptr, P + GNN_B0, keep_pre ? Z[0].p : nullptr, Hh[0].p,
+                       Hb[0].p);
     for (int l = 1; l <= GNN_L; ++l) {
         const int o = GNN_LAYER + (l - 1) * GNN_LAYER_SIZE;
-        hipLaunchKernelGGL(k_aggregate, dim3((nv + 15) / 16), dim3(256), 0, st, sv, a32, Hh[l - 1].p,
-                           Mm[l].p, 0);
+        hipLaunchKernelGGL(k_aggregate<uint16_t>, dim3((nv + 15) / 16), dim3(256), 0, st, sv, a32,
+                           (const uint16_t*)Hb[l - 1].p, Mm[l].p, 0);
         hipLaunchKernelGGL(k_dense_mfma, dim3(gm), dim3(256), 0, st, nv, Hh[l - 1].p, Mm[l].p,
                            P + o, P + o + H * H, P + o + 2 * H * H, keep_pre ? Z[l].p : nullptr,
-                           Hh[l].p);
+                           Hh[l].p, l < GNN_L ? Hb[l].p : nullptr);
     }
     hipLaunchKernelGGL(k_output, dim3((nv + 3) / 4), dim3(256), 0, st, nv, Hh[GNN_L].p, P + GNN_WO,
                        P + GNN_BO, out.p);
@@ -655,8 +678,8 @@ void GuessNet::backward(hipStream_t st, const double* p1, const double* p2, cons
         hipLaunchKernelGGL((k_dense_t<H>), dim3(g), dim3(256), 0, st, nv, dZ.p, P + o + H * H,
                            dTmp.p, 0);
         hipLaunchKernelGGL((k_dense_t<H>), dim3(g), dim3(256), 0, st, nv, dZ.p, P + o, dH.p, 0);
-        hipLaunchKernelGGL(k_aggregate, dim3((nv + 15) / 16), dim3(256), 0, st, tsv, tval.p, dTmp.p,
-                           dH.p, 1);
+        hipLaunchKernelGGL(k_aggregate<float>, dim3((nv + 15) / 16), dim3(256), 0, st, tsv, tval.p,
+                           (const float*)dTmp.p, dH.p, 1);
     }
     hipLaunchKernelGGL(k_relu_back, dim3(blocks(nh)), dim3(256), 0, st, nh, dH.p, Z[0].p, dZ.p);
     hipLaunchKernelGGL((k_wgrad_partial<F, H, true>), dim3(WG_GRID), dim3(256), 0, st, nv, X.p,

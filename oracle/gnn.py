@@ -4,7 +4,9 @@ Restates the network the HIP kernels evaluate (memphys_amd/csrc/gnn.hpp) with
 numpy, so that the GPU forward pass, loss and parameter gradient can be
 checked.  Weights are rounded to BF16 exactly as the device does (round to
 nearest even) and then used in float64; features are rounded to float32 as the
-device stores them.  The backward pass here is itself pinned by finite
+device stores them; the hidden states a layer aggregates over the stencils are
+rounded to BF16 (the device gathers a BF16 copy of them), and the gradient
+treats that rounding as the identity, as the device's backward pass does.  The backward pass here is itself pinned by finite
 differences (tests/test_oracle_gnn.py).
 """
 from __future__ import annotations
@@ -70,7 +72,9 @@ def features(A_bordered, nv, p1, p2, b, xyz, dinv):
     return X.astype(np.float32).astype(np.float64), s, pext[:nv]
 
 
-def forward(theta, X, At):
+def forward(theta, X, At, round_h=True):
+    """round_h=False: the unrounded network, whose exact gradient the backward
+    formulas are (the finite-difference check)."""
     p = unpack(theta)
     zs, hs, ms = [], [], []
     z = X @ p["W0"] + p["b0"]
@@ -79,7 +83,7 @@ def forward(theta, X, At):
     hs.append(h)
     ms.append(None)
     for l in range(L):
-        m = At @ h
+        m = At @ (bf16(h) if round_h else h)
         z = h @ p[f"Ws{l}"] + m @ p[f"Wn{l}"] + p[f"bl{l}"]
         h = np.maximum(z, 0.0)
         zs.append(z)
@@ -89,9 +93,9 @@ def forward(theta, X, At):
     return d, (zs, hs, ms)
 
 
-def loss_and_grad(theta, X, At, target):
+def loss_and_grad(theta, X, At, target, round_h=True):
     p = unpack(theta)
-    d, (zs, hs, ms) = forward(theta, X, At)
+    d, (zs, hs, ms) = forward(theta, X, At, round_h)
     nv = X.shape[0]
     e = d - target
     loss = np.mean(e * e)

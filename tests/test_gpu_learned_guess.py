@@ -46,7 +46,12 @@ def test_guess_forward_and_gradient_match_oracle(cube3d_small):
     g_or, grad_or, loss_or = G.evaluate(theta, A, c.nv, p1, p2, b, x, pt)
     _print("guess rel", rel_err(g_gpu, g_or), "loss", loss_gpu, loss_or,
            "grad rel", rel_err(grad_gpu, grad_or))
-    assert rel_err(g_gpu, g_or) <= 1e-5
+    # The aggregated hidden states are gathered as BF16 copies: the device
+    # rounds its FP32 states, the oracle its float64 ones, so a few values fall
+    # on the other side of a BF16 rounding boundary (one BF16 ulp, 2^-8
+    # relative, on those); through three layers that leaves ~1e-4 on the guess
+    # (8e-5 measured).  A wrong layer or transposed weight differs by O(1).
+    assert rel_err(g_gpu, g_or) <= 5e-4
     assert abs(loss_gpu - loss_or) <= 1e-4 * abs(loss_or)
     # per parameter block (FP32 accumulation on the device, float64 here)
     blocks = [(G.W0, G.B0), (G.B0, G.LAYER)] + \

@@ -287,9 +287,11 @@ def test_condition_estimate_matches_oracle(ctx, cube3d_small):
     assert rel_err(cond, ocond) <= 1e-12
 
 
-def test_class_api_driver_matches_oracle(tmp_path):
+@pytest.mark.parametrize("solver_type", ["eigen_ilu_bicgstab", "hypre_ilu_gmres"])
+def test_class_api_driver_matches_oracle(tmp_path, solver_type):
     """examples/lid_driven_cavity (PARAMETERS -> POINTS -> CLOUD ->
-    FRACTIONAL_STEP_1 -> time loop) against the oracle on the same cloud."""
+    FRACTIONAL_STEP_1 -> time loop) against the oracle on the same cloud, with
+    the CSV's solver_type selecting BiCGSTAB or GMRES(gmres_kdim)."""
     import os
     import subprocess
 
@@ -298,12 +300,15 @@ def test_class_api_driver_matches_oracle(tmp_path):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     exe = os.path.join(root, "examples", "lid_driven_cavity")
     out = str(tmp_path / "fields.bin")
+    csv = tmp_path / "parameters.csv"
+    text = open(os.path.join(root, "examples", "parameters_cavity.csv")).read()
+    csv.write_text(text.replace("solver_type,eigen_ilu_bicgstab", f"solver_type,{solver_type}"))
     steps, dt, Re, n = 10, 1e-3, 100.0, 30
-    r = subprocess.run([exe, os.path.join(root, "examples", "parameters_cavity.csv"), "2", str(n),
-                        str(steps), str(dt), str(Re), out], capture_output=True, text=True,
-                       timeout=120)
+    r = subprocess.run([exe, str(csv), "2", str(n), str(steps), str(dt), str(Re), out],
+                       capture_output=True, text=True, timeout=120)
     _print(r.stdout.strip())
     assert r.returncode == 0, r.stdout + r.stderr
+    assert f"solver {solver_type}" in r.stdout
     raw = open(out, "rb").read()
     nv, dim, nsys = np.frombuffer(raw[:12], dtype=np.int32)
     data = np.frombuffer(raw[12:], dtype=np.float64)

@@ -18,8 +18,9 @@ def _setup(seed=0, nv=40):
 
 
 def test_backward_matches_finite_differences():
+    # the unrounded network: its loss is smooth in theta
     theta, X, At, target = _setup()
-    loss, g, _ = G.loss_and_grad(theta, X, At, target)
+    loss, g, _ = G.loss_and_grad(theta, X, At, target, round_h=False)
     rng = np.random.default_rng(1)
     idx = np.concatenate([rng.choice(G.NPARAM, 60, replace=False),
                           [G.W0, G.B0, G.LAYER, G.LAYER + G.H * G.H, G.WO, G.BO]])
@@ -29,8 +30,29 @@ def test_backward_matches_finite_differences():
         tp[i] += eps
         tm = theta.copy()
         tm[i] -= eps
-        fd = (G.loss_and_grad(tp, X, At, target)[0] - G.loss_and_grad(tm, X, At, target)[0]) / (2 * eps)
+        fd = (G.loss_and_grad(tp, X, At, target, round_h=False)[0] -
+              G.loss_and_grad(tm, X, At, target, round_h=False)[0]) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_aggregated_states_are_rounded_to_bf16():
+    """The rounded network differs from the unrounded one by the BF16 rounding
+    of the aggregated states only: no difference when those are BF16 numbers
+    already (a ReLU network on BF16-representable inputs with a zero first
+    layer weight), a small one otherwise."""
+    theta, X, At, target = _setup(3)
+    d1, _ = G.forward(theta, X, At)
+    d0, _ = G.forward(theta, X, At, round_h=False)
+    rel = np.linalg.norm(d1 - d0) / np.linalg.norm(d0)
+    assert 0 < rel < 2e-2
+    th = theta.copy()
+    th[G.W0:G.B0] = 0.0                       # h0 = relu(b0): the bias, BF16 already
+    th[G.LAYER:G.LAYER + 2 * G.H * G.H] = 0.0  # layer 1: h1 = relu(bias), BF16 already
+    th[G.LAYER + G.LAYER_SIZE:] = 0.0          # layers 2, 3 and the output: zero
+    th = G.bf16(th.astype(np.float32))
+    e1, (_, hs1, ms1) = G.forward(th, X, At)
+    e0, (_, hs0, ms0) = G.forward(th, X, At, round_h=False)
+    assert np.array_equal(ms1[1], ms0[1]) and np.array_equal(ms1[2], ms0[2])
 
 
 def test_zero_output_layer_gives_extrapolation():
