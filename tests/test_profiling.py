@@ -1,0 +1,81 @@
+"""CPU checks of the rocprofv3 summary tool (memphys_amd/profiling.py) on
+synthetic result databases with the rocpd tables it reads: the working-launch
+rule, the iteration extraction and the counter scaling of the HBM traffic."""
+import sqlite3
+
+import pytest
+
+import memphys_amd.profiling as P
+
+
+def _kernels_db(path, rows):
+    con = sqlite3.connect(path)
+    con.execute("create table kernels(name text, start int, end int, grid_x int)")
+    con.executemany("insert into kernels values(?,?,?,?)", rows)
+    con.commit()
+    con.close()
+    return str(path)
+
+
+def test_stats_working_launches(tmp_path):
+    # 6 launches of 100 us and 4 of 2 us (a converged system's early returns)
+    rows, t = [], 0
+    for d in [100_000] * 6 + [2_000] * 4:
+        rows.append(("void mp::k_spmv(int)", t, t + d, 999424))
+        t += d + 1000
+    out = P.stats(_kernels_db(tmp_path / "a.db", rows), 10)
+    line = [ln for ln in out.splitlines() if ln.startswith("mp::k_spmv")][0].split()
+    calls, working, work_us = int(line[2]), int(line[3]), float(line[6])
+    assert (calls, working) == (10, 6) and work_us == pytest.approx(100.0)
+
+
+def test_iteration_picks_the_median_iteration(tmp_path):
+    rows, t = [], 0
+
+    def add(name, d, grid):
+        nonlocal t
+        rows.append((name, t, t + d, grid))
+        t += d + 500
+
+    for it, spmv in enumerate([60_000, 70_000, 65_000, 1_000]):  # the last: a stopped system
+        add("mp::k_gm_scale<double, float>(x)", 6_000, 998912)
+        add("mp::k_amg_op<0, 3, double, float>(x)", 58_000, 999424)
+        add("mp::k_gm_spmv<float, float, float>(x)", spmv, 999424)
+        add("mp::k_gm_dots<float>(x)", 12_000, 998912)
+    add("mp::k_gm_scale<double, float>(x)", 6_000, 998912)
+    add("mp::k_amg_op<0, 3, double, float>(x)", 58_000, 999424)
+    out = P.iteration(_kernels_db(tmp_path / "b.db", rows), "k_gm_scale", "k_amg_op",
+                      "k_gm_spmv").splitlines()
+    assert out[0].startswith("iterations 3,")  # the stopped system's stretch is left out
+    spmv = [ln for ln in out if "k_gm_spmv" in ln][0]
+    assert "dur    65.0 us" in spmv  # median of the three working iterations
+
+
+def test_traffic_scales_counters_by_the_copy_kernel(tmp_path):
+    con = sqlite3.connect(tmp_path / "f.db")
+    con.execute("create table counters_collection(kernel_name text, grid_size int, "
+                "counter_name text, value real, duration int)")
+    copy_kib = 1 << 20  # a 1 GiB copy
+    con.execute("insert into counters_collection values(?,?,?,?,?)",
+                ("neg_kernel", 64, "FETCH_SIZE", 0.5 * copy_kib, 1000))
+    for _ in range(3):
+        con.execute("insert into counters_collection values(?,?,?,?,?)",
+                    ("mp::k_gm_spmv", 999424, "FETCH_SIZE", 0.5 * 1000.0, 70_000))
+    con.commit()
+    con.close()
+    con = sqlite3.connect(tmp_path / "w.db")
+    con.execute("create table counters_collection(kernel_name text, grid_size int, "
+                "counter_name text, value real, duration int)")
+    con.execute("insert into counters_collection values(?,?,?,?,?)",
+                ("neg_kernel", 64, "WRITE_SIZE", 1.0 * copy_kib, 1000))
+    for _ in range(3):
+        con.execute("insert into counters_collection values(?,?,?,?,?)",
+                    ("mp::k_gm_spmv", 999424, "WRITE_SIZE", 8.0, 70_000))
+    con.commit()
+    con.close()
+    out = P.traffic(str(tmp_path / "f.db"), str(tmp_path / "w.db"), "k_gm_spmv", "neg",
+                    1 << 30, 999424).splitlines()
+    vals = {ln.split(" = ")[0]: float(ln.split(" = ")[1]) for ln in out if " = " in ln}
+    assert vals["fetch_bytes_per_launch"] == pytest.approx(1000.0 * 1024)
+    assert vals["write_bytes_per_launch"] == pytest.approx(8.0 * 1024)
+    assert vals["traffic_bytes_per_launch"] == pytest.approx(1008.0 * 1024)
