@@ -118,12 +118,27 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
     ctx.step(2, it0 + steps)
     spmv = ctx.kernel_timing_get(0)
     pred = ctx.kernel_timing_get(1)
+    smooth = ctx.kernel_timing_get(5) if precond == M.MP_PRECOND_AMG else None
     ctx.kernel_timing(False)
     log(f"[{label}] {steps} steps in {t1 - t0:.3f}s, iters/step {np.mean(iters):.1f}, "
         f"max rel_res {max(res):.2e}, spmv {spmv}")
-    return dict(seconds=t1 - t0, iters=iters, rel_res=res, spmv=spmv, predictor=pred,
+    return dict(seconds=t1 - t0, iters=iters, rel_res=res, spmv=spmv, predictor=pred, smooth=smooth,
                 fields=fields, divergence=div.tolist(), hierarchy=hier, create_s=t_create,
                 poll_interval=interval)
+
+
+def smooth_roofline(t, copy_gbps):
+    """Roofline entry of the multigrid's level-0 post-smoothing product (the
+    V-cycle's largest kernel): algorithmic bytes per launch over its mean
+    HIP-event duration, launches that did the work."""
+    if not t or t[1] <= 0:
+        return None
+    n, ms, bytes_l, all_l = t
+    ach = bytes_l / (ms * 1e-3) / 1e9
+    return {"kernel": "k_amg_op<0,2,double,double> (level-0 post-smoothing, BF16 values)",
+            "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "frac_of_copy": ach / copy_gbps, "bytes_per_launch": bytes_l,
+            "mean_ms": ms, "launches": n, "launches_returned_early": all_l - n}
 
 
 def stream_copy_gbps(mib: int = 1024, reps: int = 10) -> float:
@@ -493,6 +508,7 @@ def main():
                                "bytes_per_launch": b64, "mean_ms": ms64, "launches": n64,
                                "launches_returned_early": all64 - n64,
                                "traffic": measured_traffic(args.n, 1)},
+        "amg_level0_roofline": smooth_roofline(head.get("smooth"), copy_gbps),
         "cloud_setup": setup,
         "weights_kernel_ms": wk[1],
         "weights_fp64_vs_fp32_refined": weights_check,

@@ -240,7 +240,8 @@ int mp_graph_mode(mp_ctx* ctx, int enable);
 
 /* Per-kernel timing with HIP events on the library's stream.
  * which: 0 = PPE SpMV, 1 = predictor, 2 = pressure RHS, 3 = corrector,
- *        4 = stencil weights.  Reports the launches that did the work and
+ *        4 = stencil weights, 5 = the multigrid's level-0 post-smoothing
+ *        product (with MP_PRECOND_AMG).  Reports the launches that did the work and
  * their mean milliseconds: Krylov launches of a system that converged inside
  * a chunk of iterations return at once and are left out (a launch counts
  * when it takes at least half the median launch's time).  all_launches (may

@@ -16,6 +16,8 @@ namespace mp {
 //   g == 0: SELL-64x4, one lane per row (short rows; ptr = slice pointers);
 //   g  > 0: CSR with g lanes per row and a g-lane reduction (long rows of
 //           the coarse operators and restrictions; ptr = row pointers).
+struct KernelTimer;
+
 struct AmgMat {
     int n = 0, nslices = 0, g = 0;
     long nnz = 0;
@@ -79,6 +81,7 @@ struct AmgDevice {
     // values; the border unknown passes through only in out).
     void apply(hipStream_t s, const double* in, double* out, int bordered, const int* done_flag,
                bool prescaled = false, float* out32 = nullptr);
+    KernelTimer* post_timer = nullptr;  // times the level-0 post-smoothing product when set
     float* prescaled_input() { return lv.size() > 1 ? lv[0].x.p : nullptr; }
     const float* level0_dinv() const { return lv.empty() ? nullptr : lv[0].dinv.p; }
 };

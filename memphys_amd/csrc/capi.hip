@@ -1266,6 +1266,8 @@ int mp_kernel_timing(mp_ctx* c, int enable) {
             if (s) {
                 s->spmv_timer.enabled = enable != 0;
                 s->spmv_timer.reset();
+                s->smooth_timer.enabled = enable != 0;
+                s->smooth_timer.reset();
             }
         return MP_OK;
     });
@@ -1274,15 +1276,16 @@ int mp_kernel_timing(mp_ctx* c, int enable) {
 int mp_kernel_timing_get(mp_ctx* c, int which, long* launches, double* mean_ms,
                          double* bytes_per_launch, long* all_launches) {
     return guarded(c, "mp_kernel_timing_get", [&]() -> int {
-        if (which < 0 || which > 4 || !launches || !mean_ms || !bytes_per_launch)
+        if (which < 0 || which > 5 || !launches || !mean_ms || !bytes_per_launch)
             return fail(c, MP_ERR_ARG, "mp_kernel_timing_get: bad arguments");
-        if (which == 0) {
+        if (which == 0 || which == 5) {
             PpeSolver* s = c->ppe ? c->ppe.get() : c->solver.get();
             if (!s) return fail(c, MP_ERR_STATE, "mp_kernel_timing_get: no solver");
             long all = 0;
-            s->spmv_timer.summary(launches, mean_ms, &all);
+            (which == 0 ? s->spmv_timer : s->smooth_timer).summary(launches, mean_ms, &all);
             if (all_launches) *all_launches = all;
-            *bytes_per_launch = s->spmv_bytes(s->precision == MP_PREC_MIXED ? 1 : 0);
+            *bytes_per_launch = which == 0 ? s->spmv_bytes(s->precision == MP_PREC_MIXED ? 1 : 0)
+                                           : s->smooth_bytes();
         } else {
             KernelTimer& t = c->timers[which];
             long all = 0;

@@ -1003,6 +1003,7 @@ PpeSolver::~PpeSolver() {
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
     spmv_timer.release();
+    smooth_timer.release();
 }
 
 void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Seg>& systems,
@@ -1108,6 +1109,7 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
         }
         amg.reset(new AmgDevice());
         amg->setup(st, s0, aval.p, s0k, a32k, blocks, mp_amg::Options{});
+        amg->post_timer = &smooth_timer;
         if (nseg > 1) amg->set_border_rows(st, border_rows);
         for (DevBuf<double>* b : {&yv, &zv}) {
             b->alloc(vlen);
@@ -1152,6 +1154,16 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
             }
     }
     seg_stats.assign(nseg, SolveStats{});
+}
+
+double PpeSolver::smooth_bytes() const {
+    // per entry: BF16 value + column (2 B in slices with 16-bit columns, plus
+    // their window bases, else 4 B); per row: the gathered FP32 input, b read
+    // (FP64), D^-1 (FP32), the output written in FP64 and FP32, length
+    if (!amg) return 0.0;
+    const double n16 = (double)nnz16;
+    return (double)nnz * 2.0 + 2.0 * n16 + 4.0 * ((double)nnz - n16) +
+           (double)nslices16 * 4.0 * CW_COUNT + (double)nv * (4.0 + 8.0 + 4.0 + 8.0 + 4.0 + 4.0);
 }
 
 double PpeSolver::spmv_bytes(int fp32) const {
@@ -1232,7 +1244,7 @@ void PpeSolver::launch_chunk(hipStream_t st, int chunk, const TM* a, const T* di
 template <typename TM, typename T>
 void PpeSolver::run_chunk(hipStream_t st, int chunk, const TM* a, const T* di, T* x, T* rr,
                           T* rr0, T* pp, T* vv, T* ss, T* tt) {
-    if (!use_graphs || spmv_timer.enabled) {
+    if (!use_graphs || spmv_timer.enabled || smooth_timer.enabled) {  // (events stay out of graphs)
         launch_chunk<TM, T>(st, chunk, a, di, x, rr, rr0, pp, vv, ss, tt);
         return;
     }
@@ -1672,6 +1684,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                                   hipMemcpyDeviceToHost, st));
     MP_HIP(hipStreamSynchronize(st));
     if (spmv_timer.enabled) spmv_timer.harvest();
+    if (smooth_timer.enabled) smooth_timer.harvest();
     SolveStats agg;
     double rr_all = 0, bb_all = 0;
     agg.converged = 1;

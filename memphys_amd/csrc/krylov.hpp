@@ -114,6 +114,7 @@ struct PpeSolver {
     int maxit = 1000, check_every = 8;
     long fp64_fallbacks = 0;
     KernelTimer spmv_timer;
+    KernelTimer smooth_timer;         // the multigrid's level-0 post-smoothing product
     bool use_graphs = true;           // replay each chunk of iterations from a hipGraph
     std::vector<ChunkGraph> graphs;
     long total_iterations = 0, total_solves = 0;
@@ -130,6 +131,8 @@ struct PpeSolver {
     // seg_stats).
     SolveStats solve(hipStream_t s, const double* b_dev, double* x_dev);
     double spmv_bytes(int fp32) const;
+    // algorithmic bytes of the level-0 post-smoothing product (BF16 values)
+    double smooth_bytes() const;
     // r = b - A x (FP64, unscaled operator incl. border) into `tmp`; returns tmp.p
     double* residual_of(hipStream_t s, const double* b_dev, const double* x_dev);
     // Drops the captured chunk graphs (their kernel arguments point into the
