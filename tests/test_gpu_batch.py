@@ -30,13 +30,13 @@ def _problems():
     return out
 
 
-def _run_single(cloud, re, dt, steps, precision, precond=0, krylov=0):
+def _run_single(cloud, re, dt, steps, precision, precond=0, krylov=0, tol=1e-10):
     import memphys_amd as M
     with M.Context(0) as ctx:
         ctx.upload(cloud)
         ctx.weights(3, 3, download=False)
         u_dir, p_dir, u0 = M.lid_conditions(cloud)
-        ctx.fs_create(1.0, 1.0 / re, dt, 2, 1e-10, 20000, u_dir, u_dir, None, p_dir,
+        ctx.fs_create(1.0, 1.0 / re, dt, 2, tol, 20000, u_dir, u_dir, None, p_dir,
                       precision=precision, precond=precond, krylov=krylov)
         ctx.set_field(M.MP_U, u0)
         ctx.set_field(M.MP_V, np.zeros(cloud.nv))
@@ -48,13 +48,13 @@ def _run_single(cloud, re, dt, steps, precision, precond=0, krylov=0):
         return ([ctx.get_field(w) for w in (M.MP_U, M.MP_V, M.MP_P)], iters)
 
 
-def _run_batch(probs, steps, precision, precond=0, krylov=0):
+def _run_batch(probs, steps, precision, precond=0, krylov=0, tol=1e-10):
     import memphys_amd as M
     with M.Context(0) as ctx:
         cat = ctx.upload_batch([p[0] for p in probs])
         ctx.weights(3, 3, download=False)
         u_dir, p_dir, u0 = M.lid_conditions(cat)
-        ctx.fs_create(1.0, 0.01, 1e-4, 2, 1e-10, 20000, u_dir, u_dir, None, p_dir,
+        ctx.fs_create(1.0, 0.01, 1e-4, 2, tol, 20000, u_dir, u_dir, None, p_dir,
                       precision=precision, precond=precond, krylov=krylov)
         ctx.batch_params([1.0] * len(probs), [1.0 / p[1] for p in probs], [p[2] for p in probs])
         assert ctx.n_sys == cat.nv + len(probs)  # one border unknown per cavity
@@ -117,16 +117,18 @@ def test_batch_multigrid_matches_single_and_oracle(precision):
     """One multigrid hierarchy per cavity, stored block-diagonally: each
     cavity's fields match its stand-alone multigrid run and the oracle (the
     hierarchies of a batch share one depth, so they may differ from the
-    stand-alone ones, and the fields agree to the solver tolerance, not bit
-    for bit)."""
+    stand-alone ones, and the fields agree to the solver's accuracy, not bit
+    for bit: the solves run to 1e-12 here, since a full-Neumann pressure solved
+    to a relative residual of 1e-10 is fixed to ~1e-8 only)."""
     import memphys_amd as M
     import oracle as O
     probs = _problems()
     steps = 4
-    batch = _run_batch(probs, steps, precision, precond=M.MP_PRECOND_AMG)
-    jac = _run_batch(probs, steps, precision)
+    batch = _run_batch(probs, steps, precision, precond=M.MP_PRECOND_AMG, tol=1e-12)
+    jac = _run_batch(probs, steps, precision, tol=1e-12)
     for q, (cloud, re, dt) in enumerate(probs):
-        fields, iters = _run_single(cloud, re, dt, steps, precision, precond=M.MP_PRECOND_AMG)
+        fields, iters = _run_single(cloud, re, dt, steps, precision, precond=M.MP_PRECOND_AMG,
+                                    tol=1e-12)
         bf, bi = batch[q]
         _print(f"precision {precision} problem {q}: multigrid iterations batch {bi} "
                f"single {iters}, Jacobi batch {jac[q][1]}")
