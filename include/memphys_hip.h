@@ -167,6 +167,11 @@ int mp_operator_eigenvalue(mp_ctx* ctx, int op, double tol, int max_restarts, do
  * is bordered to nv+1 (ones column and row, zero corner). */
 int mp_solver_init(mp_ctx* ctx, const uint8_t* dirichlet, double unsteady, double conv,
                    double diff, double tolerance, int n_iter, int precision);
+/* Krylov method of the solver made by mp_solver_init (SOLVER::init's Hypre
+ * GMRES branch, solver.cpp:41-53 and 140-144, restart length gmres_kdim):
+ * enum mp_krylov and the GMRES restart length 1..31 (0 = 30).  Call after
+ * mp_solver_init; the default is BiCGSTAB. */
+int mp_solver_krylov(mp_ctx* ctx, int krylov, int restart);
 int mp_solver_system_size(mp_ctx* ctx, int* n);
 /* field_new = solve(rhs) from the guess field_old; field_old[nv] = field_new[nv]
  * when bordered.  Arrays of system size. */

@@ -32,7 +32,7 @@ MP_KRYLOV_BICGSTAB, MP_KRYLOV_GMRES = 0, 1
 HIP_SYMBOLS = (
     "mp_ctx_create", "mp_ctx_destroy", "mp_last_error", "mp_sync", "mp_cloud_upload",
     "mp_weights_assemble", "mp_weights_set", "mp_operator_apply", "mp_solver_init",
-    "mp_solver_system_size", "mp_solver_solve", "mp_fs_create", "mp_fields_set",
+    "mp_solver_system_size", "mp_solver_solve", "mp_solver_krylov", "mp_fs_create", "mp_fields_set",
     "mp_fields_get", "mp_fs_step", "mp_stats_get", "mp_guess_mode", "mp_divergence",
     "mp_kernel_timing", "mp_kernel_timing_get", "mp_guess_param_count", "mp_guess_train",
     "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
@@ -93,6 +93,7 @@ def hip_lib() -> C.CDLL:
                                        C.c_int, C.c_int]
         lib.mp_solver_system_size.argtypes = [vp, C.POINTER(C.c_int)]
         lib.mp_solver_solve.argtypes = [vp, _dp, _dp, _dp]
+        lib.mp_solver_krylov.argtypes = [vp, C.c_int, C.c_int]
         lib.mp_fs_create.argtypes = [vp, C.POINTER(FsParams), _u8, _u8, vp, _u8]
         lib.mp_fields_set.argtypes = [vp, C.c_int, _dp]
         lib.mp_fields_get.argtypes = [vp, C.c_int, _dp]
@@ -387,6 +388,10 @@ class Context:
         self._check(self.lib.mp_solver_system_size(self.h, C.byref(n)), "mp_solver_system_size")
         self.n_sys = n.value
         return n.value
+
+    def solver_krylov(self, krylov, restart=0):
+        """Krylov method of the solver made by solver_init (MP_KRYLOV_*)."""
+        self._check(self.lib.mp_solver_krylov(self.h, krylov, restart), "mp_solver_krylov")
 
     def solve(self, rhs, guess):
         xn = np.zeros(self.n_sys)

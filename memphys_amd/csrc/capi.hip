@@ -744,6 +744,21 @@ int mp_solver_init(mp_ctx* c, const uint8_t* dirichlet, double unsteady, double 
     });
 }
 
+int mp_solver_krylov(mp_ctx* c, int krylov, int restart) {
+    return guarded(c, "mp_solver_krylov", [&]() -> int {
+        if (!c->solver) return fail(c, MP_ERR_STATE, "mp_solver_krylov: mp_solver_init not called");
+        if (krylov != MP_KRYLOV_BICGSTAB && krylov != MP_KRYLOV_GMRES)
+            return fail(c, MP_ERR_ARG, "mp_solver_krylov: unknown Krylov method");
+        if (restart < 0 || restart > GM_MAX)
+            return fail(c, MP_ERR_ARG,
+                        "mp_solver_krylov: GMRES restart length must be 1.." +
+                            std::to_string(GM_MAX) + " (0 = 30)");
+        c->solver->krylov = krylov;
+        c->solver->restart_m = restart > 0 ? restart : 30;
+        return MP_OK;
+    });
+}
+
 int mp_solver_system_size(mp_ctx* c, int* n) {
     return guarded(c, "mp_solver_system_size", [&]() -> int {
         PpeSolver* s = c->solver ? c->solver.get() : c->ppe.get();

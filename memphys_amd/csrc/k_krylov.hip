@@ -640,8 +640,14 @@ __global__ __launch_bounds__(FIN) void k_gm_residual_finish(SegView sg,
         g->bb = bb;
         g->tol2 = g->tol_rel * g->tol_rel * (rel_to_rhs ? bb : rr);
         if (rel_to_rhs && bb == 0.0) g->done = 4;  // zero right-hand side: x = 0
+        g->rr_cycle = rr;
     } else {
         g->cycles += 1;
+        // restarted GMRES can stagnate (a non-normal operator, a weak
+        // preconditioner): a cycle that reduced |r| by less than 10% stops
+        // the system as stalled (done = 5), for the caller's fallback
+        if (g->done == 0 && !(rr < 0.81 * g->rr_cycle)) g->done = 5;
+        g->rr_cycle = rr;
     }
     g->g[0] = sqrt(rr);
     g->rr = rr;
@@ -1539,41 +1545,53 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
     // iterations each).  BiCGSTAB stops on its recursively updated residual,
     // which drifts from b - A x by rounding; a system whose true residual misses
     // the tolerance restarts from its current x (at most 6 runs).
-    auto fp64_solve = [&](std::vector<char> todo, const std::vector<int>& budget) {
+    // bicg: BiCGSTAB from the start (the mixed form's fallback); otherwise the
+    // configured method, a GMRES system switching to BiCGSTAB (from its
+    // current x) once it stalls.  Each round runs the GMRES systems, then the
+    // BiCGSTAB ones, so every system follows the same sequence as when alone.
+    auto fp64_solve = [&](std::vector<char> todo, const std::vector<int>& budget, bool bicg) {
         std::vector<int> used(nseg, 0), runs(nseg, 0);
+        std::vector<char> on_bicg(nseg, (bicg || krylov != 1) ? 1 : 0);
         while (true) {
-            bool any = false;
-            for (int q = 0; q < nseg; ++q) {
-                host_ctl[q].active = todo[q] ? 1 : 0;
-                host_ctl[q].maxit = budget[q] - used[q];
-                host_ctl[q].tol_rel = tol;
-                any |= todo[q] != 0;
-            }
-            if (!any) break;
-            if (krylov == 1)
-                run_gmres<double, double>(st, aval.p, dinv.p, b_dev, xt.p, 1, done, its, rst);
-            else
-                run_bicgstab<double, double>(st, aval.p, dinv.p, b_dev, xt.p, r.p, r0.p, p.p,
-                                             v.p, s.p, t.p, 1, done, its, rst);
-            bool check = false;
-            for (int q = 0; q < nseg; ++q) {
-                if (!todo[q]) continue;
-                used[q] += its[q];
-                total[q] += its[q];
-                rs_total[q] += rst[q];
-                cycles[q]++;
-                runs[q]++;
-                if (done[q] == 4) {
-                    zero_system(st, xt.p, q);
-                    todo[q] = 0;
-                } else if (!(done[q] == 1 || done[q] == 5) || runs[q] >= 6 ||
-                           used[q] >= budget[q]) {
-                    todo[q] = 0;
-                } else {
-                    check = true;
+            bool any = false, check = false;
+            for (int pass = 0; pass < 2; ++pass) {
+                std::vector<char> ran(nseg, 0);
+                bool anyp = false;
+                for (int q = 0; q < nseg; ++q) {
+                    ran[q] = todo[q] && (pass == 0 ? !on_bicg[q] : on_bicg[q]);
+                    host_ctl[q].active = ran[q] ? 1 : 0;
+                    host_ctl[q].maxit = budget[q] - used[q];
+                    host_ctl[q].tol_rel = tol;
+                    anyp |= ran[q] != 0;
+                }
+                if (!anyp) continue;
+                any = true;
+                if (pass == 0)
+                    run_gmres<double, double>(st, aval.p, dinv.p, b_dev, xt.p, 1, done, its, rst);
+                else
+                    run_bicgstab<double, double>(st, aval.p, dinv.p, b_dev, xt.p, r.p, r0.p, p.p,
+                                                 v.p, s.p, t.p, 1, done, its, rst);
+                for (int q = 0; q < nseg; ++q) {
+                    if (!ran[q]) continue;
+                    used[q] += its[q];
+                    total[q] += its[q];
+                    rs_total[q] += rst[q];
+                    cycles[q]++;
+                    runs[q]++;
+                    if (done[q] == 4) {
+                        zero_system(st, xt.p, q);
+                        todo[q] = 0;
+                    } else if (pass == 0 && done[q] == 5 && used[q] < budget[q]) {
+                        on_bicg[q] = 1;  // stalled (or broke down): BiCGSTAB next
+                    } else if (!(done[q] == 1 || done[q] == 5) || runs[q] >= 6 ||
+                               used[q] >= budget[q]) {
+                        todo[q] = 0;
+                    } else {
+                        check = true;
+                    }
                 }
             }
-            if (!check) break;
+            if (!any || !check) break;
             std::vector<double> rt, bt;
             residual64(st, b_dev, rt, bt);
             for (int q = 0; q < nseg; ++q)
@@ -1581,7 +1599,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
         }
     };
     if (precision == 0) {
-        fp64_solve(std::vector<char>(nseg, 1), std::vector<int>(nseg, maxit));
+        fp64_solve(std::vector<char>(nseg, 1), std::vector<int>(nseg, maxit), false);
     } else {
         // FP64 defect correction around an inner BiCGSTAB that streams the FP32
         // copy of the operator (8 instead of 12 bytes per entry) while keeping
@@ -1668,7 +1686,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                 anyfb = true;
             }
         }
-        if (anyfb) fp64_solve(fb, budget);
+        if (anyfb) fp64_solve(fb, budget, true);
         for (int q = 0; q < nseg; ++q)
             if (bb[q] == 0) zero_system(st, xt.p, q);
     }

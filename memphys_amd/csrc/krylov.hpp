@@ -37,6 +37,7 @@ struct GmState {
     double g[GM_MAX + 1], cs[GM_MAX], sn[GM_MAX], h[GM_MAX + 1], y[GM_MAX];
     double R[GM_MAX * GM_MAX];
     double bb, tol2, tol_rel, rr, inv_eta;
+    double rr_cycle;  // |r|^2 at the start of the current cycle
     int j;       // columns of the current cycle
     int done;    // as KScalars::done (0 running, 1 converged, 2 cap, 4 zero rhs, 5 non-finite / breakdown, 6 inactive)
     int full;    // the cycle holds m columns: restart pending

@@ -446,6 +446,7 @@ void SOLVER::init(POINTS& points, CLOUD& cloud, PARAMETERS& parameters,
         fail("SOLVER::SOLVER", "solver_type should be either hypre_ilu_gmres, eigen_ilu_bicgstab or "
                                "eigen_direct; current value: " + solver_type);
     n_iter = parameters.n_iter;
+    gmres_kdim = parameters.gmres_kdim;
     dirichlet_flag_1 = dirichlet_flag;
     log_flag_1 = log_flag;
     const int nv = points.nv, dim = parameters.dimension;
@@ -469,6 +470,9 @@ void SOLVER::init(POINTS& points, CLOUD& cloud, PARAMETERS& parameters,
     check(ctx, mp_solver_init(ctx, d.data(), unsteady_term_coeff, conv_term_coeff, diff_term_coeff,
                               solver_tolerance, n_iter, parameters.precision),
           "SOLVER::init");
+    if (solver_type == "hypre_ilu_gmres")  // GMRES(gmres_kdim), solver.cpp:140-144
+        check(ctx, mp_solver_krylov(ctx, MP_KRYLOV_GMRES, std::max(1, std::min(gmres_kdim, 31))),
+              "SOLVER::init");
     parameters.factoring_timer = (double)(clock() - t1) / CLOCKS_PER_SEC;
     check(ctx, mp_solver_system_size(ctx, &system_size), "SOLVER::init");
 }

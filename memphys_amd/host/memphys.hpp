@@ -200,6 +200,7 @@ class SOLVER {
     ~SOLVER();
     std::string solver_type;
     int n_iter = 0;
+    int gmres_kdim = 0;             // GMRES restart length (hypre_ilu_gmres)
     double solver_tolerance = 0;
     std::vector<bool> dirichlet_flag_1;
     int system_size = 0;

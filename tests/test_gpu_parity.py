@@ -71,7 +71,7 @@ def test_operator_apply_matches_oracle(ctx, cube3d_small):
         assert rel_err(y, yo) <= 1e-14
 
 
-def _ppe_case(ctx, cloud, precision):
+def _ppe_case(ctx, cloud, precision, krylov=0):
     import memphys_amd as M
     import oracle as O
     import scipy.sparse as sp
@@ -84,6 +84,8 @@ def _ppe_case(ctx, cloud, precision):
     # these operators (entries ~1e5) is ~1e-12
     n = ctx.solver_init(p_dir, 0.0, 0.0, 1.0, tol=1e-10, n_iter=20000, precision=precision)
     assert n == cloud.nv + 1
+    if krylov:
+        ctx.solver_krylov(M.MP_KRYLOV_GMRES)
     ones = np.ones(cloud.nv, dtype=np.uint8)
     fs = O.FractionalStep(cloud, w, ones, ones, ones if cloud.dim == 3 else None, p_dir,
                           1.0, 0.01, 1e-3, 2, solver_kind=0, precond=2, tol=1e-14)
@@ -95,8 +97,8 @@ def _ppe_case(ctx, cloud, precision):
     x_ref = spl.spsolve(A.tocsc(), b)
     x, xo = ctx.solve(b, np.zeros(n))
     st = ctx.stats()
-    _print("PPE", cloud.dim, "D prec", precision, "iters", st.iterations, "cycles",
-           st.outer_cycles, "rel_res", st.rel_res)
+    _print("PPE", cloud.dim, "D prec", precision, "krylov", krylov, "iters", st.iterations,
+           "cycles", st.outer_cycles, "rel_res", st.rel_res)
     assert st.rel_res <= 1e-10
     assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 1e-10
     _print("x vs SuperLU rel", rel_err(x, x_ref))
@@ -104,14 +106,16 @@ def _ppe_case(ctx, cloud, precision):
     assert xo[-1] == x[-1]
 
 
-def test_ppe_solve_fp64_matches_superlu(ctx, cavity2d):
+@pytest.mark.parametrize("krylov", [0, 1])
+def test_ppe_solve_fp64_matches_superlu(ctx, cavity2d, krylov):
     import memphys_amd as M
-    _ppe_case(ctx, cavity2d, M.MP_PREC_FP64)
+    _ppe_case(ctx, cavity2d, M.MP_PREC_FP64, krylov)
 
 
-def test_ppe_solve_mixed_matches_superlu(ctx, cube3d_small):
+@pytest.mark.parametrize("krylov", [0, 1])
+def test_ppe_solve_mixed_matches_superlu(ctx, cube3d_small, krylov):
     import memphys_amd as M
-    _ppe_case(ctx, cube3d_small, M.MP_PREC_MIXED)
+    _ppe_case(ctx, cube3d_small, M.MP_PREC_MIXED, krylov)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
