@@ -54,13 +54,14 @@ def lid_bc(cloud):
     return u_dir, p_dir, u0
 
 
-def measured_traffic(n: int, kernel_index: int = 0):
-    """HBM bytes per launch of the PPE SpMV measured with rocprofv3 FETCH_SIZE and
-    WRITE_SIZE runs on the 100^3 workload (index 0: the headline form's FP32-operator
-    GMRES product, 1: the FP64 form's product), read from the committed summary; None
-    for other workloads."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "ppe_spmv_traffic_cavity3d_1M.md")
+def measured_traffic(n: int, kernel_index: int = 0,
+                     summary: str = "ppe_spmv_traffic_cavity3d_1M.md"):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE runs on
+    the 100^3 workload, read from a committed summary: the `kernel_index`-th
+    `traffic_bytes_per_launch` line (PPE SpMV summary: 0 the headline form's
+    FP32-operator GMRES product, 1 the FP64 form's product; multigrid summary: 0 the
+    level-0 post-smoothing product); None for other workloads."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", summary)
     if n != 100 or not os.path.exists(path):
         return None
     vals = [float(line.split("=")[1]) for line in open(path)
@@ -127,10 +128,11 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
                 poll_interval=interval)
 
 
-def smooth_roofline(t, copy_gbps):
+def smooth_roofline(t, copy_gbps, n_lattice: int = 0):
     """Roofline entry of the multigrid's level-0 post-smoothing product (the
     V-cycle's largest kernel): algorithmic bytes per launch over its mean
-    HIP-event duration, launches that did the work."""
+    HIP-event duration, launches that did the work; `traffic` is its measured
+    HBM bytes per launch on the 100^3 workload (None on others)."""
     if not t or t[1] <= 0:
         return None
     n, ms, bytes_l, all_l = t
@@ -138,7 +140,8 @@ def smooth_roofline(t, copy_gbps):
     return {"kernel": "k_amg_op<0,2,double,double> (level-0 post-smoothing, BF16 values)",
             "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "frac_of_copy": ach / copy_gbps, "bytes_per_launch": bytes_l,
-            "mean_ms": ms, "launches": n, "launches_returned_early": all_l - n}
+            "mean_ms": ms, "launches": n, "launches_returned_early": all_l - n,
+            "traffic": measured_traffic(n_lattice, 0, "amg_level0_traffic_cavity3d_1M.md")}
 
 
 def stream_copy_gbps(mib: int = 1024, reps: int = 10) -> float:
@@ -508,7 +511,7 @@ def main():
                                "bytes_per_launch": b64, "mean_ms": ms64, "launches": n64,
                                "launches_returned_early": all64 - n64,
                                "traffic": measured_traffic(args.n, 1)},
-        "amg_level0_roofline": smooth_roofline(head.get("smooth"), copy_gbps),
+        "amg_level0_roofline": smooth_roofline(head.get("smooth"), copy_gbps, args.n),
         "cloud_setup": setup,
         "weights_kernel_ms": wk[1],
         "weights_fp64_vs_fp32_refined": weights_check,

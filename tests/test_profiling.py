@@ -79,3 +79,24 @@ def test_traffic_scales_counters_by_the_copy_kernel(tmp_path):
     assert vals["fetch_bytes_per_launch"] == pytest.approx(1000.0 * 1024)
     assert vals["write_bytes_per_launch"] == pytest.approx(8.0 * 1024)
     assert vals["traffic_bytes_per_launch"] == pytest.approx(1008.0 * 1024)
+
+
+def test_bench_reads_committed_traffic_summaries():
+    # The bench's `traffic` fields come from the committed rocprofv3 summaries;
+    # each must be the measured figure of its own kernel, within 1 % of that
+    # kernel's algorithmic bytes per launch (SpMV FP32 279,638,428; FP64
+    # 545,879,136; multigrid level-0 post-smoothing 202,641,996), and absent
+    # on other workloads.
+    import importlib.util
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for got, alg in [(bench.measured_traffic(100, 0), 279_638_428),
+                     (bench.measured_traffic(100, 1), 545_879_136),
+                     (bench.smooth_roofline((381, 0.06, 202_641_996.0, 393), 6000.0,
+                                            100)["traffic"], 202_641_996)]:
+        assert got is not None and abs(got / alg - 1) < 0.01
+    assert bench.measured_traffic(50, 0) is None
+    assert bench.smooth_roofline((381, 0.06, 2e8, 393), 6000.0, 50)["traffic"] is None
