@@ -645,8 +645,10 @@ __global__ __launch_bounds__(FIN) void k_gm_residual_finish(SegView sg,
         g->cycles += 1;
         // restarted GMRES can stagnate (a non-normal operator, a weak
         // preconditioner): a cycle that reduced |r| by less than 10% stops
-        // the system as stalled (done = 5), for the caller's fallback
-        if (g->done == 0 && !(rr < 0.81 * g->rr_cycle)) g->done = 5;
+        // the system as stalled (done = 5), for the caller's fallback; a
+        // cycle that ended within the tolerance has converged, however little
+        // it reduced |r| (a non-finite rr is caught below)
+        if (g->done == 0 && rr > g->tol2 && !(rr < 0.81 * g->rr_cycle)) g->done = 5;
         g->rr_cycle = rr;
     }
     g->g[0] = sqrt(rr);
