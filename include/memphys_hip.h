@@ -83,6 +83,11 @@ typedef struct {
     double guess_rel_res; /* |A p0 - b| / |b| of the initial guess */
     long total_iterations;/* sum over all solves since context creation */
     long total_solves;
+    int gmres_switches;   /* GMRES cycles that stalled (|r| reduced by < 10%) or broke down,
+                             after which the system continued with BiCGSTAB */
+    int guess_rejected;   /* 1 when the initial guess was replaced: a non-finite guess of a
+                             finite system starts from zero, a learned guess whose residual
+                             exceeds its linear extrapolation's starts from the extrapolation */
 } mp_step_stats;
 
 /* Context ------------------------------------------------------------------ */

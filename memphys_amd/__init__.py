@@ -61,6 +61,7 @@ class StepStats(C.Structure):
         ("converged", C.c_int), ("rel_res", C.c_double), ("abs_res", C.c_double),
         ("alpha", C.c_double), ("guess_rel_res", C.c_double),
         ("total_iterations", C.c_long), ("total_solves", C.c_long),
+        ("gmres_switches", C.c_int), ("guess_rejected", C.c_int),
     ]
 
 
@@ -148,6 +149,7 @@ def host_lib() -> C.CDLL:
                                             C.POINTER(C.c_int), _ip, _ip, _dp, C.c_long, _ip,
                                             _ip, _dp, C.c_long]
         lib.mph_amg_bordered_inverse.argtypes = [C.c_int, _ip, _ip, _dp, _dp]
+        lib.mph_amg_dense_inverse.argtypes = [C.c_int, _ip, _ip, _dp, _dp]
         lib.mph_hessenberg_eig.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp]
         lib.mph_gmsh_read.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -652,6 +654,20 @@ def amg_bordered_inverse(A) -> np.ndarray:
                                              np.ascontiguousarray(A.data, dtype=np.float64), G)
     if rc != 0:
         raise MemphysError("mph_amg_bordered_inverse failed")
+    return G.reshape(n, n)
+
+
+def amg_dense_inverse(A) -> np.ndarray:
+    """The library's coarsest-level solve of a system with Dirichlet rows: inv(A)."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    G = np.zeros(n * n)
+    rc = host_lib().mph_amg_dense_inverse(n, np.ascontiguousarray(A.indptr, dtype=np.int32),
+                                          np.ascontiguousarray(A.indices, dtype=np.int32),
+                                          np.ascontiguousarray(A.data, dtype=np.float64), G)
+    if rc != 0:
+        raise MemphysError("mph_amg_dense_inverse failed")
     return G.reshape(n, n)
 
 

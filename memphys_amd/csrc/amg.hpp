@@ -59,12 +59,15 @@ struct AmgDevice {
     // for the host set-up; the finest level's products run on sv_apply (owned
     // by the solver, sv_apply.nv rows, possibly another order of each row's
     // slots) with a BF16 copy of val32_dev, given in sv_apply's slots.
-    // blocks: (first device row, rows) of each problem.  A batch gets one
+    // blocks: (first device row, rows) of each problem; bordered[q]: problem q
+    // is a full-Neumann (bordered) system, whose coarsest solve is the
+    // bordered dense inverse (else the plain inverse).  A batch gets one
     // hierarchy per problem, all of one depth, stored block-diagonally, so
     // every level's products run over all problems at once.
     void setup(hipStream_t s, const SellView& sv, const double* val64_dev,
                const SellView& sv_apply, const float* val32_dev,
-               const std::vector<std::pair<int, int>>& blocks, const mp_amg::Options& opt);
+               const std::vector<std::pair<int, int>>& blocks, const std::vector<int>& bordered,
+               const mp_amg::Options& opt);
     // device rows of a batch's border unknowns (copied through the V-cycle)
     void set_border_rows(hipStream_t s, const std::vector<int>& r);
     // Layout of every hierarchy matrix chosen by timing each candidate (SELL-64x4,

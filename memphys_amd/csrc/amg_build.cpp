@@ -250,16 +250,12 @@ Csr multiply(const Csr& A, const Csr& B, int threads) {
     return C;
 }
 
-std::vector<double> bordered_inverse(const Csr& A) {
-    const int n = A.nrows, m = n + 1;
-    // Gauss-Jordan with partial pivoting on [M | I], M = [[A, 1], [1^T, 0]]
-    std::vector<double> M((size_t)m * m, 0.0), I((size_t)m * m, 0.0);
-    for (int i = 0; i < n; ++i)
-        for (int t = A.ptr[i]; t < A.ptr[i + 1]; ++t) M[(size_t)i * m + A.col[t]] += A.val[t];
-    for (int i = 0; i < n; ++i) {
-        M[(size_t)i * m + n] = 1.0;
-        M[(size_t)n * m + i] = 1.0;
-    }
+namespace {
+
+// Gauss-Jordan with partial pivoting on [M | I] (m x m, row-major); returns
+// M^-1.
+std::vector<double> gauss_jordan(std::vector<double> M, int m) {
+    std::vector<double> I((size_t)m * m, 0.0);
     for (int i = 0; i < m; ++i) I[(size_t)i * m + i] = 1.0;
     for (int k = 0; k < m; ++k) {
         int piv = k;
@@ -292,13 +288,37 @@ std::vector<double> bordered_inverse(const Csr& A) {
             }
         }
     }
+    return I;
+}
+
+}  // namespace
+
+std::vector<double> bordered_inverse(const Csr& A) {
+    const int n = A.nrows, m = n + 1;
+    // M = [[A, 1], [1^T, 0]]
+    std::vector<double> M((size_t)m * m, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int t = A.ptr[i]; t < A.ptr[i + 1]; ++t) M[(size_t)i * m + A.col[t]] += A.val[t];
+    for (int i = 0; i < n; ++i) {
+        M[(size_t)i * m + n] = 1.0;
+        M[(size_t)n * m + i] = 1.0;
+    }
+    const std::vector<double> I = gauss_jordan(std::move(M), m);
     std::vector<double> G((size_t)n * n);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) G[(size_t)i * n + j] = I[(size_t)i * m + j];
     return G;
 }
 
-Hierarchy build(const Csr& A0, const Options& opt) {
+std::vector<double> dense_inverse(const Csr& A) {
+    const int n = A.nrows;
+    std::vector<double> M((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int t = A.ptr[i]; t < A.ptr[i + 1]; ++t) M[(size_t)i * n + A.col[t]] += A.val[t];
+    return gauss_jordan(std::move(M), n);
+}
+
+Hierarchy build(const Csr& A0, const Options& opt, bool bordered) {
     if (A0.nrows != A0.ncols || A0.nrows < 1) throw std::runtime_error("amg: operator not square");
     Hierarchy h;
     h.A.emplace_back();  // level 0 stays with the caller
@@ -322,7 +342,7 @@ Hierarchy build(const Csr& A0, const Options& opt) {
         throw std::runtime_error("amg: coarsest level has " + std::to_string(cur->nrows) +
                                  " rows (dense limit 4096)");
     h.nc = cur->nrows;
-    h.coarse_inv = bordered_inverse(*cur);
+    h.coarse_inv = bordered ? bordered_inverse(*cur) : dense_inverse(*cur);
     return h;
 }
 

@@ -12,9 +12,10 @@
 // rescaled to keep the row sum, so constants are still reproduced); R = P^T;
 // A_{l+1} = R A_l P (Galerkin), optionally sparsified (off-diagonal entries
 // below drop x sqrt|a_ii a_jj| moved onto the diagonal, which keeps row sums
-// and so the constant null vector).  The coarsest level is solved with the dense
-// inverse of its matrix bordered by a ones row and column (the constant mode
-// is singular there, as in the full-Neumann pressure system).
+// and so the constant null vector).  The coarsest level of a full-Neumann
+// (bordered) system is solved with the dense inverse of its matrix bordered by
+// a ones row and column (the constant mode is singular there); a system with
+// Dirichlet rows is nonsingular and uses the plain dense inverse.
 #pragma once
 #include <vector>
 
@@ -55,6 +56,10 @@ Csr transpose(const Csr& A);
 Csr multiply(const Csr& A, const Csr& B, int threads);
 // Top-left n x n block of the inverse of [[A, 1], [1^T, 0]] (A dense n x n).
 std::vector<double> bordered_inverse(const Csr& A);
-Hierarchy build(const Csr& A0, const Options& opt);
+// Inverse of A (dense n x n, row-major); throws when A is singular.
+std::vector<double> dense_inverse(const Csr& A);
+// bordered: the coarsest solve is bordered_inverse (full-Neumann system),
+// else dense_inverse.
+Hierarchy build(const Csr& A0, const Options& opt, bool bordered = true);
 
 }  // namespace mp_amg

@@ -158,6 +158,20 @@ std::vector<Seg> make_systems(const mp_ctx* c, const uint8_t* dir) {
     return out;
 }
 
+// The solve's statistics into the ABI record (totals are the caller's).
+void put_stats(mp_step_stats& o, const SolveStats& st) {
+    o.iterations = st.iterations;
+    o.outer_cycles = st.outer_cycles;
+    o.restarts = st.restarts;
+    o.converged = st.converged;
+    o.rel_res = st.rel_res;
+    o.abs_res = st.abs_res;
+    o.alpha = st.alpha;
+    o.guess_rel_res = st.guess_rel_res;
+    o.gmres_switches = st.gmres_switches;
+    o.guess_rejected = st.guess_rejected;
+}
+
 void upload(const mp_ctx* c, double* dev, const std::vector<double>& h) {
     MP_HIP(hipMemcpyAsync(dev, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
                           c->stream));
@@ -782,14 +796,7 @@ int mp_solver_solve(mp_ctx* c, const double* rhs, double* field_new, double* fie
         download(c, c->vec_b.p, hx);
         from_device_order(c, hx, field_new, &S);
         for (int k = c->nv; k < S.n_sys; ++k) field_old[k] = field_new[k];  // solver.cpp:361-362
-        c->stats.iterations = st.iterations;
-        c->stats.outer_cycles = st.outer_cycles;
-        c->stats.restarts = st.restarts;
-        c->stats.converged = st.converged;
-        c->stats.rel_res = st.rel_res;
-        c->stats.abs_res = st.abs_res;
-        c->stats.alpha = st.alpha;
-        c->stats.guess_rel_res = st.guess_rel_res;
+        put_stats(c->stats, st);
         c->stats.total_iterations += st.iterations;
         c->stats.total_solves += 1;
         return MP_OK;
@@ -991,14 +998,7 @@ void fs_step_one(mp_ctx* c, int it) {
     // extras (AB2 history) and the driver's old <- new
     if (q.temporal_order == 2) c->s_cur ^= 1;
     c->cur = nw;
-    c->stats.iterations = st.iterations;
-    c->stats.outer_cycles = st.outer_cycles;
-    c->stats.restarts = st.restarts;
-    c->stats.converged = st.converged;
-    c->stats.rel_res = st.rel_res;
-    c->stats.abs_res = st.abs_res;
-    c->stats.alpha = st.alpha;
-    c->stats.guess_rel_res = st.guess_rel_res;
+    put_stats(c->stats, st);
     c->stats.total_iterations += st.iterations;
     c->stats.total_solves += 1;
     c->prob_total_iters.resize(c->nprob, 0);
@@ -1188,14 +1188,7 @@ int mp_batch_stats(mp_ctx* c, int problem, mp_step_stats* stats) {
             return fail(c, MP_ERR_ARG, "mp_batch_stats: bad problem index");
         const SolveStats& g = c->ppe->seg_stats[problem];
         mp_step_stats o{};
-        o.iterations = g.iterations;
-        o.outer_cycles = g.outer_cycles;
-        o.restarts = g.restarts;
-        o.converged = g.converged;
-        o.rel_res = g.rel_res;
-        o.abs_res = g.abs_res;
-        o.alpha = g.alpha;
-        o.guess_rel_res = g.guess_rel_res;
+        put_stats(o, g);
         o.total_iterations = c->prob_total_iters.empty() ? 0 : c->prob_total_iters[problem];
         o.total_solves = c->stats.total_solves;
         *stats = o;

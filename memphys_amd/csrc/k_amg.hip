@@ -409,9 +409,11 @@ mp_amg::Csr block_diag(const std::vector<const mp_amg::Csr*>& m, const std::vect
 
 void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_dev,
                       const SellView& sv_apply, const float* val32_dev,
-                      const std::vector<std::pair<int, int>>& blocks, const mp_amg::Options& opt) {
+                      const std::vector<std::pair<int, int>>& blocks,
+                      const std::vector<int>& bordered, const mp_amg::Options& opt) {
     const auto t0 = std::chrono::steady_clock::now();
     if (blocks.empty()) throw std::runtime_error("amg: no problems");
+    if (bordered.size() != blocks.size()) throw std::runtime_error("amg: border flags");
     // finest operator of each problem as CSR in device order, local indices
     // (padding slots dropped)
     std::vector<int> hsptr(sv.nslices + 1), hlen((size_t)sv.nslices * SLICE);
@@ -454,7 +456,7 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
     std::vector<mp_amg::Hierarchy> hs(nb);
     int L = 2;
     for (int q = 0; q < nb; ++q) {
-        hs[q] = mp_amg::build(A0s[q], opt);
+        hs[q] = mp_amg::build(A0s[q], opt, bordered[q] != 0);
         L = std::max(L, hs[q].levels());
     }
     for (int q = 0; q < nb; ++q)
@@ -462,7 +464,7 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
             mp_amg::Options o = opt;
             o.coarse_max = 0;
             o.max_levels = L;
-            hs[q] = mp_amg::build(A0s[q], o);
+            hs[q] = mp_amg::build(A0s[q], o, bordered[q] != 0);
             if (hs[q].levels() != L)
                 throw std::runtime_error("amg: problem " + std::to_string(q) +
                                          " cannot be coarsened to " + std::to_string(L) +

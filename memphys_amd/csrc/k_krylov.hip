@@ -645,10 +645,12 @@ __global__ __launch_bounds__(FIN) void k_gm_residual_finish(SegView sg,
         g->cycles += 1;
         // restarted GMRES can stagnate (a non-normal operator, a weak
         // preconditioner): a cycle that reduced |r| by less than 10% stops
-        // the system as stalled (done = 5), for the caller's fallback; a
-        // cycle that ended within the tolerance has converged, however little
-        // it reduced |r| (a non-finite rr is caught below)
-        if (g->done == 0 && rr > g->tol2 && !(rr < 0.81 * g->rr_cycle)) g->done = 5;
+        // the system as stalled (done = 7; x keeps the cycle's result), for the
+        // caller's fallback; a cycle that ended within the tolerance has
+        // converged, however little it reduced |r| (a non-finite rr is caught
+        // below)
+        if (g->done == 0 && rr > g->tol2 && !(rr < 0.81 * g->rr_cycle) && rr <= 1.0e300)
+            g->done = 7;
         g->rr_cycle = rr;
     }
     g->g[0] = sqrt(rr);
@@ -1110,13 +1112,14 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
         s0.nv = s0k.nv = nseg == 1 ? segs[0].nv : nrows;
         s0.nslices = s0k.nslices = (s0.nv + SLICE - 1) / SLICE;
         std::vector<std::pair<int, int>> blocks;
-        std::vector<int> border_rows;
+        std::vector<int> border_rows, border_flag;
         for (const Seg& S : segs) {
             blocks.emplace_back(S.row0, S.nv);
+            border_flag.push_back(S.border);
             if (S.border) border_rows.push_back(S.row0 + S.nv);
         }
         amg.reset(new AmgDevice());
-        amg->setup(st, s0, aval.p, s0k, a32k, blocks, mp_amg::Options{});
+        amg->setup(st, s0, aval.p, s0k, a32k, blocks, border_flag, mp_amg::Options{});
         amg->post_timer = &smooth_timer;
         if (nseg > 1) amg->set_border_rows(st, border_rows);
         for (DevBuf<double>* b : {&yv, &zv}) {
@@ -1541,6 +1544,18 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
     for (int q = 0; q < nseg; ++q)
         out[q].guess_rel_res = !(std::isfinite(rr0[q]) && std::isfinite(bb[q])) ? std::nan("")
                                : (bb[q] > 0) ? std::sqrt(rr0[q] / bb[q]) : 0.0;
+    {
+        // a system with a finite right-hand side whose guess is not finite (or
+        // leaves a non-finite residual) starts from zero instead
+        bool reset = false;
+        for (int q = 0; q < nseg; ++q)
+            if (std::isfinite(bb[q]) && !std::isfinite(rr0[q])) {
+                zero_system(st, xt.p, q);
+                out[q].guess_rejected = 1;
+                reset = true;
+            }
+        if (reset) residual64(st, b_dev, rr0, bb);
+    }
     std::vector<int> total(nseg, 0), cycles(nseg, 0), rs_total(nseg, 0);
     std::vector<int> done(nseg, 0), its(nseg, 0), rst(nseg, 0);
     // FP64 BiCGSTAB from the current x~ for the systems in `todo` (budget:
@@ -1583,8 +1598,10 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                     if (done[q] == 4) {
                         zero_system(st, xt.p, q);
                         todo[q] = 0;
-                    } else if (pass == 0 && done[q] == 5 && used[q] < budget[q]) {
+                    } else if (pass == 0 && (done[q] == 5 || done[q] == 7) &&
+                               used[q] < budget[q]) {
                         on_bicg[q] = 1;  // stalled (or broke down): BiCGSTAB next
+                        out[q].gmres_switches++;
                     } else if (!(done[q] == 1 || done[q] == 5) || runs[q] >= 6 ||
                                used[q] >= budget[q]) {
                         todo[q] = 0;
@@ -1640,9 +1657,17 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                 rs_total[q] += rst[q];
                 cycles[q]++;
                 if (done[q] == 5 || its[q] == 0) {
-                    fallback[q] = 1;  // the correction of this cycle is dropped
+                    // breakdown or non-finite values: the correction is dropped
+                    fallback[q] = 1;
                     act[q] = 0;
                     zero_system(st, e64.p, q);
+                    if (krylov == 1) out[q].gmres_switches++;
+                } else if (done[q] == 7) {
+                    // a stalled restart cycle: e holds every productive cycle's
+                    // correction; the FP64 residual test below keeps or undoes it,
+                    // and the rest of the solve runs on FP64 BiCGSTAB
+                    fallback[q] = 1;
+                    out[q].gmres_switches++;
                 }
             }
             // x~ before the correction, kept in rhs64 (free once the inner
@@ -1666,7 +1691,8 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                     continue;
                 }
                 rr_prev[q] = rr[q];
-                act[q] = rr[q] > tol2[q];
+                act[q] = rr[q] > tol2[q] && !fallback[q];
+                if (!act[q]) fallback[q] = fallback[q] && rr[q] > tol2[q];
             }
             if (any_undo)
                 for (int q = 0; q < nseg; ++q)

@@ -10,6 +10,8 @@ namespace mp {
 struct SolveStats {
     int iterations = 0, outer_cycles = 0, restarts = 0, converged = 0;
     double rel_res = 0, abs_res = 0, alpha = 0, guess_rel_res = 0;
+    int gmres_switches = 0;  // GMRES runs handed to BiCGSTAB (stall or breakdown)
+    int guess_rejected = 0;  // the initial guess was replaced (see PpeSolver::solve)
 };
 
 // HIP-event timing of one kernel's launches.  Krylov kernels of a system that
@@ -39,7 +41,8 @@ struct GmState {
     double bb, tol2, tol_rel, rr, inv_eta;
     double rr_cycle;  // |r|^2 at the start of the current cycle
     int j;       // columns of the current cycle
-    int done;    // as KScalars::done (0 running, 1 converged, 2 cap, 4 zero rhs, 5 non-finite / breakdown, 6 inactive)
+    int done;    // as KScalars::done (0 running, 1 converged, 2 cap, 4 zero rhs, 5 non-finite / breakdown, 6 inactive),
+                 // and 7: stalled (a restart cycle reduced |r| by less than 10%; x holds its result)
     int full;    // the cycle holds m columns: restart pending
     int stop;    // done != 0 or full: the iteration kernels return at entry
     int apply;   // x += Z y pending with napply columns

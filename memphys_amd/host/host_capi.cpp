@@ -209,4 +209,21 @@ int mph_amg_bordered_inverse(int n, const int* ptr, const int* col, const double
     }
 }
 
+// Inverse of A (the coarsest multigrid solve of a system with Dirichlet
+// rows), row-major into G.
+int mph_amg_dense_inverse(int n, const int* ptr, const int* col, const double* val, double* G) {
+    try {
+        mp_amg::Csr A;
+        A.nrows = A.ncols = n;
+        A.ptr.assign(ptr, ptr + n + 1);
+        A.col.assign(col, col + ptr[n]);
+        A.val.assign(val, val + ptr[n]);
+        std::vector<double> g = mp_amg::dense_inverse(A);
+        std::memcpy(G, g.data(), sizeof(double) * g.size());
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
 }  // extern "C"

@@ -72,6 +72,9 @@ def lib() -> C.CDLL:
                                     _vp, _dp]
         L.or_divergence.restype = None
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_coeff_export.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _vp, _dp, _dp, _u8,
+                                      _u8, C.c_double, C.c_double, C.c_double, _vp, _vp, _vp]
+        L.or_coeff_export.restype = C.c_int
         _lib = L
     return _lib
 
@@ -205,6 +208,24 @@ class FractionalStep:
         it, rr, ra = C.c_int(), C.c_double(), C.c_double()
         lib().or_fs_last(self.h, C.byref(it), C.byref(rr), C.byref(ra))
         return it.value, rr.value, ra.value
+
+
+def coeff_matrix(cloud, w, dirichlet, unsteady=0.0, conv=0.0, diff=1.0):
+    """SOLVER::init + calc_coeff_matrix (solver.cpp:29-38, 165-214) as a scipy
+    CSR matrix: n = nv + 1 (full Neumann, bordered) or nv."""
+    import scipy.sparse as sp
+    gx, gy, gz, lap = (np.ascontiguousarray(a) if a is not None else None for a in w)
+    d = np.ascontiguousarray(dirichlet, dtype=np.uint8)
+    args = (cloud.nv, cloud.dim, cloud.row_ptr, cloud.col, gx, gy, _ptr(gz), lap,
+            np.ascontiguousarray(cloud.normal.ravel()), cloud.boundary, d, float(unsteady),
+            float(conv), float(diff))
+    nnz = lib().or_coeff_export(*args, None, None, None)
+    n_max = cloud.nv + 1
+    rp = np.zeros(n_max + 1, dtype=np.int32)
+    ci = np.zeros(nnz, dtype=np.int32)
+    v = np.zeros(nnz)
+    n = lib().or_coeff_export(*args, _ptr(rp), _ptr(ci), _ptr(v))
+    return sp.csr_matrix((v, ci, rp[:n + 1]), shape=(n, n))
 
 
 def divergence(cloud, gx, gy, gz, u, v, w):

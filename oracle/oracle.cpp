@@ -798,6 +798,26 @@ void or_fs_ppe_export(OrFs* F, int* rp, int* ci, double* v) {
     std::copy(F->A.v.begin(), F->A.v.end(), v);
 }
 
+// SOLVER::init's system (solver.cpp:29-38) and calc_coeff_matrix (165-214)
+// for any (unsteady, conv, diff): bordered to nv + 1 unless a boundary node
+// is Dirichlet.  or_coeff_export with null outputs returns the entry count;
+// otherwise fills rp (n + 1), ci, v and returns n.
+int or_coeff_export(int nv, int dim, const int* rp, const int* ci, const double* gx,
+                    const double* gy, const double* gz, const double* lap, const double* normal,
+                    const uint8_t* bnd, const uint8_t* dir, double unsteady, double conv,
+                    double diff, int* out_rp, int* out_ci, double* out_v) {
+    bool bordered = true;
+    for (int i = 0; i < nv; ++i)
+        if (bnd[i] && dir[i]) { bordered = false; break; }
+    Csr A = build_coeff(nv, dim, rp, ci, gx, gy, dim == 3 ? gz : nullptr, lap, normal, bnd, dir,
+                        unsteady, conv, diff, bordered);
+    if (!out_rp || !out_ci || !out_v) return A.rp[A.n];
+    std::copy(A.rp.begin(), A.rp.end(), out_rp);
+    std::copy(A.ci.begin(), A.ci.end(), out_ci);
+    std::copy(A.v.begin(), A.v.end(), out_v);
+    return A.n;
+}
+
 // SOLVER::general_solve (solver.cpp:283-363), eigen_ilu_bicgstab / direct
 // branches: x_new = solve(rhs) from guess x_old; x_old[nv] = x_new[nv].
 int or_ppe_solve(OrFs* F, const double* rhs, double* x_new, double* x_old, double* rel_res,

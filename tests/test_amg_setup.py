@@ -126,6 +126,24 @@ def test_bordered_inverse(ppe_block):
     assert np.abs(r - r.mean()).max() <= 1e-8 * np.linalg.norm(b)
 
 
+def test_dense_inverse(ppe_block):
+    """The coarsest solve of a system with Dirichlet rows (nonsingular): the
+    plain inverse, against LAPACK; a planted transpose fails."""
+    import memphys_amd as M
+    A = ppe_block
+    _, _, Ac = M.amg_first_level(A, k=6)
+    _, _, A2 = M.amg_first_level(Ac, k=6)
+    n = A2.shape[0]
+    D = A2.toarray() + np.diag(np.abs(A2.toarray()).sum(axis=1))  # nonsingular, non-symmetric
+    import scipy.sparse as sp
+    G = M.amg_dense_inverse(sp.csr_matrix(D))
+    G_ref = np.linalg.inv(D)
+    assert np.abs(G - G_ref).max() <= 1e-10 * np.abs(G_ref).max()
+    assert np.abs(G @ D - np.eye(n)).max() <= 1e-10
+    if np.abs(D - D.T).max() > 0:
+        assert np.abs(G.T - G_ref).max() > 1e-6 * np.abs(G_ref).max()
+
+
 def _truncate_restated(P0, trunc, rescale=True):
     """Each row: entries below trunc x its largest |p| dropped, the rest scaled
     to the full row's sum."""

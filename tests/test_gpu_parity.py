@@ -98,7 +98,12 @@ def _ppe_case(ctx, cloud, precision, krylov=0):
     x, xo = ctx.solve(b, np.zeros(n))
     st = ctx.stats()
     _print("PPE", cloud.dim, "D prec", precision, "krylov", krylov, "iters", st.iterations,
-           "cycles", st.outer_cycles, "rel_res", st.rel_res)
+           "cycles", st.outer_cycles, "rel_res", st.rel_res, "gmres switches",
+           st.gmres_switches)
+    if krylov and precision == M.MP_PREC_FP64 and cloud.dim == 2:
+        # GMRES(30) with Jacobi stalls on the 2D cavity's operator; the stalled
+        # system continues with BiCGSTAB
+        assert st.gmres_switches >= 1
     assert st.rel_res <= 1e-10
     assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 1e-10
     _print("x vs SuperLU rel", rel_err(x, x_ref))
@@ -178,6 +183,8 @@ def _fs_parity(ctx, cloud, steps, lid_axis, rho, mu, dt, precision, precond=0, k
            f" restart {restart}: errs {errs}"
            f" iters {st.iterations} total {st.total_iterations} rel_res {st.rel_res}")
     assert max(errs) <= 1e-8
+    if krylov and precond == M.MP_PRECOND_AMG:
+        assert st.gmres_switches == 0  # no cycle stalls with the multigrid preconditioner
     div = ctx.divergence()
     div_o, _ = O.divergence(cloud, gx, gy, gz, ctx.get_field(M.MP_U), ctx.get_field(M.MP_V),
                             ctx.get_field(M.MP_W) if cloud.dim == 3 else None)

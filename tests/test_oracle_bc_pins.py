@@ -1,0 +1,246 @@
+"""Closed-form pins of the oracle's pressure operator, its AB2 predictor and
+its boundary-condition branches (CPU).
+
+The stencil operators reproduce polynomials up to the appended degree (3)
+exactly, so for polynomial fields every quantity below has a closed form:
+  * SOLVER::calc_coeff_matrix (solver.cpp:165-214): interior rows
+    diff*Lap p + conv*(px + py (+ pz)) + unsteady*p, Neumann rows n.grad p,
+    Dirichlet rows p, and in the full-Neumann case the ones column (alpha)
+    and the ones row (sum of p) of the bordered system (solver.cpp:205-213);
+  * calc_vel_hat's AB2 combination uh = u + dt/rho (1.5 S - 0.5 S_old)
+    (fractional_step_1.cpp:120-134) with S_old from the previous step;
+  * calc_pressure's Dirichlet-pressure rows b = p_old
+    (fractional_step_1.cpp:140-143);
+  * calc_vel_corr's Neumann-velocity rows (fractional_step_1.cpp:159-180),
+    which solve n.grad u = 0 for the boundary value.
+Each check also runs on a copy with a planted mistake (a sign flip, a
+dropped term or column, swapped coefficients) and must reject it.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def _cloud(dim):
+    import memphys_amd as M
+    if dim == 2:
+        return M.box_cloud(dim=2, n=14, jitter=0.25, seed=3, k_interior=30)
+    return M.box_cloud(dim=3, n=9, jitter=0.3, seed=3, k_interior=40, k_layers=(60, 55, 50))
+
+
+def _face(cloud, axis, value):
+    return (cloud.boundary == 1) & (np.abs(cloud.xyz[:, axis] - value) < 1e-12)
+
+
+# a cubic pressure and its derivatives
+def _p(x):
+    p = x[:, 0] ** 3 + 0.5 * x[:, 0] * x[:, 1] - 0.3 * x[:, 1] ** 2
+    gp = [3 * x[:, 0] ** 2 + 0.5 * x[:, 1], 0.5 * x[:, 0] - 0.6 * x[:, 1]]
+    lap = 6 * x[:, 0] - 0.6
+    if x.shape[1] == 3:
+        p = p + x[:, 1] * x[:, 2] ** 2 + 0.2 * x[:, 2]
+        gp[1] = gp[1] + x[:, 2] ** 2
+        gp.append(2 * x[:, 1] * x[:, 2] + 0.2)
+        lap = lap + 2 * x[:, 1]
+    return p, gp, lap
+
+
+def _coeff_expected(cloud, dirichlet, unsteady, conv, diff, p, gp, lap, alpha):
+    """A [p; alpha] row by row from the closed forms."""
+    nv, dim = cloud.nv, cloud.dim
+    bnd = cloud.boundary == 1
+    ndot = sum(cloud.normal[:, a] * gp[a] for a in range(dim))
+    interior = diff * lap + conv * sum(gp) + unsteady * p
+    out = np.where(bnd, np.where(dirichlet == 1, p, ndot), interior)
+    if alpha is None:
+        return out
+    return np.concatenate([out + alpha, [p.sum()]])
+
+
+def _coeff_error(A, cloud, dirichlet, unsteady, conv, diff):
+    """Largest |A x - closed form| over the rows, relative to the row's scale
+    (sum |a_ij| |x_j|)."""
+    p, gp, lap = _p(cloud.xyz)
+    bordered = A.shape[0] == cloud.nv + 1
+    alpha = 0.37 if bordered else None
+    x = np.concatenate([p, [alpha]]) if bordered else p
+    exp = _coeff_expected(cloud, dirichlet, unsteady, conv, diff, p, gp, lap, alpha)
+    scale = abs(A) @ np.abs(x) + 1.0
+    return np.max(np.abs(A @ x - exp) / scale)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("case", ["full_neumann", "dirichlet_face"])
+def test_coeff_matrix_closed_form(dim, case):
+    c = _cloud(dim)
+    w = O.stencil_weights(c, 3, 3)[:4]
+    if case == "full_neumann":
+        d = np.zeros(c.nv, dtype=np.uint8)
+        coef = (0.0, 0.0, 1.0)  # the pressure equation of FRACTIONAL_STEP_1
+    else:
+        d = _face(c, 0, 1.0).astype(np.uint8)  # Dirichlet on the face x = 1
+        coef = (0.7, 0.3, 1.3)
+    A = O.coeff_matrix(c, w, d, *coef).tocsr()
+    assert A.shape[0] == (c.nv + 1 if case == "full_neumann" else c.nv)
+    err = _coeff_error(A, c, d, *coef)
+    assert err <= 1e-9, err
+    # planted mistakes, each must be rejected
+    bnd = np.flatnonzero(c.boundary == 1)
+    neu = bnd[d[bnd] == 0]
+    planted = []
+    B = A.tolil(copy=True)
+    for i in neu:
+        B[i, :] = -B[i, :]          # sign of the Neumann rows
+    planted.append(B.tocsr())
+    if case == "full_neumann":
+        B = A.tolil(copy=True)
+        B[: c.nv, c.nv] = 0.0       # the alpha column dropped
+        planted.append(B.tocsr())
+        B = A.tolil(copy=True)
+        B[c.nv, : c.nv // 2] = 0.0  # part of the ones row dropped
+        planted.append(B.tocsr())
+    else:
+        B = A.tolil(copy=True)       # a Dirichlet row left as its Neumann row
+        i = np.flatnonzero(d)[0]
+        ni = O.coeff_matrix(c, w, np.zeros(c.nv, dtype=np.uint8), *coef).tocsr()
+        cols = ni.indices[ni.indptr[i]:ni.indptr[i + 1]]
+        vals = ni.data[ni.indptr[i]:ni.indptr[i + 1]]
+        B[i, :] = 0.0
+        B[i, cols[cols < c.nv]] = vals[cols < c.nv]  # (without the border column)
+        planted.append(B.tocsr())
+        # conv and unsteady swapped in the interior rows
+        planted.append(O.coeff_matrix(c, w, d, coef[1], coef[0], coef[2]).tocsr())
+    for B in planted:
+        assert _coeff_error(B, c, d, *coef) > 1e3 * max(err, 1e-12)
+
+
+def _quadratic_fields(x, k):
+    """Two quadratic velocity sets (k = 0, 1) and their derivatives."""
+    dim = x.shape[1]
+    a = 1.0 + 0.5 * k
+    U = [0.3 * a + x[:, 0] * x[:, 1], a * x[:, 0] ** 2 - 0.5 * x[:, 1]]
+    dU = [[x[:, 1], x[:, 0]], [2 * a * x[:, 0], -0.5 + 0 * x[:, 0]]]
+    lapU = [0 * x[:, 0], 2 * a + 0 * x[:, 0]]
+    if dim == 3:
+        U = [U[0] + k * x[:, 2] ** 2, U[1] + x[:, 1] * x[:, 2], 0.2 * a * x[:, 0] * x[:, 2]]
+        dU = [[x[:, 1], x[:, 0], 2 * k * x[:, 2]], [2 * a * x[:, 0], -0.5 + x[:, 2], x[:, 1]],
+              [0.2 * a * x[:, 2], 0 * x[:, 0], 0.2 * a * x[:, 0]]]
+        lapU = [2 * k + 0 * x[:, 0], 2 * a + 0 * x[:, 0], 0 * x[:, 0]]
+    return U, dU, lapU
+
+
+def _source(U, dU, lapU, rho, mu):
+    dim = len(U)
+    return [rho * -sum(U[a] * dU[q][a] for a in range(dim)) + mu * lapU[q] for q in range(dim)]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_ab2_predictor_closed_form(dim):
+    """Step 0 (Euler) stores S(U0); the predictor of step 1 with fields U1
+    gives uh = U1 + dt/rho (1.5 S(U1) - 0.5 S(U0)) exactly."""
+    c = _cloud(dim)
+    w = O.stencil_weights(c, 3, 3)[:4]
+    nv, x = c.nv, c.xyz
+    rho, mu, dt = 1.3, 0.02, 1e-3
+    ones = np.ones(nv, dtype=np.uint8)
+    p_dir = np.zeros(nv, dtype=np.uint8)
+    fs = O.FractionalStep(c, w, ones, ones, ones if dim == 3 else None, p_dir, rho, mu, dt, 2)
+    U0, dU0, lap0 = _quadratic_fields(x, 0)
+    U1, dU1, lap1 = _quadratic_fields(x, 1)
+    S0 = _source(U0, dU0, lap0, rho, mu)
+    S1 = _source(U1, dU1, lap1, rho, mu)
+    work = [u.copy() for u in U0]
+    p = np.zeros(fs.n_sys)
+    fs.step(0, work[0], work[1], work[2] if dim == 3 else None, p)
+    uh, _ = fs.predict(1, U1[0], U1[1], U1[2] if dim == 3 else None, p)
+    err = max(np.abs(uh[q] - (U1[q] + dt / rho * (1.5 * S1[q] - 0.5 * S0[q]))).max()
+              for q in range(dim))
+    assert err <= 1e-9, err
+    # planted mistakes: Euler, the history sign, the weights swapped
+    for wrong in ((1.0, 0.0), (1.5, -0.5), (0.5, 1.5)):
+        e2 = max(np.abs(uh[q] - (U1[q] + dt / rho * (wrong[0] * S1[q] - wrong[1] * S0[q]))).max()
+                 for q in range(dim))
+        assert e2 > 1e3 * max(err, 1e-12)
+    # first-order scheme: Euler at every step, whatever the history
+    fs1 = O.FractionalStep(c, w, ones, ones, ones if dim == 3 else None, p_dir, rho, mu, dt, 1)
+    work = [u.copy() for u in U0]
+    p = np.zeros(fs1.n_sys)
+    fs1.step(0, work[0], work[1], work[2] if dim == 3 else None, p)
+    uh1, _ = fs1.predict(1, U1[0], U1[1], U1[2] if dim == 3 else None, p)
+    assert max(np.abs(uh1[q] - (U1[q] + dt / rho * S1[q])).max() for q in range(dim)) <= 1e-9
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_rhs_dirichlet_pressure_rows(dim):
+    """Pressure right-hand side with a Dirichlet-pressure face: b = p_old on
+    its rows, -rho((u - uh).n)/dt on the Neumann boundary rows, rho/dt div(uh)
+    inside; no border unknown."""
+    c = _cloud(dim)
+    w = O.stencil_weights(c, 3, 3)[:4]
+    nv, x = c.nv, c.xyz
+    rho, mu, dt = 1.3, 0.02, 1e-3
+    ones = np.ones(nv, dtype=np.uint8)
+    p_dir = _face(c, 0, 1.0).astype(np.uint8)
+    fs = O.FractionalStep(c, w, ones, ones, ones if dim == 3 else None, p_dir, rho, mu, dt, 1)
+    assert fs.n_sys == nv
+    U, _, _ = _quadratic_fields(x, 1)
+    p_old = np.sin(3 * x[:, 0]) + x[:, 1]
+    uh, b = fs.predict(0, U[0], U[1], U[2] if dim == 3 else None, p_old)
+    bnd = c.boundary == 1
+    dirf = p_dir == 1
+    assert np.array_equal(b[dirf], p_old[dirf])
+    neu = bnd & ~dirf
+    s = sum((U[q] - uh[q]) * c.normal[:, q] for q in range(dim))
+    assert np.allclose(b[neu], (-rho * s / dt)[neu], rtol=1e-12, atol=1e-12)
+    div = sum(O.spmv(c, w[q], uh[q]) for q in range(dim))
+    inter = ~bnd
+    assert np.allclose(b[inter], (div * rho / dt)[inter], rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_corrector_neumann_velocity_closed_form(dim):
+    """Corrector with Neumann velocity on the top face (n.grad u = 0 there),
+    Dirichlet pressure on the face x = 1, Dirichlet velocity elsewhere.  With
+    uh = f and p chosen so that every corrected component g_q = f - dt/rho
+    dp/dx_q has zero normal derivative on the top face, the Neumann rows
+    reproduce g_q exactly; Dirichlet-pressure rows keep the corrected value;
+    Dirichlet-velocity rows keep u_old."""
+    c = _cloud(dim)
+    w = O.stencil_weights(c, 3, 3)[:4]
+    nv, x = c.nv, c.xyz
+    top = dim - 1  # y = 1 in 2D, z = 1 in 3D
+    rho, dt = 1.3, 1e-3
+    neu_face = _face(c, top, 1.0)
+    p_dir = _face(c, 0, 1.0).astype(np.uint8)
+    u_dir = np.ones(nv, dtype=np.uint8)
+    u_dir[neu_face] = 0
+    fs = O.FractionalStep(c, w, u_dir, u_dir, u_dir if dim == 3 else None, p_dir, rho, 0.01,
+                          dt, 1)
+    t = x[:, top] - 1.0
+    f = t ** 2 + x[:, 0] + (x[:, 1] if dim == 3 else 0.0)
+    p = x[:, 0] ** 3 + t ** 3 / 3.0 + (x[:, 1] ** 3 if dim == 3 else 0.0)
+    gp = [3 * x[:, 0] ** 2] + ([3 * x[:, 1] ** 2] if dim == 3 else []) + [t ** 2]
+    uh = np.stack([f + 0.1 * q for q in range(dim)])
+    g = [uh[q] - dt / rho * gp[q] for q in range(dim)]
+    u_old = [np.cos(x[:, 0]) + q for q in range(dim)]
+    un = fs.correct(u_old[0], u_old[1], u_old[2] if dim == 3 else None, uh, p)
+    bnd = c.boundary == 1
+    for q in range(dim):
+        exp = g[q].copy()
+        keep = bnd & (p_dir == 0) & (u_dir == 1)
+        exp[keep] = u_old[q][keep]
+        err = np.abs(un[q] - exp)
+        assert err[~bnd].max() <= 1e-12
+        assert err[neu_face].max() <= 1e-9, err[neu_face].max()
+        assert err[p_dir == 1].max() <= 1e-12
+        assert np.array_equal(un[q][keep], u_old[q][keep])
+    # planted: the Neumann solve with the off-diagonal sign flipped
+    rp, ci = c.row_ptr, c.col
+    gx, gy, gz = w[0], w[1], w[2]
+    i = np.flatnonzero(neu_face)[0]
+    cf = sum(c.normal[i, a] * (gx, gy, gz)[a][rp[i]:rp[i + 1]] for a in range(dim))
+    cols = ci[rp[i]:rp[i + 1]]
+    off = cols != i
+    wrong = (np.sum(cf[off] * un[0][cols[off]])) / cf[~off][0]
+    assert abs(wrong - g[0][i]) > 1e3 * max(abs(un[0][i] - g[0][i]), 1e-12)
