@@ -3,8 +3,11 @@
 // POINTS -> CLOUD -> boundary conditions -> FRACTIONAL_STEP_1 -> time loop
 // with old <- new.
 //
-// usage: lid_driven_cavity <parameters.csv> <dim> <n> <steps> <dt> <Re> <out.bin>
+// usage: lid_driven_cavity <parameters.csv> <dim> <n> <steps> <dt> <Re> <out.bin> [g]
 // out.bin: int32 nv, dim, system_size; then xyz (nv*dim), u, v, (w), p (float64).
+// With g, every step runs the body-force overload with f_x = 0.5 g sin(pi y),
+// f_y = -g (and f_z = 0 in 3D).  GPU options (precision, preconditioner,
+// guess) come from the settings file PARAMETERS finds (MEMPHYS_GPU_SETTINGS).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -15,9 +18,11 @@
 
 int main(int argc, char** argv) {
     if (argc < 8) {
-        fprintf(stderr, "usage: %s parameters.csv dim n steps dt Re out.bin\n", argv[0]);
+        fprintf(stderr, "usage: %s parameters.csv dim n steps dt Re out.bin [g]\n", argv[0]);
         return 2;
     }
+    const bool gravity = argc > 8;
+    const double g = gravity ? atof(argv[8]) : 0.0;
     const int dim = atoi(argv[2]), n = atoi(argv[3]), steps = atoi(argv[4]);
     const double dt = atof(argv[5]), Re = atof(argv[6]);
     try {
@@ -42,10 +47,21 @@ int main(int argc, char** argv) {
                        : new FRACTIONAL_STEP_1(points, cloud, parameters, u_dir, v_dir, w_dir, p_dir, 2);
         p_new = VectorXd::Zero(fs->system_size);
         p_old = p_new;
+        VectorXd fx = VectorXd::Zero(nv), fy = VectorXd::Zero(nv), fz = VectorXd::Zero(nv);
+        for (int iv = 0; iv < nv; ++iv) {
+            fx[iv] = 0.5 * g * std::sin(std::acos(-1.0) * points.xyz[dim * iv + 1]);
+            fy[iv] = -g;
+        }
         for (int it = 0; it < steps; ++it) {
-            if (dim == 2)
+            if (dim == 2 && gravity)
+                fs->single_timestep_2d(points, cloud, parameters, u_new, v_new, p_new, u_old, v_old,
+                                       p_old, fx, fy, it);
+            else if (dim == 2)
                 fs->single_timestep_2d(points, cloud, parameters, u_new, v_new, p_new, u_old, v_old,
                                        p_old, it);
+            else if (gravity)
+                fs->single_timestep_3d(points, cloud, parameters, u_new, v_new, w_new, p_new, u_old,
+                                       v_old, w_old, p_old, fx, fy, fz, it);
             else
                 fs->single_timestep_3d(points, cloud, parameters, u_new, v_new, w_new, p_new, u_old,
                                        v_old, w_old, p_old, it);
@@ -58,9 +74,10 @@ int main(int argc, char** argv) {
         double res[6];
         calc_continuity_residual(points, cloud, parameters, u_new, v_new,
                                  dim == 3 ? &w_new : nullptr, res);
-        printf("nv %d steps %d solver %s last PPE iterations %d rel_res %.3e continuity max %.3e "
+        printf("nv %d steps %d solver %s precision %d preconditioner %d guess %d last PPE iterations %d rel_res %.3e continuity max %.3e "
                "mean %.3e cond max %.3e dx avg %.5f\n",
-               nv, steps, parameters.solver_type.c_str(),
+               nv, steps, parameters.solver_type.c_str(), parameters.precision,
+               parameters.preconditioner, parameters.guess,
                parameters.n_iter_actual.empty() ? 0 : parameters.n_iter_actual.back(),
                parameters.rel_res_log.empty() ? 0.0 : parameters.rel_res_log.back(), res[4], res[5],
                cloud.cond_num_RBF_max, parameters.avg_dx);

@@ -189,6 +189,12 @@ int mp_fs_create(mp_ctx* ctx, const mp_fs_params* params, const uint8_t* u_dir,
                  const uint8_t* v_dir, const uint8_t* w_dir, const uint8_t* p_dir);
 int mp_fields_set(mp_ctx* ctx, int which, const double* values);
 int mp_fields_get(mp_ctx* ctx, int which, double* values);
+/* Body force per unit volume (nv entries each, caller order) added to the
+ * explicit source of every following step: the overload of
+ * FRACTIONAL_STEP_1::single_timestep_2d with body_force_x/y (class.hpp:253,
+ * fractional_step_1.cpp:65-76 and 104-118), generalised to 3D (fz).  All
+ * NULL: no body force (the plain overload). */
+int mp_fs_body_force(mp_ctx* ctx, const double* fx, const double* fy, const double* fz);
 /* n_steps of single_timestep followed by old <- new; it0 is the first step index
  * (Euler at it == 0).  Fields stay on the device between calls. */
 int mp_fs_step(mp_ctx* ctx, int n_steps, int it0);

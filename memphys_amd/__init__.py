@@ -37,7 +37,7 @@ HIP_SYMBOLS = (
     "mp_kernel_timing", "mp_kernel_timing_get", "mp_guess_param_count", "mp_guess_train",
     "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
     "mp_graph_mode", "mp_batch_upload", "mp_batch_params", "mp_batch_stats", "mp_autotune",
-    "mp_operator_eigenvalue", "mp_knn_stencils", "mp_rcm_order",
+    "mp_operator_eigenvalue", "mp_knn_stencils", "mp_rcm_order", "mp_fs_body_force",
 )
 
 
@@ -98,6 +98,7 @@ def hip_lib() -> C.CDLL:
         lib.mp_fs_create.argtypes = [vp, C.POINTER(FsParams), _u8, _u8, vp, _u8]
         lib.mp_fields_set.argtypes = [vp, C.c_int, _dp]
         lib.mp_fields_get.argtypes = [vp, C.c_int, _dp]
+        lib.mp_fs_body_force.argtypes = [vp, vp, vp, vp]
         lib.mp_fs_step.argtypes = [vp, C.c_int, C.c_int]
         lib.mp_stats_get.argtypes = [vp, C.POINTER(StepStats)]
         lib.mp_guess_mode.argtypes = [vp, C.c_int]
@@ -428,6 +429,14 @@ class Context:
         out = np.zeros(n)
         self._check(self.lib.mp_fields_get(self.h, which, out), "mp_fields_get")
         return out
+
+    def body_force(self, fx=None, fy=None, fz=None):
+        """Body force of the following steps (nv entries each); no arguments
+        removes it."""
+        self._force = [np.ascontiguousarray(f, dtype=np.float64) if f is not None else None
+                       for f in (fx, fy, fz)]
+        ptrs = [f.ctypes.data_as(C.c_void_p) if f is not None else None for f in self._force]
+        self._check(self.lib.mp_fs_body_force(self.h, *ptrs), "mp_fs_body_force")
 
     def step(self, n_steps: int, it0: int):
         self._check(self.lib.mp_fs_step(self.h, n_steps, it0), "mp_fs_step")

@@ -81,6 +81,8 @@ struct mp_ctx {
     DevBuf<double> div_partial;
     // pressure history and the learned guess
     DevBuf<double> P1, P2, Ptmp;
+    DevBuf<double> force;   // body force [3][npad] (mp_fs_body_force)
+    bool has_force = false;
     int hist = 0;
     std::unique_ptr<GuessNet> net;
     bool train = false;
@@ -610,10 +612,22 @@ int mp_weights_assemble(mp_ctx* c, int phs_deg, int poly_deg, const int* exps, i
         if (cond_out) dcond.alloc(c->npad);
         c->timers[4].begin(c->stream);
         c->weights_fp64_rows = c->nv;
-        if (precision_mode == MP_PREC_FP64 || cond_out) {
+        DevBuf<int> wrows;
+        auto row_len = [&](int d) { return c->h_rowptr[c->perm[d] + 1] - c->h_rowptr[c->perm[d]]; };
+        if (cond_out) {
+            // the condition estimate needs the LU factors: the LDS kernel keeps them
             launch_weights_fp64(c->stream, c->dim, phs_deg, m, de.p, c->sv, c->xyz.p, c->npad,
-                                c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p,
-                                cond_out ? dcond.p : nullptr, c->max_len);
+                                c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p, dcond.p, c->max_len);
+        } else if (precision_mode == MP_PREC_FP64) {
+            std::vector<int> rows, lens;
+            for (int d = 0; d < c->npad; ++d) {
+                if (c->perm[d] < 0) continue;
+                rows.push_back(d);
+                lens.push_back(row_len(d));
+            }
+            launch_weights_fp64_rows(c->stream, c->dim, phs_deg, m, de.p, c->sv, c->xyz.p, c->npad,
+                                     c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p, rows, lens,
+                                     wrows);
         } else {
             // FP32 LU + FP64 refinement for the shortest (interior) stencils; the
             // longer boundary / near-boundary stencils and any stencil whose
@@ -627,9 +641,8 @@ int mp_weights_assemble(mp_ctx* c, int phs_deg, int poly_deg, const int* exps, i
                 const int len = c->h_rowptr[c->perm[d] + 1] - c->h_rowptr[c->perm[d]];
                 (len == kmin ? mixed_rows : fp64_rows).push_back(d);
             }
-            DevBuf<int> dm, df, dflag;
+            DevBuf<int> dm, dflag;
             dm.alloc(mixed_rows.size());
-            df.alloc(std::max<size_t>(1, fp64_rows.size() + mixed_rows.size()));
             dflag.alloc(c->npad);
             MP_HIP(hipMemsetAsync(dflag.p, 0, sizeof(int) * c->npad, c->stream));
             if (!mixed_rows.empty())
@@ -646,11 +659,11 @@ int mp_weights_assemble(mp_ctx* c, int phs_deg, int poly_deg, const int* exps, i
                 if (hflag[d]) fp64_rows.push_back(d);
             c->weights_fp64_rows = (int)fp64_rows.size();
             if (!fp64_rows.empty()) {
-                MP_HIP(hipMemcpy(df.p, fp64_rows.data(), sizeof(int) * fp64_rows.size(),
-                                 hipMemcpyHostToDevice));
-                launch_weights_fp64(c->stream, c->dim, phs_deg, m, de.p, c->sv, c->xyz.p,
-                                    c->npad, c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p, nullptr,
-                                    c->max_len, df.p, (int)fp64_rows.size());
+                std::vector<int> lens;
+                for (int d : fp64_rows) lens.push_back(row_len(d));
+                launch_weights_fp64_rows(c->stream, c->dim, phs_deg, m, de.p, c->sv, c->xyz.p,
+                                         c->npad, c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p,
+                                         fp64_rows, lens, wrows);
             }
         }
         c->timers[4].end(c->stream);
@@ -866,6 +879,7 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
         c->hist = 0;
         if (c->net && c->nprob == 1) ensure_net(c);
         for (int q = 0; q < 3; ++q) alloc_vec(c, c->uh[q]);
+        c->has_force = false;
         for (int k = 0; k < 2; ++k) {
             c->S[k].alloc((size_t)3 * c->npad + 64);
             MP_HIP(hipMemsetAsync(c->S[k].p, 0, sizeof(double) * c->S[k].n, c->stream));
@@ -905,6 +919,28 @@ int mp_fields_set(mp_ctx* c, int which, const double* values) {
             to_device_order(c, values, h);
             upload(c, c->U[c->cur][which].p, h);
         }
+        return MP_OK;
+    });
+}
+
+int mp_fs_body_force(mp_ctx* c, const double* fx, const double* fy, const double* fz) {
+    return guarded(c, "mp_fs_body_force", [&]() -> int {
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_fs_body_force: mp_fs_create not called");
+        if (!fx && !fy && !fz) {
+            c->has_force = false;
+            return MP_OK;
+        }
+        if (!fx || !fy || (c->dim == 3) != (fz != nullptr))
+            return fail(c, MP_ERR_ARG, "mp_fs_body_force: give fx, fy (and fz in 3D), or none");
+        if (c->force.n < (size_t)3 * c->npad + 64) c->force.alloc((size_t)3 * c->npad + 64);
+        const double* f[3] = {fx, fy, fz};
+        for (int q = 0; q < c->dim; ++q) {
+            std::vector<double> h;
+            to_device_order(c, f[q], h);
+            h.resize(c->npad);
+            upload(c, c->force.p + (size_t)q * c->npad, h);
+        }
+        c->has_force = true;
         return MP_OK;
     });
 }
@@ -949,7 +985,8 @@ void fs_step_one(mp_ctx* c, int it) {
     const double* pp = c->per_problem_prm ? c->prob_prm.p : nullptr;
     launch_predictor<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p, Uo[0].p,
                           Uo[1].p, Uo[2].p, c->uh[0].p, c->uh[1].p, c->uh[2].p, S_new, S_oo,
-                          c->npad, q.rho, q.mu, q.dt, euler, bp, pp);
+                          c->npad, q.rho, q.mu, q.dt, euler, bp, pp,
+                          c->has_force ? c->force.p : nullptr);
     c->timers[1].end(c->stream);
     // calc_pressure
     c->timers[2].begin(c->stream);
@@ -969,7 +1006,16 @@ void fs_step_one(mp_ctx* c, int it) {
             c->net->features(c->stream, c->P1.p, c->P2.p, r, c->ppe->diag_inv, c->xyz.p, c->npad,
                              c->dim);
             c->net->forward(c->stream, c->train != 0);
+            // the extrapolation stays the second candidate: the solve starts from
+            // whichever of the two leaves the smaller FP64 residual
+            hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad,
+                               c->P.p, c->Ptmp.p);
+            MP_HIP(hipGetLastError());
             c->net->guess(c->stream, c->P1.p, c->P2.p, c->P.p);
+            if (!c->train) {
+                c->ppe->fallback_rr = c->ppe->last_residual_norm2(c->stream);
+                c->ppe->fallback_x = c->Ptmp.p;
+            }
             learned = true;
         }
     }

@@ -37,7 +37,8 @@ __global__ __launch_bounds__(BLOCK) void predictor_kernel(
     const double* __restrict__ v, const double* __restrict__ w, double* __restrict__ uh,
     double* __restrict__ vh, double* __restrict__ wh, double* __restrict__ S_new,
     const double* __restrict__ S_oo, int npad, double rho_all, double mu_all, double dt_all,
-    int euler, const int* __restrict__ bprob, const double* __restrict__ prm) {
+    int euler, const int* __restrict__ bprob, const double* __restrict__ prm,
+    const double* __restrict__ force) {
     const int r = blockIdx.x * BLOCK + threadIdx.x;
     if (r >= sv.nv) return;
     double rho = rho_all, mu = mu_all, dt = dt_all;
@@ -85,7 +86,10 @@ __global__ __launch_bounds__(BLOCK) void predictor_kernel(
     for (int q = 0; q < DIM; ++q) {
         double conv = -(own[0] * acc[q][0]) - own[1] * acc[q][1];
         if (DIM == 3) conv = conv - own[DIM - 1] * acc[q][2];
-        const double S = rho * conv + mu * acc[q][3];
+        double S = rho * conv + mu * acc[q][3];
+        // body force (fractional_step_1.cpp:104-118): added after the
+        // convective and diffusive terms, so it enters the AB2 history too
+        if (force) S = S + force[(size_t)q * npad + r];
         S_new[(size_t)q * npad + r] = S;
         double res;
         if (euler)
@@ -370,10 +374,10 @@ void launch_predictor(hipStream_t s, const SellView& sv, const double* gx, const
                       const double* gz, const double* lap, const double* u, const double* v,
                       const double* w, double* uh, double* vh, double* wh, double* S_new,
                       const double* S_oo, int npad, double rho, double mu, double dt, int euler,
-                      const int* bprob, const double* prm) {
+                      const int* bprob, const double* prm, const double* force) {
     const int g = (sv.nv + BLOCK - 1) / BLOCK;
     hipLaunchKernelGGL(predictor_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, lap, u,
-                       v, w, uh, vh, wh, S_new, S_oo, npad, rho, mu, dt, euler, bprob, prm);
+                       v, w, uh, vh, wh, S_new, S_oo, npad, rho, mu, dt, euler, bprob, prm, force);
     MP_HIP(hipGetLastError());
 }
 
@@ -448,7 +452,7 @@ void launch_build_operator(hipStream_t s, const SellView& sv, int dim, const dou
                                       const double*, const double*, const double*, const double*, \
                                       const double*, double*, double*, double*, double*,        \
                                       const double*, int, double, double, double, int,          \
-                                      const int*, const double*);                               \
+                                      const int*, const double*, const double*);                \
     template void launch_ppe_rhs<D>(hipStream_t, const SellView&, const double*, const double*,   \
                                     const double*, const double*, const double*, const double*,   \
                                     const double*, const double*, const double*, const double*,   \

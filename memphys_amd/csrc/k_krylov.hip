@@ -1374,6 +1374,12 @@ void PpeSolver::residual64(hipStream_t st, const double* b_dev, std::vector<doub
     }
 }
 
+double PpeSolver::last_residual_norm2(hipStream_t st) {
+    MP_HIP(hipMemcpyAsync(host_sc, sc.p, sizeof(KScalars), hipMemcpyDeviceToHost, st));
+    MP_HIP(hipStreamSynchronize(st));
+    return host_sc[0].red[0];
+}
+
 double* PpeSolver::residual_of(hipStream_t st, const double* b_dev, const double* x_dev) {
     const int nbk = nrows / BLOCK;
     hipLaunchKernelGGL(k_scale, dim3(nbk), dim3(BLOCK), 0, st, nrows, dinv.p, x_dev, xt.p, 1);
@@ -1544,6 +1550,17 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
     for (int q = 0; q < nseg; ++q)
         out[q].guess_rel_res = !(std::isfinite(rr0[q]) && std::isfinite(bb[q])) ? std::nan("")
                                : (bb[q] > 0) ? std::sqrt(rr0[q] / bb[q]) : 0.0;
+    if (fallback_x && nseg == 1 && std::isfinite(fallback_rr) &&
+        !(rr0[0] <= FALLBACK_FACTOR * FALLBACK_FACTOR * fallback_rr)) {
+        // the guess's residual is not finite or far above the second
+        // candidate's (the learned guess's linear extrapolation): start from it
+        hipLaunchKernelGGL(k_scale, dim3(nbk), dim3(BLOCK), 0, st, nrows, dinv.p, fallback_x,
+                           xt.p, 1);
+        MP_HIP(hipGetLastError());
+        residual64(st, b_dev, rr0, bb);
+        out[0].guess_rejected = 1;
+    }
+    fallback_x = nullptr;
     {
         // a system with a finite right-hand side whose guess is not finite (or
         // leaves a non-finite residual) starts from zero instead

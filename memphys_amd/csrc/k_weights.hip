@@ -11,6 +11,9 @@
 // Follows coefficient_computations.cpp:33-328 (shifting_scaling, A and rhs
 // assembly, partialPivLu().solve) and cloud.cpp:207-279 (loop over nodes,
 // row 0 of the solution is the stencil's weight row).
+#include <algorithm>
+#include <climits>
+
 #include "mp_internal.hpp"
 
 namespace mp {
@@ -32,6 +35,90 @@ __device__ __forceinline__ double ipow_d(double x, int e) {
     double v = 1.0;
     for (int s = 0; s < e; ++s) v = v * x;
     return v;
+}
+
+// Right-hand-side entries of saddle-point row i (columns gx, gy, (gz), lap at
+// out[0..dim]) with the oracle's formulas (coefficient_computations.cpp:108-292):
+// a stencil row (is_pt) differentiates the PHS kernel between the scaled point
+// xi and the scaled centre xc; a polynomial row differentiates the monomial
+// with exponents e at xc.  sc holds the per-axis scale factors.
+__device__ __forceinline__ void rhs_entries(bool is_pt, int dim, int phs, const double* xi,
+                                            const double* xc, const double* sc, const int* e,
+                                            double* out) {
+    const double p = (double)phs;
+    if (is_pt) {
+        double r2 = 0.0;
+        for (int a = 0; a < dim; ++a) {
+            double d = xi[a] - xc[a];
+            r2 = r2 + d * d;
+        }
+        for (int a = 0; a < dim; ++a) {
+            double v = (-p) * (xi[a] - xc[a]);
+            v = v * rpow_d(r2, phs - 2);
+            out[a] = v / sc[a];
+        }
+        double lp = 0.0;
+        if (fabs(r2) > 1e-14) {
+            double g[3] = {0.0, 0.0, 0.0};
+            for (int a = 0; a < dim; ++a) {
+                double d = xi[a] - xc[a];
+                double d2 = d * d;
+                double q4 = rpow_d(r2, phs - 4);
+                double v = (p * p) * d2 * q4;
+                v = v - (2.0 * p) * d2 * q4;
+                v = v + p * rpow_d(r2, phs - 2);
+                g[a] = v / (sc[a] * sc[a]);
+            }
+            lp = g[0] + g[1];
+            if (dim == 3) lp = lp + g[2];
+        }
+        out[dim] = lp;
+        return;
+    }
+    const double x0 = xc[0], x1 = xc[1], x2 = (dim == 3) ? xc[2] : 0.0;
+    for (int a = 0; a < dim; ++a) {
+        double v = 0.0;
+        if (e[a] > 0) {
+            if (a == 1) {
+                v = ipow_d(x0, e[0]);
+                v = v * (double)e[1];
+                v = v * ipow_d(x1, e[1] - 1);
+                if (dim == 3) v = v * ipow_d(x2, e[2]);
+            } else if (a == 0) {
+                v = (double)e[0] * ipow_d(x0, e[0] - 1);
+                v = v * ipow_d(x1, e[1]);
+                if (dim == 3) v = v * ipow_d(x2, e[2]);
+            } else {
+                v = (double)e[2] * ipow_d(x2, e[2] - 1);
+                v = v * ipow_d(x0, e[0]);
+                v = v * ipow_d(x1, e[1]);
+            }
+            v = v / sc[a];
+        }
+        out[a] = v;
+    }
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    if (e[0] > 1) {
+        double v = (double)e[0] * ((double)e[0] - 1.0) * ipow_d(x0, e[0] - 2);
+        v = v * ipow_d(x1, e[1]);
+        if (dim == 3) v = v * ipow_d(x2, e[2]);
+        g0 = v / (sc[0] * sc[0]);
+    }
+    if (e[1] > 1) {
+        double v = (double)e[1] * ((double)e[1] - 1.0) * ipow_d(x1, e[1] - 2);
+        v = v * ipow_d(x0, e[0]);
+        if (dim == 3) v = v * ipow_d(x2, e[2]);
+        g1 = v / (sc[1] * sc[1]);
+    }
+    if (dim == 3 && e[2] > 1) {
+        double v = (double)e[2] * ((double)e[2] - 1.0) * ipow_d(x2, e[2] - 2);
+        v = v * ipow_d(x0, e[0]);
+        v = v * ipow_d(x1, e[1]);
+        g2 = v / (sc[2] * sc[2]);
+    }
+    double lp = g0 + g1;
+    if (dim == 3) lp = lp + g2;
+    out[dim] = lp;
 }
 
 __global__ __launch_bounds__(256) void weights_fp64_kernel(
@@ -102,84 +189,9 @@ __global__ __launch_bounds__(256) void weights_fp64_kernel(
         A[i * LDA + j] = v;
     }
     // ---- right-hand sides: gx, gy, (gz), lap ----
-    const double p = (double)phs;
-    for (int i = tid; i < M; i += blockDim.x) {
-        if (i < k) {
-            double r2 = 0.0;
-            for (int a = 0; a < dim; ++a) {
-                double d = xs[i * 3 + a] - xs[a];
-                r2 = r2 + d * d;
-            }
-            for (int a = 0; a < dim; ++a) {
-                double v = (-p) * (xs[i * 3 + a] - xs[a]);
-                v = v * rpow_d(r2, phs - 2);
-                R[i * 4 + a] = v / s_sc[a];
-            }
-            double lp = 0.0;
-            if (fabs(r2) > 1e-14) {
-                double g[3] = {0.0, 0.0, 0.0};
-                for (int a = 0; a < dim; ++a) {
-                    double d = xs[i * 3 + a] - xs[a];
-                    double d2 = d * d;
-                    double q4 = rpow_d(r2, phs - 4);
-                    double v = (p * p) * d2 * q4;
-                    v = v - (2.0 * p) * d2 * q4;
-                    v = v + p * rpow_d(r2, phs - 2);
-                    g[a] = v / (s_sc[a] * s_sc[a]);
-                }
-                lp = g[0] + g[1];
-                if (dim == 3) lp = lp + g[2];
-            }
-            R[i * 4 + dim] = lp;
-        } else {
-            const int t = i - k;
-            const int* e = s_e + t * dim;
-            const double x0 = xs[0], x1 = xs[1], x2 = (dim == 3) ? xs[2] : 0.0;
-            for (int a = 0; a < dim; ++a) {
-                double v = 0.0;
-                if (e[a] > 0) {
-                    if (a == 1) {
-                        v = ipow_d(x0, e[0]);
-                        v = v * (double)e[1];
-                        v = v * ipow_d(x1, e[1] - 1);
-                        if (dim == 3) v = v * ipow_d(x2, e[2]);
-                    } else if (a == 0) {
-                        v = (double)e[0] * ipow_d(x0, e[0] - 1);
-                        v = v * ipow_d(x1, e[1]);
-                        if (dim == 3) v = v * ipow_d(x2, e[2]);
-                    } else {
-                        v = (double)e[2] * ipow_d(x2, e[2] - 1);
-                        v = v * ipow_d(x0, e[0]);
-                        v = v * ipow_d(x1, e[1]);
-                    }
-                    v = v / s_sc[a];
-                }
-                R[i * 4 + a] = v;
-            }
-            double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-            if (e[0] > 1) {
-                double v = (double)e[0] * ((double)e[0] - 1.0) * ipow_d(x0, e[0] - 2);
-                v = v * ipow_d(x1, e[1]);
-                if (dim == 3) v = v * ipow_d(x2, e[2]);
-                g0 = v / (s_sc[0] * s_sc[0]);
-            }
-            if (e[1] > 1) {
-                double v = (double)e[1] * ((double)e[1] - 1.0) * ipow_d(x1, e[1] - 2);
-                v = v * ipow_d(x0, e[0]);
-                if (dim == 3) v = v * ipow_d(x2, e[2]);
-                g1 = v / (s_sc[1] * s_sc[1]);
-            }
-            if (dim == 3 && e[2] > 1) {
-                double v = (double)e[2] * ((double)e[2] - 1.0) * ipow_d(x2, e[2] - 2);
-                v = v * ipow_d(x0, e[0]);
-                v = v * ipow_d(x1, e[1]);
-                g2 = v / (s_sc[2] * s_sc[2]);
-            }
-            double lp = g0 + g1;
-            if (dim == 3) lp = lp + g2;
-            R[i * 4 + dim] = lp;
-        }
-    }
+    for (int i = tid; i < M; i += blockDim.x)
+        rhs_entries(i < k, dim, phs, xs + 3 * (i < k ? i : 0), xs, s_sc,
+                    s_e + (i < k ? 0 : i - k) * dim, R + 4 * i);
     __syncthreads();
 
     // ---- 1-norm of A (for the condition estimate) ----
@@ -370,6 +382,384 @@ __global__ __launch_bounds__(256) void weights_fp64_kernel(
 }
 
 
+// ---------------------------------------------------------------------------
+// FP64 weights with the saddle-point matrix in registers: one wavefront per
+// stencil (two for 64 < M <= 96), thread i holding row i.  At pivot step c a
+// row keeps its columns c..M-1 in a[0..M-1-c]: the trailing update writes a[j]
+// from a[j+1], so the row shifts by one column per step and every register
+// index is a compile-time constant.  The update width is rounded up to a
+// multiple of 8 (the entries past the row's end are never read back).  The
+// pivot row is published through LDS, where it stays as row c of U (packed,
+// row c at c*M - c(c-1)/2) for the back substitution, and its forward-
+// eliminated right-hand sides stay in Y[c].  Every entry sees the oracle's
+// roundings in the oracle's order: the same division for each multiplier,
+// fma(-l, u, a) for each update, the pivot chosen as the first maximum in
+// position order (a NaN on the diagonal position keeps it), and the column-
+// oriented back substitution.  The weights are therefore bit-identical to the
+// oracle's and to the LDS kernel's.
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// d = a * b + c, one rounding, as the three-address v_fma_f64: the result may
+// go to any register.  (The compiler's own fma becomes the two-address
+// v_fmac_f64, which writes over c; for the shifted rows below that register is
+// the neighbour's, and every step would end by copying the row back.)
+__device__ __forceinline__ double fma3(double a, double b, double c) {
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
+// Maximum of a double over the wavefront (no NaN among the inputs), through
+// DPP row shifts and broadcasts: lane 63 ends with the maximum, which every
+// lane receives.  Max is exact and order-free, so any combination order gives
+// the same value.
+__device__ __forceinline__ double wave_max_d(double v) {
+    int hi = __double2hiint(v), lo = __double2loint(v);
+#define MP_DPP_MAX(ctrl, rows)                                                    \
+    {                                                                             \
+        const int th = __builtin_amdgcn_update_dpp(hi, hi, ctrl, rows, 0xf, false); \
+        const int tl = __builtin_amdgcn_update_dpp(lo, lo, ctrl, rows, 0xf, false); \
+        v = fmax(v, __hiloint2double(th, tl));                                    \
+        hi = __double2hiint(v);                                                   \
+        lo = __double2loint(v);                                                   \
+    }
+    MP_DPP_MAX(0x111, 0xf)  // row_shr:1
+    MP_DPP_MAX(0x112, 0xf)  // row_shr:2
+    MP_DPP_MAX(0x114, 0xf)  // row_shr:4
+    MP_DPP_MAX(0x118, 0xf)  // row_shr:8: lane 15 of each row holds the row's maximum
+    MP_DPP_MAX(0x142, 0xa)  // row_bcast:15 into rows 1 and 3
+    MP_DPP_MAX(0x143, 0xc)  // row_bcast:31 into rows 2 and 3: lane 63 holds the maximum
+#undef MP_DPP_MAX
+    return __hiloint2double(__builtin_amdgcn_readlane(hi, 63), __builtin_amdgcn_readlane(lo, 63));
+}
+
+// Minimum and maximum over the lanes that hold a point, combined in lane order
+// so that a later value replaces the current one only when strictly smaller
+// (larger): the result equals the left-to-right scan of the oracle.
+__device__ __forceinline__ void ordered_minmax(double& mn, double& mx, int& has, int lane) {
+    for (int off = 1; off < 64; off <<= 1) {
+        const double omn = __shfl_xor(mn, off), omx = __shfl_xor(mx, off);
+        const int ohas = __shfl_xor(has, off);
+        const bool other_first = (lane & off) != 0;
+        const double lmn = other_first ? omn : mn, hmn = other_first ? mn : omn;
+        const double lmx = other_first ? omx : mx, hmx = other_first ? mx : omx;
+        const int lh = other_first ? ohas : has, hh = other_first ? has : ohas;
+        mn = !lh ? hmn : (!hh ? lmn : ((hmn < lmn) ? hmn : lmn));
+        mx = !lh ? hmx : (!hh ? lmx : ((hmx > lmx) ? hmx : lmx));
+        has = lh | hh;
+    }
+}
+
+// LDS doubles the register-resident kernel needs for a stencil of M = k + m.
+__host__ __device__ inline int wave_lds_doubles(int M, int k, int m) {
+    const int lu = M * (M + 1) / 2 + 8 + 4 * M;  // packed U (+ overflow) and Y
+    const int asm_ = 4 * k + m * k;             // scaled coordinates and monomials
+    return lu > asm_ ? lu : asm_;
+}
+
+// The pivot steps of one wavefront's rows.  phase<W> runs the steps whose
+// trailing width M-1-c lies in (W-8, W] with a fixed update width W, then hands
+// over to phase<W-8>: each loop body has one static width, so the shifted row
+// stays in the same registers from step to step.
+template <int NW, int AW>
+struct WaveLu {
+    double (&a)[AW + 1];
+    double (&r)[4];
+    double* U;
+    double* Y;
+    const double* zero;  // AW + 1 zeros
+    double* s_cv;
+    int* s_cp;
+    int* s_cn;
+    int M, pos, lane, wv;
+    bool valid;
+    int ub = 0;  // start of U row c
+
+    // One wavefront (M <= 64): no LDS inside the factorisation.  The pivot is
+    // found with a DPP maximum and a ballot, its row is broadcast with
+    // readlane, and only the rows below the pivot are updated (exec mask), so
+    // a pivot row keeps its U row in registers until the end (store_u).
+    template <int W>
+    __device__ __forceinline__ void step1(int c) {
+        const double av = fabs(a[0]);
+        const bool act = valid && pos >= c;
+        const bool isn = !(av == av);
+        const double cv = (act && !isn) ? av : -1.0;
+        const double mxv = wave_max_d(cv);
+        const unsigned long long tie = __ballot(act && cv == mxv && mxv >= 0.0);
+        int piv;
+        if (__ballot(valid && pos == c && isn) != 0ull || tie == 0ull) {
+            piv = c;  // a NaN on the diagonal position keeps it (nothing compares greater)
+        } else if (__popcll(tie) == 1) {
+            piv = __builtin_amdgcn_readlane(pos, __ffsll((long long)tie) - 1);
+        } else {  // exact ties: the first in position order
+            int cp = (act && cv == mxv) ? pos : INT_MAX;
+            for (int off = 32; off > 0; off >>= 1) cp = min(cp, __shfl_xor(cp, off));
+            piv = __builtin_amdgcn_readfirstlane(cp);
+        }
+        if (pos == c) pos = piv;
+        else if (pos == piv) pos = c;
+        const int pl = __ffsll((long long)__ballot(pos == c)) - 1;  // the pivot row's lane
+        const double d = readlane_d(a[0], pl);
+        if (pos > c) {
+            if (d != 0.0) {
+                const double nl = -(a[0] / d);
+#pragma unroll
+                for (int j = 0; j < W; ++j) a[j] = fma3(nl, readlane_d(a[j + 1], pl), a[j + 1]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) r[q] = fma3(nl, readlane_d(r[q], pl), r[q]);
+            } else {
+                // a zero pivot: the oracle skips the column, the rows only shift
+#pragma unroll
+                for (int j = 0; j < W; ++j) a[j] = a[j + 1];
+            }
+        }
+    }
+
+    // After the last step every row holds its U row (columns pos..M-1 in
+    // a[0..M-1-pos]) and its forward-eliminated right-hand sides.
+    __device__ __forceinline__ void store_u() {
+        if (!valid) return;
+        const int b = pos * M - pos * (pos - 1) / 2;
+#pragma unroll
+        for (int j = 0; j < AW; ++j)
+            if (j < M - pos) U[b + j] = a[j];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Y[4 * pos + q] = r[q];
+    }
+
+    template <int W>
+    __device__ __forceinline__ void step(int c) {
+        if constexpr (NW == 1) {
+            step1<W>(c);
+            return;
+        }
+        // pivot: the first maximum of |column c| in position order
+        const double av = fabs(a[0]);
+        const bool act = valid && pos >= c;
+        const bool isn = !(av == av);
+        const double cv = (act && !isn) ? av : -1.0;
+        double mxv = cv;
+        for (int off = 32; off > 0; off >>= 1) mxv = fmax(mxv, __shfl_xor(mxv, off));
+        int cp = (cv == mxv && mxv >= 0.0) ? pos : INT_MAX;
+        const unsigned long long tie = __ballot(cp != INT_MAX);
+        if (__popcll(tie) == 1) {
+            cp = __builtin_amdgcn_readlane(cp, __ffsll((long long)tie) - 1);
+        } else {
+            for (int off = 32; off > 0; off >>= 1) cp = min(cp, __shfl_xor(cp, off));
+        }
+        int nanc = __any(valid && pos == c && isn) ? 1 : 0;
+        if (NW == 2) {
+            if (lane == 0) {
+                s_cv[wv] = mxv;
+                s_cp[wv] = cp;
+                s_cn[wv] = nanc;
+            }
+            __syncthreads();
+            const double v0 = s_cv[0], v1 = s_cv[1];
+            const int p0 = s_cp[0], p1 = s_cp[1];
+            nanc = s_cn[0] | s_cn[1];
+            mxv = v0 > v1 ? v0 : v1;
+            cp = (v0 == v1) ? min(p0, p1) : (v0 > v1 ? p0 : p1);
+        }
+        const int piv = (nanc || mxv < 0.0) ? c : cp;
+        if (pos == c) pos = piv;
+        else if (pos == piv) pos = c;
+        // the pivot row becomes row c of U (entries past its end are overwritten later)
+        if (pos == c) {
+#pragma unroll
+            for (int j = 0; j <= W; ++j) U[ub + j] = a[j];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Y[4 * c + q] = r[q];
+        }
+        __syncthreads();
+        // A zero pivot leaves the rows below untouched (the oracle skips the
+        // column): multiplier -0 against a row of +0 gives x + (-0) = x exactly
+        // for every x, so the same instructions run in both cases.
+        const double d = U[ub];
+        const bool sing = (d == 0.0);
+        const double nl = sing ? -0.0 : -(a[0] / d);
+        const double* u = sing ? zero : U + ub;
+        const double* y = sing ? zero : Y + 4 * c;
+#pragma unroll
+        for (int j = 0; j < W; ++j) a[j] = fma3(nl, u[j + 1], a[j + 1]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = fma3(nl, y[q], r[q]);
+        ub += M - c;
+    }
+
+    template <int W>
+    __device__ __forceinline__ void phase(int& c) {
+        for (; c < M && M - 1 - c > W - 8; ++c) step<W>(c);
+        if constexpr (W > 0) phase<W - 8>(c);
+    }
+};
+
+template <int NW, int AW>
+__global__ __launch_bounds__(64 * NW) void weights_wave_kernel(
+    int dim, int phs, int m, const int* __restrict__ exps, SellView sv,
+    const double* __restrict__ xyz, int npad, double* __restrict__ gx, double* __restrict__ gy,
+    double* __restrict__ gz, double* __restrict__ lap, const int* __restrict__ rows) {
+    extern __shared__ double lds[];
+    __shared__ double s_mm[NW][6];
+    __shared__ int s_has[NW];
+    __shared__ double s_sc[3], s_mn[3];
+    __shared__ double s_cv[NW];
+    __shared__ int s_cp[NW], s_cn[NW];
+    __shared__ int s_e[MAX_POLY * 3];
+    __shared__ double s_zero[AW + 1];
+
+    const int r = rows[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int k = sv.len[r];
+    const int M = k + m;
+    const int base = sell_base(sv.sptr, r);
+    double* U = lds;                          // packed rows of U
+    double* Y = lds + M * (M + 1) / 2 + 8;    // forward-eliminated right-hand sides [M][4]
+    double* xs = lds;                         // assembly: scaled coordinates [k][4]
+    double* P = lds + 4 * k;                  // assembly: monomial t at point j, P[t*k + j]
+
+    for (int i = tid; i < m * dim; i += 64 * NW) s_e[i] = exps[i];
+    // ---- shifting_scaling: thread t holds point t ----
+    double xp[3] = {0.0, 0.0, 0.0};
+    int has = tid < k;
+    if (has) {
+        const int cidx = sv.col[base + sell_off(tid)];
+        for (int a = 0; a < dim; ++a) xp[a] = xyz[(size_t)a * npad + cidx];
+    }
+    for (int a = 0; a < dim; ++a) {
+        double mn = xp[a], mx = xp[a];
+        int h = has;
+        ordered_minmax(mn, mx, h, lane);
+        if (lane == 0) {
+            s_mm[wv][2 * a] = mn;
+            s_mm[wv][2 * a + 1] = mx;
+            s_has[wv] = h;
+        }
+    }
+    __syncthreads();
+    if (tid < dim) {
+        double mn = s_mm[0][2 * tid], mx = s_mm[0][2 * tid + 1];
+        if (NW == 2 && s_has[1]) {
+            const double mn1 = s_mm[1][2 * tid], mx1 = s_mm[1][2 * tid + 1];
+            if (mn > mn1) mn = mn1;
+            if (mx < mx1) mx = mx1;
+        }
+        s_mn[tid] = mn;
+        s_sc[tid] = mx - mn;
+    }
+    __syncthreads();
+    if (has) {
+        for (int a = 0; a < dim; ++a) {
+            xp[a] = (xp[a] - s_mn[a]) / s_sc[a];
+            xs[tid * 4 + a] = xp[a];
+        }
+        for (int t = 0; t < m; ++t) {
+            const int* e = s_e + t * dim;
+            double v = ipow_d(xp[0], e[0]);
+            for (int a = 1; a < dim; ++a) v = v * ipow_d(xp[a], e[a]);
+            P[t * k + tid] = v;
+        }
+    }
+    __syncthreads();
+
+    // ---- row i of A = [Phi P; P^T 0] and its right-hand sides ----
+    const int i = tid;
+    const bool valid = i < M;
+    const int ip = i < k ? i : 0;                     // the row's point (stencil rows)
+    const int it = (i >= k && i < M) ? i - k : 0;     // the row's monomial (polynomial rows)
+    const double xi0 = xs[ip * 4], xi1 = xs[ip * 4 + 1], xi2 = xs[ip * 4 + 2];
+    double a[AW + 1];
+#pragma unroll
+    for (int j = 0; j < AW; ++j) {
+        double v = 0.0;
+        if (j < M) {
+            if (j < k) {
+                const double d0 = xi0 - xs[j * 4], d1 = xi1 - xs[j * 4 + 1];
+                double s = 0.0;
+                s = s + d0 * d0;
+                s = s + d1 * d1;
+                if (dim == 3) {
+                    const double d2 = xi2 - xs[j * 4 + 2];
+                    s = s + d2 * d2;
+                }
+                const double rb = (i == j) ? 0.0 : rpow_d(s, phs);
+                v = (i < k) ? rb : P[it * k + j];
+            } else {
+                v = (i < k) ? P[(j - k) * k + ip] : 0.0;
+            }
+        }
+        a[j] = v;
+    }
+    a[AW] = 0.0;
+    double rr[4] = {0.0, 0.0, 0.0, 0.0};
+    if (valid) rhs_entries(i < k, dim, phs, xs + 4 * ip, xs, s_sc, s_e + it * dim, rr);
+    __syncthreads();  // the assembly tables are overwritten by U from here on
+
+    // ---- LU with partial pivoting, right-looking ----
+    for (int j = tid; j <= AW; j += 64 * NW) s_zero[j] = 0.0;
+    __syncthreads();
+    WaveLu<NW, AW> lu{a, rr, U, Y, s_zero, s_cv, s_cp, s_cn, M, valid ? i : (1 << 20), lane, wv, valid};
+    int c = 0;
+    lu.template phase<AW>(c);
+    if (NW == 1) lu.store_u();
+    __syncthreads();
+
+    // ---- back substitution, column oriented (wavefront 0, positions lane and lane+64) ----
+    if (wv != 0) return;
+    const int t0 = lane, t1 = lane + 64;
+    double y0[4] = {0.0, 0.0, 0.0, 0.0}, y1[4] = {0.0, 0.0, 0.0, 0.0};
+    if (t0 < M)
+        for (int q = 0; q < 4; ++q) y0[q] = Y[4 * t0 + q];
+    if (NW == 2 && t1 < M)
+        for (int q = 0; q < 4; ++q) y1[q] = Y[4 * t1 + q];
+    const int b0 = t0 * M - t0 * (t0 - 1) / 2 - t0;  // U[t][c] = U[b_t + c]
+    const int b1 = t1 * M - t1 * (t1 - 1) / 2 - t1;
+    for (int c = M - 1; c >= 0; --c) {
+        const double d = U[c * M - c * (c - 1) / 2];
+        double x[4];
+        if (NW == 1 || c < 64) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[q] = readlane_d(y0[q], c & 63) / d;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[q] = readlane_d(y1[q], c & 63) / d;
+        }
+        if (t0 < c) {
+            const double u = U[b0 + c];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) y0[q] = fma(-u, x[q], y0[q]);
+        } else if (t0 == c) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) y0[q] = x[q];
+        }
+        if (NW == 2) {
+            if (t1 < c) {
+                const double u = U[b1 + c];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) y1[q] = fma(-u, x[q], y1[q]);
+            } else if (t1 == c) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) y1[q] = x[q];
+            }
+        }
+    }
+    auto put = [&](int t, const double (&y)[4]) {
+        const size_t slot = (size_t)base + (size_t)sell_off(t);
+        gx[slot] = y[0];
+        gy[slot] = y[1];
+        if (dim == 3) gz[slot] = y[2];
+        lap[slot] = y[dim];
+    };
+    if (t0 < k) put(t0, y0);
+    if (NW == 2 && t1 < k) put(t1, y1);
+}
+
 // Entry (i, j) of the saddle-point matrix from the scaled coordinates, with the
 // arithmetic of the FP64 assembly above.
 __device__ __forceinline__ double saddle_entry(int i, int j, int k, int dim, int phs,
@@ -450,84 +840,10 @@ __global__ __launch_bounds__(256) void weights_mixed_kernel(
         Af[i * LDA + j] = (float)saddle_entry(i, j, k, dim, phs, xs, s_e);
     }
     // FP64 right-hand sides (same formulas as the FP64 kernel)
-    const double p = (double)phs;
     for (int i = tid; i < M; i += blockDim.x) {
         for (int q = 0; q < 4; ++q) B[i * 4 + q] = 0.0;
-        if (i < k) {
-            double r2 = 0.0;
-            for (int a = 0; a < dim; ++a) {
-                double d = xs[i * 3 + a] - xs[a];
-                r2 = r2 + d * d;
-            }
-            for (int a = 0; a < dim; ++a) {
-                double v = (-p) * (xs[i * 3 + a] - xs[a]);
-                v = v * rpow_d(r2, phs - 2);
-                B[i * 4 + a] = v / s_sc[a];
-            }
-            double lp = 0.0;
-            if (fabs(r2) > 1e-14) {
-                double g[3] = {0.0, 0.0, 0.0};
-                for (int a = 0; a < dim; ++a) {
-                    double d = xs[i * 3 + a] - xs[a];
-                    double d2 = d * d;
-                    double q4 = rpow_d(r2, phs - 4);
-                    double v = (p * p) * d2 * q4;
-                    v = v - (2.0 * p) * d2 * q4;
-                    v = v + p * rpow_d(r2, phs - 2);
-                    g[a] = v / (s_sc[a] * s_sc[a]);
-                }
-                lp = g[0] + g[1];
-                if (dim == 3) lp = lp + g[2];
-            }
-            B[i * 4 + dim] = lp;
-        } else {
-            const int t = i - k;
-            const int* e = s_e + t * dim;
-            const double x0 = xs[0], x1 = xs[1], x2 = (dim == 3) ? xs[2] : 0.0;
-            for (int a = 0; a < dim; ++a) {
-                double v = 0.0;
-                if (e[a] > 0) {
-                    if (a == 1) {
-                        v = ipow_d(x0, e[0]);
-                        v = v * (double)e[1];
-                        v = v * ipow_d(x1, e[1] - 1);
-                        if (dim == 3) v = v * ipow_d(x2, e[2]);
-                    } else if (a == 0) {
-                        v = (double)e[0] * ipow_d(x0, e[0] - 1);
-                        v = v * ipow_d(x1, e[1]);
-                        if (dim == 3) v = v * ipow_d(x2, e[2]);
-                    } else {
-                        v = (double)e[2] * ipow_d(x2, e[2] - 1);
-                        v = v * ipow_d(x0, e[0]);
-                        v = v * ipow_d(x1, e[1]);
-                    }
-                    v = v / s_sc[a];
-                }
-                B[i * 4 + a] = v;
-            }
-            double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-            if (e[0] > 1) {
-                double v = (double)e[0] * ((double)e[0] - 1.0) * ipow_d(x0, e[0] - 2);
-                v = v * ipow_d(x1, e[1]);
-                if (dim == 3) v = v * ipow_d(x2, e[2]);
-                g0 = v / (s_sc[0] * s_sc[0]);
-            }
-            if (e[1] > 1) {
-                double v = (double)e[1] * ((double)e[1] - 1.0) * ipow_d(x1, e[1] - 2);
-                v = v * ipow_d(x0, e[0]);
-                if (dim == 3) v = v * ipow_d(x2, e[2]);
-                g1 = v / (s_sc[1] * s_sc[1]);
-            }
-            if (dim == 3 && e[2] > 1) {
-                double v = (double)e[2] * ((double)e[2] - 1.0) * ipow_d(x2, e[2] - 2);
-                v = v * ipow_d(x0, e[0]);
-                v = v * ipow_d(x1, e[1]);
-                g2 = v / (s_sc[2] * s_sc[2]);
-            }
-            double lp = g0 + g1;
-            if (dim == 3) lp = lp + g2;
-            B[i * 4 + dim] = lp;
-        }
+        rhs_entries(i < k, dim, phs, xs + 3 * (i < k ? i : 0), xs, s_sc,
+                    s_e + (i < k ? 0 : i - k) * dim, B + 4 * i);
         perm[i] = i;
     }
     __syncthreads();
@@ -704,6 +1020,58 @@ void launch_weights_fp64(hipStream_t s, int dim, int phs, int m, const int* exps
     hipLaunchKernelGGL(weights_fp64_kernel, dim3(rows ? nrows : sv.nv), dim3(256), lds, s, dim,
                        phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, cond, rows);
     MP_HIP(hipGetLastError());
+}
+
+
+template <int NW, int AW>
+void launch_wave(hipStream_t s, int dim, int phs, int m, const int* exps_dev, const SellView& sv,
+                 const double* xyz_soa, int npad, double* gx, double* gy, double* gz, double* lap,
+                 const int* rows, int nrows) {
+    if (nrows == 0) return;
+    const size_t lds = sizeof(double) * (size_t)wave_lds_doubles(AW, AW - m, m);
+    if (lds > 64 * 1024)
+        MP_HIP(hipFuncSetAttribute((const void*)weights_wave_kernel<NW, AW>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((weights_wave_kernel<NW, AW>), dim3(nrows), dim3(64 * NW), lds, s, dim,
+                       phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, rows);
+    MP_HIP(hipGetLastError());
+}
+
+void launch_weights_fp64_rows(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
+                              const SellView& sv, const double* xyz_soa, int npad, double* gx,
+                              double* gy, double* gz, double* lap, const std::vector<int>& rows,
+                              const std::vector<int>& lens, DevBuf<int>& dev_rows) {
+    static const int cls[WAVE_CLASSES] = {32, 48, 64, 80, 96};
+    std::vector<int> bucket[WAVE_CLASSES + 1];
+    int max_long = 0;
+    for (size_t t = 0; t < rows.size(); ++t) {
+        const int M = lens[t] + m;
+        int b = 0;
+        while (b < WAVE_CLASSES && M > cls[b]) ++b;
+        bucket[b].push_back(rows[t]);
+        if (b == WAVE_CLASSES) max_long = std::max(max_long, lens[t]);
+    }
+    std::vector<int> all;
+    all.reserve(rows.size());
+    int off[WAVE_CLASSES + 2];
+    for (int b = 0; b <= WAVE_CLASSES; ++b) {
+        off[b] = (int)all.size();
+        all.insert(all.end(), bucket[b].begin(), bucket[b].end());
+    }
+    off[WAVE_CLASSES + 1] = (int)all.size();
+    if (all.empty()) return;
+    dev_rows.alloc(all.size());
+    MP_HIP(hipMemcpyAsync(dev_rows.p, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, s));
+    auto n = [&](int b) { return off[b + 1] - off[b]; };
+    const int* d = dev_rows.p;
+    launch_wave<1, 32>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[0], n(0));
+    launch_wave<1, 48>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[1], n(1));
+    launch_wave<1, 64>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[2], n(2));
+    launch_wave<2, 80>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[3], n(3));
+    launch_wave<2, 96>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[4], n(4));
+    if (n(WAVE_CLASSES) > 0)
+        launch_weights_fp64(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, nullptr,
+                            max_long, d + off[WAVE_CLASSES], n(WAVE_CLASSES));
 }
 
 }  // namespace mp

@@ -139,6 +139,19 @@ struct PpeSolver {
     double smooth_bytes() const;
     // r = b - A x (FP64, unscaled operator incl. border) into `tmp`; returns tmp.p
     double* residual_of(hipStream_t s, const double* b_dev, const double* x_dev);
+    // |b - A x|^2 of system 0 from the last residual_of (waits for the stream)
+    double last_residual_norm2(hipStream_t s);
+    // Second candidate for the next solve's guess (single system): solve()
+    // starts from it instead when the guess's FP64 residual is not finite or
+    // exceeds FALLBACK_FACTOR times the candidate's (fallback_rr = its
+    // |b - A x|^2).  Cleared by solve().  The factor is not 1: the learned
+    // guess typically leaves a larger residual than its extrapolation (its
+    // remaining error is high-frequency, which the Krylov solve removes in a
+    // few iterations) and still converges in fewer iterations; the bound
+    // catches a broken network, not a worse-looking residual.
+    static constexpr double FALLBACK_FACTOR = 10.0;
+    const double* fallback_x = nullptr;
+    double fallback_rr = 0.0;
     // Drops the captured chunk graphs (their kernel arguments point into the
     // operator and preconditioner buffers).
     void release_graphs();

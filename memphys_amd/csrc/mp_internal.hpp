@@ -257,6 +257,15 @@ struct SegCtl {
 
 // ---- launchers (defined in the .hip files) ----
 // rows == nullptr: every row; otherwise the listed device rows.
+// FP64 weights of the device rows `rows` (stencil lengths `lens`): the
+// register-resident kernel for systems of M = k + m <= 96 (one or two
+// wavefronts per stencil), the LDS kernel beyond.  dev_rows receives the
+// grouped row list and must stay allocated until the launches complete.
+constexpr int WAVE_CLASSES = 5;
+void launch_weights_fp64_rows(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
+                              const SellView& sv, const double* xyz_soa, int npad, double* gx,
+                              double* gy, double* gz, double* lap, const std::vector<int>& rows,
+                              const std::vector<int>& lens, DevBuf<int>& dev_rows);
 void launch_weights_fp64(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
                          const SellView& sv, const double* xyz_soa, int npad, double* gx,
                          double* gy, double* gz, double* lap, double* cond, int max_len,
@@ -272,7 +281,8 @@ void launch_predictor(hipStream_t s, const SellView& sv, const double* gx, const
                       const double* gz, const double* lap, const double* u, const double* v,
                       const double* w, double* uh, double* vh, double* wh, double* S_new,
                       const double* S_oo, int npad, double rho, double mu, double dt, int euler,
-                      const int* bprob = nullptr, const double* prm = nullptr);
+                      const int* bprob = nullptr, const double* prm = nullptr,
+                      const double* force = nullptr);
 template <int DIM>
 void launch_ppe_rhs(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
                     const double* gz, const double* uh, const double* vh, const double* wh,

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <ctime>
 #include <fstream>
 #include <numeric>
@@ -49,12 +50,78 @@ std::string trim(const std::string& s) {
 }
 }  // namespace
 
+namespace {
+// The GPU settings file of a parameter file: MEMPHYS_GPU_SETTINGS if set,
+// else gpu_settings.csv in the parameter file's directory when it exists.
+std::string gpu_settings_path(const std::string& parameter_file) {
+    if (const char* e = std::getenv("MEMPHYS_GPU_SETTINGS")) return e;
+    size_t sl = parameter_file.find_last_of('/');
+    std::string dir = sl == std::string::npos ? "" : parameter_file.substr(0, sl + 1);
+    std::string p = dir + "gpu_settings.csv";
+    std::ifstream f(p);
+    return f ? p : "";
+}
+}  // namespace
+
+void PARAMETERS::read_gpu_settings(std::string settings_file) {
+    std::ifstream f(settings_file);
+    if (!f) fail("PARAMETERS::read_gpu_settings", "cannot open " + settings_file);
+    gpu_settings_file = settings_file;
+    std::string line;
+    while (std::getline(f, line)) {
+        line = trim(line);
+        if (line.empty() || line[0] == '#') continue;
+        size_t c = line.find(',');
+        if (c == std::string::npos)
+            fail("PARAMETERS::read_gpu_settings", "expected key,value: " + line);
+        std::string key = trim(line.substr(0, c)), val = trim(line.substr(c + 1));
+        auto bad = [&]() {
+            fail("PARAMETERS::read_gpu_settings", "bad value for " + key + ": " + val);
+        };
+        try {
+            if (key == "precision") {
+                if (val == "fp64") precision = 0;
+                else if (val == "mixed") precision = 1;
+                else bad();
+            } else if (key == "preconditioner") {
+                if (val == "jacobi") preconditioner = 0;
+                else if (val == "multigrid") preconditioner = 1;
+                else bad();
+            } else if (key == "guess") {
+                if (val == "previous") guess = 0;
+                else if (val == "learned") guess = 1;
+                else if (val == "extrapolate") guess = 2;
+                else bad();
+            } else if (key == "guess_params") {
+                guess_params_file = val;
+            } else if (key == "graphs") {
+                if (val == "0" || val == "1") graphs = std::stoi(val);
+                else bad();
+            } else if (key == "device") {
+                device = std::stoi(val);
+            } else if (key == "inner_tolerance") {
+                inner_tolerance = std::stod(val);
+                if (!(inner_tolerance > 0.0 && inner_tolerance < 1.0)) bad();
+            } else {
+                fail("PARAMETERS::read_gpu_settings", "unknown key " + key);
+            }
+        } catch (const std::invalid_argument&) {
+            bad();
+        } catch (const std::out_of_range&) {
+            bad();
+        }
+    }
+    if (guess == 1 && guess_params_file.empty())
+        fail("PARAMETERS::read_gpu_settings", "guess learned needs guess_params");
+}
+
 PARAMETERS::PARAMETERS(std::string parameter_file, std::string gmsh_file) {
     meshfile = gmsh_file;
     std::ifstream f(meshfile);
     if (!f) fail("PARAMETERS::PARAMETERS", "cannot open " + meshfile);
     read_calc_parameters(parameter_file);
     verify_parameters();
+    if (std::string g = gpu_settings_path(parameter_file); !g.empty()) read_gpu_settings(g);
     get_problem_dimension_msh();
     calc_cloud_num_points();
     calc_polynomial_term_exponents();
@@ -64,6 +131,7 @@ PARAMETERS::PARAMETERS(std::string parameter_file, int dim) {
     meshfile = "box.msh";
     read_calc_parameters(parameter_file);
     verify_parameters();
+    if (std::string g = gpu_settings_path(parameter_file); !g.empty()) read_gpu_settings(g);
     if (dim != 2 && dim != 3) fail("PARAMETERS::PARAMETERS", "dimension must be 2 or 3");
     dimension = dim;
     calc_cloud_num_points();
@@ -441,8 +509,11 @@ void SOLVER::init(POINTS& points, CLOUD& cloud, PARAMETERS& parameters,
     solver_tolerance = parameters.solver_tolerance;
     solver_type = parameters.solver_type;
     std::transform(solver_type.begin(), solver_type.end(), solver_type.begin(), ::tolower);
+    // sor (solver.cpp:243-281) is a sequential Gauss-Seidel sweep with no GPU
+    // form here: refused with the message the reference prints for a type it
+    // cannot run (solver.cpp:73-77)
     if (solver_type != "hypre_ilu_gmres" && solver_type != "eigen_direct" &&
-        solver_type != "eigen_ilu_bicgstab" && solver_type != "sor")
+        solver_type != "eigen_ilu_bicgstab")
         fail("SOLVER::SOLVER", "solver_type should be either hypre_ilu_gmres, eigen_ilu_bicgstab or "
                                "eigen_direct; current value: " + solver_type);
     n_iter = parameters.n_iter;
@@ -471,8 +542,7 @@ void SOLVER::init(POINTS& points, CLOUD& cloud, PARAMETERS& parameters,
                               solver_tolerance, n_iter, parameters.precision),
           "SOLVER::init");
     if (solver_type == "hypre_ilu_gmres")  // GMRES(gmres_kdim), solver.cpp:140-144
-        check(ctx, mp_solver_krylov(ctx, MP_KRYLOV_GMRES, std::max(1, std::min(gmres_kdim, 31))),
-              "SOLVER::init");
+        check(ctx, mp_solver_krylov(ctx, MP_KRYLOV_GMRES, gmres_kdim), "SOLVER::init");
     parameters.factoring_timer = (double)(clock() - t1) / CLOCKS_PER_SEC;
     check(ctx, mp_solver_system_size(ctx, &system_size), "SOLVER::init");
 }
@@ -578,11 +648,18 @@ void FRACTIONAL_STEP_1::create(POINTS& points, CLOUD& cloud, PARAMETERS& paramet
     prm.solver_tolerance = parameters.solver_tolerance;
     prm.n_iter = parameters.n_iter;
     prm.precision = parameters.precision;
-    prm.inner_tolerance = 1e-3;
+    prm.inner_tolerance = parameters.inner_tolerance;
     prm.check_every = 8;
-    if (parameters.solver_type == "hypre_ilu_gmres") {  // GMRES(gmres_kdim), solver.cpp:140-144
+    prm.preconditioner = parameters.preconditioner;
+    std::string st = parameters.solver_type;
+    std::transform(st.begin(), st.end(), st.begin(), ::tolower);
+    if (st == "sor")  // solver.cpp:243-281 has no GPU form (see SOLVER::init)
+        fail("FRACTIONAL_STEP_1::FRACTIONAL_STEP_1",
+             "solver_type should be either hypre_ilu_gmres, eigen_ilu_bicgstab or eigen_direct; "
+             "current value: " + st);
+    if (st == "hypre_ilu_gmres") {  // GMRES(gmres_kdim), solver.cpp:140-144
         prm.krylov = MP_KRYLOV_GMRES;
-        prm.restart = std::max(1, std::min(parameters.gmres_kdim, 31));
+        prm.restart = parameters.gmres_kdim;
     }
     clock_t t1 = clock();
     check(ctx, mp_fs_create(ctx, &prm, fl[0].data(), fl[1].data(),
@@ -590,14 +667,37 @@ void FRACTIONAL_STEP_1::create(POINTS& points, CLOUD& cloud, PARAMETERS& paramet
           "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
     parameters.factoring_timer = (double)(clock() - t1) / CLOCKS_PER_SEC;
     check(ctx, mp_solver_system_size(ctx, &system_size), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
+    check(ctx, mp_graph_mode(ctx, parameters.graphs), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
+    if (parameters.guess == MP_GUESS_LEARNED) {
+        std::vector<float> w(mp_guess_param_count());
+        std::ifstream f(parameters.guess_params_file, std::ios::binary);
+        if (!f || !f.read(reinterpret_cast<char*>(w.data()), sizeof(float) * w.size()))
+            fail("FRACTIONAL_STEP_1::FRACTIONAL_STEP_1",
+                 "cannot read " + std::to_string(w.size()) + " float32 parameters from " +
+                     parameters.guess_params_file);
+        check(ctx, mp_guess_mode(ctx, MP_GUESS_LEARNED), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
+        check(ctx, mp_guess_params(ctx, 1, w.data()), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
+    } else {
+        check(ctx, mp_guess_mode(ctx, parameters.guess), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
+    }
 }
 
 void FRACTIONAL_STEP_1::step(POINTS& points, CLOUD& cloud, PARAMETERS& parameters,
                              VectorXd* U_new[3], VectorXd& p_new, VectorXd* U_old[3],
-                             VectorXd& p_old, int it1) {
+                             VectorXd& p_old, VectorXd* F[3], int it1) {
     (void)cloud;
     it = it1;
     const int nv = points.nv, dim = parameters.dimension;
+    if (F) {
+        for (int q = 0; q < dim; ++q)
+            if ((int)F[q]->size() != nv)
+                fail("FRACTIONAL_STEP_1::single_timestep", "body force size != nv");
+        check(ctx, mp_fs_body_force(ctx, F[0]->data(), F[1]->data(), dim == 3 ? F[2]->data() : nullptr),
+              "FRACTIONAL_STEP_1::single_timestep");
+    } else {
+        check(ctx, mp_fs_body_force(ctx, nullptr, nullptr, nullptr),
+              "FRACTIONAL_STEP_1::single_timestep");
+    }
     if ((int)p_new.size() != system_size) p_new = VectorXd::Zero(system_size);
     if ((int)p_old.size() != system_size) p_old = VectorXd::Zero(system_size);
     for (int q = 0; q < dim; ++q) {
@@ -627,7 +727,19 @@ void FRACTIONAL_STEP_1::single_timestep_2d(POINTS& points, CLOUD& cloud, PARAMET
                                            int it1) {
     VectorXd* Un[3] = {&u_new, &v_new, nullptr};
     VectorXd* Uo[3] = {&u_old, &v_old, nullptr};
-    step(points, cloud, parameters, Un, p_new, Uo, p_old, it1);
+    step(points, cloud, parameters, Un, p_new, Uo, p_old, nullptr, it1);
+}
+
+// fractional_step_1.cpp:65-76 (calc_vel_hat with the body force, 104-118)
+void FRACTIONAL_STEP_1::single_timestep_2d(POINTS& points, CLOUD& cloud, PARAMETERS& parameters,
+                                           VectorXd& u_new, VectorXd& v_new, VectorXd& p_new,
+                                           VectorXd& u_old, VectorXd& v_old, VectorXd& p_old,
+                                           VectorXd& body_force_x, VectorXd& body_force_y,
+                                           int it1) {
+    VectorXd* Un[3] = {&u_new, &v_new, nullptr};
+    VectorXd* Uo[3] = {&u_old, &v_old, nullptr};
+    VectorXd* F[3] = {&body_force_x, &body_force_y, nullptr};
+    step(points, cloud, parameters, Un, p_new, Uo, p_old, F, it1);
 }
 
 void FRACTIONAL_STEP_1::single_timestep_3d(POINTS& points, CLOUD& cloud, PARAMETERS& parameters,
@@ -637,7 +749,20 @@ void FRACTIONAL_STEP_1::single_timestep_3d(POINTS& points, CLOUD& cloud, PARAMET
     if (parameters.dimension != 3) fail("FRACTIONAL_STEP_1::single_timestep_3d", "2D cloud");
     VectorXd* Un[3] = {&u_new, &v_new, &w_new};
     VectorXd* Uo[3] = {&u_old, &v_old, &w_old};
-    step(points, cloud, parameters, Un, p_new, Uo, p_old, it1);
+    step(points, cloud, parameters, Un, p_new, Uo, p_old, nullptr, it1);
+}
+
+void FRACTIONAL_STEP_1::single_timestep_3d(POINTS& points, CLOUD& cloud, PARAMETERS& parameters,
+                                           VectorXd& u_new, VectorXd& v_new, VectorXd& w_new,
+                                           VectorXd& p_new, VectorXd& u_old, VectorXd& v_old,
+                                           VectorXd& w_old, VectorXd& p_old,
+                                           VectorXd& body_force_x, VectorXd& body_force_y,
+                                           VectorXd& body_force_z, int it1) {
+    if (parameters.dimension != 3) fail("FRACTIONAL_STEP_1::single_timestep_3d", "2D cloud");
+    VectorXd* Un[3] = {&u_new, &v_new, &w_new};
+    VectorXd* Uo[3] = {&u_old, &v_old, &w_old};
+    VectorXd* F[3] = {&body_force_x, &body_force_y, &body_force_z};
+    step(points, cloud, parameters, Un, p_new, Uo, p_old, F, it1);
 }
 
 // postprocessing_functions.cpp:192-195 / 248-251 with calc_max_l1_error

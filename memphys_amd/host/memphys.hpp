@@ -113,9 +113,18 @@ class PARAMETERS {
            grad_y_eigval_imag = 0, grad_z_eigval_real = 0, grad_z_eigval_imag = 0,
            laplace_eigval_real = 0, laplace_eigval_imag = 0;
 
-    // GPU-side choices (not in the reference's CSV): PPE and weight precision.
-    int precision = 0;  // 0 FP64, 1 FP32-operator inner solve with FP64 defect correction
+    // GPU-side choices (not in the reference's CSV).  They are read from an
+    // optional settings file of "key,value" lines: the file named by the
+    // environment variable MEMPHYS_GPU_SETTINGS, else gpu_settings.csv beside
+    // the parameter file (read_gpu_settings lists the keys).
+    int precision = 0;       // 0 FP64, 1 FP32-operator inner solve with FP64 defect correction
     int device = 0;
+    int preconditioner = 0;  // 0 Jacobi, 1 multigrid V-cycle (enum mp_precond)
+    int guess = 0;           // 0 previous pressure, 1 learned, 2 linear extrapolation
+    std::string guess_params_file;  // packed float32 network parameters (guess = 1)
+    int graphs = 1;          // replay the Krylov chunks from hipGraphs
+    double inner_tolerance = 1e-3;  // precision 1: inner relative reduction per cycle
+    std::string gpu_settings_file;  // the settings file read, if any
 
   public:
     PARAMETERS(std::string parameter_file, std::string gmsh_file);
@@ -123,6 +132,11 @@ class PARAMETERS {
     PARAMETERS(std::string parameter_file, int dimension);
     void read_calc_parameters(std::string parameter_file);
     void verify_parameters();
+    // Keys: precision (fp64 | mixed), preconditioner (jacobi | multigrid),
+    // guess (previous | extrapolate | learned), guess_params (file),
+    // graphs (0 | 1), device (int), inner_tolerance (double).  Unknown keys
+    // and values are errors.
+    void read_gpu_settings(std::string settings_file);
     void calc_cloud_num_points();
     void get_problem_dimension_msh();
     void calc_polynomial_term_exponents();
@@ -222,9 +236,19 @@ class FRACTIONAL_STEP_1 {
     void single_timestep_2d(POINTS& points, CLOUD& cloud, PARAMETERS& parameters, VectorXd& u_new,
                             VectorXd& v_new, VectorXd& p_new, VectorXd& u_old, VectorXd& v_old,
                             VectorXd& p_old, int it1);
+    // with a body force per unit volume (class.hpp:253, fractional_step_1.cpp:65-76)
+    void single_timestep_2d(POINTS& points, CLOUD& cloud, PARAMETERS& parameters, VectorXd& u_new,
+                            VectorXd& v_new, VectorXd& p_new, VectorXd& u_old, VectorXd& v_old,
+                            VectorXd& p_old, VectorXd& body_force_x, VectorXd& body_force_y,
+                            int it1);
     void single_timestep_3d(POINTS& points, CLOUD& cloud, PARAMETERS& parameters, VectorXd& u_new,
                             VectorXd& v_new, VectorXd& w_new, VectorXd& p_new, VectorXd& u_old,
                             VectorXd& v_old, VectorXd& w_old, VectorXd& p_old, int it1);
+    void single_timestep_3d(POINTS& points, CLOUD& cloud, PARAMETERS& parameters, VectorXd& u_new,
+                            VectorXd& v_new, VectorXd& w_new, VectorXd& p_new, VectorXd& u_old,
+                            VectorXd& v_old, VectorXd& w_old, VectorXd& p_old,
+                            VectorXd& body_force_x, VectorXd& body_force_y,
+                            VectorXd& body_force_z, int it1);
 
     std::vector<bool> u_dirichlet_flag, v_dirichlet_flag, w_dirichlet_flag, p_dirichlet_flag;
     bool p_bc_full_neumann = true;
@@ -234,7 +258,7 @@ class FRACTIONAL_STEP_1 {
   private:
     void create(POINTS& points, CLOUD& cloud, PARAMETERS& parameters);
     void step(POINTS& points, CLOUD& cloud, PARAMETERS& parameters, VectorXd* U_new[3],
-              VectorXd& p_new, VectorXd* U_old[3], VectorXd& p_old, int it1);
+              VectorXd& p_new, VectorXd* U_old[3], VectorXd& p_old, VectorXd* F[3], int it1);
     mp_ctx* ctx = nullptr;
 };
 

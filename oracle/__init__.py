@@ -59,6 +59,8 @@ def lib() -> C.CDLL:
         L.or_ppe_solve.argtypes = [_vp, _dp, _dp, _dp, C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
         L.or_ppe_solve.restype = C.c_int
+        L.or_fs_set_body_force.argtypes = [_vp, _vp, _vp, _vp]
+        L.or_fs_set_body_force.restype = None
         L.or_fs_step.argtypes = [_vp, C.c_int, _dp, _dp, _vp, _dp]
         L.or_fs_step.restype = C.c_int
         L.or_fs_predict.argtypes = [_vp, C.c_int, _dp, _dp, _vp, _dp, _dp, _dp]
@@ -177,6 +179,16 @@ class FractionalStep:
         it = lib().or_ppe_solve(self.h, np.ascontiguousarray(rhs, dtype=np.float64), xn, xo,
                                 C.byref(rr), C.byref(ra))
         return xn, xo, it, rr.value
+
+    def set_body_force(self, fx=None, fy=None, fz=None):
+        """Body force added to the explicit source of the following steps
+        (fractional_step_1.cpp:104-118); no arguments removes it."""
+        if fx is None:
+            lib().or_fs_set_body_force(self.h, None, None, None)
+            return
+        self._force = [np.ascontiguousarray(f, dtype=np.float64) if f is not None else None
+                       for f in (fx, fy, fz)]
+        lib().or_fs_set_body_force(self.h, *[_ptr(f) for f in self._force])
 
     def step(self, it, u, v, w, p):
         """Advances in place (u, v, w, p hold old values on entry)."""

@@ -722,6 +722,7 @@ struct OrFs {
     double tol;
     int maxit;
     std::vector<double> uh[3], S_oo[3];
+    std::vector<double> force[3];  // body force (empty: none)
     // logs of the last step
     int last_iters = 0;
     double last_rel = 0, last_abs = 0;
@@ -787,6 +788,16 @@ OrFs* or_fs_create(int nv, int dim, const int* rp, const int* ci, const double* 
 }
 
 void or_fs_destroy(OrFs* F) { delete F; }
+
+// Body force of the following steps (single_timestep_2d's body_force_x/y
+// overload, fractional_step_1.cpp:65-76; fz in 3D); fx == NULL removes it.
+void or_fs_set_body_force(OrFs* F, const double* fx, const double* fy, const double* fz) {
+    const double* f[3] = {fx, fy, fz};
+    for (int c = 0; c < 3; ++c) {
+        if (fx && c < F->dim && f[c]) F->force[c].assign(f[c], f[c] + F->nv);
+        else F->force[c].clear();
+    }
+}
 
 int or_fs_system_size(OrFs* F) { return F->A.n; }
 
@@ -873,6 +884,8 @@ void fs_predict_rhs(OrFs* F, int it, double* const U[3], const double* p,
             double conv = -(u[i] * g[0][i]) - v[i] * g[1][i];
             if (dim == 3) conv = conv - w[i] * g[2][i];
             S[c][i] = rho * conv + mu * g[3][i];
+            // the body-force overload (fractional_step_1.cpp:104-118): + f
+            if (!F->force[c].empty()) S[c][i] = S[c][i] + F->force[c][i];
         }
         for (int i = 0; i < nv; ++i) {
             if (F->order == 1 || it == 0)

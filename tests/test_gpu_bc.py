@@ -94,10 +94,12 @@ def _run(ctx, cloud, w, steps, precision, precond, krylov, restart=0, rho=1.0, m
            f"rel_res {st.rel_res:.2e} switches {st.gmres_switches}")
     assert max(errs) <= 1e-8
     assert st.converged == 1
-    # the outlet keeps its pressure and the slip wall's tangential velocity
-    # satisfies n.grad u = 0 (checked through the oracle's equal fields)
+    # the outlet keeps its pressure (identity rows, b = p_old: met to the
+    # solver's tolerance, as by the reference's Krylov solve) and the slip
+    # wall's tangential velocity satisfies n.grad u = 0 (checked through the
+    # oracle's equal fields)
     outlet = p_dir == 1
-    assert np.array_equal(pg[outlet], p0[outlet])
+    assert np.abs(pg[outlet] - p0[outlet]).max() <= 1e-10 * np.abs(p0).max()
     return st
 
 
@@ -185,3 +187,66 @@ def test_nonfinite_guess_starts_from_zero(ctx, cube3d_small, precision, precond,
         assert np.all(np.isfinite(ctx.get_field(f)))
     ctx.step(1, 3)
     assert ctx.stats().guess_rejected == 0 and ctx.stats().converged == 1
+
+
+@pytest.mark.parametrize("precision,precond,krylov", [(0, 0, 0), (1, 1, 1)])
+def test_body_force_matches_oracle(ctx, channel2d, precision, precond, krylov):
+    """The body-force overload of single_timestep_2d (fractional_step_1.cpp:
+    65-76, 104-118) through mp_fs_body_force: a force that changes from step
+    to step, then none, against the oracle's set_body_force; fields to 1e-8."""
+    import memphys_amd as M
+    import oracle as O
+    cloud = channel2d
+    ctx.upload(cloud)
+    w = O.stencil_weights(cloud, 3, 3)[:4]
+    ctx.set_weights(*w)
+    nv, x = cloud.nv, cloud.xyz
+    u_dir, p_dir, U0, p0 = channel_bc(cloud)
+    rho, mu, dt = 1.0, 0.01, 2e-4
+    ctx.fs_create(rho, mu, dt, 2, 1e-12, 20000, u_dir[0], u_dir[1], None, p_dir,
+                  precision=precision, precond=precond, krylov=krylov)
+    for q, f in enumerate((M.MP_U, M.MP_V)):
+        ctx.set_field(f, U0[q])
+    ctx.set_field(M.MP_P, p0)
+    fs = O.FractionalStep(cloud, w, u_dir[0], u_dir[1], None, p_dir, rho, mu, dt, 2,
+                          solver_kind=0, precond=2, tol=1e-14, maxit=20000)
+    U = [u.copy() for u in U0]
+    p = p0.copy()
+    _print("case: body force, precision", precision, "precond", precond, "krylov", krylov)
+    for it in range(8):
+        if it < 6:
+            fx = 2.0 * np.sin(np.pi * x[:, 1]) * (1.0 + 0.1 * it)
+            fy = -9.81 + 0.5 * x[:, 0]
+            ctx.body_force(fx, fy)
+            fs.set_body_force(fx, fy)
+        else:
+            ctx.body_force()
+            fs.set_body_force()
+        ctx.step(1, it)
+        fs.step(it, U[0], U[1], None, p)
+    got = [ctx.get_field(f) for f in (M.MP_U, M.MP_V)] + [ctx.get_field(M.MP_P)]
+    errs = [rel_err(g, u) for g, u in zip(got, U + [p])]
+    st = ctx.stats()
+    _print(f"body force errs {['%.2e' % e for e in errs]} rel_res {st.rel_res:.2e}")
+    assert max(errs) <= 1e-8 and st.converged == 1
+    # the force changed the run: a force-free run differs
+    ctx.fs_create(rho, mu, dt, 2, 1e-12, 20000, u_dir[0], u_dir[1], None, p_dir,
+                  precision=precision, precond=precond, krylov=krylov)
+    for q, f in enumerate((M.MP_U, M.MP_V)):
+        ctx.set_field(f, U0[q])
+    ctx.set_field(M.MP_P, p0)
+    ctx.step(8, 0)
+    assert rel_err(ctx.get_field(M.MP_V), U[1]) > 1e-3
+
+
+def test_body_force_arguments_checked(ctx, channel2d):
+    import memphys_amd as M
+    cloud = channel2d
+    ctx.upload(cloud)
+    ctx.weights(3, 3, download=False)
+    u_dir, p_dir, U0, p0 = channel_bc(cloud)
+    ctx.fs_create(1.0, 0.01, 2e-4, 2, 1e-10, 2000, u_dir[0], u_dir[1], None, p_dir)
+    with pytest.raises(M.MemphysError):
+        ctx.body_force(np.zeros(cloud.nv))  # fy missing
+    with pytest.raises(M.MemphysError):
+        ctx.body_force(np.zeros(cloud.nv), np.zeros(cloud.nv), np.zeros(cloud.nv))  # fz in 2D

@@ -12,13 +12,49 @@ def _print(*a):
     print(*a, flush=True)
 
 
-def _fs(ctx, cloud, precision=0, tol=1e-10):
+def _fs(ctx, cloud, precision=0, tol=1e-10, precond=0, krylov=0):
     u_dir, p_dir, u0 = lid_bc(cloud, cloud.dim - 1)
     ctx.upload(cloud)
     (gx, gy, gz, lap), _ = ctx.weights(3, 3)
     ctx.fs_create(1.0, 0.0025, 5e-4, 2, tol, 20000, u_dir, u_dir,
-                  u_dir if cloud.dim == 3 else None, p_dir, precision=precision)
+                  u_dir if cloud.dim == 3 else None, p_dir, precision=precision,
+                  inner_tol=1e-5, precond=precond, krylov=krylov)
     return (gx, gy, gz, lap), u_dir, p_dir, u0
+
+
+def _run_modes(c, params, modes, steps, tol, **solver):
+    """Mean Krylov iterations per step (after the first 20) of a fresh cavity
+    for each guess mode, and the number of steps whose learned guess was
+    replaced by its extrapolation."""
+    import memphys_amd as M
+    res, rejected, guess_res = {}, {}, {}
+    for mode in modes:
+        with M.Context(0) as ctx:
+            _fs(ctx, c, tol=tol, **solver)
+            u_dir, p_dir, u0 = M.lid_conditions(c, 1.0)
+            ctx.set_field(M.MP_U, u0)
+            ctx.set_field(M.MP_V, np.zeros(c.nv))
+            ctx.set_field(M.MP_W, np.zeros(c.nv))
+            ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
+            ctx.guess_mode(mode)
+            if mode == M.MP_GUESS_LEARNED:
+                ctx.guess_params(params)
+            its, rej, g = [], 0, []
+            for it in range(steps):
+                ctx.step(1, it)
+                st = ctx.stats()
+                its.append(st.iterations)
+                g.append(st.guess_rel_res)
+                rej += st.guess_rejected
+                assert st.rel_res <= tol and st.converged == 1
+            res[mode] = float(np.mean(its[20:]))
+            rejected[mode] = rej
+            guess_res[mode] = np.array(g)
+    if M.MP_GUESS_LEARNED in modes and M.MP_GUESS_EXTRAPOLATE in modes:
+        ratio = guess_res[M.MP_GUESS_LEARNED][20:] / guess_res[M.MP_GUESS_EXTRAPOLATE][20:]
+        _print("initial residual, learned / extrapolated: percentiles 10/50/90",
+               np.percentile(ratio, [10, 50, 90]))
+    return res, rejected
 
 
 def test_guess_forward_and_gradient_match_oracle(cube3d_small):
@@ -69,10 +105,40 @@ def test_trained_guess_saves_iterations():
     c = M.box_cloud(dim=3, n=16, jitter=0.3, seed=11, k_interior=40, k_layers=(60, 55, 50))
     params = M.train_guess(c, 1.0, 0.0025, 5e-4, 1e-8, episodes=4, steps_per_episode=100,
                            seed=3, log=_print)
-    res = {}
-    for mode in (M.MP_GUESS_PREVIOUS, M.MP_GUESS_EXTRAPOLATE, M.MP_GUESS_LEARNED):
+    modes = (M.MP_GUESS_PREVIOUS, M.MP_GUESS_EXTRAPOLATE, M.MP_GUESS_LEARNED)
+    res, rej = _run_modes(c, params, modes, 120, 1e-8)
+    _print("Jacobi BiCGSTAB, mean iterations: previous", res[M.MP_GUESS_PREVIOUS], "extrapolate",
+           res[M.MP_GUESS_EXTRAPOLATE], "learned", res[M.MP_GUESS_LEARNED],
+           "learned guesses replaced by the extrapolation", rej[M.MP_GUESS_LEARNED])
+    assert res[M.MP_GUESS_LEARNED] <= 0.85 * res[M.MP_GUESS_PREVIOUS]
+    # against its own extrapolation term the network gains little or nothing
+    # on this short training (recorded, bounded: no more than 5 % worse)
+    assert res[M.MP_GUESS_LEARNED] <= 1.05 * res[M.MP_GUESS_EXTRAPOLATE]
+    # the headline solver: FP32-operator GMRES(30) with the multigrid V-cycle
+    res, rej = _run_modes(c, params, modes, 120, 1e-8, precision=M.MP_PREC_MIXED,
+                          precond=M.MP_PRECOND_AMG, krylov=M.MP_KRYLOV_GMRES)
+    _print("mixed GMRES + multigrid, mean iterations: previous", res[M.MP_GUESS_PREVIOUS],
+           "extrapolate", res[M.MP_GUESS_EXTRAPOLATE], "learned", res[M.MP_GUESS_LEARNED],
+           "learned guesses replaced by the extrapolation", rej[M.MP_GUESS_LEARNED])
+    assert res[M.MP_GUESS_LEARNED] < res[M.MP_GUESS_PREVIOUS]
+    assert res[M.MP_GUESS_LEARNED] <= 1.05 * res[M.MP_GUESS_EXTRAPOLATE]
+
+
+@pytest.mark.parametrize("precision,precond,krylov", [(0, 0, 0), (1, 1, 1)])
+def test_nan_network_falls_back_to_extrapolation(cube3d_small, precision, precond, krylov):
+    """A network whose parameters are NaN gives a NaN guess: every solve starts
+    from the linear extrapolation instead (guess_rejected), converges to the
+    tolerance, and the fields stay finite and equal to the extrapolation
+    form's run."""
+    import memphys_amd as M
+    from oracle import gnn as G
+    c = cube3d_small
+    nan_params = np.full(G.NPARAM, np.nan, dtype=np.float32)
+    _print("case: NaN network parameters, precision", precision, "precond", precond)
+    out = {}
+    for mode in (M.MP_GUESS_EXTRAPOLATE, M.MP_GUESS_LEARNED):
         with M.Context(0) as ctx:
-            _fs(ctx, c, tol=1e-8)
+            _fs(ctx, c, precision=precision, tol=1e-10, precond=precond, krylov=krylov)
             u_dir, p_dir, u0 = M.lid_conditions(c, 1.0)
             ctx.set_field(M.MP_U, u0)
             ctx.set_field(M.MP_V, np.zeros(c.nv))
@@ -80,13 +146,19 @@ def test_trained_guess_saves_iterations():
             ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
             ctx.guess_mode(mode)
             if mode == M.MP_GUESS_LEARNED:
-                ctx.guess_params(params)
-            its = []
-            for it in range(120):
+                ctx.guess_params(nan_params)
+            rej = []
+            for it in range(6):
                 ctx.step(1, it)
-                its.append(ctx.stats().iterations)
-                assert ctx.stats().rel_res <= 1e-8
-            res[mode] = np.mean(its[20:])
-    _print("mean iterations: previous", res[M.MP_GUESS_PREVIOUS], "extrapolate",
-           res[M.MP_GUESS_EXTRAPOLATE], "learned", res[M.MP_GUESS_LEARNED])
-    assert res[M.MP_GUESS_LEARNED] <= 0.85 * res[M.MP_GUESS_PREVIOUS]
+                st = ctx.stats()
+                rej.append(st.guess_rejected)
+                assert st.converged == 1 and st.rel_res <= 1e-10
+            fields = [ctx.get_field(f) for f in (M.MP_U, M.MP_V, M.MP_W, M.MP_P)]
+            assert all(np.all(np.isfinite(f)) for f in fields)
+            out[mode] = (fields, rej)
+    _print("guess replaced per step", out[M.MP_GUESS_LEARNED][1])
+    # steps 0 and 1 have no pressure history (previous pressure); from step 2
+    # on every NaN guess is replaced by the extrapolation
+    assert out[M.MP_GUESS_LEARNED][1][2:] == [1] * 4
+    for a, b in zip(out[M.MP_GUESS_LEARNED][0], out[M.MP_GUESS_EXTRAPOLATE][0]):
+        assert np.array_equal(a, b)

@@ -298,11 +298,25 @@ def test_condition_estimate_matches_oracle(ctx, cube3d_small):
     assert rel_err(cond, ocond) <= 1e-12
 
 
-@pytest.mark.parametrize("solver_type", ["eigen_ilu_bicgstab", "hypre_ilu_gmres"])
-def test_class_api_driver_matches_oracle(tmp_path, solver_type):
+CLASS_API_CASES = [
+    ("eigen_ilu_bicgstab", None, None),
+    ("hypre_ilu_gmres", None, None),
+    # the fast path reached from a reference driver through the settings file
+    ("hypre_ilu_gmres", "precision,mixed\npreconditioner,multigrid\nguess,extrapolate\n"
+                        "inner_tolerance,1e-5\n", None),
+    # the body-force overload of single_timestep_2d
+    ("eigen_ilu_bicgstab", None, 2.0),
+    ("hypre_ilu_gmres", "precision,mixed\npreconditioner,multigrid\n", 2.0),
+]
+
+
+@pytest.mark.parametrize("solver_type,settings,gravity", CLASS_API_CASES)
+def test_class_api_driver_matches_oracle(tmp_path, solver_type, settings, gravity):
     """examples/lid_driven_cavity (PARAMETERS -> POINTS -> CLOUD ->
     FRACTIONAL_STEP_1 -> time loop) against the oracle on the same cloud, with
-    the CSV's solver_type selecting BiCGSTAB or GMRES(gmres_kdim)."""
+    the CSV's solver_type selecting BiCGSTAB or GMRES(gmres_kdim), GPU options
+    from a settings file (MEMPHYS_GPU_SETTINGS) and, with `gravity`, the
+    body-force overload (fractional_step_1.cpp:65-76)."""
     import os
     import subprocess
 
@@ -314,12 +328,22 @@ def test_class_api_driver_matches_oracle(tmp_path, solver_type):
     csv = tmp_path / "parameters.csv"
     text = open(os.path.join(root, "examples", "parameters_cavity.csv")).read()
     csv.write_text(text.replace("solver_type,eigen_ilu_bicgstab", f"solver_type,{solver_type}"))
+    env = dict(os.environ)
+    env.pop("MEMPHYS_GPU_SETTINGS", None)
+    if settings:
+        (tmp_path / "gpu.csv").write_text(settings)
+        env["MEMPHYS_GPU_SETTINGS"] = str(tmp_path / "gpu.csv")
     steps, dt, Re, n = 10, 1e-3, 100.0, 30
-    r = subprocess.run([exe, str(csv), "2", str(n), str(steps), str(dt), str(Re), out],
-                       capture_output=True, text=True, timeout=120)
+    cmd = [exe, str(csv), "2", str(n), str(steps), str(dt), str(Re), out]
+    if gravity is not None:
+        cmd.append(str(gravity))
+    _print("case: class API", solver_type, repr(settings), "gravity", gravity)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, env=env)
     _print(r.stdout.strip())
     assert r.returncode == 0, r.stdout + r.stderr
     assert f"solver {solver_type}" in r.stdout
+    if settings and "multigrid" in settings:
+        assert "precision 1 preconditioner 1" in r.stdout
     raw = open(out, "rb").read()
     nv, dim, nsys = np.frombuffer(raw[:12], dtype=np.int32)
     data = np.frombuffer(raw[12:], dtype=np.float64)
@@ -331,6 +355,8 @@ def test_class_api_driver_matches_oracle(tmp_path, solver_type):
     u_dir, p_dir, u0 = lid_bc(cloud, 1)
     fs = O.FractionalStep(cloud, w, u_dir, u_dir, None, p_dir, 1.0, 1.0 / Re, dt, 2, precond=2,
                           tol=1e-14, maxit=20000)
+    if gravity is not None:
+        fs.set_body_force(0.5 * gravity * np.sin(np.pi * cloud.xyz[:, 1]), np.full(nv, -gravity))
     uo, vo, po = u0.copy(), np.zeros(nv), np.zeros(fs.n_sys)
     for it in range(steps):
         fs.step(it, uo, vo, None, po)

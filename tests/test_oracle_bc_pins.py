@@ -172,6 +172,46 @@ def test_ab2_predictor_closed_form(dim):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
+def test_body_force_predictor_closed_form(dim):
+    """The body-force overload (fractional_step_1.cpp:65-76, 104-118): S gains
+    + f, so step 0 with fields U0 and force f0 stores S(U0) + f0, and the
+    predictor of step 1 with U1 and f1 gives
+    uh = U1 + dt/rho (1.5 (S(U1) + f1) - 0.5 (S(U0) + f0))."""
+    c = _cloud(dim)
+    w = O.stencil_weights(c, 3, 3)[:4]
+    nv, x = c.nv, c.xyz
+    rho, mu, dt = 1.3, 0.02, 1e-3
+    ones = np.ones(nv, dtype=np.uint8)
+    p_dir = np.zeros(nv, dtype=np.uint8)
+    fs = O.FractionalStep(c, w, ones, ones, ones if dim == 3 else None, p_dir, rho, mu, dt, 2)
+    U0, dU0, lap0 = _quadratic_fields(x, 0)
+    U1, dU1, lap1 = _quadratic_fields(x, 1)
+    S0 = _source(U0, dU0, lap0, rho, mu)
+    S1 = _source(U1, dU1, lap1, rho, mu)
+    f0 = [np.sin(2 * x[:, 0]) + q for q in range(dim)]
+    f1 = [np.cos(3 * x[:, 1]) - 2.0 * q for q in range(dim)]
+    fs.set_body_force(*f0)
+    work = [u.copy() for u in U0]
+    p = np.zeros(fs.n_sys)
+    fs.step(0, work[0], work[1], work[2] if dim == 3 else None, p)
+    fs.set_body_force(*f1)
+    uh, _ = fs.predict(1, U1[0], U1[1], U1[2] if dim == 3 else None, p)
+    ref = [U1[q] + dt / rho * (1.5 * (S1[q] + f1[q]) - 0.5 * (S0[q] + f0[q])) for q in range(dim)]
+    err = max(np.abs(uh[q] - ref[q]).max() for q in range(dim))
+    assert err <= 1e-9, err
+    # planted mistakes: the force left out, subtracted, or kept out of the history
+    for wrong in ((0.0, 0.0), (-1.0, -1.0), (1.0, 0.0)):
+        alt = [U1[q] + dt / rho * (1.5 * (S1[q] + wrong[0] * f1[q]) -
+                                   0.5 * (S0[q] + wrong[1] * f0[q])) for q in range(dim)]
+        assert max(np.abs(uh[q] - alt[q]).max() for q in range(dim)) > 1e3 * max(err, 1e-12)
+    # removed again: the plain predictor
+    fs.set_body_force()
+    uh0, _ = fs.predict(1, U1[0], U1[1], U1[2] if dim == 3 else None, p)
+    ref0 = [U1[q] + dt / rho * (1.5 * S1[q] - 0.5 * (S0[q] + f0[q])) for q in range(dim)]
+    assert max(np.abs(uh0[q] - ref0[q]).max() for q in range(dim)) <= 1e-9
+
+
+@pytest.mark.parametrize("dim", [2, 3])
 def test_rhs_dirichlet_pressure_rows(dim):
     """Pressure right-hand side with a Dirichlet-pressure face: b = p_old on
     its rows, -rho((u - uh).n)/dt on the Neumann boundary rows, rho/dt div(uh)
