@@ -125,13 +125,14 @@ __global__ __launch_bounds__(256) void weights_fp64_kernel(
     int dim, int phs, int m, const int* __restrict__ exps, SellView sv,
     const double* __restrict__ xyz, int npad, double* __restrict__ gx, double* __restrict__ gy,
     double* __restrict__ gz, double* __restrict__ lap, double* __restrict__ cond,
-    const int* __restrict__ rows) {
+    const int* __restrict__ rows, const int* __restrict__ only) {
     extern __shared__ double lds[];
     __shared__ double s_mn[3], s_sc[3];
     __shared__ double s_anorm, s_red[4];
     __shared__ int s_piv, s_jm;
     __shared__ int s_e[MAX_POLY * 3];
 
+    if (only && !only[blockIdx.x]) return;  // a stencil the register kernel finished
     const int r = rows ? rows[blockIdx.x] : (int)blockIdx.x;
     if (r >= sv.nv || sv.len[r] == 0) return;  // padding rows carry no stencil
     const int tid = threadIdx.x;
@@ -438,21 +439,22 @@ __device__ __forceinline__ double wave_max_d(double v) {
     return __hiloint2double(__builtin_amdgcn_readlane(hi, 63), __builtin_amdgcn_readlane(lo, 63));
 }
 
-// Minimum and maximum over the lanes that hold a point, combined in lane order
-// so that a later value replaces the current one only when strictly smaller
-// (larger): the result equals the left-to-right scan of the oracle.
+// Minimum and maximum over the lanes that hold a point, each equal to the
+// oracle's left-to-right scan (a later value replaces the current one only when
+// strictly smaller / larger): the extreme value by DPP reductions, then the
+// exact bits of the first lane (lowest index) holding an equal value, which is
+// the one the scan keeps among equal values such as -0 and +0.
 __device__ __forceinline__ void ordered_minmax(double& mn, double& mx, int& has, int lane) {
-    for (int off = 1; off < 64; off <<= 1) {
-        const double omn = __shfl_xor(mn, off), omx = __shfl_xor(mx, off);
-        const int ohas = __shfl_xor(has, off);
-        const bool other_first = (lane & off) != 0;
-        const double lmn = other_first ? omn : mn, hmn = other_first ? mn : omn;
-        const double lmx = other_first ? omx : mx, hmx = other_first ? mx : omx;
-        const int lh = other_first ? ohas : has, hh = other_first ? has : ohas;
-        mn = !lh ? hmn : (!hh ? lmn : ((hmn < lmn) ? hmn : lmn));
-        mx = !lh ? hmx : (!hh ? lmx : ((hmx > lmx) ? hmx : lmx));
-        has = lh | hh;
-    }
+    (void)lane;
+    const double big = __builtin_inf();
+    const double x = mn;
+    const double vmax = wave_max_d(has ? x : -big);
+    const double vmin = -wave_max_d(has ? -x : -big);
+    const unsigned long long bmax = __ballot(has && x == vmax);
+    const unsigned long long bmin = __ballot(has && x == vmin);
+    mx = bmax ? readlane_d(x, __ffsll((long long)bmax) - 1) : vmax;
+    mn = bmin ? readlane_d(x, __ffsll((long long)bmin) - 1) : vmin;
+    has = __ballot(has) != 0ull;
 }
 
 // LDS doubles the register-resident kernel needs for a stencil of M = k + m.
@@ -478,7 +480,8 @@ struct WaveLu {
     int* s_cn;
     int M, pos, lane, wv;
     bool valid;
-    int ub = 0;  // start of U row c
+    int ub = 0;         // start of U row c
+    bool zero_pivot = false;  // a single wavefront met an exactly zero pivot (stencil redone)
 
     // One wavefront (M <= 64): no LDS inside the factorisation.  The pivot is
     // found with a DPP maximum and a ballot, its row is broadcast with
@@ -505,20 +508,32 @@ struct WaveLu {
         if (pos == c) pos = piv;
         else if (pos == piv) pos = c;
         const int pl = __ffsll((long long)__ballot(pos == c)) - 1;  // the pivot row's lane
-        const double d = readlane_d(a[0], pl);
-        if (pos > c) {
-            if (d != 0.0) {
-                const double nl = -(a[0] / d);
+        // The pivot row reaches the other rows two ways at once: its first
+        // L+1 entries through LDS (one lane writes, every lane reads the same
+        // address), the rest by readlane, which balances the LDS and vector
+        // pipes (LDS alone or readlane alone each take about 1.5x longer).
+        constexpr int L = (3 * W / 4) & ~1;
+        if (pos == c) {
 #pragma unroll
-                for (int j = 0; j < W; ++j) a[j] = fma3(nl, readlane_d(a[j + 1], pl), a[j + 1]);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) r[q] = fma3(nl, readlane_d(r[q], pl), r[q]);
-            } else {
-                // a zero pivot: the oracle skips the column, the rows only shift
-#pragma unroll
-                for (int j = 0; j < W; ++j) a[j] = a[j + 1];
-            }
+            for (int j = 0; j <= L; ++j) U[ub + j] = a[j];
         }
+        __syncthreads();
+        const double* u = U + ub;
+        const double d = readlane_d(a[0], pl);
+        if (d == 0.0) {  // a singular column (the oracle skips it): left to the LDS kernel
+            zero_pivot = true;
+            return;
+        }
+        if (pos > c) {
+            const double nl = -(a[0] / d);
+#pragma unroll
+            for (int j = 0; j < L; ++j) a[j] = fma3(nl, u[j + 1], a[j + 1]);
+#pragma unroll
+            for (int j = L; j < W; ++j) a[j] = fma3(nl, readlane_d(a[j + 1], pl), a[j + 1]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) r[q] = fma3(nl, readlane_d(r[q], pl), r[q]);
+        }
+        ub += M - c;
     }
 
     // After the last step every row holds its U row (columns pos..M-1 in
@@ -595,7 +610,7 @@ struct WaveLu {
 
     template <int W>
     __device__ __forceinline__ void phase(int& c) {
-        for (; c < M && M - 1 - c > W - 8; ++c) step<W>(c);
+        for (; c < M && M - 1 - c > W - 8 && !zero_pivot; ++c) step<W>(c);
         if constexpr (W > 0) phase<W - 8>(c);
     }
 };
@@ -604,7 +619,8 @@ template <int NW, int AW>
 __global__ __launch_bounds__(64 * NW) void weights_wave_kernel(
     int dim, int phs, int m, const int* __restrict__ exps, SellView sv,
     const double* __restrict__ xyz, int npad, double* __restrict__ gx, double* __restrict__ gy,
-    double* __restrict__ gz, double* __restrict__ lap, const int* __restrict__ rows) {
+    double* __restrict__ gz, double* __restrict__ lap, const int* __restrict__ rows,
+    int* __restrict__ redo) {
     extern __shared__ double lds[];
     __shared__ double s_mm[NW][6];
     __shared__ int s_has[NW];
@@ -645,10 +661,12 @@ __global__ __launch_bounds__(64 * NW) void weights_wave_kernel(
     __syncthreads();
     if (tid < dim) {
         double mn = s_mm[0][2 * tid], mx = s_mm[0][2 * tid + 1];
-        if (NW == 2 && s_has[1]) {
-            const double mn1 = s_mm[1][2 * tid], mx1 = s_mm[1][2 * tid + 1];
-            if (mn > mn1) mn = mn1;
-            if (mx < mx1) mx = mx1;
+        if constexpr (NW == 2) {
+            if (s_has[1]) {
+                const double mn1 = s_mm[1][2 * tid], mx1 = s_mm[1][2 * tid + 1];
+                if (mn > mn1) mn = mn1;
+                if (mx < mx1) mx = mx1;
+            }
         }
         s_mn[tid] = mn;
         s_sc[tid] = mx - mn;
@@ -707,6 +725,10 @@ __global__ __launch_bounds__(64 * NW) void weights_wave_kernel(
     WaveLu<NW, AW> lu{a, rr, U, Y, s_zero, s_cv, s_cp, s_cn, M, valid ? i : (1 << 20), lane, wv, valid};
     int c = 0;
     lu.template phase<AW>(c);
+    if (lu.zero_pivot) {  // uniform: the whole wavefront saw the zero pivot
+        if (tid == 0) redo[blockIdx.x] = 1;
+        return;
+    }
     if (NW == 1) lu.store_u();
     __syncthreads();
 
@@ -1010,7 +1032,7 @@ void launch_weights_mixed(hipStream_t s, int dim, int phs, int m, const int* exp
 void launch_weights_fp64(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
                          const SellView& sv, const double* xyz_soa, int npad, double* gx,
                          double* gy, double* gz, double* lap, double* cond, int max_len,
-                         const int* rows, int nrows) {
+                         const int* rows, int nrows, const int* only) {
     if (rows && nrows == 0) return;
     const int M = max_len + m;
     const size_t lds = sizeof(double) * ((size_t)M * (M + 1) + (size_t)M * 4 + (size_t)max_len * 3);
@@ -1018,7 +1040,7 @@ void launch_weights_fp64(hipStream_t s, int dim, int phs, int m, const int* exps
         MP_HIP(hipFuncSetAttribute((const void*)weights_fp64_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(weights_fp64_kernel, dim3(rows ? nrows : sv.nv), dim3(256), lds, s, dim,
-                       phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, cond, rows);
+                       phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, cond, rows, only);
     MP_HIP(hipGetLastError());
 }
 
@@ -1026,14 +1048,14 @@ void launch_weights_fp64(hipStream_t s, int dim, int phs, int m, const int* exps
 template <int NW, int AW>
 void launch_wave(hipStream_t s, int dim, int phs, int m, const int* exps_dev, const SellView& sv,
                  const double* xyz_soa, int npad, double* gx, double* gy, double* gz, double* lap,
-                 const int* rows, int nrows) {
+                 const int* rows, int nrows, int* redo) {
     if (nrows == 0) return;
     const size_t lds = sizeof(double) * (size_t)wave_lds_doubles(AW, AW - m, m);
     if (lds > 64 * 1024)
         MP_HIP(hipFuncSetAttribute((const void*)weights_wave_kernel<NW, AW>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((weights_wave_kernel<NW, AW>), dim3(nrows), dim3(64 * NW), lds, s, dim,
-                       phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, rows);
+                       phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, rows, redo);
     MP_HIP(hipGetLastError());
 }
 
@@ -1060,15 +1082,31 @@ void launch_weights_fp64_rows(hipStream_t s, int dim, int phs, int m, const int*
     }
     off[WAVE_CLASSES + 1] = (int)all.size();
     if (all.empty()) return;
-    dev_rows.alloc(all.size());
-    MP_HIP(hipMemcpyAsync(dev_rows.p, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, s));
+    // the grouped row list, then one flag per row (set by a single-wavefront
+    // stencil that met an exactly zero pivot)
+    const size_t nall = all.size();
+    dev_rows.alloc(2 * nall);
+    MP_HIP(hipMemcpyAsync(dev_rows.p, all.data(), sizeof(int) * nall, hipMemcpyHostToDevice, s));
+    MP_HIP(hipMemsetAsync(dev_rows.p + nall, 0, sizeof(int) * nall, s));
     auto n = [&](int b) { return off[b + 1] - off[b]; };
     const int* d = dev_rows.p;
-    launch_wave<1, 32>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[0], n(0));
-    launch_wave<1, 48>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[1], n(1));
-    launch_wave<1, 64>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[2], n(2));
-    launch_wave<2, 80>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[3], n(3));
-    launch_wave<2, 96>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[4], n(4));
+    int* redo = dev_rows.p + nall;
+    launch_wave<1, 32>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[0],
+                       n(0), redo + off[0]);
+    launch_wave<1, 48>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[1],
+                       n(1), redo + off[1]);
+    launch_wave<1, 64>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[2],
+                       n(2), redo + off[2]);
+    launch_wave<2, 80>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[3],
+                       n(3), redo + off[3]);
+    launch_wave<2, 96>(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, d + off[4],
+                       n(4), redo + off[4]);
+    // single-wavefront stencils with a zero pivot: the LDS kernel, which
+    // follows the oracle through the singular column
+    const int n1 = off[3] - off[0];
+    if (n1 > 0)
+        launch_weights_fp64(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, nullptr,
+                            64 - m, d + off[0], n1, redo + off[0]);
     if (n(WAVE_CLASSES) > 0)
         launch_weights_fp64(s, dim, phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, nullptr,
                             max_long, d + off[WAVE_CLASSES], n(WAVE_CLASSES));

@@ -269,7 +269,7 @@ void launch_weights_fp64_rows(hipStream_t s, int dim, int phs, int m, const int*
 void launch_weights_fp64(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
                          const SellView& sv, const double* xyz_soa, int npad, double* gx,
                          double* gy, double* gz, double* lap, double* cond, int max_len,
-                         const int* rows = nullptr, int nrows = 0);
+                         const int* rows = nullptr, int nrows = 0, const int* only = nullptr);
 // FP32 LU + FP64 refinement for the listed rows; flag[r] = 1 where it did not converge.
 void launch_weights_mixed(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
                           const SellView& sv, const double* xyz_soa, int npad, double* gx,
