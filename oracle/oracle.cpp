@@ -484,9 +484,14 @@ double dot(const std::vector<double>& a, const std::vector<double>& b) {
 }
 
 // ILUT(droptol, fill) after Saad, "ILUT: a dual threshold incomplete LU
-// factorization" (1994), with the per-row fill limit fill*nnz(A)/n and the
-// zero-pivot replacement sqrt(eps)*rownorm used by Eigen's IncompleteLUT.
-// No fill-reducing permutation (the stencil set is already RCM ordered).
+// factorization" (1994), with the rules of Eigen 3.3.7's IncompleteLUT (the
+// reference's preconditioner, solver.cpp:66, 156): a multiplier is dropped when
+// |l_ik| <= droptol; an upper entry when |u_ij| <= droptol * |a_i|_2; each row
+// keeps at most fill_in / 2 multipliers and fill_in / 2 upper entries besides
+// the diagonal (the largest), fill_in = nnz(A) * fill / n + 1 (at most n); a
+// zero diagonal becomes sqrt(droptol) * |a_i|_2.  Eigen first orders the
+// unknowns by AMD on A + A^T; this restatement keeps the stencil set's (RCM)
+// order, so its factors are not Eigen's (parity unpinned).
 struct Ilut {
     int n = 0;
     std::vector<int> lrp, lci, urp, uci;  // L strictly lower (unit diag), U with diagonal first
@@ -495,7 +500,9 @@ struct Ilut {
     void compute(const Csr& A, double droptol, double fillfactor) {
         n = A.n;
         long nnz = (long)A.rp[n];
-        int fill = (int)((double)nnz * fillfactor / (double)n) + 1;
+        long fill_in = (long)((double)nnz * fillfactor / (double)n) + 1;
+        if (fill_in > n) fill_in = n;
+        const int fill = (int)(fill_in / 2);
         lrp.assign(1, 0);
         urp.assign(1, 0);
         lci.clear(); lv.clear(); uci.clear(); uv.clear();
@@ -525,7 +532,7 @@ struct Ilut {
                 int kcol = nzl.back();
                 nzl.pop_back();
                 double lik = w[kcol] / uv[urp[kcol]];
-                if (std::fabs(lik) < droptol * rn) {
+                if (std::fabs(lik) <= droptol) {
                     w[kcol] = 0.0;
                     in[kcol] = 0;
                     continue;
@@ -561,7 +568,7 @@ struct Ilut {
             std::vector<int> ukeep;
             for (int c : nzu) {
                 if (c == i) continue;
-                if (std::fabs(w[c]) >= droptol * rn) ukeep.push_back(c);
+                if (std::fabs(w[c]) > droptol * rn) ukeep.push_back(c);
                 else { w[c] = 0.0; in[c] = 0; }
             }
             if ((int)ukeep.size() > fill) {
@@ -571,7 +578,7 @@ struct Ilut {
                 ukeep.resize(fill);
             }
             std::sort(ukeep.begin(), ukeep.end());
-            if (diag == 0.0) diag = std::sqrt(std::numeric_limits<double>::epsilon()) * rn;
+            if (diag == 0.0) diag = std::sqrt(droptol) * rn;
             uci.push_back(i);
             uv.push_back(diag);
             w[i] = 0.0; in[i] = 0;
