@@ -126,7 +126,7 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
     else:
         interval = prm["check_every"]
     first = it
-    iters, res, dts, rej = [], [], [], 0
+    iters, res, dts, rej, skipped = [], [], [], 0, 0
     ctx.sync()
     t0 = time.perf_counter()
     tp = t0
@@ -139,6 +139,7 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
         iters.append(st.iterations)
         res.append(st.rel_res)
         rej += st.guess_rejected
+        skipped += st.guess_skipped
         if save_at is not None and it == save_at:
             saved = {f: ctx.get_field(f) for f in fields}
             saved["it"] = it + 1
@@ -159,7 +160,7 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
     return dict(seconds=float(sum(dts)), wall=t1 - t0, first=first, step_s=dts, iters=iters,
                 rel_res=res, spmv=spmv, predictor=pred, smooth=smooth, fields=out_fields,
                 divergence=div.tolist(), hierarchy=hier, create_s=t_create,
-                poll_interval=interval, saved=saved, guess_rejected=rej)
+                poll_interval=interval, saved=saved, guess_rejected=rej, guess_skipped=skipped)
 
 
 def window_seconds(form, lo, hi):
@@ -523,7 +524,9 @@ def main():
         t = time.perf_counter()
         episodes = 10
         guess = M.train_guess(small, prm["rho"], prm["mu"], prm["dt"], 1e-8, episodes,
-                              max(1, args.train_steps // episodes), seed=2024, log=log)
+                              max(1, args.train_steps // episodes), seed=2024, log=log,
+                              precision=M.MP_PREC_MIXED, precond=M.MP_PRECOND_AMG,
+                              krylov=M.MP_KRYLOV_GMRES, inner_tol=1e-5)
         t_train = time.perf_counter() - t
         log(f"guess training: {t_train:.1f}s on {small.nv} nodes")
     t = time.perf_counter()
@@ -698,7 +701,15 @@ def main():
             "copy_kernel_GBps": copy_gbps, "frac_of_copy": achieved / copy_gbps,
         },
         **ratios,
-        "headline": form_summary(head) | {"guess_rejected_steps": head["guess_rejected"]},
+        "headline": form_summary(head) | {
+            "guess": {"steps": len(head["step_s"]),
+                      "network_evaluated": len(head["step_s"]) - head["guess_skipped"],
+                      "rejected_for_extrapolation": head["guess_rejected"],
+                      "accepted": len(head["step_s"]) - head["guess_skipped"] -
+                                  head["guess_rejected"],
+                      "rule": "a guess whose FP64 residual is non-finite or above 10x its "
+                              "extrapolation's is replaced by the extrapolation; after each "
+                              "rejection the network is skipped for 1, 2, 4 ... 64 steps"}},
         "full_run_forms": {k: form_summary(f) for k, f in full.items()},
         "jacobi_forms_windows": win,
         "fields_headline_vs_fp64_multigrid_rel_diff": fields_diff,

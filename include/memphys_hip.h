@@ -86,8 +86,12 @@ typedef struct {
     int gmres_switches;   /* GMRES cycles that stalled (|r| reduced by < 10%) or broke down,
                              after which the system continued with BiCGSTAB */
     int guess_rejected;   /* 1 when the initial guess was replaced: a non-finite guess of a
-                             finite system starts from zero, a learned guess whose residual
-                             exceeds its linear extrapolation's starts from the extrapolation */
+                             finite system starts from zero, a learned guess whose residual is
+                             not finite or exceeds 10x its linear extrapolation's starts from
+                             the extrapolation */
+    int guess_skipped;    /* 1 when the learned guess was not evaluated: after a rejected
+                             guess the network is skipped for 1, 2, 4 ... 64 steps (the
+                             extrapolation alone); an accepted guess resets the back-off */
 } mp_step_stats;
 
 /* Context ------------------------------------------------------------------ */
@@ -223,6 +227,7 @@ int mp_guess_eval(mp_ctx* ctx, const double* p1, const double* p2, const double*
  * (postprocessing_functions.cpp:192-195, 248-251):
  * out = {max_int, mean_int, max_bnd, mean_bnd, max_all, mean_all}. */
 int mp_divergence(mp_ctx* ctx, double* out);
+
 
 /* Per-problem rho, mu, dt of a batch (after mp_fs_create; arrays of nprob);
  * replaces the scalars of mp_fs_params for the time steps. */

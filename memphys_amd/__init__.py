@@ -62,6 +62,7 @@ class StepStats(C.Structure):
         ("alpha", C.c_double), ("guess_rel_res", C.c_double),
         ("total_iterations", C.c_long), ("total_solves", C.c_long),
         ("gmres_switches", C.c_int), ("guess_rejected", C.c_int),
+        ("guess_skipped", C.c_int),
     ]
 
 
@@ -539,10 +540,12 @@ def lid_conditions(cloud: Cloud, lid_speed: float = 1.0):
 
 def train_guess(cloud: Cloud, rho: float, mu: float, dt: float, tol: float, episodes: int,
                 steps_per_episode: int, lid_range=(0.5, 1.5), seed: int = 1, lr: float = 1e-3,
-                device: int = 0, log=None) -> np.ndarray:
+                device: int = 0, log=None, **solver) -> np.ndarray:
     """Trains the learned pressure guess on solver-generated pairs: `episodes`
     cavity runs from rest with seeded random lid speeds in lid_range, one Adam
     step per pressure solve (forward/backward/Adam run in the HIP library).
+    `solver`: fs_create's solver options (precision, precond, krylov,
+    inner_tol); the targets, converged pressures, do not depend on them.
     Returns the packed FP32 parameters."""
     rng = np.random.default_rng(seed)
     with Context(device) as ctx:
@@ -550,7 +553,7 @@ def train_guess(cloud: Cloud, rho: float, mu: float, dt: float, tol: float, epis
         ctx.weights(3, 3, download=False)
         u_dir, p_dir, u0 = lid_conditions(cloud)
         w_dir = u_dir if cloud.dim == 3 else None
-        ctx.fs_create(rho, mu, dt, 2, tol, 20000, u_dir, u_dir, w_dir, p_dir)
+        ctx.fs_create(rho, mu, dt, 2, tol, 20000, u_dir, u_dir, w_dir, p_dir, **solver)
         ctx.guess_mode(MP_GUESS_LEARNED)
         ctx.guess_train(True, lr, seed)
         nv = cloud.nv

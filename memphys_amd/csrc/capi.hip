@@ -78,6 +78,9 @@ struct mp_ctx {
     int s_cur = 0;
     mp_step_stats stats{};
     int guess_mode = MP_GUESS_PREVIOUS;
+    // back-off of the learned guess: after a rejected guess the network is
+    // skipped for guess_backoff steps (1, 2, 4, ... 64), an accepted one resets it
+    int guess_backoff = 0, guess_skip = 0;
     DevBuf<double> div_partial;
     // pressure history and the learned guess
     DevBuf<double> P1, P2, Ptmp;
@@ -996,12 +999,15 @@ void fs_step_one(mp_ctx* c, int it) {
     c->timers[2].end(c->stream);
     const int n_sys = c->ppe->n_sys;  // single problem: border unknown at row nv
     const int nv = c->nv;
-    bool learned = false;
+    bool learned = false, skipped = false;
     if (c->guess_mode != MP_GUESS_PREVIOUS && c->hist >= 2) {
         hipLaunchKernelGGL(k_extrapolate, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, nv,
                            n_sys, c->P1.p, c->P2.p, c->P.p);
         MP_HIP(hipGetLastError());
-        if (c->guess_mode == MP_GUESS_LEARNED) {
+        if (c->guess_mode == MP_GUESS_LEARNED && !c->train && c->guess_skip > 0) {
+            --c->guess_skip;  // backing off: the extrapolation alone
+            skipped = true;
+        } else if (c->guess_mode == MP_GUESS_LEARNED) {
             const double* r = c->ppe->residual_of(c->stream, c->b.p, c->P.p);
             c->net->features(c->stream, c->P1.p, c->P2.p, r, c->ppe->diag_inv, c->xyz.p, c->npad,
                              c->dim);
@@ -1020,6 +1026,14 @@ void fs_step_one(mp_ctx* c, int it) {
         }
     }
     SolveStats st = c->ppe->solve(c->stream, c->b.p, c->P.p);
+    if (learned && !c->train) {
+        if (st.guess_rejected) {
+            c->guess_backoff = c->guess_backoff ? std::min(2 * c->guess_backoff, 64) : 1;
+            c->guess_skip = c->guess_backoff;
+        } else {
+            c->guess_backoff = 0;
+        }
+    }
     if (learned && c->train) {
         c->net->backward(c->stream, c->P1.p, c->P2.p, c->P.p);
         c->net->adam_step(c->stream);
@@ -1045,6 +1059,7 @@ void fs_step_one(mp_ctx* c, int it) {
     if (q.temporal_order == 2) c->s_cur ^= 1;
     c->cur = nw;
     put_stats(c->stats, st);
+    c->stats.guess_skipped = skipped ? 1 : 0;
     c->stats.total_iterations += st.iterations;
     c->stats.total_solves += 1;
     c->prob_total_iters.resize(c->nprob, 0);
@@ -1090,6 +1105,7 @@ int mp_guess_mode(mp_ctx* c, int mode) {
             if (!c->net) ensure_net(c);
         }
         c->guess_mode = mode;
+        c->guess_backoff = c->guess_skip = 0;
         return MP_OK;
     });
 }

@@ -147,18 +147,22 @@ def test_nan_network_falls_back_to_extrapolation(cube3d_small, precision, precon
             ctx.guess_mode(mode)
             if mode == M.MP_GUESS_LEARNED:
                 ctx.guess_params(nan_params)
-            rej = []
-            for it in range(6):
+            rej, skip = [], []
+            for it in range(8):
                 ctx.step(1, it)
                 st = ctx.stats()
                 rej.append(st.guess_rejected)
+                skip.append(st.guess_skipped)
                 assert st.converged == 1 and st.rel_res <= 1e-10
             fields = [ctx.get_field(f) for f in (M.MP_U, M.MP_V, M.MP_W, M.MP_P)]
             assert all(np.all(np.isfinite(f)) for f in fields)
-            out[mode] = (fields, rej)
-    _print("guess replaced per step", out[M.MP_GUESS_LEARNED][1])
+            out[mode] = (fields, rej, skip)
+    rej, skip = out[M.MP_GUESS_LEARNED][1], out[M.MP_GUESS_LEARNED][2]
+    _print("guess replaced per step", rej, "network skipped per step", skip)
     # steps 0 and 1 have no pressure history (previous pressure); from step 2
-    # on every NaN guess is replaced by the extrapolation
-    assert out[M.MP_GUESS_LEARNED][1][2:] == [1] * 4
+    # on every NaN guess is replaced by the extrapolation, and after each
+    # rejection the network is skipped for 1, 2, 4 ... steps (back-off)
+    assert rej[:2] == [0, 0] and skip[:2] == [0, 0]
+    assert rej[2:] == [1, 0, 1, 0, 0, 1] and skip[2:] == [0, 1, 0, 1, 1, 0]
     for a, b in zip(out[M.MP_GUESS_LEARNED][0], out[M.MP_GUESS_EXTRAPOLATE][0]):
         assert np.array_equal(a, b)
