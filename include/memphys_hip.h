@@ -228,6 +228,17 @@ int mp_guess_eval(mp_ctx* ctx, const double* p1, const double* p2, const double*
  * out = {max_int, mean_int, max_bnd, mean_bnd, max_all, mean_all}. */
 int mp_divergence(mp_ctx* ctx, double* out);
 
+/* mp_divergence without allocating (no reference counterpart; for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops).  out_dev: 6 doubles in DEVICE memory, written on the context's
+ * stream in the same order and with the same values as mp_divergence; the
+ * call does not wait for the stream.  The per-block partial sums live in a
+ * fixed device array of the library, one per GPU and shared by all contexts on
+ * it: clouds of more than MP_NOALLOC_MAX_ROWS device rows (nodes padded to
+ * whole blocks of 256 per problem) are refused with MP_ERR_ARG, and calls that
+ * overlap on one GPU (other threads or streams) race on that array. */
+#define MP_NOALLOC_MAX_ROWS 4194304
+int mp_divergence_noalloc(mp_ctx* ctx, double* out_dev);
 
 /* Per-problem rho, mu, dt of a batch (after mp_fs_create; arrays of nprob);
  * replaces the scalars of mp_fs_params for the time steps. */

@@ -38,6 +38,7 @@ HIP_SYMBOLS = (
     "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
     "mp_graph_mode", "mp_batch_upload", "mp_batch_params", "mp_batch_stats", "mp_autotune",
     "mp_operator_eigenvalue", "mp_knn_stencils", "mp_rcm_order", "mp_fs_body_force",
+    "mp_divergence_noalloc",
 )
 
 
@@ -104,6 +105,7 @@ def hip_lib() -> C.CDLL:
         lib.mp_stats_get.argtypes = [vp, C.POINTER(StepStats)]
         lib.mp_guess_mode.argtypes = [vp, C.c_int]
         lib.mp_divergence.argtypes = [vp, _dp]
+        lib.mp_divergence_noalloc.argtypes = [vp, vp]
         lib.mp_kernel_timing.argtypes = [vp, C.c_int]
         lib.mp_kernel_timing_get.argtypes = [vp, C.c_int, C.POINTER(C.c_long),
                                              C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -446,6 +448,12 @@ class Context:
         s = StepStats()
         self._check(self.lib.mp_stats_get(self.h, C.byref(s)), "mp_stats_get")
         return s
+
+    def divergence_noalloc(self, out_dev: int):
+        """mp_divergence_noalloc: the 6 statistics into device memory at out_dev
+        (an integer device address), stream-ordered, without allocating."""
+        self._check(self.lib.mp_divergence_noalloc(self.h, C.c_void_p(out_dev)),
+                    "mp_divergence_noalloc")
 
     def divergence(self) -> np.ndarray:
         out = np.zeros(6)

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <stdexcept>
@@ -258,6 +259,54 @@ void ensure_net(mp_ctx* c) {
 
 }  // namespace
 
+// Scratch of mp_divergence_noalloc: the per-row-block partial sums of the
+// continuity residual (4 doubles per block of 256 rows), so that entry point
+// allocates nothing.  One copy per GPU (each device holds its own instance of
+// a __device__ array), shared by every context on that GPU; its address on a
+// device is taken once, with the device current, when the first context on it
+// is created.  In host code the array's name is not a device address: only
+// the recorded pointer is handed to kernels.
+constexpr int NOALLOC_MAX_BLOCKS = MP_NOALLOC_MAX_ROWS / BLOCK;
+static __device__ __align__(256) double scratch[4 * NOALLOC_MAX_BLOCKS];
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double* scratch_ptr[MAX_DEVICES] = {};
+
+// Address of `scratch` on `device` (which must be current), recorded on first use.
+double* noalloc_scratch_address(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+        scratch_ptr[device] = static_cast<double*>(ptr);
+    }
+    return scratch_ptr[device];
+}
+
+// out[0..5] = {max_int, mean_int, max_bnd, mean_bnd, max_all, mean_all} from
+// the block partials, summed in block order by one thread (mp_divergence's
+// order, so both entry points return the same bits).
+__global__ void k_div_finish(const double* __restrict__ part, int g, int ni, int nb,
+                             double* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double mi = 0, si = 0, mb = 0, sb = 0;
+    for (int k = 0; k < g; ++k) {
+        mi = (mi < part[k * 4 + 0]) ? part[k * 4 + 0] : mi;
+        si += part[k * 4 + 1];
+        mb = (mb < part[k * 4 + 2]) ? part[k * 4 + 2] : mb;
+        sb += part[k * 4 + 3];
+    }
+    out[0] = mi;
+    out[1] = ni ? si / ni : 0.0;
+    out[2] = mb;
+    out[3] = nb ? sb / nb : 0.0;
+    out[4] = (mi < mb) ? mb : mi;
+    out[5] = (si + sb) / (ni + nb);
+}
+}  // namespace
+
 extern "C" {
 
 int mp_ctx_create(int device, mp_ctx** out) {
@@ -269,6 +318,11 @@ int mp_ctx_create(int device, mp_ctx** out) {
     mp_ctx* c = new mp_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return MP_ERR_HIP;
+    }
+    if (!noalloc_scratch_address(device)) {  // the device is current
+        (void)hipStreamDestroy(c->stream);
         delete c;
         return MP_ERR_HIP;
     }
@@ -1219,6 +1273,33 @@ int mp_divergence(mp_ctx* c, double* out) {
         out[3] = nb ? sb / nb : 0.0;
         out[4] = std::max(mi, mb);
         out[5] = (si + sb) / c->nv;
+        return MP_OK;
+    });
+}
+
+int mp_divergence_noalloc(mp_ctx* c, double* out_dev) {
+    return guarded(c, "mp_divergence_noalloc", [&]() -> int {
+        if (!c->has_fs)
+            return fail(c, MP_ERR_STATE, "mp_divergence_noalloc: mp_fs_create not called");
+        if (!out_dev) return fail(c, MP_ERR_ARG, "mp_divergence_noalloc: NULL");
+        const int g = c->npad / BLOCK;
+        if (c->npad > MP_NOALLOC_MAX_ROWS)
+            return fail(c, MP_ERR_ARG,
+                        "mp_divergence_noalloc: " + std::to_string(c->npad) +
+                            " device rows exceed the scratch limit of " +
+                            std::to_string(MP_NOALLOC_MAX_ROWS));
+        double* part = scratch_ptr[c->device];  // recorded at mp_ctx_create
+        if (!part) return fail(c, MP_ERR_STATE, "mp_divergence_noalloc: no scratch address");
+        DevBuf<double>(&U)[3] = c->U[c->cur];
+        if (c->dim == 2)
+            launch_divergence<2>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, U[0].p, U[1].p,
+                                 U[2].p, c->bnd.p, part, g);
+        else
+            launch_divergence<3>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, U[0].p, U[1].p,
+                                 U[2].p, c->bnd.p, part, g);
+        hipLaunchKernelGGL(k_div_finish, dim3(1), dim3(64), 0, c->stream, part, g, c->nv - c->nb,
+                           c->nb, out_dev);
+        MP_HIP(hipGetLastError());
         return MP_OK;
     });
 }
