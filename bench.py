@@ -153,14 +153,31 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
     ctx.step(timing_steps, it)
     spmv = ctx.kernel_timing_get(0)
     pred = ctx.kernel_timing_get(1)
+    rhs = ctx.kernel_timing_get(2)
+    corr = ctx.kernel_timing_get(3)
     smooth = ctx.kernel_timing_get(5) if precond == M.MP_PRECOND_AMG else None
     ctx.kernel_timing(False)
     log(f"[{label}] steps {first}-{first + steps - 1}: {sum(dts):.3f}s, iters/step "
         f"{np.mean(iters):.1f}, max rel_res {max(res):.2e}, spmv {spmv}")
     return dict(seconds=float(sum(dts)), wall=t1 - t0, first=first, step_s=dts, iters=iters,
-                rel_res=res, spmv=spmv, predictor=pred, smooth=smooth, fields=out_fields,
+                rel_res=res, spmv=spmv, predictor=pred, rhs=rhs, corrector=corr, smooth=smooth, fields=out_fields,
                 divergence=div.tolist(), hierarchy=hier, create_s=t_create,
                 poll_interval=interval, saved=saved, guess_rejected=rej, guess_skipped=skipped)
+
+
+def step_kernels(form, copy_gbps):
+    """Predictor (with its packing of u, v, w), right-hand side and corrector of
+    the form's timing steps: mean ms per launch (HIP events) and the
+    algorithmic bytes per launch over that time, against the 8 TB/s peak."""
+    out = {}
+    for name in ("predictor", "rhs", "corrector"):
+        t = form.get(name)
+        if not t or t[0] == 0:
+            continue
+        gbps = t[2] / (t[1] * 1e-3) / 1e9
+        out[name] = {"mean_ms": t[1], "bytes_per_launch": t[2], "achieved_GBps": gbps,
+                     "frac": gbps / 8000.0, "frac_of_copy": gbps / copy_gbps}
+    return out
 
 
 def window_seconds(form, lo, hi):
@@ -723,6 +740,7 @@ def main():
                                "launches_returned_early": all64 - n64,
                                "traffic": measured_traffic(args.n, 1)},
         "amg_level0_roofline": smooth_roofline(head.get("smooth"), copy_gbps, args.n),
+        "step_kernels": step_kernels(head, copy_gbps),
         "cloud_setup": setup,
         "weights_kernel_ms": wk[1],
         "weights_fp64_vs_fp32_refined": weights_check,

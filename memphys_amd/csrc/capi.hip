@@ -74,6 +74,7 @@ struct mp_ctx {
     DevBuf<uint8_t> udir[3], pdir;
     std::unique_ptr<PpeSolver> ppe;
     DevBuf<double> U[2][3];
+    DevBuf<double> upk, uhpk;  // velocity before / after the predictor, interleaved per node
     int cur = 0;
     DevBuf<double> P, b, uh[3], S[2];
     int s_cur = 0;
@@ -936,6 +937,8 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
         c->hist = 0;
         if (c->net && c->nprob == 1) ensure_net(c);
         for (int q = 0; q < 3; ++q) alloc_vec(c, c->uh[q]);
+        c->upk.alloc((size_t)4 * c->npad);
+        c->uhpk.alloc((size_t)4 * c->npad);
         c->has_force = false;
         for (int k = 0; k < 2; ++k) {
             c->S[k].alloc((size_t)3 * c->npad + 64);
@@ -1042,13 +1045,13 @@ void fs_step_one(mp_ctx* c, int it) {
     const double* pp = c->per_problem_prm ? c->prob_prm.p : nullptr;
     launch_predictor<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p, Uo[0].p,
                           Uo[1].p, Uo[2].p, c->uh[0].p, c->uh[1].p, c->uh[2].p, S_new, S_oo,
-                          c->npad, q.rho, q.mu, q.dt, euler, bp, pp,
+                          c->npad, q.rho, q.mu, q.dt, euler, c->upk.p, c->uhpk.p, bp, pp,
                           c->has_force ? c->force.p : nullptr);
     c->timers[1].end(c->stream);
     // calc_pressure
     c->timers[2].begin(c->stream);
     launch_ppe_rhs<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->uh[0].p, c->uh[1].p,
-                        c->uh[2].p, Uo[0].p, Uo[1].p, Uo[2].p, c->P.p, c->bnd.p, c->pdir.p,
+                        c->uh[2].p, c->uhpk.p, Uo[0].p, Uo[1].p, Uo[2].p, c->P.p, c->bnd.p, c->pdir.p,
                         c->normal.p, c->npad, q.rho, q.dt, c->b.p, bp, pp);
     c->timers[2].end(c->stream);
     const int n_sys = c->ppe->n_sys;  // single problem: border unknown at row nv

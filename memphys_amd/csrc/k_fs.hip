@@ -30,12 +30,41 @@ __device__ __forceinline__ d4 vals4(const double* a, int s4) {
     return *reinterpret_cast<const d4*>(a + s4);
 }
 
+// Velocity components of one node, interleaved (u, v[, w, 0]): PACK doubles per
+// node, so a neighbour's components arrive in one 16- or 32-byte load (one
+// cache line) instead of one load from each component array.
+template <int DIM>
+constexpr int PACK = DIM == 3 ? 4 : 2;
+typedef double d2 __attribute__((ext_vector_type(2)));
+template <int DIM>
+__device__ __forceinline__ d4 packed_at(const double* __restrict__ a, int c) {
+    if constexpr (DIM == 3) {
+        return *reinterpret_cast<const d4*>(a + 4 * (size_t)c);
+    } else {
+        const d2 t = *reinterpret_cast<const d2*>(a + 2 * (size_t)c);
+        return d4{t[0], t[1], 0.0, 0.0};
+    }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void pack_kernel(int n, const double* __restrict__ u,
+                                                     const double* __restrict__ v,
+                                                     const double* __restrict__ w,
+                                                     double* __restrict__ out) {
+    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r >= n) return;
+    if constexpr (DIM == 3)
+        *reinterpret_cast<d4*>(out + 4 * (size_t)r) = d4{u[r], v[r], w[r], 0.0};
+    else
+        *reinterpret_cast<d2*>(out + 2 * (size_t)r) = d2{u[r], v[r]};
+}
+
 template <int DIM>
 __global__ __launch_bounds__(BLOCK) void predictor_kernel(
     SellView sv, const double* __restrict__ gx, const double* __restrict__ gy,
-    const double* __restrict__ gz, const double* __restrict__ lap, const double* __restrict__ u,
-    const double* __restrict__ v, const double* __restrict__ w, double* __restrict__ uh,
-    double* __restrict__ vh, double* __restrict__ wh, double* __restrict__ S_new,
+    const double* __restrict__ gz, const double* __restrict__ lap,
+    const double* __restrict__ upk, double* __restrict__ uh, double* __restrict__ vh,
+    double* __restrict__ wh, double* __restrict__ uhpk, double* __restrict__ S_new,
     const double* __restrict__ S_oo, int npad, double rho_all, double mu_all, double dt_all,
     int euler, const int* __restrict__ bprob, const double* __restrict__ prm,
     const double* __restrict__ force) {
@@ -63,11 +92,7 @@ __global__ __launch_bounds__(BLOCK) void predictor_kernel(
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             if (j0 + e >= len) break;
-            const int c = c4[e];
-            double f[DIM];
-            f[0] = u[c];
-            f[1] = v[c];
-            if (DIM == 3) f[DIM - 1] = w[c];
+            const d4 f = packed_at<DIM>(upk, c4[e]);
 #pragma unroll
             for (int q = 0; q < DIM; ++q) {
                 acc[q][0] = fma(A0[e], f[q], acc[q][0]);
@@ -77,15 +102,13 @@ __global__ __launch_bounds__(BLOCK) void predictor_kernel(
             }
         }
     }
-    double own[DIM];
-    own[0] = u[r];
-    own[1] = v[r];
-    if (DIM == 3) own[DIM - 1] = w[r];
+    const d4 own = packed_at<DIM>(upk, r);
     double* out[3] = {uh, vh, wh};
+    d4 hat = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < DIM; ++q) {
         double conv = -(own[0] * acc[q][0]) - own[1] * acc[q][1];
-        if (DIM == 3) conv = conv - own[DIM - 1] * acc[q][2];
+        if (DIM == 3) conv = conv - own[2] * acc[q][2];
         double S = rho * conv + mu * acc[q][3];
         // body force (fractional_step_1.cpp:104-118): added after the
         // convective and diffusive terms, so it enters the AB2 history too
@@ -97,15 +120,20 @@ __global__ __launch_bounds__(BLOCK) void predictor_kernel(
         else
             res = own[q] + (dt / rho) * (1.5 * S - 0.5 * S_oo[(size_t)q * npad + r]);
         out[q][r] = res;
+        hat[q] = res;
     }
+    if constexpr (DIM == 3)
+        *reinterpret_cast<d4*>(uhpk + 4 * (size_t)r) = hat;
+    else
+        *reinterpret_cast<d2*>(uhpk + 2 * (size_t)r) = d2{hat[0], hat[1]};
 }
 
 template <int DIM>
 __global__ __launch_bounds__(BLOCK) void ppe_rhs_kernel(
     SellView sv, const double* __restrict__ gx, const double* __restrict__ gy,
     const double* __restrict__ gz, const double* __restrict__ uh, const double* __restrict__ vh,
-    const double* __restrict__ wh, const double* __restrict__ u, const double* __restrict__ v,
-    const double* __restrict__ w, const double* __restrict__ p_old,
+    const double* __restrict__ wh, const double* __restrict__ uhpk, const double* __restrict__ u,
+    const double* __restrict__ v, const double* __restrict__ w, const double* __restrict__ p_old,
     const uint8_t* __restrict__ bnd, const uint8_t* __restrict__ pdir,
     const double* __restrict__ nrm, int npad, double rho_all, double dt_all,
     double* __restrict__ b, const int* __restrict__ bprob, const double* __restrict__ prm) {
@@ -138,10 +166,10 @@ __global__ __launch_bounds__(BLOCK) void ppe_rhs_kernel(
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 if (j0 + e >= len) break;
-                const int c = c4[e];
-                ax = fma(X[e], uh[c], ax);
-                ay = fma(Y[e], vh[c], ay);
-                if (DIM == 3) az = fma(Z[e], wh[c], az);
+                const d4 f = packed_at<DIM>(uhpk, c4[e]);
+                ax = fma(X[e], f[0], ax);
+                ay = fma(Y[e], f[1], ay);
+                if (DIM == 3) az = fma(Z[e], f[2], az);
             }
         }
         out = ax + ay;
@@ -374,22 +402,25 @@ void launch_predictor(hipStream_t s, const SellView& sv, const double* gx, const
                       const double* gz, const double* lap, const double* u, const double* v,
                       const double* w, double* uh, double* vh, double* wh, double* S_new,
                       const double* S_oo, int npad, double rho, double mu, double dt, int euler,
-                      const int* bprob, const double* prm, const double* force) {
+                      double* upk, double* uhpk, const int* bprob, const double* prm,
+                      const double* force) {
     const int g = (sv.nv + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL(predictor_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, lap, u,
-                       v, w, uh, vh, wh, S_new, S_oo, npad, rho, mu, dt, euler, bprob, prm, force);
+    hipLaunchKernelGGL(pack_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv.nv, u, v, w, upk);
+    hipLaunchKernelGGL(predictor_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, lap, upk,
+                       uh, vh, wh, uhpk, S_new, S_oo, npad, rho, mu, dt, euler, bprob, prm, force);
     MP_HIP(hipGetLastError());
 }
 
 template <int DIM>
 void launch_ppe_rhs(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
                     const double* gz, const double* uh, const double* vh, const double* wh,
-                    const double* u, const double* v, const double* w, const double* p_old,
-                    const uint8_t* bnd, const uint8_t* pdir, const double* normal_soa, int npad,
-                    double rho, double dt, double* b, const int* bprob, const double* prm) {
+                    const double* uhpk, const double* u, const double* v, const double* w,
+                    const double* p_old, const uint8_t* bnd, const uint8_t* pdir,
+                    const double* normal_soa, int npad, double rho, double dt, double* b,
+                    const int* bprob, const double* prm) {
     const int g = (sv.nv + BLOCK - 1) / BLOCK;
     hipLaunchKernelGGL(ppe_rhs_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, uh, vh, wh,
-                       u, v, w, p_old, bnd, pdir, normal_soa, npad, rho, dt, b, bprob, prm);
+                       uhpk, u, v, w, p_old, bnd, pdir, normal_soa, npad, rho, dt, b, bprob, prm);
     MP_HIP(hipGetLastError());
 }
 
@@ -452,11 +483,13 @@ void launch_build_operator(hipStream_t s, const SellView& sv, int dim, const dou
                                       const double*, const double*, const double*, const double*, \
                                       const double*, double*, double*, double*, double*,        \
                                       const double*, int, double, double, double, int,          \
-                                      const int*, const double*, const double*);                \
+                                      double*, double*, const int*, const double*,              \
+                                      const double*);                                           \
     template void launch_ppe_rhs<D>(hipStream_t, const SellView&, const double*, const double*,   \
                                     const double*, const double*, const double*, const double*,   \
                                     const double*, const double*, const double*, const double*,   \
-                                    const uint8_t*, const uint8_t*, const double*, int, double,   \
+                                    const double*, const uint8_t*, const uint8_t*, const double*, \
+                                    int, double,                                                  \
                                     double, double*, const int*, const double*);                  \
     template void launch_corrector<D>(hipStream_t, const SellView&, const double*, const double*, \
                                       const double*, const double*, const double*, const double*, \

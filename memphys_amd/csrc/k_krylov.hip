@@ -1022,6 +1022,7 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
                       const uint8_t* dir_dev, double unsteady, double conv, double diff,
                       int precond_) {
     release_graphs();
+    gm_cols = 0;  // (the GMRES work arrays are sized for the rows set here)
     sv = view;
     nrows = view.nv;
     dim = dim_;
@@ -1259,16 +1260,26 @@ void PpeSolver::run_chunk(hipStream_t st, int chunk, const TM* a, const T* di, T
         launch_chunk<TM, T>(st, chunk, a, di, x, rr, rr0, pp, vv, ss, tt);
         return;
     }
-    const void* key[9] = {a, di, x, rr, rr0, pp, vv, ss, tt};
+    const uintptr_t key[ChunkGraph::KEY] = {
+        1, sizeof(TM) * 16 + sizeof(T), (uintptr_t)chunk, (uintptr_t)a, (uintptr_t)di,
+        (uintptr_t)x, (uintptr_t)rr, (uintptr_t)rr0, (uintptr_t)pp, (uintptr_t)vv,
+        (uintptr_t)ss, (uintptr_t)tt};
+    replay_or_capture(st, key,
+                      [&]() { launch_chunk<TM, T>(st, chunk, a, di, x, rr, rr0, pp, vv, ss, tt); });
+}
+
+template <typename F>
+void PpeSolver::replay_or_capture(hipStream_t st, const uintptr_t (&key)[ChunkGraph::KEY],
+                                  F&& launch) {
     for (ChunkGraph& g : graphs)
-        if (g.chunk == chunk && std::memcmp(g.key, key, sizeof(key)) == 0) {
+        if (std::memcmp(g.key, key, sizeof(key)) == 0) {
             MP_HIP(hipGraphLaunch(g.exec, st));
             return;
         }
     hipGraph_t graph = nullptr;
     MP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     try {
-        launch_chunk<TM, T>(st, chunk, a, di, x, rr, rr0, pp, vv, ss, tt);
+        launch();
     } catch (...) {
         (void)hipStreamEndCapture(st, &graph);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1277,12 +1288,27 @@ void PpeSolver::run_chunk(hipStream_t st, int chunk, const TM* a, const T* di, T
     MP_HIP(hipStreamEndCapture(st, &graph));
     ChunkGraph g{};
     std::memcpy(g.key, key, sizeof(key));
-    g.chunk = chunk;
     const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     MP_HIP(e);
     graphs.push_back(g);
     MP_HIP(hipGraphLaunch(g.exec, st));
+}
+
+template <typename TM, typename TV>
+void PpeSolver::run_gmres_chunk(hipStream_t st, int j, int c, int m, const TM* a,
+                                const double* di) {
+    if (!use_graphs || spmv_timer.enabled || smooth_timer.enabled) {  // (events stay out of graphs)
+        for (int k = j; k < j + c; ++k) gmres_iteration<TM, TV>(st, k, m, a, di);
+        return;
+    }
+    const uintptr_t key[ChunkGraph::KEY] = {
+        2, sizeof(TM) * 16 + sizeof(TV), (uintptr_t)j, (uintptr_t)c, (uintptr_t)m,
+        (uintptr_t)a, (uintptr_t)di, (uintptr_t)gm_v.p, (uintptr_t)gm_d.p,
+        (uintptr_t)gm_partial.p, (uintptr_t)gm_w.p, (uintptr_t)amg.get()};
+    replay_or_capture(st, key, [&]() {
+        for (int k = j; k < j + c; ++k) gmres_iteration<TM, TV>(st, k, m, a, di);
+    });
 }
 
 void PpeSolver::release_graphs() {
@@ -1393,6 +1419,7 @@ double* PpeSolver::residual_of(hipStream_t st, const double* b_dev, const double
 
 void PpeSolver::gmres_alloc(hipStream_t st, int m) {
     if (gm_cols == m && gm_d.p) return;
+    release_graphs();  // (captured chunks hold the buffers replaced here)
     const size_t ld = (size_t)nrows;
     gm_v.alloc(ld * (m + 1));
     MP_HIP(hipMemsetAsync(gm_v.p, 0, sizeof(double) * ld * (m + 1), st));
@@ -1498,7 +1525,7 @@ void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const d
         bool have_prev = false;
         for (int j = 0; j < m;) {
             const int c = std::min(chunk, m - j);
-            for (int k = j; k < j + c; ++k) gmres_iteration<TM, TV>(st, k, m, a, di);
+            run_gmres_chunk<TM, TV>(st, j, c, m, a, di);
             j += c;
             MP_HIP(hipMemcpyAsync(hf[slot], gmf_d.p, sizeof(GmFlags) * nseg,
                                   hipMemcpyDeviceToHost, st));

@@ -53,11 +53,14 @@ struct GmFlags {
     int done, full, iter, cycles;
 };
 
-// An instantiated graph of one chunk of BiCGSTAB iterations for one set of
-// operand pointers.
+// An instantiated graph of one chunk of Krylov iterations.  The key holds the
+// method, the value types, the chunk's position and length and every operand
+// pointer passed in; the solver's own buffers that the captured kernels also
+// read (scalars, partial sums, basis, V-cycle levels) are allocated before any
+// capture and released together with the graphs.
 struct ChunkGraph {
-    const void* key[9];
-    int chunk;
+    static constexpr int KEY = 12;
+    uintptr_t key[KEY];
     hipGraphExec_t exec;
 };
 
@@ -185,6 +188,14 @@ struct PpeSolver {
     template <typename TM, typename T>
     void run_chunk(hipStream_t s, int chunk, const TM* a, const T* di, T* x, T* rr, T* rr0,
                    T* pp, T* vv, T* ss, T* tt);
+    // Iterations j .. j+c-1 of a GMRES cycle: replayed from a cached graph
+    // when graphs are on, else launched.
+    template <typename TM, typename TV>
+    void run_gmres_chunk(hipStream_t s, int j, int c, int m, const TM* a, const double* di);
+    // Replays the graph cached under key, or captures launch() on the stream,
+    // instantiates it, caches it and replays it.
+    template <typename F>
+    void replay_or_capture(hipStream_t s, const uintptr_t (&key)[ChunkGraph::KEY], F&& launch);
     void zero_system(hipStream_t s, double* x, int q);
 };
 

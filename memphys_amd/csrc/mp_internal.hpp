@@ -281,12 +281,17 @@ void launch_predictor(hipStream_t s, const SellView& sv, const double* gx, const
                       const double* gz, const double* lap, const double* u, const double* v,
                       const double* w, double* uh, double* vh, double* wh, double* S_new,
                       const double* S_oo, int npad, double rho, double mu, double dt, int euler,
-                      const int* bprob = nullptr, const double* prm = nullptr,
-                      const double* force = nullptr);
+                      double* upk, double* uhpk, const int* bprob = nullptr,
+                      const double* prm = nullptr, const double* force = nullptr);
+// upk, uhpk: the velocity before and after the predictor interleaved per node
+// (2 doubles per node in 2D, 4 in 3D: u, v, w, 0); the predictor packs u, v, w
+// into upk for its gathers and writes uhpk beside uh, vh, wh for the
+// right-hand side's gathers.
 template <int DIM>
 void launch_ppe_rhs(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
                     const double* gz, const double* uh, const double* vh, const double* wh,
-                    const double* u, const double* v, const double* w, const double* p_old,
+                    const double* uhpk, const double* u, const double* v, const double* w,
+                    const double* p_old,
                     const uint8_t* bnd, const uint8_t* pdir, const double* normal_soa,
                     int npad, double rho, double dt, double* b, const int* bprob = nullptr,
                     const double* prm = nullptr);

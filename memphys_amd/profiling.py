@@ -12,6 +12,11 @@
         `product` launch (default k_spmv_alpha; k_gm_spmv for GMRES); their
         median wall and busy time, and the kernel list of the iteration of
         median wall time
+    python -m memphys_amd.profiling steps DB [--marker K] [--last N]
+        per timestep of the largest system (stretches between launches of
+        `marker`, default predictor_kernel): wall, busy and idle time, launches
+        and kernel time per category (predictor, right-hand side, corrector,
+        learned guess, Krylov, multigrid finest and coarse levels)
     python -m memphys_amd.profiling pmc DB [--top N]
         per kernel, grid size and counter: dispatches and mean value per dispatch
     python -m memphys_amd.profiling traffic FETCH_DB WRITE_DB --kernel SUBSTR
@@ -105,6 +110,68 @@ def iteration(db: str, first: str = "k_update_p<double>", marker: str = "k_amg_o
     return "\n".join(lines)
 
 
+_CATEGORIES = (  # (label, name substrings), first match wins
+    ("predictor", ("predictor_kernel",)),
+    ("ppe rhs", ("ppe_rhs_kernel",)),
+    ("corrector + boundary", ("corrector_kernel", "boundary_fix_kernel")),
+    ("learned guess", ("gnn", "k_guess")),
+    ("gmres", ("k_gm_",)),
+    ("bicgstab", ("k_update_", "k_spmv_alpha", "k_spmv_omega", "k_finish_", "k_residual_init",
+                  "k_sc_reset")),
+)
+
+
+def steps(db: str, marker: str = "predictor_kernel", last: int = 0) -> str:
+    """Per timestep of the largest system: the stretches between consecutive
+    launches of `marker` at its largest grid.  Wall time, busy time (union of
+    kernel intervals), launches, and kernel time per category; multigrid
+    launches split by grid into the finest level and the coarse levels."""
+    cur = sqlite3.connect(db).cursor()
+    rows = cur.execute("select name, start, end, grid_x from kernels order by start").fetchall()
+    grid = max((r[3] for r in rows if marker in r[0]), default=None)
+    if grid is None:
+        raise SystemExit(f"no launch of '{marker}' in {db}")
+    heads = [i for i, r in enumerate(rows) if marker in r[0] and r[3] == grid]
+    amg_grid = max((r[3] for r in rows if "k_amg" in r[0]), default=0)
+    wins = list(zip(heads[:-1], heads[1:]))
+    if last:
+        wins = wins[-last:]
+    cats = [c for c, _ in _CATEGORIES] + ["multigrid finest", "multigrid coarse", "other"]
+    lines = [f"timesteps {len(wins)} (marker {marker}, grid {grid}); times in ms"]
+    lines.append(f"{'step':>4s} {'wall':>7s} {'busy':>7s} {'idle':>6s} {'launch':>6s} " +
+                 " ".join(f"{c[:12]:>12s}" for c in cats))
+    other_names = {}
+    for n, (a, b) in enumerate(wins):
+        seg = rows[a:b]
+        per = dict.fromkeys(cats, 0.0)
+        busy, cur_s, cur_e = 0, None, None
+        for name, s, e, g in seg:
+            if cur_e is None or s > cur_e:
+                if cur_e is not None:
+                    busy += cur_e - cur_s
+                cur_s, cur_e = s, e
+            else:
+                cur_e = max(cur_e, e)
+            for c, keys in _CATEGORIES:
+                if any(k in name for k in keys):
+                    per[c] += e - s
+                    break
+            else:
+                if "k_amg" in name:
+                    per["multigrid finest" if g >= amg_grid // 2 else "multigrid coarse"] += e - s
+                else:
+                    per["other"] += e - s
+                    k = _short(name)
+                    other_names[k] = other_names.get(k, 0) + e - s
+        busy += cur_e - cur_s
+        wall = rows[b][1] - seg[0][1]
+        lines.append(f"{n:4d} {wall / 1e6:7.2f} {busy / 1e6:7.2f} {(wall - busy) / 1e6:6.2f} "
+                     f"{len(seg):6d} " + " ".join(f"{per[c] / 1e6:12.3f}" for c in cats))
+    top = sorted(other_names.items(), key=lambda kv: -kv[1])[:8]
+    lines.append("other: " + ", ".join(f"{k} {v / 1e6:.3f}" for k, v in top))
+    return "\n".join(lines)
+
+
 def _pmc_rows(db: str):
     cur = sqlite3.connect(db).cursor()
     return cur.execute(
@@ -180,6 +247,10 @@ def main(argv=None):
     it.add_argument("--first", default="k_update_p<double>")
     it.add_argument("--marker", default="k_amg_op")
     it.add_argument("--product", default="k_spmv_alpha")
+    st = sub.add_parser("steps")
+    st.add_argument("db")
+    st.add_argument("--marker", default="predictor_kernel")
+    st.add_argument("--last", type=int, default=0)
     b = sub.add_parser("pmc")
     b.add_argument("db")
     b.add_argument("--top", type=int, default=40)
@@ -195,6 +266,8 @@ def main(argv=None):
         print(stats(args.db, args.top))
     elif args.cmd == "iteration":
         print(iteration(args.db, args.first, args.marker, args.product))
+    elif args.cmd == "steps":
+        print(steps(args.db, args.marker, args.last))
     elif args.cmd == "pmc":
         print(pmc(args.db, args.top))
     else:
