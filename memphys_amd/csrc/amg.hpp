@@ -84,9 +84,19 @@ struct AmgDevice {
     // values; the border unknown passes through only in out).
     void apply(hipStream_t s, const double* in, double* out, int bordered, const int* done_flag,
                bool prescaled = false, float* out32 = nullptr);
+    // out = M^-1 in on the node block for a Krylov basis vector (FP32 or
+    // FP64 values; the border unknown is not touched), written in FP32; the
+    // smoother input w D^-1 in is already in prescaled_input().
+    template <typename TI>
+    void apply_basis(hipStream_t s, const TI* in, float* out, const int* done_flag);
     KernelTimer* post_timer = nullptr;  // times the level-0 post-smoothing product when set
     float* prescaled_input() { return lv.size() > 1 ? lv[0].x.p : nullptr; }
     const float* level0_dinv() const { return lv.empty() ? nullptr : lv[0].dinv.p; }
+
+  private:
+    template <typename TI, typename TO>
+    void vcycle(hipStream_t s, const TI* in, TO* out, const double* border_in, double* border_out,
+                const int* done_flag, bool prescaled, float* out32);
 };
 
 }  // namespace mp

@@ -633,30 +633,48 @@ void AmgDevice::retune(hipStream_t st) {
 
 void AmgDevice::apply(hipStream_t st, const double* in, double* out, int bordered,
                       const int* done, bool prescaled, float* out32) {
-    const int L = (int)lv.size();
     const int nv = lv[0].n;
     // one problem: its border unknown (row nv) passes through in the kernels;
     // a batch copies every border unknown at the end
     const double* bin = (bordered && !blocked) ? in + nv : nullptr;
     double* bout = (bordered && !blocked) ? out + nv : nullptr;
+    vcycle(st, in, out, bin, bout, done, prescaled, out32);
+    if (bordered && blocked && border_rows.n > 0)
+        hipLaunchKernelGGL(k_border_copy, dim3((border_rows.n + 63) / 64), dim3(64), 0, st,
+                           (int)border_rows.n, border_rows.p, in, out);
+    MP_HIP(hipGetLastError());
+}
+
+template <typename TI>
+void AmgDevice::apply_basis(hipStream_t st, const TI* in, float* out, const int* done) {
+    vcycle(st, in, out, nullptr, nullptr, done, true, nullptr);
+    MP_HIP(hipGetLastError());
+}
+template void AmgDevice::apply_basis<float>(hipStream_t, const float*, float*, const int*);
+template void AmgDevice::apply_basis<double>(hipStream_t, const double*, float*, const int*);
+
+template <typename TI, typename TO>
+void AmgDevice::vcycle(hipStream_t st, const TI* in, TO* out, const double* bin, double* bout,
+                       const int* done, bool prescaled, float* out32) {
+    const int L = (int)lv.size();
     auto coarse = [&](const auto* b, auto* y, float* y32, const double* cbin, double* cbout) {
         using TB = std::remove_cv_t<std::remove_pointer_t<decltype(b)>>;
-        using TO = std::remove_pointer_t<decltype(y)>;
+        using TY = std::remove_pointer_t<decltype(y)>;
         if (blocked)
-            hipLaunchKernelGGL((k_amg_blockdense<TB, TO>), dim3((nc + 3) / 4), dim3(BLOCK), 0, st,
+            hipLaunchKernelGGL((k_amg_blockdense<TB, TY>), dim3((nc + 3) / 4), dim3(BLOCK), 0, st,
                                nc, cinv.p, cfirst.p, csize.p, cdata.p, b, y, y32);
         else
-            hipLaunchKernelGGL((k_amg_dense<TB, TO>), dim3((nc + 3) / 4), dim3(BLOCK), 0, st, nc,
+            hipLaunchKernelGGL((k_amg_dense<TB, TY>), dim3((nc + 3) / 4), dim3(BLOCK), 0, st, nc,
                                cinv.p, b, y, cbin, cbout, done, y32);
     };
     {  // (setup builds at least two levels)
-        // downward: level 0 from the FP64 input
+        // downward: level 0 from the input (FP64, or an FP32 basis vector)
         if (prescaled)
-            launch_op<OP_PRES, double, float>(st, lv[0].A, nullptr, in, lv[0].x.p, nullptr,
-                                              nullptr, lv[0].res.p, (float*)nullptr, bin, bout,
-                                              done);
+            launch_op<OP_PRES, TI, float>(st, lv[0].A, nullptr, in, lv[0].x.p, nullptr,
+                                          nullptr, lv[0].res.p, (float*)nullptr, bin, bout,
+                                          done);
         else
-            launch_op<OP_PRE, double, float>(st, lv[0].A, lv[0].dinv.p, in, nullptr, nullptr,
+            launch_op<OP_PRE, TI, float>(st, lv[0].A, lv[0].dinv.p, in, nullptr, nullptr,
                                              lv[0].x.p, lv[0].res.p, (float*)nullptr, bin, bout,
                                              done);
         for (int l = 0; l + 1 < L; ++l) {
@@ -681,8 +699,8 @@ void AmgDevice::apply(hipStream_t st, const double* in, double* out, int bordere
                                              nullptr, nullptr, f.x.p, nullptr, nullptr, done);
             if (l == 0) {
                 if (post_timer) post_timer->begin(st);
-                launch_op<OP_POST, double, double>(st, f.A, f.dinv.p, in, f.x.p, nullptr, nullptr,
-                                                   nullptr, out, nullptr, nullptr, done, out32);
+                launch_op<OP_POST, TI, TO>(st, f.A, f.dinv.p, in, f.x.p, nullptr, nullptr,
+                                           nullptr, out, nullptr, nullptr, done, out32);
                 if (post_timer) post_timer->end(st);
             }
             else
@@ -690,10 +708,6 @@ void AmgDevice::apply(hipStream_t st, const double* in, double* out, int bordere
                                                  nullptr, nullptr, f.y.p, nullptr, nullptr, done);
         }
     }
-    if (blocked && border_rows.n > 0)
-        hipLaunchKernelGGL(k_border_copy, dim3((border_rows.n + 63) / 64), dim3(64), 0, st,
-                           (int)border_rows.n, border_rows.p, in, out);
-    MP_HIP(hipGetLastError());
 }
 
 }  // namespace mp

@@ -39,7 +39,8 @@ typedef double d2 __attribute__((ext_vector_type(2)));
 template <int DIM>
 __device__ __forceinline__ d4 packed_at(const double* __restrict__ a, int c) {
     if constexpr (DIM == 3) {
-        return *reinterpret_cast<const d4*>(a + 4 * (size_t)c);
+        const d2 t = *reinterpret_cast<const d2*>(a + 4 * (size_t)c);
+        return d4{t[0], t[1], a[4 * (size_t)c + 2], 0.0};
     } else {
         const d2 t = *reinterpret_cast<const d2*>(a + 2 * (size_t)c);
         return d4{t[0], t[1], 0.0, 0.0};
@@ -89,10 +90,15 @@ __global__ __launch_bounds__(BLOCK) void predictor_kernel(
         const i4 c4 = cols4(sv, s4);
         const d4 A0 = vals4(gx, s4), A1 = vals4(gy, s4), A3 = vals4(lap, s4);
         const d4 A2 = (DIM == 3) ? vals4(gz, s4) : d4{0.0, 0.0, 0.0, 0.0};
+        // the four gathers are issued together (slots past the row's length
+        // read the row's own node, and their terms are skipped)
+        d4 fg[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) fg[e] = packed_at<DIM>(upk, j0 + e < len ? c4[e] : r);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            if (j0 + e >= len) break;
-            const d4 f = packed_at<DIM>(upk, c4[e]);
+            if (j0 + e >= len) continue;
+            const d4 f = fg[e];
 #pragma unroll
             for (int q = 0; q < DIM; ++q) {
                 acc[q][0] = fma(A0[e], f[q], acc[q][0]);
@@ -163,10 +169,13 @@ __global__ __launch_bounds__(BLOCK) void ppe_rhs_kernel(
             const i4 c4 = cols4(sv, s4);
             const d4 X = vals4(gx, s4), Y = vals4(gy, s4);
             const d4 Z = (DIM == 3) ? vals4(gz, s4) : d4{0.0, 0.0, 0.0, 0.0};
+            d4 fg[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) fg[e] = packed_at<DIM>(uhpk, j0 + e < len ? c4[e] : r);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                if (j0 + e >= len) break;
-                const d4 f = packed_at<DIM>(uhpk, c4[e]);
+                if (j0 + e >= len) continue;
+                const d4 f = fg[e];
                 ax = fma(X[e], f[0], ax);
                 ay = fma(Y[e], f[1], ay);
                 if (DIM == 3) az = fma(Z[e], f[2], az);
@@ -202,10 +211,13 @@ __global__ __launch_bounds__(BLOCK) void corrector_kernel(
         const i4 c4 = cols4(sv, s4);
         const d4 X = vals4(gx, s4), Y = vals4(gy, s4);
         const d4 Z = (DIM == 3) ? vals4(gz, s4) : d4{0.0, 0.0, 0.0, 0.0};
+        double pg[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) pg[e] = p[j0 + e < len ? c4[e] : r];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            if (j0 + e >= len) break;
-            const double pc = p[c4[e]];
+            if (j0 + e >= len) continue;
+            const double pc = pg[e];
             ax = fma(X[e], pc, ax);
             ay = fma(Y[e], pc, ay);
             if (DIM == 3) az = fma(Z[e], pc, az);

@@ -577,9 +577,11 @@ __global__ void k_gm_reset(int nseg, GmState* gm, GmFlags* fl, const SegCtl* ctl
     fl[q] = GmFlags{g->done, 0, 0, 0};
 }
 
-// v0 = rhs - A x (unnormalised) for the systems starting (mode 0) or
-// restarting (mode 1: a full cycle, still running); partial sums of v0.v0,
-// rhs.rhs and the border row's D^-1 x.
+// v0 = rhs - A x (unnormalised; the basis keeps it so, with sig[0] = 1/|v0|)
+// for the systems starting (mode 0) or restarting (mode 1: a full cycle,
+// still running), and the V-cycle's FP32 input (d0 v0, or v0 without the
+// scaling) over the node rows; partial sums of v0.v0, rhs.rhs and the border
+// row's D^-1 x.
 template <typename TM, typename TV>
 __global__ __launch_bounds__(BLOCK) void k_gm_residual(SellView sv, SegView sg,
                                                        const TM* __restrict__ a,
@@ -587,6 +589,8 @@ __global__ __launch_bounds__(BLOCK) void k_gm_residual(SellView sv, SegView sg,
                                                        const double* __restrict__ rhs,
                                                        const double* __restrict__ x,
                                                        TV* __restrict__ v0,
+                                                       const float* __restrict__ d0,
+                                                       float* __restrict__ xs,
                                                        double* __restrict__ partial,
                                                        const GmState* gm, int mode) {
     const int nbk = sg.nrows / BLOCK;
@@ -605,7 +609,9 @@ __global__ __launch_bounds__(BLOCK) void k_gm_residual(SellView sv, SegView sg,
         if (i < iend) {
             if (S.border) ax = ax + x[iend];
             const double ri = rhs[i] - ax;
-            v0[i] = (TV)ri;
+            const TV vi = (TV)ri;
+            v0[i] = vi;
+            if (xs) xs[i] = d0 ? d0[i] * (float)vi : (float)vi;
             vals[0] = ri * ri;
             vals[1] = rhs[i] * rhs[i];
             if (S.border) vals[2] = dinv[i] * x[i];
@@ -656,6 +662,7 @@ __global__ __launch_bounds__(FIN) void k_gm_residual_finish(SegView sg,
     g->g[0] = sqrt(rr);
     g->rr = rr;
     g->inv_eta = 1.0 / g->g[0];
+    g->sig[0] = g->inv_eta;
     g->j = 0;
     g->full = 0;
     g->apply = 0;
@@ -734,10 +741,13 @@ __global__ __launch_bounds__(BLOCK) void k_gm_dots(SegView sg, const double* __r
     }
 }
 
-// Hessenberg column: h_k = sum of the partials of v_k . w plus the border
-// term; w's border entry from its partial sums.  One block per (system, k):
-// each sums its row of block partials and the border row (every 256th
-// partial per thread, four in flight, then a fixed reduction tree).
+// Hessenberg column: h_k = sum of the partials of V'_k . w plus the border
+// term, scaled to the normalised vectors (w = A M V'_j is sig[j] times the
+// product of v_j): h_k = sig[k] sig[j] (V'_k . w), and V'_k's coefficient in the
+// Gram-Schmidt update hc[k] = h_k sig[k]; w's border entry from its partial
+// sums.  One block per (system, k): each sums its row of block partials and
+// the border row (every 256th partial per thread, four in flight, then a
+// fixed reduction tree).
 constexpr int GMF = 256;  // threads of the Hessenberg-column kernel
 __device__ __forceinline__ double gm_row_sum(const double* __restrict__ p, int cnt) {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -784,15 +794,21 @@ __global__ __launch_bounds__(GMF) void k_gm_finish_h(SegView sg, double* __restr
         if (k == 0) w[ib] = wb;
         tk += (double)V[k * ld + ib] * wb;
     }
-    g->h[k] = tk;
+    const double h = g->sig[k] * g->sig[nk - 1] * tk;
+    g->h[k] = h;
+    g->hc[k] = h * g->sig[k];
 }
 
-// w -= sum_k h_k v_k over the node rows and the border unknown; partial sums
-// of the new w.w
+// The next basis vector, unnormalised: V'_nk = sig[nk-1] w - sum_k hc[k] V'_k
+// over the node rows and the border unknown (rounded to the basis type), the
+// V-cycle's FP32 input (d0 V'_nk, or V'_nk without the scaling) over the node
+// rows, and partial sums of V'_nk . V'_nk (its 1/norm becomes sig[nk]).
 template <typename TV>
-__global__ __launch_bounds__(BLOCK) void k_gm_orth(SegView sg, double* __restrict__ w,
-                                                   const TV* __restrict__ V, size_t ld,
-                                                   int nk, double* __restrict__ partial,
+__global__ __launch_bounds__(BLOCK) void k_gm_orth(SegView sg, const double* __restrict__ w,
+                                                   TV* __restrict__ V, size_t ld, int nk,
+                                                   const float* __restrict__ d0,
+                                                   float* __restrict__ xs,
+                                                   double* __restrict__ partial,
                                                    const GmState* gm) {
     const int nbk = sg.nrows / BLOCK;
     const int lb = blockIdx.x;
@@ -803,10 +819,12 @@ __global__ __launch_bounds__(BLOCK) void k_gm_orth(SegView sg, double* __restric
     const int i = lb * BLOCK + threadIdx.x;
     double vals[1] = {0.0};
     if (i < S.row0 + S.nv + S.border) {
-        double wi = w[i];
-        for (int k = 0; k < nk; ++k) wi = fma(-g->h[k], (double)V[k * ld + i], wi);
-        w[i] = wi;
-        vals[0] = wi * wi;
+        double wi = g->sig[nk - 1] * w[i];
+        for (int k = 0; k < nk; ++k) wi = fma(-g->hc[k], (double)V[k * ld + i], wi);
+        const TV v = (TV)wi;
+        V[(size_t)nk * ld + i] = v;
+        if (xs && i < S.row0 + S.nv) xs[i] = d0 ? d0[i] * (float)v : (float)v;
+        vals[0] = (double)v * (double)v;
     }
     block_store<1>(vals, partial, lb, nbk);
 }
@@ -867,6 +885,7 @@ __global__ __launch_bounds__(FIN) void k_gm_finish_norm(SegView sg,
     const double res2 = g->g[j + 1] * g->g[j + 1];
     g->rr = res2;
     g->inv_eta = 1.0 / eta;
+    g->sig[j + 1] = g->inv_eta;
     if (!(res2 <= 1.0e300) || !(fabs(r) <= 1.0e300))
         g->done = 5;
     else if (res2 <= g->tol2)
@@ -879,29 +898,6 @@ __global__ __launch_bounds__(FIN) void k_gm_finish_norm(SegView sg,
         g->full = 1;
     g->stop = (g->done != 0 || g->full) ? 1 : 0;
     fl[blockIdx.x] = GmFlags{g->done, g->full, g->iter, g->cycles};
-}
-
-// v_next = w / |w| (rounded to the basis type) over the node rows and the
-// border unknown; its FP64 copy vd (when the basis is FP32 and the V-cycle
-// reads FP64: the same values), and the V-cycle's FP32 input (d0 v, or v
-// without the scaling) over the node rows.  w may alias vn (the start of a
-// cycle normalises v0 in place).
-template <typename TW, typename TV>
-__global__ __launch_bounds__(BLOCK) void k_gm_scale(SegView sg, const TW* w, TV* vn,
-                                                    double* __restrict__ vd,
-                                                    const float* __restrict__ d0,
-                                                    float* __restrict__ xs, const GmState* gm) {
-    const int q = block_seg(sg, blockIdx.x);
-    const GmState* g = gm + q;
-    if (g->stop) return;
-    const Seg S = sg.seg[q];
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    const int iend = S.row0 + S.nv;
-    if (i >= iend + S.border) return;
-    const TV v = (TV)((double)w[i] * g->inv_eta);
-    vn[i] = v;
-    if (vd) vd[i] = (double)v;
-    if (xs && i < iend) xs[i] = d0 ? d0[i] * (float)v : (float)v;
 }
 
 // end of a cycle: y = R^-1 g over the cycle's columns (back substitution)
@@ -931,8 +927,9 @@ __global__ void k_gm_solve(int nseg, GmState* gm, GmFlags* fl) {
     g->napply = j;
 }
 
-// x += sum_k y_k z_k over the node rows and the border unknown (z_k's border
-// entry is v_k's: the V-cycle passes it through)
+// x += sum_k y_k z_k over the node rows and the border unknown, with
+// z_k = sig[k] Z'_k, Z'_k = M V'_k (z_k's border entry is v_k's: the
+// preconditioner is the identity on the border unknown)
 template <typename TZ, typename TV>
 __global__ __launch_bounds__(BLOCK) void k_gm_update_x(SegView sg, double* __restrict__ x,
                                                        const TZ* __restrict__ Z,
@@ -947,9 +944,9 @@ __global__ __launch_bounds__(BLOCK) void k_gm_update_x(SegView sg, double* __res
     if (i >= iend + S.border) return;
     double xi = x[i];
     if (i < iend)
-        for (int k = 0; k < g->napply; ++k) xi = fma(g->y[k], (double)Z[k * ld + i], xi);
+        for (int k = 0; k < g->napply; ++k) xi = fma(g->y[k] * g->sig[k], (double)Z[k * ld + i], xi);
     else
-        for (int k = 0; k < g->napply; ++k) xi = fma(g->y[k], (double)V[k * ld + i], xi);
+        for (int k = 0; k < g->napply; ++k) xi = fma(g->y[k] * g->sig[k], (double)V[k * ld + i], xi);
     x[i] = xi;
 }
 
@@ -1426,8 +1423,6 @@ void PpeSolver::gmres_alloc(hipStream_t st, int m) {
     if (amg) {
         gm_z.alloc(ld * m);
         MP_HIP(hipMemsetAsync(gm_z.p, 0, sizeof(float) * ld * m, st));
-        gm_vd.alloc(ld);
-        MP_HIP(hipMemsetAsync(gm_vd.p, 0, sizeof(double) * ld, st));
     }
     gm_w.alloc(ld);
     MP_HIP(hipMemsetAsync(gm_w.p, 0, sizeof(double) * ld, st));
@@ -1449,16 +1444,12 @@ void PpeSolver::gmres_iteration(hipStream_t st, int k, int m, const TM* a, const
     const size_t ld = (size_t)nrows;
     TV* V = reinterpret_cast<TV*>(gm_v.p);
     TV* vk = V + ld * k;
-    TV* vk1 = V + ld * (k + 1);
-    constexpr bool fp32_basis = sizeof(TV) == 4;
     if (amg) {
         // one system: its V-cycle stops with it; a batch runs it while any
         // system iterates (the others ignore the result).  The V-cycle reads
-        // FP64 (an FP32 basis vector's FP64 copy) and its FP32 output is z_k.
-        const double* vin = fp32_basis ? gm_vd.p : reinterpret_cast<const double*>(vk);
+        // the basis vector and writes Z'_k in FP32.
         float* zk = gm_z.p + ld * k;
-        amg->apply(st, vin, yv.p, nseg == 1 ? segs[0].border : 1,
-                   nseg == 1 ? &gm_d.p[0].stop : nullptr, true, zk);
+        amg->apply_basis<TV>(st, vk, zk, nseg == 1 ? &gm_d.p[0].stop : nullptr);
         spmv_timer.begin(st);
         hipLaunchKernelGGL((k_gm_spmv<TM, float, TV>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
                            view_of<TM>(), sg, a, di, (const TV*)vk, (const float*)zk, gm_w.p,
@@ -1474,14 +1465,11 @@ void PpeSolver::gmres_iteration(hipStream_t st, int k, int m, const TM* a, const
                        k + 1, gm_partial.p, gm_d.p);
     hipLaunchKernelGGL(k_gm_finish_h<TV>, dim3(nseg * GM_MAX), dim3(GMF), 0, st, sg, gm_w.p,
                        (const TV*)V, ld, k + 1, gm_partial.p, gm_d.p);
-    hipLaunchKernelGGL(k_gm_orth<TV>, dim3(nbk), dim3(BLOCK), 0, st, sg, gm_w.p, (const TV*)V, ld,
-                       k + 1, gm_partial.p, gm_d.p);
+    hipLaunchKernelGGL(k_gm_orth<TV>, dim3(nbk), dim3(BLOCK), 0, st, sg, (const double*)gm_w.p,
+                       V, ld, k + 1, amg ? amg->level0_dinv() : nullptr,
+                       amg ? amg->prescaled_input() : nullptr, gm_partial.p, gm_d.p);
     hipLaunchKernelGGL(k_gm_finish_norm, dim3(nseg), dim3(FIN), 0, st, sg, gm_partial.p, gm_d.p,
                        gmf_d.p, m);
-    hipLaunchKernelGGL((k_gm_scale<double, TV>), dim3(nbk), dim3(BLOCK), 0, st, sg,
-                       (const double*)gm_w.p, vk1, (fp32_basis && amg) ? gm_vd.p : nullptr,
-                       amg ? amg->level0_dinv() : nullptr, amg ? amg->prescaled_input() : nullptr,
-                       gm_d.p);
     MP_HIP(hipGetLastError());
 }
 
@@ -1494,21 +1482,19 @@ void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const d
     gmres_alloc(st, m);
     const size_t ld = (size_t)nrows;
     TV* V = reinterpret_cast<TV*>(gm_v.p);
-    constexpr bool fp32_basis = sizeof(TV) == 4;
     const float* d0 = amg ? amg->level0_dinv() : nullptr;
     float* xs = amg ? amg->prescaled_input() : nullptr;
     MP_HIP(hipMemcpyAsync(ctl_d.p, host_ctl, sizeof(SegCtl) * nseg, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_gm_reset, dim3((nseg + 63) / 64), dim3(64), 0, st, nseg, gm_d.p, gmf_d.p,
                        ctl_d.p);
-    // v0 = (rhs - A x) / |rhs - A x| for the systems taking part (mode 0), or
-    // for those whose full cycle restarts (mode 1)
+    // V'_0 = rhs - A x, sig[0] = 1 / |rhs - A x| for the systems taking part
+    // (mode 0), or for those whose full cycle restarts (mode 1)
     auto start = [&](int mode) {
         hipLaunchKernelGGL((k_gm_residual<TM, TV>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
-                           view_of<TM>(), sg, a, di, rhs, x, V, gm_partial.p, gm_d.p, mode);
+                           view_of<TM>(), sg, a, di, rhs, x, V, d0, xs, gm_partial.p, gm_d.p,
+                           mode);
         hipLaunchKernelGGL(k_gm_residual_finish<TV>, dim3(nseg), dim3(FIN), 0, st, sg, rhs, V,
                            gm_partial.p, gm_d.p, gmf_d.p, mode, rel_to_rhs);
-        hipLaunchKernelGGL((k_gm_scale<TV, TV>), dim3(nbk), dim3(BLOCK), 0, st, sg, (const TV*)V,
-                           V, (fp32_basis && amg) ? gm_vd.p : nullptr, d0, xs, gm_d.p);
         MP_HIP(hipGetLastError());
     };
     start(0);

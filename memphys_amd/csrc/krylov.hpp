@@ -37,6 +37,9 @@ struct KernelTimer {
 constexpr int GM_MAX = 31;
 struct GmState {
     double g[GM_MAX + 1], cs[GM_MAX], sn[GM_MAX], h[GM_MAX + 1], y[GM_MAX];
+    // basis vectors are stored unnormalised: v_k = sig[k] V'_k; hc[c] = h[c] sig[c]
+    // is the coefficient of V'_c in the Gram-Schmidt update
+    double sig[GM_MAX + 1], hc[GM_MAX];
     double R[GM_MAX * GM_MAX];
     double bb, tol2, tol_rel, rr, inv_eta;
     double rr_cycle;  // |r|^2 at the start of the current cycle
@@ -104,7 +107,6 @@ struct PpeSolver {
     int restart_m = 30;               // GMRES columns per cycle
     DevBuf<double> gm_v, gm_w, gm_partial;  // basis (FP64 or FP32 in the same bytes), work, partials
     DevBuf<float> gm_z;               // preconditioned basis: the V-cycle's FP32 outputs
-    DevBuf<double> gm_vd;             // FP64 copy of an FP32 basis vector (the V-cycle's input)
     DevBuf<GmState> gm_d;             // [nseg]
     DevBuf<GmFlags> gmf_d;            // [nseg]
     GmFlags* host_gmf = nullptr;      // pinned, 2 x nseg
