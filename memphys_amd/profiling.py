@@ -169,6 +169,28 @@ def steps(db: str, marker: str = "predictor_kernel", last: int = 0) -> str:
                      f"{len(seg):6d} " + " ".join(f"{per[c] / 1e6:12.3f}" for c in cats))
     top = sorted(other_names.items(), key=lambda kv: -kv[1])[:8]
     lines.append("other: " + ", ".join(f"{k} {v / 1e6:.3f}" for k, v in top))
+    # idle stretches of the last step: where the device waited for the host
+    a, b = wins[-1]
+    gaps = []
+    for k in range(a, b):
+        end = max(r[2] for r in rows[a:k + 1])
+        g = rows[k + 1][1] - end
+        if g > 0:
+            gaps.append((g, k))
+    gaps.sort(reverse=True)
+    lines.append(f"largest idle stretches of step {len(wins) - 1} (total "
+                 f"{sum(g for g, _ in gaps) / 1e6:.3f} ms over {len(gaps)} stretches):")
+    for g, k in gaps[:12]:
+        lines.append(f"  {g / 1e3:8.1f} us after {_short(rows[k][0])[:40]} "
+                     f"(grid {rows[k][3]}) before {_short(rows[k + 1][0])[:40]}")
+    hist = {}
+    for g, _ in gaps:
+        key = "<2us" if g < 2e3 else "2-10us" if g < 1e4 else "10-50us" if g < 5e4 else ">=50us"
+        hist[key] = hist.get(key, [0, 0])
+        hist[key][0] += 1
+        hist[key][1] += g
+    lines.append("  by size: " + ", ".join(f"{k}: {v[0]} stretches {v[1] / 1e6:.3f} ms"
+                                          for k, v in sorted(hist.items())))
     return "\n".join(lines)
 
 
