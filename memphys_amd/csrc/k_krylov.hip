@@ -496,6 +496,14 @@ __global__ void k_sc_reset(int nseg, KScalars* sc, const SegCtl* ctl) {
     sc[q] = z;
 }
 
+// dst = src (the solver's device-to-device copies: a plain kernel streams
+// them several times faster than the runtime's copy for these sizes)
+__global__ __launch_bounds__(BLOCK) void k_copy_vec(int n, const double* __restrict__ src,
+                                                     double* __restrict__ dst) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(BLOCK) void k_fill(double* __restrict__ x, double v, int n) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) x[i] = v;
@@ -901,22 +909,32 @@ __global__ __launch_bounds__(FIN) void k_gm_finish_norm(SegView sg,
 }
 
 // end of a cycle: y = R^-1 g over the cycle's columns (back substitution)
-__global__ void k_gm_solve(int nseg, GmState* gm, GmFlags* fl) {
-    const int q = blockIdx.x * 64 + threadIdx.x;
-    if (q >= nseg) return;
+__global__ __launch_bounds__(64) void k_gm_solve(GmState* gm, GmFlags* fl) {
+    // one block per system: the cycle's triangle and g are staged in LDS,
+    // then one thread substitutes (the same operations in the same order)
+    __shared__ double sR[GM_MAX * GM_MAX], sg[GM_MAX + 1], sy[GM_MAX];
+    const int q = blockIdx.x;
     GmState* g = gm + q;
     const int j = g->j;
+    const bool run = j > 0 && g->done < 4;
+    if (run) {
+        for (int t = threadIdx.x; t < j * GM_MAX; t += 64) sR[t] = g->R[t];
+        if ((int)threadIdx.x < j) sg[threadIdx.x] = g->g[threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     g->apply = 0;
     g->napply = 0;
     g->j = 0;  // the cycle's columns are consumed here, once
-    if (j == 0 || g->done >= 4) return;
+    if (!run) return;
     bool finite = true;
     for (int i = j - 1; i >= 0; --i) {
-        double t = g->g[i];
-        for (int k = i + 1; k < j; ++k) t -= g->R[k * GM_MAX + i] * g->y[k];
-        g->y[i] = t / g->R[i * GM_MAX + i];
-        finite = finite && fabs(g->y[i]) <= 1.0e300;
+        double t = sg[i];
+        for (int k = i + 1; k < j; ++k) t -= sR[k * GM_MAX + i] * sy[k];
+        sy[i] = t / sR[i * GM_MAX + i];
+        finite = finite && fabs(sy[i]) <= 1.0e300;
     }
+    for (int i = 0; i < j; ++i) g->y[i] = sy[i];
     if (!finite) {
         g->done = 5;
         g->stop = 1;
@@ -955,6 +973,12 @@ __global__ __launch_bounds__(BLOCK) void k_gm_update_x(SegView sg, double* __res
 void launch_fill(hipStream_t s, double* x, double v, int n) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(BLOCK), 0, s, x, v, n);
+    MP_HIP(hipGetLastError());
+}
+
+void launch_copy(hipStream_t s, double* dst, const double* src, int n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy_vec, dim3(nblk(n)), dim3(BLOCK), 0, s, n, src, dst);
     MP_HIP(hipGetLastError());
 }
 
@@ -1523,7 +1547,7 @@ void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const d
             have_prev = true;
             slot ^= 1;
         }
-        hipLaunchKernelGGL(k_gm_solve, dim3((nseg + 63) / 64), dim3(64), 0, st, nseg, gm_d.p,
+        hipLaunchKernelGGL(k_gm_solve, dim3(nseg), dim3(64), 0, st, gm_d.p,
                            gmf_d.p);
         if (amg)
             hipLaunchKernelGGL((k_gm_update_x<float, TV>), dim3(nbk), dim3(BLOCK), 0, st, sg, x,
@@ -1673,8 +1697,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                 any |= act[q] != 0;
             }
             if (!any) break;
-            MP_HIP(hipMemcpyAsync(rhs64.p, tmp.p, sizeof(double) * nrows, hipMemcpyDeviceToDevice,
-                                  st));
+            launch_copy(st, rhs64.p, tmp.p, nrows);
             launch_fill(st, e64.p, 0.0, nrows);
             if (krylov == 1)
                 run_gmres<float, float>(st, a32k, dinv.p, rhs64.p, e64.p, 1, done, its, rst);
@@ -1702,8 +1725,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
             }
             // x~ before the correction, kept in rhs64 (free once the inner
             // solve is done); e is zero on the systems that took no part
-            MP_HIP(hipMemcpyAsync(rhs64.p, xt.p, sizeof(double) * nrows, hipMemcpyDeviceToDevice,
-                                  st));
+            launch_copy(st, rhs64.p, xt.p, nrows);
             hipLaunchKernelGGL(k_axpy, dim3(nbk), dim3(BLOCK), 0, st, nrows, 1.0, e64.p, xt.p);
             MP_HIP(hipGetLastError());
             residual64(st, b_dev, rr, bb);

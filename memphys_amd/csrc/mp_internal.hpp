@@ -337,5 +337,6 @@ int largest_eigenvalue(hipStream_t s, const SellView& sv, const double* val, int
 
 // Krylov building blocks (k_krylov.hip).
 void launch_fill(hipStream_t s, double* x, double v, int n);
+void launch_copy(hipStream_t s, double* dst, const double* src, int n);
 
 }  // namespace mp
