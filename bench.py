@@ -509,6 +509,9 @@ def main():
     ap.add_argument("--krylov", choices=["gmres", "bicgstab"], default="gmres",
                     help="Krylov method of the multigrid forms and of the batch (the plain FP64 "
                          "and mixed Jacobi forms use BiCGSTAB, the reference's)")
+    ap.add_argument("--only-weights", action="store_true",
+                    help="build the 1M-node cloud, time the stencil weights (FP64, then FP32 + "
+                         "FP64 refinement), print them as one JSON line and stop (for profilers)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default poll interval and multigrid layouts (for counter "
                          "runs, under which the tuning's timed solves take minutes)")
@@ -540,7 +543,7 @@ def main():
     guess = None
     t_train = 0.0
     small = M.box_cloud(dim=3, n=40, jitter=0.3, seed=101, k_interior=40, k_layers=(60, 55, 50))
-    if "headline" in forms:
+    if "headline" in forms and not args.only_weights:
         t = time.perf_counter()
         episodes = 10
         guess = M.train_guess(small, prm["rho"], prm["mu"], prm["dt"], 1e-8, episodes,
@@ -598,8 +601,13 @@ def main():
                 cpu_weights_rate(cloud, np.random.default_rng(3).choice(cloud.nv, 3000,
                                                                         replace=False), 1)})
         del mx, my, mz, ml
-    ctx.weights(3, 3, download=False)  # FP64 weights back for the time steps
     log(f"weights: {weights_check}")
+    if args.only_weights:
+        print(json.dumps({"workload": f"stencil weights, {cloud.nv} nodes (PHS r^3 + deg-3 "
+                                      f"poly, k=40/50/55/60)", "weights": weights_check}))
+        ctx.close()
+        return
+    ctx.weights(3, 3, download=False)  # FP64 weights back for the time steps
 
     # ---- full runs: W warm-up steps + K timed steps each
     full = {}

@@ -16,7 +16,8 @@
         per timestep of the largest system (stretches between launches of
         `marker`, default predictor_kernel): wall, busy and idle time, launches
         and kernel time per category (predictor, right-hand side, corrector,
-        learned guess, Krylov, multigrid finest and coarse levels)
+        learned guess, Krylov, the multigrid's level-0 smoothing and the rest)
+        and the idle stretches of the last step
     python -m memphys_amd.profiling pmc DB [--top N]
         per kernel, grid size and counter: dispatches and mean value per dispatch
     python -m memphys_amd.profiling traffic FETCH_DB WRITE_DB --kernel SUBSTR
@@ -125,18 +126,19 @@ def steps(db: str, marker: str = "predictor_kernel", last: int = 0) -> str:
     """Per timestep of the largest system: the stretches between consecutive
     launches of `marker` at its largest grid.  Wall time, busy time (union of
     kernel intervals), launches, and kernel time per category; multigrid
-    launches split by grid into the finest level and the coarse levels."""
+    launches split into the finest level's two smoothing products and the rest
+    (restriction, prolongation, coarse levels).  Then the idle stretches of
+    the last step: the largest, and a histogram by length."""
     cur = sqlite3.connect(db).cursor()
     rows = cur.execute("select name, start, end, grid_x from kernels order by start").fetchall()
     grid = max((r[3] for r in rows if marker in r[0]), default=None)
     if grid is None:
         raise SystemExit(f"no launch of '{marker}' in {db}")
     heads = [i for i, r in enumerate(rows) if marker in r[0] and r[3] == grid]
-    amg_grid = max((r[3] for r in rows if "k_amg" in r[0]), default=0)
     wins = list(zip(heads[:-1], heads[1:]))
     if last:
         wins = wins[-last:]
-    cats = [c for c, _ in _CATEGORIES] + ["multigrid finest", "multigrid coarse", "other"]
+    cats = [c for c, _ in _CATEGORIES] + ["mg l0 smooth", "multigrid other", "other"]
     lines = [f"timesteps {len(wins)} (marker {marker}, grid {grid}); times in ms"]
     lines.append(f"{'step':>4s} {'wall':>7s} {'busy':>7s} {'idle':>6s} {'launch':>6s} " +
                  " ".join(f"{c[:12]:>12s}" for c in cats))
@@ -158,7 +160,10 @@ def steps(db: str, marker: str = "predictor_kernel", last: int = 0) -> str:
                     break
             else:
                 if "k_amg" in name:
-                    per["multigrid finest" if g >= amg_grid // 2 else "multigrid coarse"] += e - s
+                    # the finest level's two smoothing products (SELL layout,
+                    # one lane per row: their grid is the marker's)
+                    l0 = ("k_amg_op<0, 2" in name or "k_amg_op<0, 3" in name) and g >= grid
+                    per["mg l0 smooth" if l0 else "multigrid other"] += e - s
                 else:
                     per["other"] += e - s
                     k = _short(name)

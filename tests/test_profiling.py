@@ -100,3 +100,31 @@ def test_bench_reads_committed_traffic_summaries():
         assert got is not None and abs(got / alg - 1) < 0.01
     assert bench.measured_traffic(50, 0) is None
     assert bench.smooth_roofline((381, 0.06, 2e8, 393), 6000.0, 50)["traffic"] is None
+
+
+def test_steps_splits_time_and_idle(tmp_path):
+    # two whole timesteps (three predictor launches), each: predictor 500 us,
+    # a 20 us gap, one level-0 smoothing product 50 us, a coarse product 5 us,
+    # a GMRES SpMV 60 us, a 100 us gap, the corrector 300 us
+    rows, t = [], 0
+    for step in range(3):
+        seq = [("void mp::predictor_kernel<3>(int)", 500_000, 998_912, 20_000),
+               ("void mp::k_amg_op<0, 3, float, float>(int)", 50_000, 999_424, 0),
+               ("void mp::k_amg_op<8, 1, float, float>(int)", 5_000, 655_360, 0),
+               ("void mp::k_gm_spmv<float, float, float>(int)", 60_000, 999_424, 100_000),
+               ("void mp::corrector_kernel<3>(int)", 300_000, 998_912, 0)]
+        for name, d, grid, gap in seq:
+            rows.append((name, t, t + d, grid))
+            t += d + gap
+    out = P.steps(_kernels_db(tmp_path / "s.db", rows)).splitlines()
+    assert out[0].startswith("timesteps 2")
+    cells = out[2].split()
+    wall, busy, idle, launches = float(cells[1]), float(cells[2]), float(cells[3]), int(cells[4])
+    assert (wall, busy, idle) == pytest.approx((1.035, 0.915, 0.12), abs=0.006)  # (2 decimals)
+    assert launches == 5
+    pred, rhs, corr, guess, gmres, bicg, l0, mg, other = map(float, cells[5:14])
+    assert (pred, corr, gmres, l0, mg) == (0.5, 0.3, 0.06, 0.05, 0.005)
+    assert rhs == guess == bicg == other == 0.0
+    idle_line = [ln for ln in out if ln.startswith("largest idle")][0]
+    assert "0.120 ms over 2 stretches" in idle_line
+    assert "100.0 us after mp::k_gm_spmv<float, float, float>" in "\n".join(out)
