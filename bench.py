@@ -309,7 +309,7 @@ def cavity2d_c1(M, O, steps=20, warmup=5, with_cpu=True):
     return out
 
 
-def smooth_roofline(t, copy_gbps, n_lattice: int = 0):
+def smooth_roofline(t, copy_gbps, n_lattice: int = 0, gmres: bool = True):
     """Roofline entry of the multigrid's level-0 post-smoothing product (the
     V-cycle's largest kernel): algorithmic bytes per launch over its mean
     HIP-event duration, launches that did the work; `traffic` is its measured
@@ -318,11 +318,15 @@ def smooth_roofline(t, copy_gbps, n_lattice: int = 0):
         return None
     n, ms, bytes_l, all_l = t
     ach = bytes_l / (ms * 1e-3) / 1e9
-    return {"kernel": "k_amg_op<0,2,double,double> (level-0 post-smoothing, BF16 values)",
+    name = ("k_amg_op<0,2,float,float> (level-0 post-smoothing, BF16 values; reads the FP32 "
+            "GMRES basis vector, writes the FP32 direction)" if gmres else
+            "k_amg_op<0,2,double,double> (level-0 post-smoothing, BF16 values)")
+    return {"kernel": name,
             "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "frac_of_copy": ach / copy_gbps, "bytes_per_launch": bytes_l,
             "mean_ms": ms, "launches": n, "launches_returned_early": all_l - n,
-            "traffic": measured_traffic(n_lattice, 0, "amg_level0_traffic_cavity3d_1M.md")}
+            "traffic": measured_traffic(n_lattice, 0, "amg_level0_traffic_cavity3d_1M.md")
+            if gmres else None}
 
 
 def stream_copy_gbps(mib: int = 1024, reps: int = 10) -> float:
@@ -756,7 +760,7 @@ def main():
                                "bytes_per_launch": b64, "mean_ms": ms64, "launches": n64,
                                "launches_returned_early": all64 - n64,
                                "traffic": measured_traffic(args.n, 1)},
-        "amg_level0_roofline": smooth_roofline(head.get("smooth"), copy_gbps, args.n),
+        "amg_level0_roofline": smooth_roofline(head.get("smooth"), copy_gbps, args.n, bool(kry)),
         "step_kernels": step_kernels(head, copy_gbps),
         "cloud_setup": setup,
         "weights_kernel_ms": wk[1],

@@ -69,11 +69,15 @@ __global__ __launch_bounds__(BLOCK) void predictor_kernel(
     const double* __restrict__ S_oo, int npad, double rho_all, double mu_all, double dt_all,
     int euler, const int* __restrict__ bprob, const double* __restrict__ prm,
     const double* __restrict__ force) {
-    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    // XCD-aware numbering: each XCD walks a contiguous eighth of the rows, so
+    // the neighbour values it gathers stay in its own L2
+    const int lb = xcd_block((sv.nv + BLOCK - 1) / BLOCK);
+    if (lb < 0) return;
+    const int r = lb * BLOCK + threadIdx.x;
     if (r >= sv.nv) return;
     double rho = rho_all, mu = mu_all, dt = dt_all;
     if (bprob) {  // batch: parameters of this block's problem
-        const int q = bprob[blockIdx.x];
+        const int q = bprob[lb];
         rho = prm[3 * q];
         mu = prm[3 * q + 1];
         dt = prm[3 * q + 2];
@@ -143,11 +147,15 @@ __global__ __launch_bounds__(BLOCK) void ppe_rhs_kernel(
     const uint8_t* __restrict__ bnd, const uint8_t* __restrict__ pdir,
     const double* __restrict__ nrm, int npad, double rho_all, double dt_all,
     double* __restrict__ b, const int* __restrict__ bprob, const double* __restrict__ prm) {
-    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    // XCD-aware numbering: each XCD walks a contiguous eighth of the rows, so
+    // the neighbour values it gathers stay in its own L2
+    const int lb = xcd_block((sv.nv + BLOCK - 1) / BLOCK);
+    if (lb < 0) return;
+    const int r = lb * BLOCK + threadIdx.x;
     if (r >= sv.nv) return;  // rows without a stencil (border unknowns, padding) get b = 0
     double rho = rho_all, dt = dt_all;
     if (bprob) {
-        const int q = bprob[blockIdx.x];
+        const int q = bprob[lb];
         rho = prm[3 * q];
         dt = prm[3 * q + 2];
     }
@@ -195,11 +203,15 @@ __global__ __launch_bounds__(BLOCK) void corrector_kernel(
     const double* __restrict__ vh, const double* __restrict__ wh, double* __restrict__ un,
     double* __restrict__ vn, double* __restrict__ wn, double rho_all, double dt_all,
     const int* __restrict__ bprob, const double* __restrict__ prm) {
-    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    // XCD-aware numbering: each XCD walks a contiguous eighth of the rows, so
+    // the neighbour values it gathers stay in its own L2
+    const int lb = xcd_block((sv.nv + BLOCK - 1) / BLOCK);
+    if (lb < 0) return;
+    const int r = lb * BLOCK + threadIdx.x;
     if (r >= sv.nv) return;
     double rho = rho_all, dt = dt_all;
     if (bprob) {
-        const int q = bprob[blockIdx.x];
+        const int q = bprob[lb];
         rho = prm[3 * q];
         dt = prm[3 * q + 2];
     }
@@ -305,8 +317,11 @@ template <int DIM>
 __global__ __launch_bounds__(BLOCK) void divergence_kernel(
     SellView sv, const double* __restrict__ gx, const double* __restrict__ gy,
     const double* __restrict__ gz, const double* __restrict__ u, const double* __restrict__ v,
-    const double* __restrict__ w, const uint8_t* __restrict__ bnd, double* __restrict__ partial) {
-    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    const double* __restrict__ w, const uint8_t* __restrict__ bnd, double* __restrict__ partial,
+    int nblocks) {
+    const int lb = xcd_block(nblocks);  // (XCD-aware, as the predictor)
+    if (lb < 0) return;
+    const int r = lb * BLOCK + threadIdx.x;
     double vals[4] = {0.0, 0.0, 0.0, 0.0};
     if (r < sv.nv) {
         const int base = row_base(sv, r);
@@ -356,7 +371,7 @@ __global__ __launch_bounds__(BLOCK) void divergence_kernel(
             o[2] = fmax(o[2], sh[2][t]);
             o[3] += sh[3][t];
         }
-        for (int q = 0; q < 4; ++q) partial[blockIdx.x * 4 + q] = o[q];
+        for (int q = 0; q < 4; ++q) partial[lb * 4 + q] = o[q];
     }
 }
 
@@ -418,7 +433,7 @@ void launch_predictor(hipStream_t s, const SellView& sv, const double* gx, const
                       const double* force) {
     const int g = (sv.nv + BLOCK - 1) / BLOCK;
     hipLaunchKernelGGL(pack_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv.nv, u, v, w, upk);
-    hipLaunchKernelGGL(predictor_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, lap, upk,
+    hipLaunchKernelGGL(predictor_kernel<DIM>, dim3(grid_xcd(g)), dim3(BLOCK), 0, s, sv, gx, gy, gz, lap, upk,
                        uh, vh, wh, uhpk, S_new, S_oo, npad, rho, mu, dt, euler, bprob, prm, force);
     MP_HIP(hipGetLastError());
 }
@@ -431,7 +446,7 @@ void launch_ppe_rhs(hipStream_t s, const SellView& sv, const double* gx, const d
                     const double* normal_soa, int npad, double rho, double dt, double* b,
                     const int* bprob, const double* prm) {
     const int g = (sv.nv + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL(ppe_rhs_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, uh, vh, wh,
+    hipLaunchKernelGGL(ppe_rhs_kernel<DIM>, dim3(grid_xcd(g)), dim3(BLOCK), 0, s, sv, gx, gy, gz, uh, vh, wh,
                        uhpk, u, v, w, p_old, bnd, pdir, normal_soa, npad, rho, dt, b, bprob, prm);
     MP_HIP(hipGetLastError());
 }
@@ -443,7 +458,7 @@ void launch_corrector(hipStream_t s, const SellView& sv, const double* gx, const
                       double dt, const int* bprob, const double* prm) {
     (void)npad;
     const int g = (sv.nv + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL(corrector_kernel<DIM>, dim3(g), dim3(BLOCK), 0, s, sv, gx, gy, gz, p, uh,
+    hipLaunchKernelGGL(corrector_kernel<DIM>, dim3(grid_xcd(g)), dim3(BLOCK), 0, s, sv, gx, gy, gz, p, uh,
                        vh, wh, un, vn, wn, rho, dt, bprob, prm);
     MP_HIP(hipGetLastError());
 }
@@ -474,8 +489,8 @@ template <int DIM>
 void launch_divergence(hipStream_t s, const SellView& sv, const double* gx, const double* gy,
                        const double* gz, const double* u, const double* v, const double* w,
                        const uint8_t* bnd, double* partial, int nblocks) {
-    hipLaunchKernelGGL(divergence_kernel<DIM>, dim3(nblocks), dim3(BLOCK), 0, s, sv, gx, gy, gz, u,
-                       v, w, bnd, partial);
+    hipLaunchKernelGGL(divergence_kernel<DIM>, dim3(grid_xcd(nblocks)), dim3(BLOCK), 0, s, sv, gx, gy, gz, u,
+                       v, w, bnd, partial, nblocks);
     MP_HIP(hipGetLastError());
 }
 

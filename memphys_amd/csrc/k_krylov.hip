@@ -1191,12 +1191,16 @@ void PpeSolver::setup(hipStream_t st, const SellView& view, const std::vector<Se
 
 double PpeSolver::smooth_bytes() const {
     // per entry: BF16 value + column (2 B in slices with 16-bit columns, plus
-    // their window bases, else 4 B); per row: the gathered FP32 input, b read
-    // (FP64), D^-1 (FP32), the output written in FP64 and FP32, length
+    // their window bases, else 4 B); per row: the gathered FP32 input, b read,
+    // D^-1 (FP32), the output, length.  GMRES: b is the FP32 basis vector and
+    // the output the FP32 direction; BiCGSTAB: b FP64, the output in FP64 and
+    // FP32
     if (!amg) return 0.0;
     const double n16 = (double)nnz16;
+    const double per_row = krylov == 1 ? 4.0 + 4.0 + 4.0 + 4.0 + 4.0
+                                       : 4.0 + 8.0 + 4.0 + 8.0 + 4.0 + 4.0;
     return (double)nnz * 2.0 + 2.0 * n16 + 4.0 * ((double)nnz - n16) +
-           (double)nslices16 * 4.0 * CW_COUNT + (double)nv * (4.0 + 8.0 + 4.0 + 8.0 + 4.0 + 4.0);
+           (double)nslices16 * 4.0 * CW_COUNT + (double)nv * per_row;
 }
 
 double PpeSolver::spmv_bytes(int fp32) const {
