@@ -809,7 +809,8 @@ __device__ __forceinline__ double saddle_entry(int i, int j, int k, int dim, int
 // FP32 factors and adds it in FP64.  Stops once the correction reaches the
 // FP64 floor; a stencil that has not converged after `max_ref` rounds is
 // flagged for the FP64 kernel.
-__global__ __launch_bounds__(256) void weights_mixed_kernel(
+constexpr int MIX_THREADS = 256;  // four wavefronts per stencil (one: 8 per CU by LDS, 43 % slower)
+__global__ __launch_bounds__(MIX_THREADS) void weights_mixed_kernel(
     int dim, int phs, int m, const int* __restrict__ exps, SellView sv,
     const double* __restrict__ xyz, int npad, double* __restrict__ gx, double* __restrict__ gy,
     double* __restrict__ gz, double* __restrict__ lap, const int* __restrict__ rows,
@@ -857,10 +858,9 @@ __global__ __launch_bounds__(256) void weights_mixed_kernel(
         xs[j * 3 + a] = (xs[j * 3 + a] - s_mn[a]) / s_sc[a];
     }
     __syncthreads();
-    for (int idx = tid; idx < M * M; idx += blockDim.x) {
-        const int i = idx / M, j = idx - i * M;
-        Af[i * LDA + j] = (float)saddle_entry(i, j, k, dim, phs, xs, s_e);
-    }
+    for (int i = tid >> 6; i < M; i += MIX_THREADS / 64)
+        for (int j = tid & 63; j < M; j += 64)
+            Af[i * LDA + j] = (float)saddle_entry(i, j, k, dim, phs, xs, s_e);
     // FP64 right-hand sides (same formulas as the FP64 kernel)
     for (int i = tid; i < M; i += blockDim.x) {
         for (int q = 0; q < 4; ++q) B[i * 4 + q] = 0.0;
@@ -915,11 +915,12 @@ __global__ __launch_bounds__(256) void weights_mixed_kernel(
         }
         for (int i = c + 1 + tid; i < M; i += blockDim.x) Af[i * LDA + c] = Af[i * LDA + c] / d;
         __syncthreads();
-        const int nrow = M - c - 1;
-        for (int idx = tid; idx < nrow * nrow; idx += blockDim.x) {
-            const int ii = idx / nrow;
-            const int i = c + 1 + ii, j = c + 1 + (idx - ii * nrow);
-            Af[i * LDA + j] = fmaf(-Af[i * LDA + c], Af[c * LDA + j], Af[i * LDA + j]);
+        // trailing update: the wavefronts take the rows in turn, the lanes the
+        // columns (no index division; every entry sees the same single fma)
+        for (int i = c + 1 + (tid >> 6); i < M; i += MIX_THREADS / 64) {
+            const float l = Af[i * LDA + c];
+            for (int j = c + 1 + (tid & 63); j < M; j += 64)
+                Af[i * LDA + j] = fmaf(-l, Af[c * LDA + j], Af[i * LDA + j]);
         }
         __syncthreads();
     }
@@ -934,19 +935,20 @@ __global__ __launch_bounds__(256) void weights_mixed_kernel(
     double dprev = 1e300;
     for (int it = 0; it <= max_ref && !singular; ++it) {
         // forward (unit L) and backward (U) substitution on the nr columns of W
+        // (four columns per row, t = 4 i + q; column q >= nr is not used)
         for (int c = 0; c < M; ++c) {
-            for (int t = tid; t < (M - c - 1) * nr; t += blockDim.x) {
-                const int i = c + 1 + t / nr, q = t % nr;
-                W[i * 4 + q] = fmaf(-Af[i * LDA + c], W[c * 4 + q], W[i * 4 + q]);
+            for (int t = tid; t < (M - c - 1) * 4; t += blockDim.x) {
+                const int i = c + 1 + (t >> 2), q = t & 3;
+                if (q < nr) W[i * 4 + q] = fmaf(-Af[i * LDA + c], W[c * 4 + q], W[i * 4 + q]);
             }
             __syncthreads();
         }
         for (int c = M - 1; c >= 0; --c) {
             if (tid < nr) W[c * 4 + tid] = W[c * 4 + tid] / Af[c * LDA + c];
             __syncthreads();
-            for (int t = tid; t < c * nr; t += blockDim.x) {
-                const int i = t / nr, q = t % nr;
-                W[i * 4 + q] = fmaf(-Af[i * LDA + c], W[c * 4 + q], W[i * 4 + q]);
+            for (int t = tid; t < c * 4; t += blockDim.x) {
+                const int i = t >> 2, q = t & 3;
+                if (q < nr) W[i * 4 + q] = fmaf(-Af[i * LDA + c], W[c * 4 + q], W[i * 4 + q]);
             }
             __syncthreads();
         }
@@ -1024,7 +1026,7 @@ void launch_weights_mixed(hipStream_t s, int dim, int phs, int m, const int* exp
     if (lds > 64 * 1024)
         MP_HIP(hipFuncSetAttribute((const void*)weights_mixed_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(weights_mixed_kernel, dim3(nrows), dim3(256), lds, s, dim, phs, m, exps_dev,
+    hipLaunchKernelGGL(weights_mixed_kernel, dim3(nrows), dim3(MIX_THREADS), lds, s, dim, phs, m, exps_dev,
                        sv, xyz_soa, npad, gx, gy, gz, lap, rows, flag, max_ref);
     MP_HIP(hipGetLastError());
 }
