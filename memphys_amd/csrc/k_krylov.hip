@@ -1615,6 +1615,9 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
         if (reset) residual64(st, b_dev, rr0, bb);
     }
     std::vector<int> total(nseg, 0), cycles(nseg, 0), rs_total(nseg, 0);
+    // the last FP64 residual, when it is still that of the current x~ (the
+    // mixed form's defect correction ends with one): the logged residual
+    std::vector<double> rr_last;
     std::vector<int> done(nseg, 0), its(nseg, 0), rst(nseg, 0);
     // FP64 BiCGSTAB from the current x~ for the systems in `todo` (budget:
     // iterations each).  BiCGSTAB stops on its recursively updated residual,
@@ -1771,12 +1774,22 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
             }
         }
         if (anyfb) fp64_solve(fb, budget, true);
+        bool zeroed = false;
         for (int q = 0; q < nseg; ++q)
-            if (bb[q] == 0) zero_system(st, xt.p, q);
+            if (bb[q] == 0) {
+                zero_system(st, xt.p, q);
+                zeroed = true;
+            }
+        // rr holds the residual of x~ as it stands (an undone cycle restored
+        // both), unless the FP64 fallback or a zero right-hand side moved x~
+        if (!anyfb && !zeroed) rr_last = rr;
     }
     // x = D^-1 x~ and the logged residual |A x - b| (solver.cpp:342-345)
     std::vector<double> rr;
-    residual64(st, b_dev, rr, bb);
+    if (!rr_last.empty())
+        rr = rr_last;
+    else
+        residual64(st, b_dev, rr, bb);
     hipLaunchKernelGGL(k_scale, dim3(nbk), dim3(BLOCK), 0, st, nrows, dinv.p, xt.p, x_dev, 0);
     MP_HIP(hipGetLastError());
     std::vector<double> alpha(nseg, 0.0);

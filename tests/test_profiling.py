@@ -85,8 +85,9 @@ def test_bench_reads_committed_traffic_summaries():
     # The bench's `traffic` fields come from the committed rocprofv3 summaries;
     # each must be the measured figure of its own kernel, within 1 % of that
     # kernel's algorithmic bytes per launch (SpMV FP32 279,638,428; FP64
-    # 545,879,136; multigrid level-0 post-smoothing 202,641,996), and absent
-    # on other workloads.
+    # 545,879,136; multigrid level-0 post-smoothing of the GMRES form, which
+    # reads the FP32 basis vector and writes the FP32 direction, 190,656,204),
+    # and absent on other workloads and for the BiCGSTAB form's product.
     import importlib.util
     import os
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -95,11 +96,12 @@ def test_bench_reads_committed_traffic_summaries():
     spec.loader.exec_module(bench)
     for got, alg in [(bench.measured_traffic(100, 0), 279_638_428),
                      (bench.measured_traffic(100, 1), 545_879_136),
-                     (bench.smooth_roofline((381, 0.06, 202_641_996.0, 393), 6000.0,
-                                            100)["traffic"], 202_641_996)]:
+                     (bench.smooth_roofline((381, 0.06, 190_656_204.0, 393), 6000.0,
+                                            100)["traffic"], 190_656_204)]:
         assert got is not None and abs(got / alg - 1) < 0.01
     assert bench.measured_traffic(50, 0) is None
     assert bench.smooth_roofline((381, 0.06, 2e8, 393), 6000.0, 50)["traffic"] is None
+    assert bench.smooth_roofline((381, 0.06, 2e8, 393), 6000.0, 100, gmres=False)["traffic"] is None
 
 
 def test_steps_splits_time_and_idle(tmp_path):
