@@ -578,6 +578,8 @@ def main():
     tflops = flops / (wk[1] * 1e-3) / 1e12
     weights_check = {"fp64_ms": wk[1], "fp32_refined_ms": wkm[1],
                      "gpu_fp64_stencils_per_s": cloud.nv / (wk[1] * 1e-3),
+                     # the four FP64 weight arrays written (8 B per stencil entry each)
+                     "fp64_write_GBps": 4 * 8 * cloud.nnz / (wk[1] * 1e-3) / 1e9,
                      "fp64_roofline": {"bound": "fp64", "achieved": tflops, "peak": 78.6,
                                        "unit": "TFLOP/s", "frac": tflops / 78.6,
                                        "flops_per_launch": flops}}

@@ -98,6 +98,26 @@ struct mp_ctx {
     KernelTimer timers[5];
     double bytes[5] = {0, 0, 0, 0, 0};
     int weights_fp64_rows = 0;  // stencils the last assembly solved in FP64
+
+    // The fixed parts of a timestep (before the solve: packing, predictor,
+    // right-hand side, extrapolated guess; after it: pressure history,
+    // corrector, boundary rows) replayed from hipGraphs.  The key holds the
+    // part, the ping-pong indices, the Euler/AB2 switch and the buffers that
+    // can change between steps; every call that reallocates a buffer the
+    // captured kernels read drops the graphs first (release_step_graphs).
+    struct StepGraph {
+        uintptr_t key[8];
+        hipGraphExec_t exec;
+    };
+    std::vector<StepGraph> step_graphs;
+    void release_step_graphs() {
+        if (step_graphs.empty()) return;
+        (void)hipStreamSynchronize(stream);  // (a replay may still be running)
+        for (StepGraph& g : step_graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        step_graphs.clear();
+    }
+    ~mp_ctx() { release_step_graphs(); }
 };
 
 namespace {
@@ -395,6 +415,7 @@ int upload_clouds(mp_ctx* c, const char* fn, int nprob, const int* nvp, int dim,
                   const double* xyz, const uint8_t* boundary, const double* normal,
                   const int* row_ptr, const int* col_idx) {
     const std::string f(fn);
+    c->release_step_graphs();  // (every buffer of the cloud is replaced)
     if (nprob < 1 || !nvp || (dim != 2 && dim != 3) || !xyz || !boundary || !normal || !row_ptr ||
         !col_idx)
         return fail(c, MP_ERR_ARG, f + ": bad arguments");
@@ -649,6 +670,7 @@ int mp_weights_assemble(mp_ctx* c, int phs_deg, int poly_deg, const int* exps, i
                         double* gx, double* gy, double* gz, double* lap, double* cond_out) {
     return guarded(c, "mp_weights_assemble", [&]() -> int {
         if (!c->has_cloud) return fail(c, MP_ERR_STATE, "mp_weights_assemble: no cloud uploaded");
+        c->release_step_graphs();
         if (phs_deg < 3 || phs_deg > 11 || (phs_deg % 2) == 0)
             return fail(c, MP_ERR_ARG, "mp_weights_assemble: phs_deg must be 3, 5, 7, 9 or 11");
         int m = (c->dim == 2) ? (poly_deg + 1) * (poly_deg + 2) / 2
@@ -750,6 +772,7 @@ int mp_weights_set(mp_ctx* c, const double* gx, const double* gy, const double* 
                    const double* lap) {
     return guarded(c, "mp_weights_set", [&]() -> int {
         if (!c->has_cloud) return fail(c, MP_ERR_STATE, "mp_weights_set: no cloud uploaded");
+        c->release_step_graphs();
         if (!gx || !gy || !lap || (c->dim == 3 && !gz))
             return fail(c, MP_ERR_ARG, "mp_weights_set: missing operator");
         csr_to_slots(c, gx, c->w[0]);
@@ -878,6 +901,7 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
                  const uint8_t* w_dir, const uint8_t* p_dir) {
     return guarded(c, "mp_fs_create", [&]() -> int {
         if (!c->has_weights) return fail(c, MP_ERR_STATE, "mp_fs_create: no weights");
+        c->release_step_graphs();
         if (!prm || !u_dir || !v_dir || !p_dir || (c->dim == 3 && !w_dir))
             return fail(c, MP_ERR_ARG, "mp_fs_create: NULL argument");
         // FRACTIONAL_STEP_1::check_bc, fractional_step_1.cpp:78-102
@@ -986,6 +1010,7 @@ int mp_fields_set(mp_ctx* c, int which, const double* values) {
 int mp_fs_body_force(mp_ctx* c, const double* fx, const double* fy, const double* fz) {
     return guarded(c, "mp_fs_body_force", [&]() -> int {
         if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_fs_body_force: mp_fs_create not called");
+        c->release_step_graphs();
         if (!fx && !fy && !fz) {
             c->has_force = false;
             return MP_OK;
@@ -1030,6 +1055,39 @@ int mp_fields_get(mp_ctx* c, int which, double* values) {
 
 namespace {
 
+// Runs launch() on the context's stream: replayed from the graph cached under
+// key, captured into one on first use; launched directly when graphs are off
+// or the step kernels are being timed (events stay out of graphs).
+template <typename F>
+void run_step_part(mp_ctx* c, const uintptr_t (&key)[8], F&& launch) {
+    if (!c->use_graphs || c->timers[1].enabled || c->timers[2].enabled || c->timers[3].enabled) {
+        launch();
+        return;
+    }
+    for (mp_ctx::StepGraph& g : c->step_graphs)
+        if (std::memcmp(g.key, key, sizeof(key)) == 0) {
+            MP_HIP(hipGraphLaunch(g.exec, c->stream));
+            return;
+        }
+    hipGraph_t graph = nullptr;
+    MP_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        launch();
+    } catch (...) {
+        (void)hipStreamEndCapture(c->stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    MP_HIP(hipStreamEndCapture(c->stream, &graph));
+    mp_ctx::StepGraph g{};
+    std::memcpy(g.key, key, sizeof(key));
+    const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MP_HIP(e);
+    c->step_graphs.push_back(g);
+    MP_HIP(hipGraphLaunch(g.exec, c->stream));
+}
+
 template <int DIM>
 void fs_step_one(mp_ctx* c, int it) {
     const mp_fs_params& q = c->prm;
@@ -1039,28 +1097,37 @@ void fs_step_one(mp_ctx* c, int it) {
     const int euler = (q.temporal_order == 1 || it == 0) ? 1 : 0;
     double* S_new = c->S[c->s_cur ^ 1].p;
     const double* S_oo = c->S[c->s_cur].p;
-    // calc_vel_hat
-    c->timers[1].begin(c->stream);
     const int* bp = c->per_problem_prm ? c->blk_prob.p : nullptr;
     const double* pp = c->per_problem_prm ? c->prob_prm.p : nullptr;
-    launch_predictor<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p, Uo[0].p,
-                          Uo[1].p, Uo[2].p, c->uh[0].p, c->uh[1].p, c->uh[2].p, S_new, S_oo,
-                          c->npad, q.rho, q.mu, q.dt, euler, c->upk.p, c->uhpk.p, bp, pp,
-                          c->has_force ? c->force.p : nullptr);
-    c->timers[1].end(c->stream);
-    // calc_pressure
-    c->timers[2].begin(c->stream);
-    launch_ppe_rhs<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->uh[0].p, c->uh[1].p,
-                        c->uh[2].p, c->uhpk.p, Uo[0].p, Uo[1].p, Uo[2].p, c->P.p, c->bnd.p, c->pdir.p,
-                        c->normal.p, c->npad, q.rho, q.dt, c->b.p, bp, pp);
-    c->timers[2].end(c->stream);
+    const double* force = c->has_force ? c->force.p : nullptr;
     const int n_sys = c->ppe->n_sys;  // single problem: border unknown at row nv
     const int nv = c->nv;
+    const bool extrapolate = c->guess_mode != MP_GUESS_PREVIOUS && c->hist >= 2;
+    const uintptr_t key_pre[8] = {1, (uintptr_t)o, (uintptr_t)c->s_cur, (uintptr_t)euler,
+                                  (uintptr_t)force, (uintptr_t)extrapolate, (uintptr_t)bp,
+                                  (uintptr_t)pp};
+    run_step_part(c, key_pre, [&]() {
+        // calc_vel_hat
+        c->timers[1].begin(c->stream);
+        launch_predictor<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->w[3].p,
+                              Uo[0].p, Uo[1].p, Uo[2].p, c->uh[0].p, c->uh[1].p, c->uh[2].p, S_new,
+                              S_oo, c->npad, q.rho, q.mu, q.dt, euler, c->upk.p, c->uhpk.p, bp, pp,
+                              force);
+        c->timers[1].end(c->stream);
+        // calc_pressure
+        c->timers[2].begin(c->stream);
+        launch_ppe_rhs<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->uh[0].p,
+                            c->uh[1].p, c->uh[2].p, c->uhpk.p, Uo[0].p, Uo[1].p, Uo[2].p, c->P.p,
+                            c->bnd.p, c->pdir.p, c->normal.p, c->npad, q.rho, q.dt, c->b.p, bp, pp);
+        c->timers[2].end(c->stream);
+        if (extrapolate) {
+            hipLaunchKernelGGL(k_extrapolate, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream,
+                               nv, n_sys, c->P1.p, c->P2.p, c->P.p);
+            MP_HIP(hipGetLastError());
+        }
+    });
     bool learned = false, skipped = false;
-    if (c->guess_mode != MP_GUESS_PREVIOUS && c->hist >= 2) {
-        hipLaunchKernelGGL(k_extrapolate, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, nv,
-                           n_sys, c->P1.p, c->P2.p, c->P.p);
-        MP_HIP(hipGetLastError());
+    if (extrapolate) {
         if (c->guess_mode == MP_GUESS_LEARNED && !c->train && c->guess_skip > 0) {
             --c->guess_skip;  // backing off: the extrapolation alone
             skipped = true;
@@ -1096,22 +1163,26 @@ void fs_step_one(mp_ctx* c, int it) {
         c->net->adam_step(c->stream);
         c->last_loss = c->net->loss_host(c->stream);
     }
-    // pressure history for the extrapolated / learned guesses
-    hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad, c->P1.p,
-                       c->P2.p);
-    hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad, c->P.p,
-                       c->P1.p);
-    MP_HIP(hipGetLastError());
+    const uintptr_t key_post[8] = {2, (uintptr_t)o, (uintptr_t)bp, (uintptr_t)pp, 0, 0, 0, 0};
+    run_step_part(c, key_post, [&]() {
+        // pressure history for the extrapolated / learned guesses
+        hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad, c->P1.p,
+                           c->P2.p);
+        hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad, c->P.p,
+                           c->P1.p);
+        MP_HIP(hipGetLastError());
+        // calc_vel_corr
+        c->timers[3].begin(c->stream);
+        launch_corrector<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->P.p,
+                              c->uh[0].p, c->uh[1].p, c->uh[2].p, Un[0].p, Un[1].p, Un[2].p,
+                              c->npad, q.rho, q.dt, bp, pp);
+        c->timers[3].end(c->stream);
+        launch_boundary_fix<DIM>(c->stream, c->sv, c->brows.p, c->nb, c->w[0].p, c->w[1].p,
+                                 c->w[2].p, c->normal.p, c->npad, c->pdir.p, c->udir[0].p,
+                                 c->udir[1].p, c->udir[2].p, Uo[0].p, Uo[1].p, Uo[2].p, Un[0].p,
+                                 Un[1].p, Un[2].p);
+    });
     c->hist = std::min(c->hist + 1, 2);
-    // calc_vel_corr
-    c->timers[3].begin(c->stream);
-    launch_corrector<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->P.p, c->uh[0].p,
-                          c->uh[1].p, c->uh[2].p, Un[0].p, Un[1].p, Un[2].p, c->npad, q.rho, q.dt,
-                          bp, pp);
-    c->timers[3].end(c->stream);
-    launch_boundary_fix<DIM>(c->stream, c->sv, c->brows.p, c->nb, c->w[0].p, c->w[1].p, c->w[2].p,
-                             c->normal.p, c->npad, c->pdir.p, c->udir[0].p, c->udir[1].p,
-                             c->udir[2].p, Uo[0].p, Uo[1].p, Uo[2].p, Un[0].p, Un[1].p, Un[2].p);
     // extras (AB2 history) and the driver's old <- new
     if (q.temporal_order == 2) c->s_cur ^= 1;
     c->cur = nw;
@@ -1310,6 +1381,7 @@ int mp_divergence_noalloc(mp_ctx* c, double* out_dev) {
 int mp_batch_params(mp_ctx* c, const double* rho, const double* mu, const double* dt) {
     return guarded(c, "mp_batch_params", [&]() -> int {
         if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_batch_params: mp_fs_create not called");
+        c->release_step_graphs();
         if (!rho || !mu || !dt) return fail(c, MP_ERR_ARG, "mp_batch_params: NULL");
         std::vector<double> h((size_t)3 * c->nprob);
         for (int q = 0; q < c->nprob; ++q) {
@@ -1404,6 +1476,7 @@ int mp_precond_info(mp_ctx* c, int max_levels, int* levels, int* rows, long* nnz
 int mp_graph_mode(mp_ctx* c, int enable) {
     return guarded(c, "mp_graph_mode", [&]() -> int {
         c->use_graphs = enable != 0;
+        c->release_step_graphs();
         for (PpeSolver* s : {c->solver.get(), c->ppe.get()})
             if (s) s->use_graphs = c->use_graphs;
         return MP_OK;

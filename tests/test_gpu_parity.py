@@ -285,6 +285,44 @@ def test_graph_replay_matches_plain_launches(ctx, cube3d_small, precision, preco
         assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
+def test_step_graphs_follow_guess_and_force_changes(ctx, cube3d_small):
+    """The fixed parts of a step (predictor, right-hand side, extrapolated
+    guess; history, corrector, boundary rows) are replayed from graphs keyed by
+    the ping-pong buffers, the Euler/AB2 switch, the guess and the body force:
+    a run that switches the guess on, sets a body force and clears it again
+    gives the same fields with graphs as with plain launches."""
+    import memphys_amd as M
+    cloud = cube3d_small
+    ctx.upload(cloud)
+    ctx.weights(3, 3, download=False)
+    u_dir, p_dir, u0 = lid_bc(cloud, 2)
+    g = np.sin(3.0 * cloud.xyz[:, 0]) * 0.2
+    out = []
+    for graphs in (True, False):
+        ctx.graph_mode(graphs)
+        ctx.fs_create(1.0, 0.0025, 5e-4, 2, 1e-10, 20000, u_dir, u_dir, u_dir, p_dir,
+                      precision=M.MP_PREC_MIXED, precond=M.MP_PRECOND_AMG,
+                      krylov=M.MP_KRYLOV_GMRES)
+        ctx.set_field(M.MP_U, u0)
+        for w in (M.MP_V, M.MP_W):
+            ctx.set_field(w, np.zeros(cloud.nv))
+        ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
+        ctx.guess_mode(M.MP_GUESS_PREVIOUS)
+        ctx.step(2, 0)
+        ctx.guess_mode(M.MP_GUESS_EXTRAPOLATE)
+        ctx.step(2, 2)
+        ctx.body_force(g, np.zeros(cloud.nv), -g)
+        ctx.step(2, 4)
+        ctx.body_force()
+        ctx.step(2, 6)
+        out.append([ctx.get_field(w) for w in (M.MP_U, M.MP_V, M.MP_W, M.MP_P)])
+        _print(f"graphs {graphs}: total iterations", ctx.stats().total_iterations)
+    ctx.graph_mode(True)
+    ctx.guess_mode(M.MP_GUESS_PREVIOUS)
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_condition_estimate_matches_oracle(ctx, cube3d_small):
     """cond = ||A||_1 * Hager/Higham estimate of ||A^-1||_1 (the estimator behind
     Eigen's rcond, coefficient_computations.cpp:312), same arithmetic order."""
