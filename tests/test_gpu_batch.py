@@ -112,8 +112,8 @@ def test_batch_problem_matches_oracle():
     assert max(errs) <= 1e-8
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_batch_multigrid_matches_single_and_oracle(precision):
+@pytest.mark.parametrize("precision,krylov", [(0, 0), (1, 0), (1, 1)])
+def test_batch_multigrid_matches_single_and_oracle(precision, krylov):
     """One multigrid hierarchy per cavity, stored block-diagonally: each
     cavity's fields match its stand-alone multigrid run and the oracle (the
     hierarchies of a batch share one depth, so they may differ from the
@@ -124,14 +124,15 @@ def test_batch_multigrid_matches_single_and_oracle(precision):
     import oracle as O
     probs = _problems()
     steps = 4
-    batch = _run_batch(probs, steps, precision, precond=M.MP_PRECOND_AMG, tol=1e-12)
+    batch = _run_batch(probs, steps, precision, precond=M.MP_PRECOND_AMG, krylov=krylov,
+                       tol=1e-12)
     jac = _run_batch(probs, steps, precision, tol=1e-12)
     for q, (cloud, re, dt) in enumerate(probs):
         fields, iters = _run_single(cloud, re, dt, steps, precision, precond=M.MP_PRECOND_AMG,
-                                    tol=1e-12)
+                                    krylov=krylov, tol=1e-12)
         bf, bi = batch[q]
-        _print(f"precision {precision} problem {q}: multigrid iterations batch {bi} "
-               f"single {iters}, Jacobi batch {jac[q][1]}")
+        _print(f"precision {precision} krylov {krylov} problem {q}: multigrid iterations "
+               f"batch {bi} single {iters}, Jacobi batch {jac[q][1]}")
         errs = [rel_err(b, a) for a, b in zip(fields, bf)]
         assert max(errs) <= 1e-8, errs
         assert sum(bi) < sum(jac[q][1])

@@ -230,7 +230,8 @@ def test_fs_2d_cavity_multigrid_matches_oracle(ctx, cavity2d):
     _fs_parity(ctx, cavity2d, 10, 1, 1.0, 0.01, 2e-4, M.MP_PREC_MIXED, M.MP_PRECOND_AMG)
 
 
-@pytest.mark.parametrize("precision,precond,restart", [(0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 1, 4)])
+@pytest.mark.parametrize("precision,precond,restart",
+                         [(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (1, 1, 4)])
 def test_fs_3d_cube_gmres_matches_oracle(ctx, cube3d_small, precision, precond, restart):
     """Restarted GMRES (FP64 or FP32-inner, Jacobi or multigrid, the default
     30 columns or 4, which restarts many times) reaches the oracle's fields."""
