@@ -103,7 +103,7 @@ int mp_sync(mp_ctx* ctx);
 /* Point cloud and stencils (replaces the CLOUD data handed to the weight loop,
  * cloud.cpp:207-279; POINTS xyz/normal/boundary_flag, class.hpp:99-124).
  * xyz and normal: nv*dim AoS; boundary: nv bytes; CSR row_ptr[nv+1], col[...];
- * entry 0 of every row must be the node itself. */
+ * entry 0 of every row must be the node itself; rows of 1..512 entries. */
 int mp_cloud_upload(mp_ctx* ctx, int nv, int dim, const double* xyz, const uint8_t* boundary,
                     const double* normal, const int* row_ptr, const int* col_idx);
 
@@ -128,7 +128,10 @@ int mp_batch_upload(mp_ctx* ctx, int nprob, const int* nv_per_problem, int dim, 
  * precision_mode: MP_PREC_FP64, or MP_PREC_MIXED (FP32 LU + FP64 refinement, FP64
  * fallback per stencil).  Output arrays (CSR layout of the upload, may be NULL
  * to keep the weights device-resident): gx, gy, gz (3D), lap; cond_out[nv]
- * (may be NULL) receives a 1-norm condition estimate of each stencil matrix. */
+ * (may be NULL) receives a 1-norm condition estimate of each stencil matrix.
+ * poly_deg: any degree with m <= 136 monomials (2D up to degree 15, the reference's
+ * range; 3D up to degree 7); systems of M = k + m too large for LDS are solved from a global
+ * work area, more slowly, and the mixed mode solves them in FP64. */
 int mp_weights_assemble(mp_ctx* ctx, int phs_deg, int poly_deg, const int* exps,
                         int precision_mode, double* gx, double* gy, double* gz, double* lap,
                         double* cond_out);
@@ -144,7 +147,7 @@ int mp_operator_apply(mp_ctx* ctx, int op, const double* x, double* y);
  * cloud.cpp:570-648): an interior node gets its k nearest nodes (itself
  * first), a boundary node itself and its k-1 nearest interior nodes; each
  * row is ordered by (squared distance, index), exact ties going to the lower
- * index.  Host arrays: xyz nv*dim (AoS), boundary nv, k_per_node nv (1..96).
+ * index.  Host arrays: xyz nv*dim (AoS), boundary nv, k_per_node nv (1..512).
  * Output CSR in host memory: row_ptr nv+1 (filled from k_per_node) and col
  * (row_ptr[nv] entries).  Needs no uploaded cloud; leaves the context's
  * cloud untouched. */

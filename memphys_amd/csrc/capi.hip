@@ -716,10 +716,13 @@ int mp_weights_assemble(mp_ctx* c, int phs_deg, int poly_deg, const int* exps, i
             for (int o = 0; o < c->nv; ++o)
                 kmin = std::min(kmin, c->h_rowptr[o + 1] - c->h_rowptr[o]);
             std::vector<int> mixed_rows, fp64_rows;
+            // (high polynomial degrees, or systems the mixed kernel cannot hold
+            // in LDS: everything in FP64)
+            const bool mixed_ok = m <= WAVE_MAX_POLY && weights_mixed_lds(kmin, m) <= WEIGHTS_LDS_MAX;
             for (int d = 0; d < c->npad; ++d) {
                 if (c->perm[d] < 0) continue;
                 const int len = c->h_rowptr[c->perm[d] + 1] - c->h_rowptr[c->perm[d]];
-                (len == kmin ? mixed_rows : fp64_rows).push_back(d);
+                (mixed_ok && len == kmin ? mixed_rows : fp64_rows).push_back(d);
             }
             DevBuf<int> dm, dflag;
             dm.alloc(mixed_rows.size());

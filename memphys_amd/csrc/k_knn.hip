@@ -67,6 +67,8 @@ __device__ __forceinline__ bool key_less(double da, int ia, double db, int ib) {
 }
 
 // One node per block of one wavefront (so that __syncthreads is a wave barrier).
+// KCAP: capacity of the sorted candidate lists (the largest stencil).
+template <int KCAP>
 __global__ __launch_bounds__(64) void k_knn(int nv, const double* __restrict__ xyz,
                                             const uint8_t* __restrict__ bnd,
                                             const int* __restrict__ kpn,
@@ -74,8 +76,8 @@ __global__ __launch_bounds__(64) void k_knn(int nv, const double* __restrict__ x
                                             const int* __restrict__ cstart,
                                             const int* __restrict__ items,
                                             int* __restrict__ col_out) {
-    __shared__ double td[2][MAX_STENCIL];
-    __shared__ int ti[2][MAX_STENCIL];
+    __shared__ double td[2][KCAP];
+    __shared__ int ti[2][KCAP];
     __shared__ double cd[64];
     __shared__ int cix[64];
     __shared__ int cell_first[64];
@@ -310,8 +312,13 @@ void knn_stencils(hipStream_t st, int nv, int dim, const double* xyz, const uint
     hipLaunchKernelGGL(k_cell_fill, dim3(nb), dim3(BLOCK), 0, st, nv, cid.p, fill.p, items.p);
     MP_HIP(hipGetLastError());
     MP_HIP(hipMemsetAsync(dcol.p, 0xFF, sizeof(int) * row_ptr[nv], st));  // -1: unfilled
-    hipLaunchKernelGGL(k_knn, dim3(nv), dim3(64), 0, st, nv, dxyz.p, dbnd.p, dk.p, drp.p, g,
-                       cstart.p, items.p, dcol.p);
+    // (the small lists keep the common stencil sizes at full occupancy)
+    if (kmax <= 96)
+        hipLaunchKernelGGL(k_knn<96>, dim3(nv), dim3(64), 0, st, nv, dxyz.p, dbnd.p, dk.p, drp.p,
+                           g, cstart.p, items.p, dcol.p);
+    else
+        hipLaunchKernelGGL(k_knn<MAX_STENCIL>, dim3(nv), dim3(64), 0, st, nv, dxyz.p, dbnd.p,
+                           dk.p, drp.p, g, cstart.p, items.p, dcol.p);
     MP_HIP(hipGetLastError());
     MP_HIP(hipMemcpyAsync(col, dcol.p, sizeof(int) * row_ptr[nv], hipMemcpyDeviceToHost, st));
     MP_HIP(hipStreamSynchronize(st));

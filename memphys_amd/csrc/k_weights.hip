@@ -125,22 +125,27 @@ __global__ __launch_bounds__(256) void weights_fp64_kernel(
     int dim, int phs, int m, const int* __restrict__ exps, SellView sv,
     const double* __restrict__ xyz, int npad, double* __restrict__ gx, double* __restrict__ gy,
     double* __restrict__ gz, double* __restrict__ lap, double* __restrict__ cond,
-    const int* __restrict__ rows, const int* __restrict__ only) {
+    const int* __restrict__ rows, const int* __restrict__ only, int t0, double* gwork,
+    long gstride) {
     extern __shared__ double lds[];
     __shared__ double s_mn[3], s_sc[3];
     __shared__ double s_anorm, s_red[4];
     __shared__ int s_piv, s_jm;
     __shared__ int s_e[MAX_POLY * 3];
 
-    if (only && !only[blockIdx.x]) return;  // a stencil the register kernel finished
-    const int r = rows ? rows[blockIdx.x] : (int)blockIdx.x;
+    // stencil t of the list (or row t); the work arrays in LDS, or for
+    // systems too large for it in this block's share of a global work area
+    // (the same operations in the same order either way)
+    const int t = t0 + (int)blockIdx.x;
+    if (only && !only[t]) return;  // a stencil the register kernel finished
+    const int r = rows ? rows[t] : t;
     if (r >= sv.nv || sv.len[r] == 0) return;  // padding rows carry no stencil
     const int tid = threadIdx.x;
     const int k = sv.len[r];
     const int M = k + m;
     const int LDA = M + 1;
     const int nr = dim + 1;
-    double* A = lds;
+    double* A = gwork ? gwork + (size_t)blockIdx.x * (size_t)gstride : lds;
     double* R = A + M * LDA;
     double* xs = R + M * 4;
     const int base = sell_base(sv.sptr, r);
@@ -627,7 +632,7 @@ __global__ __launch_bounds__(64 * NW) void weights_wave_kernel(
     __shared__ double s_sc[3], s_mn[3];
     __shared__ double s_cv[NW];
     __shared__ int s_cp[NW], s_cn[NW];
-    __shared__ int s_e[MAX_POLY * 3];
+    __shared__ int s_e[WAVE_MAX_POLY * 3];
     __shared__ double s_zero[AW + 1];
 
     const int r = rows[blockIdx.x];
@@ -820,7 +825,7 @@ __global__ __launch_bounds__(MIX_THREADS) void weights_mixed_kernel(
     __shared__ float s_bestf;
     __shared__ int s_piv, s_conv;
     __shared__ double s_red[2];
-    __shared__ int s_e[MAX_POLY * 3];
+    __shared__ int s_e[WAVE_MAX_POLY * 3];
     const int r = rows[blockIdx.x];
     const int tid = threadIdx.x;
     const int k = sv.len[r];
@@ -1015,14 +1020,20 @@ __global__ __launch_bounds__(MIX_THREADS) void weights_mixed_kernel(
 
 }  // namespace
 
+size_t weights_mixed_lds(int max_len, int m) {
+    const int M = max_len + m;
+    return sizeof(double) * ((size_t)M * 8 + (size_t)max_len * 3) +
+           sizeof(float) * ((size_t)M * (M + 1) + (size_t)M * 4) + sizeof(int) * M;
+}
+
 void launch_weights_mixed(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
                           const SellView& sv, const double* xyz_soa, int npad, double* gx,
                           double* gy, double* gz, double* lap, const int* rows, int nrows,
                           int* flag, int max_len, int max_ref) {
     if (nrows == 0) return;
-    const int M = max_len + m;
-    const size_t lds = sizeof(double) * ((size_t)M * 8 + (size_t)max_len * 3) +
-                       sizeof(float) * ((size_t)M * (M + 1) + (size_t)M * 4) + sizeof(int) * M;
+    if (m > WAVE_MAX_POLY || weights_mixed_lds(max_len, m) > WEIGHTS_LDS_MAX)
+        throw std::runtime_error("launch_weights_mixed: system too large for the mixed kernel");
+    const size_t lds = weights_mixed_lds(max_len, m);
     if (lds > 64 * 1024)
         MP_HIP(hipFuncSetAttribute((const void*)weights_mixed_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1036,14 +1047,34 @@ void launch_weights_fp64(hipStream_t s, int dim, int phs, int m, const int* exps
                          double* gy, double* gz, double* lap, double* cond, int max_len,
                          const int* rows, int nrows, const int* only) {
     if (rows && nrows == 0) return;
+    const int n = rows ? nrows : sv.nv;
     const int M = max_len + m;
-    const size_t lds = sizeof(double) * ((size_t)M * (M + 1) + (size_t)M * 4 + (size_t)max_len * 3);
-    if (lds > 64 * 1024)
-        MP_HIP(hipFuncSetAttribute((const void*)weights_fp64_kernel,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(weights_fp64_kernel, dim3(rows ? nrows : sv.nv), dim3(256), lds, s, dim,
-                       phs, m, exps_dev, sv, xyz_soa, npad, gx, gy, gz, lap, cond, rows, only);
-    MP_HIP(hipGetLastError());
+    const size_t doubles = (size_t)M * (M + 1) + (size_t)M * 4 + (size_t)max_len * 3;
+    const size_t lds = sizeof(double) * doubles;
+    if (lds <= WEIGHTS_LDS_MAX) {
+        if (lds > 64 * 1024)
+            MP_HIP(hipFuncSetAttribute((const void*)weights_fp64_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(weights_fp64_kernel, dim3(n), dim3(256), lds, s, dim, phs, m, exps_dev,
+                           sv, xyz_soa, npad, gx, gy, gz, lap, cond, rows, only, 0,
+                           (double*)nullptr, 0L);
+        MP_HIP(hipGetLastError());
+        return;
+    }
+    // systems too large for LDS (high polynomial degrees): a global work area
+    // of `doubles` per block, in launches of at most 256 MiB of it
+    const long stride = (long)((doubles + 31) & ~(size_t)31);
+    const int chunk = (int)std::max<long>(1, std::min<long>(n, (256L << 20) / 8 / stride));
+    DevBuf<double> work;
+    work.alloc((size_t)chunk * (size_t)stride);
+    for (int t0 = 0; t0 < n; t0 += chunk) {
+        const int cnt = std::min(chunk, n - t0);
+        hipLaunchKernelGGL(weights_fp64_kernel, dim3(cnt), dim3(256), 0, s, dim, phs, m, exps_dev,
+                           sv, xyz_soa, npad, gx, gy, gz, lap, cond, rows, only, t0, work.p,
+                           stride);
+        MP_HIP(hipGetLastError());
+    }
+    MP_HIP(hipStreamSynchronize(s));  // (the work area is freed on return)
 }
 
 
@@ -1072,6 +1103,7 @@ void launch_weights_fp64_rows(hipStream_t s, int dim, int phs, int m, const int*
         const int M = lens[t] + m;
         int b = 0;
         while (b < WAVE_CLASSES && M > cls[b]) ++b;
+        if (m > WAVE_MAX_POLY) b = WAVE_CLASSES;  // (the register kernels' monomial table)
         bucket[b].push_back(rows[t]);
         if (b == WAVE_CLASSES) max_long = std::max(max_long, lens[t]);
     }

@@ -14,8 +14,17 @@ constexpr int SLICE = 64;       // rows per SELL slice (one wavefront)
 constexpr int SGROUP = 4;       // neighbours of a row stored side by side in a slice
 constexpr int SIGMA = 4096;     // rows per length-sorting window
 constexpr int BLOCK = 256;      // threads per block of the row kernels
-constexpr int MAX_STENCIL = 96; // largest stencil (k) the weight kernel accepts
-constexpr int MAX_POLY = 35;    // largest monomial count (degree 4 in 3D)
+// Largest stencil (k) and monomial count (m) accepted: the reference's
+// parameter range (poly_deg up to 15, parameters.cpp:84-88, 122-130) gives
+// m = 136 in 2D; stencils of M = k + m above what LDS holds run from a global
+// work area.  The register-resident weights kernels and the mixed kernel see
+// m <= WAVE_MAX_POLY only (their stencils have M <= 96, and M >= 3m).
+constexpr int MAX_STENCIL = 512;
+constexpr int MAX_POLY = 136;
+constexpr int WAVE_MAX_POLY = 35;
+// dynamic LDS the weights kernel's per-stencil system may use (of 160 KB per CU,
+// leaving room for its static arrays)
+constexpr size_t WEIGHTS_LDS_MAX = 150 * 1024;
 
 struct HipError {
     hipError_t code;
@@ -270,6 +279,8 @@ void launch_weights_fp64(hipStream_t s, int dim, int phs, int m, const int* exps
                          const SellView& sv, const double* xyz_soa, int npad, double* gx,
                          double* gy, double* gz, double* lap, double* cond, int max_len,
                          const int* rows = nullptr, int nrows = 0, const int* only = nullptr);
+// Dynamic LDS of the mixed kernel for stencils of up to max_len points.
+size_t weights_mixed_lds(int max_len, int m);
 // FP32 LU + FP64 refinement for the listed rows; flag[r] = 1 where it did not converge.
 void launch_weights_mixed(hipStream_t s, int dim, int phs, int m, const int* exps_dev,
                           const SellView& sv, const double* xyz_soa, int npad, double* gx,

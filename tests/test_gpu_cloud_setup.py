@@ -50,6 +50,8 @@ def _compare(ctx, name, xyz, bnd, ks, brute=False):
     (3, 16, 0.3, 40, (60, 55, 50)),
     (2, 30, 0.0, 30, (None, None, None)),   # lattice: exact distance ties
     (3, 12, 0.0, 40, (60, 55, 50)),
+    (3, 8, 0.25, 112, (None, None, None)),  # stencils above 96: the larger candidate lists
+    (2, 20, 0.0, 108, (None, None, None)),
 ])
 def test_knn_matches_host(dim, n, jitter, k, layers):
     import memphys_amd as M
@@ -100,7 +102,7 @@ def test_knn_rejects_bad_sizes():
     c = M.box_cloud(dim=2, n=10, jitter=0.2, seed=1, k_interior=12, rcm=False)
     with M.Context(0) as ctx:
         with pytest.raises(M.MemphysError):
-            ctx.knn_stencils(c.xyz, c.boundary, np.full(c.nv, 97, dtype=np.int32))
+            ctx.knn_stencils(c.xyz, c.boundary, np.full(c.nv, 513, dtype=np.int32))
         with pytest.raises(M.MemphysError):
             ctx.knn_stencils(c.xyz, c.boundary, np.full(c.nv, c.nv + 1, dtype=np.int32))
         with pytest.raises(M.MemphysError):

@@ -3,7 +3,8 @@
 The FP64 weights run in a register-resident kernel: one wavefront per stencil
 for saddle-point systems of M = k + m <= 64 (classes M <= 32, 48, 64), two
 wavefronts up to M <= 96 (classes 80, 96), and the LDS kernel beyond that and
-whenever condition estimates are requested.  Each class, an unjittered
+whenever condition estimates are requested; systems too large for LDS (high
+polynomial degrees) run the same kernel on a global work area.  Each class, an unjittered
 lattice (exact ties in the pivot search), other PHS and polynomial degrees
 and a singular stencil (two coincident nodes: a zero pivot, weights inf/NaN)
 must give the oracle's weights exactly: the same roundings in the same order
@@ -41,6 +42,14 @@ CASES = [
     ("3D k=76, M=96", dict(dim=3, n=9, jitter=0.3, seed=6, k_interior=76), 3, 3),
     ("3D k=80, M=100 (LDS kernel)", dict(dim=3, n=9, jitter=0.3, seed=6, k_interior=80), 3, 3),
     ("3D k=30 PHS7 deg2, M=40", dict(dim=3, n=9, jitter=0.3, seed=2, k_interior=30), 7, 2),
+    # high polynomial degrees (the reference accepts up to 15): the LDS kernel,
+    # and above its LDS capacity the same kernel on a global work area
+    ("3D k=70 deg4, M=105 (LDS kernel)", dict(dim=3, n=9, jitter=0.25, seed=3, k_interior=70),
+     3, 4),
+    ("3D k=112 deg5, M=168 (global work area)",
+     dict(dim=3, n=8, jitter=0.25, seed=3, k_interior=112), 3, 5),
+    ("2D k=108 deg7, M=144 (global work area)",
+     dict(dim=2, n=20, jitter=0.25, seed=3, k_interior=108), 3, 7),
 ]
 
 
@@ -96,6 +105,20 @@ def test_singular_stencil_matches_oracle(ctx):
                   if {i, j} <= set(cloud.col[cloud.row_ptr[r]:cloud.row_ptr[r + 1]].tolist())]
     _print("rows with non-finite weights", len(bad), "rows holding both nodes", len(holds_both))
     assert bad and set(bad) <= set(holds_both)
+
+
+def test_mixed_weights_at_high_degree_run_in_fp64(ctx):
+    """A mixed-precision request whose systems the FP32 kernel cannot take
+    (degree 5 in 3D: 56 monomials, M = 168) is solved in FP64 throughout:
+    bit-identical to the oracle."""
+    import memphys_amd as M
+    cloud = M.box_cloud(dim=3, n=8, jitter=0.25, seed=3, k_interior=112)
+    ctx.upload(cloud)
+    (gx, gy, gz, lap), _ = ctx.weights(3, 5, precision=M.MP_PREC_MIXED)
+    import oracle as O
+    ox, oy, oz, ol, _ = O.stencil_weights(cloud, 3, 5)
+    for g, r in ((gx, ox), (gy, oy), (gz, oz), (lap, ol)):
+        assert _same(g, r)
 
 
 def test_mixed_weights_remainder_uses_same_kernel(ctx):
