@@ -267,6 +267,17 @@ int mp_autotune(mp_ctx* ctx, int* chosen, double* ms_per_candidate);
 int mp_precond_info(mp_ctx* ctx, int max_levels, int* levels, int* rows, long* nnz,
                     double* setup_seconds);
 
+/* One application of the pressure solve's multigrid preconditioner (the FP32
+ * V(1,1)-cycle of MP_PRECOND_AMG): out = M in, both of the system's size n
+ * (nv node rows in caller order, then the border unknowns, which pass
+ * through).  cycle 0:
+ * the cycle the solver runs (with three or more levels, the coarse levels
+ * from composed matrices); 1: the level-by-level cycle, the same linear map
+ * with each factor rounded on its own.  No counterpart in the reference (its
+ * preconditioners live inside Eigen's solvers): for callers that
+ * precondition a solver of their own, and for checks of the cycle. */
+int mp_precond_apply(mp_ctx* ctx, int cycle, int n, const double* in, double* out);
+
 /* Krylov iterations between two convergence polls are replayed from a
  * hipGraph captured on first use (enable = 1, the default) or launched one by
  * one (enable = 0).  Results are identical; kernel timing (below) always

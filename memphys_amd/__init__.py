@@ -38,7 +38,7 @@ HIP_SYMBOLS = (
     "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
     "mp_graph_mode", "mp_batch_upload", "mp_batch_params", "mp_batch_stats", "mp_autotune",
     "mp_operator_eigenvalue", "mp_knn_stencils", "mp_rcm_order", "mp_fs_body_force",
-    "mp_divergence_noalloc",
+    "mp_divergence_noalloc", "mp_precond_apply",
 )
 
 
@@ -125,6 +125,7 @@ def hip_lib() -> C.CDLL:
                                                C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.mp_precond_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), vp, vp,
                                         C.POINTER(C.c_double)]
+        lib.mp_precond_apply.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
         lib.mp_knn_stencils.argtypes = [vp, C.c_int, C.c_int, _dp, _u8, _ip, _ip, _ip]
         lib.mp_rcm_order.argtypes = [vp, C.c_int, _ip, _ip, _ip]
         for name in HIP_SYMBOLS:
@@ -477,6 +478,15 @@ class Context:
         L = min(lv.value, 16)
         return {"levels": lv.value, "rows": rows[:L].tolist(), "nnz": nnz[:L].tolist(),
                 "setup_s": secs.value}
+
+    def precond_apply(self, x: np.ndarray, plain: bool = False) -> np.ndarray:
+        """One multigrid V-cycle of the pressure solve on x (system size): the
+        solver's cycle, or with plain=True the level-by-level one."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros_like(x)
+        self._check(self.lib.mp_precond_apply(self.h, 1 if plain else 0, len(x), x, out),
+                    "mp_precond_apply")
+        return out
 
     def guess_mode(self, mode: int):
         self._check(self.lib.mp_guess_mode(self.h, mode), "mp_guess_mode")

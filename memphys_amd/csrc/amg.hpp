@@ -23,17 +23,23 @@ struct AmgMat {
     long nnz = 0;
     DevBuf<int> ptr_b, col_b;
     DevBuf<uint16_t> val_b;  // BF16 values (round to nearest even)
+    DevBuf<uint16_t> val2_b; // a second value array in the same slots (composed restriction)
     const int* ptr = nullptr;
     const int* col = nullptr;
     const uint16_t* val = nullptr;
     const uint16_t* col16 = nullptr;  // compressed columns of level 0 (the solver's)
     const int* cbase = nullptr;
+    const uint16_t* val2 = nullptr;
 };
 
 struct AmgLevelDev {
     int n = 0;
     AmgMat A;      // operator of this level
     AmgMat P, R;   // prolongation from / restriction to the next level
+    // composed steps (three or more levels): Q = R_{l-1} - A D R_{l-1} (values)
+    // with R_{l-1} in the same slots (levels 2 .. L-2), T = P - D A P (levels
+    // 1 .. L-2)
+    AmgMat Q, T;
     DevBuf<float> dinv;  // omega / a_ii
     DevBuf<float> b, x, res, y;
 };
@@ -51,8 +57,19 @@ struct AmgDevice {
     std::vector<long> nnz;
     std::string layout;  // chosen layout per matrix, for reports
     double setup_seconds = 0.0;
-    // host copies of the hierarchy (A[0] stays empty), for re-tuning layouts
-    std::vector<mp_amg::Csr> hA, hP, hR;
+    // host copies of the hierarchy (A[0] stays empty) and of the T_l, for
+    // re-tuning layouts
+    std::vector<mp_amg::Csr> hA, hP, hR, hT;
+    // Three or more levels: the cycle below level 1 runs from the composed
+    // matrices (Q_l, T_l and the coarsest solve times R_{L-2}, row-dense BF16
+    // in cr_val: row r's size[r] entries at cr_data[r] act on the level-(L-2)
+    // vector from row cr_first[r]).  plain_cycle runs the level-by-level
+    // cycle instead (the same map, for comparisons).
+    bool composed = false;
+    bool plain_cycle = false;
+    DevBuf<uint16_t> cr_val;
+    DevBuf<int> cr_first, cr_size;
+    DevBuf<long> cr_data;
 
     // Builds the hierarchy from the finest operator, given in SELL-64x4 slots
     // (device order) by the solver: FP64 values val64_dev in the slots of sv

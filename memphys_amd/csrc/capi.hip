@@ -1476,6 +1476,37 @@ int mp_precond_info(mp_ctx* c, int max_levels, int* levels, int* rows, long* nnz
     });
 }
 
+int mp_precond_apply(mp_ctx* c, int cycle, int n, const double* in, double* out) {
+    return guarded(c, "mp_precond_apply", [&]() -> int {
+        if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_precond_apply: mp_fs_create not called");
+        if (!in || !out || cycle < 0 || cycle > 1)
+            return fail(c, MP_ERR_ARG, "mp_precond_apply: bad arguments");
+        PpeSolver& S = *c->ppe;
+        AmgDevice* a = S.amg.get();
+        if (!a)
+            return fail(c, MP_ERR_STATE,
+                        "mp_precond_apply: the pressure solve has no multigrid preconditioner");
+        if (n != S.n_sys)
+            return fail(c, MP_ERR_ARG, "mp_precond_apply: n is not the system size " +
+                                           std::to_string(S.n_sys));
+        std::vector<double> h;
+        to_device_order(c, in, h, &S);
+        upload(c, c->vec_a.p, h);
+        a->plain_cycle = cycle == 1;
+        try {
+            a->apply(c->stream, c->vec_a.p, c->vec_b.p, S.nseg == 1 ? S.segs[0].border : 1,
+                     nullptr);
+        } catch (...) {
+            a->plain_cycle = false;
+            throw;
+        }
+        a->plain_cycle = false;
+        download(c, c->vec_b.p, h);
+        from_device_order(c, h, out, &S);
+        return MP_OK;
+    });
+}
+
 int mp_graph_mode(mp_ctx* c, int enable) {
     return guarded(c, "mp_graph_mode", [&]() -> int {
         c->use_graphs = enable != 0;

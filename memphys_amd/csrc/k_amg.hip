@@ -11,7 +11,21 @@
 //   post-smooth            y_l = x_l + w D^-1 (b_l - A_l x_l)
 // Level 0 reads its right-hand side from the Krylov vector (FP64) and writes
 // the FP64 result; the border unknown of the full-Neumann system passes
-// through unchanged.  A matrix is SELL-64x4 (one wavefront per slice of 64
+// through unchanged.
+//
+// With three or more levels the cycle below level 1 runs from matrices that
+// compose its steps, built at set-up (the same linear map, one rounding of
+// the composed values instead of one per factor), so each coarse level costs
+// one product on the way down and one on the way up:
+//   restrict + pre-smooth  b_l = R_{l-1} res_{l-1},  res_l = Q_l res_{l-1},
+//                          Q_l = R_{l-1} - A_l D_l R_{l-1}        (levels 2 .. L-2)
+//   coarsest solve         y_{L-1} = (G R_{L-2}) res_{L-2}        (dense rows)
+//   prolong + post-smooth  y_l = D_l (b_l + res_l) + T_l y_{l+1},
+//                          T_l = P_l - D_l A_l P_l                (levels 1 .. L-2)
+// (with x_l = D_l b_l after pre-smoothing: x_l + P_l y + D_l (b_l - A_l (x_l +
+// P_l y)) = D_l b_l + D_l res_l + (P_l - D_l A_l P_l) y).  Levels 0 and 1 keep
+// the separate restriction and pre-smoothing (Q_1 would be about 25 times
+// denser than A_1), and level 0 the separate prolongation and post-smoothing.  A matrix is SELL-64x4 (one wavefront per slice of 64
 // rows, four neighbours of a row side by side, padding slots with value 0)
 // or CSR with several lanes per row, whichever suits its row lengths.
 #include "amg.hpp"
@@ -25,7 +39,7 @@ namespace mp {
 
 namespace {
 
-enum { OP_PRE = 0, OP_SPMV = 1, OP_POST = 2, OP_PRES = 3 };
+enum { OP_PRE = 0, OP_SPMV = 1, OP_POST = 2, OP_PRES = 3, OP_RQ = 4, OP_TPOST = 5 };
 
 __host__ __device__ inline int nblk_rows(int nslices) { return (nslices * SLICE + BLOCK - 1) / BLOCK; }
 
@@ -38,6 +52,7 @@ struct MatView {
     const uint16_t* val;    // BF16
     const uint16_t* col16;  // compressed columns (SELL-64x4 only; may be null)
     const int* cbase;
+    const uint16_t* val2;   // second BF16 value array in the same slots (OP_RQ)
 };
 
 __device__ __forceinline__ float bf2f(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
@@ -73,6 +88,9 @@ __device__ __forceinline__ float gathered(int c, const TB* __restrict__ b, const
 //   OP_PRES: res = b - A xin, with xin = D^-1 b already formed
 //   OP_SPMV: out = base + M xin (base may be null and may alias out)
 //   OP_POST: out = xin + D^-1 (b - A xin)
+//   OP_RQ  : xout = M2 xin, res = M xin (M = Q_l in val, M2 = R_{l-1} in val2;
+//            CSR layouts only)
+//   OP_TPOST: out = D^-1 (b + base) + M xin (M = T_l, base = res_l)
 // Level 0 also copies the border unknown (border_in -> border_out).
 template <int G, int OP, typename TB, typename TO>
 __global__ __launch_bounds__(BLOCK) void k_amg_op(MatView m, const float* __restrict__ dinv,
@@ -83,9 +101,10 @@ __global__ __launch_bounds__(BLOCK) void k_amg_op(MatView m, const float* __rest
                                                   double* __restrict__ border_out,
                                                   const int* done, float* __restrict__ out32) {
     if (stopped(done)) return;
-    float acc = 0.0f;
+    float acc = 0.0f, acc2 = 0.0f;
     int i;
     if constexpr (G == 0) {
+        static_assert(OP != OP_RQ, "two value arrays: CSR layouts only");
         const int lb = xcd_block(nblk_rows(m.nslices));
         if (lb < 0) return;
         if (border_out && lb == 0 && threadIdx.x == 0) *border_out = *border_in;
@@ -119,12 +138,26 @@ __global__ __launch_bounds__(BLOCK) void k_amg_op(MatView m, const float* __rest
                 acc = fmaf(bf2f(v1), x1, acc);
                 acc = fmaf(bf2f(v2), x2, acc);
                 acc = fmaf(bf2f(v3), x3, acc);
+                if constexpr (OP == OP_RQ) {
+                    const uint16_t u0 = m.val2[t], u1 = m.val2[t + G], u2 = m.val2[t + 2 * G],
+                                   u3 = m.val2[t + 3 * G];
+                    acc2 = fmaf(bf2f(u0), x0, acc2);
+                    acc2 = fmaf(bf2f(u1), x1, acc2);
+                    acc2 = fmaf(bf2f(u2), x2, acc2);
+                    acc2 = fmaf(bf2f(u3), x3, acc2);
+                }
             }
-            for (; t < t1; t += G)
-                acc = fmaf(bf2f(m.val[t]), gathered<OP>(m.col[t], b, xin, dinv), acc);
+            for (; t < t1; t += G) {
+                const float xv = gathered<OP>(m.col[t], b, xin, dinv);
+                acc = fmaf(bf2f(m.val[t]), xv, acc);
+                if constexpr (OP == OP_RQ) acc2 = fmaf(bf2f(m.val2[t]), xv, acc2);
+            }
         }
 #pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        for (int off = G / 2; off > 0; off >>= 1) {
+            acc += __shfl_xor(acc, off);
+            if constexpr (OP == OP_RQ) acc2 += __shfl_xor(acc2, off);
+        }
         if (sub != 0 || i >= m.n) return;
     }
     if constexpr (OP == OP_PRE) {
@@ -135,6 +168,11 @@ __global__ __launch_bounds__(BLOCK) void k_amg_op(MatView m, const float* __rest
         res[i] = (float)b[i] - acc;
     } else if constexpr (OP == OP_SPMV) {
         out[i] = (TO)(base ? base[i] + acc : acc);
+    } else if constexpr (OP == OP_RQ) {
+        xout[i] = acc2;
+        res[i] = acc;
+    } else if constexpr (OP == OP_TPOST) {
+        out[i] = (TO)(dinv[i] * ((float)b[i] + base[i]) + acc);
     } else {
         const float y = xin[i] + dinv[i] * ((float)b[i] - acc);
         out[i] = (TO)y;
@@ -189,6 +227,43 @@ __global__ __launch_bounds__(BLOCK) void k_amg_blockdense(int nc, const float* _
     }
 }
 
+// y[r] = sum_j C[r, j] x[first[r] + j] over j < size[r]: the coarsest solve
+// composed with the restriction to it (C = G R_{L-2} per problem, BF16, each
+// row starting at data[r], a multiple of 4).  One block per row; each thread
+// sums entries 4t .. 4t+3, 4t+1024 .., then a fixed reduction tree.
+constexpr int RD_THREADS = 256;
+__global__ __launch_bounds__(RD_THREADS) void k_amg_rowdense(int nrows,
+                                                              const uint16_t* __restrict__ C,
+                                                              const int* __restrict__ first,
+                                                              const int* __restrict__ size,
+                                                              const long* __restrict__ data,
+                                                              const float* __restrict__ x,
+                                                              float* __restrict__ y,
+                                                              const int* done) {
+    if (stopped(done)) return;
+    const int r = blockIdx.x;
+    if (r >= nrows) return;
+    const int f = first[r], m = size[r];
+    const uint16_t* c = C + data[r];
+    float acc = 0.0f;
+    for (int j = 4 * (int)threadIdx.x; j < m; j += 4 * RD_THREADS) {
+        const uint2 v = *reinterpret_cast<const uint2*>(c + j);  // four BF16, 8-byte aligned
+        acc = fmaf(__uint_as_float(v.x << 16), x[f + j], acc);
+        if (j + 1 < m) acc = fmaf(__uint_as_float(v.x & 0xFFFF0000u), x[f + j + 1], acc);
+        if (j + 2 < m) acc = fmaf(__uint_as_float(v.y << 16), x[f + j + 2], acc);
+        if (j + 3 < m) acc = fmaf(__uint_as_float(v.y & 0xFFFF0000u), x[f + j + 3], acc);
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    __shared__ float part[RD_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = part[0];
+        for (int w = 1; w < RD_THREADS / 64; ++w) t += part[w];
+        y[r] = t;
+    }
+}
+
 // out[border row] = in[border row] for every bordered problem of a batch
 __global__ void k_border_copy(int n, const int* __restrict__ rows, const double* __restrict__ in,
                               double* __restrict__ out) {
@@ -200,7 +275,7 @@ template <int G, int OP, typename TB, typename TO>
 void launch_g(hipStream_t st, const AmgMat& M, const float* dinv, const TB* b, const float* xin,
               const float* base, float* xout, float* res, TO* out, const double* bin,
               double* bout, const int* done, float* out32) {
-    const MatView v{M.n, M.nslices, M.ptr, M.col, M.val, M.col16, M.cbase};
+    const MatView v{M.n, M.nslices, M.ptr, M.col, M.val, M.col16, M.cbase, M.val2};
     int nb;
     if constexpr (G == 0)
         nb = nblk_rows(M.nslices);
@@ -215,7 +290,12 @@ void launch_op(hipStream_t st, const AmgMat& M, const float* dinv, const TB* b, 
                const float* base, float* xout, float* res, TO* out, const double* bin = nullptr,
                double* bout = nullptr, const int* done = nullptr, float* out32 = nullptr) {
     switch (M.g) {
-        case 0: launch_g<0, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done, out32); break;
+        case 0:
+            if constexpr (OP == OP_RQ)
+                throw std::runtime_error("amg: a composed restriction needs a CSR layout");
+            else
+                launch_g<0, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done, out32);
+            break;
         case 4: launch_g<4, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done, out32); break;
         case 8: launch_g<8, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done, out32); break;
         case 16: launch_g<16, OP>(st, M, dinv, b, xin, base, xout, res, out, bin, bout, done, out32); break;
@@ -405,6 +485,44 @@ mp_amg::Csr block_diag(const std::vector<const mp_amg::Csr*>& m, const std::vect
     return out;
 }
 
+// X - diag(d) Y (d null: X - Y) over the union of the two patterns, columns
+// sorted; xv (may be null) receives X's values in the same slots (0 where X
+// has no entry).
+mp_amg::Csr minus_scaled(const mp_amg::Csr& X, const std::vector<float>* d, const mp_amg::Csr& Y,
+                         std::vector<double>* xv) {
+    if (X.nrows != Y.nrows || X.ncols != Y.ncols) throw std::runtime_error("amg: shape mismatch");
+    mp_amg::Csr out;
+    out.nrows = X.nrows;
+    out.ncols = X.ncols;
+    out.ptr.assign(X.nrows + 1, 0);
+    std::vector<int> mark(X.ncols, -1), pos(X.ncols, 0), cols;
+    std::vector<double> vx, vy;
+    if (xv) xv->clear();
+    for (int i = 0; i < X.nrows; ++i) {
+        cols.clear();
+        for (const mp_amg::Csr* M : {&X, &Y})
+            for (int t = M->ptr[i]; t < M->ptr[i + 1]; ++t)
+                if (mark[M->col[t]] != i) {
+                    mark[M->col[t]] = i;
+                    cols.push_back(M->col[t]);
+                }
+        std::sort(cols.begin(), cols.end());
+        for (size_t k = 0; k < cols.size(); ++k) pos[cols[k]] = (int)k;
+        vx.assign(cols.size(), 0.0);
+        vy.assign(cols.size(), 0.0);
+        for (int t = X.ptr[i]; t < X.ptr[i + 1]; ++t) vx[pos[X.col[t]]] += X.val[t];
+        for (int t = Y.ptr[i]; t < Y.ptr[i + 1]; ++t) vy[pos[Y.col[t]]] += Y.val[t];
+        const double di = d ? (double)(*d)[i] : 1.0;
+        for (size_t k = 0; k < cols.size(); ++k) {
+            out.col.push_back(cols[k]);
+            out.val.push_back(vx[k] - di * vy[k]);
+            if (xv) xv->push_back(vx[k]);
+        }
+        out.ptr[i + 1] = (int)out.col.size();
+    }
+    return out;
+}
+
 }  // namespace
 
 void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_dev,
@@ -501,6 +619,7 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
     }
     lv.clear();
     lv.resize(L);
+    std::vector<std::vector<float>> dvs(L);  // host copies of omega / a_ii per level
     rows.assign(L, 0);
     nnz.assign(L, 0);
     layout.clear();
@@ -527,6 +646,7 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
             std::copy(dq.begin(), dq.end(), dv.begin() + off[l][q]);
         }
         put(d.dinv, dv, st);
+        dvs[l] = dv;
         if (l == 0) {
             d.A.n = n0;
             d.A.nslices = sv_apply.nslices;
@@ -573,6 +693,58 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
         }
     }
     put(cinv, g, st);
+    // composed steps below level 1 (three or more levels)
+    composed = L >= 3;
+    plain_cycle = false;
+    hT.assign(L, mp_amg::Csr());
+    if (composed) {
+        for (int l = 1; l <= L - 2; ++l) {
+            const mp_amg::Csr AP = mp_amg::multiply(Ab[l], Pb[l], opt.threads);
+            hT[l] = minus_scaled(Pb[l], &dvs[l], AP, nullptr);
+            upload_mat(lv[l].T, hT[l], rule_layout(hT[l]), st);
+            if (l >= 2) {
+                mp_amg::Csr DR = Rb[l - 1];
+                for (int i = 0; i < DR.nrows; ++i)
+                    for (int t = DR.ptr[i]; t < DR.ptr[i + 1]; ++t) DR.val[t] *= (double)dvs[l][i];
+                const mp_amg::Csr ADR = mp_amg::multiply(Ab[l], DR, opt.threads);
+                std::vector<double> rv;
+                const mp_amg::Csr Q = minus_scaled(Rb[l - 1], nullptr, ADR, &rv);
+                upload_mat(lv[l].Q, Q, std::max(4, rule_layout(Q)), st);
+                std::vector<uint16_t> r16(rv.size());
+                for (size_t t = 0; t < rv.size(); ++t) r16[t] = f2bf((float)rv[t]);
+                put(lv[l].Q.val2_b, r16, st);
+                lv[l].Q.val2 = lv[l].Q.val2_b.p;
+            }
+        }
+        // the coarsest solve times R_{L-2}, one dense row per coarsest row
+        std::vector<uint16_t> cv;
+        std::vector<int> cf(nc), cs(nc);
+        std::vector<long> cd(nc);
+        for (int q = 0; q < nb; ++q) {
+            const int m = hs[q].nc;
+            const mp_amg::Csr& Rq = hs[q].R[L - 2];
+            if (Rq.nrows != m) throw std::runtime_error("amg: coarsest level size");
+            const int n = Rq.ncols;
+            std::vector<double> row(n);
+            for (int r = 0; r < m; ++r) {
+                std::fill(row.begin(), row.end(), 0.0);
+                for (int k = 0; k < m; ++k) {
+                    const double gk = hs[q].coarse_inv[(size_t)r * m + k];
+                    for (int t = Rq.ptr[k]; t < Rq.ptr[k + 1]; ++t) row[Rq.col[t]] += gk * Rq.val[t];
+                }
+                const int rr = off[L - 1][q] + r;
+                cf[rr] = off[L - 2][q];
+                cs[rr] = n;
+                cd[rr] = (long)cv.size();
+                for (int j = 0; j < n; ++j) cv.push_back(f2bf((float)row[j]));
+                while (cv.size() % 4) cv.push_back(0);
+            }
+        }
+        put(cr_val, cv, st);
+        put(cr_first, cf, st);
+        put(cr_size, cs, st);
+        put(cr_data, cd, st);
+    }
     blocked = nb > 1;
     if (blocked) {
         put(cfirst, hfirst, st);
@@ -584,10 +756,18 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
     hP = std::move(Pb);
     hR = std::move(Rb);
     for (int l = 0; l + 1 < L; ++l) {
-        char buf[64];
+        char buf[96];
         std::snprintf(buf, sizeof(buf), "A%d:%d P%d:%d R%d:%d ", l, lv[l].A.g, l, lv[l].P.g, l,
                       lv[l].R.g);
         layout += buf;
+        if (composed && l >= 1) {
+            std::snprintf(buf, sizeof(buf), "T%d:%d ", l, lv[l].T.g);
+            layout += buf;
+            if (l >= 2) {
+                std::snprintf(buf, sizeof(buf), "Q%d:%d ", l, lv[l].Q.g);
+                layout += buf;
+            }
+        }
     }
     setup_seconds =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -614,12 +794,21 @@ void AmgDevice::retune(hipStream_t st) {
     try {
         for (int l = 0; l + 1 < L; ++l) {
             AmgLevelDev& d = lv[l];
-            if (l > 0) upload_tuned(d.A, hA[l], st, tx.p, ty.p, e0, e1, layout,
-                                    ("A" + std::to_string(l)).c_str());
-            upload_tuned(d.P, hP[l], st, tx.p, ty.p, e0, e1, layout,
-                         ("P" + std::to_string(l)).c_str());
-            upload_tuned(d.R, hR[l], st, tx.p, ty.p, e0, e1, layout,
-                         ("R" + std::to_string(l)).c_str());
+            // the composed cycle uses A_1, P_0, R_0 and the T_l (the Q_l keep
+            // their CSR layout); the plain cycle every A_l, P_l, R_l
+            const bool plain = !composed || l == 0;
+            if (l > 0 && (plain || l == 1))
+                upload_tuned(d.A, hA[l], st, tx.p, ty.p, e0, e1, layout,
+                             ("A" + std::to_string(l)).c_str());
+            if (plain) {
+                upload_tuned(d.P, hP[l], st, tx.p, ty.p, e0, e1, layout,
+                             ("P" + std::to_string(l)).c_str());
+                upload_tuned(d.R, hR[l], st, tx.p, ty.p, e0, e1, layout,
+                             ("R" + std::to_string(l)).c_str());
+            } else {
+                upload_tuned(d.T, hT[l], st, tx.p, ty.p, e0, e1, layout,
+                             ("T" + std::to_string(l)).c_str());
+            }
         }
     } catch (...) {
         (void)hipEventDestroy(e0);
@@ -677,6 +866,37 @@ void AmgDevice::vcycle(hipStream_t st, const TI* in, TO* out, const double* bin,
             launch_op<OP_PRE, TI, float>(st, lv[0].A, lv[0].dinv.p, in, nullptr, nullptr,
                                              lv[0].x.p, lv[0].res.p, (float*)nullptr, bin, bout,
                                              done);
+        if (composed && !plain_cycle) {
+            // level 0 -> 1 and level 1's pre-smoothing as below; then one
+            // composed product per coarse level down, the composed coarsest
+            // solve, one composed product per level up to level 1
+            launch_op<OP_SPMV, float, float>(st, lv[0].R, nullptr, (const float*)nullptr,
+                                             lv[0].res.p, nullptr, nullptr, nullptr, lv[1].b.p,
+                                             nullptr, nullptr, done);
+            launch_op<OP_PRE, float, float>(st, lv[1].A, lv[1].dinv.p, lv[1].b.p, nullptr,
+                                            nullptr, lv[1].x.p, lv[1].res.p, (float*)nullptr,
+                                            nullptr, nullptr, done);
+            for (int l = 2; l <= L - 2; ++l)
+                launch_op<OP_RQ, float, float>(st, lv[l].Q, nullptr, (const float*)nullptr,
+                                               lv[l - 1].res.p, nullptr, lv[l].b.p, lv[l].res.p,
+                                               (float*)nullptr, nullptr, nullptr, done);
+            hipLaunchKernelGGL(k_amg_rowdense, dim3(nc), dim3(RD_THREADS), 0, st, nc, cr_val.p,
+                               cr_first.p, cr_size.p, cr_data.p, (const float*)lv[L - 2].res.p,
+                               lv[L - 1].y.p, done);
+            for (int l = L - 2; l >= 1; --l)
+                launch_op<OP_TPOST, float, float>(st, lv[l].T, lv[l].dinv.p, lv[l].b.p,
+                                                  lv[l + 1].y.p, lv[l].res.p, nullptr, nullptr,
+                                                  lv[l].y.p, nullptr, nullptr, done);
+            AmgLevelDev& f = lv[0];
+            launch_op<OP_SPMV, float, float>(st, f.P, nullptr, (const float*)nullptr, lv[1].y.p,
+                                             f.x.p, nullptr, nullptr, f.x.p, nullptr, nullptr,
+                                             done);
+            if (post_timer) post_timer->begin(st);
+            launch_op<OP_POST, TI, TO>(st, f.A, f.dinv.p, in, f.x.p, nullptr, nullptr, nullptr,
+                                       out, nullptr, nullptr, done, out32);
+            if (post_timer) post_timer->end(st);
+            return;
+        }
         for (int l = 0; l + 1 < L; ++l) {
             AmgLevelDev& f = lv[l];
             AmgLevelDev& c = lv[l + 1];
