@@ -147,10 +147,24 @@ __global__ __launch_bounds__(BLOCK) void k_amg_op(MatView m, const float* __rest
                     acc2 = fmaf(bf2f(u3), x3, acc2);
                 }
             }
-            for (; t < t1; t += G) {
-                const float xv = gathered<OP>(m.col[t], b, xin, dinv);
-                acc = fmaf(bf2f(m.val[t]), xv, acc);
-                if constexpr (OP == OP_RQ) acc2 = fmaf(bf2f(m.val2[t]), xv, acc2);
+            if (t < t1) {
+                // the lane's last one to three entries, loads issued together
+                // (an absent entry re-reads entry t and its term is skipped)
+                const bool h1 = t + G < t1, h2 = t + 2 * G < t1;
+                const int ta = h1 ? t + G : t, tb = h2 ? t + 2 * G : t;
+                const int c0 = m.col[t], c1 = m.col[ta], c2 = m.col[tb];
+                const uint16_t v0 = m.val[t], v1 = m.val[ta], v2 = m.val[tb];
+                const float x0 = gathered<OP>(c0, b, xin, dinv), x1 = gathered<OP>(c1, b, xin, dinv),
+                            x2 = gathered<OP>(c2, b, xin, dinv);
+                acc = fmaf(bf2f(v0), x0, acc);
+                if (h1) acc = fmaf(bf2f(v1), x1, acc);
+                if (h2) acc = fmaf(bf2f(v2), x2, acc);
+                if constexpr (OP == OP_RQ) {
+                    const uint16_t u0 = m.val2[t], u1 = m.val2[ta], u2 = m.val2[tb];
+                    acc2 = fmaf(bf2f(u0), x0, acc2);
+                    if (h1) acc2 = fmaf(bf2f(u1), x1, acc2);
+                    if (h2) acc2 = fmaf(bf2f(u2), x2, acc2);
+                }
             }
         }
 #pragma unroll
@@ -230,8 +244,8 @@ __global__ __launch_bounds__(BLOCK) void k_amg_blockdense(int nc, const float* _
 // y[r] = sum_j C[r, j] x[first[r] + j] over j < size[r]: the coarsest solve
 // composed with the restriction to it (C = G R_{L-2} per problem, BF16, each
 // row starting at data[r], a multiple of 4).  One block per row; each thread
-// sums entries 4t .. 4t+3, 4t+1024 .., then a fixed reduction tree.
-constexpr int RD_THREADS = 256;
+// sums entries 4t .. 4t+3, 4t+4096 .., then a fixed reduction tree.
+constexpr int RD_THREADS = 1024;
 __global__ __launch_bounds__(RD_THREADS) void k_amg_rowdense(int nrows,
                                                               const uint16_t* __restrict__ C,
                                                               const int* __restrict__ first,
@@ -371,15 +385,35 @@ void upload_mat(AmgMat& d, const mp_amg::Csr& M, int g, hipStream_t st) {
     d.val = d.val_b.p;
 }
 
+// Reads n4 float4s (grid-stride), so that a product timed after it finds its
+// matrix and vector outside the caches, as inside a V-cycle, where the finest
+// level's passes evict them between two uses.
+__global__ __launch_bounds__(BLOCK) void k_stream_read(long n4, const float4* __restrict__ a,
+                                                       float* __restrict__ sink) {
+    float acc = 0.0f;
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < n4; i += (long)gridDim.x * BLOCK) {
+        const float4 v = a[i];
+        acc += (v.x + v.y) + (v.z + v.w);
+    }
+    if (acc == 1.25e-30f) sink[0] = acc;  // (the buffer holds zeros: keeps the loads only)
+}
+
 // Autotuned layout: every candidate (SELL-64x4 for rows of at most 24 entries
 // on average unless its padding more than triples the stored entries; CSR
 // with 4..64 lanes per row) is uploaded, then all are timed on products with
-// this matrix in interleaved rounds (so clock and cache drift hit every
-// candidate alike), each by its median.  The set-up rule's layout stays
-// unless another is more than 10% faster.
+// this matrix in interleaved rounds (so clock drift hits every candidate
+// alike), each product after a read of a buffer larger than the caches, each
+// candidate by its median.  The set-up rule's layout stays unless another is
+// more than 10% faster.
+struct Flush {
+    const float4* buf;
+    long n4;
+    float* sink;
+};
 int rule_layout(const mp_amg::Csr& M);
 void upload_tuned(AmgMat& d, const mp_amg::Csr& M, hipStream_t st, float* xin, float* out,
-                  hipEvent_t e0, hipEvent_t e1, std::string& report, const char* label) {
+                  hipEvent_t e0, hipEvent_t e1, const Flush& fl, std::string& report,
+                  const char* label) {
     std::vector<int> cand;
     {
         long slots = 0;
@@ -406,15 +440,20 @@ void upload_tuned(AmgMat& d, const mp_amg::Csr& M, hipStream_t st, float* xin, f
     std::vector<std::vector<float>> t(nc);
     for (int r = 0; r < rounds; ++r)
         for (int c = 0; c < nc; ++c) {
-            MP_HIP(hipEventRecord(e0, st));
-            for (int k = 0; k < reps; ++k)
+            float tot = 0.0f;
+            for (int k = 0; k < reps; ++k) {
+                hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(BLOCK), 0, st, fl.n4, fl.buf,
+                                   fl.sink);
+                MP_HIP(hipEventRecord(e0, st));
                 launch_op<OP_SPMV, float, float>(st, trial[c], nullptr, (const float*)nullptr,
                                                  xin, nullptr, nullptr, nullptr, out);
-            MP_HIP(hipEventRecord(e1, st));
-            MP_HIP(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            MP_HIP(hipEventElapsedTime(&ms, e0, e1));
-            t[c].push_back(ms / reps);
+                MP_HIP(hipEventRecord(e1, st));
+                MP_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                MP_HIP(hipEventElapsedTime(&ms, e0, e1));
+                tot += ms;
+            }
+            t[c].push_back(tot / reps);
         }
     std::vector<float> med(nc);
     for (int c = 0; c < nc; ++c) {
@@ -784,9 +823,11 @@ void AmgDevice::retune(hipStream_t st) {
     if (L < 2) return;
     size_t big = 64;
     for (int l = 0; l < L; ++l) big = std::max(big, (size_t)lv[l].n + 64);
-    DevBuf<float> tx, ty;
+    DevBuf<float> tx, ty, fb;
     alloc_zero(tx, big, st);
     alloc_zero(ty, big, st);
+    alloc_zero(fb, (size_t)128 << 20, st);  // 512 MiB, above the L2s and the last-level cache
+    const Flush fl{reinterpret_cast<const float4*>(fb.p), (long)(fb.n / 4), ty.p + big - 1};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MP_HIP(hipEventCreate(&e0));
     MP_HIP(hipEventCreate(&e1));
@@ -798,15 +839,15 @@ void AmgDevice::retune(hipStream_t st) {
             // their CSR layout); the plain cycle every A_l, P_l, R_l
             const bool plain = !composed || l == 0;
             if (l > 0 && (plain || l == 1))
-                upload_tuned(d.A, hA[l], st, tx.p, ty.p, e0, e1, layout,
+                upload_tuned(d.A, hA[l], st, tx.p, ty.p, e0, e1, fl, layout,
                              ("A" + std::to_string(l)).c_str());
             if (plain) {
-                upload_tuned(d.P, hP[l], st, tx.p, ty.p, e0, e1, layout,
+                upload_tuned(d.P, hP[l], st, tx.p, ty.p, e0, e1, fl, layout,
                              ("P" + std::to_string(l)).c_str());
-                upload_tuned(d.R, hR[l], st, tx.p, ty.p, e0, e1, layout,
+                upload_tuned(d.R, hR[l], st, tx.p, ty.p, e0, e1, fl, layout,
                              ("R" + std::to_string(l)).c_str());
             } else {
-                upload_tuned(d.T, hT[l], st, tx.p, ty.p, e0, e1, layout,
+                upload_tuned(d.T, hT[l], st, tx.p, ty.p, e0, e1, fl, layout,
                              ("T" + std::to_string(l)).c_str());
             }
         }

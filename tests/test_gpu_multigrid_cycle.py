@@ -97,3 +97,41 @@ def test_composed_cycle_batch(ctx):
     _print("batch hierarchy", info)
     assert info["levels"] >= 3
     _check_cycles(ctx, ctx.n_sys, cat.nv, True, 13)
+
+
+def test_autotune_leaves_fields_and_keeps_accuracy(ctx, cavity2d):
+    """mp_autotune (poll interval by timing; multigrid layouts by timing each
+    candidate after a cache-clearing read) leaves the fields and pressure
+    history alone, and the steps after it match an untuned run to the
+    solver's accuracy (the layouts change the V-cycle at the rounding level)."""
+    import memphys_amd as M
+    cloud = cavity2d
+    u_dir, p_dir, u0 = lid_bc(cloud, 1)
+
+    def run(tune):
+        ctx.upload(cloud)
+        ctx.weights(3, 3, download=False)
+        ctx.fs_create(1.0, 0.01, 2e-4, 2, 1e-12, 2000, u_dir, u_dir, None, p_dir,
+                      precision=M.MP_PREC_MIXED, precond=M.MP_PRECOND_AMG,
+                      krylov=M.MP_KRYLOV_GMRES)
+        ctx.set_field(M.MP_U, u0)
+        ctx.set_field(M.MP_V, np.zeros(cloud.nv))
+        ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
+        ctx.step(3, 0)
+        if tune:
+            before = [ctx.get_field(f) for f in (M.MP_U, M.MP_V, M.MP_P)]
+            chosen, ms = ctx.autotune()
+            _print("autotune: interval", chosen, "ms", ms)
+            assert chosen in (4, 8, 16, 32, 64)
+            assert all(v > 0 for v in ms.values())
+            after = [ctx.get_field(f) for f in (M.MP_U, M.MP_V, M.MP_P)]
+            for a, b in zip(before, after):
+                assert np.array_equal(a, b)
+        ctx.step(3, 3)
+        return [ctx.get_field(f) for f in (M.MP_U, M.MP_V, M.MP_P)]
+
+    ref = run(False)
+    tuned = run(True)
+    errs = [rel_err(a, b) for a, b in zip(tuned, ref)]
+    _print("tuned vs untuned after 6 steps:", errs)
+    assert max(errs) <= 1e-8
