@@ -43,9 +43,12 @@ enum mp_precision { MP_PREC_FP64 = 0, MP_PREC_MIXED = 1 };
 
 /* Initial guess for each pressure solve: the previous pressure (reference
  * behaviour, solveWithGuess(rhs, p_old)), the learned message-passing guess
- * (linear extrapolation 2 p(t-1) - p(t-2) plus the network's correction), or
- * the linear extrapolation alone.  The last two fall back to the previous
- * pressure until two pressures are in the history. */
+ * (the extrapolation in time plus the network's correction), or the
+ * extrapolation alone: the polynomial through the pressures of the last 2, 3
+ * or 4 steps (mp_guess_order; equal time steps), evaluated at the next step.
+ * The last two fall back to the previous pressure until two pressures are in
+ * the history, and use the highest order the history holds until it is
+ * full.  Single problems only (a batch starts from the previous pressure). */
 enum mp_guess { MP_GUESS_PREVIOUS = 0, MP_GUESS_LEARNED = 1, MP_GUESS_EXTRAPOLATE = 2 };
 
 /* PPE preconditioner: Jacobi (right diagonal scaling, the plain path), or a
@@ -207,6 +210,12 @@ int mp_fs_body_force(mp_ctx* ctx, const double* fx, const double* fy, const doub
 int mp_fs_step(mp_ctx* ctx, int n_steps, int it0);
 int mp_stats_get(mp_ctx* ctx, mp_step_stats* stats);
 int mp_guess_mode(mp_ctx* ctx, int mode);
+/* Order of the extrapolated guess (2: 2 p1 - p2, 3: 3 (p1 - p2) + p3, 4:
+ * 4 (p1 + p3) - 6 p2 - p4, p1 the latest pressure); default 4.  The higher
+ * orders amplify the error the history carries from the solves (by 2.2, 4.4
+ * and 8.3 in norm): with loose tolerances a lower order can start closer.
+ * No counterpart in the reference (its guess is the previous pressure). */
+int mp_guess_order(mp_ctx* ctx, int order);
 
 /* Learned guess: a 3-layer message-passing network over the stencil graph
  * (width 64, BF16 weights, FP32 accumulation).  The parameters are a packed
@@ -221,8 +230,9 @@ int mp_guess_params(mp_ctx* ctx, int set, float* params);
 int mp_guess_last_loss(mp_ctx* ctx, double* loss);
 /* One evaluation on caller-order inputs (needs mp_fs_create): guess (nv
  * entries) from p1 = p(t-1), p2 = p(t-2) and the right-hand side b (system
- * size); with p_true != NULL also the loss and its gradient (packed like the
- * parameters).  No parameter update. */
+ * size), with the network's correction on top of 2 p1 - p2; with p_true !=
+ * NULL also the loss and its gradient (packed like the parameters).  No
+ * parameter update. */
 int mp_guess_eval(mp_ctx* ctx, const double* p1, const double* p2, const double* b,
                   const double* p_true, double* guess, float* grad, double* loss);
 

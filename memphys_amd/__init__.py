@@ -38,7 +38,7 @@ HIP_SYMBOLS = (
     "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
     "mp_graph_mode", "mp_batch_upload", "mp_batch_params", "mp_batch_stats", "mp_autotune",
     "mp_operator_eigenvalue", "mp_knn_stencils", "mp_rcm_order", "mp_fs_body_force",
-    "mp_divergence_noalloc", "mp_precond_apply",
+    "mp_divergence_noalloc", "mp_precond_apply", "mp_guess_order",
 )
 
 
@@ -126,6 +126,7 @@ def hip_lib() -> C.CDLL:
         lib.mp_precond_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), vp, vp,
                                         C.POINTER(C.c_double)]
         lib.mp_precond_apply.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
+        lib.mp_guess_order.argtypes = [vp, C.c_int]
         lib.mp_knn_stencils.argtypes = [vp, C.c_int, C.c_int, _dp, _u8, _ip, _ip, _ip]
         lib.mp_rcm_order.argtypes = [vp, C.c_int, _ip, _ip, _ip]
         for name in HIP_SYMBOLS:
@@ -490,6 +491,10 @@ class Context:
 
     def guess_mode(self, mode: int):
         self._check(self.lib.mp_guess_mode(self.h, mode), "mp_guess_mode")
+
+    def guess_order(self, order: int):
+        """Pressures (2, 3 or 4) the extrapolated guess goes through; default 4."""
+        self._check(self.lib.mp_guess_order(self.h, order), "mp_guess_order")
 
     def guess_train(self, enable: bool, lr: float = 1e-3, seed: int = 0):
         self._check(self.lib.mp_guess_train(self.h, 1 if enable else 0, lr, seed), "mp_guess_train")

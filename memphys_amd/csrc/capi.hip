@@ -80,12 +80,13 @@ struct mp_ctx {
     int s_cur = 0;
     mp_step_stats stats{};
     int guess_mode = MP_GUESS_PREVIOUS;
+    int guess_order = 4;  // pressures the extrapolated guess goes through (2..4)
     // back-off of the learned guess: after a rejected guess the network is
     // skipped for guess_backoff steps (1, 2, 4, ... 64), an accepted one resets it
     int guess_backoff = 0, guess_skip = 0;
     DevBuf<double> div_partial;
     // pressure history and the learned guess
-    DevBuf<double> P1, P2, Ptmp;
+    DevBuf<double> P1, P2, P3, P4, Ptmp;  // pressures of the last four steps (P1 the latest)
     DevBuf<double> force;   // body force [3][npad] (mp_fs_body_force)
     bool has_force = false;
     int hist = 0;
@@ -241,15 +242,27 @@ void upload_flags(const mp_ctx* c, const uint8_t* src, DevBuf<uint8_t>& dev) {
     MP_HIP(hipMemcpy(dev.p, h.data(), h.size(), hipMemcpyHostToDevice));
 }
 
-// out = 2 p1 - p2 on the node block; the border unknown keeps p1's value
-__global__ __launch_bounds__(256) void k_extrapolate(int nv, int n, const double* __restrict__ p1,
+// The polynomial through the pressures of the last `order` steps (equal time
+// steps), at the next step, on the node block: 2 p1 - p2, 3 (p1 - p2) + p3,
+// or 4 (p1 + p3) - 6 p2 - p4 (p1 the latest); the border unknown keeps p1's
+// value.
+__global__ __launch_bounds__(256) void k_extrapolate(int nv, int n, int order,
+                                                     const double* __restrict__ p1,
                                                      const double* __restrict__ p2,
+                                                     const double* __restrict__ p3,
+                                                     const double* __restrict__ p4,
                                                      double* __restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < nv)
-        out[i] = 2.0 * p1[i] - p2[i];
-    else if (i < n)
+    if (i < nv) {
+        if (order == 2)
+            out[i] = 2.0 * p1[i] - p2[i];
+        else if (order == 3)
+            out[i] = 3.0 * (p1[i] - p2[i]) + p3[i];
+        else
+            out[i] = 4.0 * (p1[i] + p3[i]) - 6.0 * p2[i] - p4[i];
+    } else if (i < n) {
         out[i] = p1[i];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_copy(int n, const double* __restrict__ a,
@@ -960,6 +973,8 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
         alloc_vec(c, c->b);
         alloc_vec(c, c->P1);
         alloc_vec(c, c->P2);
+        alloc_vec(c, c->P3);
+        alloc_vec(c, c->P4);
         alloc_vec(c, c->Ptmp);
         c->hist = 0;
         if (c->net && c->nprob == 1) ensure_net(c);
@@ -1106,9 +1121,10 @@ void fs_step_one(mp_ctx* c, int it) {
     const int n_sys = c->ppe->n_sys;  // single problem: border unknown at row nv
     const int nv = c->nv;
     const bool extrapolate = c->guess_mode != MP_GUESS_PREVIOUS && c->hist >= 2;
+    const int order = std::min(c->guess_order, c->hist);  // (the history held so far)
     const uintptr_t key_pre[8] = {1, (uintptr_t)o, (uintptr_t)c->s_cur, (uintptr_t)euler,
-                                  (uintptr_t)force, (uintptr_t)extrapolate, (uintptr_t)bp,
-                                  (uintptr_t)pp};
+                                  (uintptr_t)force, (uintptr_t)(extrapolate ? order : 0),
+                                  (uintptr_t)bp, (uintptr_t)pp};
     run_step_part(c, key_pre, [&]() {
         // calc_vel_hat
         c->timers[1].begin(c->stream);
@@ -1125,7 +1141,7 @@ void fs_step_one(mp_ctx* c, int it) {
         c->timers[2].end(c->stream);
         if (extrapolate) {
             hipLaunchKernelGGL(k_extrapolate, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream,
-                               nv, n_sys, c->P1.p, c->P2.p, c->P.p);
+                               nv, n_sys, order, c->P1.p, c->P2.p, c->P3.p, c->P4.p, c->P.p);
             MP_HIP(hipGetLastError());
         }
     });
@@ -1144,7 +1160,7 @@ void fs_step_one(mp_ctx* c, int it) {
             hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad,
                                c->P.p, c->Ptmp.p);
             MP_HIP(hipGetLastError());
-            c->net->guess(c->stream, c->P1.p, c->P2.p, c->P.p);
+            c->net->guess(c->stream, c->Ptmp.p, c->P.p);
             if (!c->train) {
                 c->ppe->fallback_rr = c->ppe->last_residual_norm2(c->stream);
                 c->ppe->fallback_x = c->Ptmp.p;
@@ -1162,13 +1178,21 @@ void fs_step_one(mp_ctx* c, int it) {
         }
     }
     if (learned && c->train) {
-        c->net->backward(c->stream, c->P1.p, c->P2.p, c->P.p);
+        c->net->backward(c->stream, c->Ptmp.p, c->P.p);
         c->net->adam_step(c->stream);
         c->last_loss = c->net->loss_host(c->stream);
     }
-    const uintptr_t key_post[8] = {2, (uintptr_t)o, (uintptr_t)bp, (uintptr_t)pp, 0, 0, 0, 0};
+    const int depth = c->guess_order;  // (history kept: as many pressures as the order uses)
+    const uintptr_t key_post[8] = {2, (uintptr_t)o, (uintptr_t)bp, (uintptr_t)pp,
+                                   (uintptr_t)depth, 0, 0, 0};
     run_step_part(c, key_post, [&]() {
         // pressure history for the extrapolated / learned guesses
+        if (depth >= 4)
+            hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad,
+                               c->P3.p, c->P4.p);
+        if (depth >= 3)
+            hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad,
+                               c->P2.p, c->P3.p);
         hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad, c->P1.p,
                            c->P2.p);
         hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad, c->P.p,
@@ -1185,7 +1209,7 @@ void fs_step_one(mp_ctx* c, int it) {
                                  c->udir[1].p, c->udir[2].p, Uo[0].p, Uo[1].p, Uo[2].p, Un[0].p,
                                  Un[1].p, Un[2].p);
     });
-    c->hist = std::min(c->hist + 1, 2);
+    c->hist = std::min(c->hist + 1, depth);
     // extras (AB2 history) and the driver's old <- new
     if (q.temporal_order == 2) c->s_cur ^= 1;
     c->cur = nw;
@@ -1241,6 +1265,15 @@ int mp_guess_mode(mp_ctx* c, int mode) {
     });
 }
 
+int mp_guess_order(mp_ctx* c, int order) {
+    return guarded(c, "mp_guess_order", [&]() -> int {
+        if (order < 2 || order > 4) return fail(c, MP_ERR_ARG, "mp_guess_order: order 2, 3 or 4");
+        if (order > c->guess_order) c->hist = std::min(c->hist, c->guess_order);  // (kept so far)
+        c->guess_order = order;
+        return MP_OK;
+    });
+}
+
 int mp_guess_param_count(void) { return GNN_NPARAM; }
 
 int mp_guess_train(mp_ctx* c, int enable, double learning_rate, unsigned long long seed) {
@@ -1289,8 +1322,8 @@ int mp_guess_eval(mp_ctx* c, const double* p1, const double* p2, const double* b
         const int n_sys = c->ppe->n_sys;
         const PpeSolver* S = c->ppe.get();
         std::vector<double> h;
-        DevBuf<double> d1, d2, db, dt, dg;
-        for (DevBuf<double>* q : {&d1, &d2, &db, &dt, &dg}) alloc_vec(c, *q);
+        DevBuf<double> d1, d2, db, dt, dg, dx;
+        for (DevBuf<double>* q : {&d1, &d2, &db, &dt, &dg, &dx}) alloc_vec(c, *q);
         to_device_order(c, p1, h, S);
         upload(c, d1.p, h);
         to_device_order(c, p2, h, S);
@@ -1298,17 +1331,17 @@ int mp_guess_eval(mp_ctx* c, const double* p1, const double* p2, const double* b
         to_device_order(c, b, h, S);
         upload(c, db.p, h);
         hipLaunchKernelGGL(k_extrapolate, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, c->nv,
-                           n_sys, d1.p, d2.p, dg.p);
-        const double* r = c->ppe->residual_of(c->stream, db.p, dg.p);
+                           n_sys, 2, d1.p, d2.p, d2.p, d2.p, dx.p);
+        const double* r = c->ppe->residual_of(c->stream, db.p, dx.p);
         c->net->features(c->stream, d1.p, d2.p, r, c->ppe->diag_inv, c->xyz.p, c->npad, c->dim);
         c->net->forward(c->stream, p_true != nullptr);
-        c->net->guess(c->stream, d1.p, d2.p, dg.p);
+        c->net->guess(c->stream, dx.p, dg.p);
         download(c, dg.p, h);
         from_device_order(c, h, guess);
         if (p_true) {
             to_device_order(c, p_true, h, S);
             upload(c, dt.p, h);
-            c->net->backward(c->stream, d1.p, d2.p, dt.p);
+            c->net->backward(c->stream, dx.p, dt.p);
             double ls = c->net->loss_host(c->stream);
             if (loss) *loss = ls;
             if (grad) {

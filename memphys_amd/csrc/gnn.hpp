@@ -2,7 +2,8 @@
 // the stencil graph (3 layers, width 64, BF16 weights, FP32 accumulation).
 //
 //   features  X_i = [ (p1-p2)_i / s, (D^-1 (b - A p_ext))_i / s, x_i, y_i, z_i ],
-//             p_ext = 2 p1 - p2,  s = rms(p1 - p2)
+//             p_ext = the extrapolated pressure (2 p1 - p2, or through the
+//             last three or four pressures),  s = rms(p1 - p2)
 //   embed     h0 = relu(X W0 + b0)
 //   layer l   m_l = A~ h_{l-1}  (A~ = A D^-1 restricted to the node block)
 //             h_l = relu(h_{l-1} Ws_l + m_l Wn_l + b_l),   l = 1..3
@@ -58,9 +59,9 @@ struct GuessNet {
     // keep_pre: store the pre-activations, which a following backward() needs
     void forward(hipStream_t st, bool keep_pre = true);
     // guess = p_ext + s d  (FP64 output, node block only)
-    void guess(hipStream_t st, const double* p1, const double* p2, double* out);
+    void guess(hipStream_t st, const double* p_ext, double* out);
     // loss and gradient of the MSE against p_true (sets grad); returns nothing (loss in scal[1])
-    void backward(hipStream_t st, const double* p1, const double* p2, const double* p_true);
+    void backward(hipStream_t st, const double* p_ext, const double* p_true);
     void adam_step(hipStream_t st);
     double loss_host(hipStream_t st);
 };

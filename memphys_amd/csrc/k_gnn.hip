@@ -339,20 +339,19 @@ __global__ __launch_bounds__(256) void k_output(int nv, const float* __restrict_
     if (c == 0) d[row] = v + bf2f(bo[0]);
 }
 
-__global__ __launch_bounds__(256) void k_guess(int nv, const double* __restrict__ p1,
-                                               const double* __restrict__ p2,
+// guess = base (the extrapolated pressure) + s d on the node rows
+__global__ __launch_bounds__(256) void k_guess(int nv, const double* __restrict__ base,
                                                const float* __restrict__ d,
                                                const double* __restrict__ scal,
                                                double* __restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nv) return;
-    out[i] = (2.0 * p1[i] - p2[i]) + scal[0] * (double)d[i];
+    out[i] = base[i] + scal[0] * (double)d[i];
 }
 
-// e_i = d_i - t_i with t = (p* - p_ext)/s; dd_i = 2 e_i / nv; partial sums of e^2
+// e_i = d_i - t_i with t = (p* - base)/s; dd_i = 2 e_i / nv; partial sums of e^2
 __global__ __launch_bounds__(256) void k_loss(int nv, const float* __restrict__ d,
-                                              const double* __restrict__ p1,
-                                              const double* __restrict__ p2,
+                                              const double* __restrict__ base,
                                               const double* __restrict__ pt,
                                               const double* __restrict__ scal,
                                               float* __restrict__ dd, double* __restrict__ part) {
@@ -360,7 +359,7 @@ __global__ __launch_bounds__(256) void k_loss(int nv, const float* __restrict__ 
     double acc = 0.0;
     const double inv = 1.0 / scal[0];
     for (int i = blockIdx.x * 256 + threadIdx.x; i < nv; i += gridDim.x * 256) {
-        const double t = (pt[i] - (2.0 * p1[i] - p2[i])) * inv;
+        const double t = (pt[i] - base[i]) * inv;
         const double e = (double)d[i] - t;
         acc += e * e;
         dd[i] = (float)(2.0 * e / (double)nv);
@@ -643,16 +642,16 @@ void GuessNet::forward(hipStream_t st, bool keep_pre) {
     MP_HIP(hipGetLastError());
 }
 
-void GuessNet::guess(hipStream_t st, const double* p1, const double* p2, double* dst) {
-    hipLaunchKernelGGL(k_guess, dim3(blocks(nv)), dim3(256), 0, st, nv, p1, p2, out.p, scal.p, dst);
+void GuessNet::guess(hipStream_t st, const double* base, double* dst) {
+    hipLaunchKernelGGL(k_guess, dim3(blocks(nv)), dim3(256), 0, st, nv, base, out.p, scal.p, dst);
     MP_HIP(hipGetLastError());
 }
 
-void GuessNet::backward(hipStream_t st, const double* p1, const double* p2, const double* pt) {
+void GuessNet::backward(hipStream_t st, const double* base, const double* pt) {
     const uint16_t* P = theta_bf.p;
     const int nb = 256;
     const size_t nh = (size_t)nv * H;
-    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, st, nv, out.p, p1, p2, pt, scal.p, dbuf.p,
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, st, nv, out.p, base, pt, scal.p, dbuf.p,
                        partial.p);
     hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(64), 0, st, nb, nv, partial.p, scal.p);
     // output layer: grad wo, bo; dH3

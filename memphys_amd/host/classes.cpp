@@ -94,6 +94,9 @@ void PARAMETERS::read_gpu_settings(std::string settings_file) {
                 else bad();
             } else if (key == "guess_params") {
                 guess_params_file = val;
+            } else if (key == "extrapolation_order") {
+                if (val == "2" || val == "3" || val == "4") guess_order = std::stoi(val);
+                else bad();
             } else if (key == "graphs") {
                 if (val == "0" || val == "1") graphs = std::stoi(val);
                 else bad();
@@ -668,6 +671,7 @@ void FRACTIONAL_STEP_1::create(POINTS& points, CLOUD& cloud, PARAMETERS& paramet
     parameters.factoring_timer = (double)(clock() - t1) / CLOCKS_PER_SEC;
     check(ctx, mp_solver_system_size(ctx, &system_size), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
     check(ctx, mp_graph_mode(ctx, parameters.graphs), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
+    check(ctx, mp_guess_order(ctx, parameters.guess_order), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
     if (parameters.guess == MP_GUESS_LEARNED) {
         std::vector<float> w(mp_guess_param_count());
         std::ifstream f(parameters.guess_params_file, std::ios::binary);
