@@ -19,7 +19,9 @@
 // one product on the way down and one on the way up:
 //   restrict + pre-smooth  b_l = R_{l-1} res_{l-1},  res_l = Q_l res_{l-1},
 //                          Q_l = R_{l-1} - A_l D_l R_{l-1}        (levels 2 .. L-2)
-//   coarsest solve         y_{L-1} = (G R_{L-2}) res_{L-2}        (dense rows)
+//   coarsest solve         y_{L-1} = (G R_{L-2}) res_{L-2}        (dense rows; one
+//                          problem: a batch's many small problems keep
+//                          R_{L-2} and the block-diagonal G, fewer bytes)
 //   prolong + post-smooth  y_l = D_l (b_l + res_l) + T_l y_{l+1},
 //                          T_l = P_l - D_l A_l P_l                (levels 1 .. L-2)
 // (with x_l = D_l b_l after pre-smoothing: x_l + P_l y + D_l (b_l - A_l (x_l +
@@ -755,11 +757,12 @@ void AmgDevice::setup(hipStream_t st, const SellView& sv, const double* val64_de
                 lv[l].Q.val2 = lv[l].Q.val2_b.p;
             }
         }
-        // the coarsest solve times R_{L-2}, one dense row per coarsest row
+        // one problem: the coarsest solve times R_{L-2}, one dense row per
+        // coarsest row
         std::vector<uint16_t> cv;
         std::vector<int> cf(nc), cs(nc);
         std::vector<long> cd(nc);
-        for (int q = 0; q < nb; ++q) {
+        for (int q = 0; q < nb && nb == 1; ++q) {
             const int m = hs[q].nc;
             const mp_amg::Csr& Rq = hs[q].R[L - 2];
             if (Rq.nrows != m) throw std::runtime_error("amg: coarsest level size");
@@ -835,21 +838,24 @@ void AmgDevice::retune(hipStream_t st) {
     try {
         for (int l = 0; l + 1 < L; ++l) {
             AmgLevelDev& d = lv[l];
-            // the composed cycle uses A_1, P_0, R_0 and the T_l (the Q_l keep
-            // their CSR layout); the plain cycle every A_l, P_l, R_l
-            const bool plain = !composed || l == 0;
-            if (l > 0 && (plain || l == 1))
+            // the composed cycle uses A_1, P_0, R_0, the T_l (the Q_l keep
+            // their CSR layout) and, in a batch, R_{L-2}; the plain cycle
+            // every A_l, P_l, R_l
+            const bool useA = l > 0 && (!composed || l == 1);
+            const bool useP = !composed || l == 0;
+            const bool useR = !composed || l == 0 || (blocked && l == L - 2);
+            if (useA)
                 upload_tuned(d.A, hA[l], st, tx.p, ty.p, e0, e1, fl, layout,
                              ("A" + std::to_string(l)).c_str());
-            if (plain) {
+            if (useP)
                 upload_tuned(d.P, hP[l], st, tx.p, ty.p, e0, e1, fl, layout,
                              ("P" + std::to_string(l)).c_str());
+            if (useR)
                 upload_tuned(d.R, hR[l], st, tx.p, ty.p, e0, e1, fl, layout,
                              ("R" + std::to_string(l)).c_str());
-            } else {
+            if (composed && l >= 1)
                 upload_tuned(d.T, hT[l], st, tx.p, ty.p, e0, e1, fl, layout,
                              ("T" + std::to_string(l)).c_str());
-            }
         }
     } catch (...) {
         (void)hipEventDestroy(e0);
@@ -921,9 +927,17 @@ void AmgDevice::vcycle(hipStream_t st, const TI* in, TO* out, const double* bin,
                 launch_op<OP_RQ, float, float>(st, lv[l].Q, nullptr, (const float*)nullptr,
                                                lv[l - 1].res.p, nullptr, lv[l].b.p, lv[l].res.p,
                                                (float*)nullptr, nullptr, nullptr, done);
-            hipLaunchKernelGGL(k_amg_rowdense, dim3(nc), dim3(RD_THREADS), 0, st, nc, cr_val.p,
-                               cr_first.p, cr_size.p, cr_data.p, (const float*)lv[L - 2].res.p,
-                               lv[L - 1].y.p, done);
+            if (!blocked) {
+                hipLaunchKernelGGL(k_amg_rowdense, dim3(nc), dim3(RD_THREADS), 0, st, nc,
+                                   cr_val.p, cr_first.p, cr_size.p, cr_data.p,
+                                   (const float*)lv[L - 2].res.p, lv[L - 1].y.p, done);
+            } else {
+                launch_op<OP_SPMV, float, float>(st, lv[L - 2].R, nullptr, (const float*)nullptr,
+                                                 lv[L - 2].res.p, nullptr, nullptr, nullptr,
+                                                 lv[L - 1].b.p, nullptr, nullptr, done);
+                coarse((const float*)lv[L - 1].b.p, lv[L - 1].y.p, (float*)nullptr,
+                       (const double*)nullptr, (double*)nullptr);
+            }
             for (int l = L - 2; l >= 1; --l)
                 launch_op<OP_TPOST, float, float>(st, lv[l].T, lv[l].dinv.p, lv[l].b.p,
                                                   lv[l + 1].y.p, lv[l].res.p, nullptr, nullptr,
