@@ -48,7 +48,8 @@ enum mp_precision { MP_PREC_FP64 = 0, MP_PREC_MIXED = 1 };
  * or 4 steps (mp_guess_order; equal time steps), evaluated at the next step.
  * The last two fall back to the previous pressure until two pressures are in
  * the history, and use the highest order the history holds until it is
- * full.  Single problems only (a batch starts from the previous pressure). */
+ * full.  The learned guess is for single problems; a batch may extrapolate
+ * (every problem's rows, border unknowns included). */
 enum mp_guess { MP_GUESS_PREVIOUS = 0, MP_GUESS_LEARNED = 1, MP_GUESS_EXTRAPOLATE = 2 };
 
 /* PPE preconditioner: Jacobi (right diagonal scaling, the plain path), or a
@@ -230,7 +231,8 @@ int mp_guess_params(mp_ctx* ctx, int set, float* params);
 int mp_guess_last_loss(mp_ctx* ctx, double* loss);
 /* One evaluation on caller-order inputs (needs mp_fs_create): guess (nv
  * entries) from p1 = p(t-1), p2 = p(t-2) and the right-hand side b (system
- * size), with the network's correction on top of 2 p1 - p2; with p_true !=
+ * size), with the network's correction on top of 2 p1 - p2 (the border
+ * unknown taken from p1 in the residual feature); with p_true !=
  * NULL also the loss and its gradient (packed like the parameters).  No
  * parameter update. */
 int mp_guess_eval(mp_ctx* ctx, const double* p1, const double* p2, const double* b,

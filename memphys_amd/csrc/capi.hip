@@ -243,26 +243,26 @@ void upload_flags(const mp_ctx* c, const uint8_t* src, DevBuf<uint8_t>& dev) {
 }
 
 // The polynomial through the pressures of the last `order` steps (equal time
-// steps), at the next step, on the node block: 2 p1 - p2, 3 (p1 - p2) + p3,
-// or 4 (p1 + p3) - 6 p2 - p4 (p1 the latest); the border unknown keeps p1's
-// value.
-__global__ __launch_bounds__(256) void k_extrapolate(int nv, int n, int order,
+// steps), at the next step: 2 p1 - p2, 3 (p1 - p2) + p3, or 4 (p1 + p3) -
+// 6 p2 - p4 (p1 the latest), on rows below n_ext; rows n_ext .. n-1 keep p1.
+// The time step extrapolates every device row (node rows and border unknowns
+// of every problem; padding rows hold zeros throughout).
+__global__ __launch_bounds__(256) void k_extrapolate(int n_ext, int n, int order,
                                                      const double* __restrict__ p1,
                                                      const double* __restrict__ p2,
                                                      const double* __restrict__ p3,
                                                      const double* __restrict__ p4,
                                                      double* __restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < nv) {
-        if (order == 2)
-            out[i] = 2.0 * p1[i] - p2[i];
-        else if (order == 3)
-            out[i] = 3.0 * (p1[i] - p2[i]) + p3[i];
-        else
-            out[i] = 4.0 * (p1[i] + p3[i]) - 6.0 * p2[i] - p4[i];
-    } else if (i < n) {
+    if (i >= n) return;
+    if (i >= n_ext)
         out[i] = p1[i];
-    }
+    else if (order == 2)
+        out[i] = 2.0 * p1[i] - p2[i];
+    else if (order == 3)
+        out[i] = 3.0 * (p1[i] - p2[i]) + p3[i];
+    else
+        out[i] = 4.0 * (p1[i] + p3[i]) - 6.0 * p2[i] - p4[i];
 }
 
 __global__ __launch_bounds__(256) void k_copy(int n, const double* __restrict__ a,
@@ -1118,8 +1118,6 @@ void fs_step_one(mp_ctx* c, int it) {
     const int* bp = c->per_problem_prm ? c->blk_prob.p : nullptr;
     const double* pp = c->per_problem_prm ? c->prob_prm.p : nullptr;
     const double* force = c->has_force ? c->force.p : nullptr;
-    const int n_sys = c->ppe->n_sys;  // single problem: border unknown at row nv
-    const int nv = c->nv;
     const bool extrapolate = c->guess_mode != MP_GUESS_PREVIOUS && c->hist >= 2;
     const int order = std::min(c->guess_order, c->hist);  // (the history held so far)
     const uintptr_t key_pre[8] = {1, (uintptr_t)o, (uintptr_t)c->s_cur, (uintptr_t)euler,
@@ -1140,8 +1138,9 @@ void fs_step_one(mp_ctx* c, int it) {
                             c->bnd.p, c->pdir.p, c->normal.p, c->npad, q.rho, q.dt, c->b.p, bp, pp);
         c->timers[2].end(c->stream);
         if (extrapolate) {
-            hipLaunchKernelGGL(k_extrapolate, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream,
-                               nv, n_sys, order, c->P1.p, c->P2.p, c->P3.p, c->P4.p, c->P.p);
+            hipLaunchKernelGGL(k_extrapolate, dim3(c->npad / 256), dim3(256), 0, c->stream,
+                               c->npad, c->npad, order, c->P1.p, c->P2.p, c->P3.p, c->P4.p,
+                               c->P.p);
             MP_HIP(hipGetLastError());
         }
     });
@@ -1253,8 +1252,8 @@ int mp_guess_mode(mp_ctx* c, int mode) {
     return guarded(c, "mp_guess_mode", [&]() -> int {
         if (mode != MP_GUESS_PREVIOUS && mode != MP_GUESS_LEARNED && mode != MP_GUESS_EXTRAPOLATE)
             return fail(c, MP_ERR_ARG, "mp_guess_mode: unknown mode");
-        if (mode != MP_GUESS_PREVIOUS && c->nprob > 1)
-            return fail(c, MP_ERR_ARG, "mp_guess_mode: batches use the previous pressure");
+        if (mode == MP_GUESS_LEARNED && c->nprob > 1)
+            return fail(c, MP_ERR_ARG, "mp_guess_mode: the learned guess is for single problems");
         if (mode == MP_GUESS_LEARNED) {
             if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_mode: mp_fs_create not called");
             if (!c->net) ensure_net(c);
@@ -1319,7 +1318,6 @@ int mp_guess_eval(mp_ctx* c, const double* p1, const double* p2, const double* b
         if (!p1 || !p2 || !b || !guess) return fail(c, MP_ERR_ARG, "mp_guess_eval: NULL");
         if (!c->net) ensure_net(c);
         if (c->nprob > 1) return fail(c, MP_ERR_ARG, "mp_guess_eval: single problems only");
-        const int n_sys = c->ppe->n_sys;
         const PpeSolver* S = c->ppe.get();
         std::vector<double> h;
         DevBuf<double> d1, d2, db, dt, dg, dx;
@@ -1330,8 +1328,10 @@ int mp_guess_eval(mp_ctx* c, const double* p1, const double* p2, const double* b
         upload(c, d2.p, h);
         to_device_order(c, b, h, S);
         upload(c, db.p, h);
-        hipLaunchKernelGGL(k_extrapolate, dim3((n_sys + 255) / 256), dim3(256), 0, c->stream, c->nv,
-                           n_sys, 2, d1.p, d2.p, d2.p, d2.p, dx.p);
+        // (the two-pressure form: 2 p1 - p2 on the node rows, the border
+        // unknown keeps p1)
+        hipLaunchKernelGGL(k_extrapolate, dim3(c->npad / 256), dim3(256), 0, c->stream, c->nv,
+                           c->npad, 2, d1.p, d2.p, d2.p, d2.p, dx.p);
         const double* r = c->ppe->residual_of(c->stream, db.p, dx.p);
         c->net->features(c->stream, d1.p, d2.p, r, c->ppe->diag_inv, c->xyz.p, c->npad, c->dim);
         c->net->forward(c->stream, p_true != nullptr);
