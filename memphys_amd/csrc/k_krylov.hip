@@ -589,7 +589,8 @@ __global__ void k_gm_reset(int nseg, GmState* gm, GmFlags* fl, const SegCtl* ctl
 // for the systems starting (mode 0) or restarting (mode 1: a full cycle,
 // still running), and the V-cycle's FP32 input (d0 v0, or v0 without the
 // scaling) over the node rows; partial sums of v0.v0, rhs.rhs and the border
-// row's D^-1 x.
+// row's D^-1 x.  x_zero: x is known to be zero (a defect-correction cycle's
+// start), so A x is not formed.
 template <typename TM, typename TV>
 __global__ __launch_bounds__(BLOCK) void k_gm_residual(SellView sv, SegView sg,
                                                        const TM* __restrict__ a,
@@ -600,7 +601,7 @@ __global__ __launch_bounds__(BLOCK) void k_gm_residual(SellView sv, SegView sg,
                                                        const float* __restrict__ d0,
                                                        float* __restrict__ xs,
                                                        double* __restrict__ partial,
-                                                       const GmState* gm, int mode) {
+                                                       const GmState* gm, int mode, int x_zero) {
     const int nbk = sg.nrows / BLOCK;
     const int lb = xcd_block(nbk);
     if (lb < 0) return;
@@ -613,16 +614,16 @@ __global__ __launch_bounds__(BLOCK) void k_gm_residual(SellView sv, SegView sg,
     const int s = wave_slice(lb);
     double vals[3] = {0.0, 0.0, 0.0};
     if (s < sv.nslices) {
-        double ax = sell_dot<double>(sv, a, x, s, threadIdx.x & 63);
+        double ax = x_zero ? 0.0 : sell_dot<double>(sv, a, x, s, threadIdx.x & 63);
         if (i < iend) {
-            if (S.border) ax = ax + x[iend];
+            if (S.border && !x_zero) ax = ax + x[iend];
             const double ri = rhs[i] - ax;
             const TV vi = (TV)ri;
             v0[i] = vi;
             if (xs) xs[i] = d0 ? d0[i] * (float)vi : (float)vi;
             vals[0] = ri * ri;
             vals[1] = rhs[i] * rhs[i];
-            if (S.border) vals[2] = dinv[i] * x[i];
+            if (S.border && !x_zero) vals[2] = dinv[i] * x[i];
         }
     }
     block_store<3>(vals, partial, lb, nbk);
@@ -1504,7 +1505,7 @@ void PpeSolver::gmres_iteration(hipStream_t st, int k, int m, const TM* a, const
 template <typename TM, typename TV>
 void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const double* rhs,
                           double* x, int rel_to_rhs, std::vector<int>& done,
-                          std::vector<int>& iters, std::vector<int>& restarts) {
+                          std::vector<int>& iters, std::vector<int>& restarts, bool x_zero) {
     const int nbk = nrows / BLOCK;
     const int m = std::max(1, std::min(GM_MAX, restart_m));
     gmres_alloc(st, m);
@@ -1520,7 +1521,7 @@ void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const d
     auto start = [&](int mode) {
         hipLaunchKernelGGL((k_gm_residual<TM, TV>), dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st,
                            view_of<TM>(), sg, a, di, rhs, x, V, d0, xs, gm_partial.p, gm_d.p,
-                           mode);
+                           mode, (mode == 0 && x_zero) ? 1 : 0);
         hipLaunchKernelGGL(k_gm_residual_finish<TV>, dim3(nseg), dim3(FIN), 0, st, sg, rhs, V,
                            gm_partial.p, gm_d.p, gmf_d.p, mode, rel_to_rhs);
         MP_HIP(hipGetLastError());
@@ -1707,7 +1708,8 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
             launch_copy(st, rhs64.p, tmp.p, nrows);
             launch_fill(st, e64.p, 0.0, nrows);
             if (krylov == 1)
-                run_gmres<float, float>(st, a32k, dinv.p, rhs64.p, e64.p, 1, done, its, rst);
+                run_gmres<float, float>(st, a32k, dinv.p, rhs64.p, e64.p, 1, done, its, rst,
+                                        true);
             else
                 run_bicgstab<float, double>(st, a32k, dinv.p, rhs64.p, e64.p, r.p, r0.p, p.p,
                                             v.p, s.p, t.p, 1, done, its, rst);

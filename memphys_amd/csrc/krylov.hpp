@@ -173,11 +173,12 @@ struct PpeSolver {
     // GMRES(restart_m) on operator values aval_t (FP64 x and right-hand
     // side, basis vectors of type TV), right-preconditioned by the V-cycle
     // when there is one; same controls and outcome codes as run_bicgstab
-    // (restarts = GMRES cycles after the first).
+    // (restarts = GMRES cycles after the first).  x_zero: x is zero on entry
+    // (the first residual is rhs itself).
     template <typename TM, typename TV>
     void run_gmres(hipStream_t s, const TM* aval_t, const double* dinv_t, const double* rhs,
                    double* x, int rel_to_rhs, std::vector<int>& done, std::vector<int>& iters,
-                   std::vector<int>& restarts);
+                   std::vector<int>& restarts, bool x_zero = false);
     template <typename TM, typename TV>
     void gmres_iteration(hipStream_t s, int k, int m, const TM* aval_t, const double* dinv_t);
     void gmres_alloc(hipStream_t s, int m);
