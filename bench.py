@@ -97,6 +97,8 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
     log(f"[{label}] fs_create {t_create:.2f}s, hierarchy {hier}")
     if guess is None:
         ctx.guess_mode(M.MP_GUESS_PREVIOUS)
+    elif guess == "project":
+        ctx.guess_mode(M.MP_GUESS_PROJECT)
     elif isinstance(guess, str):
         ctx.guess_mode(M.MP_GUESS_EXTRAPOLATE)
     else:
@@ -127,7 +129,7 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
     else:
         interval = prm["check_every"]
     first = it
-    iters, res, dts, rej, skipped = [], [], [], 0, 0
+    iters, res, dts, rej, skipped, cycles, g_res = [], [], [], 0, 0, [], []
     ctx.sync()
     t0 = time.perf_counter()
     tp = t0
@@ -139,6 +141,8 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
         tp = tn
         iters.append(st.iterations)
         res.append(st.rel_res)
+        cycles.append(st.outer_cycles)
+        g_res.append(st.guess_rel_res)
         rej += st.guess_rejected
         skipped += st.guess_skipped
         if save_at is not None and it == save_at:
@@ -163,7 +167,8 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
     return dict(seconds=float(sum(dts)), wall=t1 - t0, first=first, step_s=dts, iters=iters,
                 rel_res=res, spmv=spmv, predictor=pred, rhs=rhs, corrector=corr, smooth=smooth, fields=out_fields,
                 divergence=div.tolist(), hierarchy=hier, create_s=t_create,
-                poll_interval=interval, saved=saved, guess_rejected=rej, guess_skipped=skipped)
+                poll_interval=interval, saved=saved, guess_rejected=rej, guess_skipped=skipped,
+                cycles=cycles, guess_res=g_res)
 
 
 def step_kernels(form, copy_gbps):
@@ -188,8 +193,12 @@ def window_seconds(form, lo, hi):
 
 
 def form_summary(f):
+    g = np.asarray(f["guess_res"], dtype=float)
+    g = g[np.isfinite(g)]
     return {"timesteps_per_s": len(f["step_s"]) / f["seconds"], "steps": [f["first"],
             f["first"] + len(f["step_s"]) - 1], "ppe_iters_per_step": float(np.mean(f["iters"])),
+            "outer_cycles_per_step": float(np.mean(f["cycles"])),
+            "guess_rel_res_median": float(np.median(g)) if g.size else None,
             "max_ppe_rel_res": max(f["rel_res"]), "poll_interval": f["poll_interval"]}
 
 
@@ -633,6 +642,11 @@ def main():
         full["mixed_multigrid"] = run_form(M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps,
                                            args.warmup, "mixed+multigrid", precond=M.MP_PRECOND_AMG,
                                            tune=tune, krylov=kry)
+    if "project" in forms:
+        full["mixed_multigrid_project"] = run_form(
+            M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps, args.warmup,
+            "mixed+multigrid+projection", guess="project", precond=M.MP_PRECOND_AMG,
+            tune=tune, krylov=kry)
     if "extrapolate" in forms:
         full["mixed_multigrid_extrapolate"] = run_form(
             M, ctx, cloud, prm, M.MP_PREC_MIXED, args.steps, args.warmup,

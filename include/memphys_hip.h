@@ -43,14 +43,25 @@ enum mp_precision { MP_PREC_FP64 = 0, MP_PREC_MIXED = 1 };
 
 /* Initial guess for each pressure solve: the previous pressure (reference
  * behaviour, solveWithGuess(rhs, p_old)), the learned message-passing guess
- * (the extrapolation in time plus the network's correction), or the
+ * (the extrapolation in time plus the network's correction), the
  * extrapolation alone: the polynomial through the pressures of the last 2, 3
- * or 4 steps (mp_guess_order; equal time steps), evaluated at the next step.
- * The last two fall back to the previous pressure until two pressures are in
- * the history, and use the highest order the history holds until it is
- * full.  The learned guess is for single problems; a batch may extrapolate
- * (every problem's rows, border unknowns included). */
-enum mp_guess { MP_GUESS_PREVIOUS = 0, MP_GUESS_LEARNED = 1, MP_GUESS_EXTRAPOLATE = 2 };
+ * or 4 steps (mp_guess_order; equal time steps), evaluated at the next step,
+ * or the projection: the combination of the last K solutions (mp_guess_window,
+ * default 8) whose residual against the new right-hand side is smallest
+ * (Fischer's successive-right-hand-side projection; one FP64 product with the
+ * operator per step stores A p of each solution).  The extrapolating guesses
+ * fall back to the previous pressure until two pressures are in the history,
+ * and use the highest order the history holds until it is full; the
+ * projection uses the solutions held so far.  The learned and projected
+ * guesses are for single problems; a batch may extrapolate (every problem's
+ * rows, border unknowns included).  No counterpart in the reference beyond
+ * MP_GUESS_PREVIOUS. */
+enum mp_guess {
+    MP_GUESS_PREVIOUS = 0,
+    MP_GUESS_LEARNED = 1,
+    MP_GUESS_EXTRAPOLATE = 2,
+    MP_GUESS_PROJECT = 3
+};
 
 /* PPE preconditioner: Jacobi (right diagonal scaling, the plain path), or a
  * smoothed-aggregation algebraic multigrid V(1,1)-cycle in FP32 (set up on the
@@ -217,6 +228,8 @@ int mp_guess_mode(mp_ctx* ctx, int mode);
  * and 8.3 in norm): with loose tolerances a lower order can start closer.
  * No counterpart in the reference (its guess is the previous pressure). */
 int mp_guess_order(mp_ctx* ctx, int order);
+/* Solutions the projected guess combines (1..8, default 8). */
+int mp_guess_window(mp_ctx* ctx, int k);
 
 /* Learned guess: a 3-layer message-passing network over the stencil graph
  * (width 64, BF16 weights, FP32 accumulation).  The parameters are a packed

@@ -24,7 +24,7 @@ MP_U, MP_V, MP_W, MP_P = 0, 1, 2, 3
 MP_SU, MP_SV, MP_SW = 4, 5, 6  # AB2 source history (exact restart)
 MP_GX, MP_GY, MP_GZ, MP_LAP = 0, 1, 2, 3
 MP_PREC_FP64, MP_PREC_MIXED = 0, 1
-MP_GUESS_PREVIOUS, MP_GUESS_LEARNED, MP_GUESS_EXTRAPOLATE = 0, 1, 2
+MP_GUESS_PREVIOUS, MP_GUESS_LEARNED, MP_GUESS_EXTRAPOLATE, MP_GUESS_PROJECT = 0, 1, 2, 3
 MP_PRECOND_JACOBI, MP_PRECOND_AMG = 0, 1
 MP_KRYLOV_BICGSTAB, MP_KRYLOV_GMRES = 0, 1
 
@@ -38,7 +38,7 @@ HIP_SYMBOLS = (
     "mp_guess_params", "mp_guess_last_loss", "mp_guess_eval", "mp_precond_info",
     "mp_graph_mode", "mp_batch_upload", "mp_batch_params", "mp_batch_stats", "mp_autotune",
     "mp_operator_eigenvalue", "mp_knn_stencils", "mp_rcm_order", "mp_fs_body_force",
-    "mp_divergence_noalloc", "mp_precond_apply", "mp_guess_order",
+    "mp_divergence_noalloc", "mp_precond_apply", "mp_guess_order", "mp_guess_window",
 )
 
 
@@ -127,6 +127,7 @@ def hip_lib() -> C.CDLL:
                                         C.POINTER(C.c_double)]
         lib.mp_precond_apply.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
         lib.mp_guess_order.argtypes = [vp, C.c_int]
+        lib.mp_guess_window.argtypes = [vp, C.c_int]
         lib.mp_knn_stencils.argtypes = [vp, C.c_int, C.c_int, _dp, _u8, _ip, _ip, _ip]
         lib.mp_rcm_order.argtypes = [vp, C.c_int, _ip, _ip, _ip]
         for name in HIP_SYMBOLS:
@@ -495,6 +496,10 @@ class Context:
     def guess_order(self, order: int):
         """Pressures (2, 3 or 4) the extrapolated guess goes through; default 4."""
         self._check(self.lib.mp_guess_order(self.h, order), "mp_guess_order")
+
+    def guess_window(self, k: int):
+        """Solutions (1..8) the projected guess combines; default 8."""
+        self._check(self.lib.mp_guess_window(self.h, k), "mp_guess_window")
 
     def guess_train(self, enable: bool, lr: float = 1e-3, seed: int = 0):
         self._check(self.lib.mp_guess_train(self.h, 1 if enable else 0, lr, seed), "mp_guess_train")

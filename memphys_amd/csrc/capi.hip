@@ -90,6 +90,11 @@ struct mp_ctx {
     DevBuf<double> force;   // body force [3][npad] (mp_fs_body_force)
     bool has_force = false;
     int hist = 0;
+    // projection guess: rings of the last PROJ_MAX pressures and their
+    // products with the operator, the device record, entries held, window
+    DevBuf<double> Hp, Hw, proj_partial;
+    DevBuf<ProjState> pstate;
+    int pcount = 0, proj_window = PROJ_MAX;
     std::unique_ptr<GuessNet> net;
     bool train = false;
     bool use_graphs = true;
@@ -240,6 +245,20 @@ void upload_flags(const mp_ctx* c, const uint8_t* src, DevBuf<uint8_t>& dev) {
         for (int o = 0; o < c->nv; ++o) h[c->iperm[o]] = src[o] ? 1 : 0;
     dev.alloc(h.size());
     MP_HIP(hipMemcpy(dev.p, h.data(), h.size(), hipMemcpyHostToDevice));
+}
+
+// Rings of the projection guess for the current rows, emptied.
+void ensure_projection(mp_ctx* c) {
+    const size_t ring = (size_t)PROJ_MAX * c->npad;
+    if (c->Hp.n < ring) {
+        c->release_step_graphs();  // (captured steps hold the buffers replaced here)
+        c->Hp.alloc(ring);
+        c->Hw.alloc(ring);
+    }
+    if (!c->pstate.p) c->pstate.alloc(1);
+    if (c->proj_partial.n < proj_partial_size()) c->proj_partial.alloc(proj_partial_size());
+    launch_proj_reset(c->stream, c->pstate.p);
+    c->pcount = 0;
 }
 
 // The polynomial through the pressures of the last `order` steps (equal time
@@ -977,6 +996,7 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
         alloc_vec(c, c->P4);
         alloc_vec(c, c->Ptmp);
         c->hist = 0;
+        if (c->guess_mode == MP_GUESS_PROJECT) ensure_projection(c);  // (rows may have changed)
         if (c->net && c->nprob == 1) ensure_net(c);
         for (int q = 0; q < 3; ++q) alloc_vec(c, c->uh[q]);
         c->upk.alloc((size_t)4 * c->npad);
@@ -1013,6 +1033,7 @@ int mp_fields_set(mp_ctx* c, int which, const double* values) {
             to_device_order(c, values, h, c->ppe.get());
             upload(c, c->P.p, h);
             c->hist = 0;  // the pressure history no longer matches
+            c->pcount = 0;
         } else if (which >= MP_SU) {  // AB2 source history, component which - MP_SU
             to_device_order(c, values, h);
             h.resize(c->npad);
@@ -1118,10 +1139,14 @@ void fs_step_one(mp_ctx* c, int it) {
     const int* bp = c->per_problem_prm ? c->blk_prob.p : nullptr;
     const double* pp = c->per_problem_prm ? c->prob_prm.p : nullptr;
     const double* force = c->has_force ? c->force.p : nullptr;
-    const bool extrapolate = c->guess_mode != MP_GUESS_PREVIOUS && c->hist >= 2;
+    const bool extrapolate =
+        (c->guess_mode == MP_GUESS_EXTRAPOLATE || c->guess_mode == MP_GUESS_LEARNED) && c->hist >= 2;
+    const bool project = c->guess_mode == MP_GUESS_PROJECT && c->pcount >= 1;
+    const int kp = std::min(c->pcount, c->proj_window);
     const int order = std::min(c->guess_order, c->hist);  // (the history held so far)
     const uintptr_t key_pre[8] = {1, (uintptr_t)o, (uintptr_t)c->s_cur, (uintptr_t)euler,
-                                  (uintptr_t)force, (uintptr_t)(extrapolate ? order : 0),
+                                  (uintptr_t)force,
+                                  (uintptr_t)(project ? 16 + kp : extrapolate ? order : 0),
                                   (uintptr_t)bp, (uintptr_t)pp};
     run_step_part(c, key_pre, [&]() {
         // calc_vel_hat
@@ -1142,6 +1167,9 @@ void fs_step_one(mp_ctx* c, int it) {
                                c->npad, c->npad, order, c->P1.p, c->P2.p, c->P3.p, c->P4.p,
                                c->P.p);
             MP_HIP(hipGetLastError());
+        } else if (project) {
+            launch_proj_guess(c->stream, kp, c->npad, c->Hp.p, c->Hw.p, (size_t)c->npad,
+                              c->pstate.p, c->b.p, c->proj_partial.p, c->P.p);
         }
     });
     bool learned = false, skipped = false;
@@ -1182,8 +1210,9 @@ void fs_step_one(mp_ctx* c, int it) {
         c->last_loss = c->net->loss_host(c->stream);
     }
     const int depth = c->guess_order;  // (history kept: as many pressures as the order uses)
+    const bool keep_proj = c->guess_mode == MP_GUESS_PROJECT;
     const uintptr_t key_post[8] = {2, (uintptr_t)o, (uintptr_t)bp, (uintptr_t)pp,
-                                   (uintptr_t)depth, 0, 0, 0};
+                                   (uintptr_t)depth, (uintptr_t)keep_proj, 0, 0};
     run_step_part(c, key_post, [&]() {
         // pressure history for the extrapolated / learned guesses
         if (depth >= 4)
@@ -1197,6 +1226,11 @@ void fs_step_one(mp_ctx* c, int it) {
         hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad, c->P.p,
                            c->P1.p);
         MP_HIP(hipGetLastError());
+        if (keep_proj) {  // the solution and A p = b - (b - A p) as the newest entry
+            const double* r = c->ppe->residual_of(c->stream, c->b.p, c->P.p);
+            launch_proj_store(c->stream, c->npad, c->P.p, c->b.p, r, c->Hp.p, c->Hw.p,
+                              (size_t)c->npad, c->pstate.p);
+        }
         // calc_vel_corr
         c->timers[3].begin(c->stream);
         launch_corrector<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->P.p,
@@ -1209,6 +1243,7 @@ void fs_step_one(mp_ctx* c, int it) {
                                  Un[1].p, Un[2].p);
     });
     c->hist = std::min(c->hist + 1, depth);
+    if (keep_proj) c->pcount = std::min(c->pcount + 1, PROJ_MAX);
     // extras (AB2 history) and the driver's old <- new
     if (q.temporal_order == 2) c->s_cur ^= 1;
     c->cur = nw;
@@ -1250,10 +1285,16 @@ int mp_stats_get(mp_ctx* c, mp_step_stats* stats) {
 
 int mp_guess_mode(mp_ctx* c, int mode) {
     return guarded(c, "mp_guess_mode", [&]() -> int {
-        if (mode != MP_GUESS_PREVIOUS && mode != MP_GUESS_LEARNED && mode != MP_GUESS_EXTRAPOLATE)
+        if (mode != MP_GUESS_PREVIOUS && mode != MP_GUESS_LEARNED &&
+            mode != MP_GUESS_EXTRAPOLATE && mode != MP_GUESS_PROJECT)
             return fail(c, MP_ERR_ARG, "mp_guess_mode: unknown mode");
-        if (mode == MP_GUESS_LEARNED && c->nprob > 1)
-            return fail(c, MP_ERR_ARG, "mp_guess_mode: the learned guess is for single problems");
+        if ((mode == MP_GUESS_LEARNED || mode == MP_GUESS_PROJECT) && c->nprob > 1)
+            return fail(c, MP_ERR_ARG,
+                        "mp_guess_mode: the learned and projected guesses are for single problems");
+        if (mode == MP_GUESS_PROJECT) {
+            if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_mode: mp_fs_create not called");
+            ensure_projection(c);
+        }
         if (mode == MP_GUESS_LEARNED) {
             if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_mode: mp_fs_create not called");
             if (!c->net) ensure_net(c);
@@ -1269,6 +1310,14 @@ int mp_guess_order(mp_ctx* c, int order) {
         if (order < 2 || order > 4) return fail(c, MP_ERR_ARG, "mp_guess_order: order 2, 3 or 4");
         if (order > c->guess_order) c->hist = std::min(c->hist, c->guess_order);  // (kept so far)
         c->guess_order = order;
+        return MP_OK;
+    });
+}
+
+int mp_guess_window(mp_ctx* c, int k) {
+    return guarded(c, "mp_guess_window", [&]() -> int {
+        if (k < 1 || k > PROJ_MAX) return fail(c, MP_ERR_ARG, "mp_guess_window: 1 to 8 pressures");
+        c->proj_window = k;
         return MP_OK;
     });
 }

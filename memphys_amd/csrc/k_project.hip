@@ -1,0 +1,251 @@
+// Initial guess of the pressure solve by projection onto the last K solutions:
+// the combination of the stored pressures whose residual against the new
+// right-hand side is smallest (P. F. Fischer, "Projection techniques for
+// iterative solution of Ax = b with successive right-hand sides", Comput.
+// Methods Appl. Mech. Engrg. 163 (1998) 193-204).  The stored pressures p_j
+// (newest first) and their products A p_j live in rings of PROJ_MAX slots.
+// Successive pressures are nearly parallel, so the least-squares problem is
+// posed in the basis of backward differences in time, d_k = (Delta^k p) at the
+// newest step, each column scaled to unit norm; its normal equations are then
+// well conditioned enough for FP64 Cholesky:
+//   G_kl = (A d_k).(A d_l),  c_k = (A d_k).b,  (S G S + eps I) y = S c,
+//   beta = S y,  x0 = sum_k beta_k d_k   (S = diag(1/|A d_k|)).
+// The differences of A p are formed per row from the stored products (no
+// product with A at guess time); the one product per step, A p for the new
+// solution, is taken from the FP64 residual b - A p after the solve.
+#include "mp_internal.hpp"
+
+namespace mp {
+
+namespace {
+
+constexpr int PROJ_THREADS = 256;
+constexpr int PROJ_BLOCKS = 512;
+
+__device__ __forceinline__ int ring_slot(int head, int j) {
+    return (head - j + PROJ_MAX) % PROJ_MAX;
+}
+
+// d[k] = Delta^k v at the newest entry (Newton's backward differences of
+// v[0] (newest) .. v[K-1])
+template <int K>
+__device__ __forceinline__ void backward_differences(double (&v)[K], double (&d)[K]) {
+    d[0] = v[0];
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+#pragma unroll
+        for (int j = 0; j < K - k; ++j) v[j] = v[j] - v[j + 1];
+        d[k] = v[0];
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_gram(int n, const double* __restrict__ hw,
+                                                            size_t ld, const ProjState* ps,
+                                                            const double* __restrict__ b,
+                                                            double* __restrict__ partial) {
+    constexpr int NG = K * (K + 1) / 2 + K;
+    double acc[NG];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) acc[q] = 0.0;
+    const int head = ps->head;
+    for (int i = blockIdx.x * PROJ_THREADS + threadIdx.x; i < n; i += gridDim.x * PROJ_THREADS) {
+        double w[K], d[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) w[j] = hw[(size_t)ring_slot(head, j) * ld + i];
+        backward_differences<K>(w, d);
+        const double bi = b[i];
+        int q = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int l = k; l < K; ++l) {
+                acc[q] = fma(d[k], d[l], acc[q]);
+                ++q;
+            }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            acc[q] = fma(d[k], bi, acc[q]);
+            ++q;
+        }
+    }
+    __shared__ double sh[PROJ_THREADS / 64][NG];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+        double v = acc[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) sh[wid][q] = v;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < NG; q += PROJ_THREADS) {
+        double v = sh[0][q];
+        for (int u = 1; u < PROJ_THREADS / 64; ++u) v += sh[u][q];
+        partial[(size_t)blockIdx.x * NG + q] = v;
+    }
+}
+
+// Sums the block partials (each thread a fixed strided subset, then a fixed
+// reduction tree) and solves for beta on one thread.
+template <int K>
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(int nblocks,
+                                                             const double* __restrict__ partial,
+                                                             ProjState* ps) {
+    constexpr int NG = K * (K + 1) / 2 + K;
+    __shared__ double tot[NG];
+    __shared__ double sh[PROJ_THREADS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int q = 0; q < NG; ++q) {
+        double v = 0.0;
+        for (int blk = threadIdx.x; blk < nblocks; blk += PROJ_THREADS)
+            v += partial[(size_t)blk * NG + q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) sh[wid] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = sh[0];
+            for (int u = 1; u < PROJ_THREADS / 64; ++u) t += sh[u];
+            tot[q] = t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double G[K][K], c[K], s[K];
+    int q = 0;
+    for (int k = 0; k < K; ++k)
+        for (int l = k; l < K; ++l) {
+            G[k][l] = tot[q];
+            G[l][k] = tot[q];
+            ++q;
+        }
+    for (int k = 0; k < K; ++k) c[k] = tot[q++];
+    // unit columns; a column with no length (no change in time) drops out
+    for (int k = 0; k < K; ++k) s[k] = G[k][k] > 0.0 ? 1.0 / sqrt(G[k][k]) : 0.0;
+    for (int k = 0; k < K; ++k) {
+        for (int l = 0; l < K; ++l) G[k][l] *= s[k] * s[l];
+        G[k][k] += 1e-13;  // (keeps the factorisation defined for dependent columns)
+        c[k] *= s[k];
+    }
+    // Cholesky G = L L^T in place (lower triangle), then L L^T y = c
+    bool ok = true;
+    for (int k = 0; k < K && ok; ++k) {
+        double dkk = G[k][k];
+        for (int m = 0; m < k; ++m) dkk -= G[k][m] * G[k][m];
+        if (!(dkk > 0.0)) {
+            ok = false;
+            break;
+        }
+        const double lkk = sqrt(dkk);
+        G[k][k] = lkk;
+        for (int r = k + 1; r < K; ++r) {
+            double v = G[r][k];
+            for (int m = 0; m < k; ++m) v -= G[r][m] * G[k][m];
+            G[r][k] = v / lkk;
+        }
+    }
+    double y[K];
+    if (ok) {
+        for (int k = 0; k < K; ++k) {
+            double v = c[k];
+            for (int m = 0; m < k; ++m) v -= G[k][m] * y[m];
+            y[k] = v / G[k][k];
+        }
+        for (int k = K - 1; k >= 0; --k) {
+            double v = y[k];
+            for (int m = k + 1; m < K; ++m) v -= G[m][k] * y[m];
+            y[k] = v / G[k][k];
+        }
+    }
+    for (int k = 0; k < PROJ_MAX; ++k) ps->beta[k] = 0.0;
+    if (ok) {
+        for (int k = 0; k < K; ++k) ps->beta[k] = (isfinite(y[k]) ? y[k] : 0.0) * s[k];
+    } else {
+        ps->beta[0] = 1.0;  // the newest pressure alone
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_combine(int n, const double* __restrict__ hp,
+                                                               size_t ld, const ProjState* ps,
+                                                               double* __restrict__ out) {
+    const int i = blockIdx.x * PROJ_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const int head = ps->head;
+    double v[K], d[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j] = hp[(size_t)ring_slot(head, j) * ld + i];
+    backward_differences<K>(v, d);
+    double x = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) x = fma(ps->beta[k], d[k], x);
+    out[i] = x;
+}
+
+// The new solution and its product, A p = b - r, into the slot after head.
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_store(int n, const double* __restrict__ p,
+                                                             const double* __restrict__ b,
+                                                             const double* __restrict__ r,
+                                                             double* __restrict__ hp,
+                                                             double* __restrict__ hw, size_t ld,
+                                                             const ProjState* ps) {
+    const int i = blockIdx.x * PROJ_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const size_t o = (size_t)((ps->head + 1) % PROJ_MAX) * ld + i;
+    hp[o] = p[i];
+    hw[o] = b[i] - r[i];
+}
+
+__global__ void k_proj_advance(ProjState* ps, int reset) {
+    if (reset) {
+        ps->head = 0;
+        ps->count = 0;
+        return;
+    }
+    ps->head = (ps->head + 1) % PROJ_MAX;
+    ps->count = ps->count < PROJ_MAX ? ps->count + 1 : PROJ_MAX;
+}
+
+template <int K>
+void launch_guess_k(hipStream_t s, int n, const double* hp, const double* hw, size_t ld,
+                    ProjState* ps, const double* b, double* partial, double* out) {
+    hipLaunchKernelGGL(k_proj_gram<K>, dim3(PROJ_BLOCKS), dim3(PROJ_THREADS), 0, s, n, hw, ld, ps,
+                       b, partial);
+    hipLaunchKernelGGL(k_proj_solve<K>, dim3(1), dim3(PROJ_THREADS), 0, s, PROJ_BLOCKS, partial,
+                       ps);
+    hipLaunchKernelGGL(k_proj_combine<K>, dim3((n + PROJ_THREADS - 1) / PROJ_THREADS),
+                       dim3(PROJ_THREADS), 0, s, n, hp, ld, ps, out);
+}
+
+}  // namespace
+
+size_t proj_partial_size() { return (size_t)PROJ_BLOCKS * (PROJ_MAX * (PROJ_MAX + 1) / 2 + PROJ_MAX); }
+
+void launch_proj_guess(hipStream_t s, int k, int n, const double* hp, const double* hw, size_t ld,
+                       ProjState* ps, const double* b, double* partial, double* out) {
+    switch (k) {
+        case 1: launch_guess_k<1>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 2: launch_guess_k<2>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 3: launch_guess_k<3>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 4: launch_guess_k<4>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 5: launch_guess_k<5>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 6: launch_guess_k<6>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 7: launch_guess_k<7>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        default: launch_guess_k<8>(s, n, hp, hw, ld, ps, b, partial, out); break;
+    }
+    MP_HIP(hipGetLastError());
+}
+
+void launch_proj_store(hipStream_t s, int n, const double* p, const double* b, const double* r,
+                       double* hp, double* hw, size_t ld, ProjState* ps) {
+    hipLaunchKernelGGL(k_proj_store, dim3((n + PROJ_THREADS - 1) / PROJ_THREADS),
+                       dim3(PROJ_THREADS), 0, s, n, p, b, r, hp, hw, ld, ps);
+    hipLaunchKernelGGL(k_proj_advance, dim3(1), dim3(1), 0, s, ps, 0);
+    MP_HIP(hipGetLastError());
+}
+
+void launch_proj_reset(hipStream_t s, ProjState* ps) {
+    hipLaunchKernelGGL(k_proj_advance, dim3(1), dim3(1), 0, s, ps, 1);
+    MP_HIP(hipGetLastError());
+}
+
+}  // namespace mp

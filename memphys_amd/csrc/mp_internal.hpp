@@ -350,4 +350,24 @@ int largest_eigenvalue(hipStream_t s, const SellView& sv, const double* val, int
 void launch_fill(hipStream_t s, double* x, double v, int n);
 void launch_copy(hipStream_t s, double* dst, const double* src, int n);
 
+// Projection guess (k_project.hip): rings of PROJ_MAX stored pressures and
+// their products with the pressure operator (vectors of ld doubles each); the
+// device record holds the newest slot, the entries held and the last
+// coefficients.
+constexpr int PROJ_MAX = 8;
+struct ProjState {
+    int head;
+    int count;
+    double beta[PROJ_MAX];
+};
+size_t proj_partial_size();
+// out = the combination of the newest k stored pressures (k <= PROJ_MAX, at
+// most the entries held) with the smallest residual against b, over n rows
+void launch_proj_guess(hipStream_t s, int k, int n, const double* hp, const double* hw, size_t ld,
+                       ProjState* ps, const double* b, double* partial, double* out);
+// stores p and A p = b - r as the newest entry
+void launch_proj_store(hipStream_t s, int n, const double* p, const double* b, const double* r,
+                       double* hp, double* hw, size_t ld, ProjState* ps);
+void launch_proj_reset(hipStream_t s, ProjState* ps);
+
 }  // namespace mp

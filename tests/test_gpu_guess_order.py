@@ -73,3 +73,48 @@ def test_raising_the_order_waits_for_the_history(cube3d_small):
     _print("raised at step 6 vs order 2 after 8 steps:", errs, "iterations", it_r)
     assert max(errs) <= 1e-8
     assert not all(np.array_equal(a, b) for a, b in zip(raised[-1], base[-1]))
+
+
+def _run_project(cloud, steps, window=8):
+    import memphys_amd as M
+    u_dir, p_dir, u0 = lid_bc(cloud, cloud.dim - 1)
+    out, iters, gres = [], [], []
+    with M.Context(0) as ctx:
+        ctx.upload(cloud)
+        ctx.weights(3, 3, download=False)
+        ctx.fs_create(1.0, 0.0025, 5e-4, 2, 1e-10, 20000, u_dir, u_dir,
+                      u_dir if cloud.dim == 3 else None, p_dir, precision=M.MP_PREC_MIXED,
+                      inner_tol=1e-5, precond=M.MP_PRECOND_AMG, krylov=1)
+        ctx.set_field(M.MP_U, u0)
+        for f in (M.MP_V, M.MP_W):
+            ctx.set_field(f, np.zeros(cloud.nv))
+        ctx.set_field(M.MP_P, np.zeros(ctx.n_sys))
+        ctx.guess_mode(M.MP_GUESS_PROJECT)
+        ctx.guess_window(window)
+        with pytest.raises(M.MemphysError):
+            ctx.guess_window(9)
+        for it in range(steps):
+            ctx.step(1, it)
+            st = ctx.stats()
+            assert st.converged == 1 and st.rel_res <= 1e-10
+            iters.append(st.iterations)
+            gres.append(st.guess_rel_res)
+            out.append([ctx.get_field(f) for f in (M.MP_U, M.MP_V, M.MP_W, M.MP_P)])
+    return out, iters, gres
+
+
+def test_projected_guess_agrees_and_starts_closest(cube3d_small):
+    """The projection onto the last 8 solutions reaches the same fields as the
+    order-2 extrapolation, and from step 10 on starts from a smaller residual
+    and takes fewer iterations than the order-4 extrapolation."""
+    steps = 40
+    proj, it_p, g_p = _run_project(cube3d_small, steps)
+    ext2, it_2 = _run(cube3d_small, steps, 2)
+    ext4, it_4 = _run(cube3d_small, steps, 4)
+    errs = [rel_err(a, b) for a, b in zip(proj[-1], ext2[-1])]
+    _print("projection vs order 2 after", steps, "steps:", errs)
+    _print("mean iterations (steps 10-39): projection", np.mean(it_p[10:]), "order 4",
+           np.mean(it_4[10:]), "order 2", np.mean(it_2[10:]))
+    _print("projection: guess residual, median of steps 10-39", np.median(g_p[10:]))
+    assert max(errs) <= 1e-8
+    assert np.mean(it_p[10:]) < np.mean(it_4[10:])
