@@ -498,7 +498,7 @@ class Context:
         self._check(self.lib.mp_guess_order(self.h, order), "mp_guess_order")
 
     def guess_window(self, k: int):
-        """Solutions (1..8) the projected guess combines; default 8."""
+        """Solutions (1..12) the projected guess combines; default 12."""
         self._check(self.lib.mp_guess_window(self.h, k), "mp_guess_window")
 
     def guess_train(self, enable: bool, lr: float = 1e-3, seed: int = 0):

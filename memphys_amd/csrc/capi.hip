@@ -1226,11 +1226,9 @@ void fs_step_one(mp_ctx* c, int it) {
         hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad, c->P.p,
                            c->P1.p);
         MP_HIP(hipGetLastError());
-        if (keep_proj) {  // the solution and A p = b - (b - A p) as the newest entry
-            const double* r = c->ppe->residual_of(c->stream, c->b.p, c->P.p);
-            launch_proj_store(c->stream, c->npad, c->P.p, c->b.p, r, c->Hp.p, c->Hw.p,
-                              (size_t)c->npad, c->pstate.p);
-        }
+        if (keep_proj)  // the solution and A p = b - (b - A p) as the newest entry
+            launch_proj_store(c->stream, c->npad, c->P.p, c->b.p, c->ppe->final_residual(),
+                              c->Hp.p, c->Hw.p, (size_t)c->npad, c->pstate.p);
         // calc_vel_corr
         c->timers[3].begin(c->stream);
         launch_corrector<DIM>(c->stream, c->sv, c->w[0].p, c->w[1].p, c->w[2].p, c->P.p,
@@ -1316,7 +1314,8 @@ int mp_guess_order(mp_ctx* c, int order) {
 
 int mp_guess_window(mp_ctx* c, int k) {
     return guarded(c, "mp_guess_window", [&]() -> int {
-        if (k < 1 || k > PROJ_MAX) return fail(c, MP_ERR_ARG, "mp_guess_window: 1 to 8 pressures");
+        if (k < 1 || k > PROJ_MAX)
+            return fail(c, MP_ERR_ARG, "mp_guess_window: 1 to " + std::to_string(PROJ_MAX) + " pressures");
         c->proj_window = k;
         return MP_OK;
     });

@@ -1690,6 +1690,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
         // breaks down or stalls hands the rest of its solve to FP64 BiCGSTAB.
         std::vector<double> rr = rr0, rr_prev = rr0, tol2(nseg);
         std::vector<char> act(nseg, 0), fallback(nseg, 0);
+        bool undone = false;  // (an undone cycle leaves the residual vector of the undone x~)
         for (int q = 0; q < nseg; ++q) {
             tol2[q] = tol * tol * bb[q];
             act[q] = (rr[q] > tol2[q] && bb[q] > 0) ? 1 : 0;
@@ -1755,6 +1756,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                 act[q] = rr[q] > tol2[q] && !fallback[q];
                 if (!act[q]) fallback[q] = fallback[q] && rr[q] > tol2[q];
             }
+            undone |= any_undo;
             if (any_undo)
                 for (int q = 0; q < nseg; ++q)
                     if (undo[q]) {
@@ -1784,7 +1786,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
             }
         // rr holds the residual of x~ as it stands (an undone cycle restored
         // both), unless the FP64 fallback or a zero right-hand side moved x~
-        if (!anyfb && !zeroed) rr_last = rr;
+        if (!anyfb && !zeroed && !undone) rr_last = rr;
     }
     // x = D^-1 x~ and the logged residual |A x - b| (solver.cpp:342-345)
     std::vector<double> rr;

@@ -11,8 +11,8 @@
 //   G_kl = (A d_k).(A d_l),  c_k = (A d_k).b,  (S G S + eps I) y = S c,
 //   beta = S y,  x0 = sum_k beta_k d_k   (S = diag(1/|A d_k|)).
 // The differences of A p are formed per row from the stored products (no
-// product with A at guess time); the one product per step, A p for the new
-// solution, is taken from the FP64 residual b - A p after the solve.
+// product with A at guess time); A p of each new solution comes from the
+// solve's own final FP64 residual, A p = b - (b - A p).
 #include "mp_internal.hpp"
 
 namespace mp {
@@ -230,7 +230,11 @@ void launch_proj_guess(hipStream_t s, int k, int n, const double* hp, const doub
         case 5: launch_guess_k<5>(s, n, hp, hw, ld, ps, b, partial, out); break;
         case 6: launch_guess_k<6>(s, n, hp, hw, ld, ps, b, partial, out); break;
         case 7: launch_guess_k<7>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        default: launch_guess_k<8>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 8: launch_guess_k<8>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 9: launch_guess_k<9>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 10: launch_guess_k<10>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 11: launch_guess_k<11>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        default: launch_guess_k<12>(s, n, hp, hw, ld, ps, b, partial, out); break;
     }
     MP_HIP(hipGetLastError());
 }

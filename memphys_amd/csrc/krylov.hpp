@@ -144,6 +144,9 @@ struct PpeSolver {
     double smooth_bytes() const;
     // r = b - A x (FP64, unscaled operator incl. border) into `tmp`; returns tmp.p
     double* residual_of(hipStream_t s, const double* b_dev, const double* x_dev);
+    // b - A x for the x the last solve() returned (FP64, in `tmp`, valid until
+    // the next solve or residual_of)
+    const double* final_residual() const { return tmp.p; }
     // |b - A x|^2 of system 0 from the last residual_of (waits for the stream)
     double last_residual_norm2(hipStream_t s);
     // Second candidate for the next solve's guess (single system): solve()
