@@ -11,12 +11,15 @@ dt = 5e-4, AB2, 500 steps from rest; pressure-Poisson relative residual 1e-9
 (defaults 5 and 495: the whole 500-step run after 5 warm-up steps): FP32-
 operator inner GMRES(30) preconditioned by an FP32 smoothed-aggregation
 multigrid V-cycle, in FP64 defect correction, with the learned initial guess
-(trained first, untimed, on a 63,536-node cube with the same physics: 2000
-solver steps in 10 episodes from rest with seeded lid speeds in [0.5, 1.5]).
+correcting the projection onto the last 12 solutions (the network trained
+first, untimed, on a 63,536-node cube with the same physics: 2000 solver steps
+in 10 episodes from rest with seeded lid speeds in [0.5, 1.5]).
 
-Beside it, over the same steps of the same run: the multigrid form without the
-guess (previous pressure), with linear extrapolation, and the same solver in
-FP64 (FP64 GMRES(30) + the multigrid V-cycle): `speedup_vs_fp64_same_solver`.
+Beside it, over the same steps of the same run: the multigrid form without a
+guess (previous pressure), with the order-4 extrapolation in time, with the
+projection alone, and the same solver in FP64 started from the previous
+pressure (FP64 GMRES(30) + the multigrid V-cycle):
+`speedup_vs_fp64_same_solver`.
 The plain FP64 path (FP64 Jacobi-scaled BiCGSTAB, the reference's
 solveWithGuess(rhs, p_old)) and its FP32-operator variant are too slow for 500
 steps: they run an early window (steps 5-24) and a late window (steps 482-499,
@@ -97,7 +100,7 @@ def run_form(M, ctx, cloud, prm, precision, steps, warmup, label, guess=None, pr
     log(f"[{label}] fs_create {t_create:.2f}s, hierarchy {hier}")
     if guess is None:
         ctx.guess_mode(M.MP_GUESS_PREVIOUS)
-    elif guess == "project":
+    elif isinstance(guess, str) and guess == "project":
         ctx.guess_mode(M.MP_GUESS_PROJECT)
     elif isinstance(guess, str):
         ctx.guess_mode(M.MP_GUESS_EXTRAPOLATE)
@@ -533,10 +536,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default poll interval and multigrid layouts (for counter "
                          "runs, under which the tuning's timed solves take minutes)")
-    ap.add_argument("--forms", default="fp64,mixed,multigrid,extrapolate,fp64_multigrid,headline",
+    ap.add_argument("--forms", default="fp64,mixed,multigrid,extrapolate,project,fp64_multigrid,headline",
                     help="comma list of forms: fp64 (plain FP64 Jacobi BiCGSTAB, two windows), "
                          "mixed (FP32-inner Jacobi BiCGSTAB, two windows), multigrid, extrapolate, "
-                         "fp64_multigrid, headline (full runs)")
+                         "project, fp64_multigrid, headline (full runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
@@ -743,7 +746,8 @@ def main():
                          f"residual {args.tol:g}"),
             "nv": cloud.nv, "nnz": cloud.nnz,
             "form": (f"FP32-operator inner {kname} with FP32 smoothed-aggregation multigrid "
-                     "V-cycle, in FP64 defect correction, + learned guess"),
+                     "V-cycle, in FP64 defect correction, + learned guess correcting the "
+                     "projection onto the last 12 solutions"),
             "guess_training": f"{args.train_steps} solver steps on {small.nv} nodes, "
                               f"lid speeds U[0.5,1.5], {t_train:.1f}s",
         },
@@ -763,12 +767,13 @@ def main():
         "headline": form_summary(head) | {
             "guess": {"steps": len(head["step_s"]),
                       "network_evaluated": len(head["step_s"]) - head["guess_skipped"],
-                      "rejected_for_extrapolation": head["guess_rejected"],
+                      "rejected_for_projection": head["guess_rejected"],
                       "accepted": len(head["step_s"]) - head["guess_skipped"] -
                                   head["guess_rejected"],
-                      "rule": "a guess whose FP64 residual is non-finite or above 10x its "
-                              "extrapolation's is replaced by the extrapolation; after each "
-                              "rejection the network is skipped for 1, 2, 4 ... 64 steps"}},
+                      "rule": "a guess whose FP64 residual is non-finite or above 10x that "
+                              "of the projection it corrects is replaced by the projection; "
+                              "after each rejection the network is skipped for 1, 2, 4 ... 64 "
+                              "steps"}},
         "full_run_forms": {k: form_summary(f) for k, f in full.items()},
         "jacobi_forms_windows": win,
         "fields_headline_vs_fp64_multigrid_rel_diff": fields_diff,

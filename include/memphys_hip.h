@@ -43,7 +43,7 @@ enum mp_precision { MP_PREC_FP64 = 0, MP_PREC_MIXED = 1 };
 
 /* Initial guess for each pressure solve: the previous pressure (reference
  * behaviour, solveWithGuess(rhs, p_old)), the learned message-passing guess
- * (the extrapolation in time plus the network's correction), the
+ * (a classical guess, mp_guess_base, plus the network's correction), the
  * extrapolation alone: the polynomial through the pressures of the last 2, 3
  * or 4 steps (mp_guess_order; equal time steps), evaluated at the next step,
  * or the projection: the combination of the last K solutions (mp_guess_window,
@@ -230,6 +230,10 @@ int mp_guess_mode(mp_ctx* ctx, int mode);
 int mp_guess_order(mp_ctx* ctx, int order);
 /* Solutions the projected guess combines (1..12, default 12). */
 int mp_guess_window(mp_ctx* ctx, int k);
+/* The classical guess the learned network corrects: MP_GUESS_PROJECT (the
+ * default) or MP_GUESS_EXTRAPOLATE.  It is also the second candidate the
+ * solve falls back to. */
+int mp_guess_base(mp_ctx* ctx, int base);
 
 /* Learned guess: a 3-layer message-passing network over the stencil graph
  * (width 64, BF16 weights, FP32 accumulation).  The parameters are a packed
@@ -315,7 +319,7 @@ int mp_graph_mode(mp_ctx* ctx, int enable);
  *        product (with MP_PRECOND_AMG).  Reports the launches that did the work and
  * their mean milliseconds: Krylov launches of a system that converged inside
  * a chunk of iterations return at once and are left out (a launch counts
- * when it takes at least half the median launch's time).  all_launches (may
+ * when it takes at least a quarter of the longest launch's time).  all_launches (may
  * be NULL) receives every timed launch.  bytes_per_launch: the kernel's
  * algorithmic HBM bytes per working launch. */
 int mp_kernel_timing(mp_ctx* ctx, int enable);

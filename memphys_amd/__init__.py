@@ -39,6 +39,7 @@ HIP_SYMBOLS = (
     "mp_graph_mode", "mp_batch_upload", "mp_batch_params", "mp_batch_stats", "mp_autotune",
     "mp_operator_eigenvalue", "mp_knn_stencils", "mp_rcm_order", "mp_fs_body_force",
     "mp_divergence_noalloc", "mp_precond_apply", "mp_guess_order", "mp_guess_window",
+    "mp_guess_base",
 )
 
 
@@ -128,6 +129,7 @@ def hip_lib() -> C.CDLL:
         lib.mp_precond_apply.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
         lib.mp_guess_order.argtypes = [vp, C.c_int]
         lib.mp_guess_window.argtypes = [vp, C.c_int]
+        lib.mp_guess_base.argtypes = [vp, C.c_int]
         lib.mp_knn_stencils.argtypes = [vp, C.c_int, C.c_int, _dp, _u8, _ip, _ip, _ip]
         lib.mp_rcm_order.argtypes = [vp, C.c_int, _ip, _ip, _ip]
         for name in HIP_SYMBOLS:
@@ -500,6 +502,11 @@ class Context:
     def guess_window(self, k: int):
         """Solutions (1..12) the projected guess combines; default 12."""
         self._check(self.lib.mp_guess_window(self.h, k), "mp_guess_window")
+
+    def guess_base(self, base: int):
+        """The classical guess the learned network corrects (and falls back to):
+        MP_GUESS_PROJECT (default) or MP_GUESS_EXTRAPOLATE."""
+        self._check(self.lib.mp_guess_base(self.h, base), "mp_guess_base")
 
     def guess_train(self, enable: bool, lr: float = 1e-3, seed: int = 0):
         self._check(self.lib.mp_guess_train(self.h, 1 if enable else 0, lr, seed), "mp_guess_train")

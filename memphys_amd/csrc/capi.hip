@@ -81,6 +81,7 @@ struct mp_ctx {
     mp_step_stats stats{};
     int guess_mode = MP_GUESS_PREVIOUS;
     int guess_order = 4;  // pressures the extrapolated guess goes through (2..4)
+    int guess_base = MP_GUESS_PROJECT;  // the guess the learned network corrects
     // back-off of the learned guess: after a rejected guess the network is
     // skipped for guess_backoff steps (1, 2, 4, ... 64), an accepted one resets it
     int guess_backoff = 0, guess_skip = 0;
@@ -245,6 +246,13 @@ void upload_flags(const mp_ctx* c, const uint8_t* src, DevBuf<uint8_t>& dev) {
         for (int o = 0; o < c->nv; ++o) h[c->iperm[o]] = src[o] ? 1 : 0;
     dev.alloc(h.size());
     MP_HIP(hipMemcpy(dev.p, h.data(), h.size(), hipMemcpyHostToDevice));
+}
+
+// The step keeps the projection's rings (projected guesses, or learned ones
+// correcting a projection).
+bool uses_projection(const mp_ctx* c) {
+    return c->guess_mode == MP_GUESS_PROJECT ||
+           (c->guess_mode == MP_GUESS_LEARNED && c->guess_base == MP_GUESS_PROJECT);
 }
 
 // Rings of the projection guess for the current rows, emptied.
@@ -996,7 +1004,7 @@ int mp_fs_create(mp_ctx* c, const mp_fs_params* prm, const uint8_t* u_dir, const
         alloc_vec(c, c->P4);
         alloc_vec(c, c->Ptmp);
         c->hist = 0;
-        if (c->guess_mode == MP_GUESS_PROJECT) ensure_projection(c);  // (rows may have changed)
+        if (uses_projection(c)) ensure_projection(c);  // (rows may have changed)
         if (c->net && c->nprob == 1) ensure_net(c);
         for (int q = 0; q < 3; ++q) alloc_vec(c, c->uh[q]);
         c->upk.alloc((size_t)4 * c->npad);
@@ -1139,9 +1147,11 @@ void fs_step_one(mp_ctx* c, int it) {
     const int* bp = c->per_problem_prm ? c->blk_prob.p : nullptr;
     const double* pp = c->per_problem_prm ? c->prob_prm.p : nullptr;
     const double* force = c->has_force ? c->force.p : nullptr;
-    const bool extrapolate =
-        (c->guess_mode == MP_GUESS_EXTRAPOLATE || c->guess_mode == MP_GUESS_LEARNED) && c->hist >= 2;
-    const bool project = c->guess_mode == MP_GUESS_PROJECT && c->pcount >= 1;
+    // the classical guess: the mode's own, or the one the learned network corrects
+    const bool learned_mode = c->guess_mode == MP_GUESS_LEARNED;
+    const int base = learned_mode ? c->guess_base : c->guess_mode;
+    const bool extrapolate = base == MP_GUESS_EXTRAPOLATE && c->hist >= 2;
+    const bool project = base == MP_GUESS_PROJECT && c->pcount >= 1;
     const int kp = std::min(c->pcount, c->proj_window);
     const int order = std::min(c->guess_order, c->hist);  // (the history held so far)
     const uintptr_t key_pre[8] = {1, (uintptr_t)o, (uintptr_t)c->s_cur, (uintptr_t)euler,
@@ -1173,17 +1183,18 @@ void fs_step_one(mp_ctx* c, int it) {
         }
     });
     bool learned = false, skipped = false;
-    if (extrapolate) {
-        if (c->guess_mode == MP_GUESS_LEARNED && !c->train && c->guess_skip > 0) {
-            --c->guess_skip;  // backing off: the extrapolation alone
+    if (learned_mode && c->hist >= 2 && (extrapolate || project)) {
+        if (!c->train && c->guess_skip > 0) {
+            --c->guess_skip;  // backing off: the classical guess alone
             skipped = true;
-        } else if (c->guess_mode == MP_GUESS_LEARNED) {
+        } else {
             const double* r = c->ppe->residual_of(c->stream, c->b.p, c->P.p);
             c->net->features(c->stream, c->P1.p, c->P2.p, r, c->ppe->diag_inv, c->xyz.p, c->npad,
                              c->dim);
             c->net->forward(c->stream, c->train != 0);
-            // the extrapolation stays the second candidate: the solve starts from
-            // whichever of the two leaves the smaller FP64 residual
+            // the classical guess stays the second candidate: the solve starts
+            // from it when the network's leaves a non-finite or far larger
+            // FP64 residual
             hipLaunchKernelGGL(k_copy, dim3(c->npad / 256), dim3(256), 0, c->stream, c->npad,
                                c->P.p, c->Ptmp.p);
             MP_HIP(hipGetLastError());
@@ -1210,7 +1221,7 @@ void fs_step_one(mp_ctx* c, int it) {
         c->last_loss = c->net->loss_host(c->stream);
     }
     const int depth = c->guess_order;  // (history kept: as many pressures as the order uses)
-    const bool keep_proj = c->guess_mode == MP_GUESS_PROJECT;
+    const bool keep_proj = base == MP_GUESS_PROJECT;
     const uintptr_t key_post[8] = {2, (uintptr_t)o, (uintptr_t)bp, (uintptr_t)pp,
                                    (uintptr_t)depth, (uintptr_t)keep_proj, 0, 0};
     run_step_part(c, key_post, [&]() {
@@ -1296,6 +1307,7 @@ int mp_guess_mode(mp_ctx* c, int mode) {
         if (mode == MP_GUESS_LEARNED) {
             if (!c->has_fs) return fail(c, MP_ERR_STATE, "mp_guess_mode: mp_fs_create not called");
             if (!c->net) ensure_net(c);
+            if (c->guess_base == MP_GUESS_PROJECT) ensure_projection(c);
         }
         c->guess_mode = mode;
         c->guess_backoff = c->guess_skip = 0;
@@ -1308,6 +1320,17 @@ int mp_guess_order(mp_ctx* c, int order) {
         if (order < 2 || order > 4) return fail(c, MP_ERR_ARG, "mp_guess_order: order 2, 3 or 4");
         if (order > c->guess_order) c->hist = std::min(c->hist, c->guess_order);  // (kept so far)
         c->guess_order = order;
+        return MP_OK;
+    });
+}
+
+int mp_guess_base(mp_ctx* c, int base) {
+    return guarded(c, "mp_guess_base", [&]() -> int {
+        if (base != MP_GUESS_EXTRAPOLATE && base != MP_GUESS_PROJECT)
+            return fail(c, MP_ERR_ARG, "mp_guess_base: MP_GUESS_EXTRAPOLATE or MP_GUESS_PROJECT");
+        c->guess_base = base;
+        c->guess_backoff = c->guess_skip = 0;
+        if (c->guess_mode == MP_GUESS_LEARNED && base == MP_GUESS_PROJECT) ensure_projection(c);
         return MP_OK;
     });
 }

@@ -1010,9 +1010,10 @@ void KernelTimer::summary(long* working, double* mean_ms, long* all) const {
     *mean_ms = 0.0;
     *all = (long)samples.size();
     if (samples.empty()) return;
-    std::vector<float> v(samples);
-    std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
-    const float cut = 0.5f * v[v.size() / 2];
+    // (a quarter of the longest launch: launches that returned at once are
+    // a few microseconds, working ones tens; the median fails to separate
+    // them when most launches of a short solve return at once)
+    const float cut = 0.25f * *std::max_element(samples.begin(), samples.end());
     double tot = 0.0;
     for (float ms : samples)
         if (ms >= cut) {
