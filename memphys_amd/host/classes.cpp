@@ -91,11 +91,19 @@ void PARAMETERS::read_gpu_settings(std::string settings_file) {
                 if (val == "previous") guess = 0;
                 else if (val == "learned") guess = 1;
                 else if (val == "extrapolate") guess = 2;
+                else if (val == "project") guess = 3;
                 else bad();
             } else if (key == "guess_params") {
                 guess_params_file = val;
             } else if (key == "extrapolation_order") {
                 if (val == "2" || val == "3" || val == "4") guess_order = std::stoi(val);
+                else bad();
+            } else if (key == "projection_window") {
+                guess_window = std::stoi(val);
+                if (guess_window < 1 || guess_window > 12) bad();
+            } else if (key == "learned_base") {
+                if (val == "extrapolate") guess_base = 2;
+                else if (val == "project") guess_base = 3;
                 else bad();
             } else if (key == "graphs") {
                 if (val == "0" || val == "1") graphs = std::stoi(val);
@@ -672,6 +680,8 @@ void FRACTIONAL_STEP_1::create(POINTS& points, CLOUD& cloud, PARAMETERS& paramet
     check(ctx, mp_solver_system_size(ctx, &system_size), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
     check(ctx, mp_graph_mode(ctx, parameters.graphs), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
     check(ctx, mp_guess_order(ctx, parameters.guess_order), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
+    check(ctx, mp_guess_window(ctx, parameters.guess_window), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
+    check(ctx, mp_guess_base(ctx, parameters.guess_base), "FRACTIONAL_STEP_1::FRACTIONAL_STEP_1");
     if (parameters.guess == MP_GUESS_LEARNED) {
         std::vector<float> w(mp_guess_param_count());
         std::ifstream f(parameters.guess_params_file, std::ios::binary);

@@ -120,8 +120,10 @@ class PARAMETERS {
     int precision = 0;       // 0 FP64, 1 FP32-operator inner solve with FP64 defect correction
     int device = 0;
     int preconditioner = 0;  // 0 Jacobi, 1 multigrid V-cycle (enum mp_precond)
-    int guess = 0;           // 0 previous pressure, 1 learned, 2 extrapolation in time
+    int guess = 0;           // 0 previous pressure, 1 learned, 2 extrapolation, 3 projection
     int guess_order = 4;     // pressures the extrapolation goes through (2..4)
+    int guess_window = 12;   // solutions the projection combines (1..12)
+    int guess_base = 3;      // the classical guess the learned one corrects (2 or 3)
     std::string guess_params_file;  // packed float32 network parameters (guess = 1)
     int graphs = 1;          // replay the Krylov chunks from hipGraphs
     double inner_tolerance = 1e-3;  // precision 1: inner relative reduction per cycle
@@ -134,8 +136,9 @@ class PARAMETERS {
     void read_calc_parameters(std::string parameter_file);
     void verify_parameters();
     // Keys: precision (fp64 | mixed), preconditioner (jacobi | multigrid),
-    // guess (previous | extrapolate | learned), guess_params (file),
-    // extrapolation_order (2 | 3 | 4), graphs (0 | 1), device (int),
+    // guess (previous | extrapolate | project | learned), guess_params (file),
+    // extrapolation_order (2 | 3 | 4), projection_window (1..12),
+    // learned_base (extrapolate | project), graphs (0 | 1), device (int),
     // inner_tolerance (double).  Unknown keys and values are errors.
     void read_gpu_settings(std::string settings_file);
     void calc_cloud_num_points();

@@ -345,6 +345,8 @@ CLASS_API_CASES = [
                         "inner_tolerance,1e-5\n", None),
     ("hypre_ilu_gmres", "precision,mixed\npreconditioner,multigrid\nguess,extrapolate\n"
                         "extrapolation_order,3\n", None),
+    ("hypre_ilu_gmres", "precision,mixed\npreconditioner,multigrid\nguess,project\n"
+                        "projection_window,6\n", None),
     # the body-force overload of single_timestep_2d
     ("eigen_ilu_bicgstab", None, 2.0),
     ("hypre_ilu_gmres", "precision,mixed\npreconditioner,multigrid\n", 2.0),
