@@ -20,7 +20,7 @@ namespace mp {
 namespace {
 
 constexpr int PROJ_THREADS = 256;
-constexpr int PROJ_BLOCKS = 512;
+constexpr int PROJ_BLOCKS = 256;
 
 __device__ __forceinline__ int ring_slot(int head, int j) {
     return (head - j + PROJ_MAX) % PROJ_MAX;
@@ -85,30 +85,25 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_gram(int n, const double*
     }
 }
 
-// Sums the block partials (each thread a fixed strided subset, then a fixed
-// reduction tree) and solves for beta on one thread.
+// Sums the block partials (wavefront u sums entries u, u+4, ...: each lane
+// the partials of blocks lane, lane+64, ..., then a shuffle tree) and solves
+// for beta on one thread.
 template <int K>
-__global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(int nblocks,
-                                                             const double* __restrict__ partial,
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(const double* __restrict__ partial,
                                                              ProjState* ps) {
     constexpr int NG = K * (K + 1) / 2 + K;
+    static_assert(PROJ_BLOCKS % 64 == 0, "whole wavefronts of block partials");
     __shared__ double tot[NG];
-    __shared__ double sh[PROJ_THREADS / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int q = 0; q < NG; ++q) {
+#pragma unroll
+    for (int q = wid; q < NG; q += PROJ_THREADS / 64) {
         double v = 0.0;
-        for (int blk = threadIdx.x; blk < nblocks; blk += PROJ_THREADS)
-            v += partial[(size_t)blk * NG + q];
+#pragma unroll
+        for (int blk = lane; blk < PROJ_BLOCKS; blk += 64) v += partial[(size_t)blk * NG + q];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) sh[wid] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = sh[0];
-            for (int u = 1; u < PROJ_THREADS / 64; ++u) t += sh[u];
-            tot[q] = t;
-        }
-        __syncthreads();
+        if (lane == 0) tot[q] = v;
     }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     double G[K][K], c[K], s[K];
     int q = 0;
@@ -210,8 +205,7 @@ void launch_guess_k(hipStream_t s, int n, const double* hp, const double* hw, si
                     ProjState* ps, const double* b, double* partial, double* out) {
     hipLaunchKernelGGL(k_proj_gram<K>, dim3(PROJ_BLOCKS), dim3(PROJ_THREADS), 0, s, n, hw, ld, ps,
                        b, partial);
-    hipLaunchKernelGGL(k_proj_solve<K>, dim3(1), dim3(PROJ_THREADS), 0, s, PROJ_BLOCKS, partial,
-                       ps);
+    hipLaunchKernelGGL(k_proj_solve<K>, dim3(1), dim3(PROJ_THREADS), 0, s, partial, ps);
     hipLaunchKernelGGL(k_proj_combine<K>, dim3((n + PROJ_THREADS - 1) / PROJ_THREADS),
                        dim3(PROJ_THREADS), 0, s, n, hp, ld, ps, out);
 }
