@@ -93,69 +93,91 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(const double* __res
                                                              ProjState* ps) {
     constexpr int NG = K * (K + 1) / 2 + K;
     static_assert(PROJ_BLOCKS % 64 == 0, "whole wavefronts of block partials");
+    constexpr int NW = PROJ_THREADS / 64, QPW = (NG + NW - 1) / NW;
     __shared__ double tot[NG];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // every load of the wavefront's entries first (one round trip), then the
+    // shuffle trees
+    double v[QPW];
 #pragma unroll
-    for (int q = wid; q < NG; q += PROJ_THREADS / 64) {
-        double v = 0.0;
+    for (int j = 0; j < QPW; ++j) {
+        const int q = wid + NW * j;
+        double a = 0.0;
+        if (q < NG) {
 #pragma unroll
-        for (int blk = lane; blk < PROJ_BLOCKS; blk += 64) v += partial[(size_t)blk * NG + q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) tot[q] = v;
+            for (int blk = lane; blk < PROJ_BLOCKS; blk += 64) a += partial[(size_t)blk * NG + q];
+        }
+        v[j] = a;
+    }
+#pragma unroll
+    for (int j = 0; j < QPW; ++j) {
+        double a = v[j];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+        const int q = wid + NW * j;
+        if (lane == 0 && q < NG) tot[q] = a;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    double G[K][K], c[K], s[K];
-    int q = 0;
+    // (every loop below has constant bounds and is unrolled, so the small
+    // system stays in registers)
+    double G[K][K], c[K], sc[K], y[K];
+#pragma unroll
     for (int k = 0; k < K; ++k)
+#pragma unroll
         for (int l = k; l < K; ++l) {
-            G[k][l] = tot[q];
-            G[l][k] = tot[q];
-            ++q;
+            const double g = tot[k * K - k * (k - 1) / 2 + (l - k)];
+            G[k][l] = g;
+            G[l][k] = g;
         }
-    for (int k = 0; k < K; ++k) c[k] = tot[q++];
+#pragma unroll
+    for (int k = 0; k < K; ++k) c[k] = tot[K * (K + 1) / 2 + k];
     // unit columns; a column with no length (no change in time) drops out
-    for (int k = 0; k < K; ++k) s[k] = G[k][k] > 0.0 ? 1.0 / sqrt(G[k][k]) : 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) sc[k] = G[k][k] > 0.0 ? 1.0 / sqrt(G[k][k]) : 0.0;
+#pragma unroll
     for (int k = 0; k < K; ++k) {
-        for (int l = 0; l < K; ++l) G[k][l] *= s[k] * s[l];
+#pragma unroll
+        for (int l = 0; l < K; ++l) G[k][l] *= sc[k] * sc[l];
         G[k][k] += 1e-13;  // (keeps the factorisation defined for dependent columns)
-        c[k] *= s[k];
+        c[k] *= sc[k];
     }
     // Cholesky G = L L^T in place (lower triangle), then L L^T y = c
     bool ok = true;
-    for (int k = 0; k < K && ok; ++k) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
         double dkk = G[k][k];
+#pragma unroll
         for (int m = 0; m < k; ++m) dkk -= G[k][m] * G[k][m];
-        if (!(dkk > 0.0)) {
-            ok = false;
-            break;
-        }
-        const double lkk = sqrt(dkk);
+        ok = ok && dkk > 0.0;
+        const double lkk = sqrt(ok ? dkk : 1.0);
         G[k][k] = lkk;
+#pragma unroll
         for (int r = k + 1; r < K; ++r) {
             double v = G[r][k];
+#pragma unroll
             for (int m = 0; m < k; ++m) v -= G[r][m] * G[k][m];
             G[r][k] = v / lkk;
         }
     }
-    double y[K];
-    if (ok) {
-        for (int k = 0; k < K; ++k) {
-            double v = c[k];
-            for (int m = 0; m < k; ++m) v -= G[k][m] * y[m];
-            y[k] = v / G[k][k];
-        }
-        for (int k = K - 1; k >= 0; --k) {
-            double v = y[k];
-            for (int m = k + 1; m < K; ++m) v -= G[m][k] * y[m];
-            y[k] = v / G[k][k];
-        }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double v = c[k];
+#pragma unroll
+        for (int m = 0; m < k; ++m) v -= G[k][m] * y[m];
+        y[k] = v / G[k][k];
     }
-    for (int k = 0; k < PROJ_MAX; ++k) ps->beta[k] = 0.0;
-    if (ok) {
-        for (int k = 0; k < K; ++k) ps->beta[k] = (isfinite(y[k]) ? y[k] : 0.0) * s[k];
-    } else {
-        ps->beta[0] = 1.0;  // the newest pressure alone
+#pragma unroll
+    for (int k = K - 1; k >= 0; --k) {
+        double v = y[k];
+#pragma unroll
+        for (int m = k + 1; m < K; ++m) v -= G[m][k] * y[m];
+        y[k] = v / G[k][k];
+    }
+#pragma unroll
+    for (int k = 0; k < PROJ_MAX; ++k) {
+        double b = 0.0;
+        if (k < K) b = ok ? (isfinite(y[k]) ? y[k] : 0.0) * sc[k] : (k == 0 ? 1.0 : 0.0);
+        ps->beta[k] = b;  // (no factorisation: the newest pressure alone)
     }
 }
 
