@@ -447,10 +447,13 @@ __global__ __launch_bounds__(BLOCK) void k_residual64(SellView sv, SegView sg,
     block_store<3>(vals, partial, lb, nbk);
 }
 
+// (red[2]: the border unknown as the solve returns it, D^-1 x~, so the host
+// reads it with the residual)
 __global__ __launch_bounds__(FIN) void k_finish_res64(SegView sg, const double* __restrict__ b,
                                                       double* __restrict__ out,
                                                       const double* __restrict__ partial,
-                                                      KScalars* scs) {
+                                                      KScalars* scs, const double* __restrict__ x,
+                                                      const double* __restrict__ dinv) {
     KScalars* sc = scs + blockIdx.x;
     const Seg S = sg.seg[blockIdx.x];
     double tot[3];
@@ -466,6 +469,7 @@ __global__ __launch_bounds__(FIN) void k_finish_res64(SegView sg, const double* 
     }
     sc->red[0] = rr;
     sc->red[1] = bb;
+    sc->red[2] = S.border ? x[S.row0 + S.nv] * dinv[S.row0 + S.nv] : 0.0;
 }
 
 // y = A x with the unscaled operator: A x = A~ (D x)
@@ -1415,7 +1419,7 @@ void PpeSolver::residual64(hipStream_t st, const double* b_dev, std::vector<doub
     hipLaunchKernelGGL(k_residual64, dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, sv, sg, aval.p,
                        dinv.p, b_dev, xt.p, tmp.p, partial.p);
     hipLaunchKernelGGL(k_finish_res64, dim3(nseg), dim3(FIN), 0, st, sg, b_dev, tmp.p, partial.p,
-                       sc.p);
+                       sc.p, xt.p, dinv.p);
     MP_HIP(hipGetLastError());
     MP_HIP(hipMemcpyAsync(host_sc, sc.p, sizeof(KScalars) * nseg, hipMemcpyDeviceToHost, st));
     MP_HIP(hipStreamSynchronize(st));
@@ -1439,7 +1443,7 @@ double* PpeSolver::residual_of(hipStream_t st, const double* b_dev, const double
     hipLaunchKernelGGL(k_residual64, dim3(grid_xcd(nbk)), dim3(BLOCK), 0, st, sv, sg, aval.p,
                        dinv.p, b_dev, xt.p, tmp.p, partial.p);
     hipLaunchKernelGGL(k_finish_res64, dim3(nseg), dim3(FIN), 0, st, sg, b_dev, tmp.p, partial.p,
-                       sc.p);
+                       sc.p, xt.p, dinv.p);
     MP_HIP(hipGetLastError());
     return tmp.p;
 }
@@ -1795,16 +1799,18 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
         rr = rr_last;
     else
         residual64(st, b_dev, rr, bb);
-    hipLaunchKernelGGL(k_scale, dim3(nbk), dim3(BLOCK), 0, st, nrows, dinv.p, xt.p, x_dev, 0);
-    MP_HIP(hipGetLastError());
+    // the border unknowns came with the last residual (the record of the
+    // current x~), so x = D^-1 x~ needs no wait
     std::vector<double> alpha(nseg, 0.0);
     for (int q = 0; q < nseg; ++q)
-        if (segs[q].border)
-            MP_HIP(hipMemcpyAsync(&alpha[q], x_dev + segs[q].row0 + segs[q].nv, sizeof(double),
-                                  hipMemcpyDeviceToHost, st));
-    MP_HIP(hipStreamSynchronize(st));
-    if (spmv_timer.enabled) spmv_timer.harvest();
-    if (smooth_timer.enabled) smooth_timer.harvest();
+        if (segs[q].border) alpha[q] = host_sc[q].red[2];
+    hipLaunchKernelGGL(k_scale, dim3(nbk), dim3(BLOCK), 0, st, nrows, dinv.p, xt.p, x_dev, 0);
+    MP_HIP(hipGetLastError());
+    if (spmv_timer.enabled || smooth_timer.enabled) {
+        MP_HIP(hipStreamSynchronize(st));
+        if (spmv_timer.enabled) spmv_timer.harvest();
+        if (smooth_timer.enabled) smooth_timer.harvest();
+    }
     SolveStats agg;
     double rr_all = 0, bb_all = 0;
     agg.converged = 1;
