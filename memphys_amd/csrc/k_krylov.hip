@@ -1539,12 +1539,16 @@ void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const d
             if (f[q].done == 0 && !f[q].full) return true;
         return false;
     };
+    bool first_cycle = true;
     while (true) {
         // one cycle: chunks of iterations, the flags polled one chunk behind
+        // (the first cycle's chunks follow the last run's iteration count)
         int slot = 0;
         bool have_prev = false;
         for (int j = 0; j < m;) {
-            const int c = std::min(chunk, m - j);
+            int c = chunk;
+            if (first_cycle && gm_hint > 0) c = j == 0 ? std::max(1, gm_hint - 1) : 2;
+            c = std::min(c, m - j);
             run_gmres_chunk<TM, TV>(st, j, c, m, a, di);
             j += c;
             MP_HIP(hipMemcpyAsync(hf[slot], gmf_d.p, sizeof(GmFlags) * nseg,
@@ -1572,8 +1576,13 @@ void PpeSolver::run_gmres(hipStream_t st, const TM* a, const double* di, const d
         bool restart = false;
         for (int q = 0; q < nseg; ++q) restart |= host_gmf[q].full && host_gmf[q].done == 0;
         if (!restart) break;
+        first_cycle = false;
         start(1);
     }
+    int most = 0;
+    for (int q = 0; q < nseg; ++q)
+        if (host_ctl[q].active) most = std::max(most, host_gmf[q].iter);
+    gm_hint = most;
     for (int q = 0; q < nseg; ++q) {
         if (!host_ctl[q].active) continue;
         done[q] = host_gmf[q].done;

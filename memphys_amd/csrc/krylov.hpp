@@ -121,6 +121,11 @@ struct PpeSolver {
     int precision = 0;
     double tol = 1e-8, inner_tol = 1e-3;
     int maxit = 1000, check_every = 8;
+    // GMRES iterations of the last run (0: none yet): the next run's first
+    // chunk stops one short of it, and the chunks after it are of two
+    // iterations, so a solve that converges as fast as the last one leaves
+    // few queued iterations to return at once
+    int gm_hint = 0;
     long fp64_fallbacks = 0;
     KernelTimer spmv_timer;
     KernelTimer smooth_timer;         // the multigrid's level-0 post-smoothing product
