@@ -39,43 +39,80 @@ __device__ __forceinline__ void backward_differences(double (&v)[K], double (&d)
     }
 }
 
+// Gram entry q of the K differences (q < K(K+1)/2: the pair (k, l), k <= l,
+// in row order of the upper triangle; after them c_k = d_k . b): its two
+// factor indices, K standing for b.
+constexpr int gram_a(int K, int q) {
+    for (int k = 0; k < K; ++k) {
+        if (q < K - k) return k;
+        q -= K - k;
+    }
+    return q;
+}
+constexpr int gram_b(int K, int q) {
+    for (int k = 0; k < K; ++k) {
+        if (q < K - k) return k + q;
+        q -= K - k;
+    }
+    return K;
+}
+// threads per row: windows above 12 split each row's Gram entries over two
+// threads (the second loads the row again, from cache), so the accumulators
+// stay in registers (beyond 16 the compiler no longer keeps the unrolled
+// loops in registers)
+template <int K>
+constexpr int gram_split() { return K <= 12 ? 1 : 2; }
+static_assert(PROJ_MAX <= 16, "windows up to 16 stay in registers");
+
 template <int K>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_gram(int n, const double* __restrict__ hw,
                                                             size_t ld, const ProjState* ps,
                                                             const double* __restrict__ b,
                                                             double* __restrict__ partial) {
     constexpr int NG = K * (K + 1) / 2 + K;
-    double acc[NG];
+    constexpr int NS = gram_split<K>();
+    constexpr int NPER = (NG + NS - 1) / NS;
+    double acc[NPER];
 #pragma unroll
-    for (int q = 0; q < NG; ++q) acc[q] = 0.0;
+    for (int j = 0; j < NPER; ++j) acc[j] = 0.0;
     const int head = ps->head;
-    for (int i = blockIdx.x * PROJ_THREADS + threadIdx.x; i < n; i += gridDim.x * PROJ_THREADS) {
+    const int part = NS == 1 ? 0 : (int)(threadIdx.x % NS);
+    for (int i = (blockIdx.x * PROJ_THREADS + threadIdx.x) / NS; i < n;
+         i += gridDim.x * (PROJ_THREADS / NS)) {
         double w[K], d[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) w[j] = hw[(size_t)ring_slot(head, j) * ld + i];
         backward_differences<K>(w, d);
         const double bi = b[i];
-        int q = 0;
+        // (unrolled: every index below is a constant, so d stays in registers)
 #pragma unroll
-        for (int k = 0; k < K; ++k)
+        for (int j = 0; j < NPER; ++j) {
+            // this thread's entry j: part p takes entry p * NPER + j
+            double u = 0.0, v = 0.0;
+            bool any = false;
 #pragma unroll
-            for (int l = k; l < K; ++l) {
-                acc[q] = fma(d[k], d[l], acc[q]);
-                ++q;
+            for (int p = 0; p < NS; ++p) {
+                const int q = p * NPER + j;
+                if (q < NG) {
+                    const int bq = gram_b(K, q);
+                    const double uq = d[gram_a(K, q)], vq = bq >= K ? bi : d[bq];
+                    u = part == p ? uq : u;
+                    v = part == p ? vq : v;
+                    any = any || part == p;
+                }
             }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            acc[q] = fma(d[k], bi, acc[q]);
-            ++q;
+            if (any) acc[j] = fma(u, v, acc[j]);
         }
     }
+    // block sums per entry: the shuffle tree keeps the row parts apart
     __shared__ double sh[PROJ_THREADS / 64][NG];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < NG; ++q) {
-        double v = acc[q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) sh[wid][q] = v;
+    for (int j = 0; j < NPER; ++j) {
+        double v = acc[j];
+        for (int off = 32; off >= NS; off >>= 1) v += __shfl_xor(v, off);
+        const int q = lane * NPER + j;  // lane < NS holds part `lane`
+        if (lane < NS && q < NG) sh[wid][q] = v;
     }
     __syncthreads();
     for (int q = threadIdx.x; q < NG; q += PROJ_THREADS) {
@@ -86,98 +123,78 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_gram(int n, const double*
 }
 
 // Sums the block partials (wavefront u sums entries u, u+4, ...: each lane
-// the partials of blocks lane, lane+64, ..., then a shuffle tree) and solves
-// for beta on one thread.
+// the partials of blocks lane, lane+64, ..., then a shuffle tree), then one
+// wavefront solves for beta: lane r holds row r of the scaled Gram matrix,
+// and the right-looking Cholesky and the two triangular solves pass columns
+// between lanes by shuffles.
 template <int K>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(const double* __restrict__ partial,
                                                              ProjState* ps) {
     constexpr int NG = K * (K + 1) / 2 + K;
     static_assert(PROJ_BLOCKS % 64 == 0, "whole wavefronts of block partials");
-    constexpr int NW = PROJ_THREADS / 64, QPW = (NG + NW - 1) / NW;
+    static_assert(K <= 64, "one row per lane");
     __shared__ double tot[NG];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    // every load of the wavefront's entries first (one round trip), then the
-    // shuffle trees
-    double v[QPW];
-#pragma unroll
-    for (int j = 0; j < QPW; ++j) {
-        const int q = wid + NW * j;
+#pragma unroll 8
+    for (int q = wid; q < NG; q += PROJ_THREADS / 64) {
         double a = 0.0;
-        if (q < NG) {
 #pragma unroll
-            for (int blk = lane; blk < PROJ_BLOCKS; blk += 64) a += partial[(size_t)blk * NG + q];
-        }
-        v[j] = a;
-    }
-#pragma unroll
-    for (int j = 0; j < QPW; ++j) {
-        double a = v[j];
+        for (int blk = lane; blk < PROJ_BLOCKS; blk += 64) a += partial[(size_t)blk * NG + q];
         for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
-        const int q = wid + NW * j;
-        if (lane == 0 && q < NG) tot[q] = a;
+        if (lane == 0) tot[q] = a;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    // (every loop below has constant bounds and is unrolled, so the small
-    // system stays in registers)
-    double G[K][K], c[K], sc[K], y[K];
+    if (wid != 0) return;
+    const int r = lane < K ? lane : 0;  // (lanes past K mirror row 0 and are not used)
+    auto idx = [](int k, int l) { return k * K - k * (k - 1) / 2 + (l - k); };  // k <= l
+    double g[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-#pragma unroll
-        for (int l = k; l < K; ++l) {
-            const double g = tot[k * K - k * (k - 1) / 2 + (l - k)];
-            G[k][l] = g;
-            G[l][k] = g;
-        }
-#pragma unroll
-    for (int k = 0; k < K; ++k) c[k] = tot[K * (K + 1) / 2 + k];
+    for (int l = 0; l < K; ++l) g[l] = tot[r <= l ? idx(r, l) : idx(l, r)];
+    const double grr = tot[idx(r, r)];
     // unit columns; a column with no length (no change in time) drops out
+    const double s = grr > 0.0 ? 1.0 / sqrt(grr) : 0.0;
+    double c = tot[K * (K + 1) / 2 + r] * s;
 #pragma unroll
-    for (int k = 0; k < K; ++k) sc[k] = G[k][k] > 0.0 ? 1.0 / sqrt(G[k][k]) : 0.0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-#pragma unroll
-        for (int l = 0; l < K; ++l) G[k][l] *= sc[k] * sc[l];
-        G[k][k] += 1e-13;  // (keeps the factorisation defined for dependent columns)
-        c[k] *= sc[k];
+    for (int l = 0; l < K; ++l) {
+        g[l] *= s * __shfl(s, l);
+        if (l == r) g[l] += 1e-13;  // (keeps the factorisation defined for dependent columns)
     }
-    // Cholesky G = L L^T in place (lower triangle), then L L^T y = c
+    // Cholesky, column by column: lane r's g[k] becomes L[r][k] (r >= k)
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        double dkk = G[k][k];
-#pragma unroll
-        for (int m = 0; m < k; ++m) dkk -= G[k][m] * G[k][m];
+        const double dkk = __shfl(g[k], k);
         ok = ok && dkk > 0.0;
         const double lkk = sqrt(ok ? dkk : 1.0);
-        G[k][k] = lkk;
+        g[k] = r > k ? g[k] / lkk : (r == k ? lkk : g[k]);
 #pragma unroll
-        for (int r = k + 1; r < K; ++r) {
-            double v = G[r][k];
-#pragma unroll
-            for (int m = 0; m < k; ++m) v -= G[r][m] * G[k][m];
-            G[r][k] = v / lkk;
+        for (int m = k + 1; m < K; ++m) {
+            const double lmk = __shfl(g[k], m);
+            if (r > k) g[m] -= g[k] * lmk;
         }
     }
+    // L y = c, then L^T x = y (y, x the same in every lane)
+    double y[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        double v = c[k];
-#pragma unroll
-        for (int m = 0; m < k; ++m) v -= G[k][m] * y[m];
-        y[k] = v / G[k][k];
+        y[k] = __shfl(c, k) / __shfl(g[k], k);
+        if (r > k) c -= g[k] * y[k];
     }
 #pragma unroll
     for (int k = K - 1; k >= 0; --k) {
-        double v = y[k];
+        y[k] = y[k] / __shfl(g[k], k);
 #pragma unroll
-        for (int m = k + 1; m < K; ++m) v -= G[m][k] * y[m];
-        y[k] = v / G[k][k];
+        for (int m = 0; m < k; ++m) y[m] -= __shfl(g[m], k) * y[k];
     }
+    double sk[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) sk[k] = __shfl(s, k);
+    if (lane != 0) return;
 #pragma unroll
     for (int k = 0; k < PROJ_MAX; ++k) {
-        double b = 0.0;
-        if (k < K) b = ok ? (isfinite(y[k]) ? y[k] : 0.0) * sc[k] : (k == 0 ? 1.0 : 0.0);
-        ps->beta[k] = b;  // (no factorisation: the newest pressure alone)
+        double bk = 0.0;
+        if (k < K) bk = ok ? (isfinite(y[k]) ? y[k] : 0.0) * sk[k] : (k == 0 ? 1.0 : 0.0);
+        ps->beta[k] = bk;  // (no factorisation: the newest pressure alone)
     }
 }
 
@@ -250,7 +267,11 @@ void launch_proj_guess(hipStream_t s, int k, int n, const double* hp, const doub
         case 9: launch_guess_k<9>(s, n, hp, hw, ld, ps, b, partial, out); break;
         case 10: launch_guess_k<10>(s, n, hp, hw, ld, ps, b, partial, out); break;
         case 11: launch_guess_k<11>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        default: launch_guess_k<12>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 12: launch_guess_k<12>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 13: launch_guess_k<13>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 14: launch_guess_k<14>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        case 15: launch_guess_k<15>(s, n, hp, hw, ld, ps, b, partial, out); break;
+        default: launch_guess_k<16>(s, n, hp, hw, ld, ps, b, partial, out); break;
     }
     MP_HIP(hipGetLastError());
 }

@@ -75,7 +75,7 @@ def test_raising_the_order_waits_for_the_history(cube3d_small):
     assert not all(np.array_equal(a, b) for a, b in zip(raised[-1], base[-1]))
 
 
-def _run_project(cloud, steps, window=12):
+def _run_project(cloud, steps, window=16):
     import memphys_amd as M
     u_dir, p_dir, u0 = lid_bc(cloud, cloud.dim - 1)
     out, iters, gres = [], [], []
@@ -92,7 +92,7 @@ def _run_project(cloud, steps, window=12):
         ctx.guess_mode(M.MP_GUESS_PROJECT)
         ctx.guess_window(window)
         with pytest.raises(M.MemphysError):
-            ctx.guess_window(13)
+            ctx.guess_window(17)
         for it in range(steps):
             ctx.step(1, it)
             st = ctx.stats()
@@ -104,7 +104,7 @@ def _run_project(cloud, steps, window=12):
 
 
 def test_projected_guess_agrees_and_starts_closest(cube3d_small):
-    """The projection onto the last 12 solutions reaches the same fields as the
+    """The projection onto the last 16 solutions reaches the same fields as the
     order-2 extrapolation, and from step 10 on starts from a smaller residual
     and takes fewer iterations than the order-4 extrapolation."""
     steps = 40
