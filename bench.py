@@ -11,7 +11,7 @@ dt = 5e-4, AB2, 500 steps from rest; pressure-Poisson relative residual 1e-9
 (defaults 5 and 495: the whole 500-step run after 5 warm-up steps): FP32-
 operator inner GMRES(30) preconditioned by an FP32 smoothed-aggregation
 multigrid V-cycle, in FP64 defect correction, with the learned initial guess
-correcting the projection onto the last 16 solutions (the network trained
+correcting the projection onto the last 20 solutions (the network trained
 first, untimed, on a 63,536-node cube with the same physics: 2000 solver steps
 in 10 episodes from rest with seeded lid speeds in [0.5, 1.5]).
 
@@ -747,7 +747,7 @@ def main():
             "nv": cloud.nv, "nnz": cloud.nnz,
             "form": (f"FP32-operator inner {kname} with FP32 smoothed-aggregation multigrid "
                      "V-cycle, in FP64 defect correction, + learned guess correcting the "
-                     "projection onto the last 16 solutions"),
+                     "projection onto the last 20 solutions"),
             "guess_training": f"{args.train_steps} solver steps on {small.nv} nodes, "
                               f"lid speeds U[0.5,1.5], {t_train:.1f}s",
         },

@@ -47,7 +47,7 @@ enum mp_precision { MP_PREC_FP64 = 0, MP_PREC_MIXED = 1 };
  * extrapolation alone: the polynomial through the pressures of the last 2, 3
  * or 4 steps (mp_guess_order; equal time steps), evaluated at the next step,
  * or the projection: the combination of the last K solutions (mp_guess_window,
- * default 16) whose residual against the new right-hand side is smallest
+ * default 20) whose residual against the new right-hand side is smallest
  * (Fischer's successive-right-hand-side projection; A p of each solution is
  * kept from its solve's final FP64 residual).  The extrapolating guesses
  * fall back to the previous pressure until two pressures are in the history,
@@ -228,7 +228,7 @@ int mp_guess_mode(mp_ctx* ctx, int mode);
  * and 8.3 in norm): with loose tolerances a lower order can start closer.
  * No counterpart in the reference (its guess is the previous pressure). */
 int mp_guess_order(mp_ctx* ctx, int order);
-/* Solutions the projected guess combines (1..16, default 16). */
+/* Solutions the projected guess combines (1..24, default 20). */
 int mp_guess_window(mp_ctx* ctx, int k);
 /* The classical guess the learned network corrects: MP_GUESS_PROJECT (the
  * default) or MP_GUESS_EXTRAPOLATE.  It is also the second candidate the

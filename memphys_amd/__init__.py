@@ -500,7 +500,7 @@ class Context:
         self._check(self.lib.mp_guess_order(self.h, order), "mp_guess_order")
 
     def guess_window(self, k: int):
-        """Solutions (1..16) the projected guess combines; default 16."""
+        """Solutions (1..24) the projected guess combines; default 20."""
         self._check(self.lib.mp_guess_window(self.h, k), "mp_guess_window")
 
     def guess_base(self, base: int):

@@ -20,7 +20,11 @@ namespace mp {
 namespace {
 
 constexpr int PROJ_THREADS = 256;
-constexpr int PROJ_BLOCKS = 256;
+constexpr int PROJ_BLOCKS = 512;  // two per CU
+constexpr int PROJ_TILE = PROJ_THREADS;  // rows per pass of a Gram block, one per thread
+static_assert(PROJ_MAX < 32, "the Gram matrix with b fits two 16-column blocks");
+
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int ring_slot(int head, int j) {
     return (head - j + PROJ_MAX) % PROJ_MAX;
@@ -39,30 +43,40 @@ __device__ __forceinline__ void backward_differences(double (&v)[K], double (&d)
     }
 }
 
-// Gram entry q of the K differences (q < K(K+1)/2: the pair (k, l), k <= l,
-// in row order of the upper triangle; after them c_k = d_k . b): its two
-// factor indices, K standing for b.
-constexpr int gram_a(int K, int q) {
-    for (int k = 0; k < K; ++k) {
-        if (q < K - k) return k;
-        q -= K - k;
-    }
-    return q;
-}
-constexpr int gram_b(int K, int q) {
-    for (int k = 0; k < K; ++k) {
-        if (q < K - k) return k + q;
-        q -= K - k;
-    }
-    return K;
-}
-// threads per row: windows above 12 split each row's Gram entries over two
-// threads (the second loads the row again, from cache), so the accumulators
-// stay in registers (beyond 16 the compiler no longer keeps the unrolled
-// loops in registers)
+// The Gram pass on the matrix cores.  The tile's rows hold the K differences
+// and b (columns 0..K-1 and K); v_mfma_f64_16x16x4f64 adds, per four rows,
+// C[i][j] += sum_r D[r][16 I + i] D[r][16 J + j] over 16-column blocks I, J
+// (lane l supplies D[r = l/16][16 I + l%16] as both operands' element, and
+// holds C[l/16 + 4 v][l % 16] in its register v).  Tiles: K < 16, the one
+// block (b in column K); K = 16, block 0 against itself and against b alone;
+// K > 16, blocks (0,0), (0,1) and (1,1), b in column K of block 1.
 template <int K>
-constexpr int gram_split() { return K <= 12 ? 1 : 2; }
-static_assert(PROJ_MAX <= 16, "windows up to 16 stay in registers");
+constexpr int gram_tiles() { return K < 16 ? 1 : (K == 16 ? 2 : 3); }
+// Row stride of the tile in LDS (doubles): odd, so a quarter-wavefront's
+// sixteen consecutive columns and a wavefront's rows spread over the banks.
+template <int K>
+constexpr int tile_ld() { return (K + 1) % 2 ? K + 1 : K + 2; }
+template <int K>
+constexpr int gram_lds() {
+    constexpr int a = PROJ_TILE * tile_ld<K>() + 32;  // (+32: the second block's reads past the last row)
+    constexpr int b = (PROJ_THREADS / 64) * gram_tiles<K>() * 64 * 4;
+    return a > b ? a : b;
+}
+
+// Gram entry (k, l), k <= l, l = K standing for b: where a wavefront's
+// accumulators hold it (tile, lane, register).
+template <int K>
+__device__ __forceinline__ int gram_slot(int k, int l) {
+    int t, i, j;
+    if (K < 16) {
+        t = 0, i = k, j = l;
+    } else if (K == 16) {
+        t = l == K ? 1 : 0, i = k, j = l == K ? 0 : l;
+    } else {
+        t = k >= 16 ? 2 : (l >= 16 ? 1 : 0), i = k & 15, j = l & 15;
+    }
+    return (t * 64 + j + 16 * (i & 3)) * 4 + (i >> 2);
+}
 
 template <int K>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_gram(int n, const double* __restrict__ hw,
@@ -70,76 +84,123 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_gram(int n, const double*
                                                             const double* __restrict__ b,
                                                             double* __restrict__ partial) {
     constexpr int NG = K * (K + 1) / 2 + K;
-    constexpr int NS = gram_split<K>();
-    constexpr int NPER = (NG + NS - 1) / NS;
-    double acc[NPER];
-#pragma unroll
-    for (int j = 0; j < NPER; ++j) acc[j] = 0.0;
+    constexpr int NT = gram_tiles<K>();
+    constexpr int LDT = tile_ld<K>();
+    __shared__ double sd[gram_lds<K>()];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int col = lane & 15, sub = lane >> 4;
     const int head = ps->head;
-    const int part = NS == 1 ? 0 : (int)(threadIdx.x % NS);
-    for (int i = (blockIdx.x * PROJ_THREADS + threadIdx.x) / NS; i < n;
-         i += gridDim.x * (PROJ_THREADS / NS)) {
-        double w[K], d[K];
+    // two accumulator sets (even and odd row groups) so consecutive matrix
+    // operations do not wait on each other
+    const d4 zero = {0.0, 0.0, 0.0, 0.0};
+    d4 acc[2][NT];
 #pragma unroll
-        for (int j = 0; j < K; ++j) w[j] = hw[(size_t)ring_slot(head, j) * ld + i];
-        backward_differences<K>(w, d);
-        const double bi = b[i];
-        // (unrolled: every index below is a constant, so d stays in registers)
+    for (int e = 0; e < 2; ++e)
 #pragma unroll
-        for (int j = 0; j < NPER; ++j) {
-            // this thread's entry j: part p takes entry p * NPER + j
-            double u = 0.0, v = 0.0;
-            bool any = false;
+        for (int t = 0; t < NT; ++t) acc[e][t] = zero;
+    const int ntiles = (n + PROJ_TILE - 1) / PROJ_TILE;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int i = tile * PROJ_TILE + threadIdx.x;
+        double* row = sd + threadIdx.x * LDT;
+        if (i < n) {
+            double w[K], d[K];
 #pragma unroll
-            for (int p = 0; p < NS; ++p) {
-                const int q = p * NPER + j;
-                if (q < NG) {
-                    const int bq = gram_b(K, q);
-                    const double uq = d[gram_a(K, q)], vq = bq >= K ? bi : d[bq];
-                    u = part == p ? uq : u;
-                    v = part == p ? vq : v;
-                    any = any || part == p;
+            for (int j = 0; j < K; ++j) w[j] = hw[(size_t)ring_slot(head, j) * ld + i];
+            backward_differences<K>(w, d);
+#pragma unroll
+            for (int k = 0; k < K; ++k) row[k] = d[k];
+            row[K] = b[i];
+        } else {
+#pragma unroll
+            for (int k = 0; k <= K; ++k) row[k] = 0.0;
+        }
+        __syncthreads();
+        // wavefront wid takes the four-row groups wid, wid + 4, ..., alternately
+        // into its two accumulator sets
+        constexpr int NW = PROJ_THREADS / 64;
+        static_assert((PROJ_TILE / 4) % (2 * NW) == 0, "whole pairs of row groups per wavefront");
+#pragma unroll 2
+        for (int it = 0; it < PROJ_TILE / 4 / (2 * NW); ++it) {
+            const int g0 = wid + 2 * NW * it;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const double* rr = sd + (4 * (g0 + e * NW) + sub) * LDT;
+                const double a0 = col <= K ? rr[col] : 0.0;
+                if constexpr (K < 16) {
+                    acc[e][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, acc[e][0], 0, 0, 0);
+                } else if constexpr (K == 16) {
+                    const double bb = col == 0 ? rr[K] : 0.0;
+                    acc[e][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, acc[e][0], 0, 0, 0);
+                    acc[e][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bb, acc[e][1], 0, 0, 0);
+                } else {
+                    const double a1 = 16 + col <= K ? rr[16 + col] : 0.0;
+                    acc[e][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, acc[e][0], 0, 0, 0);
+                    acc[e][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a1, acc[e][1], 0, 0, 0);
+                    acc[e][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, acc[e][2], 0, 0, 0);
                 }
             }
-            if (any) acc[j] = fma(u, v, acc[j]);
         }
+        __syncthreads();
     }
-    // block sums per entry: the shuffle tree keeps the row parts apart
-    __shared__ double sh[PROJ_THREADS / 64][NG];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // block sums: each wavefront's accumulators through LDS, then one thread per entry
 #pragma unroll
-    for (int j = 0; j < NPER; ++j) {
-        double v = acc[j];
-        for (int off = 32; off >= NS; off >>= 1) v += __shfl_xor(v, off);
-        const int q = lane * NPER + j;  // lane < NS holds part `lane`
-        if (lane < NS && q < NG) sh[wid][q] = v;
+    for (int t = 0; t < NT; ++t) {
+        const d4 s = acc[0][t] + acc[1][t];
+        double* dst = sd + ((wid * NT + t) * 64 + lane) * 4;
+        dst[0] = s[0];
+        dst[1] = s[1];
+        dst[2] = s[2];
+        dst[3] = s[3];
     }
     __syncthreads();
     for (int q = threadIdx.x; q < NG; q += PROJ_THREADS) {
-        double v = sh[0][q];
-        for (int u = 1; u < PROJ_THREADS / 64; ++u) v += sh[u][q];
+        int k, l;
+        if (q < K * (K + 1) / 2) {
+            int r = q;
+            k = 0;
+            while (r >= K - k) r -= K - k++;
+            l = k + r;
+        } else {
+            k = q - K * (K + 1) / 2;
+            l = K;
+        }
+        const int s = gram_slot<K>(k, l);
+        double v = 0.0;
+#pragma unroll
+        for (int u = 0; u < PROJ_THREADS / 64; ++u) v += sd[u * NT * 64 * 4 + s];
         partial[(size_t)blockIdx.x * NG + q] = v;
     }
+}
+
+// Value of lane src (wave-uniform) in every lane.
+__device__ __forceinline__ double lane_value(double v, int src) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Sums the block partials (wavefront u sums entries u, u+4, ...: each lane
 // the partials of blocks lane, lane+64, ..., then a shuffle tree), then one
 // wavefront solves for beta: lane r holds row r of the scaled Gram matrix,
-// and the right-looking Cholesky and the two triangular solves pass columns
-// between lanes by shuffles.
+// the right-looking Cholesky and the two triangular solves take the values of
+// other rows by lane reads, and lane k ends with beta_k.
 template <int K>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(const double* __restrict__ partial,
                                                              ProjState* ps) {
     constexpr int NG = K * (K + 1) / 2 + K;
     static_assert(PROJ_BLOCKS % 64 == 0, "whole wavefronts of block partials");
-    static_assert(K <= 64, "one row per lane");
+    static_assert(K <= 64 && PROJ_MAX <= 64, "one row per lane");
     __shared__ double tot[NG];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll 8
-    for (int q = wid; q < NG; q += PROJ_THREADS / 64) {
+    constexpr int NW = PROJ_THREADS / 64;
+#pragma unroll 4
+    for (int j = 0; j < (NG + NW - 1) / NW; ++j) {
+        const int q = wid + NW * j;
+        if (q >= NG) break;
         double a = 0.0;
 #pragma unroll
-        for (int blk = lane; blk < PROJ_BLOCKS; blk += 64) a += partial[(size_t)blk * NG + q];
+        for (int u = 0; u < PROJ_BLOCKS / 64; ++u) a += partial[(size_t)(lane + 64 * u) * NG + q];
         for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
         if (lane == 0) tot[q] = a;
     }
@@ -156,46 +217,45 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(const double* __res
     double c = tot[K * (K + 1) / 2 + r] * s;
 #pragma unroll
     for (int l = 0; l < K; ++l) {
-        g[l] *= s * __shfl(s, l);
+        g[l] *= s * lane_value(s, l);
         if (l == r) g[l] += 1e-13;  // (keeps the factorisation defined for dependent columns)
     }
     // Cholesky, column by column: lane r's g[k] becomes L[r][k] (r >= k)
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const double dkk = __shfl(g[k], k);
+        const double dkk = lane_value(g[k], k);
         ok = ok && dkk > 0.0;
         const double lkk = sqrt(ok ? dkk : 1.0);
         g[k] = r > k ? g[k] / lkk : (r == k ? lkk : g[k]);
 #pragma unroll
         for (int m = k + 1; m < K; ++m) {
-            const double lmk = __shfl(g[k], m);
+            const double lmk = lane_value(g[k], m);
             if (r > k) g[m] -= g[k] * lmk;
         }
     }
-    // L y = c, then L^T x = y (y, x the same in every lane)
-    double y[K];
+    // L y = c (lane k keeps y_k in yl), then L^T x = y (lane k ends with x_k)
+    double yl = 0.0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        y[k] = __shfl(c, k) / __shfl(g[k], k);
-        if (r > k) c -= g[k] * y[k];
+        const double yk = lane_value(c, k) / lane_value(g[k], k);
+        if (r > k) c -= g[k] * yk;
+        if (r == k) yl = yk;
     }
 #pragma unroll
     for (int k = K - 1; k >= 0; --k) {
-        y[k] = y[k] / __shfl(g[k], k);
+        const double xk = lane_value(yl, k) / lane_value(g[k], k);
+        if (r == k) yl = xk;
 #pragma unroll
-        for (int m = 0; m < k; ++m) y[m] -= __shfl(g[m], k) * y[k];
+        for (int m = 0; m < k; ++m) {
+            const double lkm = lane_value(g[m], k);
+            if (r == m) yl -= lkm * xk;
+        }
     }
-    double sk[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) sk[k] = __shfl(s, k);
-    if (lane != 0) return;
-#pragma unroll
-    for (int k = 0; k < PROJ_MAX; ++k) {
-        double bk = 0.0;
-        if (k < K) bk = ok ? (isfinite(y[k]) ? y[k] : 0.0) * sk[k] : (k == 0 ? 1.0 : 0.0);
-        ps->beta[k] = bk;  // (no factorisation: the newest pressure alone)
-    }
+    if (lane >= PROJ_MAX) return;
+    double bk = 0.0;
+    if (lane < K) bk = ok ? (isfinite(yl) ? yl : 0.0) * s : (lane == 0 ? 1.0 : 0.0);
+    ps->beta[lane] = bk;  // (no factorisation: the newest pressure alone)
 }
 
 template <int K>
@@ -249,30 +309,23 @@ void launch_guess_k(hipStream_t s, int n, const double* hp, const double* hw, si
                        dim3(PROJ_THREADS), 0, s, n, hp, ld, ps, out);
 }
 
+// launch_guess_k<k> for the window k (1 <= k <= K)
+template <int K>
+void launch_guess_upto(hipStream_t s, int k, int n, const double* hp, const double* hw, size_t ld,
+                       ProjState* ps, const double* b, double* partial, double* out) {
+    if constexpr (K > 1) {
+        if (k < K) return launch_guess_upto<K - 1>(s, k, n, hp, hw, ld, ps, b, partial, out);
+    }
+    launch_guess_k<K>(s, n, hp, hw, ld, ps, b, partial, out);
+}
+
 }  // namespace
 
 size_t proj_partial_size() { return (size_t)PROJ_BLOCKS * (PROJ_MAX * (PROJ_MAX + 1) / 2 + PROJ_MAX); }
 
 void launch_proj_guess(hipStream_t s, int k, int n, const double* hp, const double* hw, size_t ld,
                        ProjState* ps, const double* b, double* partial, double* out) {
-    switch (k) {
-        case 1: launch_guess_k<1>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 2: launch_guess_k<2>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 3: launch_guess_k<3>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 4: launch_guess_k<4>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 5: launch_guess_k<5>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 6: launch_guess_k<6>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 7: launch_guess_k<7>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 8: launch_guess_k<8>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 9: launch_guess_k<9>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 10: launch_guess_k<10>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 11: launch_guess_k<11>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 12: launch_guess_k<12>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 13: launch_guess_k<13>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 14: launch_guess_k<14>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        case 15: launch_guess_k<15>(s, n, hp, hw, ld, ps, b, partial, out); break;
-        default: launch_guess_k<16>(s, n, hp, hw, ld, ps, b, partial, out); break;
-    }
+    launch_guess_upto<PROJ_MAX>(s, k, n, hp, hw, ld, ps, b, partial, out);
     MP_HIP(hipGetLastError());
 }
 

@@ -354,7 +354,7 @@ void launch_copy(hipStream_t s, double* dst, const double* src, int n);
 // their products with the pressure operator (vectors of ld doubles each); the
 // device record holds the newest slot, the entries held and the last
 // coefficients.
-constexpr int PROJ_MAX = 16;
+constexpr int PROJ_MAX = 24;
 struct ProjState {
     int head;
     int count;

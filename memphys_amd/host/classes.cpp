@@ -100,7 +100,7 @@ void PARAMETERS::read_gpu_settings(std::string settings_file) {
                 else bad();
             } else if (key == "projection_window") {
                 guess_window = std::stoi(val);
-                if (guess_window < 1 || guess_window > 16) bad();
+                if (guess_window < 1 || guess_window > 24) bad();
             } else if (key == "learned_base") {
                 if (val == "extrapolate") guess_base = 2;
                 else if (val == "project") guess_base = 3;

@@ -122,7 +122,7 @@ class PARAMETERS {
     int preconditioner = 0;  // 0 Jacobi, 1 multigrid V-cycle (enum mp_precond)
     int guess = 0;           // 0 previous pressure, 1 learned, 2 extrapolation, 3 projection
     int guess_order = 4;     // pressures the extrapolation goes through (2..4)
-    int guess_window = 16;   // solutions the projection combines (1..16)
+    int guess_window = 20;   // solutions the projection combines (1..24)
     int guess_base = 3;      // the classical guess the learned one corrects (2 or 3)
     std::string guess_params_file;  // packed float32 network parameters (guess = 1)
     int graphs = 1;          // replay the Krylov chunks from hipGraphs
@@ -137,7 +137,7 @@ class PARAMETERS {
     void verify_parameters();
     // Keys: precision (fp64 | mixed), preconditioner (jacobi | multigrid),
     // guess (previous | extrapolate | project | learned), guess_params (file),
-    // extrapolation_order (2 | 3 | 4), projection_window (1..16),
+    // extrapolation_order (2 | 3 | 4), projection_window (1..24),
     // learned_base (extrapolate | project), graphs (0 | 1), device (int),
     // inner_tolerance (double).  Unknown keys and values are errors.
     void read_gpu_settings(std::string settings_file);

@@ -92,7 +92,7 @@ def _run_project(cloud, steps, window=16):
         ctx.guess_mode(M.MP_GUESS_PROJECT)
         ctx.guess_window(window)
         with pytest.raises(M.MemphysError):
-            ctx.guess_window(17)
+            ctx.guess_window(25)
         for it in range(steps):
             ctx.step(1, it)
             st = ctx.stats()
