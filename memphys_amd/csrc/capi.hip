@@ -1209,6 +1209,7 @@ void fs_step_one(mp_ctx* c, int it) {
             learned = true;
         }
     }
+    c->ppe->target_margin = project ? 0.25 : 0.5;
     SolveStats st = c->ppe->solve(c->stream, c->b.p, c->P.p);
     if (learned && !c->train) {
         if (st.guess_rejected) {

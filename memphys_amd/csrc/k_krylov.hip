@@ -1699,16 +1699,16 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
         // FP64 defect correction around an inner BiCGSTAB or GMRES that streams
         // the FP32 copy of the operator while keeping FP64 reductions: each
         // cycle solves A32 e = r with r = b - A x~ from the FP64 operator, to a
-        // relative tolerance max(inner_tol, TARGET_MARGIN tol |b| / |r|), then
+        // relative tolerance max(inner_tol, target_margin tol |b| / |r|), then
         // x~ += e.  A system whose cycle breaks down or stalls hands the rest
-        // of its solve to FP64 BiCGSTAB.  The margin ends a solve well below
-        // the tolerance: with a guess close to the tolerance (the projected
-        // guess starts at about 1.5 tol), solving to a quarter of it lets the
-        // next step's guess meet the tolerance more often (over the 495 timed
+        // of its solve to FP64 BiCGSTAB.  The projected guess starts near the
+        // tolerance (about 1.5 tol), and a solve that ends further below it
+        // lets the next step's guess meet it more often: over the 495 timed
         // steps of the 1M-node cavity, margins 0.5 / 0.35 / 0.25 / 0.15 / 0.1
         // gave 7.2 / 6.9 / 6.6 / 6.9 / 7.2 GMRES iterations per step, with
-        // 119 / 179 / 224 / 250 / 264 steps needing none).
-        constexpr double TARGET_MARGIN = 0.25;
+        // 119 / 179 / 224 / 250 / 264 steps needing none.  The other guesses
+        // keep 0.5 (with 0.25 the order-4 extrapolation took 16.6 instead of
+        // 16.0 iterations per step, and the 64-cavity batch lost 9 %).
         std::vector<double> rr = rr0, rr_prev = rr0, tol2(nseg);
         std::vector<char> act(nseg, 0), fallback(nseg, 0);
         bool undone = false;  // (an undone cycle leaves the residual vector of the undone x~)
@@ -1722,7 +1722,7 @@ SolveStats PpeSolver::solve(hipStream_t st, const double* b_dev, double* x_dev) 
                 act[q] = act[q] && total[q] < maxit;
                 host_ctl[q].active = act[q];
                 host_ctl[q].maxit = maxit - total[q];
-                host_ctl[q].tol_rel = act[q] ? std::max(inner_tol, TARGET_MARGIN * std::sqrt(tol2[q] / rr[q]))
+                host_ctl[q].tol_rel = act[q] ? std::max(inner_tol, target_margin * std::sqrt(tol2[q] / rr[q]))
                                              : 1.0;
                 any |= act[q] != 0;
             }

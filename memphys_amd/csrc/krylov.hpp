@@ -120,6 +120,10 @@ struct PpeSolver {
     hipEvent_t ev[2] = {nullptr, nullptr};
     int precision = 0;
     double tol = 1e-8, inner_tol = 1e-3;
+    // mixed form: each cycle's inner solve ends at target_margin * tol (0.5,
+    // or 0.25 under the projected guess, whose next guess starts below the
+    // tolerance more often when this solve ends further below it)
+    double target_margin = 0.5;
     int maxit = 1000, check_every = 8;
     // GMRES iterations of the last run (0: none yet): the next run's first
     // chunk stops one short of it, and the chunks after it are of two
