@@ -20,7 +20,7 @@ namespace mp {
 namespace {
 
 constexpr int PROJ_THREADS = 256;
-constexpr int PROJ_BLOCKS = 512;  // two per CU
+constexpr int PROJ_BLOCKS = 768;  // three per CU (LDS holds three up to K = 20)
 constexpr int PROJ_TILE = PROJ_THREADS;  // rows per pass of a Gram block, one per thread
 static_assert(PROJ_MAX < 32, "the Gram matrix with b fits two 16-column blocks");
 
@@ -180,32 +180,42 @@ __device__ __forceinline__ double lane_value(double v, int src) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Sums the block partials (wavefront u sums entries u, u+4, ...: each lane
-// the partials of blocks lane, lane+64, ..., then a shuffle tree), then one
-// wavefront solves for beta: lane r holds row r of the scaled Gram matrix,
-// the right-looking Cholesky and the two triangular solves take the values of
-// other rows by lane reads, and lane k ends with beta_k.
+// Sums the Gram blocks' partial sums: block q adds entry q over the blocks
+// (a fixed order, so the sums repeat bit for bit).  One block per entry keeps
+// the many loads in flight; a single block walking every entry waited on them
+// (110 us for K = 20 against a few here).
 template <int K>
-__global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(const double* __restrict__ partial,
-                                                             ProjState* ps) {
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_reduce(const double* __restrict__ partial,
+                                                              double* __restrict__ tot) {
     constexpr int NG = K * (K + 1) / 2 + K;
-    static_assert(PROJ_BLOCKS % 64 == 0, "whole wavefronts of block partials");
-    static_assert(K <= 64 && PROJ_MAX <= 64, "one row per lane");
-    __shared__ double tot[NG];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     constexpr int NW = PROJ_THREADS / 64;
-#pragma unroll 4
-    for (int j = 0; j < (NG + NW - 1) / NW; ++j) {
-        const int q = wid + NW * j;
-        if (q >= NG) break;
-        double a = 0.0;
+    __shared__ double ws[NW];
+    const int q = blockIdx.x;
+    double a = 0.0;
 #pragma unroll
-        for (int u = 0; u < PROJ_BLOCKS / 64; ++u) a += partial[(size_t)(lane + 64 * u) * NG + q];
-        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
-        if (lane == 0) tot[q] = a;
+    for (int u = 0; u < (PROJ_BLOCKS + PROJ_THREADS - 1) / PROJ_THREADS; ++u) {
+        const int blk = threadIdx.x + PROJ_THREADS * u;
+        if (blk < PROJ_BLOCKS) a += partial[(size_t)blk * NG + q];
     }
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = a;
     __syncthreads();
-    if (wid != 0) return;
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int u = 0; u < NW; ++u) t += ws[u];
+        tot[q] = t;
+    }
+}
+
+// One wavefront solves for beta from the summed Gram entries: lane r holds
+// row r of the scaled Gram matrix, the right-looking Cholesky and the two
+// triangular solves take the values of other rows by lane reads, and lane k
+// ends with beta_k.
+template <int K>
+__global__ __launch_bounds__(64) void k_proj_solve(const double* __restrict__ tot, ProjState* ps) {
+    static_assert(K <= 64 && PROJ_MAX <= 64, "one row per lane");
+    const int lane = threadIdx.x;
     const int r = lane < K ? lane : 0;  // (lanes past K mirror row 0 and are not used)
     auto idx = [](int k, int l) { return k * K - k * (k - 1) / 2 + (l - k); };  // k <= l
     double g[K];
@@ -304,7 +314,10 @@ void launch_guess_k(hipStream_t s, int n, const double* hp, const double* hw, si
                     ProjState* ps, const double* b, double* partial, double* out) {
     hipLaunchKernelGGL(k_proj_gram<K>, dim3(PROJ_BLOCKS), dim3(PROJ_THREADS), 0, s, n, hw, ld, ps,
                        b, partial);
-    hipLaunchKernelGGL(k_proj_solve<K>, dim3(1), dim3(PROJ_THREADS), 0, s, partial, ps);
+    double* tot = partial + (size_t)PROJ_BLOCKS * (K * (K + 1) / 2 + K);
+    hipLaunchKernelGGL(k_proj_reduce<K>, dim3(K * (K + 1) / 2 + K), dim3(PROJ_THREADS), 0, s,
+                       partial, tot);
+    hipLaunchKernelGGL(k_proj_solve<K>, dim3(1), dim3(64), 0, s, tot, ps);
     hipLaunchKernelGGL(k_proj_combine<K>, dim3((n + PROJ_THREADS - 1) / PROJ_THREADS),
                        dim3(PROJ_THREADS), 0, s, n, hp, ld, ps, out);
 }
@@ -321,7 +334,7 @@ void launch_guess_upto(hipStream_t s, int k, int n, const double* hp, const doub
 
 }  // namespace
 
-size_t proj_partial_size() { return (size_t)PROJ_BLOCKS * (PROJ_MAX * (PROJ_MAX + 1) / 2 + PROJ_MAX); }
+size_t proj_partial_size() { return (size_t)(PROJ_BLOCKS + 1) * (PROJ_MAX * (PROJ_MAX + 1) / 2 + PROJ_MAX); }
 
 void launch_proj_guess(hipStream_t s, int k, int n, const double* hp, const double* hw, size_t ld,
                        ProjState* ps, const double* b, double* partial, double* out) {
