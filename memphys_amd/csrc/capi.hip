@@ -93,9 +93,9 @@ struct mp_ctx {
     int hist = 0;
     // projection guess: rings of the last PROJ_MAX pressures and their
     // products with the operator, the device record, entries held, window
-    // (20 by default: over the 495 timed steps of the 1M-node cavity, windows
-    // 16 / 20 / 24 took 6.6 / 6.3 / 6.2 GMRES iterations per step, and 20
-    // the least time)
+    // (20 by default: over the 495 timed steps of the 1M-node cavity, window
+    // 16 took 6.6 GMRES iterations per step and windows 20 / 22 / 24 took
+    // 6.4 / 6.2 / 6.3, at rates within 1 % of each other)
     DevBuf<double> Hp, Hw, proj_partial;
     DevBuf<ProjState> pstate;
     int pcount = 0, proj_window = 20;
