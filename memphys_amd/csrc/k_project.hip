@@ -156,9 +156,12 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_gram(int n, const double*
     for (int q = threadIdx.x; q < NG; q += PROJ_THREADS) {
         int k, l;
         if (q < K * (K + 1) / 2) {
-            int r = q;
+            int r = q;  // (row k of the upper triangle holds K - k entries)
             k = 0;
-            while (r >= K - k) r -= K - k++;
+            while (r >= K - k) {
+                r -= K - k;
+                ++k;
+            }
             l = k + r;
         } else {
             k = q - K * (K + 1) / 2;
