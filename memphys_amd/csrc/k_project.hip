@@ -99,22 +99,24 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_gram(int n, const double*
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[e][t] = zero;
     const int ntiles = (n + PROJ_TILE - 1) / PROJ_TILE;
+    // the next tile's row is loaded while the matrix cores work on this one
+    double w[K], bi = 0.0;
+    auto load = [&](int t) {
+        const int i = t * PROJ_TILE + threadIdx.x;
+#pragma unroll
+        for (int j = 0; j < K; ++j) w[j] = i < n ? hw[(size_t)ring_slot(head, j) * ld + i] : 0.0;
+        bi = i < n ? b[i] : 0.0;
+    };
+    if (blockIdx.x < ntiles) load(blockIdx.x);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int i = tile * PROJ_TILE + threadIdx.x;
         double* row = sd + threadIdx.x * LDT;
-        if (i < n) {
-            double w[K], d[K];
+        double d[K];
+        backward_differences<K>(w, d);  // (rows past n hold zeros)
 #pragma unroll
-            for (int j = 0; j < K; ++j) w[j] = hw[(size_t)ring_slot(head, j) * ld + i];
-            backward_differences<K>(w, d);
-#pragma unroll
-            for (int k = 0; k < K; ++k) row[k] = d[k];
-            row[K] = b[i];
-        } else {
-#pragma unroll
-            for (int k = 0; k <= K; ++k) row[k] = 0.0;
-        }
+        for (int k = 0; k < K; ++k) row[k] = d[k];
+        row[K] = bi;
         __syncthreads();
+        if (tile + (int)gridDim.x < ntiles) load(tile + gridDim.x);
         // wavefront wid takes the four-row groups wid, wid + 4, ..., alternately
         // into its two accumulator sets
         constexpr int NW = PROJ_THREADS / 64;
